@@ -1,0 +1,147 @@
+// Test harness around the reference AniFem++ element core, compiled from the
+// reference's own sources (anifem++/fem) by oracle/Makefile into
+// oracle/_ref/libanifem_ref.so.  It is test infrastructure only: the golden
+// fixture generator (tests/golden/make_golden.py), the reference-vs-oracle tests
+// and the bench's CPU baseline load it; nothing in the product path does.
+//
+// The harness exposes a flat C interface over a fixed set of template
+// instantiations of Ani::fem3Dtet (anifem++/fem/operations/int_tet.inl:92-122)
+// so that Python can feed it seeded tetrahedra and per-tet tensors.
+#include "anifem++/fem/operations/operations.h"
+#include "anifem++/fem/operators.h"
+#include "anifem++/fem/spaces/spaces.h"
+#include "anifem++/fem/quadrature_formulas.h"
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+
+using namespace Ani;
+
+namespace {
+
+struct TetData {
+    const double* D;   // dimB x dimA column-major tensor for this tet (or 1 value for SCALAR)
+    long ddim;
+};
+
+// D callback returning the tensor stored in user data; TT is the declared type.
+// With XDEP the stored tensor is multiplied by the scalar field
+// m(x) = 1 + x0/2 - x1/4 + x2^2 / 8, evaluated at the physical quadrature point,
+// which exercises the per-point path with a tensor that varies inside the tet.
+inline double xdep_field(const std::array<double, 3>& X) { return 1.0 + 0.5 * X[0] - 0.25 * X[1] + 0.125 * X[2] * X[2]; }
+
+template <int TT, bool XDEP>
+struct StoredTensor {
+    TensorType operator()(const std::array<double, 3>& X, double* D, std::pair<ulong, ulong> dims,
+                          void* user_data, int iTet) const {
+        (void)iTet;
+        const TetData* td = static_cast<const TetData*>(user_data);
+        const double m = XDEP ? xdep_field(X) : 1.0;
+        if (TT == TENSOR_SCALAR) {
+            D[0] = m * td->D[0];
+        } else if (TT != TENSOR_NULL) {
+            for (std::size_t k = 0; k < dims.first * dims.second; ++k) D[k] = m * td->D[k];
+        }
+        return static_cast<TensorType>(TT);
+    }
+};
+
+template <typename OpA, typename OpB, int TT, int SPARSITY, bool XDEP = false>
+int run_case(int order, long n, const double* xyz, const double* D, long ddim, double* A) {
+    constexpr long nfa = OpA::Nfa::value, nfb = OpB::Nfa::value;
+    double buf[nfa * nfb > 1 ? nfa * nfb : 1];
+    for (long r = 0; r < n; ++r) {
+        const double* p = xyz + 12 * r;
+        Tetra<const double> XYZ(p, p + 3, p + 6, p + 9);
+        DenseMatrix<double> Am(buf, nfb, nfa, nfa * nfb);
+        TetData td{D ? D + ddim * r : nullptr, ddim};
+        fem3Dtet<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, StoredTensor<TT, XDEP>(), Am, order, &td);
+        std::memcpy(A + nfa * nfb * r, buf, sizeof(double) * nfa * nfb);
+    }
+    return 0;
+}
+
+template <typename OpA, typename OpB>
+int dispatch_tensor(int tt, int per_point, int order, long n, const double* xyz, const double* D, long ddim,
+                    double* A) {
+    if (per_point == 2) {
+        switch (tt) {
+            case TENSOR_SCALAR: return run_case<OpA, OpB, TENSOR_SCALAR, PerPoint, true>(order, n, xyz, D, ddim, A);
+            case TENSOR_SYMMETRIC: return run_case<OpA, OpB, TENSOR_SYMMETRIC, PerPoint, true>(order, n, xyz, D, ddim, A);
+            case TENSOR_GENERAL: return run_case<OpA, OpB, TENSOR_GENERAL, PerPoint, true>(order, n, xyz, D, ddim, A);
+        }
+    } else if (per_point) {
+        switch (tt) {
+            case TENSOR_NULL: return run_case<OpA, OpB, TENSOR_NULL, PerPoint>(order, n, xyz, D, ddim, A);
+            case TENSOR_SCALAR: return run_case<OpA, OpB, TENSOR_SCALAR, PerPoint>(order, n, xyz, D, ddim, A);
+            case TENSOR_SYMMETRIC: return run_case<OpA, OpB, TENSOR_SYMMETRIC, PerPoint>(order, n, xyz, D, ddim, A);
+            case TENSOR_GENERAL: return run_case<OpA, OpB, TENSOR_GENERAL, PerPoint>(order, n, xyz, D, ddim, A);
+        }
+    } else {
+        switch (tt) {
+            case TENSOR_NULL: return run_case<OpA, OpB, TENSOR_NULL, TENSOR_NULL>(order, n, xyz, D, ddim, A);
+            case TENSOR_SCALAR: return run_case<OpA, OpB, TENSOR_SCALAR, TENSOR_SCALAR>(order, n, xyz, D, ddim, A);
+            case TENSOR_SYMMETRIC:
+                return run_case<OpA, OpB, TENSOR_SYMMETRIC, TENSOR_SYMMETRIC>(order, n, xyz, D, ddim, A);
+            case TENSOR_GENERAL:
+                return run_case<OpA, OpB, TENSOR_GENERAL, TENSOR_GENERAL>(order, n, xyz, D, ddim, A);
+        }
+    }
+    return -3;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Number of points of the reference rule; fills p[4*q], w[q] when non-null.
+int ref_quadrature(int order, double* p, double* w) {
+    try {
+        auto f = tetrahedron_quadrature_formulas(order);
+        int q = f.GetNumPoints();
+        if (p) std::memcpy(p, f.p, sizeof(double) * 4 * q);
+        if (w) std::memcpy(w, f.w, sizeof(double) * q);
+        return q;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Pair codes (OpA, OpB): 0=(GRAD P1, GRAD P1); 1=(GRAD P2, GRAD P2); 2=(GRAD P3, GRAD P3); 3=(GRAD V1, GRAD V1); 4=(GRAD V2, GRAD V2); 5=(IDEN P0, IDEN P1); 6=(IDEN P0, IDEN P2); 7=(IDEN P0, IDEN P3); 8=(IDEN P0, IDEN V2); 9=(IDEN P1, IDEN P1); 10=(IDEN P2, IDEN P2); 11=(IDEN P3, IDEN P3); 12=(GRAD P3, GRAD V1); 13=(IDEN P0, IDEN V1); 14=(GRAD P2, IDEN P2).
+//  xyz is n x 12 (P0,P1,P2,P3 per tet), D is n x ddim.
+// A receives n element matrices, each nfB x nfA column-major.
+int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, const double* xyz, const double* D,
+                 long ddim, double* A) {
+    using P0 = FemFix<FEM_P0>;
+    using P1 = FemFix<FEM_P1>;
+    using P2 = FemFix<FEM_P2>;
+    using P3 = FemFix<FEM_P3>;
+    using V1 = FemVec<3, FEM_P1>;
+    using V2 = FemVec<3, FEM_P2>;
+    try {
+        switch (pair) {
+            case 0: return dispatch_tensor<Operator<GRAD, P1>, Operator<GRAD, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 1: return dispatch_tensor<Operator<GRAD, P2>, Operator<GRAD, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 2: return dispatch_tensor<Operator<GRAD, P3>, Operator<GRAD, P3>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 3: return dispatch_tensor<Operator<GRAD, V1>, Operator<GRAD, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 4: return dispatch_tensor<Operator<GRAD, V2>, Operator<GRAD, V2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 5: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 6: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 7: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, P3>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 8: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, V2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 9: return dispatch_tensor<Operator<IDEN, P1>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 10: return dispatch_tensor<Operator<IDEN, P2>, Operator<IDEN, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 11: return dispatch_tensor<Operator<IDEN, P3>, Operator<IDEN, P3>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 12: return dispatch_tensor<Operator<GRAD, P3>, Operator<GRAD, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 13: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 14: return dispatch_tensor<Operator<GRAD, P2>, Operator<IDEN, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            default: break;
+        }
+    } catch (const std::exception&) {
+        return -2;
+    }
+    return -1;
+}
+
+}  // extern "C"
