@@ -1,0 +1,226 @@
+/* anifem_gpu.h -- C ABI of the MI355X element-matrix / global-assembly library.
+ *
+ * Plain C: pointers, sizes and enums only.  Device buffers are HIP device
+ * pointers owned by the caller; `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream).  All calls are stream-ordered; calls on
+ * different streams with separate buffers are independent.
+ *
+ * Reference interfaces each entry point replaces are cited per declaration
+ * (paths relative to the AniFem++ source tree).
+ */
+#ifndef ANIFEM_GPU_H
+#define ANIFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ANIFEM_OK 0
+#define ANIFEM_ERR_NONFINITE (-1)   /* Assemble's -1: NaN/Inf in a local matrix or rhs (assembler.inl:419-424) */
+#define ANIFEM_ERR_CONNECTIVITY (-2) /* Assemble's -2: broken cell connectivity (assembler.inl:309-312) */
+#define ANIFEM_ERR_ARG (-3)          /* invalid argument / unsupported combination */
+#define ANIFEM_ERR_ORDER (-4)        /* quadrature order outside 0..6 (int_tet.inl:99-100 throws) */
+#define ANIFEM_ERR_DIMS (-5)         /* NULL/SCALAR tensor with incompatible operator dims (diff_tensor.h:313-315) */
+#define ANIFEM_ERR_HIP (-6)          /* HIP runtime failure */
+#define ANIFEM_ERR_NOMEM (-7)        /* allocation failure */
+#define ANIFEM_ERR_NOT_CONVERGED (-8) /* PCG reached maxit */
+
+/* ---- operators: Operator<OP, FemFix<FEM>> or Operator<OP, FemVec<ncomp, FEM>>
+ * Numbering follows the reference enums (anifem++/fem/operators.h:25-45). */
+enum { ANIFEM_IDEN = 1, ANIFEM_GRAD = 2 };
+enum { ANIFEM_FEM_P0 = 1, ANIFEM_FEM_P1 = 2, ANIFEM_FEM_P2 = 3, ANIFEM_FEM_P3 = 4 };
+
+typedef struct anifem_op {
+    int32_t op;    /* ANIFEM_IDEN or ANIFEM_GRAD */
+    int32_t fem;   /* ANIFEM_FEM_P0 .. ANIFEM_FEM_P3 */
+    int32_t ncomp; /* 1 for FemFix, >1 for FemVec<ncomp, fem> (operators.h:63-67,127-155) */
+} anifem_op;
+
+/* ---- tensor coefficient D (dimB x dimA), the Dfnc of fem3Dtet ----------
+ * Types follow TensorType (diff_tensor.h:17-22); LAME is an isotropic
+ * elasticity tensor C((i,j),(k,l)) = lambda d_ij d_kl + mu (d_ik d_jl + d_il d_jk)
+ * over 9x9 gradient components (the 9x9 TENSOR_SYMMETRIC of
+ * examples/tutorials/lin_elast.cpp:101-117) given by its 2 parameters.
+ *
+ * Storage (device, FP64, structure-of-arrays over tets):
+ *   CONST     : ddim values shared by all tets  (DfuncTraits<..., true>)
+ *   PER_TET   : value v of tet r at data[v*n_tets + r]   (per-tet tensor, PerTetra)
+ *   PER_POINT : value v of tet r at quadrature point n at data[(v*q + n)*n_tets + r]
+ *               (a host functor evaluated at the physical quadrature points:
+ *                the DfuncTraitsFusive layout of diff_tensor.h:31-32,52-53)
+ * ddim: NULL 0; SCALAR 1; SYMMETRIC dimB*(dimB+1)/2 packed upper triangle,
+ * column-major (D(i,j), i<=j, at i + j*(j+1)/2); GENERAL dimB*dimA
+ * column-major; LAME 2 (lambda, mu). */
+enum {
+    ANIFEM_TENSOR_NULL = 1,
+    ANIFEM_TENSOR_SCALAR = 2,
+    ANIFEM_TENSOR_SYMMETRIC = 3,
+    ANIFEM_TENSOR_GENERAL = 4,
+    ANIFEM_TENSOR_LAME = 101
+};
+enum { ANIFEM_COEF_CONST = 0, ANIFEM_COEF_PER_TET = 1, ANIFEM_COEF_PER_POINT = 2 };
+
+typedef struct anifem_tensor {
+    int32_t type;       /* ANIFEM_TENSOR_* */
+    int32_t storage;    /* ANIFEM_COEF_* */
+    const double* data; /* device pointer (may be NULL for TENSOR_NULL) */
+} anifem_tensor;
+
+/* Arithmetic of the element contraction.  Geometry (Jacobian, inverse,
+ * determinant) is always FP64; global CSR / RHS accumulation is always FP64. */
+enum {
+    ANIFEM_PREC_FP64 = 0, /* FP64 contraction, FP64 element output */
+    ANIFEM_PREC_FP32 = 1  /* FP32 contraction, FP32 element output */
+};
+
+/* ---- element level ------------------------------------------------------
+ * Batched fem3Dtet<OpA, OpB, DfuncTraits>(XYZ, Dfnc, A, order): the element
+ * matrices  A_r(ib, ia) = int_{T_r} (D OpA(phi_ia)) . OpB(phi_ib) dx  of n_tets
+ * tetrahedra (anifem++/fem/operations/int_tet.h:17-20, int_tet.inl:92-122,
+ * core.inl:278-367).
+ *   xyz   : device, 12 x n_tets SoA: coordinate c of vertex v of tet r at
+ *           xyz[(3*v + c)*n_tets + r]  (the Tetras XY0..XY3 columns)
+ *   A_out : device, n_tets matrices of nfB x nfA, column-major, one after the
+ *           other (the reference's fused layout A[ib + nfB*(ia + nfA*r)]);
+ *           double for ANIFEM_PREC_FP64, float for ANIFEM_PREC_FP32.
+ * Errors mirror the reference's throws: ANIFEM_ERR_ORDER (order > 6),
+ * ANIFEM_ERR_DIMS (NULL/SCALAR with dimA != dimB), ANIFEM_ERR_ARG. */
+int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
+                          const double* xyz, int precision, void* A_out, void* stream);
+
+/* Number of basis functions (Nfa) and operator dimension (Dim) of an operator
+ * (the Nfa/Dim typedefs of poly_*.h and operators.h:127-155).  <0 if invalid. */
+int anifem_op_nfa(anifem_op op);
+int anifem_op_dim(anifem_op op);
+/* Number of quadrature points of the rule used for `order` (<0 if invalid). */
+int anifem_quadrature_npoints(int order);
+/* Quadrature rule (barycentric p[4q], weights w[q] summing to 1), host arrays. */
+int anifem_quadrature(int order, double* p, double* w);
+
+/* ---- whole-mesh assembly --------------------------------------------------
+ * The reference assembles cell by cell through a virtual plugin
+ * (MatFuncWrap::operator(), inmost_interface/func_wrap.h:96-187) called from
+ * AssemblerT::Assemble (assembler.inl:314-488).  A per-cell virtual call
+ * cannot feed a GPU, so the drop-in sits at the whole-mesh call:
+ * anifem_assembly_create ~ PrepareProblem (assembler.inl:194-275) and
+ * anifem_assembly_run ~ Assemble(matrix, rhs) with the element evaluator
+ * being fem3Dtet<OpA, OpB> on every tet plus the load vector
+ * fem3Dtet<IDEN P0, IDEN test> (examples/tutorials/ex1.cpp:92-106). */
+typedef struct anifem_assembly anifem_assembly;
+
+typedef struct anifem_mesh_desc {
+    int64_t n_nodes;
+    int64_t n_tets;
+    const double* coords; /* host, n_nodes x 3 */
+    const int32_t* tets;  /* host, n_tets x 4 node ids (cell order) */
+} anifem_mesh_desc;
+
+enum { ANIFEM_DIRICHLET_NONE = 0, ANIFEM_DIRICHLET_BOUNDARY = 1, ANIFEM_DIRICHLET_DOF_MASK = 2 };
+
+typedef struct anifem_problem_desc {
+    anifem_op trial;        /* OpA: the unknown's operator, e.g. GRAD FemFix<P2> */
+    anifem_op test;         /* OpB: must live on the same space as trial */
+    int32_t order;          /* quadrature order (0..6) */
+    int32_t tensor_type;    /* ANIFEM_TENSOR_* of D */
+    int32_t tensor_storage; /* ANIFEM_COEF_* of D */
+    int32_t rhs_order;      /* quadrature order of the load vector */
+    int32_t rhs_tensor_type;    /* ANIFEM_TENSOR_SCALAR/NULL (or GENERAL ncomp x 1) */
+    int32_t rhs_tensor_storage; /* ANIFEM_COEF_* of f */
+    int32_t dirichlet;          /* ANIFEM_DIRICHLET_* (homogeneous values by default) */
+    const uint8_t* dirichlet_dof_mask; /* host, ndof flags, for ANIFEM_DIRICHLET_DOF_MASK */
+} anifem_problem_desc;
+
+typedef struct anifem_status {
+    int32_t code;            /* ANIFEM_OK or ANIFEM_ERR_NONFINITE */
+    int32_t reserved;
+    int64_t n_nonfinite;     /* local matrix / rhs entries that were NaN/Inf */
+    int64_t n_nonpositive;   /* tets with det <= 0 after the node reorder */
+} anifem_status;
+
+/* Builds on the host, then uploads: node reorder (ordering.inl:8-26), NATURAL
+ * DOF enumeration with P3 edge orientation (global_enumerator.cpp:562-777,
+ * ordering.inl:137-151, tetdofmap.inl:98-127), CSR pattern with the diagonal
+ * and Dirichlet drop rules (assembler.inl:114-136,416), the element-to-CSR-slot
+ * map and a conflict-free tet colouring.  `stream` orders the uploads. */
+int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
+                           void* stream);
+int anifem_assembly_destroy(anifem_assembly* ctx);
+/* Host-only variant: builds the same setup without touching the device
+ * (inspection of the enumeration, pattern, slot map and colouring through
+ * the info / copy_setup calls; run and coords return ANIFEM_ERR_ARG). */
+int anifem_assembly_create_host(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob);
+
+/* Sizes: number of DOFs, stored nonzeros, colours, local DOFs per tet. */
+int anifem_assembly_info(const anifem_assembly* ctx, int64_t* ndof, int64_t* nnz, int32_t* n_colours,
+                         int32_t* nf);
+/* Device views of the CSR pattern (row_ptr: ndof+1 int64, col_idx: nnz int32). */
+int anifem_assembly_csr(const anifem_assembly* ctx, const int64_t** row_ptr, const int32_t** col_idx);
+/* Host copies for inspection: pattern, per-tet global DOFs (n_tets x nf, the
+ * reference's local order), slot map (n_tets x nf*nf, entry ib + nf*ia, -1 =
+ * dropped), tet colours, Dirichlet DOF flags.  Any pointer may be NULL. */
+int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int32_t* col_idx, int32_t* dofs,
+                               int64_t* slot, int32_t* colour, uint8_t* dirichlet);
+/* Device node coordinates (n_nodes x 3) used by run; the caller may update
+ * them in place (moving mesh) between runs. */
+int anifem_assembly_coords(anifem_assembly* ctx, double** coords);
+
+/* Assemble: csr_val (nnz, device FP64) and rhs (ndof, device FP64) are
+ * overwritten with  sum_T P^T A_T P  and  sum_T P^T F_T  (assembler.inl:397-481)
+ * after element-level Dirichlet (dc_on_dof.h:38-45) on the flagged DOFs.
+ * Tets are processed colour by colour; within a colour no two tets share a
+ * DOF, so there are no atomics and repeated runs are bitwise identical.
+ * D and F use the problem's tensor kinds; F may be NULL (no rhs; rhs may be
+ * NULL too).  dir_values (device, ndof, may be NULL = homogeneous) gives the
+ * Dirichlet values.  If status is non-NULL the call synchronizes the stream
+ * and fills it; otherwise it returns after enqueueing. */
+int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
+                        const double* dir_values, int precision, double* csr_val, double* rhs,
+                        anifem_status* status, void* stream);
+
+/* Per-phase GPU timers of the last run, the counterpart of the assembler
+ * timers of the reference (assembler.h:234-248, GetTimeEvalLocFunc and
+ * friends).  When enabled, run records HIP events on its stream around each
+ * phase and synchronizes at its end.  Times in milliseconds; launch counts. */
+typedef struct anifem_timers {
+    double ms_fill;     /* zeroing of csr_val / rhs */
+    double ms_matrix;   /* element matrices + scatter, all colours */
+    double ms_rhs;      /* load vectors + scatter, all colours */
+    int32_t n_matrix_launches;
+    int32_t n_rhs_launches;
+} anifem_timers;
+int anifem_assembly_enable_timers(anifem_assembly* ctx, int on);
+int anifem_assembly_timers(const anifem_assembly* ctx, anifem_timers* out);
+
+/* ---- solver ---------------------------------------------------------------
+ * Jacobi-preconditioned CG on a device CSR matrix in FP64 (the reference
+ * solves with INMOST's Solver, examples/tutorials/ex1.cpp:161-163; there is
+ * no CG in the reference).  Stops when ||b - A x||_2 <= rtol ||b||_2 measured
+ * on the recursively updated residual; true_relres receives the FP64
+ * residual recomputed from x at exit.  x holds the initial guess. */
+typedef struct anifem_pcg_result {
+    int64_t iterations;
+    double relres;      /* recursive residual at exit */
+    double true_relres; /* ||b - A x|| / ||b|| recomputed at exit */
+} anifem_pcg_result;
+
+int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* val,
+                      const double* b, double* x, double rtol, int64_t maxit, anifem_pcg_result* res,
+                      void* stream);
+
+/* ---- synthetic meshes (host) ----------------------------------------------
+ * Unit cube split into n^3 sub-cubes, each into 6 Kuhn tets; node
+ * (i,j,k) has id (i*(n+1)+j)*(n+1)+k (the V_ID of utils/mesh_utils.cpp:124).
+ * Interior nodes are moved by i.i.d. U[-jitter*h, jitter*h] per coordinate from
+ * a Philox-4x32-10 stream keyed by seed.  coords: (n+1)^3 x 3, tets: 6 n^3 x 4. */
+int anifem_mesh_kuhn_cube(int64_t n, double jitter, uint64_t seed, double* coords, int32_t* tets);
+
+/* Library version string. */
+const char* anifem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANIFEM_GPU_H */

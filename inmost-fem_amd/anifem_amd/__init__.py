@@ -1,0 +1,326 @@
+"""Python access to libanifem_gpu.so (the MI355X element-matrix / assembly
+library) through its C ABI (include/anifem_gpu.h).
+
+PyTorch is used only for device memory and streams: tensors passed in must be
+contiguous, on the current HIP device, and of the dtype documented per call.
+The library is loaded from this package's directory; importing the GPU entry
+points when it has not been built raises immediately -- there is no CPU
+fallback.
+
+Names mirror the reference interface: Operator/FemFix/FemVec, TensorType,
+fem3Dtet, Assembler (anifem++/fem/operators.h, diff_tensor.h,
+operations/int_tet.h, inmost_interface/assembler.h).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libanifem_gpu.so")
+
+# ---- enums (include/anifem_gpu.h) -----------------------------------------
+OK, ERR_NONFINITE, ERR_CONNECTIVITY, ERR_ARG, ERR_ORDER, ERR_DIMS, ERR_HIP, ERR_NOMEM, ERR_NOT_CONVERGED = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8)
+IDEN, GRAD = 1, 2
+FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
+TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
+COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+DIRICHLET_NONE, DIRICHLET_BOUNDARY, DIRICHLET_DOF_MASK = 0, 1, 2
+
+_ERRNAMES = {ERR_NONFINITE: "non-finite local entry", ERR_CONNECTIVITY: "broken connectivity",
+             ERR_ARG: "invalid argument", ERR_ORDER: "unsupported quadrature order",
+             ERR_DIMS: "identity/scalar tensor needs equal operator dimensions", ERR_HIP: "HIP runtime error",
+             ERR_NOMEM: "out of memory", ERR_NOT_CONVERGED: "solver did not converge"}
+
+
+class AnifemError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"{what}: {_ERRNAMES.get(code, 'error')} (code {code})")
+        self.code = code
+
+
+class anifem_op(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("fem", ctypes.c_int32), ("ncomp", ctypes.c_int32)]
+
+
+class anifem_tensor(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("storage", ctypes.c_int32), ("data", ctypes.c_void_p)]
+
+
+class anifem_mesh_desc(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_tets", ctypes.c_int64),
+                ("coords", ctypes.c_void_p), ("tets", ctypes.c_void_p)]
+
+
+class anifem_problem_desc(ctypes.Structure):
+    _fields_ = [("trial", anifem_op), ("test", anifem_op), ("order", ctypes.c_int32),
+                ("tensor_type", ctypes.c_int32), ("tensor_storage", ctypes.c_int32),
+                ("rhs_order", ctypes.c_int32), ("rhs_tensor_type", ctypes.c_int32),
+                ("rhs_tensor_storage", ctypes.c_int32), ("dirichlet", ctypes.c_int32),
+                ("dirichlet_dof_mask", ctypes.c_void_p)]
+
+
+class anifem_status(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n_nonfinite", ctypes.c_int64), ("n_nonpositive", ctypes.c_int64)]
+
+
+class anifem_timers(ctypes.Structure):
+    _fields_ = [("ms_fill", ctypes.c_double), ("ms_matrix", ctypes.c_double), ("ms_rhs", ctypes.c_double),
+                ("n_matrix_launches", ctypes.c_int32), ("n_rhs_launches", ctypes.c_int32)]
+
+
+class anifem_pcg_result(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("relres", ctypes.c_double), ("true_relres", ctypes.c_double)]
+
+
+# Every symbol include/anifem_gpu.h declares, with its ctypes signature.
+_VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+SIGNATURES = {
+    "anifem_fem3dtet_batch": (_I32, [anifem_op, anifem_op, ctypes.POINTER(anifem_tensor), _I32, _I64, _VP, _I32, _VP,
+                                     _VP]),
+    "anifem_op_nfa": (_I32, [anifem_op]),
+    "anifem_op_dim": (_I32, [anifem_op]),
+    "anifem_quadrature_npoints": (_I32, [_I32]),
+    "anifem_quadrature": (_I32, [_I32, _VP, _VP]),
+    "anifem_assembly_create": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
+                                      ctypes.POINTER(anifem_problem_desc), _VP]),
+    "anifem_assembly_create_host": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
+                                           ctypes.POINTER(anifem_problem_desc)]),
+    "anifem_assembly_destroy": (_I32, [_VP]),
+    "anifem_assembly_info": (_I32, [_VP, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_csr": (_I32, [_VP, _VP, _VP]),
+    "anifem_assembly_copy_setup": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_coords": (_I32, [_VP, _VP]),
+    "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
+                                   _VP, ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_enable_timers": (_I32, [_VP, _I32]),
+    "anifem_assembly_timers": (_I32, [_VP, ctypes.POINTER(anifem_timers)]),
+    "anifem_pcg_jacobi": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_mesh_kuhn_cube": (_I32, [_I64, _D, ctypes.c_uint64, _VP, _VP]),
+    "anifem_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run `make -C inmost-fem_amd` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise AnifemError(rc, what)
+
+
+def _ptr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+# ---- operators (Operator<OP, FemFix<FEM>> / FemVec<n, FEM>) ------------------
+@dataclass(frozen=True)
+class Op:
+    op: int
+    fem: int
+    ncomp: int = 1
+
+    def c(self):
+        return anifem_op(self.op, self.fem, self.ncomp)
+
+    @property
+    def nfa(self):
+        return lib().anifem_op_nfa(self.c())
+
+    @property
+    def dim(self):
+        return lib().anifem_op_dim(self.c())
+
+
+def Operator(op, fem, ncomp=1):
+    return Op(op, fem, ncomp)
+
+
+def tensor_ddim(ttype, dimA, dimB):
+    return {TENSOR_NULL: 0, TENSOR_SCALAR: 1, TENSOR_SYMMETRIC: dimB * (dimB + 1) // 2,
+            TENSOR_GENERAL: dimA * dimB, TENSOR_LAME: 2}[ttype]
+
+
+def quadrature(order):
+    q = lib().anifem_quadrature_npoints(order)
+    if q < 0:
+        raise AnifemError(ERR_ORDER, "quadrature")
+    p = np.zeros(4 * q)
+    w = np.zeros(q)
+    rc = lib().anifem_quadrature(order, _ptr(p), _ptr(w))
+    if rc < 0:
+        raise AnifemError(rc, "quadrature")
+    return p.reshape(q, 4), w
+
+
+# ---- element level -------------------------------------------------------
+def fem3dtet_batch(opA: Op, opB: Op, ttype, storage, D, order, xyz, precision=PREC_FP64, out=None, stream=None):
+    """Element matrices of n tets (anifem_fem3dtet_batch).
+
+    xyz: device float64 (12, n) SoA; D: device float64 coefficient array in the
+    documented layout (or None for TENSOR_NULL); returns out (n, nfB, nfA) in
+    column-major per tet, i.e. out[r].T is the nfB x nfA matrix in row-major.
+    """
+    import torch
+    n = xyz.shape[1]
+    nfa, nfb = opA.nfa, opB.nfa
+    dt = torch.float64 if precision == PREC_FP64 else torch.float32
+    if out is None:
+        out = torch.empty((n, nfa, nfb), dtype=dt, device=xyz.device)
+    t = anifem_tensor(ttype, storage, _ptr(D))
+    rc = lib().anifem_fem3dtet_batch(opA.c(), opB.c(), ctypes.byref(t), order, n, _ptr(xyz), precision, _ptr(out),
+                                     _stream_handle(stream))
+    _check(rc, "anifem_fem3dtet_batch")
+    return out
+
+
+# ---- meshes ----------------------------------------------------------------
+def kuhn_cube(n, jitter=0.2, seed=0):
+    """Jittered Kuhn cube: (coords (N,3) float64, tets (6n^3, 4) int32)."""
+    m = n + 1
+    coords = np.zeros((m ** 3, 3))
+    tets = np.zeros((6 * n ** 3, 4), dtype=np.int32)
+    _check(lib().anifem_mesh_kuhn_cube(n, jitter, seed, _ptr(coords), _ptr(tets)), "anifem_mesh_kuhn_cube")
+    return coords, tets
+
+
+# ---- global assembly ---------------------------------------------------------
+class Assembler:
+    """Whole-mesh GPU assembler (anifem_assembly_*), the counterpart of
+    AssemblerT::PrepareProblem + Assemble (inmost_interface/assembler.h:232-446).
+    With host_only=True only the setup is built (no device use)."""
+
+    def __init__(self, coords, tets, trial: Op, test: Op = None, order=2, tensor_type=TENSOR_NULL,
+                 tensor_storage=COEF_CONST, rhs_order=None, rhs_tensor_type=TENSOR_SCALAR,
+                 rhs_tensor_storage=COEF_CONST, dirichlet=DIRICHLET_NONE, dirichlet_dof_mask=None, host_only=False,
+                 stream=None):
+        self._coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self._tets = np.ascontiguousarray(tets, dtype=np.int32)
+        test = test or trial
+        self.trial, self.test = trial, test
+        self.tensor_type, self.tensor_storage = tensor_type, tensor_storage
+        self.rhs_tensor_type, self.rhs_tensor_storage = rhs_tensor_type, rhs_tensor_storage
+        self._mask = None if dirichlet_dof_mask is None else np.ascontiguousarray(dirichlet_dof_mask, dtype=np.uint8)
+        mesh = anifem_mesh_desc(len(self._coords), len(self._tets), _ptr(self._coords), _ptr(self._tets))
+        prob = anifem_problem_desc(trial.c(), test.c(), order, tensor_type, tensor_storage,
+                                   order if rhs_order is None else rhs_order, rhs_tensor_type, rhs_tensor_storage,
+                                   dirichlet, _ptr(self._mask))
+        h = ctypes.c_void_p()
+        L = lib()
+        if host_only:
+            rc = L.anifem_assembly_create_host(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob))
+        else:
+            rc = L.anifem_assembly_create(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob),
+                                          _stream_handle(stream))
+        _check(rc, "anifem_assembly_create")
+        self._h = h
+        self.host_only = host_only
+        ndof, nnz, ncol, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        L.anifem_assembly_info(h, ctypes.byref(ndof), ctypes.byref(nnz), ctypes.byref(ncol), ctypes.byref(nf))
+        self.ndof, self.nnz, self.n_colours, self.nf = ndof.value, nnz.value, ncol.value, nf.value
+        self.n_tets = len(self._tets)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.anifem_assembly_destroy(h)
+            self._h = None
+
+    def setup(self):
+        """Host copies: dict of row_ptr, col_idx, dofs, slot, colour, dirichlet."""
+        rp = np.zeros(self.ndof + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int32)
+        dofs = np.zeros((self.n_tets, self.nf), np.int32)
+        slot = np.zeros((self.n_tets, self.nf * self.nf), np.int64)
+        col = np.zeros(self.n_tets, np.int32)
+        dr = np.zeros(self.ndof, np.uint8)
+        _check(lib().anifem_assembly_copy_setup(self._h, _ptr(rp), _ptr(ci), _ptr(dofs), _ptr(slot), _ptr(col),
+                                                _ptr(dr)), "copy_setup")
+        return dict(row_ptr=rp, col_idx=ci, dofs=dofs, slot=slot, colour=col, dirichlet=dr)
+
+    def csr(self):
+        """Device (row_ptr, col_idx) addresses."""
+        rp, ci = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().anifem_assembly_csr(self._h, ctypes.byref(rp), ctypes.byref(ci)), "csr")
+        return rp.value, ci.value
+
+    def assemble(self, D=None, F=None, precision=PREC_FP64, val=None, rhs=None, status=True, stream=None):
+        """Assemble into device FP64 val (nnz) and rhs (ndof); returns (val, rhs, status)."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if val is None:
+            val = torch.empty(self.nnz, dtype=torch.float64, device=dev)
+        if rhs is None and F is not None:
+            rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
+        tD = anifem_tensor(self.tensor_type, self.tensor_storage, _ptr(D))
+        tF = anifem_tensor(self.rhs_tensor_type, self.rhs_tensor_storage, _ptr(F)) if F is not None else None
+        st = anifem_status()
+        rc = lib().anifem_assembly_run(self._h, ctypes.byref(tD), ctypes.byref(tF) if tF is not None else None, None,
+                                       precision, _ptr(val), _ptr(rhs), ctypes.byref(st) if status else None,
+                                       _stream_handle(stream))
+        if rc not in (OK, ERR_NONFINITE):
+            raise AnifemError(rc, "anifem_assembly_run")
+        return val, rhs, (st if status else None)
+
+    def enable_timers(self, on=True):
+        _check(lib().anifem_assembly_enable_timers(self._h, 1 if on else 0), "enable_timers")
+
+    def timers(self):
+        t = anifem_timers()
+        _check(lib().anifem_assembly_timers(self._h, ctypes.byref(t)), "timers")
+        return t
+
+    def device_csr_tensors(self):
+        """(row_ptr, col_idx) as torch tensors copied from the device views."""
+        import torch
+        s = self.setup()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        return torch.from_numpy(s["row_ptr"]).to(dev), torch.from_numpy(s["col_idx"]).to(dev)
+
+
+def pcg_jacobi(n, row_ptr, col_idx, val, b, x, rtol=1e-8, maxit=100000, stream=None):
+    """FP64 Jacobi-PCG on device CSR (addresses or tensors); returns result struct."""
+    res = anifem_pcg_result()
+    rp = row_ptr if isinstance(row_ptr, int) else _ptr(row_ptr)
+    ci = col_idx if isinstance(col_idx, int) else _ptr(col_idx)
+    rc = lib().anifem_pcg_jacobi(n, rp, ci, _ptr(val), _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(res),
+                                 _stream_handle(stream))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        raise AnifemError(rc, "anifem_pcg_jacobi")
+    return res, rc == OK
+
+
+def version():
+    return lib().anifem_version().decode()

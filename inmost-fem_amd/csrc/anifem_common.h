@@ -1,0 +1,60 @@
+// Internal declarations shared by the host and device translation units of
+// libanifem_gpu.so.  Not part of the C ABI (see include/anifem_gpu.h).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/anifem_gpu.h"
+
+namespace anifem {
+
+// Scalar finite-element space data (poly_0.h .. poly_3.h of the reference).
+int nf_scalar(int fem);          // 1, 4, 10, 20 for P0..P3; -1 otherwise
+int ref_dim(int op);             // 1 for IDEN, 3 for GRAD; -1 otherwise
+bool op_valid(const anifem_op& op);
+
+// Quadrature rule for `order` (generated table, tet_rules.inc); returns q or -1.
+int quadrature(int order, const double** p, const double** w);
+
+// Reference-space operator table of a scalar space at the rule's points:
+// tab[(n*nf + i)*d + a] = component a (d = ref_dim) of OP(phi_i) at point n,
+// with gradients taken in the reference tetrahedron's coordinates
+// (x1, x2, x3) = (l1, l2, l3).
+void reference_table(int op, int fem, int q, const double* p, std::vector<double>& tab);
+
+// Reference tensor of the element contraction for scalar bases (OpA0, OpB0).
+//   per_point == false:  R[rr*KB + b*dA + a] = sum_n w_n V_b(n, ib) U_a(n, ia)
+//   per_point == true :  R[(rr*q + n)*KB + b*dA + a] = V_b(n, ib) U_a(n, ia)
+// with rr = ib + nfB0*ia and KB = dA*dB.
+void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R);
+
+// Device-resident copies of reference tensors, built lazily per (pair, order,
+// per_point) and kept for the life of the process.
+struct RefTensorDev {
+    const double* r64 = nullptr;
+    const float* r32 = nullptr;
+    int E0 = 0;  // nfA0*nfB0
+    int KB = 0;  // dA*dB
+    int q = 0;   // points (per-point tensors) or 1
+    const double* w = nullptr;  // quadrature weights on the device (q values)
+};
+int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out);
+
+// Symmetric-tensor packing: D(i,j), i<=j, at i + j*(j+1)/2.
+inline int sym_index(int i, int j) { return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }
+
+// Tensor data size per tet (or per point) for a tensor type and operator dims.
+int tensor_ddim(int type, int dimA, int dimB);
+
+#define ANIFEM_HIP_CHECK(expr)                                                      \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            std::fprintf(stderr, "anifem: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return ANIFEM_ERR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+}  // namespace anifem

@@ -1,0 +1,244 @@
+// Host-side basis tables and reference tensors.
+//
+// The element matrix of fem3Dtet (anifem++/fem/operations/core.inl:278-367)
+//   A(ib, ia) = sum_n w_n |T| sum_{k,j} V_k(n, ib) D_n(k, j) U_j(n, ia)
+// is evaluated on the GPU in factored form.  For scalar spaces the physical
+// operator values are a per-tet linear map of reference-space values,
+// U_j(n, i) = sum_a T_A(j, a) Uh_a(n, i) with T = PSI^T for GRAD
+// (poly_2.h:94-100: U = PSI^T GRAD_P2) and T = 1 for IDEN, so
+//   A(ib, ia) = sum_n sum_{b,a} K_n(b, a) Vh_b(n, ib) Uh_a(n, ia),
+//   K_n = w_n |T| T_B^T D_n T_A.
+// K (per tet, FP64) is computed on the GPU; the products Vh_b Uh_a depend only
+// on the element pair and the quadrature rule and are tabulated here once
+// (summed over n with the weights when D is constant over the tet).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "anifem_common.h"
+#include "tet_rules.inc"
+
+namespace anifem {
+
+int nf_scalar(int fem) {
+    switch (fem) {
+        case ANIFEM_FEM_P0: return 1;
+        case ANIFEM_FEM_P1: return 4;
+        case ANIFEM_FEM_P2: return 10;
+        case ANIFEM_FEM_P3: return 20;
+    }
+    return -1;
+}
+
+int ref_dim(int op) { return op == ANIFEM_IDEN ? 1 : (op == ANIFEM_GRAD ? 3 : -1); }
+
+bool op_valid(const anifem_op& op) {
+    if (ref_dim(op.op) < 0 || nf_scalar(op.fem) < 0 || op.ncomp < 1 || op.ncomp > 3) return false;
+    if (op.op == ANIFEM_GRAD && op.fem == ANIFEM_FEM_P0) return false;
+    return true;
+}
+
+int quadrature(int order, const double** p, const double** w) {
+    if (order < 0 || order > ANIFEM_TET_MAX_ORDER) return -1;
+    int q = ANIFEM_TET_NPTS_BY_ORDER[order];
+    switch (q) {
+        case 1: *p = ANIFEM_TET_Q1_P, *w = ANIFEM_TET_Q1_W; break;
+        case 4: *p = ANIFEM_TET_Q4_P, *w = ANIFEM_TET_Q4_W; break;
+        case 14: *p = ANIFEM_TET_Q14_P, *w = ANIFEM_TET_Q14_W; break;
+        case 24: *p = ANIFEM_TET_Q24_P, *w = ANIFEM_TET_Q24_W; break;
+        default: return -1;
+    }
+    return q;
+}
+
+int tensor_ddim(int type, int dimA, int dimB) {
+    switch (type) {
+        case ANIFEM_TENSOR_NULL: return 0;
+        case ANIFEM_TENSOR_SCALAR: return 1;
+        case ANIFEM_TENSOR_SYMMETRIC: return dimA == dimB ? dimB * (dimB + 1) / 2 : -1;
+        case ANIFEM_TENSOR_GENERAL: return dimA * dimB;
+        case ANIFEM_TENSOR_LAME: return (dimA == 9 && dimB == 9) ? 2 : -1;
+    }
+    return -1;
+}
+
+namespace {
+
+// Edges of the local tetrahedron in the order 01,02,03,12,13,23 and the P3
+// face node triples {0,1,2},{1,2,3},{2,3,0},{3,0,1}
+// (docs: local numbering; poly_2.h:75, poly_3.h:83).
+const int kEdgeA[6] = {0, 0, 0, 1, 1, 2};
+const int kEdgeB[6] = {1, 2, 3, 2, 3, 3};
+const int kFace[4][3] = {{0, 1, 2}, {1, 2, 3}, {2, 3, 0}, {3, 0, 1}};
+
+// d lambda_i / d x_a in reference coordinates (lambda_0 = 1 - x1 - x2 - x3).
+inline double dlam(int i, int a) { return i == 0 ? -1.0 : (i - 1 == a ? 1.0 : 0.0); }
+
+void basis_values(int fem, const double* L, double* phi) {
+    switch (fem) {
+        case ANIFEM_FEM_P0: phi[0] = 1.0; break;
+        case ANIFEM_FEM_P1:
+            for (int i = 0; i < 4; ++i) phi[i] = L[i];
+            break;
+        case ANIFEM_FEM_P2:
+            for (int i = 0; i < 4; ++i) phi[i] = L[i] * (2.0 * L[i] - 1.0);
+            for (int e = 0; e < 6; ++e) phi[4 + e] = 4.0 * L[kEdgeA[e]] * L[kEdgeB[e]];
+            break;
+        case ANIFEM_FEM_P3:
+            for (int i = 0; i < 4; ++i) phi[i] = 0.5 * L[i] * (3.0 * L[i] - 1.0) * (3.0 * L[i] - 2.0);
+            for (int e = 0; e < 6; ++e) {
+                // two DOFs per edge (I, J): the one near I first (poly_3.h:39-44)
+                double a = L[kEdgeA[e]], b = L[kEdgeB[e]];
+                phi[4 + 2 * e] = 4.5 * a * (3.0 * a - 1.0) * b;
+                phi[5 + 2 * e] = 4.5 * a * (3.0 * b - 1.0) * b;
+            }
+            for (int f = 0; f < 4; ++f) phi[16 + f] = 27.0 * L[kFace[f][0]] * L[kFace[f][1]] * L[kFace[f][2]];
+            break;
+    }
+}
+
+// g[i*3 + a] = d phi_i / d x_a (reference coordinates)
+void basis_gradients(int fem, const double* L, double* g) {
+    switch (fem) {
+        case ANIFEM_FEM_P1:
+            for (int i = 0; i < 4; ++i)
+                for (int a = 0; a < 3; ++a) g[i * 3 + a] = dlam(i, a);
+            break;
+        case ANIFEM_FEM_P2:
+            for (int i = 0; i < 4; ++i)
+                for (int a = 0; a < 3; ++a) g[i * 3 + a] = (4.0 * L[i] - 1.0) * dlam(i, a);
+            for (int e = 0; e < 6; ++e) {
+                int I = kEdgeA[e], J = kEdgeB[e];
+                for (int a = 0; a < 3; ++a) g[(4 + e) * 3 + a] = 4.0 * (L[I] * dlam(J, a) + L[J] * dlam(I, a));
+            }
+            break;
+        case ANIFEM_FEM_P3:
+            for (int i = 0; i < 4; ++i) {
+                double s = L[i];
+                for (int a = 0; a < 3; ++a) g[i * 3 + a] = ((13.5 * s - 9.0) * s + 1.0) * dlam(i, a);
+            }
+            for (int e = 0; e < 6; ++e) {
+                int I = kEdgeA[e], J = kEdgeB[e];
+                double s = L[I], t = L[J];
+                for (int a = 0; a < 3; ++a) {
+                    g[(4 + 2 * e) * 3 + a] = 4.5 * ((6.0 * s - 1.0) * t * dlam(I, a) + (3.0 * s - 1.0) * s * dlam(J, a));
+                    g[(5 + 2 * e) * 3 + a] = 4.5 * ((6.0 * t - 1.0) * s * dlam(J, a) + (3.0 * t - 1.0) * t * dlam(I, a));
+                }
+            }
+            for (int f = 0; f < 4; ++f) {
+                int A = kFace[f][0], B = kFace[f][1], C = kFace[f][2];
+                for (int a = 0; a < 3; ++a)
+                    g[(16 + f) * 3 + a] =
+                        27.0 * (dlam(A, a) * L[B] * L[C] + L[A] * dlam(B, a) * L[C] + L[A] * L[B] * dlam(C, a));
+            }
+            break;
+        default:
+            for (int a = 0; a < 3; ++a) g[a] = 0.0;
+    }
+}
+
+}  // namespace
+
+void reference_table(int op, int fem, int q, const double* p, std::vector<double>& tab) {
+    int nf = nf_scalar(fem), d = ref_dim(op);
+    tab.assign((size_t)q * nf * d, 0.0);
+    double buf[60];
+    for (int n = 0; n < q; ++n) {
+        // the reference recomputes l0 = 1 - l1 - l2 - l3 for stability
+        // (quadrature_formulas.h:195-200); the generated table already satisfies it
+        const double* L = p + 4 * n;
+        if (op == ANIFEM_IDEN) basis_values(fem, L, buf);
+        else basis_gradients(fem, L, buf);
+        for (int i = 0; i < nf; ++i)
+            for (int a = 0; a < d; ++a) tab[((size_t)n * nf + i) * d + a] = buf[i * d + a];
+    }
+}
+
+void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R) {
+    const double *p, *w;
+    int q = quadrature(order, &p, &w);
+    std::vector<double> TA, TB;
+    reference_table(opA, femA, q, p, TA);
+    reference_table(opB, femB, q, p, TB);
+    int nfA = nf_scalar(femA), nfB = nf_scalar(femB), dA = ref_dim(opA), dB = ref_dim(opB);
+    int KB = dA * dB, E0 = nfA * nfB, qq = per_point ? q : 1;
+    R.assign((size_t)E0 * qq * KB, 0.0);
+    for (int ia = 0; ia < nfA; ++ia)
+        for (int ib = 0; ib < nfB; ++ib) {
+            int rr = ib + nfB * ia;
+            for (int n = 0; n < q; ++n)
+                for (int b = 0; b < dB; ++b)
+                    for (int a = 0; a < dA; ++a) {
+                        double v = TB[((size_t)n * nfB + ib) * dB + b] * TA[((size_t)n * nfA + ia) * dA + a];
+                        if (per_point) R[((size_t)rr * q + n) * KB + b * dA + a] = v;
+                        else R[(size_t)rr * KB + b * dA + a] += w[n] * v;
+                    }
+        }
+}
+
+int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int, int, int, int>, RefTensorDev> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ANIFEM_ERR_HIP;
+    auto key = std::make_tuple(dev, opA, femA, opB, femB, order, (int)per_point);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        *out = it->second;
+        return ANIFEM_OK;
+    }
+    std::vector<double> R;
+    reference_tensor(opA, femA, opB, femB, order, per_point, R);
+    std::vector<float> Rf(R.begin(), R.end());
+    RefTensorDev t;
+    const double* pp;
+    const double* ww;
+    const int nq = quadrature(order, &pp, &ww);
+    t.q = per_point ? nq : 1;
+    t.E0 = nf_scalar(femA) * nf_scalar(femB);
+    t.KB = ref_dim(opA) * ref_dim(opB);
+    void* d64 = nullptr;
+    void* d32 = nullptr;
+    void* dw = nullptr;
+    ANIFEM_HIP_CHECK(hipMalloc(&dw, sizeof(double) * (size_t)nq));
+    ANIFEM_HIP_CHECK(hipMemcpy(dw, ww, sizeof(double) * (size_t)nq, hipMemcpyHostToDevice));
+    t.w = static_cast<const double*>(dw);
+    ANIFEM_HIP_CHECK(hipMalloc(&d64, R.size() * sizeof(double)));
+    ANIFEM_HIP_CHECK(hipMalloc(&d32, Rf.size() * sizeof(float)));
+    ANIFEM_HIP_CHECK(hipMemcpy(d64, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice));
+    ANIFEM_HIP_CHECK(hipMemcpy(d32, Rf.data(), Rf.size() * sizeof(float), hipMemcpyHostToDevice));
+    t.r64 = static_cast<const double*>(d64);
+    t.r32 = static_cast<const float*>(d32);
+    cache[key] = t;
+    *out = t;
+    return ANIFEM_OK;
+}
+
+}  // namespace anifem
+
+extern "C" {
+
+int anifem_op_nfa(anifem_op op) { return anifem::op_valid(op) ? anifem::nf_scalar(op.fem) * op.ncomp : -1; }
+int anifem_op_dim(anifem_op op) { return anifem::op_valid(op) ? anifem::ref_dim(op.op) * op.ncomp : -1; }
+
+int anifem_quadrature_npoints(int order) {
+    const double *p, *w;
+    return anifem::quadrature(order, &p, &w);
+}
+
+int anifem_quadrature(int order, double* p, double* w) {
+    const double *pp, *ww;
+    int q = anifem::quadrature(order, &pp, &ww);
+    if (q < 0) return ANIFEM_ERR_ORDER;
+    for (int i = 0; i < 4 * q; ++i) p[i] = pp[i];
+    for (int i = 0; i < q; ++i) w[i] = ww[i];
+    return q;
+}
+
+const char* anifem_version(void) { return "anifem-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

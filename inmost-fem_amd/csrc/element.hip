@@ -1,0 +1,105 @@
+// Batched element matrices: anifem_fem3dtet_batch (C ABI) and the launch
+// helpers shared with the fused assembly.
+#include <hip/hip_runtime.h>
+
+#include "anifem_common.h"
+#include "element_kernels.h"
+
+namespace anifem {
+
+// Operator-dependent part of ElemArgs; validates the pair like the reference's
+// compile-time / runtime checks (int_tet.inl:94-104, diff_tensor.h:313-315).
+int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a) {
+    if (!op_valid(opA) || !op_valid(opB) || !D) return ANIFEM_ERR_ARG;
+    const double *p, *w;
+    int q = quadrature(order, &p, &w);
+    if (q < 0) return ANIFEM_ERR_ORDER;
+    a.nfA0 = nf_scalar(opA.fem);
+    a.nfB0 = nf_scalar(opB.fem);
+    a.dimA = ref_dim(opA.op) * opA.ncomp;
+    a.dimB = ref_dim(opB.op) * opB.ncomp;
+    a.E = a.nfA0 * opA.ncomp * a.nfB0 * opB.ncomp;
+    a.ttype = D->type;
+    a.storage = D->storage;
+    a.D = D->data;
+    const int nfA = a.nfA0 * opA.ncomp;
+    a.p0_bcast = (a.dimA != a.dimB && nfA == 1 && a.dimA == 1) ? 1 : 0;
+    if ((D->type == ANIFEM_TENSOR_NULL || D->type == ANIFEM_TENSOR_SCALAR) && a.dimA != a.dimB && !a.p0_bcast)
+        return ANIFEM_ERR_DIMS;
+    if (tensor_ddim(D->type, a.dimA, a.dimB) < 0) return ANIFEM_ERR_ARG;
+    if (D->type != ANIFEM_TENSOR_NULL && !D->data) return ANIFEM_ERR_ARG;
+    if (D->storage < ANIFEM_COEF_CONST || D->storage > ANIFEM_COEF_PER_POINT) return ANIFEM_ERR_ARG;
+    RefTensorDev R;
+    int rc = get_ref_tensor(opA.op, opA.fem, opB.op, opB.fem, order, D->storage == ANIFEM_COEF_PER_POINT, &R);
+    if (rc) return rc;
+    a.q = R.q;
+    a.Rd = R.r64;
+    a.Rf = R.r32;
+    a.wq = R.w;
+    return ANIFEM_OK;
+}
+
+// Tets per workgroup: as many as keep the LDS geometry tensors <= 32 KiB
+// (several workgroups per CU), at most `cap`.
+static int choose_tile(int kstride, int elem_bytes, int cap) {
+    int t = cap;
+    while (t > 4 && (size_t)t * kstride * elem_bytes > 32768) t /= 2;
+    return t;
+}
+
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool MESH, int MODE>
+static int launch_one(ElemArgs a, int cap, hipStream_t s) {
+    const int kstride = NCA * NCB * a.q * DA0 * DB0;
+    a.tile = choose_tile(kstride, (int)sizeof(T), cap);
+    const size_t lds = (size_t)a.tile * kstride * sizeof(T);
+    const int64_t blocks = (a.n + a.tile - 1) / a.tile;
+    if (blocks == 0) return ANIFEM_OK;
+    hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T, MESH, MODE>), dim3((unsigned)blocks), dim3(kBlock),
+                       lds, s, a);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
+template <typename T, bool MESH, int MODE, int DA0, int DB0>
+static int launch_nc(int ncA, int ncB, const ElemArgs& a, int cap, hipStream_t s) {
+    if (ncA == 1 && ncB == 1) return launch_one<DA0, DB0, 1, 1, T, MESH, MODE>(a, cap, s);
+    if (ncA == 3 && ncB == 3) return launch_one<DA0, DB0, 3, 3, T, MESH, MODE>(a, cap, s);
+    if (ncA == 1 && ncB == 3) return launch_one<DA0, DB0, 1, 3, T, MESH, MODE>(a, cap, s);
+    if (ncA == 3 && ncB == 1) return launch_one<DA0, DB0, 3, 1, T, MESH, MODE>(a, cap, s);
+    return ANIFEM_ERR_ARG;
+}
+
+template <typename T, bool MESH, int MODE>
+int launch_element(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a, int cap, hipStream_t s) {
+    const int dA = ref_dim(opA.op), dB = ref_dim(opB.op);
+    if (dA == 3 && dB == 3) return launch_nc<T, MESH, MODE, 3, 3>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 1 && dB == 1) return launch_nc<T, MESH, MODE, 1, 1>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 3 && dB == 1) return launch_nc<T, MESH, MODE, 3, 1>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 1 && dB == 3) return launch_nc<T, MESH, MODE, 1, 3>(opA.ncomp, opB.ncomp, a, cap, s);
+    return ANIFEM_ERR_ARG;
+}
+
+template int launch_element<double, true, 1>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
+template int launch_element<float, true, 1>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
+template int launch_element<double, true, 2>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
+template int launch_element<float, true, 2>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
+
+}  // namespace anifem
+
+extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
+                                     const double* xyz, int precision, void* A_out, void* stream) {
+    using namespace anifem;
+    if (n_tets < 0 || (n_tets > 0 && (!xyz || !A_out))) return ANIFEM_ERR_ARG;
+    ElemArgs a{};
+    int rc = prepare_elem_args(opA, opB, D, order, a);
+    if (rc) return rc;
+    if (n_tets == 0) return ANIFEM_OK;
+    a.n = n_tets;
+    a.coef_n = n_tets;
+    a.xyz = xyz;
+    a.out = A_out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (precision == ANIFEM_PREC_FP64) return launch_element<double, false, 0>(opA, opB, a, 256, s);
+    if (precision == ANIFEM_PREC_FP32) return launch_element<float, false, 0>(opA, opB, a, 256, s);
+    return ANIFEM_ERR_ARG;
+}

@@ -1,0 +1,355 @@
+// Host-side assembly setup (see setup.h).
+//
+// Follows the reference's assembly-side semantics:
+//   * local node reorder: swap nodes 2 and 3 when det(P0-P3, P1-P3, P2-P3) < 0
+//     (inmost_interface/ordering.inl:8-26; on by default, assembler.h:298);
+//   * NATURAL global enumeration: (component, geometric type {node, edge,
+//     face}, element id, dof on element) (global_enumerator.h:392-404,
+//     global_enumerator.cpp:562-777);
+//   * P3 edge DOF pairs ordered by global node id: the pair keeps its local
+//     order when perm[I] < perm[J] with perm = createOrderPermutation(global
+//     node ids) (ordering.inl:137-151, tetdofmap.inl:98-105, applied in
+//     fill_assemble_templates, assembler.inl:140-184);
+//   * CSR pattern: every DOF pair sharing a tet plus the diagonal
+//     (assembler.inl:114-136), minus the off-diagonals of Dirichlet rows and
+//     columns, whose element entries applyDir makes exactly zero and the drop
+//     rule |a| <= drop_val never inserts (assembler.inl:416, dc_on_dof.h:38-45).
+// Edge and face ids are assigned by first appearance in the caller's cell
+// order over the local edges 01,02,03,12,13,23 and local faces
+// {k, k+1, k+2 mod 4}; the reference takes them from INMOST (absent here).
+#include "setup.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <functional>
+#include <thread>
+
+namespace anifem {
+
+int nf_scalar(int fem);
+
+void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int64_t nt = std::max<int64_t>(1, std::min<int64_t>(hw ? hw : 1, 32));
+    if (n <= grain || nt == 1) {
+        fn(0, n);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (;;) {
+                int64_t b = next.fetch_add(grain);
+                if (b >= n) break;
+                fn(b, std::min(n, b + grain));
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+namespace {
+
+const int kEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+const int kFace[4][3] = {{0, 1, 2}, {1, 2, 3}, {2, 3, 0}, {3, 0, 1}};
+
+// Ids by first appearance.  keys[t*ne + e] identifies the entity; returns the
+// number of distinct entities and fills ids (same indexing as keys).
+template <typename Key>
+int64_t number_by_first_appearance(const std::vector<Key>& keys, std::vector<int64_t>& ids,
+                                   std::vector<int32_t>* mult = nullptr) {
+    const int64_t m = (int64_t)keys.size();
+    std::vector<int64_t> order(m);
+    for (int64_t i = 0; i < m; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return keys[a] < keys[b] || (keys[a] == keys[b] && a < b);
+    });
+    // first occurrence of each distinct key, in key order
+    std::vector<int64_t> first;
+    std::vector<int64_t> group(m);
+    for (int64_t i = 0; i < m; ++i) {
+        if (i == 0 || keys[order[i]] != keys[order[i - 1]]) first.push_back(order[i]);
+        group[order[i]] = (int64_t)first.size() - 1;
+    }
+    const int64_t nu = (int64_t)first.size();
+    std::vector<int64_t> rank(nu), by_first(nu);
+    for (int64_t u = 0; u < nu; ++u) by_first[u] = u;
+    std::sort(by_first.begin(), by_first.end(), [&](int64_t a, int64_t b) { return first[a] < first[b]; });
+    for (int64_t r = 0; r < nu; ++r) rank[by_first[r]] = r;
+    ids.resize(m);
+    for (int64_t i = 0; i < m; ++i) ids[i] = rank[group[i]];
+    if (mult) {
+        mult->assign(nu, 0);
+        for (int64_t i = 0; i < m; ++i) (*mult)[ids[i]]++;
+    }
+    return nu;
+}
+
+// createOrderPermutation (ordering.inl:137-151)
+void order_permutation(const int64_t g[4], unsigned char perm[4]) {
+    unsigned char i = 0, j = 0;
+    for (int k = 1; k < 4; ++k) {
+        if (g[k] < g[i]) i = (unsigned char)k;
+        else if (g[k] > g[j]) j = (unsigned char)k;
+    }
+    perm[i] = 0;
+    perm[j] = 3;
+    unsigned char qid = (unsigned char)(i + j - 1 + (std::min(i, j) > 0 ? 1 : 0));
+    static const unsigned char lookup[12] = {2, 3, 1, 3, 1, 2, 0, 3, 0, 2, 0, 1};
+    i = lookup[2 * qid], j = lookup[2 * qid + 1];
+    if (g[i] < g[j]) perm[i] = 1, perm[j] = 2;
+    else perm[i] = 2, perm[j] = 1;
+}
+
+}  // namespace
+
+int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
+    const int fem = prob.trial.fem;
+    if (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
+    if (prob.test.fem != fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
+    S.fem = fem;
+    S.ncomp = prob.trial.ncomp;
+    const int nf0 = nf_scalar(fem);
+    S.nf = nf0 * S.ncomp;
+    S.E = S.nf * S.nf;
+    if (S.nf > 64) return ANIFEM_ERR_ARG;  // Dirichlet masks are 64-bit
+    S.n_nodes = mesh.n_nodes;
+    S.n_tets = mesh.n_tets;
+    const int64_t NT = S.n_tets, NN = S.n_nodes;
+    if (NT <= 0 || NN <= 0 || NT > INT32_MAX / 64) return ANIFEM_ERR_ARG;
+
+    // node reorder (ordering.inl:8-26) and connectivity check
+    S.tets.assign(mesh.tets, mesh.tets + 4 * NT);
+    std::atomic<int64_t> bad{0}, nonpos{0};
+    const double* X = mesh.coords;
+    host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
+        int64_t lbad = 0, lnp = 0;
+        for (int64_t t = b; t < e; ++t) {
+            int32_t* v = &S.tets[4 * t];
+            bool ok = true;
+            for (int k = 0; k < 4; ++k) ok = ok && v[k] >= 0 && v[k] < NN;
+            for (int k = 0; k < 4 && ok; ++k)
+                for (int l = k + 1; l < 4; ++l) ok = ok && v[k] != v[l];
+            if (!ok) {
+                ++lbad;
+                continue;
+            }
+            double m[3][3];
+            for (int j = 0; j < 3; ++j) {
+                m[0][j] = X[3 * v[0] + j] - X[3 * v[3] + j];
+                m[1][j] = X[3 * v[1] + j] - X[3 * v[3] + j];
+                m[2][j] = X[3 * v[2] + j] - X[3 * v[3] + j];
+            }
+            double det = 0;
+            det += m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+            det += m[0][1] * (m[1][2] * m[2][0] - m[1][0] * m[2][2]);
+            det += m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+            if (det < 0) std::swap(v[2], v[3]);
+            if (det == 0 || !std::isfinite(det)) ++lnp;
+        }
+        bad += lbad;
+        nonpos += lnp;
+    });
+    if (bad.load()) return ANIFEM_ERR_CONNECTIVITY;
+    S.n_nonpositive = nonpos.load();
+
+    // edges and faces by first appearance
+    std::vector<int64_t> eid, fid;
+    std::vector<int32_t> fmult;
+    const bool need_edges = fem >= ANIFEM_FEM_P2 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    const bool need_faces = fem >= ANIFEM_FEM_P3 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    if (need_edges) {
+        std::vector<uint64_t> keys(6 * NT);
+        host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
+            for (int64_t t = b; t < e; ++t)
+                for (int k = 0; k < 6; ++k) {
+                    uint64_t a = (uint64_t)S.tets[4 * t + kEdge[k][0]], c = (uint64_t)S.tets[4 * t + kEdge[k][1]];
+                    if (a > c) std::swap(a, c);
+                    keys[6 * t + k] = a * (uint64_t)NN + c;
+                }
+        });
+        S.n_edges = number_by_first_appearance(keys, eid);
+    }
+    if (need_faces) {
+        std::vector<unsigned __int128> keys(4 * NT);
+        host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
+            for (int64_t t = b; t < e; ++t)
+                for (int k = 0; k < 4; ++k) {
+                    uint64_t v[3];
+                    for (int i = 0; i < 3; ++i) v[i] = (uint64_t)S.tets[4 * t + kFace[k][i]];
+                    std::sort(v, v + 3);
+                    keys[4 * t + k] = ((unsigned __int128)v[0] << 64) | ((unsigned __int128)v[1] << 32) | v[2];
+                }
+        });
+        S.n_faces = number_by_first_appearance(keys, fid, &fmult);
+    }
+
+    // NATURAL enumeration
+    const int64_t per_edge = fem == ANIFEM_FEM_P2 ? 1 : (fem == ANIFEM_FEM_P3 ? 2 : 0);
+    const int64_t per_face = fem == ANIFEM_FEM_P3 ? 1 : 0;
+    const int64_t nscal = NN + per_edge * S.n_edges + per_face * S.n_faces;
+    S.ndof = nscal * S.ncomp;
+    if (S.ndof >= INT32_MAX) return ANIFEM_ERR_ARG;
+    S.dofs.resize((size_t)S.nf * NT);
+    host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
+        for (int64_t t = b; t < e; ++t) {
+            const int32_t* v = &S.tets[4 * t];
+            int64_t loc[20];
+            for (int i = 0; i < 4; ++i) loc[i] = v[i];
+            if (fem == ANIFEM_FEM_P2)
+                for (int k = 0; k < 6; ++k) loc[4 + k] = NN + eid[6 * t + k];
+            if (fem == ANIFEM_FEM_P3) {
+                int64_t g[4] = {v[0], v[1], v[2], v[3]};
+                unsigned char perm[4];
+                order_permutation(g, perm);
+                for (int k = 0; k < 6; ++k) {
+                    int64_t base = NN + 2 * eid[6 * t + k];
+                    bool keep = perm[kEdge[k][0]] < perm[kEdge[k][1]];
+                    loc[4 + 2 * k] = base + (keep ? 0 : 1);
+                    loc[5 + 2 * k] = base + (keep ? 1 : 0);
+                }
+                for (int k = 0; k < 4; ++k) loc[16 + k] = NN + 2 * S.n_edges + fid[4 * t + k];
+            }
+            for (int c = 0; c < S.ncomp; ++c)
+                for (int i = 0; i < nf0; ++i) S.dofs[(size_t)S.nf * t + nf0 * c + i] = (int32_t)(c * nscal + loc[i]);
+        }
+    });
+
+    // Dirichlet flags
+    S.is_dir.assign(S.ndof, 0);
+    if (prob.dirichlet == ANIFEM_DIRICHLET_DOF_MASK) {
+        if (!prob.dirichlet_dof_mask) return ANIFEM_ERR_ARG;
+        for (int64_t d = 0; d < S.ndof; ++d) S.is_dir[d] = prob.dirichlet_dof_mask[d] ? 1 : 0;
+    } else if (prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY) {
+        // boundary faces belong to exactly one tet; their closure is Dirichlet
+        std::vector<uint8_t> snode(NN, 0), sedge(S.n_edges, 0), sface(S.n_faces, 0);
+        for (int64_t t = 0; t < NT; ++t)
+            for (int k = 0; k < 4; ++k) {
+                if (fmult[fid[4 * t + k]] != 1) continue;
+                sface[fid[4 * t + k]] = 1;
+                for (int i = 0; i < 3; ++i) snode[S.tets[4 * t + kFace[k][i]]] = 1;
+                for (int q = 0; q < 6; ++q) {
+                    int a = kEdge[q][0], c = kEdge[q][1];
+                    bool in = false, in2 = false;
+                    for (int i = 0; i < 3; ++i) {
+                        in = in || kFace[k][i] == a;
+                        in2 = in2 || kFace[k][i] == c;
+                    }
+                    if (in && in2) sedge[eid[6 * t + q]] = 1;
+                }
+            }
+        for (int c = 0; c < S.ncomp; ++c) {
+            for (int64_t i = 0; i < NN; ++i) S.is_dir[c * nscal + i] = snode[i];
+            for (int64_t i = 0; i < S.n_edges; ++i)
+                for (int s = 0; s < per_edge; ++s) S.is_dir[c * nscal + NN + per_edge * i + s] = sedge[i];
+            for (int64_t i = 0; i < S.n_faces && per_face; ++i) S.is_dir[c * nscal + NN + per_edge * S.n_edges + i] = sface[i];
+        }
+    } else if (prob.dirichlet != ANIFEM_DIRICHLET_NONE) {
+        return ANIFEM_ERR_ARG;
+    }
+
+    // DOF -> tet adjacency
+    std::vector<int64_t> aptr(S.ndof + 1, 0);
+    for (int64_t i = 0; i < (int64_t)S.dofs.size(); ++i) aptr[S.dofs[i] + 1]++;
+    for (int64_t d = 0; d < S.ndof; ++d) aptr[d + 1] += aptr[d];
+    std::vector<int32_t> adj(aptr[S.ndof]);
+    {
+        std::vector<int64_t> fill(aptr.begin(), aptr.end() - 1);
+        for (int64_t t = 0; t < NT; ++t)
+            for (int i = 0; i < S.nf; ++i) adj[fill[S.dofs[(size_t)S.nf * t + i]]++] = (int32_t)t;
+    }
+
+    // CSR pattern: per row, sort + unique of the DOFs of its tets
+    S.row_ptr.assign(S.ndof + 1, 0);
+    const int nf = S.nf;
+    auto row_cols = [&](int64_t d, std::vector<int32_t>& buf) {
+        buf.clear();
+        buf.push_back((int32_t)d);
+        const bool rd = S.is_dir[d];
+        for (int64_t p = aptr[d]; p < aptr[d + 1]; ++p) {
+            const int32_t* td = &S.dofs[(size_t)nf * adj[p]];
+            for (int j = 0; j < nf; ++j)
+                if (!(rd || S.is_dir[td[j]])) buf.push_back(td[j]);
+        }
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+    };
+    host_parallel_for(S.ndof, 4096, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> buf;
+        for (int64_t d = b; d < e; ++d) {
+            row_cols(d, buf);
+            S.row_ptr[d + 1] = (int64_t)buf.size();
+        }
+    });
+    for (int64_t d = 0; d < S.ndof; ++d) S.row_ptr[d + 1] += S.row_ptr[d];
+    S.nnz = S.row_ptr[S.ndof];
+    if (S.nnz >= INT32_MAX) return ANIFEM_ERR_ARG;  // slots are int32
+    S.col_idx.resize(S.nnz);
+    host_parallel_for(S.ndof, 4096, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> buf;
+        for (int64_t d = b; d < e; ++d) {
+            row_cols(d, buf);
+            std::copy(buf.begin(), buf.end(), S.col_idx.begin() + S.row_ptr[d]);
+        }
+    });
+
+    // greedy colouring in the caller's tet order: tets sharing a node conflict
+    {
+        std::vector<int64_t> nptr(NN + 1, 0);
+        for (int64_t i = 0; i < 4 * NT; ++i) nptr[S.tets[i] + 1]++;
+        for (int64_t i = 0; i < NN; ++i) nptr[i + 1] += nptr[i];
+        std::vector<int32_t> nadj(4 * NT);
+        std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+        for (int64_t t = 0; t < NT; ++t)
+            for (int k = 0; k < 4; ++k) nadj[fill[S.tets[4 * t + k]]++] = (int32_t)t;
+        S.colour.assign(NT, -1);
+        int ncol = 0;
+        std::vector<uint64_t> used(32);
+        for (int64_t t = 0; t < NT; ++t) {
+            std::fill(used.begin(), used.end(), 0);
+            for (int k = 0; k < 4; ++k) {
+                int32_t v = S.tets[4 * t + k];
+                for (int64_t p = nptr[v]; p < nptr[v + 1]; ++p) {
+                    int32_t c = S.colour[nadj[p]];
+                    if (c >= 0) used[c >> 6] |= 1ull << (c & 63);
+                }
+            }
+            int c = 0;
+            while (used[c >> 6] >> (c & 63) & 1ull) ++c;
+            if (c >= 2048) return ANIFEM_ERR_ARG;
+            S.colour[t] = c;
+            ncol = std::max(ncol, c + 1);
+        }
+        S.n_colours = ncol;
+    }
+    // assembly order: colour-major, caller order within a colour
+    S.colour_ptr.assign(S.n_colours + 1, 0);
+    for (int64_t t = 0; t < NT; ++t) S.colour_ptr[S.colour[t] + 1]++;
+    for (int c = 0; c < S.n_colours; ++c) S.colour_ptr[c + 1] += S.colour_ptr[c];
+    S.perm.resize(NT);
+    {
+        std::vector<int64_t> fill(S.colour_ptr.begin(), S.colour_ptr.end() - 1);
+        for (int64_t t = 0; t < NT; ++t) S.perm[fill[S.colour[t]]++] = (int32_t)t;
+    }
+
+    // slot map in assembly order
+    S.slot.resize((size_t)S.E * NT);
+    host_parallel_for(NT, 1 << 12, [&](int64_t b, int64_t e) {
+        for (int64_t pos = b; pos < e; ++pos) {
+            const int32_t* td = &S.dofs[(size_t)nf * S.perm[pos]];
+            int32_t* sl = &S.slot[(size_t)S.E * pos];
+            for (int ia = 0; ia < nf; ++ia)
+                for (int ib = 0; ib < nf; ++ib) {
+                    const int32_t r = td[ib], c = td[ia];
+                    const int32_t* b0 = S.col_idx.data() + S.row_ptr[r];
+                    const int32_t* e0 = S.col_idx.data() + S.row_ptr[r + 1];
+                    const int32_t* it = std::lower_bound(b0, e0, c);
+                    sl[ib + nf * ia] = (it != e0 && *it == c) ? (int32_t)(it - S.col_idx.data()) : -1;
+                }
+        }
+    });
+    return ANIFEM_OK;
+}
+
+}  // namespace anifem

@@ -1,0 +1,170 @@
+"""Test helpers: the CPU oracle (oracle/liboracle.so, the checker) and the
+conversions between the oracle's input layouts and the product's C ABI
+layouts.  Only tests import this module."""
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libanifem_ref.so")
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+
+NF = {1: 1, 2: 4, 3: 10, 4: 20}
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True, capture_output=True)
+        L = ctypes.CDLL(ORACLE_LIB)
+        L.orc_enumerate.restype = ctypes.c_long
+        L.orc_csr_pattern.restype = ctypes.c_long
+        L.orc_jacobi_pcg.restype = ctypes.c_long
+        L.orc_inverse3x3.restype = ctypes.c_double
+        _oracle = L
+    return _oracle
+
+
+def ref_lib():
+    """The compiled reference element core, or None where it is not built."""
+    if not os.path.exists(REF_LIB):
+        return None
+    return ctypes.CDLL(REF_LIB)
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def nfa(fem, nc=1):
+    return NF[fem] * nc
+
+
+def dim(op, nc=1):
+    return (3 if op == 2 else 1) * nc
+
+
+def orc_quadrature(order):
+    p = np.zeros(4 * 24)
+    w = np.zeros(24)
+    q = oracle().orc_quadrature(order, P(p), P(w))
+    return p[:4 * q].reshape(q, 4), w[:q]
+
+
+def orc_elem(opA, femA, ncA, opB, femB, ncB, ttype, mode, order, xyz, D):
+    """Oracle element matrices: returns (n, nfB*nfA) column-major per tet."""
+    n = len(xyz)
+    A = np.zeros(n * nfa(femA, ncA) * nfa(femB, ncB))
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    D = None if D is None else np.ascontiguousarray(D, dtype=np.float64)
+    rc = oracle().orc_fem3dtet(opA, femA, ncA, opB, femB, ncB, ttype, mode, order, ctypes.c_long(n), P(xyz), P(D),
+                               P(A))
+    if rc != 0:
+        raise RuntimeError(f"oracle fem3dtet rc={rc}")
+    return A.reshape(n, -1)
+
+
+def physical_points(xyz, order):
+    """Physical quadrature points (n, q, 3) of the rule (core.inl:250-270)."""
+    p, _ = orc_quadrature(order)
+    P0 = xyz[:, 0:3]
+    X = np.repeat(P0[:, None, :], len(p), axis=1).copy()
+    for l in range(3):
+        X += p[None, :, l + 1, None] * (xyz[:, None, 3 * (l + 1):3 * (l + 2)] - P0[:, None, :])
+    return X
+
+
+def xdep_field(X):
+    """m(x) = 1 + x0/2 - x1/4 + x2^2/8 (the same field as oracle/ref_harness.cpp)."""
+    return 1.0 + 0.5 * X[..., 0] - 0.25 * X[..., 1] + 0.125 * X[..., 2] ** 2
+
+
+def pack_sym(Dfull, d):
+    """(n, d*d) column-major full -> (n, d(d+1)/2) packed upper triangle."""
+    n = len(Dfull)
+    M = Dfull.reshape(n, d, d).transpose(0, 2, 1)  # M[r, i, j] = D(i, j)
+    out = np.zeros((n, d * (d + 1) // 2))
+    for j in range(d):
+        for i in range(j + 1):
+            out[:, i + j * (j + 1) // 2] = M[:, i, j]
+    return out
+
+
+def soa_xyz(xyz):
+    """(n, 12) per-tet vertices -> (12, n) SoA of the C ABI."""
+    return np.ascontiguousarray(np.asarray(xyz).T)
+
+
+def rel_frob(A, B):
+    """max over tets of ||A-B||_F / ||B||_F."""
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    nb = np.linalg.norm(B, axis=1)
+    return float(np.max(np.linalg.norm(A - B, axis=1) / np.maximum(nb, 1e-300)))
+
+
+def golden_cases():
+    out = []
+    for f in sorted(glob.glob(os.path.join(GOLDEN_DIR, "elem_*.npz"))):
+        z = np.load(f, allow_pickle=False)
+        out.append((os.path.basename(f)[5:-4], {k: z[k] for k in z.files}))
+    return out
+
+
+def oracle_assembly(coords, tets, fem, nc, dirichlet_mask=None, with_slots=True):
+    """Oracle enumeration / pattern / slot map of a mesh (reorder applied)."""
+    O = oracle()
+    tt = np.ascontiguousarray(tets, dtype=np.int32).copy()
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    O.orc_reorder_nodes(ctypes.c_long(len(tt)), P(tt), P(coords))
+    nf = nfa(fem, nc)
+    dofs = np.zeros((len(tt), nf), np.int32)
+    ne, nfc = ctypes.c_long(), ctypes.c_long()
+    ndof = O.orc_enumerate(fem, nc, ctypes.c_long(len(coords)), ctypes.c_long(len(tt)), P(tt), P(dofs),
+                           ctypes.byref(ne), ctypes.byref(nfc))
+    mask = None if dirichlet_mask is None else np.ascontiguousarray(dirichlet_mask, dtype=np.uint8)
+    rp = np.zeros(ndof + 1, np.int64)
+    nnz = O.orc_csr_pattern(ctypes.c_long(ndof), ctypes.c_long(len(tt)), nf, P(dofs), P(mask), P(rp), None)
+    ci = np.zeros(nnz, np.int32)
+    O.orc_csr_pattern(ctypes.c_long(ndof), ctypes.c_long(len(tt)), nf, P(dofs), P(mask), P(rp), P(ci))
+    out = dict(tets=tt, dofs=dofs, ndof=ndof, row_ptr=rp, col_idx=ci, n_edges=ne.value, n_faces=nfc.value)
+    if with_slots:
+        sl = np.zeros((len(tt), nf * nf), np.int64)
+        O.orc_slot_map(ctypes.c_long(len(tt)), nf, P(dofs), P(rp), P(ci), P(sl))
+        out["slot"] = sl
+    return out
+
+
+def oracle_scatter(n_tets, nf, dofs, slot, A, F, nnz, ndof, drop_val=1e-100):
+    val = np.zeros(nnz)
+    rhs = np.zeros(ndof)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    F = None if F is None else np.ascontiguousarray(F, dtype=np.float64)
+    rc = oracle().orc_scatter(ctypes.c_long(n_tets), nf, P(dofs), P(slot), P(A), P(F), ctypes.c_double(drop_val),
+                              P(val), P(rhs))
+    return rc, val, rhs
+
+
+def apply_dir_elementwise(A, F, dofs, is_dir):
+    """Element-level homogeneous Dirichlet through the oracle's applyDir."""
+    O = oracle()
+    n, nf = dofs.shape
+    A = A.copy()
+    F = F.copy()
+    for r in range(n):
+        a = np.ascontiguousarray(A[r])
+        f = np.ascontiguousarray(F[r])
+        for k in range(nf):
+            if is_dir[dofs[r, k]]:
+                O.orc_apply_dir(nf, P(a), P(f), k, ctypes.c_double(0.0))
+        A[r] = a
+        F[r] = f
+    return A, F

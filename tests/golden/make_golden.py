@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Write the golden element-matrix fixtures (tests/golden/elem_*.npz).
+
+Runs in the development container only: it loads oracle/_ref/libanifem_ref.so,
+the reference AniFem++ element core compiled from its own sources by
+oracle/Makefile, feeds it seeded tetrahedra and coefficients shaped like the
+benchmark configurations, and stores inputs and outputs as FP64 arrays.
+The fixtures are data (inputs + reference outputs); no reference source is
+stored.  Regenerate with:  make -C oracle && python tests/golden/make_golden.py
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "libanifem_ref.so")
+
+# harness pair codes (oracle/ref_harness.cpp) -> (opA, femA, ncA, opB, femB, ncB)
+PAIRS = {
+    0: (2, 2, 1, 2, 2, 1), 1: (2, 3, 1, 2, 3, 1), 2: (2, 4, 1, 2, 4, 1), 3: (2, 2, 3, 2, 2, 3),
+    4: (2, 3, 3, 2, 3, 3), 5: (1, 1, 1, 1, 2, 1), 6: (1, 1, 1, 1, 3, 1), 7: (1, 1, 1, 1, 4, 1),
+    8: (1, 1, 1, 1, 3, 3), 9: (1, 2, 1, 1, 2, 1), 10: (1, 3, 1, 1, 3, 1), 11: (1, 4, 1, 1, 4, 1),
+    12: (2, 4, 1, 2, 2, 3), 13: (1, 1, 1, 1, 2, 3), 14: (2, 3, 1, 1, 3, 1),
+}
+NF = {1: 1, 2: 4, 3: 10, 4: 20}
+
+
+def kuhn_tets(rng, n_tets, n=16, jitter=0.2):
+    """Random Kuhn tets of a jittered n^3 cube (positively oriented after the
+    2<->3 swap rule), xyz (n_tets, 12)."""
+    h = 1.0 / n
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    out = np.zeros((n_tets, 12))
+    for r in range(n_tets):
+        c0 = rng.integers(0, n, 3)
+        p = perms[rng.integers(0, 6)]
+        verts = [c0.copy()]
+        c = c0.copy()
+        for s in range(3):
+            c[p[s]] += 1
+            verts.append(c.copy())
+        P = np.array(verts, dtype=float) * h
+        for v in range(4):
+            interior = np.all((verts[v] > 0) & (verts[v] < n))
+            if interior:
+                P[v] += rng.uniform(-jitter * h, jitter * h, 3)
+        m = np.array([P[0] - P[3], P[1] - P[3], P[2] - P[3]])
+        if np.linalg.det(m) < 0:
+            P[[2, 3]] = P[[3, 2]]
+        out[r] = P.ravel()
+    return out
+
+
+def haar_rotation(rng):
+    Q, R = np.linalg.qr(rng.normal(size=(3, 3)))
+    Q = Q @ np.diag(np.sign(np.diag(R)))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] = -Q[:, 0]
+    return Q
+
+
+def aniso_tensor(rng):
+    a, b = rng.uniform(-2, 2, 2)
+    R = haar_rotation(rng)
+    return R @ np.diag([1.0, 10.0 ** a, 10.0 ** b]) @ R.T
+
+
+def lame_tensor(lam, mu):
+    """9x9 C((i + 3j), (k + 3l)) of examples/tutorials/lin_elast.cpp:101-117."""
+    C = np.zeros((9, 9))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    C[i + 3 * j, k + 3 * l] = lam * (i == j) * (k == l) + mu * ((i == k) * (j == l) + (i == l) * (j == k))
+    return C
+
+
+def cases(rng):
+    """(name, pair, tensor_type, per_point, order, xyz, D full (n, dimB*dimA) or (n,1))."""
+    out = []
+    n = 64
+    xyz = kuhn_tets(rng, n)
+    out.append(("c1_p1_null", 0, 1, 0, 2, xyz, np.zeros((n, 1))))
+    D = np.array([aniso_tensor(rng).ravel(order="F") for _ in range(n)])
+    out.append(("c2_p2_sym", 1, 3, 0, 2, xyz, D))
+    for name, pair, order in [("c3_p1_scalar", 0, 2), ("c3_p2_scalar", 1, 2), ("c3_p3_scalar", 2, 4)]:
+        out.append((name, pair, 2, 0, order, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))))
+    ne = 16
+    E = 10.0 ** rng.uniform(0, 3, ne)
+    nu = rng.uniform(0.2, 0.45, ne)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.array([lame_tensor(lam[r], mu[r]).ravel(order="F") for r in range(ne)])
+    out.append(("c4_p2vec_lame", 4, 3, 0, 2, xyz[:ne], D, np.stack([lam, mu], axis=1)))
+    for name, pair, order in [("rhs_p1", 5, 2), ("rhs_p2", 6, 2), ("rhs_p3", 7, 4)]:
+        out.append((name, pair, 2, 0, order, xyz, np.ones((n, 1))))
+    out.append(("rhs_vec_p2", 8, 4, 0, 2, xyz, rng.normal(size=(n, 3))))
+    out.append(("mass_p2", 10, 2, 0, 2, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))))
+    out.append(("mass_p3", 11, 2, 0, 5, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))))
+    D = np.array([aniso_tensor(rng).ravel(order="F") for _ in range(n)])
+    out.append(("pp_p2_general_xdep", 1, 4, 2, 4, xyz, D))
+    out.append(("pp_p3_scalar_xdep", 2, 2, 2, 6, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))))
+    out.append(("mixed_p3_vecp1_general", 12, 4, 0, 5, xyz[:32], rng.normal(size=(32, 27))))
+    out.append(("grad_p2_iden_p2_general", 14, 4, 0, 2, xyz, rng.normal(size=(n, 3))))
+    out.append(("vec_p1_general", 3, 4, 0, 2, xyz[:32], rng.normal(size=(32, 81))))
+    return out
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit(f"{REF} missing: build it with `make -C oracle`")
+    L = ctypes.CDLL(REF)
+    L.ref_fem3dtet.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(20260101)
+    for case in cases(rng):
+        name, pair, tt, pp, order, xyz, D = case[:7]
+        extra = case[7] if len(case) > 7 else None
+        oa, fa, ca, ob, fb, cb = PAIRS[pair]
+        nfa, nfb = NF[fa] * ca, NF[fb] * cb
+        n = len(xyz)
+        xyz = np.ascontiguousarray(xyz)
+        D = np.ascontiguousarray(D)
+        A = np.zeros(n * nfa * nfb)
+        rc = L.ref_fem3dtet(pair, tt, pp, order, ctypes.c_long(n), xyz.ctypes.data_as(dp), D.ctypes.data_as(dp),
+                            ctypes.c_long(D.shape[1]), A.ctypes.data_as(dp))
+        if rc != 0:
+            sys.exit(f"{name}: reference harness returned {rc}")
+        meta = np.array([pair, tt, pp, order, oa, fa, ca, ob, fb, cb], dtype=np.int64)
+        arrays = dict(meta=meta, xyz=xyz, D=D, A=A.reshape(n, nfa * nfb))
+        if extra is not None:
+            arrays["lame"] = extra
+        np.savez_compressed(os.path.join(HERE, f"elem_{name}.npz"), **arrays)
+        print(f"{name}: {n} tets, {nfb}x{nfa}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every symbol include/anifem_gpu.h
+declares; argument validation needs no device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import anifem_amd as af
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "anifem_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(anifem_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_exports_every_declared_symbol():
+    L = af.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in af.SIGNATURES, f"{n} has no ctypes signature"
+
+
+def test_version_and_op_sizes():
+    assert "gfx950" in af.version()
+    assert af.Op(af.GRAD, af.FEM_P2).nfa == 10 and af.Op(af.GRAD, af.FEM_P2).dim == 3
+    assert af.Op(af.GRAD, af.FEM_P2, 3).nfa == 30 and af.Op(af.GRAD, af.FEM_P2, 3).dim == 9
+    assert af.Op(af.IDEN, af.FEM_P3).nfa == 20 and af.Op(af.IDEN, af.FEM_P0).nfa == 1
+    assert af.lib().anifem_op_nfa(af.anifem_op(af.GRAD, af.FEM_P0, 1)) < 0
+
+
+def test_quadrature_rules():
+    for order, q in [(0, 1), (1, 1), (2, 4), (3, 14), (4, 14), (5, 14), (6, 24)]:
+        p, w = af.quadrature(order)
+        assert len(w) == q and abs(w.sum() - 1) < 1e-15 and np.all(p > 0)
+        assert np.allclose(p.sum(axis=1), 1, atol=1e-15)
+    with pytest.raises(af.AnifemError):
+        af.quadrature(7)
+
+
+def test_argument_errors_without_device():
+    L = af.lib()
+    t = af.anifem_tensor(af.TENSOR_NULL, af.COEF_CONST, None)
+    g1, g2 = af.Op(af.GRAD, af.FEM_P1).c(), af.Op(af.IDEN, af.FEM_P1).c()
+    buf = np.zeros(16)
+    # order > 6 (int_tet.inl:99-100 throws)
+    assert L.anifem_fem3dtet_batch(g1, g1, ctypes.byref(t), 7, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
+        == af.ERR_ORDER
+    # identity tensor between GRAD (dim 3) and IDEN (dim 1) (diff_tensor.h:313-315)
+    assert L.anifem_fem3dtet_batch(g1, g2, ctypes.byref(t), 2, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
+        == af.ERR_DIMS
+    bad = af.anifem_op(af.GRAD, af.FEM_P0, 1)
+    assert L.anifem_fem3dtet_batch(bad, g1, ctypes.byref(t), 2, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
+        == af.ERR_ARG
+    assert L.anifem_mesh_kuhn_cube(0, 0.2, 0, None, None) == af.ERR_ARG
+    assert L.anifem_mesh_kuhn_cube(4, 0.3, 0, None, None) == af.ERR_ARG

@@ -1,0 +1,239 @@
+"""Pins the CPU oracle (oracle/anifem_oracle.c) before anything is checked
+against it: the reference's own known-answer tables, fixtures produced by the
+reference element core compiled from its sources, closed forms and invariants."""
+import ctypes
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import anifem_testlib as T
+
+GRAD, IDEN = 2, 1
+P0, P1, P2, P3 = 1, 2, 3, 4
+NULL, SCALAR, SYM, GEN = 1, 2, 3, 4
+CONST, PERTET, PERPT = 0, 1, 2
+
+
+def mono_mean(al):
+    a, b, c, d = al
+    f = math.factorial
+    return 6 * f(a) * f(b) * f(c) * f(d) / f(a + b + c + d + 3)
+
+
+@pytest.mark.parametrize("order,deg", [(1, 1), (2, 2), (3, 3), (4, 5), (5, 5), (6, 6)])
+def test_quadrature_exactness(order, deg):
+    # restates QuadratureFormulas.TetraFormulas (tests/fem/quadrature_formulas_test.cpp:7-45)
+    # over every barycentric monomial of degree <= deg
+    p, w = T.orc_quadrature(order)
+    assert abs(w.sum() - 1) < 1e-15
+    assert np.all(p > 0) and np.all(w > 0)
+    for s in range(deg + 1):
+        for al in itertools.product(range(s + 1), repeat=4):
+            if sum(al) != s:
+                continue
+            val = np.sum(w * np.prod(p ** np.array(al), axis=1))
+            assert abs(val - mono_mean(al)) < 1e-15 * max(1, mono_mean(al)) + 1e-16
+    # the reference test's polynomial x + 10 y^(ord-1) + 1000 z^ord
+    x, y, z = p[:, 1], p[:, 2], p[:, 3]
+    d = max(deg - 1, 1)
+    f = x + 10 * y ** d + 1000 * z ** deg
+    exact = 1 / 24 + 10 / ((d + 1) * (d + 2) * (d + 3)) + 1000 / ((deg + 1) * (deg + 2) * (deg + 3))
+    assert abs(np.sum(w * f) / 6 - exact) < 1e-13 * exact
+
+
+def test_quadrature_matches_reference_rules():
+    R = T.ref_lib()
+    if R is None:
+        pytest.skip("reference core not built here")
+    for order in [1, 2, 4, 5, 6]:
+        p, w = T.orc_quadrature(order)
+        q = R.ref_quadrature(order, None, None)
+        pr = np.zeros(4 * q)
+        wr = np.zeros(q)
+        R.ref_quadrature(order, T.P(pr), T.P(wr))
+        pr = pr.reshape(q, 4)
+        assert q == len(p)
+        for i in range(q):  # same point set (orbit order may differ)
+            j = np.argmin(np.abs(pr - p[i]).max(axis=1))
+            assert np.abs(pr[j] - p[i]).max() < 1e-15 and abs(wr[j] - w[i]) < 1e-16
+
+
+def test_inverse3x3_cases():
+    # geometry_test.cpp:113-138 cases: identity, diagonal, general, rotation
+    O = T.oracle()
+    rng = np.random.default_rng(3)
+    th = 0.7
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    for M in [np.eye(3), np.diag([2.0, 3.0, 4.0]), rng.normal(size=(3, 3)), Rz]:
+        m = np.ascontiguousarray(M.ravel(order="F"))
+        inv = np.zeros(9)
+        det = O.orc_inverse3x3(T.P(m), T.P(inv))
+        assert abs(det - np.linalg.det(M)) < 1e-13
+        assert np.allclose(inv.reshape(3, 3, order="F") @ M, np.eye(3), atol=1e-13)
+
+
+def _known():
+    with open(os.path.join(T.GOLDEN_DIR, "int_tet_known_answers.json")) as f:
+        return json.load(f)
+
+
+def test_known_answer_grad_p3_x_grad_vecp1_general_perpoint():
+    # Operations_IntTet_OnePointTensor, first table (int_tet_test.cpp:191-257)
+    K = _known()["grad_p3_x_grad_vecp1_general"]
+    xyz = np.array([sum(K["tet"], [])], dtype=float)
+    order = K["order"]
+    X = T.physical_points(xyz, order)[0]
+    q = len(X)
+    D = np.zeros((1, q, 27))
+    for n in range(q):
+        for i in range(9):
+            for j in range(3):
+                D[0, n, i + 9 * j] = (i * X[n, 0] + j * X[n, 1] + (i % 2) * X[n, 2]) * (3 * i + j)
+    A = T.orc_elem(GRAD, P3, 1, GRAD, P1, 3, GEN, PERPT, order, xyz, D.reshape(1, -1))
+    Am = A.reshape(20, 12).T  # nfB x nfA
+    Aexp = np.array(K["A_times_scale"], dtype=float) / K["scale"]
+    assert np.linalg.norm(Am - Aexp) <= 10 * (1 + np.linalg.norm(Am)) * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("ttype,mode", [(NULL, CONST), (SCALAR, CONST), (SCALAR, PERTET), (SYM, CONST),
+                                        (SYM, PERTET), (GEN, CONST), (GEN, PERTET), (GEN, PERPT)])
+def test_known_answer_grad_vecp1_identity(ttype, mode):
+    # second table of Operations_IntTet_OnePointTensor over the DfuncTraits
+    # combinations of TEST_TRAIT (int_tet_test.cpp:310-389)
+    K = _known()["grad_vecp1_identity"]
+    xyz = np.array([sum(K["tet"], [])], dtype=float)
+    order = K["order"]
+    q = len(T.orc_quadrature(order)[1])
+    if ttype == NULL:
+        D = None
+    elif ttype == SCALAR:
+        D = np.ones((1, 1))
+    else:
+        D = np.eye(9).ravel(order="F")[None, :]
+    if mode == PERPT:
+        D = np.repeat(D[:, None, :], q, axis=1).reshape(1, -1)
+    A = T.orc_elem(GRAD, P1, 3, GRAD, P1, 3, ttype, mode, order, xyz, D)
+    Aexp = np.array(K["A_times_scale"], dtype=float) / K["scale"]
+    assert np.linalg.norm(A.reshape(12, 12).T - Aexp) <= 10 * (1 + np.linalg.norm(Aexp)) * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("ttype,mode", [(SCALAR, CONST), (SCALAR, PERTET), (NULL, CONST)])
+def test_known_answer_rhs_vec_p2(ttype, mode):
+    # RhsEval (int_tet_test.cpp:448-500): int (mu, mu, mu) . phi_i over the tet
+    K = _known()["rhs_vec_p2"]
+    xyz = np.array([sum(K["tet"], [])], dtype=float)
+    mu = 40.0 if ttype == SCALAR else 1.0
+    D = np.array([[mu]]) if ttype == SCALAR else None
+    A = T.orc_elem(IDEN, P0, 1, IDEN, P2, 3, ttype, mode, K["order"], xyz, D)[0]
+    B = np.array(K["B"], dtype=float)
+    assert np.linalg.norm(A - mu / 40.0 * B) <= 10 * (1 + np.linalg.norm(A)) * np.finfo(float).eps
+
+
+def test_textbook_unit_tet():
+    xyz = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1]], dtype=float)
+    A = T.orc_elem(GRAD, P1, 1, GRAD, P1, 1, NULL, CONST, 2, xyz, None).reshape(4, 4)
+    G = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    assert np.allclose(A, G @ G.T / 6, atol=1e-16)
+    A2 = T.orc_elem(GRAD, P2, 1, GRAD, P2, 1, NULL, CONST, 2, xyz, None)
+    assert abs(A2[0, 0] - 0.3) < 1e-15
+    M = T.orc_elem(IDEN, P1, 1, IDEN, P1, 1, NULL, CONST, 2, xyz, None).reshape(4, 4)
+    assert np.allclose(M, (np.ones((4, 4)) + np.eye(4)) / 120, atol=1e-17)
+
+
+def test_golden_fixtures_vs_oracle():
+    cases = T.golden_cases()
+    assert len(cases) >= 15
+    for name, z in cases:
+        pair, tt, pp, order, oa, fa, ca, ob, fb, cb = [int(v) for v in z["meta"]]
+        xyz, D = z["xyz"], z["D"]
+        if pp == 2:  # per-point: tensor modulated by the xdep field at the physical points
+            m = T.xdep_field(T.physical_points(xyz, order))
+            Dp = (m[:, :, None] * D[:, None, :]).reshape(len(xyz), -1)
+            A = T.orc_elem(oa, fa, ca, ob, fb, cb, tt, PERPT, order, xyz, Dp)
+        else:
+            A = T.orc_elem(oa, fa, ca, ob, fb, cb, tt, PERTET, order, xyz, D if tt != NULL else None)
+        err = T.rel_frob(A, z["A"])
+        assert err < 1e-13, (name, err)
+
+
+def _mesh_system(n, fem, dirichlet):
+    import anifem_amd as af
+    coords, tets = af.kuhn_cube(n, 0.2, 11)
+    S = T.oracle_assembly(coords, tets, fem, 1)
+    xyz = coords[S["tets"]].reshape(len(tets), 12)
+    A = T.orc_elem(GRAD, fem, 1, GRAD, fem, 1, NULL, CONST, 2, xyz, None)
+    F = T.orc_elem(IDEN, P0, 1, IDEN, fem, 1, SCALAR, CONST, 2, xyz, np.ones(1))
+    return coords, tets, S, xyz, A, F
+
+
+def test_oracle_assembly_invariants():
+    coords, tets, S, xyz, A, F = _mesh_system(3, P2, False)
+    rc, val, rhs = T.oracle_scatter(len(tets), 10, S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
+    assert rc == 0
+    rp, ci = S["row_ptr"], S["col_idx"]
+    K = np.zeros((S["ndof"], S["ndof"]))
+    for i in range(S["ndof"]):
+        K[i, ci[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    # stiffness annihilates constants and, with exact quadrature, reproduces
+    # the energy of a linear field: u^T K u = |grad u|^2 |Omega|
+    assert np.abs(K.sum(axis=1)).max() < 1e-12
+    assert np.allclose(K, K.T, atol=1e-13)
+    assert abs(rhs.sum() - 1.0) < 1e-13  # int_Omega 1 = |Omega|
+    # linear field u = 2x - y + 3z, sampled at DOF positions
+    pos = np.zeros((S["ndof"], 3))
+    for t in range(len(tets)):
+        P = xyz[t].reshape(4, 3)
+        for i in range(4):
+            pos[S["dofs"][t, i]] = P[i]
+        for e, (a, b) in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]):
+            pos[S["dofs"][t, 4 + e]] = 0.5 * (P[a] + P[b])
+    u = pos @ np.array([2.0, -1.0, 3.0])
+    assert abs(u @ K @ u - 14.0) < 1e-11
+
+
+def test_oracle_dirichlet_and_cg():
+    coords, tets, S0, xyz, A, F = _mesh_system(4, P1, True)
+    # boundary nodes of the unit cube
+    isdir = np.any((coords < 1e-12) | (coords > 1 - 1e-12), axis=1).astype(np.uint8)
+    S = T.oracle_assembly(coords, tets, P1, 1, dirichlet_mask=isdir)
+    Ad, Fd = T.apply_dir_elementwise(A, F, S["dofs"], isdir)
+    rc, val, rhs = T.oracle_scatter(len(tets), 4, S["dofs"], S["slot"], Ad, Fd, len(S["col_idx"]), S["ndof"])
+    assert rc == 0
+    rp, ci = S["row_ptr"], S["col_idx"]
+    inc = np.bincount(S["dofs"].ravel(), minlength=S["ndof"])
+    for i in np.nonzero(isdir)[0]:
+        # T3: a Dirichlet row holds only its diagonal = number of incident tets
+        assert rp[i + 1] - rp[i] == 1 and ci[rp[i]] == i and val[rp[i]] == inc[i] and rhs[i] == 0
+    x = np.zeros(S["ndof"])
+    res = ctypes.c_double()
+    it = T.oracle().orc_jacobi_pcg(ctypes.c_long(S["ndof"]), T.P(rp), T.P(ci), T.P(val), T.P(rhs), T.P(x),
+                                   ctypes.c_double(1e-10), ctypes.c_long(1000), ctypes.byref(res))
+    assert it > 0 and res.value <= 1e-10
+    K = np.zeros((S["ndof"], S["ndof"]))
+    for i in range(S["ndof"]):
+        K[i, ci[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    assert np.allclose(x, np.linalg.solve(K, rhs), rtol=0, atol=1e-9 * np.abs(x).max())
+
+
+def test_planted_mistake_is_caught():
+    """A GENERAL tensor passed transposed (the DFunc wrapper's transpose,
+    diff_tensor.h:77-83, dropped) must fail the reference's known answer."""
+    K = _known()["grad_p3_x_grad_vecp1_general"]
+    xyz = np.array([sum(K["tet"], [])], dtype=float)
+    X = T.physical_points(xyz, K["order"])[0]
+    q = len(X)
+    # transpose planted by filling the 3x9 layout into the 9x3 slots
+    D = np.zeros((1, q, 27))
+    for n in range(q):
+        M = np.zeros((9, 3))
+        for i in range(9):
+            for j in range(3):
+                M[i, j] = (i * X[n, 0] + j * X[n, 1] + (i % 2) * X[n, 2]) * (3 * i + j)
+        D[0, n] = M.ravel(order="C")  # row-major where column-major is expected
+    A = T.orc_elem(GRAD, P3, 1, GRAD, P1, 3, GEN, PERPT, K["order"], xyz, D.reshape(1, -1))
+    Aexp = np.array(K["A_times_scale"], dtype=float) / K["scale"]
+    assert np.linalg.norm(A.reshape(20, 12).T - Aexp) > 1e-3

@@ -1,0 +1,126 @@
+"""Host-side assembly setup of the product (enumeration, CSR pattern, slot map,
+colouring) against the oracle, bit for bit.  Runs on the CPU through
+anifem_assembly_create_host (no device use)."""
+import numpy as np
+import pytest
+
+import anifem_amd as af
+import anifem_testlib as T
+
+
+def _shuffled_mesh(n, seed):
+    """Kuhn cube with relabelled nodes, shuffled tets and shuffled local vertex
+    order: exercises the node reorder (ordering.inl:8-26) and the P3 edge
+    orientation by global node id (ordering.inl:137-151)."""
+    coords, tets = af.kuhn_cube(n, 0.2, seed)
+    rng = np.random.default_rng(seed)
+    relabel = rng.permutation(len(coords))
+    inv = np.empty_like(relabel)
+    inv[relabel] = np.arange(len(coords))
+    coords2 = coords[inv]
+    tets2 = relabel[tets]
+    tets2 = tets2[rng.permutation(len(tets2))]
+    for r in range(len(tets2)):
+        tets2[r] = tets2[r][rng.permutation(4)]
+    return coords2, tets2.astype(np.int32)
+
+
+def _boundary_mask(A, coords):
+    s = A.setup()
+    return s["dirichlet"]
+
+
+@pytest.mark.parametrize("fem", [2, 3, 4])
+@pytest.mark.parametrize("ncomp", [1, 3])
+@pytest.mark.parametrize("dirichlet", [0, 1])
+@pytest.mark.parametrize("mesh", ["kuhn", "shuffled"])
+def test_setup_matches_oracle(fem, ncomp, dirichlet, mesh):
+    if mesh == "kuhn":
+        coords, tets = af.kuhn_cube(3, 0.2, 5)
+    else:
+        coords, tets = _shuffled_mesh(3, 7)
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), dirichlet=dirichlet, host_only=True)
+    s = A.setup()
+    mask = s["dirichlet"] if dirichlet else None
+    O = T.oracle_assembly(coords, tets, fem, ncomp, dirichlet_mask=mask)
+    assert A.ndof == O["ndof"]
+    np.testing.assert_array_equal(s["dofs"], O["dofs"])
+    np.testing.assert_array_equal(s["row_ptr"], O["row_ptr"])
+    np.testing.assert_array_equal(s["col_idx"], O["col_idx"])
+    np.testing.assert_array_equal(s["slot"], O["slot"])
+
+
+def test_boundary_dirichlet_dofs_are_on_boundary():
+    coords, tets = af.kuhn_cube(4, 0.2, 2)
+    for fem in [2, 3, 4]:
+        A = af.Assembler(coords, tets, af.Op(af.GRAD, fem), dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
+        s = A.setup()
+        onb = np.any((coords < 1e-12) | (coords > 1 - 1e-12), axis=1)
+        # node DOFs: flagged exactly on the boundary nodes
+        np.testing.assert_array_equal(s["dirichlet"][:len(coords)].astype(bool), onb)
+        if fem >= 3:
+            # an edge DOF is Dirichlet iff the edge midpoint lies on the boundary
+            tets_r = T.oracle_assembly(coords, tets, fem, 1, with_slots=False)["tets"]
+            for t in range(len(tets)):
+                P = coords[tets_r[t]]
+                for e, (a, b) in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]):
+                    mid = 0.5 * (P[a] + P[b])
+                    mb = bool(np.any((mid < 1e-12) | (mid > 1 - 1e-12)))
+                    d = s["dofs"][t, 4 + (e if fem == 3 else 2 * e)]
+                    assert bool(s["dirichlet"][d]) == mb
+
+
+def test_colouring_is_conflict_free():
+    coords, tets = _shuffled_mesh(5, 3)
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), host_only=True)
+    s = A.setup()
+    col = s["colour"]
+    assert col.min() == 0 and col.max() == A.n_colours - 1
+    for c in range(A.n_colours):
+        nodes = tets[col == c].ravel()
+        assert len(np.unique(nodes)) == len(nodes), f"colour {c} has tets sharing a node"
+
+
+@pytest.mark.parametrize("n,fem,ndof,nnz", [(16, 2, 4913, 66961), (16, 3, 35937, 977793), (16, 4, 117649, 5488081),
+                                            (32, 2, 35937, 513313)])
+def test_counts_match_mesh_tables(n, fem, ndof, nnz):
+    coords, tets = af.kuhn_cube(n, 0.2, 1)
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem), host_only=True)
+    assert (A.ndof, A.nnz) == (ndof, nnz)
+
+
+def test_dirichlet_pattern_drop_counts():
+    # C1: 46,941 stored entries after the Dirichlet row/column drop (T4)
+    coords, tets = af.kuhn_cube(16, 0.2, 1)
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P1), dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
+    assert A.nnz == 46941
+
+
+def test_connectivity_error():
+    coords, tets = af.kuhn_cube(2, 0.2, 1)
+    bad = tets.copy()
+    bad[3, 1] = bad[3, 0]  # repeated node: broken cell (assembler.inl:309-312 returns -2)
+    with pytest.raises(af.AnifemError) as e:
+        af.Assembler(coords, bad, af.Op(af.GRAD, af.FEM_P1), host_only=True)
+    assert e.value.code == af.ERR_CONNECTIVITY
+
+
+def test_mesh_generator_properties():
+    n = 8
+    coords, tets = af.kuhn_cube(n, 0.2, 9)
+    assert coords.shape == ((n + 1) ** 3, 3) and tets.shape == (6 * n ** 3, 4)
+    grid = np.stack(np.meshgrid(*[np.arange(n + 1)] * 3, indexing="ij"), -1).reshape(-1, 3) / n
+    d = coords - grid
+    assert np.abs(d).max() <= 0.2 / n + 1e-15
+    onb = np.any((grid == 0) | (grid == 1), axis=1)
+    assert np.abs(d[onb]).max() == 0.0
+    # Kuhn split with jitter 0.2h: every tet keeps positive volume after reorder
+    vol = 0.0
+    for t in tets:
+        P = coords[t]
+        v = abs(np.linalg.det(np.array([P[1] - P[0], P[2] - P[0], P[3] - P[0]]))) / 6
+        assert v > 0
+        vol += v
+    assert abs(vol - 1.0) < 1e-12
+    c2, _ = af.kuhn_cube(n, 0.2, 9)
+    np.testing.assert_array_equal(coords, c2)  # seeded reproducibility
