@@ -137,32 +137,57 @@ typedef struct anifem_status {
     int32_t code;            /* ANIFEM_OK or ANIFEM_ERR_NONFINITE */
     int32_t reserved;
     int64_t n_nonfinite;     /* local matrix / rhs entries that were NaN/Inf */
-    int64_t n_nonpositive;   /* tets with det <= 0 after the node reorder */
+    int64_t n_degenerate;    /* tets with a zero or non-finite Jacobian determinant (MES = |det|/6, core.inl:241) */
 } anifem_status;
 
 /* Builds on the host, then uploads: node reorder (ordering.inl:8-26), NATURAL
  * DOF enumeration with P3 edge orientation (global_enumerator.cpp:562-777,
  * ordering.inl:137-151, tetdofmap.inl:98-127), CSR pattern with the diagonal
  * and Dirichlet drop rules (assembler.inl:114-136,416), the element-to-CSR-slot
- * map and a conflict-free tet colouring.  `stream` orders the uploads. */
+ * map and the row blocks of the owner-computes assembly.  `stream` orders the
+ * uploads. */
 int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
                            void* stream);
 int anifem_assembly_destroy(anifem_assembly* ctx);
 /* Host-only variant: builds the same setup without touching the device
- * (inspection of the enumeration, pattern, slot map and colouring through
- * the info / copy_setup calls; run and coords return ANIFEM_ERR_ARG). */
+ * (inspection of the enumeration, pattern, slot map and row blocks through
+ * the info / copy_setup / copy_blocks calls; run and coords return
+ * ANIFEM_ERR_ARG). */
 int anifem_assembly_create_host(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob);
 
-/* Sizes: number of DOFs, stored nonzeros, colours, local DOFs per tet. */
-int anifem_assembly_info(const anifem_assembly* ctx, int64_t* ndof, int64_t* nnz, int32_t* n_colours,
+/* Sizes: number of DOFs, stored nonzeros, row blocks, local DOFs per tet. */
+int anifem_assembly_info(const anifem_assembly* ctx, int64_t* ndof, int64_t* nnz, int64_t* n_blocks,
                          int32_t* nf);
 /* Device views of the CSR pattern (row_ptr: ndof+1 int64, col_idx: nnz int32). */
 int anifem_assembly_csr(const anifem_assembly* ctx, const int64_t** row_ptr, const int32_t** col_idx);
 /* Host copies for inspection: pattern, per-tet global DOFs (n_tets x nf, the
  * reference's local order), slot map (n_tets x nf*nf, entry ib + nf*ia, -1 =
- * dropped), tet colours, Dirichlet DOF flags.  Any pointer may be NULL. */
+ * dropped), Dirichlet DOF flags.  Any pointer may be NULL. */
 int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int32_t* col_idx, int32_t* dofs,
-                               int64_t* slot, int32_t* colour, uint8_t* dirichlet);
+                               int64_t* slot, uint8_t* dirichlet);
+/* Row blocks of the owner-computes assembly.  Block b owns the CSR rows
+ * [blk_row0[b], blk_row0[b+1]) and forms every slot value and rhs entry of
+ * them exactly once.  It lists its distinct tets (ascending) in
+ * blk_tets[blk_tet0[b] ..]; for each slot s of its rows the codes
+ * codes[blk_c0[b] + sstart[s] .. blk_c0[b] + sstart[s+1]) (the last slot of a
+ * block ends at blk_c0[b+1]) name the contributions local_tet*E + entry
+ * (entry ib + nf*ia, E = nf*nf) in ascending tet order; likewise fcodes /
+ * rstart / blk_g0 for the rhs with local_tet*nf + i.  No reference
+ * counterpart: this replaces the row locks of the threaded reference
+ * assembler (assembler.inl:339-340,406,474).  Sizes first (layout), then the
+ * host copies (host-only contexts keep them; any pointer may be NULL). */
+typedef struct anifem_block_layout {
+    int64_t n_blocks;
+    int64_t n_block_tets;  /* length of blk_tets */
+    int64_t n_codes;       /* length of codes */
+    int64_t n_fcodes;      /* length of fcodes */
+    int32_t code_bytes;    /* device code width: 2 or 4 */
+    int32_t block_tet_cap; /* tets per block at most */
+} anifem_block_layout;
+int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out);
+int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_tet0, int32_t* blk_tets,
+                                int64_t* blk_c0, uint32_t* codes, uint16_t* sstart, int64_t* blk_g0,
+                                uint16_t* fcodes, uint16_t* rstart);
 /* Device node coordinates (n_nodes x 3) used by run; the caller may update
  * them in place (moving mesh) between runs. */
 int anifem_assembly_coords(anifem_assembly* ctx, double** coords);
@@ -170,8 +195,10 @@ int anifem_assembly_coords(anifem_assembly* ctx, double** coords);
 /* Assemble: csr_val (nnz, device FP64) and rhs (ndof, device FP64) are
  * overwritten with  sum_T P^T A_T P  and  sum_T P^T F_T  (assembler.inl:397-481)
  * after element-level Dirichlet (dc_on_dof.h:38-45) on the flagged DOFs.
- * Tets are processed colour by colour; within a colour no two tets share a
- * DOF, so there are no atomics and repeated runs are bitwise identical.
+ * One workgroup per row block forms each slot value (and rhs entry) as the
+ * sum of its element contributions in ascending tet order, the order of the
+ * reference's element loop; there are no atomics on values and repeated runs
+ * are bitwise identical.
  * D and F use the problem's tensor kinds; F may be NULL (no rhs; rhs may be
  * NULL too).  dir_values (device, ndof, may be NULL = homogeneous) gives the
  * Dirichlet values.  If status is non-NULL the call synchronizes the stream
@@ -185,11 +212,10 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
  * friends).  When enabled, run records HIP events on its stream around each
  * phase and synchronizes at its end.  Times in milliseconds; launch counts. */
 typedef struct anifem_timers {
-    double ms_fill;     /* zeroing of csr_val / rhs */
-    double ms_matrix;   /* element matrices + scatter, all colours */
-    double ms_rhs;      /* load vectors + scatter, all colours */
-    int32_t n_matrix_launches;
-    int32_t n_rhs_launches;
+    double ms_geometry; /* per-tet Jacobians and geometry tensors (matrix and rhs) */
+    double ms_gather;   /* row blocks: element entries formed and summed into csr_val / rhs */
+    int32_t n_launches; /* kernel launches of the run */
+    int32_t reserved;
 } anifem_timers;
 int anifem_assembly_enable_timers(anifem_assembly* ctx, int on);
 int anifem_assembly_timers(const anifem_assembly* ctx, anifem_timers* out);

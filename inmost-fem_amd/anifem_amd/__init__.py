@@ -67,12 +67,17 @@ class anifem_problem_desc(ctypes.Structure):
 
 class anifem_status(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("n_nonfinite", ctypes.c_int64), ("n_nonpositive", ctypes.c_int64)]
+                ("n_nonfinite", ctypes.c_int64), ("n_degenerate", ctypes.c_int64)]
 
 
 class anifem_timers(ctypes.Structure):
-    _fields_ = [("ms_fill", ctypes.c_double), ("ms_matrix", ctypes.c_double), ("ms_rhs", ctypes.c_double),
-                ("n_matrix_launches", ctypes.c_int32), ("n_rhs_launches", ctypes.c_int32)]
+    _fields_ = [("ms_geometry", ctypes.c_double), ("ms_gather", ctypes.c_double),
+                ("n_launches", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class anifem_block_layout(ctypes.Structure):
+    _fields_ = [("n_blocks", ctypes.c_int64), ("n_block_tets", ctypes.c_int64), ("n_codes", ctypes.c_int64),
+                ("n_fcodes", ctypes.c_int64), ("code_bytes", ctypes.c_int32), ("block_tet_cap", ctypes.c_int32)]
 
 
 class anifem_pcg_result(ctypes.Structure):
@@ -95,7 +100,9 @@ SIGNATURES = {
     "anifem_assembly_destroy": (_I32, [_VP]),
     "anifem_assembly_info": (_I32, [_VP, _VP, _VP, _VP, _VP]),
     "anifem_assembly_csr": (_I32, [_VP, _VP, _VP]),
-    "anifem_assembly_copy_setup": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_copy_setup": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_blocks": (_I32, [_VP, ctypes.POINTER(anifem_block_layout)]),
+    "anifem_assembly_copy_blocks": (_I32, [_VP] + [_VP] * 9),
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
     "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
@@ -247,9 +254,9 @@ class Assembler:
         _check(rc, "anifem_assembly_create")
         self._h = h
         self.host_only = host_only
-        ndof, nnz, ncol, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
-        L.anifem_assembly_info(h, ctypes.byref(ndof), ctypes.byref(nnz), ctypes.byref(ncol), ctypes.byref(nf))
-        self.ndof, self.nnz, self.n_colours, self.nf = ndof.value, nnz.value, ncol.value, nf.value
+        ndof, nnz, nblk, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        L.anifem_assembly_info(h, ctypes.byref(ndof), ctypes.byref(nnz), ctypes.byref(nblk), ctypes.byref(nf))
+        self.ndof, self.nnz, self.n_blocks, self.nf = ndof.value, nnz.value, nblk.value, nf.value
         self.n_tets = len(self._tets)
 
     def __del__(self):
@@ -259,16 +266,35 @@ class Assembler:
             self._h = None
 
     def setup(self):
-        """Host copies: dict of row_ptr, col_idx, dofs, slot, colour, dirichlet."""
+        """Host copies: dict of row_ptr, col_idx, dofs, slot, dirichlet."""
         rp = np.zeros(self.ndof + 1, np.int64)
         ci = np.zeros(self.nnz, np.int32)
         dofs = np.zeros((self.n_tets, self.nf), np.int32)
         slot = np.zeros((self.n_tets, self.nf * self.nf), np.int64)
-        col = np.zeros(self.n_tets, np.int32)
         dr = np.zeros(self.ndof, np.uint8)
-        _check(lib().anifem_assembly_copy_setup(self._h, _ptr(rp), _ptr(ci), _ptr(dofs), _ptr(slot), _ptr(col),
-                                                _ptr(dr)), "copy_setup")
-        return dict(row_ptr=rp, col_idx=ci, dofs=dofs, slot=slot, colour=col, dirichlet=dr)
+        _check(lib().anifem_assembly_copy_setup(self._h, _ptr(rp), _ptr(ci), _ptr(dofs), _ptr(slot), _ptr(dr)),
+               "copy_setup")
+        return dict(row_ptr=rp, col_idx=ci, dofs=dofs, slot=slot, dirichlet=dr)
+
+    def block_layout(self):
+        lay = anifem_block_layout()
+        _check(lib().anifem_assembly_blocks(self._h, ctypes.byref(lay)), "blocks")
+        return lay
+
+    def blocks(self):
+        """Host copies of the row blocks (host-only contexts): dict of blk_row0,
+        blk_tet0, blk_tets, blk_c0, codes, sstart, blk_g0, fcodes, rstart."""
+        lay = self.block_layout()
+        nb = lay.n_blocks
+        out = dict(blk_row0=np.zeros(nb + 1, np.int32), blk_tet0=np.zeros(nb + 1, np.int64),
+                   blk_tets=np.zeros(lay.n_block_tets, np.int32), blk_c0=np.zeros(nb + 1, np.int64),
+                   codes=np.zeros(lay.n_codes, np.uint32), sstart=np.zeros(self.nnz, np.uint16),
+                   blk_g0=np.zeros(nb + 1, np.int64), fcodes=np.zeros(lay.n_fcodes, np.uint16),
+                   rstart=np.zeros(self.ndof, np.uint16))
+        k = ["blk_row0", "blk_tet0", "blk_tets", "blk_c0", "codes", "sstart", "blk_g0", "fcodes", "rstart"]
+        _check(lib().anifem_assembly_copy_blocks(self._h, *[_ptr(out[x]) for x in k]), "copy_blocks")
+        out["code_bytes"], out["block_tet_cap"] = lay.code_bytes, lay.block_tet_cap
+        return out
 
     def csr(self):
         """Device (row_ptr, col_idx) addresses."""

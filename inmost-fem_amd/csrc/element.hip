@@ -47,42 +47,37 @@ static int choose_tile(int kstride, int elem_bytes, int cap) {
     return t;
 }
 
-template <int DA0, int DB0, int NCA, int NCB, typename T, bool MESH, int MODE>
+template <int DA0, int DB0, int NCA, int NCB, typename T>
 static int launch_one(ElemArgs a, int cap, hipStream_t s) {
     const int kstride = NCA * NCB * a.q * DA0 * DB0;
     a.tile = choose_tile(kstride, (int)sizeof(T), cap);
     const size_t lds = (size_t)a.tile * kstride * sizeof(T);
+    if (lds > 64 * 1024) return ANIFEM_ERR_ARG;
     const int64_t blocks = (a.n + a.tile - 1) / a.tile;
     if (blocks == 0) return ANIFEM_OK;
-    hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T, MESH, MODE>), dim3((unsigned)blocks), dim3(kBlock),
-                       lds, s, a);
+    hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), lds, s, a);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
 
-template <typename T, bool MESH, int MODE, int DA0, int DB0>
+template <typename T, int DA0, int DB0>
 static int launch_nc(int ncA, int ncB, const ElemArgs& a, int cap, hipStream_t s) {
-    if (ncA == 1 && ncB == 1) return launch_one<DA0, DB0, 1, 1, T, MESH, MODE>(a, cap, s);
-    if (ncA == 3 && ncB == 3) return launch_one<DA0, DB0, 3, 3, T, MESH, MODE>(a, cap, s);
-    if (ncA == 1 && ncB == 3) return launch_one<DA0, DB0, 1, 3, T, MESH, MODE>(a, cap, s);
-    if (ncA == 3 && ncB == 1) return launch_one<DA0, DB0, 3, 1, T, MESH, MODE>(a, cap, s);
+    if (ncA == 1 && ncB == 1) return launch_one<DA0, DB0, 1, 1, T>(a, cap, s);
+    if (ncA == 3 && ncB == 3) return launch_one<DA0, DB0, 3, 3, T>(a, cap, s);
+    if (ncA == 1 && ncB == 3) return launch_one<DA0, DB0, 1, 3, T>(a, cap, s);
+    if (ncA == 3 && ncB == 1) return launch_one<DA0, DB0, 3, 1, T>(a, cap, s);
     return ANIFEM_ERR_ARG;
 }
 
-template <typename T, bool MESH, int MODE>
+template <typename T>
 int launch_element(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a, int cap, hipStream_t s) {
     const int dA = ref_dim(opA.op), dB = ref_dim(opB.op);
-    if (dA == 3 && dB == 3) return launch_nc<T, MESH, MODE, 3, 3>(opA.ncomp, opB.ncomp, a, cap, s);
-    if (dA == 1 && dB == 1) return launch_nc<T, MESH, MODE, 1, 1>(opA.ncomp, opB.ncomp, a, cap, s);
-    if (dA == 3 && dB == 1) return launch_nc<T, MESH, MODE, 3, 1>(opA.ncomp, opB.ncomp, a, cap, s);
-    if (dA == 1 && dB == 3) return launch_nc<T, MESH, MODE, 1, 3>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 3 && dB == 3) return launch_nc<T, 3, 3>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 1 && dB == 1) return launch_nc<T, 1, 1>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 3 && dB == 1) return launch_nc<T, 3, 1>(opA.ncomp, opB.ncomp, a, cap, s);
+    if (dA == 1 && dB == 3) return launch_nc<T, 1, 3>(opA.ncomp, opB.ncomp, a, cap, s);
     return ANIFEM_ERR_ARG;
 }
-
-template int launch_element<double, true, 1>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
-template int launch_element<float, true, 1>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
-template int launch_element<double, true, 2>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
-template int launch_element<float, true, 2>(const anifem_op&, const anifem_op&, const ElemArgs&, int, hipStream_t);
 
 }  // namespace anifem
 
@@ -99,7 +94,7 @@ extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_
     a.xyz = xyz;
     a.out = A_out;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (precision == ANIFEM_PREC_FP64) return launch_element<double, false, 0>(opA, opB, a, 256, s);
-    if (precision == ANIFEM_PREC_FP32) return launch_element<float, false, 0>(opA, opB, a, 256, s);
+    if (precision == ANIFEM_PREC_FP64) return launch_element<double>(opA, opB, a, 256, s);
+    if (precision == ANIFEM_PREC_FP32) return launch_element<float>(opA, opB, a, 256, s);
     return ANIFEM_ERR_ARG;
 }

@@ -48,11 +48,6 @@ struct ElemArgs {
     const float* Rf;
     const double* wq;  // quadrature weights (per-point K)
     void* out;         // batch mode: element matrices
-    // fused assembly mode
-    const int32_t* slot;       // E per tet, -1 = dropped entry (MODE 2: nf global DOFs per tet)
-    const uint64_t* dirmask;   // per tet, bit i = local DOF i is Dirichlet (or null)
-    double* val;
-    int* status;  // [0] non-finite entries, [1] tets with det <= 0
 };
 
 __device__ __forceinline__ void tet_geometry(const double P[4][3], double PSI[9], double& det) {
@@ -151,12 +146,9 @@ __device__ __forceinline__ void load_tet(const ElemArgs& a, int64_t r, double P[
     }
 }
 
-// MODE 0: store element matrices (T precision) to a.out.
-// MODE 1: fused scatter into a.val through a.slot (colour launch: no two
-//         tets of the launch share a DOF, so plain read-modify-write is safe).
-// MODE 2: load vector (trial = IDEN P0, so e = ib): scatter into a.val at
-//         the tet's global DOFs a.slot[pos*nfB + ib]; Dirichlet entries get 0.
-template <int DA0, int DB0, int NCA, int NCB, typename T, bool MESH, int MODE>
+// Element matrices of a tile of tets, stored in T precision to a.out
+// (n_tets matrices of nfB x nfA, column-major, one after the other).
+template <int DA0, int DB0, int NCA, int NCB, typename T>
 __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     constexpr int KB = DA0 * DB0;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -168,12 +160,10 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     for (int t = threadIdx.x; t < ntile; t += kBlock) {
         const int64_t r = tile0 + t;
         double P[4][3], PSI[9], det;
-        load_tet<MESH>(a, r, P);
+        load_tet<false>(a, r, P);
         tet_geometry(P, PSI, det);
         const double mes = fabs(det) / 6.0;
-        if (MODE == 1 && !(det > 0.0)) atomicAdd(&a.status[1], 1);
-        const int64_t cr = a.orig ? (int64_t)a.orig[a.pos0 + r] : r;
-        geometry_tensor<DA0, DB0, NCA, NCB, T>(a, cr, PSI, mes, Ks + (int64_t)t * kstride);
+        geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride);
     }
     __syncthreads();
 
@@ -187,7 +177,6 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     if constexpr (sizeof(T) == 8) R = reinterpret_cast<const T*>(a.Rd);
     else R = reinterpret_cast<const T*>(a.Rf);
 
-    int nonfinite = 0;
     for (int e = es; e < E; e += G) {
         const int ib = e % nfB, ia = e / nfB;
         const int ib0 = ib % a.nfB0, cb = ib / a.nfB0;
@@ -207,32 +196,9 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
                 const T* Rn = R + (int64_t)rr * a.q * KB;
                 for (int j = 0; j < a.q * KB; ++j) acc += Kt[j] * Rn[j];
             }
-            const int64_t r = tile0 + t;
-            if (MODE == 0) {
-                static_cast<T*>(a.out)[r * E + e] = acc;
-            } else if (MODE == 1) {
-                const int64_t pos = a.pos0 + r;
-                double v = (double)acc;
-                if (a.dirmask) {
-                    // applyDir with homogeneous value (dc_on_dof.h:38-45): zero row
-                    // and column, unit diagonal
-                    const uint64_t m = a.dirmask[pos];
-                    const bool di = (m >> ib) & 1, dj = (m >> ia) & 1;
-                    if (di || dj) v = (ib == ia) ? 1.0 : 0.0;
-                }
-                if (!isfinite(v)) ++nonfinite;
-                const int32_t s = a.slot[pos * E + e];
-                if (s >= 0) a.val[s] += v;
-            } else {
-                const int64_t pos = a.pos0 + r;
-                double v = (double)acc;
-                if (a.dirmask && ((a.dirmask[pos] >> ib) & 1)) v = 0.0;
-                if (!isfinite(v)) ++nonfinite;
-                a.val[a.slot[pos * E + e]] += v;
-            }
+            static_cast<T*>(a.out)[(tile0 + t) * E + e] = acc;
         }
     }
-    if (MODE != 0 && nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
 }  // namespace anifem

@@ -28,6 +28,8 @@
 namespace anifem {
 
 int nf_scalar(int fem);
+int ref_dim(int op);
+int quadrature(int order, const double** p, const double** w);
 
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn) {
     unsigned hw = std::thread::hardware_concurrency();
@@ -152,7 +154,7 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
         nonpos += lnp;
     });
     if (bad.load()) return ANIFEM_ERR_CONNECTIVITY;
-    S.n_nonpositive = nonpos.load();
+    S.n_degenerate = nonpos.load();
 
     // edges and faces by first appearance
     std::vector<int64_t> eid, fid;
@@ -294,51 +296,12 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
         }
     });
 
-    // greedy colouring in the caller's tet order: tets sharing a node conflict
-    {
-        std::vector<int64_t> nptr(NN + 1, 0);
-        for (int64_t i = 0; i < 4 * NT; ++i) nptr[S.tets[i] + 1]++;
-        for (int64_t i = 0; i < NN; ++i) nptr[i + 1] += nptr[i];
-        std::vector<int32_t> nadj(4 * NT);
-        std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
-        for (int64_t t = 0; t < NT; ++t)
-            for (int k = 0; k < 4; ++k) nadj[fill[S.tets[4 * t + k]]++] = (int32_t)t;
-        S.colour.assign(NT, -1);
-        int ncol = 0;
-        std::vector<uint64_t> used(32);
-        for (int64_t t = 0; t < NT; ++t) {
-            std::fill(used.begin(), used.end(), 0);
-            for (int k = 0; k < 4; ++k) {
-                int32_t v = S.tets[4 * t + k];
-                for (int64_t p = nptr[v]; p < nptr[v + 1]; ++p) {
-                    int32_t c = S.colour[nadj[p]];
-                    if (c >= 0) used[c >> 6] |= 1ull << (c & 63);
-                }
-            }
-            int c = 0;
-            while (used[c >> 6] >> (c & 63) & 1ull) ++c;
-            if (c >= 2048) return ANIFEM_ERR_ARG;
-            S.colour[t] = c;
-            ncol = std::max(ncol, c + 1);
-        }
-        S.n_colours = ncol;
-    }
-    // assembly order: colour-major, caller order within a colour
-    S.colour_ptr.assign(S.n_colours + 1, 0);
-    for (int64_t t = 0; t < NT; ++t) S.colour_ptr[S.colour[t] + 1]++;
-    for (int c = 0; c < S.n_colours; ++c) S.colour_ptr[c + 1] += S.colour_ptr[c];
-    S.perm.resize(NT);
-    {
-        std::vector<int64_t> fill(S.colour_ptr.begin(), S.colour_ptr.end() - 1);
-        for (int64_t t = 0; t < NT; ++t) S.perm[fill[S.colour[t]]++] = (int32_t)t;
-    }
-
-    // slot map in assembly order
+    // slot map in the caller's tet order: slot[t*E + ib + nf*ia]
     S.slot.resize((size_t)S.E * NT);
     host_parallel_for(NT, 1 << 12, [&](int64_t b, int64_t e) {
-        for (int64_t pos = b; pos < e; ++pos) {
-            const int32_t* td = &S.dofs[(size_t)nf * S.perm[pos]];
-            int32_t* sl = &S.slot[(size_t)S.E * pos];
+        for (int64_t t = b; t < e; ++t) {
+            const int32_t* td = &S.dofs[(size_t)nf * t];
+            int32_t* sl = &S.slot[(size_t)S.E * t];
             for (int ia = 0; ia < nf; ++ia)
                 for (int ib = 0; ib < nf; ++ib) {
                     const int32_t r = td[ib], c = td[ia];
@@ -349,7 +312,16 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
                 }
         }
     });
-    return ANIFEM_OK;
+    // geometry-tensor values per tet: dA*dB*ncomp^2 per point carried (all
+    // points when D is given per point) plus the load vector's
+    {
+        const double *pp, *ww;
+        const int qm = prob.tensor_storage == ANIFEM_COEF_PER_POINT ? quadrature(prob.order, &pp, &ww) : 1;
+        const int qf = prob.rhs_tensor_storage == ANIFEM_COEF_PER_POINT ? quadrature(prob.rhs_order, &pp, &ww) : 1;
+        if (qm < 0 || qf < 0) return ANIFEM_ERR_ORDER;
+        S.k_per_tet = ref_dim(prob.trial.op) * ref_dim(prob.test.op) * S.ncomp * S.ncomp * qm + S.ncomp * qf;
+    }
+    return build_blocks(S, aptr, adj);
 }
 
 }  // namespace anifem

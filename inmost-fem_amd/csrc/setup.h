@@ -1,8 +1,7 @@
 // Host-side assembly setup: the counterpart of AssemblerT::PrepareProblem
 // (inmost_interface/assembler.inl:194-275) for a GPU assembler.  Produces the
-// global DOF numbering, the CSR pattern, the element-to-CSR-slot map and a
-// conflict-free colouring, all in "assembly order" (colour-major, caller's tet
-// order inside a colour).
+// global DOF numbering, the CSR pattern, the element-to-CSR-slot map and the
+// row blocks of the owner-computes assembly (blocks.cpp).
 #pragma once
 
 #include <cstdint>
@@ -13,6 +12,14 @@
 
 namespace anifem {
 
+// Row-block budgets of the owner-computes assembly (blocks.cpp).
+constexpr int kBlockMaxSlots = 4096;         // CSR slots per block
+constexpr int kBlockMaxContrib = 8192;       // (tet, entry) contributions per block
+constexpr int kBlockMaxTets = 256;           // default tets per block
+constexpr int kBlockHardMaxTets = 1024;      // largest DOF valence supported
+constexpr int kBlockKBytes = 24 * 1024;      // default FP64 geometry-tensor bytes per block
+constexpr int kBlockMaxKBytes = 96 * 1024;   // ... when raised to the DOF valence
+
 struct AssemblySetup {
     int fem = 0, ncomp = 1, nf = 0, E = 0;
     int64_t n_nodes = 0, n_tets = 0, n_edges = 0, n_faces = 0;
@@ -22,15 +29,27 @@ struct AssemblySetup {
     std::vector<uint8_t> is_dir;      // ndof
     std::vector<int64_t> row_ptr;     // ndof + 1
     std::vector<int32_t> col_idx;     // nnz
-    std::vector<int32_t> colour;      // n_tets
-    int n_colours = 0;
-    std::vector<int64_t> colour_ptr;  // n_colours + 1 (assembly order ranges)
-    std::vector<int32_t> perm;        // assembly position -> caller's tet id
-    std::vector<int32_t> slot;        // assembly order: n_tets x E (entry ib + nf*ia), -1 dropped
-    int64_t n_nonpositive = 0;        // tets with det <= 0 after reorder
+    std::vector<int32_t> slot;        // caller's order: n_tets x E (entry ib + nf*ia), -1 dropped
+    int64_t n_degenerate = 0;         // tets with a zero or non-finite Jacobian determinant
+    int k_per_tet = 1;                // FP64 geometry-tensor values per tet (matrix + load vector)
+    // row blocks: consecutive CSR rows [blk_row0[b], blk_row0[b+1])
+    int block_tet_cap = kBlockMaxTets;
+    bool code16 = true;               // codes fit 16 bits on the device
+    int64_t n_blocks = 0;
+    std::vector<int32_t> blk_row0;    // n_blocks + 1
+    std::vector<int64_t> blk_tet0;    // n_blocks + 1 offsets into blk_tets
+    std::vector<int32_t> blk_tets;    // distinct tets of each block, ascending
+    std::vector<int64_t> blk_c0;      // n_blocks + 1 offsets into codes
+    std::vector<uint32_t> codes;      // per block: local_tet*E + entry, grouped by slot, ascending tet
+    std::vector<uint16_t> sstart;     // nnz: first code of each slot, relative to its block
+    std::vector<int64_t> blk_g0;      // n_blocks + 1 offsets into fcodes
+    std::vector<uint16_t> fcodes;     // per block: local_tet*nf + i, grouped by row, ascending tet
+    std::vector<uint16_t> rstart;     // ndof: first load-vector code of each row, relative to its block
+    int64_t max_block_tets = 0, max_block_contrib = 0, max_block_fcontrib = 0;
 };
 
 int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S);
+int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj);
 
 // Parallel loop over [0, n) on the host.
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn);

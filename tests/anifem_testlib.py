@@ -82,6 +82,28 @@ def physical_points(xyz, order):
     return X
 
 
+def product_physical_points(xyz, order):
+    """Physical points (n, q, 3) of the product's rule, in the point order of
+    anifem_quadrature (the order its per-point coefficient arrays follow)."""
+    import anifem_amd as af
+    p, _ = af.quadrature(order)
+    P0 = xyz[:, 0:3]
+    X = np.repeat(P0[:, None, :], len(p), axis=1).copy()
+    for l in range(3):
+        X += p[None, :, l + 1, None] * (xyz[:, None, 3 * (l + 1):3 * (l + 2)] - P0[:, None, :])
+    return X
+
+
+def point_permutation(order):
+    """perm[j] = index of the oracle's point equal to the product's point j."""
+    import anifem_amd as af
+    pp, _ = af.quadrature(order)
+    po, _ = orc_quadrature(order)
+    perm = np.array([int(np.argmin(np.abs(po - pp[j]).max(axis=1))) for j in range(len(pp))])
+    assert np.abs(po[perm] - pp).max() < 1e-15 and len(set(perm.tolist())) == len(pp)
+    return perm
+
+
 def xdep_field(X):
     """m(x) = 1 + x0/2 - x1/4 + x2^2/8 (the same field as oracle/ref_harness.cpp)."""
     return 1.0 + 0.5 * X[..., 0] - 0.25 * X[..., 1] + 0.125 * X[..., 2] ** 2

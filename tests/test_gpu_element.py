@@ -59,7 +59,7 @@ def test_element_vs_reference_fixture(case):
     opA, opB = af.Op(oa, fa, ca), af.Op(ob, fb, cb)
     dA, dB = opA.dim, opB.dim
     if pp == 2:
-        X = T.physical_points(xyz, order)
+        X = T.product_physical_points(xyz, order)  # the product's point order
         q = X.shape[1]
         Dpp = T.xdep_field(X)[:, :, None] * D[:, None, :]
         storage, Dprod = af.COEF_PER_POINT, product_coef(tt, af.COEF_PER_POINT, Dpp, n, dA, dB, q)
@@ -108,8 +108,10 @@ def test_element_vs_oracle_all_storages(pair, tt):
                 if tt == af.TENSOR_SYMMETRIC:
                     M = Do.reshape(-1, dB, dB)
                     Do = (M + M.transpose(0, 2, 1)).reshape(shape)
-                Dprod = product_coef(tt, storage, Do.reshape(shape[0], -1) if storage != af.COEF_PER_POINT else Do,
-                                     shape[0] if storage != af.COEF_PER_POINT else n, dA, dB, q)
+                if storage == af.COEF_PER_POINT:  # value of product point j = value of the matching oracle point
+                    Dprod = product_coef(tt, storage, Do[:, T.point_permutation(order), :], n, dA, dB, q)
+                else:
+                    Dprod = product_coef(tt, storage, Do.reshape(shape[0], -1), shape[0], dA, dB, q)
             A_or = T.orc_elem(oa, fa, ca, ob, fb, cb, tt, storage, order, xyz,
                               None if Do is None else Do.reshape(1 if storage == af.COEF_CONST else n, -1))
             A = run_product(opA, opB, tt, storage, Dprod, order, xyz)

@@ -1,5 +1,5 @@
 """Host-side assembly setup of the product (enumeration, CSR pattern, slot map,
-colouring) against the oracle, bit for bit.  Runs on the CPU through
+row blocks) against the oracle, bit for bit.  Runs on the CPU through
 anifem_assembly_create_host (no device use)."""
 import numpy as np
 import pytest
@@ -70,17 +70,6 @@ def test_boundary_dirichlet_dofs_are_on_boundary():
                     assert bool(s["dirichlet"][d]) == mb
 
 
-def test_colouring_is_conflict_free():
-    coords, tets = _shuffled_mesh(5, 3)
-    A = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), host_only=True)
-    s = A.setup()
-    col = s["colour"]
-    assert col.min() == 0 and col.max() == A.n_colours - 1
-    for c in range(A.n_colours):
-        nodes = tets[col == c].ravel()
-        assert len(np.unique(nodes)) == len(nodes), f"colour {c} has tets sharing a node"
-
-
 @pytest.mark.parametrize("n,fem,ndof,nnz", [(16, 2, 4913, 66961), (16, 3, 35937, 977793), (16, 4, 117649, 5488081),
                                             (32, 2, 35937, 513313)])
 def test_counts_match_mesh_tables(n, fem, ndof, nnz):
@@ -124,3 +113,69 @@ def test_mesh_generator_properties():
     assert abs(vol - 1.0) < 1e-12
     c2, _ = af.kuhn_cube(n, 0.2, 9)
     np.testing.assert_array_equal(coords, c2)  # seeded reproducibility
+
+
+def _emulate_blocks(A, s, B, F=None):
+    """The row-block gather replayed on the host from the block lists: each
+    slot value is the sum of its listed element entries in list order."""
+    nf = A.shape[1] if F is None else F.shape[1]
+    E = A.shape[1]
+    rp = s["row_ptr"]
+    ndof, nnz = len(rp) - 1, rp[-1]
+    val, rhs = np.zeros(nnz), np.zeros(ndof)
+    nb = len(B["blk_row0"]) - 1
+    for b in range(nb):
+        r0, r1 = B["blk_row0"][b], B["blk_row0"][b + 1]
+        tets = B["blk_tets"][B["blk_tet0"][b]:B["blk_tet0"][b + 1]]
+        c0, c1 = B["blk_c0"][b], B["blk_c0"][b + 1]
+        s0, s1 = rp[r0], rp[r1]
+        for u in range(s0, s1):
+            lo = c0 + B["sstart"][u]
+            hi = c0 + B["sstart"][u + 1] if u + 1 < s1 else c1
+            acc = 0.0
+            last = -1
+            for code in B["codes"][lo:hi]:
+                t = tets[code // E]
+                assert t > last, "contributions of a slot in ascending tet order"
+                last = t
+                acc += A[t, code % E]
+            val[u] = acc
+        if F is not None:
+            g0, g1 = B["blk_g0"][b], B["blk_g0"][b + 1]
+            for r in range(r0, r1):
+                lo = g0 + B["rstart"][r]
+                hi = g0 + B["rstart"][r + 1] if r + 1 < r1 else g1
+                acc = 0.0
+                for code in B["fcodes"][lo:hi]:
+                    acc += F[tets[code // F.shape[1]], code % F.shape[1]]
+                rhs[r] = acc
+    return val, rhs
+
+
+@pytest.mark.parametrize("fem,ncomp", [(2, 1), (3, 1), (4, 1), (3, 3)])
+def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
+    """The block lists name every (tet, entry) contribution of every slot once,
+    in ascending tet order: replaying them gives the oracle's scatter bit for
+    bit (random element matrices with element-level Dirichlet)."""
+    coords, tets = _shuffled_mesh(3, 11) if fem != 3 else af.kuhn_cube(4, 0.2, 2)
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
+    s = A.setup()
+    B = A.blocks()
+    nf, E = A.nf, A.nf * A.nf
+    rng = np.random.default_rng(fem * 10 + ncomp)
+    Ae = rng.standard_normal((len(tets), E))
+    Fe = rng.standard_normal((len(tets), nf))
+    Ae, Fe = T.apply_dir_elementwise(Ae, Fe, s["dofs"], s["dirichlet"])
+    rc, val_o, rhs_o = T.oracle_scatter(len(tets), nf, s["dofs"], s["slot"], Ae, Fe, A.nnz, A.ndof)
+    assert rc == 0
+    val, rhs = _emulate_blocks(Ae, s, B, Fe)
+    np.testing.assert_array_equal(val, val_o)
+    np.testing.assert_array_equal(rhs, rhs_o)
+    # budgets: rows contiguous and covering, slot / tet / code limits
+    r0 = B["blk_row0"]
+    assert r0[0] == 0 and r0[-1] == A.ndof and np.all(np.diff(r0) > 0)
+    assert np.all(np.diff(s["row_ptr"][r0]) <= 4096)
+    assert np.all(np.diff(B["blk_tet0"]) <= B["block_tet_cap"])
+    assert np.all(np.diff(B["blk_c0"]) <= 8192)
+    assert B["code_bytes"] == (2 if B["block_tet_cap"] * E <= 65536 else 4)
+    assert len(B["codes"]) == int((s["slot"] >= 0).sum())
