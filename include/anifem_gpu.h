@@ -170,12 +170,14 @@ int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int
  * them exactly once.  It lists its distinct tets (ascending) in
  * blk_tets[blk_tet0[b] ..]; for each slot s of its rows the codes
  * codes[blk_c0[b] + sstart[s] .. blk_c0[b] + sstart[s+1]) (the last slot of a
- * block ends at blk_c0[b+1]) name the contributions local_tet*E + entry
- * (entry ib + nf*ia, E = nf*nf) in ascending tet order; likewise fcodes /
- * rstart / blk_g0 for the rhs with local_tet*nf + i.  No reference
- * counterpart: this replaces the row locks of the threaded reference
- * assembler (assembler.inl:339-340,406,474).  Sizes first (layout), then the
- * host copies (host-only contexts keep them; any pointer may be NULL). */
+ * block ends at blk_c0[b+1]) name the contributions local_tet << e_bits |
+ * entry (entry ib + nf*ia; entry == nf*nf marks a Dirichlet diagonal entry,
+ * whose value applyDir sets to 1) in ascending tet order; likewise fcodes /
+ * rstart / blk_g0 for the rhs with local_tet << f_bits | i (Dirichlet rows
+ * have none).  No reference counterpart: this replaces the row locks of the
+ * threaded reference assembler (assembler.inl:339-340,406,474).  Sizes first
+ * (layout), then the host copies (host-only contexts keep them; any pointer
+ * may be NULL). */
 typedef struct anifem_block_layout {
     int64_t n_blocks;
     int64_t n_block_tets;  /* length of blk_tets */
@@ -183,6 +185,8 @@ typedef struct anifem_block_layout {
     int64_t n_fcodes;      /* length of fcodes */
     int32_t code_bytes;    /* device code width: 2 or 4 */
     int32_t block_tet_cap; /* tets per block at most */
+    int32_t e_bits;        /* entry bits of a code */
+    int32_t f_bits;        /* load-vector index bits of an fcode */
 } anifem_block_layout;
 int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out);
 int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_tet0, int32_t* blk_tets,

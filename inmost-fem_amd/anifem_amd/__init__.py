@@ -77,7 +77,8 @@ class anifem_timers(ctypes.Structure):
 
 class anifem_block_layout(ctypes.Structure):
     _fields_ = [("n_blocks", ctypes.c_int64), ("n_block_tets", ctypes.c_int64), ("n_codes", ctypes.c_int64),
-                ("n_fcodes", ctypes.c_int64), ("code_bytes", ctypes.c_int32), ("block_tet_cap", ctypes.c_int32)]
+                ("n_fcodes", ctypes.c_int64), ("code_bytes", ctypes.c_int32), ("block_tet_cap", ctypes.c_int32),
+                ("e_bits", ctypes.c_int32), ("f_bits", ctypes.c_int32)]
 
 
 class anifem_pcg_result(ctypes.Structure):
@@ -294,6 +295,7 @@ class Assembler:
         k = ["blk_row0", "blk_tet0", "blk_tets", "blk_c0", "codes", "sstart", "blk_g0", "fcodes", "rstart"]
         _check(lib().anifem_assembly_copy_blocks(self._h, *[_ptr(out[x]) for x in k]), "copy_blocks")
         out["code_bytes"], out["block_tet_cap"] = lay.code_bytes, lay.block_tet_cap
+        out["e_bits"], out["f_bits"] = lay.e_bits, lay.f_bits
         return out
 
     def csr(self):

@@ -28,8 +28,8 @@ struct anifem_assembly {
     int device = 0;
     double* d_coords = nullptr;       // n_nodes x 3
     int32_t* d_tetnodes = nullptr;    // 4 x n_tets SoA
-    uint64_t* d_dirmask = nullptr;    // per tet (null without Dirichlet)
     int32_t* d_blk_row0 = nullptr;
+    int64_t* d_blk_s0 = nullptr;
     int64_t* d_blk_tet0 = nullptr;
     int32_t* d_blk_tets = nullptr;
     int64_t* d_blk_c0 = nullptr;
@@ -86,14 +86,16 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
         return ANIFEM_ERR_DIMS;
     int max_lds = 0;
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-    const int cb = S.code16 ? 2 : 4;
+    const int lenp = pad16<T>(a.m.q * DA0 * DB0);
+    const int lenFp = a.has_rhs ? pad16<T>(a.f.q) : 0;
     auto lds_of = [&](bool with_R, bool with_RF) {
-        return block_lds_layout(a.max_tets, kstride, kstrideF, (int)sizeof(T), with_R ? a.R_len : 0,
-                                with_RF ? a.RF_len : 0, a.m.E, a.nf, a.has_rhs, a.max_contrib, a.max_fcontrib, cb)
+        return block_lds_layout(a.max_tets, NCA * NCB * lenp, NCB * lenFp, (int)sizeof(T),
+                                with_R ? a.R_rows * lenp : 0, with_RF ? a.RF_rows * lenFp : 0, a.m.E, a.nf,
+                                a.has_rhs, a.max_contrib, a.max_fcontrib, a.max_slots)
             .total;
     };
-    // the reference tensors are staged while the workgroup stays within 64 KiB
-    const size_t soft = 64 * 1024;
+    // the reference tensors are staged while four workgroups still fit a CU
+    const size_t soft = (size_t)max_lds / 4;
     a.stage_R = lds_of(true, false) <= soft;
     a.stage_RF = a.has_rhs && lds_of(a.stage_R, true) <= soft;
     const size_t lds = lds_of(a.stage_R, a.stage_RF);
@@ -164,20 +166,12 @@ int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh, 
     ANIFEM_HIP_CHECK(hipGetDevice(&ctx->device));
     const int64_t NT = S.n_tets;
     std::vector<int32_t> tn(4 * NT);
-    std::vector<uint64_t> dm;
-    bool any_dir = false;
-    for (uint8_t f : S.is_dir) any_dir = any_dir || f;
-    if (any_dir) dm.assign(NT, 0);
-    for (int64_t t = 0; t < NT; ++t) {
+    for (int64_t t = 0; t < NT; ++t)
         for (int v = 0; v < 4; ++v) tn[(size_t)v * NT + t] = S.tets[4 * (size_t)t + v];
-        if (any_dir)
-            for (int i = 0; i < S.nf; ++i)
-                if (S.is_dir[S.dofs[(size_t)S.nf * t + i]]) dm[t] |= 1ull << i;
-    }
     if ((rc = upload(&ctx->d_coords, mesh->coords, 3 * (size_t)S.n_nodes, s))) return rc;
     if ((rc = upload(&ctx->d_tetnodes, tn.data(), tn.size(), s))) return rc;
-    if (any_dir && (rc = upload(&ctx->d_dirmask, dm.data(), dm.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_row0, S.blk_row0.data(), S.blk_row0.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_blk_s0, S.blk_s0.data(), S.blk_s0.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_tet0, S.blk_tet0.data(), S.blk_tet0.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_tets, S.blk_tets.data(), S.blk_tets.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_c0, S.blk_c0.data(), S.blk_c0.size(), s))) return rc;
@@ -226,7 +220,7 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (!ctx->host_only) {
-        void* bufs[] = {ctx->d_coords,  ctx->d_tetnodes, ctx->d_dirmask, ctx->d_blk_row0, ctx->d_blk_tet0,
+        void* bufs[] = {ctx->d_coords,  ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_s0, ctx->d_blk_tet0,
                         ctx->d_blk_tets, ctx->d_blk_c0,   ctx->d_codes,   ctx->d_sstart,   ctx->d_blk_g0,
                         ctx->d_fcodes,  ctx->d_rstart,   ctx->d_row_ptr, ctx->d_col_idx,  ctx->d_status,
                         ctx->d_kbuf};
@@ -275,6 +269,8 @@ int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out)
     out->n_fcodes = S.blk_g0.empty() ? 0 : S.blk_g0.back();
     out->code_bytes = S.code16 ? 2 : 4;
     out->block_tet_cap = S.block_tet_cap;
+    out->e_bits = S.ebits;
+    out->f_bits = S.fbits;
     return ANIFEM_OK;
 }
 
@@ -342,18 +338,18 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     a.m.tetnodes = ctx->d_tetnodes;
     a.m.nodes_stride = S.n_tets;
     a.m.pos0 = 0;
-    a.R_len = a.m.nfA0 * a.m.nfB0 * a.m.q * ref_dim(ctx->prob.trial.op) * ref_dim(ctx->prob.test.op);
+    a.R_rows = a.m.nfA0 * a.m.nfB0;
     if (do_rhs) {
         anifem_op p0{ANIFEM_IDEN, ANIFEM_FEM_P0, 1};
         anifem_op test_iden{ANIFEM_IDEN, ctx->prob.test.fem, ctx->prob.test.ncomp};
         if ((rc = prepare_elem_args(p0, test_iden, F, ctx->prob.rhs_order, a.f))) return rc;
         a.f.coef_n = S.n_tets;
-        a.RF_len = a.f.nfB0 * a.f.q;
+        a.RF_rows = a.f.nfB0;
     }
     a.has_rhs = do_rhs ? 1 : 0;
     a.nf = S.nf;
     a.blk_row0 = ctx->d_blk_row0;
-    a.row_ptr = ctx->d_row_ptr;
+    a.blk_s0 = ctx->d_blk_s0;
     a.blk_tet0 = ctx->d_blk_tet0;
     a.blk_tets = ctx->d_blk_tets;
     a.blk_c0 = ctx->d_blk_c0;
@@ -362,13 +358,16 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     a.blk_g0 = ctx->d_blk_g0;
     a.fcodes = ctx->d_fcodes;
     a.rstart = ctx->d_rstart;
-    a.dirmask = ctx->d_dirmask;
+    a.ebits = S.ebits;
+    a.fbits = S.fbits;
     a.val = csr_val;
     a.rhs = rhs;
     a.status = ctx->d_status;
     a.max_tets = (int)std::max<int64_t>(1, S.max_block_tets);
     a.max_contrib = (int)S.max_block_contrib;
     a.max_fcontrib = do_rhs ? (int)S.max_block_fcontrib : 0;
+    a.max_slots = (int)S.max_block_slots;
+    if (S.max_block_contrib > (int64_t)kBlockCodesPerLane * kBlock) return ANIFEM_ERR_DIMS;
 
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
     // a rhs without a load vector is zero

@@ -5,7 +5,10 @@
 // slot of its rows, the (tet, entry) pairs that add into it, in ascending tet
 // order: the order in which the reference's element loop adds the element
 // matrices into the global matrix (assembler.inl:381-430).  One workgroup per
-// block then forms every slot value of its rows exactly once.
+// block then forms every slot value of its rows exactly once.  Element-level
+// homogeneous Dirichlet (dc_on_dof.h:38-45) is resolved here: the only listed
+// entries it touches are diagonal ones, whose value becomes exactly 1, and
+// it zeroes every load-vector entry of a Dirichlet row.
 #include <algorithm>
 #include <atomic>
 
@@ -18,6 +21,7 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     const int nf = S.nf, E = S.E;
 
     // local index of row r in each incident tet, and the row's contributions
+    // (a Dirichlet row has no load-vector contributions: applyDir makes them 0)
     std::vector<uint8_t> lidx(adj.size());
     std::vector<int32_t> row_nc(ND, 0);
     host_parallel_for(ND, 4096, [&](int64_t b, int64_t e) {
@@ -44,11 +48,18 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     int64_t cap = std::max<int64_t>(1, std::min<int64_t>(kBlockMaxTets, kBlockKBytes / (8 * S.k_per_tet)));
     cap = std::max(cap, max_valence);
     if (cap > kBlockHardMaxTets || cap * 8 * S.k_per_tet > kBlockMaxKBytes) return ANIFEM_ERR_DIMS;
-    S.code16 = cap * E <= 65536;
-    if (!S.code16 && 65536 / E >= std::max<int64_t>(64, max_valence)) {
-        cap = 65536 / E;
+    // codes: local_tet << ebits | entry, entry == E marking a Dirichlet
+    // diagonal entry; load-vector codes local_tet << fbits | i
+    S.ebits = 1;
+    while ((1 << S.ebits) <= E) ++S.ebits;
+    S.fbits = 0;
+    while ((1 << S.fbits) < nf) ++S.fbits;
+    S.code16 = (cap << S.ebits) <= 65536;
+    if (!S.code16 && (65536 >> S.ebits) >= std::max<int64_t>(64, max_valence)) {
+        cap = 65536 >> S.ebits;
         S.code16 = true;
     }
+    if ((cap << S.fbits) > 65536) return ANIFEM_ERR_DIMS;
     S.block_tet_cap = (int)cap;
 
     // greedy cut: extend the current block while every budget holds
@@ -69,14 +80,15 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
             slots = nc = ng = nt = 0;
         };
         for (int64_t r = 0; r < ND; ++r) {
-            const int64_t rs = S.row_ptr[r + 1] - S.row_ptr[r], rc = row_nc[r], rg = aptr[r + 1] - aptr[r];
+            const int64_t rs = S.row_ptr[r + 1] - S.row_ptr[r], rc = row_nc[r];
+            const int64_t rg = S.is_dir[r] ? 0 : aptr[r + 1] - aptr[r];
             if (rs > kBlockMaxSlots || rc > kBlockMaxContrib || rg > kBlockMaxContrib) return ANIFEM_ERR_DIMS;
             int64_t fresh = 0;
             for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) fresh += stamp[adj[p]] != cur;
             if (slots && (slots + rs > kBlockMaxSlots || nc + rc > kBlockMaxContrib || ng + rg > kBlockMaxContrib ||
                           nt + fresh > S.block_tet_cap)) {
                 close(r);
-                fresh = rg;
+                fresh = aptr[r + 1] - aptr[r];
             }
             for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) stamp[adj[p]] = cur;
             slots += rs;
@@ -87,6 +99,8 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
         if (slots || ND == 0) close(ND);
     }
     S.n_blocks = (int64_t)S.blk_row0.size() - 1;
+    S.blk_s0.resize(S.n_blocks + 1);
+    for (int64_t b = 0; b <= S.n_blocks; ++b) S.blk_s0[b] = S.row_ptr[S.blk_row0[b]];
 
     // fill the lists, one block at a time
     S.blk_tets.resize(S.blk_tet0.back());
@@ -128,16 +142,23 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
                 const int64_t row_end = pos[rs];
                 for (int64_t u = 0; u < rs; ++u) S.sstart[base + u] = (uint16_t)pos[u];
                 S.rstart[r] = (uint16_t)g;
+                const bool rdir = S.is_dir[r] != 0;
                 for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) {
                     const int32_t t = adj[p];
                     const uint32_t tloc = (uint32_t)(std::lower_bound(tl, tl + k, t) - tl);
                     const int ib = lidx[p];
                     const int32_t* sl = &S.slot[(size_t)E * t + ib];
+                    const int32_t* td = &S.dofs[(size_t)nf * t];
                     for (int ia = 0; ia < nf; ++ia) {
                         const int32_t s = sl[(size_t)nf * ia];
-                        if (s >= 0) cd[pos[s - base]++] = tloc * (uint32_t)E + (uint32_t)(ib + nf * ia);
+                        if (s < 0) continue;
+                        // a listed entry on a Dirichlet row or column is the
+                        // diagonal one (the pattern drops the others), which
+                        // applyDir sets to 1 (dc_on_dof.h:38-45)
+                        const bool dir = rdir || S.is_dir[td[ia]];
+                        cd[pos[s - base]++] = (tloc << S.ebits) | (uint32_t)(dir ? E : ib + nf * ia);
                     }
-                    fc[g++] = (uint16_t)(tloc * nf + ib);
+                    if (!rdir) fc[g++] = (uint16_t)((tloc << S.fbits) | (uint32_t)ib);
                 }
                 c = row_end;
             }
@@ -154,6 +175,8 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     S.max_block_tets = max_tets.load();
     S.max_block_contrib = max_c.load();
     S.max_block_fcontrib = max_g.load();
+    for (int64_t b = 0; b < S.n_blocks; ++b)
+        S.max_block_slots = std::max(S.max_block_slots, S.row_ptr[S.blk_row0[b + 1]] - S.row_ptr[S.blk_row0[b]]);
     return ANIFEM_OK;
 }
 

@@ -135,10 +135,11 @@ def _emulate_blocks(A, s, B, F=None):
             acc = 0.0
             last = -1
             for code in B["codes"][lo:hi]:
-                t = tets[code // E]
+                t = tets[code >> B["e_bits"]]
+                e = code & ((1 << B["e_bits"]) - 1)
                 assert t > last, "contributions of a slot in ascending tet order"
                 last = t
-                acc += A[t, code % E]
+                acc += 1.0 if e == E else A[t, e]
             val[u] = acc
         if F is not None:
             g0, g1 = B["blk_g0"][b], B["blk_g0"][b + 1]
@@ -147,7 +148,7 @@ def _emulate_blocks(A, s, B, F=None):
                 hi = g0 + B["rstart"][r + 1] if r + 1 < r1 else g1
                 acc = 0.0
                 for code in B["fcodes"][lo:hi]:
-                    acc += F[tets[code // F.shape[1]], code % F.shape[1]]
+                    acc += F[tets[code >> B["f_bits"]], code & ((1 << B["f_bits"]) - 1)]
                 rhs[r] = acc
     return val, rhs
 
@@ -174,8 +175,9 @@ def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
     # budgets: rows contiguous and covering, slot / tet / code limits
     r0 = B["blk_row0"]
     assert r0[0] == 0 and r0[-1] == A.ndof and np.all(np.diff(r0) > 0)
-    assert np.all(np.diff(s["row_ptr"][r0]) <= 4096)
+    assert np.all(np.diff(s["row_ptr"][r0]) <= 2048)
     assert np.all(np.diff(B["blk_tet0"]) <= B["block_tet_cap"])
-    assert np.all(np.diff(B["blk_c0"]) <= 8192)
-    assert B["code_bytes"] == (2 if B["block_tet_cap"] * E <= 65536 else 4)
+    assert np.all(np.diff(B["blk_c0"]) <= 2048)
+    assert B["code_bytes"] == (2 if B["block_tet_cap"] << B["e_bits"] <= 65536 else 4)
+    assert (1 << B["e_bits"]) > E and (1 << B["f_bits"]) >= nf
     assert len(B["codes"]) == int((s["slot"] >= 0).sum())
