@@ -167,31 +167,32 @@ int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int
                                int64_t* slot, uint8_t* dirichlet);
 /* Row blocks of the owner-computes assembly.  Block b owns the CSR rows
  * [blk_row0[b], blk_row0[b+1]) and forms every slot value and rhs entry of
- * them exactly once.  It lists its distinct tets (ascending) in
- * blk_tets[blk_tet0[b] ..]; for each slot s of its rows the codes
- * codes[blk_c0[b] + sstart[s] .. blk_c0[b] + sstart[s+1]) (the last slot of a
- * block ends at blk_c0[b+1]) name the contributions local_tet << e_bits |
- * entry (entry ib + nf*ia; entry == nf*nf marks a Dirichlet diagonal entry,
- * whose value applyDir sets to 1) in ascending tet order; likewise fcodes /
- * rstart / blk_g0 for the rhs with local_tet << f_bits | i (Dirichlet rows
- * have none).  No reference counterpart: this replaces the row locks of the
- * threaded reference assembler (assembler.inl:339-340,406,474).  Sizes first
- * (layout), then the host copies (host-only contexts keep them; any pointer
+ * them exactly once.  Its pairs [blk_p0[b], blk_p0[b+1]) are the (row, incident
+ * tet) pairs of its rows, by row then ascending tet: pair p forms row
+ * pair_ib[p] & 63 of tet pair_tet[p]'s element matrix and that entry of its
+ * load vector; bit 0x80 of pair_ib marks a Dirichlet row (applyDir: the kept
+ * diagonal entry is 1, the load-vector entry 0).  The block's element entries
+ * form a value list of blk_v0[b+1] - blk_v0[b] entries grouped by CSR slot,
+ * each slot's entries in ascending tet order (the reference's element-loop
+ * order); pair_pos[p*nf + ia] is the list position of entry (ib, ia), 0xFFFF
+ * for an entry the pattern drops.  sstart[s] is the first list position of
+ * slot s and pstart[r] the first pair of row r, both relative to the block.
+ * No reference counterpart: this replaces the row locks of the threaded
+ * reference assembler (assembler.inl:339-340,406,474).  Sizes first (layout),
+ * then the host copies (host-only contexts keep the pair lists; any pointer
  * may be NULL). */
 typedef struct anifem_block_layout {
     int64_t n_blocks;
-    int64_t n_block_tets;  /* length of blk_tets */
-    int64_t n_codes;       /* length of codes */
-    int64_t n_fcodes;      /* length of fcodes */
-    int32_t code_bytes;    /* device code width: 2 or 4 */
-    int32_t block_tet_cap; /* tets per block at most */
-    int32_t e_bits;        /* entry bits of a code */
-    int32_t f_bits;        /* load-vector index bits of an fcode */
+    int64_t n_pairs;
+    int64_t n_values;         /* element entries kept in the pattern, all blocks */
+    int64_t max_block_pairs;
+    int64_t max_block_values;
+    int64_t max_block_slots;
 } anifem_block_layout;
 int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out);
-int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_tet0, int32_t* blk_tets,
-                                int64_t* blk_c0, uint32_t* codes, uint16_t* sstart, int64_t* blk_g0,
-                                uint16_t* fcodes, uint16_t* rstart);
+int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_p0, int64_t* blk_v0,
+                                int32_t* pair_tet, uint8_t* pair_ib, uint16_t* pair_pos, uint16_t* sstart,
+                                uint16_t* pstart);
 /* Device node coordinates (n_nodes x 3) used by run; the caller may update
  * them in place (moving mesh) between runs. */
 int anifem_assembly_coords(anifem_assembly* ctx, double** coords);

@@ -76,9 +76,9 @@ class anifem_timers(ctypes.Structure):
 
 
 class anifem_block_layout(ctypes.Structure):
-    _fields_ = [("n_blocks", ctypes.c_int64), ("n_block_tets", ctypes.c_int64), ("n_codes", ctypes.c_int64),
-                ("n_fcodes", ctypes.c_int64), ("code_bytes", ctypes.c_int32), ("block_tet_cap", ctypes.c_int32),
-                ("e_bits", ctypes.c_int32), ("f_bits", ctypes.c_int32)]
+    _fields_ = [("n_blocks", ctypes.c_int64), ("n_pairs", ctypes.c_int64), ("n_values", ctypes.c_int64),
+                ("max_block_pairs", ctypes.c_int64), ("max_block_values", ctypes.c_int64),
+                ("max_block_slots", ctypes.c_int64)]
 
 
 class anifem_pcg_result(ctypes.Structure):
@@ -103,7 +103,7 @@ SIGNATURES = {
     "anifem_assembly_csr": (_I32, [_VP, _VP, _VP]),
     "anifem_assembly_copy_setup": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "anifem_assembly_blocks": (_I32, [_VP, ctypes.POINTER(anifem_block_layout)]),
-    "anifem_assembly_copy_blocks": (_I32, [_VP] + [_VP] * 9),
+    "anifem_assembly_copy_blocks": (_I32, [_VP] + [_VP] * 8),
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
     "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
@@ -284,18 +284,16 @@ class Assembler:
 
     def blocks(self):
         """Host copies of the row blocks (host-only contexts): dict of blk_row0,
-        blk_tet0, blk_tets, blk_c0, codes, sstart, blk_g0, fcodes, rstart."""
+        blk_p0, blk_v0, pair_tet, pair_ib, pair_pos (n_pairs x nf), sstart, pstart."""
         lay = self.block_layout()
-        nb = lay.n_blocks
-        out = dict(blk_row0=np.zeros(nb + 1, np.int32), blk_tet0=np.zeros(nb + 1, np.int64),
-                   blk_tets=np.zeros(lay.n_block_tets, np.int32), blk_c0=np.zeros(nb + 1, np.int64),
-                   codes=np.zeros(lay.n_codes, np.uint32), sstart=np.zeros(self.nnz, np.uint16),
-                   blk_g0=np.zeros(nb + 1, np.int64), fcodes=np.zeros(lay.n_fcodes, np.uint16),
-                   rstart=np.zeros(self.ndof, np.uint16))
-        k = ["blk_row0", "blk_tet0", "blk_tets", "blk_c0", "codes", "sstart", "blk_g0", "fcodes", "rstart"]
+        nb, npair = lay.n_blocks, lay.n_pairs
+        out = dict(blk_row0=np.zeros(nb + 1, np.int32), blk_p0=np.zeros(nb + 1, np.int64),
+                   blk_v0=np.zeros(nb + 1, np.int64), pair_tet=np.zeros(npair, np.int32),
+                   pair_ib=np.zeros(npair, np.uint8), pair_pos=np.zeros((npair, self.nf), np.uint16),
+                   sstart=np.zeros(self.nnz, np.uint16), pstart=np.zeros(self.ndof, np.uint16))
+        k = ["blk_row0", "blk_p0", "blk_v0", "pair_tet", "pair_ib", "pair_pos", "sstart", "pstart"]
         _check(lib().anifem_assembly_copy_blocks(self._h, *[_ptr(out[x]) for x in k]), "copy_blocks")
-        out["code_bytes"], out["block_tet_cap"] = lay.code_bytes, lay.block_tet_cap
-        out["e_bits"], out["f_bits"] = lay.e_bits, lay.f_bits
+        out["layout"] = lay
         return out
 
     def csr(self):

@@ -30,14 +30,13 @@ struct anifem_assembly {
     int32_t* d_tetnodes = nullptr;    // 4 x n_tets SoA
     int32_t* d_blk_row0 = nullptr;
     int64_t* d_blk_s0 = nullptr;
-    int64_t* d_blk_tet0 = nullptr;
-    int32_t* d_blk_tets = nullptr;
-    int64_t* d_blk_c0 = nullptr;
-    void* d_codes = nullptr;          // uint16 or uint32 (S.code16)
+    int64_t* d_blk_p0 = nullptr;
+    int64_t* d_blk_v0 = nullptr;
+    int32_t* d_pair_tet = nullptr;
+    uint8_t* d_pair_ib = nullptr;
+    uint16_t* d_pair_pos = nullptr;
     uint16_t* d_sstart = nullptr;
-    int64_t* d_blk_g0 = nullptr;
-    uint16_t* d_fcodes = nullptr;
-    uint16_t* d_rstart = nullptr;
+    uint16_t* d_pstart = nullptr;
     int64_t* d_row_ptr = nullptr;
     int32_t* d_col_idx = nullptr;
     int* d_status = nullptr;          // [0] non-finite, [1] degenerate
@@ -79,29 +78,24 @@ template <int DA0, int DB0, int NCA, int NCB, typename T>
 int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     using namespace anifem;
     const AssemblySetup& S = ctx->S;
-    const int kstride = NCA * NCB * a.m.q * DA0 * DB0;
-    const int kstrideF = a.has_rhs ? NCB * a.f.q : 0;
-    // row_gather indexes a block's K values with 16-bit arithmetic
-    if ((int64_t)a.max_tets * std::max(kstride, 1) >= 65536 || (int64_t)a.max_tets * std::max(kstrideF, 1) >= 65536)
-        return ANIFEM_ERR_DIMS;
+    const int len = a.m.q * DA0 * DB0, lenF = a.has_rhs ? a.f.q : 0;
+    const int kstride = NCA * NCB * len;
+    a.rowlen = pad16<T>(kstride + (a.has_rhs ? NCB * lenF : 0));
     int max_lds = 0;
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-    const int lenp = pad16<T>(a.m.q * DA0 * DB0);
-    const int lenFp = a.has_rhs ? pad16<T>(a.f.q) : 0;
-    auto lds_of = [&](bool with_R, bool with_RF) {
-        return block_lds_layout(a.max_tets, NCA * NCB * lenp, NCB * lenFp, (int)sizeof(T),
-                                with_R ? a.R_rows * lenp : 0, with_RF ? a.RF_rows * lenFp : 0, a.m.E, a.nf,
-                                a.has_rhs, a.max_contrib, a.max_fcontrib, a.max_slots)
+    auto lds_of = [&](bool with_R) {
+        return block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T),
+                                with_R ? a.R_rows * pad16<T>(len) : 0,
+                                with_R && a.has_rhs ? a.RF_rows * pad16<T>(lenF) : 0)
             .total;
     };
-    // the reference tensors are staged while four workgroups still fit a CU
-    const size_t soft = (size_t)max_lds / 4;
-    a.stage_R = lds_of(true, false) <= soft;
-    a.stage_RF = a.has_rhs && lds_of(a.stage_R, true) <= soft;
-    const size_t lds = lds_of(a.stage_R, a.stage_RF);
+    // the reference tensors are staged in LDS while two workgroups fit a CU
+    // (and the staging's 16-bit indexing holds)
+    const bool staged = lds_of(true) <= (size_t)max_lds / 2 && a.R_rows * len < 65536 && a.RF_rows * lenF < 65536;
+    const size_t lds = lds_of(staged);
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
-    // geometry tensors of every tet (kept on the device between runs)
-    const size_t need = (size_t)S.n_tets * (kstride + kstrideF) * sizeof(T);
+    // geometry rows of every tet (kept on the device between runs)
+    const size_t need = (size_t)S.n_tets * a.rowlen * sizeof(T);
     if (ctx->kbuf_bytes < need) {
         if (ctx->d_kbuf) ANIFEM_HIP_CHECK(hipFree(ctx->d_kbuf));
         ctx->d_kbuf = nullptr;
@@ -110,21 +104,19 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
         ctx->kbuf_bytes = need;
     }
     T* Kg = static_cast<T*>(ctx->d_kbuf);
-    T* KFg = Kg + (size_t)S.n_tets * kstride;
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
-                       KFg, S.n_tets);
+                       S.n_tets);
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
-    if (S.n_blocks > 0) {
-        if (S.code16)
-            hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, uint16_t>), dim3((unsigned)S.n_blocks),
-                               dim3(kBlock), lds, s, a, (const T*)Kg, (const T*)KFg);
-        else
-            hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, uint32_t>), dim3((unsigned)S.n_blocks),
-                               dim3(kBlock), lds, s, a, (const T*)Kg, (const T*)KFg);
-        ANIFEM_HIP_CHECK(hipGetLastError());
-    }
+    if (S.n_blocks == 0) return ANIFEM_OK;
+    if (staged)
+        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, true>), dim3((unsigned)S.n_blocks), dim3(kBlock),
+                           lds, s, a, (const T*)Kg);
+    else
+        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, false>), dim3((unsigned)S.n_blocks), dim3(kBlock),
+                           lds, s, a, (const T*)Kg);
+    ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
 
@@ -172,35 +164,23 @@ int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh, 
     if ((rc = upload(&ctx->d_tetnodes, tn.data(), tn.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_row0, S.blk_row0.data(), S.blk_row0.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_s0, S.blk_s0.data(), S.blk_s0.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_blk_tet0, S.blk_tet0.data(), S.blk_tet0.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_blk_tets, S.blk_tets.data(), S.blk_tets.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_blk_c0, S.blk_c0.data(), S.blk_c0.size(), s))) return rc;
-    std::vector<uint16_t> codes16;
-    if (S.code16) {
-        codes16.resize(S.codes.size());
-        for (size_t i = 0; i < S.codes.size(); ++i) codes16[i] = (uint16_t)S.codes[i];
-        uint16_t* d16 = nullptr;
-        rc = upload(&d16, codes16.data(), codes16.size(), s);
-        ctx->d_codes = d16;
-    } else {
-        uint32_t* d32 = nullptr;
-        rc = upload(&d32, S.codes.data(), S.codes.size(), s);
-        ctx->d_codes = d32;
-    }
-    if (rc) return rc;
+    if ((rc = upload(&ctx->d_blk_p0, S.blk_p0.data(), S.blk_p0.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_blk_v0, S.blk_v0.data(), S.blk_v0.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_pair_tet, S.pair_tet.data(), S.pair_tet.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_pair_ib, S.pair_ib.data(), S.pair_ib.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_pair_pos, S.pair_pos.data(), S.pair_pos.size(), s))) return rc;
     if ((rc = upload(&ctx->d_sstart, S.sstart.data(), S.sstart.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_blk_g0, S.blk_g0.data(), S.blk_g0.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_fcodes, S.fcodes.data(), S.fcodes.size(), s))) return rc;
-    if ((rc = upload(&ctx->d_rstart, S.rstart.data(), S.rstart.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_pstart, S.pstart.data(), S.pstart.size(), s))) return rc;
     if ((rc = upload(&ctx->d_row_ptr, S.row_ptr.data(), S.row_ptr.size(), s))) return rc;
     if ((rc = upload(&ctx->d_col_idx, S.col_idx.data(), S.col_idx.size(), s))) return rc;
     ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_status, 4 * sizeof(int)));
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
-    // the device holds the contribution lists; the host keeps the rest for inspection
-    std::vector<uint32_t>().swap(S.codes);
+    // the device holds the pair lists; the host keeps the rest for inspection
+    std::vector<int32_t>().swap(S.pair_tet);
+    std::vector<uint8_t>().swap(S.pair_ib);
+    std::vector<uint16_t>().swap(S.pair_pos);
     std::vector<uint16_t>().swap(S.sstart);
-    std::vector<uint16_t>().swap(S.fcodes);
-    std::vector<uint16_t>().swap(S.rstart);
+    std::vector<uint16_t>().swap(S.pstart);
     *out = ctx.release();
     return ANIFEM_OK;
 }
@@ -220,10 +200,9 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (!ctx->host_only) {
-        void* bufs[] = {ctx->d_coords,  ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_s0, ctx->d_blk_tet0,
-                        ctx->d_blk_tets, ctx->d_blk_c0,   ctx->d_codes,   ctx->d_sstart,   ctx->d_blk_g0,
-                        ctx->d_fcodes,  ctx->d_rstart,   ctx->d_row_ptr, ctx->d_col_idx,  ctx->d_status,
-                        ctx->d_kbuf};
+        void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_s0,   ctx->d_blk_p0,
+                        ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
+                        ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -264,35 +243,32 @@ int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out)
     if (!ctx || !out) return ANIFEM_ERR_ARG;
     const anifem::AssemblySetup& S = ctx->S;
     out->n_blocks = S.n_blocks;
-    out->n_block_tets = (int64_t)S.blk_tets.size();
-    out->n_codes = S.blk_c0.empty() ? 0 : S.blk_c0.back();
-    out->n_fcodes = S.blk_g0.empty() ? 0 : S.blk_g0.back();
-    out->code_bytes = S.code16 ? 2 : 4;
-    out->block_tet_cap = S.block_tet_cap;
-    out->e_bits = S.ebits;
-    out->f_bits = S.fbits;
+    out->n_pairs = S.blk_p0.empty() ? 0 : S.blk_p0.back();
+    out->n_values = S.blk_v0.empty() ? 0 : S.blk_v0.back();
+    out->max_block_pairs = S.max_block_pairs;
+    out->max_block_values = S.max_block_values;
+    out->max_block_slots = S.max_block_slots;
     return ANIFEM_OK;
 }
 
-int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_tet0, int32_t* blk_tets,
-                                int64_t* blk_c0, uint32_t* codes, uint16_t* sstart, int64_t* blk_g0,
-                                uint16_t* fcodes, uint16_t* rstart) {
+int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_p0, int64_t* blk_v0,
+                                int32_t* pair_tet, uint8_t* pair_ib, uint16_t* pair_pos, uint16_t* sstart,
+                                uint16_t* pstart) {
     if (!ctx) return ANIFEM_ERR_ARG;
     const anifem::AssemblySetup& S = ctx->S;
     // contexts built for the device keep only the block offsets on the host
-    if (!ctx->host_only && (codes || sstart || fcodes || rstart)) return ANIFEM_ERR_ARG;
+    if (!ctx->host_only && (pair_tet || pair_ib || pair_pos || sstart || pstart)) return ANIFEM_ERR_ARG;
     auto cp = [](auto* dst, const auto& v) {
         if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
     };
     cp(blk_row0, S.blk_row0);
-    cp(blk_tet0, S.blk_tet0);
-    cp(blk_tets, S.blk_tets);
-    cp(blk_c0, S.blk_c0);
-    cp(codes, S.codes);
+    cp(blk_p0, S.blk_p0);
+    cp(blk_v0, S.blk_v0);
+    cp(pair_tet, S.pair_tet);
+    cp(pair_ib, S.pair_ib);
+    cp(pair_pos, S.pair_pos);
     cp(sstart, S.sstart);
-    cp(blk_g0, S.blk_g0);
-    cp(fcodes, S.fcodes);
-    cp(rstart, S.rstart);
+    cp(pstart, S.pstart);
     return ANIFEM_OK;
 }
 
@@ -348,26 +324,22 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     }
     a.has_rhs = do_rhs ? 1 : 0;
     a.nf = S.nf;
+    a.n_blocks = S.n_blocks;
     a.blk_row0 = ctx->d_blk_row0;
     a.blk_s0 = ctx->d_blk_s0;
-    a.blk_tet0 = ctx->d_blk_tet0;
-    a.blk_tets = ctx->d_blk_tets;
-    a.blk_c0 = ctx->d_blk_c0;
-    a.codes = ctx->d_codes;
+    a.blk_p0 = ctx->d_blk_p0;
+    a.blk_v0 = ctx->d_blk_v0;
+    a.pair_tet = ctx->d_pair_tet;
+    a.pair_ib = ctx->d_pair_ib;
+    a.pair_pos = ctx->d_pair_pos;
     a.sstart = ctx->d_sstart;
-    a.blk_g0 = ctx->d_blk_g0;
-    a.fcodes = ctx->d_fcodes;
-    a.rstart = ctx->d_rstart;
-    a.ebits = S.ebits;
-    a.fbits = S.fbits;
+    a.pstart = ctx->d_pstart;
     a.val = csr_val;
     a.rhs = rhs;
     a.status = ctx->d_status;
-    a.max_tets = (int)std::max<int64_t>(1, S.max_block_tets);
-    a.max_contrib = (int)S.max_block_contrib;
-    a.max_fcontrib = do_rhs ? (int)S.max_block_fcontrib : 0;
+    a.max_values = (int)S.max_block_values;
+    a.max_pairs = (int)S.max_block_pairs;
     a.max_slots = (int)S.max_block_slots;
-    if (S.max_block_contrib > (int64_t)kBlockCodesPerLane * kBlock) return ANIFEM_ERR_DIMS;
 
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
     // a rhs without a load vector is zero

@@ -1,14 +1,20 @@
 // Row blocks of the owner-computes assembly (see setup.h).
 //
-// The CSR rows are cut into blocks of consecutive rows.  A block lists the
-// distinct tets incident to its rows (ascending tet id) and, for every CSR
-// slot of its rows, the (tet, entry) pairs that add into it, in ascending tet
-// order: the order in which the reference's element loop adds the element
-// matrices into the global matrix (assembler.inl:381-430).  One workgroup per
-// block then forms every slot value of its rows exactly once.  Element-level
-// homogeneous Dirichlet (dc_on_dof.h:38-45) is resolved here: the only listed
-// entries it touches are diagonal ones, whose value becomes exactly 1, and
-// it zeroes every load-vector entry of a Dirichlet row.
+// The CSR rows are cut into blocks of consecutive rows.  Within a block:
+//   * a pair is one (row r, incident tet t): it forms row ib of t's element
+//     matrix (ib = the local index of r in t) and entry ib of t's load vector;
+//     pairs are ordered by row, then ascending tet id;
+//   * the block's element entries are laid out in a value list grouped by CSR
+//     slot, each slot's entries in ascending tet id: the order in which the
+//     reference's element loop adds the element matrices into the global
+//     matrix (assembler.inl:381-430); pair_pos gives, for every entry of a
+//     pair, its place in that list, or kPosDropped for an entry the pattern
+//     drops.
+// One workgroup per block then forms every slot value and rhs entry of its
+// rows exactly once.  Element-level homogeneous Dirichlet (dc_on_dof.h:38-45)
+// is resolved here: the pattern keeps only the diagonal entry of a Dirichlet
+// row or column, whose value applyDir sets to 1, and a Dirichlet row's load
+// vector entries are 0; such a pair carries kPairDirichlet.
 #include <algorithm>
 #include <atomic>
 
@@ -20,13 +26,12 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     const int64_t ND = S.ndof;
     const int nf = S.nf, E = S.E;
 
-    // local index of row r in each incident tet, and the row's contributions
-    // (a Dirichlet row has no load-vector contributions: applyDir makes them 0)
+    // local index of row r in each incident tet, and the row's entries
     std::vector<uint8_t> lidx(adj.size());
-    std::vector<int32_t> row_nc(ND, 0);
+    std::vector<int32_t> row_nv(ND, 0);
     host_parallel_for(ND, 4096, [&](int64_t b, int64_t e) {
         for (int64_t r = b; r < e; ++r) {
-            int32_t nc = 0;
+            int32_t nv = 0;
             for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) {
                 const int64_t t = adj[p];
                 const int32_t* td = &S.dofs[(size_t)nf * t];
@@ -34,67 +39,32 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
                 while (td[ib] != r) ++ib;
                 lidx[p] = (uint8_t)ib;
                 const int32_t* sl = &S.slot[(size_t)E * t + ib];
-                for (int ia = 0; ia < nf; ++ia) nc += sl[(size_t)nf * ia] >= 0;
+                for (int ia = 0; ia < nf; ++ia) nv += sl[(size_t)nf * ia] >= 0;
             }
-            row_nc[r] = nc;
+            row_nv[r] = nv;
         }
     });
-    int64_t max_valence = 0;
-    for (int64_t r = 0; r < ND; ++r) max_valence = std::max<int64_t>(max_valence, aptr[r + 1] - aptr[r]);
-
-    // tets per block: the LDS share of the geometry tensors sets the default,
-    // raised to the largest DOF valence (a row's tets must fit one block);
-    // codes are 16-bit when tet*E + entry fits
-    int64_t cap = std::max<int64_t>(1, std::min<int64_t>(kBlockMaxTets, kBlockKBytes / (8 * S.k_per_tet)));
-    cap = std::max(cap, max_valence);
-    if (cap > kBlockHardMaxTets || cap * 8 * S.k_per_tet > kBlockMaxKBytes) return ANIFEM_ERR_DIMS;
-    // codes: local_tet << ebits | entry, entry == E marking a Dirichlet
-    // diagonal entry; load-vector codes local_tet << fbits | i
-    S.ebits = 1;
-    while ((1 << S.ebits) <= E) ++S.ebits;
-    S.fbits = 0;
-    while ((1 << S.fbits) < nf) ++S.fbits;
-    S.code16 = (cap << S.ebits) <= 65536;
-    if (!S.code16 && (65536 >> S.ebits) >= std::max<int64_t>(64, max_valence)) {
-        cap = 65536 >> S.ebits;
-        S.code16 = true;
-    }
-    if ((cap << S.fbits) > 65536) return ANIFEM_ERR_DIMS;
-    S.block_tet_cap = (int)cap;
 
     // greedy cut: extend the current block while every budget holds
     S.blk_row0.assign(1, 0);
-    S.blk_tet0.assign(1, 0);
-    S.blk_c0.assign(1, 0);
-    S.blk_g0.assign(1, 0);
+    S.blk_p0.assign(1, 0);
+    S.blk_v0.assign(1, 0);
     {
-        std::vector<int32_t> stamp(S.n_tets, -1);
-        int32_t cur = 0;
-        int64_t slots = 0, nc = 0, ng = 0, nt = 0;
+        int64_t slots = 0, nv = 0, np = 0;
         auto close = [&](int64_t r) {
             S.blk_row0.push_back((int32_t)r);
-            S.blk_tet0.push_back(S.blk_tet0.back() + nt);
-            S.blk_c0.push_back(S.blk_c0.back() + nc);
-            S.blk_g0.push_back(S.blk_g0.back() + ng);
-            ++cur;
-            slots = nc = ng = nt = 0;
+            S.blk_p0.push_back(S.blk_p0.back() + np);
+            S.blk_v0.push_back(S.blk_v0.back() + nv);
+            slots = nv = np = 0;
         };
         for (int64_t r = 0; r < ND; ++r) {
-            const int64_t rs = S.row_ptr[r + 1] - S.row_ptr[r], rc = row_nc[r];
-            const int64_t rg = S.is_dir[r] ? 0 : aptr[r + 1] - aptr[r];
-            if (rs > kBlockMaxSlots || rc > kBlockMaxContrib || rg > kBlockMaxContrib) return ANIFEM_ERR_DIMS;
-            int64_t fresh = 0;
-            for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) fresh += stamp[adj[p]] != cur;
-            if (slots && (slots + rs > kBlockMaxSlots || nc + rc > kBlockMaxContrib || ng + rg > kBlockMaxContrib ||
-                          nt + fresh > S.block_tet_cap)) {
+            const int64_t rs = S.row_ptr[r + 1] - S.row_ptr[r], rv = row_nv[r], rp = aptr[r + 1] - aptr[r];
+            if (rs > kBlockMaxSlots || rv > kBlockMaxValues || rp > kBlockMaxPairs) return ANIFEM_ERR_DIMS;
+            if (slots && (slots + rs > kBlockMaxSlots || nv + rv > kBlockMaxValues || np + rp > kBlockMaxPairs))
                 close(r);
-                fresh = aptr[r + 1] - aptr[r];
-            }
-            for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) stamp[adj[p]] = cur;
             slots += rs;
-            nc += rc;
-            ng += rg;
-            nt += fresh;
+            nv += rv;
+            np += rp;
         }
         if (slots || ND == 0) close(ND);
     }
@@ -102,81 +72,55 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     S.blk_s0.resize(S.n_blocks + 1);
     for (int64_t b = 0; b <= S.n_blocks; ++b) S.blk_s0[b] = S.row_ptr[S.blk_row0[b]];
 
-    // fill the lists, one block at a time
-    S.blk_tets.resize(S.blk_tet0.back());
-    S.codes.resize(S.blk_c0.back());
+    // fill the pair lists and value positions, one block at a time
+    const int64_t NP = S.blk_p0.back();
+    S.pair_tet.resize(NP);
+    S.pair_ib.resize(NP);
+    S.pair_pos.resize((size_t)NP * nf);
     S.sstart.resize(S.nnz);
-    S.fcodes.resize(S.blk_g0.back());
-    S.rstart.resize(ND);
-    std::atomic<int64_t> max_tets{0}, max_c{0}, max_g{0};
-    std::atomic<bool> mismatch{false};
+    S.pstart.resize(ND);
     host_parallel_for(S.n_blocks, 16, [&](int64_t b0, int64_t b1) {
-        std::vector<int32_t> pos, buf;
-        int64_t lt = 0, lc = 0, lg = 0;
+        std::vector<int32_t> cur;
         for (int64_t b = b0; b < b1; ++b) {
             const int32_t r0 = S.blk_row0[b], r1 = S.blk_row0[b + 1];
-            const int64_t ntb = S.blk_tet0[b + 1] - S.blk_tet0[b];
-            buf.assign(adj.begin() + aptr[r0], adj.begin() + aptr[r1]);
-            std::sort(buf.begin(), buf.end());
-            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
-            const int64_t k = (int64_t)buf.size();
-            if (k != ntb) {
-                mismatch = true;
-                continue;
-            }
-            int32_t* tl = S.blk_tets.data() + S.blk_tet0[b];
-            std::copy(buf.begin(), buf.end(), tl);
-            uint32_t* cd = S.codes.data() + S.blk_c0[b];
-            uint16_t* fc = S.fcodes.data() + S.blk_g0[b];
-            int64_t c = 0, g = 0;
+            int64_t p = S.blk_p0[b];
+            int64_t v = 0;  // value offset within the block
             for (int32_t r = r0; r < r1; ++r) {
                 const int64_t base = S.row_ptr[r], rs = S.row_ptr[r + 1] - base;
-                pos.assign(rs + 1, 0);
-                for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) {
-                    const int32_t* sl = &S.slot[(size_t)E * adj[p] + lidx[p]];
+                // the row's slots: entries counted, then laid out by ascending tet
+                cur.assign(rs + 1, 0);
+                for (int64_t q = aptr[r]; q < aptr[r + 1]; ++q) {
+                    const int32_t* sl = &S.slot[(size_t)E * adj[q] + lidx[q]];
                     for (int ia = 0; ia < nf; ++ia)
-                        if (sl[(size_t)nf * ia] >= 0) pos[sl[(size_t)nf * ia] - base + 1]++;
+                        if (sl[(size_t)nf * ia] >= 0) cur[sl[(size_t)nf * ia] - base + 1]++;
                 }
-                pos[0] = (int32_t)c;
-                for (int64_t u = 0; u < rs; ++u) pos[u + 1] += pos[u];
-                const int64_t row_end = pos[rs];
-                for (int64_t u = 0; u < rs; ++u) S.sstart[base + u] = (uint16_t)pos[u];
-                S.rstart[r] = (uint16_t)g;
+                cur[0] = (int32_t)v;
+                for (int64_t u = 0; u < rs; ++u) cur[u + 1] += cur[u];
+                const int64_t row_end = cur[rs];
+                for (int64_t u = 0; u < rs; ++u) S.sstart[base + u] = (uint16_t)cur[u];
+                S.pstart[r] = (uint16_t)(p - S.blk_p0[b]);
                 const bool rdir = S.is_dir[r] != 0;
-                for (int64_t p = aptr[r]; p < aptr[r + 1]; ++p) {
-                    const int32_t t = adj[p];
-                    const uint32_t tloc = (uint32_t)(std::lower_bound(tl, tl + k, t) - tl);
-                    const int ib = lidx[p];
+                for (int64_t q = aptr[r]; q < aptr[r + 1]; ++q, ++p) {
+                    const int32_t t = adj[q];
+                    const int ib = lidx[q];
+                    S.pair_tet[p] = t;
+                    S.pair_ib[p] = (uint8_t)(ib | (rdir ? kPairDirichlet : 0));
                     const int32_t* sl = &S.slot[(size_t)E * t + ib];
-                    const int32_t* td = &S.dofs[(size_t)nf * t];
+                    uint16_t* pos = &S.pair_pos[(size_t)nf * p];
                     for (int ia = 0; ia < nf; ++ia) {
                         const int32_t s = sl[(size_t)nf * ia];
-                        if (s < 0) continue;
-                        // a listed entry on a Dirichlet row or column is the
-                        // diagonal one (the pattern drops the others), which
-                        // applyDir sets to 1 (dc_on_dof.h:38-45)
-                        const bool dir = rdir || S.is_dir[td[ia]];
-                        cd[pos[s - base]++] = (tloc << S.ebits) | (uint32_t)(dir ? E : ib + nf * ia);
+                        pos[ia] = s >= 0 ? (uint16_t)cur[s - base]++ : kPosDropped;
                     }
-                    if (!rdir) fc[g++] = (uint16_t)((tloc << S.fbits) | (uint32_t)ib);
                 }
-                c = row_end;
+                v = row_end;
             }
-            lt = std::max(lt, ntb);
-            lc = std::max(lc, c);
-            lg = std::max(lg, g);
         }
-        int64_t v;
-        while ((v = max_tets.load()) < lt && !max_tets.compare_exchange_weak(v, lt)) {}
-        while ((v = max_c.load()) < lc && !max_c.compare_exchange_weak(v, lc)) {}
-        while ((v = max_g.load()) < lg && !max_g.compare_exchange_weak(v, lg)) {}
     });
-    if (mismatch.load()) return ANIFEM_ERR_ARG;
-    S.max_block_tets = max_tets.load();
-    S.max_block_contrib = max_c.load();
-    S.max_block_fcontrib = max_g.load();
-    for (int64_t b = 0; b < S.n_blocks; ++b)
-        S.max_block_slots = std::max(S.max_block_slots, S.row_ptr[S.blk_row0[b + 1]] - S.row_ptr[S.blk_row0[b]]);
+    for (int64_t b = 0; b < S.n_blocks; ++b) {
+        S.max_block_values = std::max(S.max_block_values, S.blk_v0[b + 1] - S.blk_v0[b]);
+        S.max_block_pairs = std::max(S.max_block_pairs, S.blk_p0[b + 1] - S.blk_p0[b]);
+        S.max_block_slots = std::max(S.max_block_slots, S.blk_s0[b + 1] - S.blk_s0[b]);
+    }
     return ANIFEM_OK;
 }
 

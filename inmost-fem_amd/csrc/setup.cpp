@@ -312,15 +312,6 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
                 }
         }
     });
-    // geometry-tensor values per tet: dA*dB*ncomp^2 per point carried (all
-    // points when D is given per point) plus the load vector's
-    {
-        const double *pp, *ww;
-        const int qm = prob.tensor_storage == ANIFEM_COEF_PER_POINT ? quadrature(prob.order, &pp, &ww) : 1;
-        const int qf = prob.rhs_tensor_storage == ANIFEM_COEF_PER_POINT ? quadrature(prob.rhs_order, &pp, &ww) : 1;
-        if (qm < 0 || qf < 0) return ANIFEM_ERR_ORDER;
-        S.k_per_tet = ref_dim(prob.trial.op) * ref_dim(prob.test.op) * S.ncomp * S.ncomp * qm + S.ncomp * qf;
-    }
     return build_blocks(S, aptr, adj);
 }
 

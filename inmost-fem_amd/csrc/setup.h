@@ -13,12 +13,11 @@
 namespace anifem {
 
 // Row-block budgets of the owner-computes assembly (blocks.cpp).
-constexpr int kBlockMaxSlots = 2048;         // CSR slots per block
-constexpr int kBlockMaxContrib = 2048;       // (tet, entry) contributions per block
-constexpr int kBlockMaxTets = 128;           // default tets per block
-constexpr int kBlockHardMaxTets = 1024;      // largest DOF valence supported
-constexpr int kBlockKBytes = 24 * 1024;      // default FP64 geometry-tensor bytes per block
-constexpr int kBlockMaxKBytes = 96 * 1024;   // ... when raised to the DOF valence
+constexpr int kBlockMaxSlots = 2048;    // CSR slots per block
+constexpr int kBlockMaxValues = 2048;   // element entries per block
+constexpr int kBlockMaxPairs = 256;     // (row, incident tet) pairs per block
+constexpr uint16_t kPosDropped = 0xFFFF;  // pair_pos of an entry the pattern drops
+constexpr uint8_t kPairDirichlet = 0x80;  // pair_ib flag: the pair's row is Dirichlet
 
 struct AssemblySetup {
     int fem = 0, ncomp = 1, nf = 0, E = 0;
@@ -31,24 +30,18 @@ struct AssemblySetup {
     std::vector<int32_t> col_idx;     // nnz
     std::vector<int32_t> slot;        // caller's order: n_tets x E (entry ib + nf*ia), -1 dropped
     int64_t n_degenerate = 0;         // tets with a zero or non-finite Jacobian determinant
-    int k_per_tet = 1;                // FP64 geometry-tensor values per tet (matrix + load vector)
     // row blocks: consecutive CSR rows [blk_row0[b], blk_row0[b+1])
-    int block_tet_cap = kBlockMaxTets;
-    bool code16 = true;               // codes fit 16 bits on the device
-    int ebits = 1, fbits = 0;         // entry / load-vector index bits of the codes
     int64_t n_blocks = 0;
     std::vector<int32_t> blk_row0;    // n_blocks + 1
     std::vector<int64_t> blk_s0;      // n_blocks + 1: first CSR slot of each block (row_ptr[blk_row0])
-    std::vector<int64_t> blk_tet0;    // n_blocks + 1 offsets into blk_tets
-    std::vector<int32_t> blk_tets;    // distinct tets of each block, ascending
-    std::vector<int64_t> blk_c0;      // n_blocks + 1 offsets into codes
-    std::vector<uint32_t> codes;      // per block: local_tet << ebits | entry (E: Dirichlet diagonal, value 1),
-                                      // grouped by slot, ascending tet
-    std::vector<uint16_t> sstart;     // nnz: first code of each slot, relative to its block
-    std::vector<int64_t> blk_g0;      // n_blocks + 1 offsets into fcodes
-    std::vector<uint16_t> fcodes;     // per block: local_tet << fbits | i, grouped by row, ascending tet
-    std::vector<uint16_t> rstart;     // ndof: first load-vector code of each row, relative to its block
-    int64_t max_block_tets = 0, max_block_contrib = 0, max_block_fcontrib = 0, max_block_slots = 0;
+    std::vector<int64_t> blk_p0;      // n_blocks + 1 offsets into the pair lists
+    std::vector<int64_t> blk_v0;      // n_blocks + 1 offsets of each block's value list
+    std::vector<int32_t> pair_tet;    // per pair: tet id
+    std::vector<uint8_t> pair_ib;     // per pair: local row index | kPairDirichlet
+    std::vector<uint16_t> pair_pos;   // per pair, nf: value-list position of entry (ib, ia), or kPosDropped
+    std::vector<uint16_t> sstart;     // nnz: first value of each slot, relative to its block
+    std::vector<uint16_t> pstart;     // ndof: first pair of each row, relative to its block
+    int64_t max_block_values = 0, max_block_pairs = 0, max_block_slots = 0;
 };
 
 int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S);

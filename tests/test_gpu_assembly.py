@@ -150,3 +150,31 @@ def test_nonfinite_status():
     _, _, st = Asm.assemble(D=dev(kappa))
     # Assemble's -1 when a local matrix holds NaN (assembler.inl:419-424)
     assert st.code == af.ERR_NONFINITE and st.n_nonfinite > 0
+
+
+@pytest.mark.parametrize("fem", [2, 3])
+def test_large_assembly_invariants(fem):
+    """At 32^3 (196,608 tets, tens of thousands of row blocks): without
+    Dirichlet every row of the stiffness matrix sums to zero, the matrix is
+    symmetric, the load vector for f = 1 sums to |unit cube| = 1, and a second
+    run is bitwise identical."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(fem)
+    coords, tets = af.kuhn_cube(32, 0.2, 6)
+    kappa = 10 ** rng.uniform(-1, 1, (1, len(tets)))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem), order=2, tensor_type=af.TENSOR_SCALAR,
+                       tensor_storage=af.COEF_PER_TET, rhs_order=2, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=0)
+    assert Asm.n_blocks > 1000
+    v1, r1, st = Asm.assemble(D=dev(kappa), F=dev(np.ones(1)))
+    assert st.code == 0
+    v2, r2, _ = Asm.assemble(D=dev(kappa), F=dev(np.ones(1)))
+    v1, r1 = v1.cpu().numpy(), r1.cpu().numpy()
+    assert np.array_equal(v1, v2.cpu().numpy()) and np.array_equal(r1, r2.cpu().numpy())
+    s = Asm.setup()
+    M = sp.csr_matrix((v1, s["col_idx"], s["row_ptr"]), shape=(Asm.ndof, Asm.ndof))
+    rowmax = abs(M).max(axis=1).toarray().ravel()
+    assert np.max(np.abs(M @ np.ones(Asm.ndof)) / rowmax) < 1e-12
+    D = (M - M.T).tocsr()
+    assert np.max(np.abs(D.data) if D.nnz else [0.0]) <= 1e-12 * rowmax.max()
+    assert abs(r1.sum() - 1.0) < 1e-12

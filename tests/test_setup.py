@@ -115,49 +115,59 @@ def test_mesh_generator_properties():
     np.testing.assert_array_equal(coords, c2)  # seeded reproducibility
 
 
-def _emulate_blocks(A, s, B, F=None):
-    """The row-block gather replayed on the host from the block lists: each
-    slot value is the sum of its listed element entries in list order."""
-    nf = A.shape[1] if F is None else F.shape[1]
-    E = A.shape[1]
+def _emulate_blocks(A, F, s, B, nf):
+    """The row-block kernel replayed on the host from the block lists: each
+    pair stores its element-matrix row and load-vector entry at the listed
+    positions, then each slot (row) adds its entries in list order."""
     rp = s["row_ptr"]
     ndof, nnz = len(rp) - 1, rp[-1]
     val, rhs = np.zeros(nnz), np.zeros(ndof)
     nb = len(B["blk_row0"]) - 1
     for b in range(nb):
         r0, r1 = B["blk_row0"][b], B["blk_row0"][b + 1]
-        tets = B["blk_tets"][B["blk_tet0"][b]:B["blk_tet0"][b + 1]]
-        c0, c1 = B["blk_c0"][b], B["blk_c0"][b + 1]
+        p0, p1 = B["blk_p0"][b], B["blk_p0"][b + 1]
+        nv = B["blk_v0"][b + 1] - B["blk_v0"][b]
+        vals = np.full(nv, np.nan)
+        vtet = np.full(nv, -1)
+        fvals = np.zeros(p1 - p0)
+        for p in range(p0, p1):
+            t, ibf = B["pair_tet"][p], int(B["pair_ib"][p])
+            ib, dirrow = ibf & 63, bool(ibf & 0x80)
+            for ia in range(nf):
+                ps = int(B["pair_pos"][p, ia])
+                if ps == 0xFFFF:
+                    continue
+                assert np.isnan(vals[ps]), "each list position is written once"
+                if dirrow:
+                    assert ia == ib, "a kept entry of a Dirichlet row is its diagonal"
+                vals[ps] = 1.0 if dirrow else A[t, ib + nf * ia]
+                vtet[ps] = t
+            fvals[p - p0] = 0.0 if dirrow else F[t, ib]
+        assert not np.isnan(vals).any(), "every list position is written"
         s0, s1 = rp[r0], rp[r1]
         for u in range(s0, s1):
-            lo = c0 + B["sstart"][u]
-            hi = c0 + B["sstart"][u + 1] if u + 1 < s1 else c1
+            lo = B["sstart"][u]
+            hi = B["sstart"][u + 1] if u + 1 < s1 else nv
+            assert np.all(np.diff(vtet[lo:hi]) > 0), "a slot's entries in ascending tet order"
             acc = 0.0
-            last = -1
-            for code in B["codes"][lo:hi]:
-                t = tets[code >> B["e_bits"]]
-                e = code & ((1 << B["e_bits"]) - 1)
-                assert t > last, "contributions of a slot in ascending tet order"
-                last = t
-                acc += 1.0 if e == E else A[t, e]
+            for x in vals[lo:hi]:
+                acc += x
             val[u] = acc
-        if F is not None:
-            g0, g1 = B["blk_g0"][b], B["blk_g0"][b + 1]
-            for r in range(r0, r1):
-                lo = g0 + B["rstart"][r]
-                hi = g0 + B["rstart"][r + 1] if r + 1 < r1 else g1
-                acc = 0.0
-                for code in B["fcodes"][lo:hi]:
-                    acc += F[tets[code >> B["f_bits"]], code & ((1 << B["f_bits"]) - 1)]
-                rhs[r] = acc
+        for r in range(r0, r1):
+            lo = B["pstart"][r]
+            hi = B["pstart"][r + 1] if r + 1 < r1 else p1 - p0
+            acc = 0.0
+            for x in fvals[lo:hi]:
+                acc += x
+            rhs[r] = acc
     return val, rhs
 
 
 @pytest.mark.parametrize("fem,ncomp", [(2, 1), (3, 1), (4, 1), (3, 3)])
 def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
-    """The block lists name every (tet, entry) contribution of every slot once,
-    in ascending tet order: replaying them gives the oracle's scatter bit for
-    bit (random element matrices with element-level Dirichlet)."""
+    """The block lists place every kept (tet, entry) contribution of every slot
+    once, in ascending tet order: replaying them gives the oracle's scatter bit
+    for bit (random element matrices with element-level Dirichlet)."""
     coords, tets = _shuffled_mesh(3, 11) if fem != 3 else af.kuhn_cube(4, 0.2, 2)
     A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
     s = A.setup()
@@ -169,15 +179,14 @@ def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
     Ae, Fe = T.apply_dir_elementwise(Ae, Fe, s["dofs"], s["dirichlet"])
     rc, val_o, rhs_o = T.oracle_scatter(len(tets), nf, s["dofs"], s["slot"], Ae, Fe, A.nnz, A.ndof)
     assert rc == 0
-    val, rhs = _emulate_blocks(Ae, s, B, Fe)
+    val, rhs = _emulate_blocks(Ae, Fe, s, B, nf)
     np.testing.assert_array_equal(val, val_o)
     np.testing.assert_array_equal(rhs, rhs_o)
-    # budgets: rows contiguous and covering, slot / tet / code limits
+    # budgets: rows contiguous and covering, slot / value / pair limits
     r0 = B["blk_row0"]
     assert r0[0] == 0 and r0[-1] == A.ndof and np.all(np.diff(r0) > 0)
     assert np.all(np.diff(s["row_ptr"][r0]) <= 2048)
-    assert np.all(np.diff(B["blk_tet0"]) <= B["block_tet_cap"])
-    assert np.all(np.diff(B["blk_c0"]) <= 2048)
-    assert B["code_bytes"] == (2 if B["block_tet_cap"] << B["e_bits"] <= 65536 else 4)
-    assert (1 << B["e_bits"]) > E and (1 << B["f_bits"]) >= nf
-    assert len(B["codes"]) == int((s["slot"] >= 0).sum())
+    assert np.all(np.diff(B["blk_v0"]) <= 2048)
+    assert np.all(np.diff(B["blk_p0"]) <= 256)
+    assert B["blk_v0"][-1] == int((s["slot"] >= 0).sum())
+    assert B["blk_p0"][-1] == len(tets) * nf
