@@ -6,15 +6,20 @@ Kuhn tets = 1,572,864 tets, interior nodes jittered by +-0.2h (seed 1), P2
 Lagrange elements (2,146,689 DOFs, 60,859,905 nnz before the Dirichlet drop),
 per-tet SPD tensor D = R diag(1, 10^a, 10^b) R^T (a, b ~ U[-2, 2], R Haar),
 4-point quadrature, unit load, homogeneous Dirichlet on the boundary.  A step
-zeroes the FP64 CSR values and RHS and assembles every element matrix and
-load vector into them (FP64 geometry, FP64 contraction, FP64 accumulation).
-Inputs are resident in HBM before timing.
+forms every element matrix and load vector and writes each FP64 CSR value and
+RHS entry once (FP64 geometry, FP64 contraction, FP64 accumulation): a
+geometry launch and a row-block launch.  Inputs are resident in HBM before
+timing.
 
 value  = element matrices assembled per second (whole job; 1 GPU).
 Extra fields: nnz/s, the FP32-contraction mode, the element-matrix-only
 kernels (FP64 / FP32 output), Jacobi-PCG on the 16^3 P1 Poisson system,
-the roofline of the dominant kernel (fused element+scatter) and the CPU
-baseline (reference element core + sequential scatter on the host).
+the roofline of the assembly pass and the CPU baseline (reference element
+core + sequential scatter on the host, whole workload, one thread).
+
+--copy-calibration MIB runs one plain device copy of MIB MiB (torch copy_,
+outside the timed region) so that a counter run of the same command can
+scale FETCH_SIZE / WRITE_SIZE by what they read for known bytes.
 """
 import argparse
 import ctypes
@@ -63,7 +68,7 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
     ref = T.ref_lib()
     kind = "reference" if ref is not None else "port"
     n_all = len(tets_reordered)
-    n = min(n_all, 200000)
+    n = n_all
     xyz = coords[tets_reordered[:n]].reshape(n, 12)
     Df = np.ascontiguousarray(Dfull[:n].transpose(0, 2, 1).reshape(n, 9))
     t0 = time.perf_counter()
@@ -90,7 +95,7 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
     t2 = time.perf_counter()
     rate = n / (t2 - t0)
     return {"value": rate, "unit": "element matrices/s", "cores": 1, "kind": kind,
-            "sample": f"first {n} of {n_all} tets: P2 GRAD-GRAD element matrices with per-tet SPD D "
+            "sample": f"all {n} of {n_all} tets: P2 GRAD-GRAD element matrices with per-tet SPD D "
                       f"({'reference fem3Dtet compiled from the AniFem++ sources' if ref else 'oracle restatement'}, "
                       f"{t1 - t0:.2f}s) + oracle sequential scatter into the same CSR ({t2 - t1:.2f}s)"}
 
@@ -103,6 +108,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="sub-cubes per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
@@ -146,6 +152,15 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
+    if args.copy_calibration:
+        # known-size traffic: a DMA-style copy (copyBuffer) and an elementwise
+        # kernel that reads and writes the same bytes
+        src = torch.ones(args.copy_calibration * 2 ** 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     _, _, st = Asm.assemble(D=Dp, F=f, val=val, rhs=rhs, status=True, stream=stream)
     assert st.code == 0 and st.n_degenerate == 0, st.code
     sec64 = timed(af.PREC_FP64, args.steps, args.warmup)

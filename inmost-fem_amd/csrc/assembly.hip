@@ -29,6 +29,7 @@ struct anifem_assembly {
     double* d_coords = nullptr;       // n_nodes x 3
     int32_t* d_tetnodes = nullptr;    // 4 x n_tets SoA
     int32_t* d_blk_row0 = nullptr;
+    anifem::BlockMeta* d_blk_meta = nullptr;
     int64_t* d_blk_s0 = nullptr;
     int64_t* d_blk_p0 = nullptr;
     int64_t* d_blk_v0 = nullptr;
@@ -110,12 +111,14 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
     if (S.n_blocks == 0) return ANIFEM_OK;
+    // 8 equal XCD ranges of the processing order (padding workgroups exit)
+    const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
     if (staged)
-        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, true>), dim3((unsigned)S.n_blocks), dim3(kBlock),
-                           lds, s, a, (const T*)Kg);
+        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, true>), dim3(grid), dim3(kBlock), lds, s, a,
+                           (const T*)Kg);
     else
-        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, false>), dim3((unsigned)S.n_blocks), dim3(kBlock),
-                           lds, s, a, (const T*)Kg);
+        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, false>), dim3(grid), dim3(kBlock), lds, s, a,
+                           (const T*)Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -163,6 +166,7 @@ int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh, 
     if ((rc = upload(&ctx->d_coords, mesh->coords, 3 * (size_t)S.n_nodes, s))) return rc;
     if ((rc = upload(&ctx->d_tetnodes, tn.data(), tn.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_row0, S.blk_row0.data(), S.blk_row0.size(), s))) return rc;
+    if ((rc = upload(&ctx->d_blk_meta, S.blk_meta.data(), S.blk_meta.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_s0, S.blk_s0.data(), S.blk_s0.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_p0, S.blk_p0.data(), S.blk_p0.size(), s))) return rc;
     if ((rc = upload(&ctx->d_blk_v0, S.blk_v0.data(), S.blk_v0.size(), s))) return rc;
@@ -200,7 +204,7 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (!ctx->host_only) {
-        void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_s0,   ctx->d_blk_p0,
+        void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_meta, ctx->d_blk_s0, ctx->d_blk_p0,
                         ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf};
         for (void* b : bufs)
@@ -325,6 +329,7 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     a.has_rhs = do_rhs ? 1 : 0;
     a.nf = S.nf;
     a.n_blocks = S.n_blocks;
+    a.meta = ctx->d_blk_meta;
     a.blk_row0 = ctx->d_blk_row0;
     a.blk_s0 = ctx->d_blk_s0;
     a.blk_p0 = ctx->d_blk_p0;

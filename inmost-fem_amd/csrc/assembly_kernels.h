@@ -31,6 +31,7 @@ namespace anifem {
 
 static_assert(kBlockMaxPairs <= kBlock, "one pair per lane");
 
+
 struct BlockArgs {
     ElemArgs m;   // matrix pair (geometry source, tensor, reference tensor)
     ElemArgs f;   // load-vector pair IDEN P0 x IDEN test (when has_rhs)
@@ -38,6 +39,7 @@ struct BlockArgs {
     int nf;       // local DOFs per tet
     int rowlen;   // values per tet of the geometry rows
     int64_t n_blocks;
+    const BlockMeta* meta;      // per block, in processing order
     const int32_t* blk_row0;
     const int64_t* blk_s0;
     const int64_t* blk_p0;
@@ -166,20 +168,47 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
     uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
 
-    const int64_t b = blockIdx.x;
-    const int32_t r0 = a.blk_row0[b];
-    const int nr = a.blk_row0[b + 1] - r0;
-    const int64_t s0 = a.blk_s0[b];
-    const int ns = (int)(a.blk_s0[b + 1] - s0);
-    const int64_t p0 = a.blk_p0[b];
-    const int np = (int)(a.blk_p0[b + 1] - p0);
-    const int nv = (int)(a.blk_v0[b + 1] - a.blk_v0[b]);
+    // workgroup w runs on XCD w % 8: each XCD takes a contiguous range of the
+    // processing order, so blocks sharing tets meet in one L2
+    const int64_t per_xcd = (a.n_blocks + 7) / 8;
+    const int64_t k = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (k >= a.n_blocks) return;  // uniform over the workgroup
+    const BlockMeta M = a.meta[k];
+    const int32_t r0 = M.r0;
+    const int nr = M.nr;
+    const int64_t s0 = M.s0;
+    const int ns = M.ns;
+    const int64_t p0 = M.p0;
+    const int np = M.np;
+    const int nv = M.nv;
 
-    // the pair of this lane: requested first so that its loads overlap the staging
+    // the pair of this lane and its tet's geometry row, requested before the
+    // staging so that their loads overlap it
     const bool has_pair = (int)threadIdx.x < np;
     const int64_t gp = p0 + (has_pair ? threadIdx.x : 0);
     const int32_t tet = has_pair ? a.pair_tet[gp] : 0;
     const int ibf = has_pair ? a.pair_ib[gp] : 0;
+    const int ib = ibf & 63;
+    const int cb = ib / nfB0;
+    const T* Kt = Kg + (int64_t)tet * a.rowlen;
+    const bool qc = len == KB;  // one carried point: K blocks held in registers
+    T kreg[NCA][KB];
+    if (qc) {
+#pragma unroll
+        for (int ca = 0; ca < NCA; ++ca)
+#pragma unroll
+            for (int j = 0; j < KB; ++j) kreg[ca][j] = has_pair ? Kt[(cb * NCA + ca) * KB + j] : T(0);
+    }
+    // value positions as 16-bit pairs (nf is even), held in registers for
+    // scalar spaces (nf <= 2*PW)
+    constexpr int PW = 10;
+    const bool posr = NCA == 1 && nf <= 2 * PW;
+    uint32_t posw[PW];
+    {
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(a.pair_pos + gp * nf);
+#pragma unroll
+        for (int w = 0; w < PW; ++w) posw[w] = (has_pair && posr && 2 * w < nf) ? pw[w] : 0u;
+    }
 
     const T* Rg;
     const T* RFg;
@@ -214,41 +243,38 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     // phase A: the pair's element-matrix row and load-vector entry
     int nonfinite = 0;
     if (has_pair) {
-        const int ib = ibf & 63;
         const bool dir = (ibf & kPairDirichlet) != 0;
-        const int ib0 = ib % nfB0, cb = ib / nfB0;
-        const T* Kt = Kg + (int64_t)tet * a.rowlen;
+        const int ib0 = ib % nfB0;
         const uint16_t* pos = a.pair_pos + gp * nf;
+        auto entry = [&](int ps, const T* Kb, const T* kr, int ia0) {
+            const T* Rr = R + (ib0 + nfB0 * ia0) * rstride;
+            T acc = 0;
+            if (qc) {
 #pragma unroll
-        for (int ca = 0; ca < NCA; ++ca) {
-            const T* Kb = Kt + (cb * NCA + ca) * len;
-            const uint16_t* pc = pos + ca * nfA0;
-            if (len == KB) {
-                T k[KB];
-#pragma unroll
-                for (int j = 0; j < KB; ++j) k[j] = Kb[j];
-                for (int ia0 = 0; ia0 < nfA0; ++ia0) {
-                    const int ps = pc[ia0];
-                    if (ps == kPosDropped) continue;
-                    const T* Rr = R + (ib0 + nfB0 * ia0) * rstride;
-                    T acc = 0;
-#pragma unroll
-                    for (int j = 0; j < KB; ++j) acc += k[j] * Rr[j];
-                    // a kept entry of a Dirichlet row is its diagonal: applyDir sets it to 1
-                    const double v = dir ? 1.0 : (double)acc;
-                    nonfinite += isfinite(v) ? 0 : 1;
-                    vals[ps] = v;
-                }
+                for (int j = 0; j < KB; ++j) acc += kr[j] * Rr[j];
             } else {
+                for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
+            }
+            // a kept entry of a Dirichlet row is its diagonal: applyDir sets it to 1
+            const double v = dir ? 1.0 : (double)acc;
+            nonfinite += isfinite(v) ? 0 : 1;
+            vals[ps] = v;
+        };
+        if (posr) {
+            // scalar space: entries unrolled, positions from registers
+#pragma unroll
+            for (int ia0 = 0; ia0 < 2 * PW; ++ia0) {
+                if (ia0 >= nfA0) break;
+                const uint32_t w = posw[ia0 >> 1];
+                const int ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                if (ps != kPosDropped) entry(ps, Kt + cb * len, kreg[0], ia0);
+            }
+        } else {
+#pragma unroll
+            for (int ca = 0; ca < NCA; ++ca) {
                 for (int ia0 = 0; ia0 < nfA0; ++ia0) {
-                    const int ps = pc[ia0];
-                    if (ps == kPosDropped) continue;
-                    const T* Rr = R + (ib0 + nfB0 * ia0) * rstride;
-                    T acc = 0;
-                    for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
-                    const double v = dir ? 1.0 : (double)acc;
-                    nonfinite += isfinite(v) ? 0 : 1;
-                    vals[ps] = v;
+                    const int ps = pos[ca * nfA0 + ia0];
+                    if (ps != kPosDropped) entry(ps, Kt + (cb * NCA + ca) * len, kreg[ca], ia0);
                 }
             }
         }

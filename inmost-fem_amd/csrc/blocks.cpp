@@ -116,6 +116,35 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
             }
         }
     });
+    // processing order: by the mean id of the blocks' tets, so that blocks
+    // sharing tets (node rows and the edge / face rows around them, far apart
+    // in row order) run close in time and find the tets' geometry rows in L2
+    {
+        std::vector<double> key(S.n_blocks);
+        for (int64_t b = 0; b < S.n_blocks; ++b) {
+            double sum = 0;
+            for (int64_t p = S.blk_p0[b]; p < S.blk_p0[b + 1]; ++p) sum += S.pair_tet[p];
+            const int64_t np = S.blk_p0[b + 1] - S.blk_p0[b];
+            key[b] = np ? sum / (double)np : 0.0;
+        }
+        S.blk_order.resize(S.n_blocks);
+        for (int64_t b = 0; b < S.n_blocks; ++b) S.blk_order[b] = (int32_t)b;
+        std::stable_sort(S.blk_order.begin(), S.blk_order.end(),
+                         [&](int32_t x, int32_t y) { return key[x] < key[y]; });
+        S.blk_meta.resize(S.n_blocks);
+        for (int64_t k = 0; k < S.n_blocks; ++k) {
+            const int64_t b = S.blk_order[k];
+            BlockMeta& M = S.blk_meta[k];
+            M.s0 = S.blk_s0[b];
+            M.p0 = S.blk_p0[b];
+            M.r0 = S.blk_row0[b];
+            M.nr = (uint16_t)(S.blk_row0[b + 1] - S.blk_row0[b]);
+            M.ns = (uint16_t)(S.blk_s0[b + 1] - S.blk_s0[b]);
+            M.np = (uint16_t)(S.blk_p0[b + 1] - S.blk_p0[b]);
+            M.nv = (uint16_t)(S.blk_v0[b + 1] - S.blk_v0[b]);
+            M.pad = 0;
+        }
+    }
     for (int64_t b = 0; b < S.n_blocks; ++b) {
         S.max_block_values = std::max(S.max_block_values, S.blk_v0[b + 1] - S.blk_v0[b]);
         S.max_block_pairs = std::max(S.max_block_pairs, S.blk_p0[b + 1] - S.blk_p0[b]);

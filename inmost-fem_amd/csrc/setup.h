@@ -19,6 +19,17 @@ constexpr int kBlockMaxPairs = 256;     // (row, incident tet) pairs per block
 constexpr uint16_t kPosDropped = 0xFFFF;  // pair_pos of an entry the pattern drops
 constexpr uint8_t kPairDirichlet = 0x80;  // pair_ib flag: the pair's row is Dirichlet
 
+// Bounds of one row block, stored in processing order (one 32-byte record).
+struct BlockMeta {
+    int64_t s0;       // first CSR slot
+    int64_t p0;       // first pair
+    int32_t r0;       // first row
+    uint16_t nr, ns;  // rows, slots
+    uint16_t np, nv;  // pairs, values
+    uint32_t pad;
+};
+static_assert(sizeof(BlockMeta) == 32, "block metadata is one 32-byte record");
+
 struct AssemblySetup {
     int fem = 0, ncomp = 1, nf = 0, E = 0;
     int64_t n_nodes = 0, n_tets = 0, n_edges = 0, n_faces = 0;
@@ -33,6 +44,8 @@ struct AssemblySetup {
     // row blocks: consecutive CSR rows [blk_row0[b], blk_row0[b+1])
     int64_t n_blocks = 0;
     std::vector<int32_t> blk_row0;    // n_blocks + 1
+    std::vector<int32_t> blk_order;   // n_blocks: processing order of the blocks
+    std::vector<BlockMeta> blk_meta;  // n_blocks: bounds in processing order
     std::vector<int64_t> blk_s0;      // n_blocks + 1: first CSR slot of each block (row_ptr[blk_row0])
     std::vector<int64_t> blk_p0;      // n_blocks + 1 offsets into the pair lists
     std::vector<int64_t> blk_v0;      // n_blocks + 1 offsets of each block's value list
