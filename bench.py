@@ -100,6 +100,19 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
                       f"{t1 - t0:.2f}s) + oracle sequential scatter into the same CSR ({t2 - t1:.2f}s)"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "assembly_p2_aniso_64cube_fp64_traffic.txt")
+
+
+def read_traffic():
+    """HBM bytes per assembly pass from the committed FETCH_SIZE / WRITE_SIZE
+    summary (copy-kernel-calibrated), or None when it is absent."""
+    try:
+        kv = dict(line.split(": ", 1) for line in open(TRAFFIC_FILE).read().splitlines() if ": " in line)
+        return int(kv["traffic_bytes_per_pass"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,13 +155,17 @@ def main():
     def step(prec):
         Asm.assemble(D=Dp, F=f, precision=prec, val=val, rhs=rhs, status=False, stream=stream)
 
-    def timed(prec, steps, warmup):
+    def timed(prec, steps, warmup, events=None):
         for _ in range(warmup):
             step(prec)
         torch.cuda.synchronize()
         t = time.perf_counter()
+        if events:
+            events[0].record(stream)
         for _ in range(steps):
             step(prec)
+        if events:
+            events[1].record(stream)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
@@ -163,7 +180,10 @@ def main():
         del src, dst
     _, _, st = Asm.assemble(D=Dp, F=f, val=val, rhs=rhs, status=True, stream=stream)
     assert st.code == 0 and st.n_degenerate == 0, st.code
-    sec64 = timed(af.PREC_FP64, args.steps, args.warmup)
+    # HIP events on the launch stream bracket the timed passes
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    sec64 = timed(af.PREC_FP64, args.steps, args.warmup, ev)
+    ms_pass_events = ev[0].elapsed_time(ev[1]) / args.steps
     print(f"# fp64 assembly {sec64 * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
     # per-phase GPU times (HIP events on the launch stream)
@@ -183,7 +203,8 @@ def main():
     # value); priced against the two kernels of the pass
     per_tet = 96 + 48 + 4 * 100 + 4 * 10
     alg_bytes = n_tets * per_tet + 16 * Asm.nnz + 16 * Asm.ndof
-    achieved = alg_bytes / ((tm[0] + tm[1]) * 1e-3) / 1e9
+    achieved = alg_bytes / (ms_pass_events * 1e-3) / 1e9
+    traffic, traffic_src = read_traffic()
 
     out = {
         "metric": "element matrices/s (P2 anisotropic-diffusion assembly into FP64 CSR + RHS)",
@@ -204,10 +225,11 @@ def main():
         "nnz_per_s": Asm.nnz / sec64,
         "phase_ms": {"geometry": tm[0], "row_blocks": tm[1], "launches": nlaunch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_kernel<3,3,1,1,double,uint16_t>",
-                     "ms_per_launch": tm[0] + tm[1],
-                     "algorithmic_bytes_per_pass": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_kernel<3,3,1,1,double,true>",
+                     "ms_per_launch": ms_pass_events,
+                     "algorithmic_bytes_per_pass": alg_bytes,
+                     "traffic_source": traffic_src},
     }
 
     if not args.skip_extras:

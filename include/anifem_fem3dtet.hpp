@@ -1,0 +1,183 @@
+// anifem_fem3dtet.hpp -- C++ face of the MI355X library, mirroring the
+// reference's template API for the element evaluator and the assembler.
+//
+// The reference calls
+//     Ani::fem3Dtet<Ani::Operator<Ani::GRAD, Ani::FemFix<Ani::FEM_P2>>,
+//                   Ani::Operator<Ani::GRAD, Ani::FemFix<Ani::FEM_P2>>,
+//                   Ani::DfuncTraits<Ani::TENSOR_SYMMETRIC, false>>(XY0, XY1, XY2, XY3, Dfnc, A, order);
+// (anifem++/fem/operations/int_tet.h:17-20, int_tet.inl:92-122).  Here the
+// same tags select the operator pair and the tensor kind at compile time, and
+// the call evaluates a whole batch of tets on the GPU:
+//     anifem_gpu::fem3Dtet<anifem_gpu::Operator<anifem_gpu::GRAD, anifem_gpu::FemFix<anifem_gpu::FEM_P2>>,
+//                          anifem_gpu::Operator<anifem_gpu::GRAD, anifem_gpu::FemFix<anifem_gpu::FEM_P2>>,
+//                          anifem_gpu::DfuncTraits<anifem_gpu::TENSOR_SYMMETRIC, false>>(xyz, n, D, A, order);
+// The coefficient is a device view (per tet or per quadrature point) instead
+// of a host functor; a functor is evaluated by the caller at the points of
+// anifem_quadrature().  Errors come back as the ANIFEM_ERR_* codes of
+// anifem_gpu.h (the reference throws).
+#ifndef ANIFEM_FEM3DTET_HPP
+#define ANIFEM_FEM3DTET_HPP
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+#include "anifem_gpu.h"
+
+namespace anifem_gpu {
+
+// Finite elements and operators (numbering of operators.h:25-45)
+enum FiniteElement { FEM_P0 = ANIFEM_FEM_P0, FEM_P1 = ANIFEM_FEM_P1, FEM_P2 = ANIFEM_FEM_P2, FEM_P3 = ANIFEM_FEM_P3 };
+enum OperatorType { IDEN = ANIFEM_IDEN, GRAD = ANIFEM_GRAD };
+
+// FemFix<OP> / FemVec<DIM, OP> (operators.h:57-67)
+template <int OP>
+using FemFix = std::integral_constant<int, OP>;
+
+template <int DIM, int OP>
+struct FemVec {
+    using Dim = std::integral_constant<int, DIM>;
+    using Base = FemFix<OP>;
+};
+
+namespace detail {
+template <typename T>
+struct SpaceOf {
+    static constexpr int fem = T::value;
+    static constexpr int ncomp = 1;
+};
+template <int DIM, int OP>
+struct SpaceOf<FemVec<DIM, OP>> {
+    static constexpr int fem = OP;
+    static constexpr int ncomp = DIM;
+};
+}  // namespace detail
+
+// Operator<OPERATOR, FEMTYPE> (operators.h:50-51,127-155)
+template <int OPERATOR, typename FEMTYPE>
+struct Operator {
+    static_assert(OPERATOR == IDEN || OPERATOR == GRAD, "GPU operators: IDEN and GRAD");
+    static constexpr anifem_op c() {
+        return anifem_op{OPERATOR, detail::SpaceOf<FEMTYPE>::fem, detail::SpaceOf<FEMTYPE>::ncomp};
+    }
+};
+
+// Tensor kinds (diff_tensor.h:16-27) and DfuncTraits (diff_tensor.h:36-51)
+enum TensorType {
+    TENSOR_NULL = ANIFEM_TENSOR_NULL,
+    TENSOR_SCALAR = ANIFEM_TENSOR_SCALAR,
+    TENSOR_SYMMETRIC = ANIFEM_TENSOR_SYMMETRIC,
+    TENSOR_GENERAL = ANIFEM_TENSOR_GENERAL,
+    TENSOR_LAME = ANIFEM_TENSOR_LAME
+};
+enum TensorTypeSparsity { PerPoint = 0, PerTetra = -1, PerSelection = -2 };
+
+template <int TensorTypeSparse = PerPoint, bool isConstant = false>
+struct DfuncTraits {
+    using IsConstant = std::integral_constant<bool, isConstant>;
+    using TensorSparsity = std::integral_constant<int, TensorTypeSparse>;
+};
+
+// Device view of the coefficient: storage ANIFEM_COEF_CONST / PER_TET /
+// PER_POINT, layout as documented in anifem_gpu.h.
+struct TensorView {
+    int type;
+    int storage;
+    const double* data;
+};
+
+enum Precision { FP64 = ANIFEM_PREC_FP64, FP32 = ANIFEM_PREC_FP32 };
+
+// Checks a view against the traits: a fixed TensorType must match, and a
+// constant tensor must use constant storage.
+template <typename Traits>
+inline int check_view(const TensorView& D) {
+    if (Traits::TensorSparsity::value > 0 && D.type != Traits::TensorSparsity::value) return ANIFEM_ERR_ARG;
+    if (Traits::IsConstant::value && D.storage != ANIFEM_COEF_CONST) return ANIFEM_ERR_ARG;
+    return ANIFEM_OK;
+}
+
+// Batched fem3Dtet: element matrices of n_tets tets (xyz: device 12 x n SoA;
+// A_out: device, n matrices nfB x nfA column-major, double for FP64, float for
+// FP32); stream is a hipStream_t.
+template <typename OpA, typename OpB, typename FuncTraits = DfuncTraits<>>
+inline int fem3Dtet(const double* xyz, int64_t n_tets, const TensorView& D, void* A_out, int order = 5,
+                    Precision precision = FP64, void* stream = nullptr) {
+    const int rc = check_view<FuncTraits>(D);
+    if (rc) return rc;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dtet_batch(OpA::c(), OpB::c(), &t, order, n_tets, xyz, precision, A_out, stream);
+}
+
+// Nfa / Dim of an operator (poly_*.h, operators.h:127-155)
+template <typename Op>
+inline int nfa() {
+    return anifem_op_nfa(Op::c());
+}
+template <typename Op>
+inline int dim() {
+    return anifem_op_dim(Op::c());
+}
+
+// Whole-mesh assembler: PrepareProblem at construction (assembler.inl:194-275),
+// Assemble(matrix, rhs) per run (assembler.inl:314-488).  The test and trial
+// spaces are the Op tags' spaces; the load vector is fem3Dtet<IDEN P0, IDEN
+// test space> of the right-hand-side tensor (examples/tutorials/ex1.cpp:95).
+template <typename OpA, typename OpB>
+class Assembler {
+public:
+    Assembler(int64_t n_nodes, const double* coords, int64_t n_tets, const int32_t* tets, int order,
+              TensorView D_kind, int rhs_order, TensorView F_kind, int dirichlet,
+              const uint8_t* dirichlet_dof_mask = nullptr, void* stream = nullptr) {
+        anifem_mesh_desc mesh{n_nodes, n_tets, coords, tets};
+        anifem_problem_desc prob{};
+        prob.trial = OpA::c();
+        prob.test = OpB::c();
+        prob.order = order;
+        prob.tensor_type = D_kind.type;
+        prob.tensor_storage = D_kind.storage;
+        prob.rhs_order = rhs_order;
+        prob.rhs_tensor_type = F_kind.type;
+        prob.rhs_tensor_storage = F_kind.storage;
+        prob.dirichlet = dirichlet;
+        prob.dirichlet_dof_mask = dirichlet_dof_mask;
+        status_ = anifem_assembly_create(&ctx_, &mesh, &prob, stream);
+        if (status_) ctx_ = nullptr;
+    }
+    Assembler(const Assembler&) = delete;
+    Assembler& operator=(const Assembler&) = delete;
+    ~Assembler() {
+        if (ctx_) anifem_assembly_destroy(ctx_);
+    }
+    int status() const { return status_; }
+    int64_t ndof() const { return info().first; }
+    int64_t nnz() const { return info().second; }
+    // Device CSR pattern views
+    int csr(const int64_t** row_ptr, const int32_t** col_idx) const {
+        return ctx_ ? anifem_assembly_csr(ctx_, row_ptr, col_idx) : ANIFEM_ERR_ARG;
+    }
+    // Assemble(A, b): device csr_val (nnz) and rhs (ndof) are overwritten.
+    int Assemble(const TensorView& D, const TensorView* F, double* csr_val, double* rhs,
+                 anifem_status* st = nullptr, Precision precision = FP64, void* stream = nullptr) {
+        if (!ctx_) return status_ ? status_ : ANIFEM_ERR_ARG;
+        const anifem_tensor tD{D.type, D.storage, D.data};
+        anifem_tensor tF{};
+        if (F) tF = anifem_tensor{F->type, F->storage, F->data};
+        return anifem_assembly_run(ctx_, &tD, F ? &tF : nullptr, nullptr, precision, csr_val, rhs, st, stream);
+    }
+    anifem_assembly* handle() const { return ctx_; }
+
+private:
+    std::pair<int64_t, int64_t> info() const {
+        int64_t nd = 0, nz = 0, nb = 0;
+        int32_t nf = 0;
+        if (ctx_) anifem_assembly_info(ctx_, &nd, &nz, &nb, &nf);
+        return {nd, nz};
+    }
+    anifem_assembly* ctx_ = nullptr;
+    int status_ = ANIFEM_OK;
+};
+
+}  // namespace anifem_gpu
+
+#endif  // ANIFEM_FEM3DTET_HPP

@@ -1,0 +1,45 @@
+// Host-only check of include/anifem_fem3dtet.hpp: the reference-style tags map
+// to the C ABI's operators, the traits reject mismatched views, and the
+// template call reaches the library's argument checks (no device use).
+#include <cstdio>
+
+#include "anifem_fem3dtet.hpp"
+
+using namespace anifem_gpu;
+
+#define CHECK(c)                                            \
+    do {                                                    \
+        if (!(c)) {                                         \
+            std::printf("FAIL line %d: %s\n", __LINE__, #c); \
+            return 1;                                       \
+        }                                                   \
+    } while (0)
+
+int main() {
+    using GP2 = Operator<GRAD, FemFix<FEM_P2>>;
+    using IP3 = Operator<IDEN, FemFix<FEM_P3>>;
+    using GV2 = Operator<GRAD, FemVec<3, FEM_P2>>;
+    constexpr anifem_op g = GP2::c();
+    static_assert(g.op == ANIFEM_GRAD && g.fem == ANIFEM_FEM_P2 && g.ncomp == 1, "GRAD P2 tag");
+    constexpr anifem_op v = GV2::c();
+    static_assert(v.ncomp == 3 && v.fem == ANIFEM_FEM_P2, "FemVec tag");
+    // Nfa / Dim (poly_2.h, poly_3.h, operators.h:127-155)
+    CHECK(nfa<GP2>() == 10 && dim<GP2>() == 3);
+    CHECK(nfa<IP3>() == 20 && dim<IP3>() == 1);
+    CHECK(nfa<GV2>() == 30 && dim<GV2>() == 9);
+    // traits against views
+    const TensorView sym_tet{TENSOR_SYMMETRIC, ANIFEM_COEF_PER_TET, nullptr};
+    CHECK((check_view<DfuncTraits<TENSOR_SYMMETRIC, false>>(sym_tet) == ANIFEM_OK));
+    CHECK((check_view<DfuncTraits<TENSOR_SCALAR, false>>(sym_tet) == ANIFEM_ERR_ARG));
+    CHECK((check_view<DfuncTraits<TENSOR_SYMMETRIC, true>>(sym_tet) == ANIFEM_ERR_ARG));
+    CHECK((check_view<DfuncTraits<PerTetra, false>>(sym_tet) == ANIFEM_OK));
+    // the reference throws for order > 6 (int_tet.inl:99-100)
+    double buf[16] = {0};
+    CHECK((fem3Dtet<GP2, GP2, DfuncTraits<TENSOR_SYMMETRIC, false>>(buf, 1, sym_tet, buf, 7) == ANIFEM_ERR_ORDER));
+    // identity tensor between GRAD (dim 3) and IDEN (dim 1) (diff_tensor.h:313-315)
+    const TensorView null_const{TENSOR_NULL, ANIFEM_COEF_CONST, nullptr};
+    CHECK((fem3Dtet<GP2, Operator<IDEN, FemFix<FEM_P2>>, DfuncTraits<TENSOR_NULL, true>>(buf, 1, null_const, buf, 2) ==
+           ANIFEM_ERR_DIMS));
+    std::printf("header ok\n");
+    return 0;
+}

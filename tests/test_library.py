@@ -60,3 +60,22 @@ def test_argument_errors_without_device():
         == af.ERR_ARG
     assert L.anifem_mesh_kuhn_cube(0, 0.2, 0, None, None) == af.ERR_ARG
     assert L.anifem_mesh_kuhn_cube(4, 0.3, 0, None, None) == af.ERR_ARG
+
+
+def test_cpp_header_mirrors_reference_tags(tmp_path):
+    """include/anifem_fem3dtet.hpp compiles with the host compiler and maps the
+    reference's Operator / FemFix / FemVec / DfuncTraits tags onto the C ABI."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "inmost-fem_amd")
+    exe = tmp_path / "header_check"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "header_check.cpp"), "-L", libdir, "-lanifem_gpu",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "header ok" in out.stdout
