@@ -241,6 +241,66 @@ int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_t* col_idx,
                       const double* b, double* x, double rtol, int64_t maxit, anifem_pcg_result* res,
                       void* stream);
 
+/* Flexible PCG (Polak-Ribiere beta) with a factored preconditioner
+ * M^-1 = G G^T: G and GT are device FP64 values of G and of G^T on the CSR
+ * pattern of A (which must be symmetric).  Stopping rule and reporting as for
+ * anifem_pcg_jacobi.  No reference counterpart. */
+int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* val,
+                        const double* G, const double* GT, const double* b, double* x, double rtol, int64_t maxit,
+                        anifem_pcg_result* res, void* stream);
+
+/* ---- learned preconditioner ----------------------------------------------
+ * No reference counterpart.  A 3-layer message-passing network (32 hidden
+ * units, BF16 weights, FP32 accumulation) reads an assembled SPD matrix A and
+ * writes a lower-triangular factor G on A's pattern; M^-1 = G G^T.  With the
+ * output heads at zero the network is exactly Jacobi (G = diag(a_ii)^-1/2).
+ * Training minimises (1/(P n)) sum_p ||G^T A G z_p - z_p||^2 over P = 4
+ * Rademacher probes, with the network's own forward / backward kernels and
+ * Adam on the GPU.  Parameters: a flat FP32 vector of anifem_gnn_n_params()
+ * values in the order W_in(32x4), b_in, then per layer W_s(32x32), W_m(32x32),
+ * b, then w_o, c_o, w_d, c_d (row-major, out x in). */
+typedef struct anifem_gnn anifem_gnn;
+int anifem_gnn_n_params(void);
+/* row_ptr / col_idx: host CSR pattern of the matrices the context will see */
+int anifem_gnn_create(anifem_gnn** ctx, int64_t n, const int64_t* row_ptr, const int32_t* col_idx, void* stream);
+int anifem_gnn_destroy(anifem_gnn* ctx);
+int anifem_gnn_set_params(anifem_gnn* ctx, const float* theta);                   /* host; resets Adam */
+int anifem_gnn_get_params(const anifem_gnn* ctx, float* theta, float* theta_bf16); /* host, either may be NULL */
+int anifem_gnn_get_grad(const anifem_gnn* ctx, float* grad);                       /* host, last loss_grad */
+/* G (and G^T) of the matrix with device values a; G/GT NULL: kept in the context */
+int anifem_gnn_factor(anifem_gnn* ctx, const double* a, double* G, double* GT, void* stream);
+/* device views of the context's pattern and last factor */
+int anifem_gnn_views(const anifem_gnn* ctx, int64_t* n, const int64_t** row_ptr, const int32_t** col_idx,
+                     const double** G, const double** GT);
+/* forward + loss + gradient for matrix values a; probes: device n x 4 FP32
+ * (row-major) or NULL for Rademacher probes from seed; loss (host) may be NULL */
+int anifem_gnn_loss_grad(anifem_gnn* ctx, const double* a, const float* probes, uint64_t seed, float* loss,
+                         void* stream);
+int anifem_gnn_adam_step(anifem_gnn* ctx, float lr, float beta1, float beta2, float eps, void* stream);
+
+typedef struct anifem_pcg_learned_result {
+    int64_t iterations;          /* of the accepted solve */
+    double relres;
+    double true_relres;          /* FP64 ||b - A x|| / ||b|| of the returned x */
+    int64_t learned_iterations;  /* of the learned-preconditioned attempt */
+    double learned_true_relres;
+    int32_t fallback;            /* 1: Jacobi-PCG was used */
+    int32_t reserved;
+} anifem_pcg_learned_result;
+/* The network's factor for A (values val on the context's pattern), flexible
+ * PCG with it; when its FP64 true residual exceeds rtol (1% slack for the
+ * recursive-residual drift) or it fails, Jacobi-PCG from the same initial
+ * guess. */
+int anifem_pcg_learned(anifem_gnn* ctx, const double* val, const double* b, double* x, double rtol, int64_t maxit,
+                       anifem_pcg_learned_result* res, void* stream);
+
+/* Gaussian random field at n points (device xyz, n x 3): random Fourier
+ * features for the covariance sigma^2 exp(-|x - y|^2 / (2 ell^2)),
+ * g(x) = sigma sqrt(2/M) sum_m cos(w_m . x + phi_m), w_m ~ N(0, I/ell^2),
+ * phi_m ~ U[0, 2 pi) drawn from seed; out = g, or exp(g) with lognormal. */
+int anifem_grf_sample(int64_t n, const double* xyz, double ell, double sigma, int n_features, uint64_t seed,
+                      int lognormal, double* out, void* stream);
+
 /* ---- synthetic meshes (host) ----------------------------------------------
  * Unit cube split into n^3 sub-cubes, each into 6 Kuhn tets; node
  * (i,j,k) has id (i*(n+1)+j)*(n+1)+k (the V_ID of utils/mesh_utils.cpp:124).

@@ -85,6 +85,12 @@ class anifem_pcg_result(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("relres", ctypes.c_double), ("true_relres", ctypes.c_double)]
 
 
+class anifem_pcg_learned_result(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("relres", ctypes.c_double), ("true_relres", ctypes.c_double),
+                ("learned_iterations", ctypes.c_int64), ("learned_true_relres", ctypes.c_double),
+                ("fallback", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 # Every symbol include/anifem_gpu.h declares, with its ctypes signature.
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
 SIGNATURES = {
@@ -110,6 +116,20 @@ SIGNATURES = {
     "anifem_assembly_enable_timers": (_I32, [_VP, _I32]),
     "anifem_assembly_timers": (_I32, [_VP, ctypes.POINTER(anifem_timers)]),
     "anifem_pcg_jacobi": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
+                                   ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_gnn_n_params": (_I32, []),
+    "anifem_gnn_create": (_I32, [ctypes.POINTER(_VP), _I64, _VP, _VP, _VP]),
+    "anifem_gnn_destroy": (_I32, [_VP]),
+    "anifem_gnn_set_params": (_I32, [_VP, _VP]),
+    "anifem_gnn_get_params": (_I32, [_VP, _VP, _VP]),
+    "anifem_gnn_get_grad": (_I32, [_VP, _VP]),
+    "anifem_gnn_factor": (_I32, [_VP, _VP, _VP, _VP, _VP]),
+    "anifem_gnn_views": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_gnn_loss_grad": (_I32, [_VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
+    "anifem_gnn_adam_step": (_I32, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _VP]),
+    "anifem_pcg_learned": (_I32, [_VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_learned_result), _VP]),
+    "anifem_grf_sample": (_I32, [_I64, _VP, _D, _D, _I32, ctypes.c_uint64, _I32, _VP, _VP]),
     "anifem_mesh_kuhn_cube": (_I32, [_I64, _D, ctypes.c_uint64, _VP, _VP]),
     "anifem_version": (ctypes.c_char_p, []),
 }
@@ -138,9 +158,11 @@ def _check(rc, what):
 
 
 def _ptr(a):
-    """Address of a numpy array or torch tensor (None -> NULL)."""
+    """Address of a numpy array or torch tensor (None -> NULL; an int is an address)."""
     if a is None:
         return None
+    if isinstance(a, int):
+        return a
     if isinstance(a, np.ndarray):
         return a.ctypes.data
     return a.data_ptr()
@@ -346,6 +368,115 @@ def pcg_jacobi(n, row_ptr, col_idx, val, b, x, rtol=1e-8, maxit=100000, stream=N
     if rc not in (OK, ERR_NOT_CONVERGED):
         raise AnifemError(rc, "anifem_pcg_jacobi")
     return res, rc == OK
+
+
+def pcg_factored(n, row_ptr, col_idx, val, G, GT, b, x, rtol=1e-8, maxit=100000, stream=None):
+    """Flexible PCG with M^-1 = G G^T on device CSR; returns (result, converged)."""
+    res = anifem_pcg_result()
+    rp = row_ptr if isinstance(row_ptr, int) else _ptr(row_ptr)
+    ci = col_idx if isinstance(col_idx, int) else _ptr(col_idx)
+    rc = lib().anifem_pcg_factored(n, rp, ci, _ptr(val), _ptr(G), _ptr(GT), _ptr(b), _ptr(x), rtol, maxit,
+                                   ctypes.byref(res), _stream_handle(stream))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        raise AnifemError(rc, "anifem_pcg_factored")
+    return res, rc == OK
+
+
+def grf_sample(xyz, ell, sigma, n_features=512, seed=0, lognormal=True, out=None, stream=None):
+    """Gaussian random field (or its exponential) at device points xyz (n x 3, float64)."""
+    import torch
+    n = xyz.shape[0]
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=xyz.device)
+    _check(lib().anifem_grf_sample(n, _ptr(xyz), ell, sigma, n_features, seed, 1 if lognormal else 0, _ptr(out),
+                                   _stream_handle(stream)), "anifem_grf_sample")
+    return out
+
+
+class LearnedPreconditioner:
+    """The message-passing network of anifem_gnn_* over one CSR pattern (host
+    row_ptr / col_idx); matrices are device FP64 value arrays on that pattern."""
+
+    def __init__(self, row_ptr, col_idx, stream=None):
+        self._rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        self._ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+        self.n = len(self._rp) - 1
+        self.nnz = int(self._rp[-1])
+        h = ctypes.c_void_p()
+        _check(lib().anifem_gnn_create(ctypes.byref(h), self.n, _ptr(self._rp), _ptr(self._ci),
+                                       _stream_handle(stream)), "anifem_gnn_create")
+        self._h = h
+        self.n_params = lib().anifem_gnn_n_params()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.anifem_gnn_destroy(h)
+            self._h = None
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == self.n_params
+        _check(lib().anifem_gnn_set_params(self._h, _ptr(t)), "anifem_gnn_set_params")
+
+    def params(self, bf16=False):
+        t = np.zeros(self.n_params, np.float32)
+        if bf16:
+            _check(lib().anifem_gnn_get_params(self._h, None, _ptr(t)), "anifem_gnn_get_params")
+        else:
+            _check(lib().anifem_gnn_get_params(self._h, _ptr(t), None), "anifem_gnn_get_params")
+        return t
+
+    def grad(self):
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib().anifem_gnn_get_grad(self._h, _ptr(g)), "anifem_gnn_get_grad")
+        return g
+
+    def loss_grad(self, a, probes=None, seed=0, stream=None):
+        loss = ctypes.c_float()
+        _check(lib().anifem_gnn_loss_grad(self._h, _ptr(a), _ptr(probes), seed, ctypes.byref(loss),
+                                          _stream_handle(stream)), "anifem_gnn_loss_grad")
+        return loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        _check(lib().anifem_gnn_adam_step(self._h, lr, beta1, beta2, eps, _stream_handle(stream)),
+               "anifem_gnn_adam_step")
+
+    def factor(self, a, G=None, GT=None, stream=None):
+        """Factor for matrix values a: into the device tensors G / GT when given
+        (returned), else (G, GT) device addresses of the context's copies."""
+        _check(lib().anifem_gnn_factor(self._h, _ptr(a), _ptr(G), _ptr(GT), _stream_handle(stream)),
+               "anifem_gnn_factor")
+        if G is not None and GT is not None:
+            return G, GT
+        G, GT = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().anifem_gnn_views(self._h, None, None, None, ctypes.byref(G), ctypes.byref(GT)),
+               "anifem_gnn_views")
+        return G.value, GT.value
+
+    def solve(self, a, b, x, rtol=1e-8, maxit=100000, stream=None):
+        """Learned-preconditioned flexible PCG with Jacobi-PCG fallback."""
+        res = anifem_pcg_learned_result()
+        rc = lib().anifem_pcg_learned(self._h, _ptr(a), _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(res),
+                                      _stream_handle(stream))
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            raise AnifemError(rc, "anifem_pcg_learned")
+        return res, rc == OK
+
+
+def _gnn_init(n_params, seed):
+    """Initial network parameters: Gaussian weights (scaled by fan-in), zero
+    biases and zero output heads, so that the initial factor is Jacobi."""
+    H, F = 32, 4
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, 1 / np.sqrt(F), (H, F)).ravel(), np.zeros(H)]
+    for _ in range(3):
+        parts += [rng.normal(0, 1 / np.sqrt(H), (H, H)).ravel(), 0.5 * rng.normal(0, 1 / np.sqrt(H), (H, H)).ravel(),
+                  np.zeros(H)]
+    parts += [np.zeros(H), np.zeros(1), np.zeros(H), np.zeros(1)]
+    theta = np.concatenate(parts).astype(np.float32)
+    assert theta.size == n_params
+    return theta
 
 
 def version():

@@ -1,0 +1,96 @@
+"""Learned preconditioner workflow on the GPU: Gaussian-random-field diffusion
+systems produced by this assembler, training of the message-passing network
+(anifem_gnn_* kernels) and flexible PCG against Jacobi-PCG.
+
+System family (one mesh, many coefficients): P1 diffusion on the unit cube
+split into n^3 Kuhn sub-cubes (jittered), kappa = exp(g) per tet with g a
+Gaussian random field (squared-exponential covariance, correlation length
+ell, standard deviation sigma) at the tet centroids, f = 1, homogeneous
+Dirichlet on the boundary.
+"""
+import time
+
+import numpy as np
+
+from . import (Assembler, LearnedPreconditioner, Op, GRAD, FEM_P1, TENSOR_SCALAR, COEF_PER_TET, COEF_CONST,
+               DIRICHLET_BOUNDARY, grf_sample, kuhn_cube, pcg_jacobi)
+
+
+class GrfSystems:
+    """Seeded GRF diffusion systems on one mesh; matrices assembled on demand."""
+
+    def __init__(self, n=32, ell=0.1, sigma=1.5, jitter=0.2, mesh_seed=1, n_features=512):
+        import torch
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        coords, tets = kuhn_cube(n, jitter, mesh_seed)
+        self.asm = Assembler(coords, tets, Op(GRAD, FEM_P1), order=2, tensor_type=TENSOR_SCALAR,
+                             tensor_storage=COEF_PER_TET, rhs_order=2, rhs_tensor_type=TENSOR_SCALAR,
+                             rhs_tensor_storage=COEF_CONST, dirichlet=DIRICHLET_BOUNDARY)
+        cen = coords[tets].mean(axis=1)
+        self.centroids = torch.from_numpy(np.ascontiguousarray(cen)).to(self.dev)
+        self.ell, self.sigma, self.n_features = ell, sigma, n_features
+        s = self.asm.setup()
+        self.row_ptr, self.col_idx = s["row_ptr"], s["col_idx"]
+        self.rp_dev, self.ci_dev = self.asm.csr()
+        self.n = self.asm.ndof
+        self.f = torch.ones(1, dtype=torch.float64, device=self.dev)
+        self.kappa = torch.empty(len(tets), dtype=torch.float64, device=self.dev)
+
+    def system(self, seed, val=None, rhs=None):
+        """(val, rhs) device FP64 for coefficient seed `seed`."""
+        grf_sample(self.centroids, self.ell, self.sigma, self.n_features, seed, True, out=self.kappa)
+        val, rhs, _ = self.asm.assemble(D=self.kappa.view(1, -1), F=self.f, val=val, rhs=rhs, status=False)
+        return val, rhs
+
+
+def train(systems, seeds, lr=1e-3, epochs=1, init_seed=0, log=None):
+    """Adam on the probe loss over the systems of `seeds`; returns (net, losses, seconds)."""
+    import torch
+    from . import _gnn_init
+    net = LearnedPreconditioner(systems.row_ptr, systems.col_idx)
+    net.set_params(_gnn_init(net.n_params, init_seed))
+    val = torch.empty(systems.asm.nnz, dtype=torch.float64, device=systems.dev)
+    rhs = torch.empty(systems.n, dtype=torch.float64, device=systems.dev)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        for sd in seeds:
+            systems.system(int(sd), val, rhs)
+            loss = net.loss_grad(val, seed=1_000_003 * (step + 1))
+            net.adam_step(lr=lr)
+            losses.append(loss)
+            step += 1
+            if log and step % 200 == 0:
+                log(f"step {step} loss {np.mean(losses[-200:]):.5f}")
+    torch.cuda.synchronize()
+    return net, np.array(losses), time.perf_counter() - t0
+
+
+def compare(systems, net, seeds, rtol=1e-8, maxit=20000):
+    """Per held-out seed: Jacobi-PCG and learned FPCG times (ms, synchronized
+    wall clock of the solve calls), iterations, true residuals, fallbacks."""
+    import torch
+    val = torch.empty(systems.asm.nnz, dtype=torch.float64, device=systems.dev)
+    rhs = torch.empty(systems.n, dtype=torch.float64, device=systems.dev)
+    x = torch.zeros(systems.n, dtype=torch.float64, device=systems.dev)
+    rows = []
+    for sd in seeds:
+        systems.system(int(sd), val, rhs)
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rj, okj = pcg_jacobi(systems.n, systems.rp_dev, systems.ci_dev, val, rhs, x, rtol=rtol, maxit=maxit)
+        torch.cuda.synchronize()
+        tj = (time.perf_counter() - t) * 1e3
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rl, okl = net.solve(val, rhs, x, rtol=rtol, maxit=maxit)
+        torch.cuda.synchronize()
+        tl = (time.perf_counter() - t) * 1e3
+        rows.append(dict(seed=int(sd), jacobi_ms=tj, jacobi_it=rj.iterations, jacobi_true=rj.true_relres,
+                         learned_ms=tl, learned_it=rl.iterations, learned_true=rl.true_relres,
+                         fallback=rl.fallback, ok=bool(okj and okl)))
+    return rows

@@ -17,7 +17,7 @@
 namespace anifem {
 
 struct PcgState {
-    double rz, rz_new, pAp, rr, bb, tol2;
+    double rz, rz_new, pAp, rr, bb, tol2, rzo;
     int64_t it;
     int done;
 };
@@ -99,14 +99,15 @@ __global__ __launch_bounds__(kThreads) void init_kernel(int64_t n, const double*
     }
 }
 
-// Second reduction stage; `which`: 0 init (rz, rr, bb), 1 pAp, 2 (rz_new, rr).
+// Second reduction stage; `which`: 0 init (rz, rr, bb), 1 pAp, 2 (rz_new, rr),
+// 3 init of the factored method (rr, bb), 4 rr after an update, 5 (rz_new, rzo).
 __global__ __launch_bounds__(kThreads) void reduce_kernel(const double* partial, int nparts, int which,
                                                            PcgState* st, double rtol) {
-    if (which != 0 && st->done) return;
+    if (which != 0 && which != 3 && st->done) return;
     double a = 0, b = 0, c = 0;
     for (int i = threadIdx.x; i < nparts; i += kThreads) {
         a += partial[i];
-        if (which != 1) b += partial[kRedBlocks + i];
+        if (which != 1 && which != 4) b += partial[kRedBlocks + i];
         if (which == 0) c += partial[2 * kRedBlocks + i];
     }
     a = block_sum(a);
@@ -122,6 +123,19 @@ __global__ __launch_bounds__(kThreads) void reduce_kernel(const double* partial,
             st->done = (b <= st->tol2) ? 1 : 0;
         } else if (which == 1) {
             st->pAp = a;
+        } else if (which == 3) {
+            st->rr = a;
+            st->bb = b > 0 ? b : 1.0;
+            st->tol2 = rtol * rtol * st->bb;
+            st->it = 0;
+            st->done = (a <= st->tol2) ? 1 : 0;
+        } else if (which == 4) {
+            st->rr = a;
+            st->it += 1;
+            if (a <= st->tol2) st->done = 1;
+        } else if (which == 5) {
+            st->rz_new = a;
+            st->rzo = b;
         } else {
             st->rz_new = a;
             st->rr = b;
@@ -165,6 +179,78 @@ __global__ void commit_kernel(PcgState* st) {
     if (!st->done) st->rz = st->rz_new;
 }
 
+// ---- factored preconditioner M^-1 = G G^T (flexible CG) -------------------
+// r = b - Ax; partials r.r, b.b
+__global__ __launch_bounds__(kThreads) void init_f_kernel(int64_t n, const double* b, const double* Ax, double* r,
+                                                           double* partial) {
+    double rr = 0, bb = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = b[i] - Ax[i];
+        r[i] = ri;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    const double s0 = block_sum(rr), s1 = block_sum(bb);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial[kRedBlocks + blockIdx.x] = s1;
+    }
+}
+
+// x += alpha p, r -= alpha Ap; partial r.r
+__global__ __launch_bounds__(kThreads) void update_f_kernel(int64_t n, double* x, double* r, const double* p,
+                                                             const double* Ap, double* partial, const PcgState* st) {
+    if (st->done) return;
+    const double alpha = st->rz / st->pAp;
+    double rr = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    const double s = block_sum(rr);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// z = V t (one lane group per row) with partials r.z and r.zo (zo may be null)
+template <int L>
+__global__ __launch_bounds__(kThreads) void prec_spmv_kernel(int64_t n, const int64_t* __restrict__ rp,
+                                                              const int32_t* __restrict__ ci,
+                                                              const double* __restrict__ v, const double* t,
+                                                              double* z, const double* r, const double* zo,
+                                                              double* partial, const PcgState* st) {
+    if (st->done) return;
+    const int64_t rows_per_block = kThreads / L;
+    double d0 = 0, d1 = 0;
+    for (int64_t base = (int64_t)blockIdx.x * rows_per_block; base < n; base += (int64_t)gridDim.x * rows_per_block) {
+        const int64_t row = base + threadIdx.x / L;
+        const int lane = threadIdx.x % L;
+        double s = 0;
+        if (row < n)
+            for (int64_t k = rp[row] + lane; k < rp[row + 1]; k += L) s += v[k] * t[ci[k]];
+        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
+        if (row < n && lane == 0) {
+            z[row] = s;
+            d0 += r[row] * s;
+            if (zo) d1 += r[row] * zo[row];
+        }
+    }
+    const double s0 = block_sum(d0), s1 = block_sum(d1);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial[kRedBlocks + blockIdx.x] = s1;
+    }
+}
+
+// p = z + beta p, beta = (rz_new - rzo) / rz (Polak-Ribiere; 0 on the first step)
+__global__ void direction_f_kernel(int64_t n, const double* z, double* p, PcgState* st, int first) {
+    if (st->done) return;
+    const double beta = first ? 0.0 : (st->rz_new - st->rzo) / st->rz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
 __global__ __launch_bounds__(kThreads) void residual_kernel(int64_t n, const double* b, const double* Ax,
                                                              double* partial) {
     double rr = 0;
@@ -196,6 +282,25 @@ int spmv_dispatch(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* ci, 
     if (avg <= 24) return launch_spmv<8>(n, rp, ci, v, x, y, partial, st, s);
     if (avg <= 48) return launch_spmv<16>(n, rp, ci, v, x, y, partial, st, s);
     return launch_spmv<32>(n, rp, ci, v, x, y, partial, st, s);
+}
+
+template <int L>
+static int launch_prec(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* t, double* z,
+                       const double* r, const double* zo, double* partial, const PcgState* st, hipStream_t s) {
+    const int64_t rows_per_block = kThreads / L;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, kRedBlocks));
+    hipLaunchKernelGGL(prec_spmv_kernel<L>, dim3((unsigned)blocks), dim3(kThreads), 0, s, n, rp, ci, v, t, z, r, zo,
+                       partial, st);
+    return (int)blocks;
+}
+
+int prec_dispatch(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* ci, const double* v, const double* t,
+                  double* z, const double* r, const double* zo, double* partial, const PcgState* st, hipStream_t s) {
+    const double avg = n > 0 ? (double)nnz / n : 0;
+    if (avg <= 12) return launch_prec<4>(n, rp, ci, v, t, z, r, zo, partial, st, s);
+    if (avg <= 24) return launch_prec<8>(n, rp, ci, v, t, z, r, zo, partial, st, s);
+    if (avg <= 48) return launch_prec<16>(n, rp, ci, v, t, z, r, zo, partial, st, s);
+    return launch_prec<32>(n, rp, ci, v, t, z, r, zo, partial, st, s);
 }
 
 }  // namespace anifem
@@ -260,5 +365,130 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
         delete[] hp;
     }
     (void)hipFree(work);
+    return rc;
+}
+
+// Flexible PCG with the factored preconditioner M^-1 = G G^T (G, GT: values of
+// G and of G^T on the pattern of A).  Same stopping rule and reporting as
+// anifem_pcg_jacobi.
+extern "C" int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* val,
+                                   const double* G, const double* GT, const double* b, double* x, double rtol,
+                                   int64_t maxit, anifem_pcg_result* res, void* stream) {
+    using namespace anifem;
+    if (n <= 0 || !row_ptr || !col_idx || !val || !G || !GT || !b || !x || rtol <= 0 || maxit < 0)
+        return ANIFEM_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t nnz = 0;
+    ANIFEM_HIP_CHECK(hipMemcpyAsync(&nnz, row_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+    double* work = nullptr;
+    const size_t nv = (size_t)n;
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 3 * kRedBlocks) + sizeof(PcgState)));
+    double *r = work, *t = r + nv, *za = t + nv, *zb = za + nv, *p = zb + nv, *Ap = p + nv, *part = Ap + nv;
+    PcgState* st = reinterpret_cast<PcgState*>(part + 3 * kRedBlocks);
+    const unsigned vb = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, kRedBlocks);
+    int rc = ANIFEM_OK;
+    // r0 = b - A x0, z0 = G G^T r0, p0 = z0
+    spmv_dispatch(n, nnz, row_ptr, col_idx, val, x, Ap, nullptr, nullptr, s);
+    hipLaunchKernelGGL(init_f_kernel, dim3(vb), dim3(kThreads), 0, s, n, b, Ap, r, part);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, (int)vb, 3, st, rtol);
+    spmv_dispatch(n, nnz, row_ptr, col_idx, GT, r, t, nullptr, st, s);
+    int pb = prec_dispatch(n, nnz, row_ptr, col_idx, G, t, za, r, nullptr, part, st, s);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, pb, 5, st, rtol);
+    hipLaunchKernelGGL(direction_f_kernel, dim3(vb), dim3(kThreads), 0, s, n, za, p, st, 1);
+    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(1), 0, s, st);
+    double* zold = za;
+    double* znew = zb;
+    PcgState hs{};
+    const int64_t poll = 16;
+    for (int64_t it = 0; it < maxit; ++it) {
+        const int spmv_blocks = spmv_dispatch(n, nnz, row_ptr, col_idx, val, p, Ap, part, st, s);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, spmv_blocks, 1, st, rtol);
+        hipLaunchKernelGGL(update_f_kernel, dim3(vb), dim3(kThreads), 0, s, n, x, r, p, Ap, part, st);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, (int)vb, 4, st, rtol);
+        spmv_dispatch(n, nnz, row_ptr, col_idx, GT, r, t, nullptr, st, s);
+        pb = prec_dispatch(n, nnz, row_ptr, col_idx, G, t, znew, r, zold, part, st, s);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, pb, 5, st, rtol);
+        hipLaunchKernelGGL(direction_f_kernel, dim3(vb), dim3(kThreads), 0, s, n, znew, p, st, 0);
+        hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(1), 0, s, st);
+        std::swap(zold, znew);
+        if ((it + 1) % poll == 0 || it + 1 == maxit) {
+            if (hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = ANIFEM_ERR_HIP;
+                break;
+            }
+            if (hs.done) break;
+        }
+    }
+    if (rc == ANIFEM_OK) {
+        spmv_dispatch(n, nnz, row_ptr, col_idx, val, x, Ap, nullptr, nullptr, s);
+        hipLaunchKernelGGL(residual_kernel, dim3(vb), dim3(kThreads), 0, s, n, b, Ap, part);
+        std::vector<double> hp(vb);
+        if (hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(hp.data(), part, sizeof(double) * vb, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = ANIFEM_ERR_HIP;
+        } else {
+            double tr = 0;
+            for (unsigned i = 0; i < vb; ++i) tr += hp[i];
+            if (res) {
+                res->iterations = hs.it;
+                res->relres = std::sqrt(hs.rr / hs.bb);
+                res->true_relres = std::sqrt(tr / hs.bb);
+            }
+            if (!hs.done) rc = ANIFEM_ERR_NOT_CONVERGED;
+        }
+    }
+    (void)hipFree(work);
+    return rc;
+}
+
+// Learned-preconditioned solve: the network's factor G for this matrix, then
+// flexible PCG with M^-1 = G G^T; the FP64 true residual judges the result
+// (||b - A x|| <= rtol ||b||, up to 1% for the recursive-residual drift) and
+// Jacobi-PCG from the same initial guess takes over when it fails.
+extern "C" int anifem_pcg_learned(anifem_gnn* g, const double* val, const double* b, double* x, double rtol,
+                                  int64_t maxit, anifem_pcg_learned_result* res, void* stream) {
+    using namespace anifem;
+    if (!g || !val || !b || !x) return ANIFEM_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t n = 0;
+    const int64_t* rp = nullptr;
+    const int32_t* ci = nullptr;
+    const double *G = nullptr, *GT = nullptr;
+    int rc = anifem_gnn_views(g, &n, &rp, &ci, &G, &GT);
+    if (rc) return rc;
+    double* x0 = nullptr;
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&x0, n * sizeof(double)));
+    anifem_pcg_learned_result out{};
+    if (hipMemcpyAsync(x0, x, n * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) rc = ANIFEM_ERR_HIP;
+    if (!rc) rc = anifem_gnn_factor(g, val, nullptr, nullptr, stream);
+    anifem_pcg_result r1{};
+    int rc1 = rc ? rc : anifem_pcg_factored(n, rp, ci, val, G, GT, b, x, rtol, maxit, &r1, stream);
+    out.learned_iterations = r1.iterations;
+    out.learned_true_relres = r1.true_relres;
+    const bool ok = (rc1 == ANIFEM_OK || rc1 == ANIFEM_ERR_NOT_CONVERGED) && r1.true_relres <= 1.01 * rtol &&
+                    std::isfinite(r1.true_relres);
+    if (rc == ANIFEM_OK && (rc1 == ANIFEM_ERR_HIP || rc1 == ANIFEM_ERR_NOMEM)) rc = rc1;
+    if (!rc) {
+        if (ok) {
+            out.iterations = r1.iterations;
+            out.relres = r1.relres;
+            out.true_relres = r1.true_relres;
+            out.fallback = 0;
+        } else {
+            anifem_pcg_result r2{};
+            if (hipMemcpyAsync(x, x0, n * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) rc = ANIFEM_ERR_HIP;
+            const int rc2 = rc ? rc : anifem_pcg_jacobi(n, rp, ci, val, b, x, rtol, maxit, &r2, stream);
+            out.iterations = r2.iterations;
+            out.relres = r2.relres;
+            out.true_relres = r2.true_relres;
+            out.fallback = 1;
+            if (!rc) rc = rc2;
+        }
+    }
+    (void)hipFree(x0);
+    if (res) *res = out;
     return rc;
 }
