@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--learned-train", type=int, default=2000, help="training systems (learned preconditioner)")
+    ap.add_argument("--learned-eval", type=int, default=200, help="held-out systems (learned preconditioner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
@@ -275,6 +277,28 @@ def main():
         torch.cuda.synchronize()
         out["pcg_jacobi_p1_16"] = {"ms": (time.perf_counter() - tp) * 1e3, "iterations": r.iterations,
                                    "true_relres": r.true_relres, "converged": ok}
+        # learned preconditioner: P1 GRF diffusion at 32^3, trained on 2,000
+        # seeded systems from this assembler, 200 held-out seeds, rel-res 1e-8
+        from anifem_amd.learned import GrfSystems, train, compare
+        grf = GrfSystems(n=32, ell=0.1, sigma=1.5)
+        net, losses, train_s = train(grf, np.arange(args.learned_train), lr=2e-3, epochs=1)
+        rows = compare(grf, net, np.arange(1_000_000, 1_000_000 + args.learned_eval), rtol=1e-8)
+        jm = np.array([r["jacobi_ms"] for r in rows])
+        lm = np.array([r["learned_ms"] for r in rows])
+        out["pcg_learned_vs_jacobi_grf32"] = {
+            "ndof": grf.n, "nnz": grf.asm.nnz, "train_systems": int(args.learned_train),
+            "train_seconds": train_s, "loss_first_100": float(losses[:100].mean()),
+            "loss_last_100": float(losses[-100:].mean()), "eval_systems": len(rows), "rtol": 1e-8,
+            "jacobi_ms_mean": float(jm.mean()), "learned_ms_mean": float(lm.mean()),
+            "speedup_mean": float(jm.mean() / lm.mean()),
+            "jacobi_iterations_mean": float(np.mean([r["jacobi_it"] for r in rows])),
+            "learned_iterations_mean": float(np.mean([r["learned_it"] for r in rows])),
+            "learned_faster_fraction": float(np.mean(lm < jm)),
+            "fallbacks": int(sum(r["fallback"] for r in rows)),
+            "max_true_relres": float(max(max(r["learned_true"], r["jacobi_true"]) for r in rows)),
+            "all_converged": bool(all(r["ok"] for r in rows)),
+            "note": "wall time of each solve call (learned includes the network's factor), FP64 true residual"}
+        del net, grf
 
     if not args.no_cpu_baseline:
         s = Asm.setup()
