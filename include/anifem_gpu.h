@@ -225,6 +225,41 @@ typedef struct anifem_timers {
 int anifem_assembly_enable_timers(anifem_assembly* ctx, int on);
 int anifem_assembly_timers(const anifem_assembly* ctx, anifem_timers* out);
 
+/* ---- batched assembly of many meshes --------------------------------------
+ * No reference counterpart (the reference assembles one mesh per Assembler).
+ * Each mesh is scalar diffusion -div(kappa grad u) = f: the element matrices
+ * fem3Dtet<GRAD FemFix<Pk>, GRAD FemFix<Pk>, DfuncTraits<TENSOR_SCALAR, false>>
+ * with quadrature order 2 (P1, P2) or 4 (P3), load vectors of a constant f,
+ * homogeneous Dirichlet on the boundary (or none).  All meshes are assembled
+ * by one geometry launch and one row-block launch, captured in a HIP graph:
+ * mesh i's CSR values occupy val[nnz_offset_i ..] and its rhs
+ * rhs[dof_offset_i ..], bitwise identical to assembling it alone with
+ * anifem_assembly_run.  With autotune, the row-block budgets are chosen per
+ * element order by timing candidates on the GPU (anifem_batch_tuning). */
+typedef struct anifem_batch anifem_batch;
+typedef struct anifem_batch_mesh {
+    int64_t n_nodes;
+    int64_t n_tets;
+    const double* coords; /* host, n_nodes x 3 */
+    const int32_t* tets;  /* host, n_tets x 4 */
+    int32_t fem;          /* ANIFEM_FEM_P1 .. ANIFEM_FEM_P3 */
+    int32_t reserved;
+} anifem_batch_mesh;
+int anifem_batch_create(anifem_batch** ctx, int n_meshes, const anifem_batch_mesh* meshes, int dirichlet,
+                        int autotune, void* stream);
+int anifem_batch_destroy(anifem_batch* ctx);
+int anifem_batch_totals(const anifem_batch* ctx, int64_t* n_tets, int64_t* ndof, int64_t* nnz, int64_t* n_blocks);
+int anifem_batch_info(const anifem_batch* ctx, int mesh, int64_t* tet_offset, int64_t* dof_offset,
+                      int64_t* nnz_offset, int64_t* ndof, int64_t* nnz);
+/* device views of mesh i's own CSR pattern (row_ptr starts at 0) */
+int anifem_batch_csr(const anifem_batch* ctx, int mesh, const int64_t** row_ptr, const int32_t** col_idx);
+/* chosen budgets {slots, entries, pairs} per block and the timed candidates (ms; -1: not applicable) */
+int anifem_batch_tuning(const anifem_batch* ctx, int fem, int32_t* caps, double* ms);
+/* kappa: device, all tets in mesh order; f: device, one FP64 value; val (total
+ * nnz) and rhs (total ndof): device FP64, overwritten */
+int anifem_batch_run(anifem_batch* ctx, const double* kappa, const double* f, double* val, double* rhs,
+                     anifem_status* status, void* stream);
+
 /* ---- solver ---------------------------------------------------------------
  * Jacobi-preconditioned CG on a device CSR matrix in FP64 (the reference
  * solves with INMOST's Solver, examples/tutorials/ex1.cpp:161-163; there is

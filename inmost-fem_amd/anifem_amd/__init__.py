@@ -85,6 +85,11 @@ class anifem_pcg_result(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("relres", ctypes.c_double), ("true_relres", ctypes.c_double)]
 
 
+class anifem_batch_mesh(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_tets", ctypes.c_int64), ("coords", ctypes.c_void_p),
+                ("tets", ctypes.c_void_p), ("fem", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class anifem_pcg_learned_result(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("relres", ctypes.c_double), ("true_relres", ctypes.c_double),
                 ("learned_iterations", ctypes.c_int64), ("learned_true_relres", ctypes.c_double),
@@ -116,6 +121,13 @@ SIGNATURES = {
     "anifem_assembly_enable_timers": (_I32, [_VP, _I32]),
     "anifem_assembly_timers": (_I32, [_VP, ctypes.POINTER(anifem_timers)]),
     "anifem_pcg_jacobi": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_batch_create": (_I32, [ctypes.POINTER(_VP), _I32, ctypes.POINTER(anifem_batch_mesh), _I32, _I32, _VP]),
+    "anifem_batch_destroy": (_I32, [_VP]),
+    "anifem_batch_totals": (_I32, [_VP, _VP, _VP, _VP, _VP]),
+    "anifem_batch_info": (_I32, [_VP, _I32, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_batch_csr": (_I32, [_VP, _I32, _VP, _VP]),
+    "anifem_batch_tuning": (_I32, [_VP, _I32, _VP, _VP]),
+    "anifem_batch_run": (_I32, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
                                    ctypes.POINTER(anifem_pcg_result), _VP]),
     "anifem_gnn_n_params": (_I32, []),
@@ -462,6 +474,60 @@ class LearnedPreconditioner:
         if rc not in (OK, ERR_NOT_CONVERGED):
             raise AnifemError(rc, "anifem_pcg_learned")
         return res, rc == OK
+
+
+class Batch:
+    """Batched scalar-diffusion assembly of many meshes (anifem_batch_*).
+    meshes: list of (coords (n,3) float64, tets (m,4) int32, fem)."""
+
+    def __init__(self, meshes, dirichlet=DIRICHLET_BOUNDARY, autotune=True, stream=None):
+        self._keep = []
+        arr = (anifem_batch_mesh * len(meshes))()
+        for i, (c, t, fem) in enumerate(meshes):
+            c = np.ascontiguousarray(c, dtype=np.float64)
+            t = np.ascontiguousarray(t, dtype=np.int32)
+            self._keep += [c, t]
+            arr[i] = anifem_batch_mesh(len(c), len(t), _ptr(c), _ptr(t), fem, 0)
+        h = ctypes.c_void_p()
+        _check(lib().anifem_batch_create(ctypes.byref(h), len(meshes), arr, dirichlet, 1 if autotune else 0,
+                                         _stream_handle(stream)), "anifem_batch_create")
+        self._h = h
+        self._keep = []
+        self.n_meshes = len(meshes)
+        nt, nd, nz, nb = (ctypes.c_int64() for _ in range(4))
+        lib().anifem_batch_totals(h, ctypes.byref(nt), ctypes.byref(nd), ctypes.byref(nz), ctypes.byref(nb))
+        self.n_tets, self.ndof, self.nnz, self.n_blocks = nt.value, nd.value, nz.value, nb.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.anifem_batch_destroy(h)
+            self._h = None
+
+    def info(self, i):
+        v = [ctypes.c_int64() for _ in range(5)]
+        _check(lib().anifem_batch_info(self._h, i, *[ctypes.byref(x) for x in v]), "anifem_batch_info")
+        return dict(zip(["tet_offset", "dof_offset", "nnz_offset", "ndof", "nnz"], [x.value for x in v]))
+
+    def tuning(self, fem):
+        caps = np.zeros(3, np.int32)
+        ms = np.zeros(4)
+        _check(lib().anifem_batch_tuning(self._h, fem, _ptr(caps), _ptr(ms)), "anifem_batch_tuning")
+        return caps, ms
+
+    def run(self, kappa, f, val=None, rhs=None, status=True, stream=None):
+        import torch
+        dev = kappa.device
+        if val is None:
+            val = torch.empty(self.nnz, dtype=torch.float64, device=dev)
+        if rhs is None:
+            rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
+        st = anifem_status()
+        rc = lib().anifem_batch_run(self._h, _ptr(kappa), _ptr(f), _ptr(val), _ptr(rhs),
+                                    ctypes.byref(st) if status else None, _stream_handle(stream))
+        if rc not in (OK, ERR_NONFINITE):
+            raise AnifemError(rc, "anifem_batch_run")
+        return val, rhs, (st if status else None)
 
 
 def _gnn_init(n_params, seed):

@@ -86,13 +86,15 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
     auto lds_of = [&](bool with_R) {
         return block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T),
-                                with_R ? a.R_rows * pad16<T>(len) : 0,
-                                with_R && a.has_rhs ? a.RF_rows * pad16<T>(lenF) : 0)
+                                with_R ? a.max_R_rows * pad16<T>(len) : 0,
+                                with_R && a.has_rhs ? a.max_RF_rows * pad16<T>(lenF) : 0)
             .total;
     };
     // the reference tensors are staged in LDS while two workgroups fit a CU
     // (and the staging's 16-bit indexing holds)
-    const bool staged = lds_of(true) <= (size_t)max_lds / 2 && a.R_rows * len < 65536 && a.RF_rows * lenF < 65536;
+    const bool staged = lds_of(true) <= (size_t)max_lds / 2 && a.max_R_rows * len < 65536 &&
+                        a.max_RF_rows * lenF < 65536;
+    if (a.max_pairs > kBlock) return ANIFEM_ERR_DIMS;
     const size_t lds = lds_of(staged);
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
     // geometry rows of every tet (kept on the device between runs)
@@ -136,25 +138,35 @@ int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, hipStream_
     return ANIFEM_ERR_ARG;
 }
 
-int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob) {
+int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
+               const anifem::BlockCaps& caps = anifem::BlockCaps()) {
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
-    return build_setup(*mesh, *prob, ctx->S);
+    return build_setup(*mesh, *prob, ctx->S, caps);
 }
 
 }  // namespace
 
-extern "C" {
+namespace anifem {
+int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
+                         const BlockCaps& caps, void* stream);
+}
 
-int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-                           void* stream) {
+extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
+                                      const anifem_problem_desc* prob, void* stream) {
+    return anifem::assembly_create_caps(out, mesh, prob, anifem::BlockCaps(), stream);
+}
+
+// Assembly context with given row-block budgets (the batch autotuner's trials).
+int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh,
+                                 const anifem_problem_desc* prob, const BlockCaps& caps, void* stream) {
     using namespace anifem;
     if (!out) return ANIFEM_ERR_ARG;
     std::unique_ptr<anifem_assembly> ctx(new anifem_assembly());
-    int rc = build_host(ctx.get(), mesh, prob);
+    int rc = build_host(ctx.get(), mesh, prob, caps);
     if (rc) return rc;
     AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -188,6 +200,8 @@ int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh, 
     *out = ctx.release();
     return ANIFEM_OK;
 }
+
+extern "C" {
 
 int anifem_assembly_create_host(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob) {
     if (!out) return ANIFEM_ERR_ARG;
@@ -318,16 +332,18 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     a.m.tetnodes = ctx->d_tetnodes;
     a.m.nodes_stride = S.n_tets;
     a.m.pos0 = 0;
-    a.R_rows = a.m.nfA0 * a.m.nfB0;
     if (do_rhs) {
         anifem_op p0{ANIFEM_IDEN, ANIFEM_FEM_P0, 1};
         anifem_op test_iden{ANIFEM_IDEN, ctx->prob.test.fem, ctx->prob.test.ncomp};
         if ((rc = prepare_elem_args(p0, test_iden, F, ctx->prob.rhs_order, a.f))) return rc;
         a.f.coef_n = S.n_tets;
-        a.RF_rows = a.f.nfB0;
     }
     a.has_rhs = do_rhs ? 1 : 0;
-    a.nf = S.nf;
+    a.n_kinds = 1;
+    a.kinds[0] = KindTable{a.m.Rd, a.m.Rf, a.f.Rd, a.f.Rf, a.m.nfA0, a.m.nfB0, S.nf, a.m.nfA0 * a.m.nfB0,
+                           do_rhs ? a.f.nfB0 : 0, 0};
+    a.max_R_rows = a.kinds[0].R_rows;
+    a.max_RF_rows = a.kinds[0].RF_rows;
     a.n_blocks = S.n_blocks;
     a.meta = ctx->d_blk_meta;
     a.blk_row0 = ctx->d_blk_row0;

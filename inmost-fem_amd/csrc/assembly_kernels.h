@@ -29,15 +29,25 @@
 
 namespace anifem {
 
-static_assert(kBlockMaxPairs <= kBlock, "one pair per lane");
+// Element kinds of one launch: the batched assembly mixes element orders, one
+// reference tensor per kind (a single-mesh assembly has one kind).
+constexpr int kMaxKinds = 4;
+struct KindTable {
+    const double* Rd;   // reference tensor R (R_rows x len), FP64 / FP32 copies
+    const float* Rf;
+    const double* RFd;  // load-vector reference tensor (RF_rows x lenF)
+    const float* RFf;
+    int nfA0, nfB0, nf, R_rows, RF_rows, pad;
+};
 
 
 struct BlockArgs {
     ElemArgs m;   // matrix pair (geometry source, tensor, reference tensor)
     ElemArgs f;   // load-vector pair IDEN P0 x IDEN test (when has_rhs)
     int has_rhs;
-    int nf;       // local DOFs per tet
     int rowlen;   // values per tet of the geometry rows
+    int n_kinds;
+    KindTable kinds[kMaxKinds];
     int64_t n_blocks;
     const BlockMeta* meta;      // per block, in processing order
     const int32_t* blk_row0;
@@ -52,8 +62,8 @@ struct BlockArgs {
     double* val;
     double* rhs;
     int* status;
-    int max_values, max_pairs, max_slots;
-    int R_rows, RF_rows;        // rows of the reference tensors (len / lenF values each)
+    int max_values, max_pairs, max_slots;  // over all blocks (at most the workgroup size of pairs)
+    int max_R_rows, max_RF_rows;           // over all kinds
 };
 
 // Row length padded to a 16-byte multiple: rows read as 16-byte vectors stay
@@ -160,10 +170,9 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int len = a.m.q * KB, lenp = pad16<T>(len);
     const int lenF = a.has_rhs ? a.f.q : 0, lenFp = a.has_rhs ? pad16<T>(lenF) : 0;
-    const int nf = a.nf, nfA0 = a.m.nfA0, nfB0 = a.m.nfB0;
     const int kstride = NCA * NCB * len;
     const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T),
-                                        RS ? a.R_rows * lenp : 0, RS && a.has_rhs ? a.RF_rows * lenFp : 0);
+                                        RS ? a.max_R_rows * lenp : 0, RS && a.has_rhs ? a.max_RF_rows * lenFp : 0);
     double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
     double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
     uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
@@ -181,11 +190,14 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     const int64_t p0 = M.p0;
     const int np = M.np;
     const int nv = M.nv;
+    const KindTable& KT = a.kinds[M.kind];  // uniform over the workgroup
+    const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0;
 
     // the pair of this lane and its tet's geometry row, requested before the
     // staging so that their loads overlap it
     const bool has_pair = (int)threadIdx.x < np;
     const int64_t gp = p0 + (has_pair ? threadIdx.x : 0);
+    const int64_t qp = M.q0 + (int64_t)(has_pair ? threadIdx.x : 0) * nf;  // this pair's value positions
     const int32_t tet = has_pair ? a.pair_tet[gp] : 0;
     const int ibf = has_pair ? a.pair_ib[gp] : 0;
     const int ib = ibf & 63;
@@ -205,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     const bool posr = NCA == 1 && nf <= 2 * PW;
     uint32_t posw[PW];
     {
-        const uint32_t* pw = reinterpret_cast<const uint32_t*>(a.pair_pos + gp * nf);
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(a.pair_pos + qp);
 #pragma unroll
         for (int w = 0; w < PW; ++w) posw[w] = (has_pair && posr && 2 * w < nf) ? pw[w] : 0u;
     }
@@ -213,11 +225,11 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     const T* Rg;
     const T* RFg;
     if constexpr (sizeof(T) == 8) {
-        Rg = reinterpret_cast<const T*>(a.m.Rd);
-        RFg = reinterpret_cast<const T*>(a.f.Rd);
+        Rg = reinterpret_cast<const T*>(KT.Rd);
+        RFg = reinterpret_cast<const T*>(KT.RFd);
     } else {
-        Rg = reinterpret_cast<const T*>(a.m.Rf);
-        RFg = reinterpret_cast<const T*>(a.f.Rf);
+        Rg = reinterpret_cast<const T*>(KT.Rf);
+        RFg = reinterpret_cast<const T*>(KT.RFf);
     }
     const T* R;
     const T* RF;
@@ -225,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     if constexpr (RS) {
         T* Rs = reinterpret_cast<T*>(smem_raw + L.R);
         T* RFs = reinterpret_cast<T*>(smem_raw + L.RF);
-        pad_copy(Rs, Rg, a.R_rows, len, lenp);
-        if (a.has_rhs) pad_copy(RFs, RFg, a.RF_rows, lenF, lenFp);
+        pad_copy(Rs, Rg, KT.R_rows, len, lenp);
+        if (a.has_rhs) pad_copy(RFs, RFg, KT.RF_rows, lenF, lenFp);
         R = Rs;
         RF = RFs;
         rstride = lenp;
@@ -245,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     if (has_pair) {
         const bool dir = (ibf & kPairDirichlet) != 0;
         const int ib0 = ib % nfB0;
-        const uint16_t* pos = a.pair_pos + gp * nf;
+        const uint16_t* pos = a.pair_pos + qp;
         auto entry = [&](int ps, const T* Kb, const T* kr, int ia0) {
             const T* Rr = R + (ib0 + nfB0 * ia0) * rstride;
             T acc = 0;

@@ -1,0 +1,456 @@
+// Batched assembly of many meshes of uneven size and mixed element order:
+// anifem_batch_* (C ABI).
+//
+// Every mesh is a scalar diffusion problem -div(kappa grad u) = f with P1, P2
+// or P3 elements (fem3Dtet<GRAD FemFix<Pk>, GRAD FemFix<Pk>> with a per-tet
+// SCALAR kappa, load vector fem3Dtet<IDEN P0, IDEN Pk> with a constant f,
+// quadrature order 2 for P1/P2 and 4 for P3; int_tet.h:17-20) and homogeneous
+// Dirichlet on its boundary.  The batch is assembled as one block-diagonal
+// system: the host setup of each mesh (setup.cpp, blocks.cpp) is concatenated
+// with offsets, so a single geometry launch covers all tets and a single
+// row-block launch covers the row blocks of all meshes, each block selecting
+// the reference tensor of its element order (KindTable).  Each mesh's CSR
+// values and rhs land in their own slices of the output arrays and are
+// bitwise identical to assembling that mesh alone.  The launch sequence is
+// captured once in a HIP graph and replayed.
+//
+// Autotuning: the row-block budgets (slots / entries / pairs per block) are
+// chosen per element order by timing candidate budgets on the GPU on a
+// representative mesh of that order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "anifem_common.h"
+#include "assembly_kernels.h"
+#include "setup.h"
+
+namespace anifem {
+int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a);
+int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
+                         const BlockCaps& caps, void* stream);
+}  // namespace anifem
+
+namespace {
+
+constexpr int kKinds = 3;  // P1, P2, P3
+constexpr int kCandidates = 4;
+const anifem::BlockCaps kCandidateCaps[kCandidates] = {
+    {2048, 1024, 128}, {2048, 2048, 256}, {4096, 4096, 256}, {1024, 1024, 256}};
+
+int kind_of(int fem) { return fem - ANIFEM_FEM_P1; }
+int order_of(int fem) { return fem == ANIFEM_FEM_P3 ? 4 : 2; }
+
+anifem_problem_desc problem_of(int fem, int dirichlet) {
+    anifem_problem_desc p{};
+    p.trial = anifem_op{ANIFEM_GRAD, fem, 1};
+    p.test = p.trial;
+    p.order = order_of(fem);
+    p.tensor_type = ANIFEM_TENSOR_SCALAR;
+    p.tensor_storage = ANIFEM_COEF_PER_TET;
+    p.rhs_order = order_of(fem);
+    p.rhs_tensor_type = ANIFEM_TENSOR_SCALAR;
+    p.rhs_tensor_storage = ANIFEM_COEF_CONST;
+    p.dirichlet = dirichlet;
+    return p;
+}
+
+__global__ void batch_status_clear(int* p) {
+    if (threadIdx.x < 4) p[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+struct anifem_batch {
+    int n_meshes = 0;
+    int dirichlet = 0;
+    int device = 0;
+    std::vector<int32_t> fem;
+    std::vector<int64_t> tet_off, node_off, dof_off, nnz_off, rp_off;
+    int64_t n_tets = 0, n_nodes = 0, ndof = 0, nnz = 0, n_blocks = 0;
+    int max_values = 0, max_pairs = 0, max_slots = 0;
+    anifem::BlockCaps caps[kKinds];
+    double tune_ms[kKinds][kCandidates] = {};
+    bool kind_used[kKinds] = {false, false, false};
+    // device
+    double* d_coords = nullptr;
+    int32_t* d_tetnodes = nullptr;
+    anifem::BlockMeta* d_meta = nullptr;
+    int32_t* d_pair_tet = nullptr;
+    uint8_t* d_pair_ib = nullptr;
+    uint16_t* d_pair_pos = nullptr;
+    uint16_t* d_sstart = nullptr;
+    uint16_t* d_pstart = nullptr;
+    int64_t* d_row_ptr = nullptr;  // each mesh's own row_ptr (ndof+1 entries), concatenated
+    int32_t* d_col_idx = nullptr;
+    int* d_status = nullptr;
+    double* d_K = nullptr;
+    // captured launch sequence
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<void*> bufs;
+};
+
+namespace {
+
+template <typename T>
+int upload_b(anifem_batch* b, T** dst, const std::vector<T>& v, hipStream_t s) {
+    ANIFEM_HIP_CHECK(hipMalloc((void**)dst, std::max<size_t>(v.size(), 1) * sizeof(T)));
+    b->bufs.push_back(*dst);
+    if (!v.empty()) ANIFEM_HIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return ANIFEM_OK;
+}
+
+// Times the row-block pass of one mesh under each candidate budget.
+int autotune_kind(anifem_batch* b, int kind, const anifem_batch_mesh& m, hipStream_t s) {
+    const int fem = kind + ANIFEM_FEM_P1;
+    anifem_mesh_desc md{m.n_nodes, m.n_tets, m.coords, m.tets};
+    const anifem_problem_desc prob = problem_of(fem, b->dirichlet);
+    double* kap = nullptr;
+    double* f = nullptr;
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&kap, m.n_tets * sizeof(double)));
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&f, sizeof(double)));
+    std::vector<double> ones(m.n_tets, 1.0);
+    ANIFEM_HIP_CHECK(hipMemcpy(kap, ones.data(), m.n_tets * sizeof(double), hipMemcpyHostToDevice));
+    ANIFEM_HIP_CHECK(hipMemcpy(f, ones.data(), sizeof(double), hipMemcpyHostToDevice));
+    double best = 1e300;
+    int best_i = 1;
+    int rc = ANIFEM_OK;
+    for (int c = 0; c < kCandidates && !rc; ++c) {
+        anifem_assembly* ctx = nullptr;
+        rc = anifem::assembly_create_caps(&ctx, &md, &prob, kCandidateCaps[c], s);
+        if (rc == ANIFEM_ERR_DIMS) {  // a row does not fit this budget
+            rc = ANIFEM_OK;
+            b->tune_ms[kind][c] = -1;
+            continue;
+        }
+        if (rc) break;
+        int64_t ndof = 0, nnz = 0;
+        anifem_assembly_info(ctx, &ndof, &nnz, nullptr, nullptr);
+        double *val = nullptr, *rhs = nullptr;
+        if (hipMalloc((void**)&val, nnz * sizeof(double)) != hipSuccess ||
+            hipMalloc((void**)&rhs, ndof * sizeof(double)) != hipSuccess) {
+            rc = ANIFEM_ERR_NOMEM;
+        }
+        const anifem_tensor D{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_PER_TET, kap};
+        const anifem_tensor F{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_CONST, f};
+        double ms = 0;
+        if (!rc) rc = anifem_assembly_run(ctx, &D, &F, nullptr, ANIFEM_PREC_FP64, val, rhs, nullptr, s);
+        if (!rc) rc = anifem_assembly_enable_timers(ctx, 1);
+        const int reps = 3;
+        for (int r = 0; r < reps && !rc; ++r) {
+            rc = anifem_assembly_run(ctx, &D, &F, nullptr, ANIFEM_PREC_FP64, val, rhs, nullptr, s);
+            anifem_timers t{};
+            anifem_assembly_timers(ctx, &t);
+            ms += t.ms_gather / reps;
+        }
+        if (val) (void)hipFree(val);
+        if (rhs) (void)hipFree(rhs);
+        anifem_assembly_destroy(ctx);
+        b->tune_ms[kind][c] = ms;
+        if (!rc && ms < best) {
+            best = ms;
+            best_i = c;
+        }
+    }
+    (void)hipFree(kap);
+    (void)hipFree(f);
+    b->caps[kind] = kCandidateCaps[best_i];
+    return rc;
+}
+
+int capture(anifem_batch* b, const double* kappa, const double* f, double* val, double* rhs) {
+    using namespace anifem;
+    BlockArgs a{};
+    int rc;
+    const anifem_tensor D{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_PER_TET, kappa};
+    const anifem_tensor F{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_CONST, f};
+    int first = -1;
+    for (int k = 0; k < kKinds; ++k) {
+        if (!b->kind_used[k]) continue;
+        const int fem = k + ANIFEM_FEM_P1;
+        ElemArgs em{}, ef{};
+        if ((rc = prepare_elem_args(anifem_op{ANIFEM_GRAD, fem, 1}, anifem_op{ANIFEM_GRAD, fem, 1}, &D, order_of(fem),
+                                    em)))
+            return rc;
+        if ((rc = prepare_elem_args(anifem_op{ANIFEM_IDEN, ANIFEM_FEM_P0, 1}, anifem_op{ANIFEM_IDEN, fem, 1}, &F,
+                                    order_of(fem), ef)))
+            return rc;
+        a.kinds[k] = KindTable{em.Rd, em.Rf, ef.Rd, ef.Rf, em.nfA0, em.nfB0, em.nfB0, em.nfA0 * em.nfB0, ef.nfB0, 0};
+        a.max_R_rows = std::max(a.max_R_rows, a.kinds[k].R_rows);
+        a.max_RF_rows = std::max(a.max_RF_rows, a.kinds[k].RF_rows);
+        if (first < 0) {
+            first = k;
+            a.m = em;  // the geometry tensors of all kinds coincide: K = |T| kappa PSI^T PSI, K_F = |T| f
+            a.f = ef;
+        }
+    }
+    if (first < 0) return ANIFEM_ERR_ARG;
+    a.n_kinds = kKinds;
+    a.m.coef_n = b->n_tets;
+    a.m.coords = b->d_coords;
+    a.m.tetnodes = b->d_tetnodes;
+    a.m.nodes_stride = b->n_tets;
+    a.m.pos0 = 0;
+    a.f.coef_n = b->n_tets;
+    a.has_rhs = 1;
+    a.rowlen = pad16<double>(9 + 1);
+    a.n_blocks = b->n_blocks;
+    a.meta = b->d_meta;
+    a.pair_tet = b->d_pair_tet;
+    a.pair_ib = b->d_pair_ib;
+    a.pair_pos = b->d_pair_pos;
+    a.sstart = b->d_sstart;
+    a.pstart = b->d_pstart;
+    a.val = val;
+    a.rhs = rhs;
+    a.status = b->d_status;
+    a.max_values = b->max_values;
+    a.max_pairs = b->max_pairs;
+    a.max_slots = b->max_slots;
+    if (a.max_pairs > kBlock) return ANIFEM_ERR_DIMS;
+    int max_lds = 0;
+    ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
+    const size_t lds_staged =
+        block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, a.max_R_rows * pad16<double>(1 * 9),
+                         a.max_RF_rows * pad16<double>(1))
+            .total;
+    const bool staged = lds_staged <= (size_t)max_lds / 2;
+    const size_t lds = staged ? lds_staged
+                              : block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0).total;
+    if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
+
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hipStream_t s = b->cap_stream;
+    ANIFEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, s, b->d_status);
+    const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
+                       b->n_tets);
+    const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
+    if (staged)
+        hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
+                           (const double*)b->d_K);
+    else
+        hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, false>), dim3(grid), dim3(kBlock), lds, s, a,
+                           (const double*)b->d_K);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) return ANIFEM_ERR_HIP;
+    b->graph = g;
+    ANIFEM_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->key[0] = kappa;
+    b->key[1] = f;
+    b->key[2] = val;
+    b->key[3] = rhs;
+    return ANIFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mesh* meshes, int dirichlet,
+                        int autotune, void* stream) {
+    using namespace anifem;
+    if (!out || n_meshes <= 0 || !meshes) return ANIFEM_ERR_ARG;
+    if (dirichlet != ANIFEM_DIRICHLET_NONE && dirichlet != ANIFEM_DIRICHLET_BOUNDARY) return ANIFEM_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::unique_ptr<anifem_batch> b(new anifem_batch());
+    b->n_meshes = n_meshes;
+    b->dirichlet = dirichlet;
+    ANIFEM_HIP_CHECK(hipGetDevice(&b->device));
+    for (int i = 0; i < n_meshes; ++i) {
+        const anifem_batch_mesh& m = meshes[i];
+        if (m.fem < ANIFEM_FEM_P1 || m.fem > ANIFEM_FEM_P3 || m.n_tets <= 0 || m.n_nodes <= 0 || !m.coords || !m.tets)
+            return ANIFEM_ERR_ARG;
+        b->fem.push_back(m.fem);
+        b->kind_used[kind_of(m.fem)] = true;
+    }
+    // budgets per element order: timed candidates on the largest mesh of that
+    // order within 200k tets (else the smallest)
+    for (int k = 0; k < kKinds; ++k) {
+        b->caps[k] = BlockCaps();
+        if (!b->kind_used[k] || !autotune) continue;
+        int pick = -1;
+        for (int i = 0; i < n_meshes; ++i) {
+            if (kind_of(meshes[i].fem) != k) continue;
+            const bool fits = meshes[i].n_tets <= 200000;
+            if (pick < 0) {
+                pick = i;
+                continue;
+            }
+            const bool pfits = meshes[pick].n_tets <= 200000;
+            if ((fits && (!pfits || meshes[i].n_tets > meshes[pick].n_tets)) ||
+                (!fits && !pfits && meshes[i].n_tets < meshes[pick].n_tets))
+                pick = i;
+        }
+        const int rc = autotune_kind(b.get(), k, meshes[pick], s);
+        if (rc) return rc;
+    }
+    // per-mesh setup, concatenated with offsets
+    std::vector<double> coords;
+    std::vector<int32_t> tets;  // global node ids, AoS (after the node reorder)
+    std::vector<BlockMeta> meta;
+    std::vector<int32_t> pair_tet, col_idx;
+    std::vector<uint8_t> pair_ib;
+    std::vector<uint16_t> pair_pos, sstart, pstart;
+    std::vector<int64_t> row_ptr;
+    b->tet_off.assign(1, 0);
+    b->node_off.assign(1, 0);
+    b->dof_off.assign(1, 0);
+    b->nnz_off.assign(1, 0);
+    b->rp_off.assign(1, 0);
+    int64_t pair_off = 0, pos_off = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+        const anifem_batch_mesh& m = meshes[i];
+        const int k = kind_of(m.fem);
+        anifem_mesh_desc md{m.n_nodes, m.n_tets, m.coords, m.tets};
+        const anifem_problem_desc prob = problem_of(m.fem, dirichlet);
+        AssemblySetup S;
+        int rc = build_setup(md, prob, S, b->caps[k]);
+        if (rc) return rc;
+        const int64_t t0 = b->tet_off.back(), n0 = b->node_off.back(), d0 = b->dof_off.back(),
+                      z0 = b->nnz_off.back();
+        coords.insert(coords.end(), m.coords, m.coords + 3 * m.n_nodes);
+        for (int64_t q = 0; q < 4 * S.n_tets; ++q) tets.push_back((int32_t)(S.tets[q] + n0));
+        for (const BlockMeta& M0 : S.blk_meta) {
+            BlockMeta M = M0;
+            M.s0 += z0;
+            M.q0 = pos_off + (M0.p0) * S.nf;
+            M.p0 += pair_off;
+            M.r0 += (int32_t)d0;
+            M.kind = (uint32_t)k;
+            meta.push_back(M);
+        }
+        for (int32_t t : S.pair_tet) pair_tet.push_back((int32_t)(t + t0));
+        pair_ib.insert(pair_ib.end(), S.pair_ib.begin(), S.pair_ib.end());
+        pair_pos.insert(pair_pos.end(), S.pair_pos.begin(), S.pair_pos.end());
+        sstart.insert(sstart.end(), S.sstart.begin(), S.sstart.end());
+        pstart.insert(pstart.end(), S.pstart.begin(), S.pstart.end());
+        row_ptr.insert(row_ptr.end(), S.row_ptr.begin(), S.row_ptr.end());
+        col_idx.insert(col_idx.end(), S.col_idx.begin(), S.col_idx.end());
+        b->max_values = std::max<int>(b->max_values, (int)S.max_block_values);
+        b->max_pairs = std::max<int>(b->max_pairs, (int)S.max_block_pairs);
+        b->max_slots = std::max<int>(b->max_slots, (int)S.max_block_slots);
+        pair_off += (int64_t)S.pair_tet.size();
+        pos_off += (int64_t)S.pair_pos.size();
+        b->tet_off.push_back(t0 + S.n_tets);
+        b->node_off.push_back(n0 + S.n_nodes);
+        b->dof_off.push_back(d0 + S.ndof);
+        b->nnz_off.push_back(z0 + S.nnz);
+        b->rp_off.push_back(b->rp_off.back() + S.ndof + 1);
+        if (b->tet_off.back() >= INT32_MAX || b->dof_off.back() >= INT32_MAX) return ANIFEM_ERR_ARG;
+    }
+    b->n_tets = b->tet_off.back();
+    b->n_nodes = b->node_off.back();
+    b->ndof = b->dof_off.back();
+    b->nnz = b->nnz_off.back();
+    b->n_blocks = (int64_t)meta.size();
+    std::vector<int32_t> tn(4 * (size_t)b->n_tets);
+    for (int64_t t = 0; t < b->n_tets; ++t)
+        for (int v = 0; v < 4; ++v) tn[(size_t)v * b->n_tets + t] = tets[4 * (size_t)t + v];
+    std::vector<int32_t>().swap(tets);
+    anifem_batch* p = b.get();
+    int rc = 0;
+    if ((rc = upload_b(p, &p->d_coords, coords, s))) return rc;
+    if ((rc = upload_b(p, &p->d_tetnodes, tn, s))) return rc;
+    if ((rc = upload_b(p, &p->d_meta, meta, s))) return rc;
+    if ((rc = upload_b(p, &p->d_pair_tet, pair_tet, s))) return rc;
+    if ((rc = upload_b(p, &p->d_pair_ib, pair_ib, s))) return rc;
+    if ((rc = upload_b(p, &p->d_pair_pos, pair_pos, s))) return rc;
+    if ((rc = upload_b(p, &p->d_sstart, sstart, s))) return rc;
+    if ((rc = upload_b(p, &p->d_pstart, pstart, s))) return rc;
+    if ((rc = upload_b(p, &p->d_row_ptr, row_ptr, s))) return rc;
+    if ((rc = upload_b(p, &p->d_col_idx, col_idx, s))) return rc;
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&p->d_status, 4 * sizeof(int)));
+    p->bufs.push_back(p->d_status);
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&p->d_K, (size_t)p->n_tets * pad16<double>(10) * sizeof(double)));
+    p->bufs.push_back(p->d_K);
+    ANIFEM_HIP_CHECK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+    *out = b.release();
+    return ANIFEM_OK;
+}
+
+int anifem_batch_destroy(anifem_batch* b) {
+    if (!b) return ANIFEM_OK;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    for (void* p : b->bufs) (void)hipFree(p);
+    delete b;
+    return ANIFEM_OK;
+}
+
+int anifem_batch_totals(const anifem_batch* b, int64_t* n_tets, int64_t* ndof, int64_t* nnz, int64_t* n_blocks) {
+    if (!b) return ANIFEM_ERR_ARG;
+    if (n_tets) *n_tets = b->n_tets;
+    if (ndof) *ndof = b->ndof;
+    if (nnz) *nnz = b->nnz;
+    if (n_blocks) *n_blocks = b->n_blocks;
+    return ANIFEM_OK;
+}
+
+int anifem_batch_info(const anifem_batch* b, int mesh, int64_t* tet_offset, int64_t* dof_offset, int64_t* nnz_offset,
+                      int64_t* ndof, int64_t* nnz) {
+    if (!b || mesh < 0 || mesh >= b->n_meshes) return ANIFEM_ERR_ARG;
+    if (tet_offset) *tet_offset = b->tet_off[mesh];
+    if (dof_offset) *dof_offset = b->dof_off[mesh];
+    if (nnz_offset) *nnz_offset = b->nnz_off[mesh];
+    if (ndof) *ndof = b->dof_off[mesh + 1] - b->dof_off[mesh];
+    if (nnz) *nnz = b->nnz_off[mesh + 1] - b->nnz_off[mesh];
+    return ANIFEM_OK;
+}
+
+int anifem_batch_csr(const anifem_batch* b, int mesh, const int64_t** row_ptr, const int32_t** col_idx) {
+    if (!b || mesh < 0 || mesh >= b->n_meshes) return ANIFEM_ERR_ARG;
+    if (row_ptr) *row_ptr = b->d_row_ptr + b->rp_off[mesh];
+    if (col_idx) *col_idx = b->d_col_idx + b->nnz_off[mesh];
+    return ANIFEM_OK;
+}
+
+int anifem_batch_tuning(const anifem_batch* b, int fem, int32_t* caps, double* ms) {
+    if (!b || fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
+    const int k = kind_of(fem);
+    if (caps) {
+        caps[0] = b->caps[k].max_slots;
+        caps[1] = b->caps[k].max_values;
+        caps[2] = b->caps[k].max_pairs;
+    }
+    if (ms)
+        for (int c = 0; c < kCandidates; ++c) ms[c] = b->tune_ms[k][c];
+    return ANIFEM_OK;
+}
+
+int anifem_batch_run(anifem_batch* b, const double* kappa, const double* f, double* val, double* rhs,
+                     anifem_status* status, void* stream) {
+    if (!b || !kappa || !f || !val || !rhs) return ANIFEM_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!b->exec || b->key[0] != kappa || b->key[1] != f || b->key[2] != val || b->key[3] != rhs) {
+        const int rc = capture(b, kappa, f, val, rhs);
+        if (rc) return rc;
+    }
+    ANIFEM_HIP_CHECK(hipGraphLaunch(b->exec, s));
+    if (status) {
+        int h[4] = {0, 0, 0, 0};
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(h, b->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+        status->n_nonfinite = h[0];
+        status->n_degenerate = h[1];
+        status->code = h[0] ? ANIFEM_ERR_NONFINITE : ANIFEM_OK;
+        status->reserved = 0;
+        return status->code;
+    }
+    return ANIFEM_OK;
+}
+
+}  // extern "C"

@@ -22,7 +22,11 @@
 
 namespace anifem {
 
-int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj) {
+int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj,
+                 const BlockCaps& caps) {
+    if (caps.max_slots < 1 || caps.max_slots > 65535 || caps.max_values < 1 || caps.max_values > 65535 ||
+        caps.max_pairs < 1 || caps.max_pairs > 256)
+        return ANIFEM_ERR_ARG;
     const int64_t ND = S.ndof;
     const int nf = S.nf, E = S.E;
 
@@ -59,8 +63,8 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
         };
         for (int64_t r = 0; r < ND; ++r) {
             const int64_t rs = S.row_ptr[r + 1] - S.row_ptr[r], rv = row_nv[r], rp = aptr[r + 1] - aptr[r];
-            if (rs > kBlockMaxSlots || rv > kBlockMaxValues || rp > kBlockMaxPairs) return ANIFEM_ERR_DIMS;
-            if (slots && (slots + rs > kBlockMaxSlots || nv + rv > kBlockMaxValues || np + rp > kBlockMaxPairs))
+            if (rs > caps.max_slots || rv > caps.max_values || rp > caps.max_pairs) return ANIFEM_ERR_DIMS;
+            if (slots && (slots + rs > caps.max_slots || nv + rv > caps.max_values || np + rp > caps.max_pairs))
                 close(r);
             slots += rs;
             nv += rv;
@@ -137,11 +141,13 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
             BlockMeta& M = S.blk_meta[k];
             M.s0 = S.blk_s0[b];
             M.p0 = S.blk_p0[b];
+            M.q0 = S.blk_p0[b] * nf;
             M.r0 = S.blk_row0[b];
             M.nr = (uint16_t)(S.blk_row0[b + 1] - S.blk_row0[b]);
             M.ns = (uint16_t)(S.blk_s0[b + 1] - S.blk_s0[b]);
             M.np = (uint16_t)(S.blk_p0[b + 1] - S.blk_p0[b]);
             M.nv = (uint16_t)(S.blk_v0[b + 1] - S.blk_v0[b]);
+            M.kind = 0;
             M.pad = 0;
         }
     }

@@ -106,7 +106,8 @@ void order_permutation(const int64_t g[4], unsigned char perm[4]) {
 
 }  // namespace
 
-int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
+int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
+                const BlockCaps& caps) {
     const int fem = prob.trial.fem;
     if (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
     if (prob.test.fem != fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
@@ -312,7 +313,7 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
                 }
         }
     });
-    return build_blocks(S, aptr, adj);
+    return build_blocks(S, aptr, adj, caps);
 }
 
 }  // namespace anifem

@@ -12,23 +12,29 @@
 
 namespace anifem {
 
-// Row-block budgets of the owner-computes assembly (blocks.cpp).
-constexpr int kBlockMaxSlots = 2048;    // CSR slots per block
-constexpr int kBlockMaxValues = 2048;   // element entries per block
-constexpr int kBlockMaxPairs = 256;     // (row, incident tet) pairs per block
+// Row-block budgets of the owner-computes assembly (blocks.cpp): a block is
+// closed before any of them is exceeded.  Pairs are one per lane of the
+// row-block workgroup; values and slots are 16-bit positions.
+struct BlockCaps {
+    int max_slots = 2048;   // CSR slots per block (<= 65535)
+    int max_values = 2048;  // element entries per block (<= 65535)
+    int max_pairs = 256;    // (row, incident tet) pairs per block (<= workgroup size)
+};
 constexpr uint16_t kPosDropped = 0xFFFF;  // pair_pos of an entry the pattern drops
 constexpr uint8_t kPairDirichlet = 0x80;  // pair_ib flag: the pair's row is Dirichlet
 
-// Bounds of one row block, stored in processing order (one 32-byte record).
+// Bounds of one row block, stored in processing order (one 48-byte record).
 struct BlockMeta {
     int64_t s0;       // first CSR slot
     int64_t p0;       // first pair
+    int64_t q0;       // value positions of pair p0 + i start at pair_pos[q0 + i*nf]
     int32_t r0;       // first row
     uint16_t nr, ns;  // rows, slots
     uint16_t np, nv;  // pairs, values
+    uint32_t kind;    // index of the block's element kind (batched assembly), else 0
     uint32_t pad;
 };
-static_assert(sizeof(BlockMeta) == 32, "block metadata is one 32-byte record");
+static_assert(sizeof(BlockMeta) == 48, "block metadata is one 48-byte record");
 
 struct AssemblySetup {
     int fem = 0, ncomp = 1, nf = 0, E = 0;
@@ -57,8 +63,10 @@ struct AssemblySetup {
     int64_t max_block_values = 0, max_block_pairs = 0, max_block_slots = 0;
 };
 
-int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S);
-int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj);
+int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
+                const BlockCaps& caps = BlockCaps());
+int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj,
+                 const BlockCaps& caps);
 
 // Parallel loop over [0, n) on the host.
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn);
