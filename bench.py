@@ -103,6 +103,81 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "assembly_p2_aniso_64cube_fp64_traffic.txt")
 
 
+def timed_events(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_batched(args, torch):
+    """256 jittered cubes, side n log-uniform in [4, 40], P1/P2/P3 uniform,
+    per-tet kappa = 10^U[-1,1], f = 1, boundary Dirichlet: one graph launch."""
+    import anifem_amd as af
+    rng = np.random.default_rng(3)
+    sides = np.round(np.exp(rng.uniform(np.log(4), np.log(40), 256))).astype(int)
+    fems = rng.integers(af.FEM_P1, af.FEM_P3 + 1, 256)
+    meshes = [af.kuhn_cube(int(n), 0.2, 1000 + i) + (int(f),) for i, (n, f) in enumerate(zip(sides, fems))]
+    t0 = time.perf_counter()
+    B = af.Batch(meshes, autotune=True)
+    setup_s = time.perf_counter() - t0
+    kappa = torch.from_numpy(10 ** rng.uniform(-1, 1, B.n_tets)).cuda()
+    fv = torch.ones(1, dtype=torch.float64, device="cuda")
+    val, rhs, st = B.run(kappa, fv)
+    assert st.code == 0
+    ms = timed_events(lambda: B.run(kappa, fv, val, rhs, status=False), 10, 2, torch)
+    tuning = {f"P{f - 1}": {"budget_slots_entries_pairs": [int(x) for x in B.tuning(f)[0]],
+                            "candidate_ms": [float(x) for x in B.tuning(f)[1]]} for f in (2, 3, 4)}
+    out = {"meshes": 256, "n_tets": B.n_tets, "ndof": B.ndof, "nnz": B.nnz, "row_blocks": B.n_blocks,
+           "ms_per_pass": ms, "element_matrices_per_s": B.n_tets / ms * 1e3, "nnz_per_s": B.nnz / ms * 1e3,
+           "setup_s": setup_s, "autotune": tuning,
+           "note": "one geometry + one row-block launch for all meshes, replayed from a HIP graph (FP64)"}
+    del B, val, rhs
+    return out
+
+
+def bench_elasticity(args, torch):
+    """P2^3 linear elasticity on the 48^3 jittered cube (663,552 tets):
+    per-tet Lame parameters from E log-uniform [1, 1e3], nu ~ U[0.2, 0.45],
+    gravity load, boundary Dirichlet; FP64 and FP32 contraction."""
+    import anifem_amd as af
+    rng = np.random.default_rng(4)
+    coords, tets = af.kuhn_cube(48, 0.2, 1)
+    n = len(tets)
+    E = 10 ** rng.uniform(0, 3, n)
+    nu = rng.uniform(0.2, 0.45, n)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    t0 = time.perf_counter()
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2, 3), order=2, tensor_type=af.TENSOR_LAME,
+                     tensor_storage=af.COEF_PER_TET, rhs_tensor_type=af.TENSOR_GENERAL,
+                     rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
+    setup_s = time.perf_counter() - t0
+    D = torch.from_numpy(np.stack([lam, mu])).cuda()
+    g = torch.tensor([0.0, 0.0, -1.0], dtype=torch.float64, device="cuda")
+    val = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(A.ndof, dtype=torch.float64, device="cuda")
+    _, _, st = A.assemble(D=D, F=g, val=val, rhs=rhs)
+    assert st.code == 0
+    ref = val.clone()
+    res = {"n_tets": n, "ndof": A.ndof, "nnz": A.nnz, "setup_s": setup_s}
+    for prec, name in [(af.PREC_FP64, "fp64"), (af.PREC_FP32, "fp32")]:
+        ms = timed_events(lambda: A.assemble(D=D, F=g, precision=prec, val=val, rhs=rhs, status=False), 10, 2,
+                          torch)
+        res[name] = {"ms_per_pass": ms, "element_matrices_per_s": n / ms * 1e3, "nnz_per_s": A.nnz / ms * 1e3}
+    A.assemble(D=D, F=g, precision=af.PREC_FP32, val=val, rhs=rhs, status=False)
+    torch.cuda.synchronize()
+    res["fp32_max_entry_error_rel_to_max"] = float((val - ref).abs().max() / ref.abs().max())
+    del A, val, rhs, ref
+    return res
+
+
 def read_traffic():
     """HBM bytes per assembly pass from the committed FETCH_SIZE / WRITE_SIZE
     summary (copy-kernel-calibrated), or None when it is absent."""
@@ -261,7 +336,24 @@ def main():
                          "GB_per_s": bytes_ / (ms * 1e-3) / 1e9}
             del A
         res["fp32_speedup_vs_fp64"] = res["fp64"]["ms"] / res["fp32"]["ms"]
+        # per-element error of the FP32 contraction against FP64
+        A64 = torch.empty((n_tets, 10, 10), dtype=torch.float64, device=dev)
+        A32 = torch.empty((n_tets, 10, 10), dtype=torch.float32, device=dev)
+        af.fem3dtet_batch(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, 2, xyz, af.PREC_FP64, out=A64)
+        af.fem3dtet_batch(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, 2, xyz, af.PREC_FP32, out=A32)
+        err = (A32.double() - A64).flatten(1).norm(dim=1) / A64.flatten(1).norm(dim=1)
+        res["fp32_element_rel_frobenius_error"] = {"max": err.max().item(), "median": err.median().item(),
+                                                   "p99.9": torch.quantile(err.float()[::7], 0.999).item()}
+        del A64, A32, err
         out["element_only"] = res
+        # the plain FP64 method on the GPU (reference algorithm per tet, FP64
+        # atomic scatter) against the mixed-precision row-block assembly
+        plain_ms = timed_events(lambda: Asm.assemble_plain(D=Dp, F=f, val=val, rhs=rhs, status=False), 5, 1, torch)
+        out["mixed_precision"] = {
+            "plain_fp64_kernel_ms": plain_ms, "rowblock_fp64_ms": sec64 * 1e3, "rowblock_fp32_ms": sec32 * 1e3,
+            "fp32_speedup_vs_plain_fp64": plain_ms / (sec32 * 1e3),
+            "fp32_speedup_vs_rowblock_fp64": sec64 / sec32,
+            "fp32_element_rel_frobenius_error_max": res["fp32_element_rel_frobenius_error"]["max"]}
         del xyz
         # Jacobi-PCG on the 16^3 P1 Poisson system (rel-res 1e-10)
         c1, t1 = af.kuhn_cube(16, 0.2, 1)
@@ -299,6 +391,8 @@ def main():
             "all_converged": bool(all(r["ok"] for r in rows)),
             "note": "wall time of each solve call (learned includes the network's factor), FP64 true residual"}
         del net, grf
+        out["batched_uneven_256"] = bench_batched(args, torch)
+        out["elasticity_p2vec_48"] = bench_elasticity(args, torch)
 
     if not args.no_cpu_baseline:
         s = Asm.setup()

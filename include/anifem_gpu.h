@@ -212,6 +212,15 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
                         const double* dir_values, int precision, double* csr_val, double* rhs,
                         anifem_status* status, void* stream);
 
+/* The plain FP64 method as a measurement baseline: one lane per tet follows
+ * the reference's algorithm step by step (U = PSI^T Ghat and DU = w|T| D U at
+ * each quadrature point, A = V^T DU; core.inl:218-367), element-level
+ * Dirichlet, then FP64 atomic adds through the element-to-slot map
+ * (assembler.inl:397-481).  Same outputs as anifem_assembly_run up to the
+ * (unfixed) order of the atomic adds.  Scalar GRAD x GRAD problems only. */
+int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, double* csr_val,
+                              double* rhs, anifem_status* status, void* stream);
+
 /* Per-phase GPU timers of the last run, the counterpart of the assembler
  * timers of the reference (assembler.h:234-248, GetTimeEvalLocFunc and
  * friends).  When enabled, run records HIP events on its stream around each

@@ -118,6 +118,8 @@ SIGNATURES = {
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
     "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_run_plain": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _VP,
+                                         ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_enable_timers": (_I32, [_VP, _I32]),
     "anifem_assembly_timers": (_I32, [_VP, ctypes.POINTER(anifem_timers)]),
     "anifem_pcg_jacobi": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_result), _VP]),
@@ -352,6 +354,24 @@ class Assembler:
                                        _stream_handle(stream))
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run")
+        return val, rhs, (st if status else None)
+
+    def assemble_plain(self, D=None, F=None, val=None, rhs=None, status=True, stream=None):
+        """The plain FP64 baseline (anifem_assembly_run_plain)."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if val is None:
+            val = torch.empty(self.nnz, dtype=torch.float64, device=dev)
+        if rhs is None and F is not None:
+            rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
+        tD = anifem_tensor(self.tensor_type, self.tensor_storage, _ptr(D))
+        tF = anifem_tensor(self.rhs_tensor_type, self.rhs_tensor_storage, _ptr(F)) if F is not None else None
+        st = anifem_status()
+        rc = lib().anifem_assembly_run_plain(self._h, ctypes.byref(tD), ctypes.byref(tF) if tF is not None else None,
+                                             _ptr(val), _ptr(rhs), ctypes.byref(st) if status else None,
+                                             _stream_handle(stream))
+        if rc not in (OK, ERR_NONFINITE):
+            raise AnifemError(rc, "anifem_assembly_run_plain")
         return val, rhs, (st if status else None)
 
     def enable_timers(self, on=True):

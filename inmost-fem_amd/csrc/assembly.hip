@@ -42,6 +42,11 @@ struct anifem_assembly {
     int32_t* d_col_idx = nullptr;
     int* d_status = nullptr;          // [0] non-finite, [1] degenerate
     void* d_kbuf = nullptr;           // per-tet geometry tensors of the last run
+    // plain baseline (anifem_assembly_run_plain): slot map, DOFs, flags, tables
+    int32_t* d_slot = nullptr;
+    int32_t* d_dofs = nullptr;
+    uint8_t* d_isdir = nullptr;
+    double* d_plain_tab = nullptr;
     size_t kbuf_bytes = 0;
     bool host_only = false;           // built by anifem_assembly_create_host
     bool timers_on = false;
@@ -138,6 +143,78 @@ int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, hipStream_
     return ANIFEM_ERR_ARG;
 }
 
+// Plain FP64 method, one lane per tet, following the reference's algorithm
+// step by step (core.inl:218-367): J, J^-1, |T|; at each quadrature point the
+// physical gradients U = PSI^T Ghat (poly_*.h), DU = w |T| D U
+// (diff_tensor.h applyDU), A += V^T DU (fusive_AT_mul_B, core.inl:29-60);
+// the load vector with the basis values; element-level Dirichlet
+// (dc_on_dof.h:38-45); then FP64 atomic adds into the CSR through the
+// element-to-slot map (assembler.inl:397-481).  A measurement baseline: the
+// order of the atomic adds is not fixed.
+template <int NF>
+__global__ __launch_bounds__(128) void plain_kernel(anifem::ElemArgs m, anifem::ElemArgs f, int has_rhs,
+                                                    const double* gtab, const double* wq, int q,
+                                                    const double* vtab, const double* wf, int qf,
+                                                    const int32_t* slot, const int32_t* dofs, const uint8_t* isdir,
+                                                    double* val, double* rhs, int64_t n, int* status) {
+    using namespace anifem;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double P[4][3], PSI[9], det;
+    load_tet<true>(m, t, P);
+    tet_geometry(P, PSI, det);
+    const double mes = fabs(det) / 6.0;
+    double A[NF * NF];
+    for (int e = 0; e < NF * NF; ++e) A[e] = 0.0;
+    for (int qn = 0; qn < q; ++qn) {
+        double U[NF][3], DU[NF][3], D[3][3];
+        for (int i = 0; i < NF; ++i)
+            for (int k = 0; k < 3; ++k) {
+                double acc = 0;
+                for (int j = 0; j < 3; ++j) acc += PSI[j + 3 * k] * gtab[(qn * NF + i) * 3 + j];
+                U[i][k] = acc;
+            }
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) D[k][j] = tensor_at(m, t, qn, k, j);
+        const double s = wq[qn] * mes;
+        for (int i = 0; i < NF; ++i)
+            for (int k = 0; k < 3; ++k) DU[i][k] = s * (D[k][0] * U[i][0] + D[k][1] * U[i][1] + D[k][2] * U[i][2]);
+        for (int ia = 0; ia < NF; ++ia)
+            for (int ib = 0; ib < NF; ++ib)
+                A[ib + NF * ia] += U[ib][0] * DU[ia][0] + U[ib][1] * DU[ia][1] + U[ib][2] * DU[ia][2];
+    }
+    double F[NF];
+    for (int i = 0; i < NF; ++i) F[i] = 0.0;
+    if (has_rhs)
+        for (int qn = 0; qn < qf; ++qn) {
+            const double s = wf[qn] * mes * tensor_at(f, t, qn, 0, 0);
+            for (int i = 0; i < NF; ++i) F[i] += s * vtab[qn * NF + i];
+        }
+    // applyDir with value 0 on the flagged local DOFs
+    const int32_t* td = dofs + t * NF;
+    for (int k = 0; k < NF; ++k) {
+        if (!isdir[td[k]]) continue;
+        for (int i = 0; i < NF; ++i) {
+            A[k + NF * i] = 0.0;
+            A[i + NF * k] = 0.0;
+        }
+        A[k + NF * k] = 1.0;
+        F[k] = 0.0;
+    }
+    int bad = 0;
+    const int32_t* sl = slot + t * NF * NF;
+    for (int e = 0; e < NF * NF; ++e) {
+        bad += isfinite(A[e]) ? 0 : 1;
+        if (sl[e] >= 0 && fabs(A[e]) > 1e-100) atomicAdd(val + sl[e], A[e]);
+    }
+    if (has_rhs)
+        for (int i = 0; i < NF; ++i) {
+            bad += isfinite(F[i]) ? 0 : 1;
+            atomicAdd(rhs + td[i], F[i]);
+        }
+    if (bad) atomicAdd(status, bad);
+}
+
 int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
                const anifem::BlockCaps& caps = anifem::BlockCaps()) {
     using namespace anifem;
@@ -220,7 +297,8 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
     if (!ctx->host_only) {
         void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_meta, ctx->d_blk_s0, ctx->d_blk_p0,
                         ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
-                        ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf};
+                        ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
+                        ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -386,6 +464,110 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
         ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
         status->n_nonfinite = h[0];
         status->n_degenerate = h[1];
+        status->code = h[0] ? ANIFEM_ERR_NONFINITE : ANIFEM_OK;
+        status->reserved = 0;
+        return status->code;
+    }
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, double* csr_val,
+                              double* rhs, anifem_status* status, void* stream) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only || !D || !csr_val) return ANIFEM_ERR_ARG;
+    const anifem_problem_desc& P = ctx->prob;
+    // scalar GRAD x GRAD spaces only (the C2 / C3 problems)
+    if (P.trial.op != ANIFEM_GRAD || P.test.op != ANIFEM_GRAD || P.trial.ncomp != 1) return ANIFEM_ERR_ARG;
+    if (D->type != P.tensor_type || D->storage != P.tensor_storage || D->type == ANIFEM_TENSOR_LAME)
+        return ANIFEM_ERR_ARG;
+    const bool do_rhs = F && rhs;
+    if (do_rhs && (F->type != P.rhs_tensor_type || F->storage != P.rhs_tensor_storage)) return ANIFEM_ERR_ARG;
+    AssemblySetup& S = ctx->S;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    const int nf = S.nf;
+    const double *pq, *wq, *pf, *wf;
+    const int q = quadrature(P.order, &pq, &wq);
+    const int qf = quadrature(P.rhs_order, &pf, &wf);
+    if (q < 0 || qf < 0) return ANIFEM_ERR_ORDER;
+    if (!ctx->d_slot) {
+        if ((rc = upload(&ctx->d_slot, S.slot.data(), S.slot.size(), s))) return rc;
+        if ((rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
+        if ((rc = upload(&ctx->d_isdir, S.is_dir.data(), S.is_dir.size(), s))) return rc;
+        std::vector<double> g, v, tab;
+        reference_table(ANIFEM_GRAD, S.fem, q, pq, g);
+        reference_table(ANIFEM_IDEN, S.fem, qf, pf, v);
+        tab = g;
+        tab.insert(tab.end(), wq, wq + q);
+        tab.insert(tab.end(), v.begin(), v.end());
+        tab.insert(tab.end(), wf, wf + qf);
+        if ((rc = upload(&ctx->d_plain_tab, tab.data(), tab.size(), s))) return rc;
+    }
+    const double* gtab = ctx->d_plain_tab;
+    const double* wqd = gtab + (size_t)q * nf * 3;
+    const double* vtab = wqd + q;
+    const double* wfd = vtab + (size_t)qf * nf;
+    ElemArgs m{}, fa{};
+    if ((rc = prepare_elem_args(P.trial, P.test, D, P.order, m))) return rc;
+    m.coef_n = S.n_tets;
+    m.coords = ctx->d_coords;
+    m.tetnodes = ctx->d_tetnodes;
+    m.nodes_stride = S.n_tets;
+    m.pos0 = 0;
+    // the plain method evaluates D at every quadrature point
+    m.q = D->storage == ANIFEM_COEF_PER_POINT ? q : 1;
+    if (do_rhs) {
+        if ((rc = prepare_elem_args(anifem_op{ANIFEM_IDEN, ANIFEM_FEM_P0, 1}, anifem_op{ANIFEM_IDEN, S.fem, 1}, F,
+                                    P.rhs_order, fa)))
+            return rc;
+        fa.coef_n = S.n_tets;
+        fa.q = F->storage == ANIFEM_COEF_PER_POINT ? qf : 1;
+    }
+    if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
+    if ((rc = fill(csr_val, S.nnz, s))) return rc;
+    if (rhs && (rc = fill(rhs, S.ndof, s))) return rc;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
+    if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
+    const unsigned blocks = (unsigned)((S.n_tets + 127) / 128);
+    // tensor_at reads per-point tensors at index qn; constant / per-tet ignore it
+    const int qpts = q;
+    switch (nf) {
+        case 4:
+            hipLaunchKernelGGL(plain_kernel<4>, dim3(blocks), dim3(128), 0, s, m, fa, do_rhs ? 1 : 0, gtab, wqd,
+                               qpts, vtab, wfd, qf, ctx->d_slot, ctx->d_dofs, ctx->d_isdir, csr_val, rhs, S.n_tets,
+                               ctx->d_status);
+            break;
+        case 10:
+            hipLaunchKernelGGL(plain_kernel<10>, dim3(blocks), dim3(128), 0, s, m, fa, do_rhs ? 1 : 0, gtab, wqd,
+                               qpts, vtab, wfd, qf, ctx->d_slot, ctx->d_dofs, ctx->d_isdir, csr_val, rhs, S.n_tets,
+                               ctx->d_status);
+            break;
+        case 20:
+            hipLaunchKernelGGL(plain_kernel<20>, dim3(blocks), dim3(128), 0, s, m, fa, do_rhs ? 1 : 0, gtab, wqd,
+                               qpts, vtab, wfd, qf, ctx->d_slot, ctx->d_dofs, ctx->d_isdir, csr_val, rhs, S.n_tets,
+                               ctx->d_status);
+            break;
+        default:
+            return ANIFEM_ERR_ARG;
+    }
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    if (ctx->timers_on) {
+        ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
+        ANIFEM_HIP_CHECK(hipEventSynchronize(ctx->ev[2]));
+        float t01 = 0, t12 = 0;
+        ANIFEM_HIP_CHECK(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+        ANIFEM_HIP_CHECK(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+        ctx->timers.ms_geometry = t01;
+        ctx->timers.ms_gather = t12;
+        ctx->timers.n_launches = rhs ? 4 : 3;
+        ctx->timers.reserved = 0;
+    }
+    if (status) {
+        int h[4] = {0, 0, 0, 0};
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+        status->n_nonfinite = h[0];
+        status->n_degenerate = 0;
         status->code = h[0] ? ANIFEM_ERR_NONFINITE : ANIFEM_OK;
         status->reserved = 0;
         return status->code;
