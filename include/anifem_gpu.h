@@ -212,6 +212,12 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
                         const double* dir_values, int precision, double* csr_val, double* rhs,
                         anifem_status* status, void* stream);
 
+/* Row-block kernel choice (default 1): a persistent pass (each workgroup
+ * walks many blocks and prefetches the next one's data) where it applies
+ * (scalar spaces, tensor constant over each tet); 0 forces one workgroup per
+ * block.  Both sum in the same order: the results are bitwise identical. */
+int anifem_assembly_set_persistent(anifem_assembly* ctx, int on);
+
 /* The plain FP64 method as a measurement baseline: one lane per tet follows
  * the reference's algorithm step by step (U = PSI^T Ghat and DU = w|T| D U at
  * each quadrature point, A = V^T DU; core.inl:218-367), element-level

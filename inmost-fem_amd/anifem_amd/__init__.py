@@ -120,6 +120,7 @@ SIGNATURES = {
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_run_plain": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _VP,
                                          ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_set_persistent": (_I32, [_VP, _I32]),
     "anifem_assembly_enable_timers": (_I32, [_VP, _I32]),
     "anifem_assembly_timers": (_I32, [_VP, ctypes.POINTER(anifem_timers)]),
     "anifem_pcg_jacobi": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_result), _VP]),
@@ -373,6 +374,9 @@ class Assembler:
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run_plain")
         return val, rhs, (st if status else None)
+
+    def set_persistent(self, on=True):
+        _check(lib().anifem_assembly_set_persistent(self._h, 1 if on else 0), "set_persistent")
 
     def enable_timers(self, on=True):
         _check(lib().anifem_assembly_enable_timers(self._h, 1 if on else 0), "enable_timers")

@@ -50,6 +50,7 @@ struct anifem_assembly {
     size_t kbuf_bytes = 0;
     bool host_only = false;           // built by anifem_assembly_create_host
     bool timers_on = false;
+    bool persistent = true;           // persistent row-block pass where it applies
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     anifem_timers timers{};
 };
@@ -118,6 +119,22 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
     if (S.n_blocks == 0) return ANIFEM_OK;
+    // persistent pass for scalar spaces with one carried point
+    constexpr bool scalar_space = NCA == 1 && NCB == 1;
+    if (scalar_space && ctx->persistent && staged && len == DA0 * DB0 && (!a.has_rhs || lenF == 1) &&
+        a.kinds[0].nf <= 20 && a.n_kinds == 1) {
+        int occ = 0, n_cu = 0;
+        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, T>), kBlock, lds));
+        ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
+        G = std::min<int64_t>(G, S.n_blocks);
+        G = std::max<int64_t>(8, G / 8 * 8);  // whole XCD groups; surplus workgroups exit at once
+        hipLaunchKernelGGL((row_block_persistent<DA0, DB0, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a,
+                           (const T*)Kg);
+        ANIFEM_HIP_CHECK(hipGetLastError());
+        return ANIFEM_OK;
+    }
     // 8 equal XCD ranges of the processing order (padding workgroups exit)
     const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
     if (staged)
@@ -365,6 +382,12 @@ int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, i
     cp(pair_pos, S.pair_pos);
     cp(sstart, S.sstart);
     cp(pstart, S.pstart);
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_set_persistent(anifem_assembly* ctx, int on) {
+    if (!ctx) return ANIFEM_ERR_ARG;
+    ctx->persistent = on != 0;
     return ANIFEM_OK;
 }
 

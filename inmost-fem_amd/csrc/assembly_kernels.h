@@ -276,10 +276,9 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
             // scalar space: entries unrolled, positions from registers
 #pragma unroll
             for (int ia0 = 0; ia0 < 2 * PW; ++ia0) {
-                if (ia0 >= nfA0) break;
                 const uint32_t w = posw[ia0 >> 1];
                 const int ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
-                if (ps != kPosDropped) entry(ps, Kt + cb * len, kreg[0], ia0);
+                if (ia0 < nfA0 && ps != kPosDropped) entry(ps, Kt + cb * len, kreg[0], ia0);
             }
         } else {
 #pragma unroll
@@ -329,6 +328,180 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
             for (int p = pb; p < pe; ++p) sum += fvals[p];
             a.rhs[r0 + u] = sum;
         }
+    }
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
+}  // namespace anifem
+
+namespace anifem {
+
+// Block sequence of a persistent workgroup: workgroup w runs on XCD w % 8, and
+// the blocks of the processing order are dealt to the XCDs in contiguous
+// ranges (the grid is a multiple of 8 and at most n_blocks).
+struct BlockWalk {
+    int64_t next, end, step;
+    __device__ void init(int64_t n_blocks) {
+        const int64_t G = gridDim.x, w = blockIdx.x;
+        const int64_t x = w % 8, l = w / 8;
+        next = n_blocks * x / 8 + l;
+        end = n_blocks * (x + 1) / 8;
+        step = G / 8;
+    }
+    __device__ int64_t take() {
+        const int64_t b = next < end ? next : -1;
+        next += step;
+        return b;
+    }
+};
+
+// One lane's pair of a block: its tet's geometry row (K block and load-vector
+// factor) and its value positions as 16-bit pairs.
+template <int KB, typename T>
+struct PairRegs {
+    int32_t tet;
+    int ibf;
+    uint32_t posw[10];
+    T k[KB];
+    T kf;
+};
+
+// Persistent row-block pass for scalar spaces with one carried point (the
+// matrix tensor constant over each tet): each workgroup stages the reference
+// tensors once and walks its blocks; while block b is evaluated, the next
+// block's pair records and value positions are in flight, and its geometry
+// rows are requested once those have arrived, so the gather latency hides
+// behind phases A and B.  Same arithmetic and summation order as
+// row_block_kernel: the results are bitwise identical.
+template <int DA0, int DB0, typename T>
+__global__ __launch_bounds__(kBlock) void row_block_persistent(BlockArgs a, const T* __restrict__ Kg) {
+    constexpr int KB = DA0 * DB0;
+    constexpr int PW = 10;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lenp = pad16<T>(KB);
+    const int lenFp = a.has_rhs ? pad16<T>(1) : 0;
+    const KindTable& KT = a.kinds[0];
+    const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0;
+    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T), KT.R_rows * lenp,
+                                        a.has_rhs ? KT.RF_rows * lenFp : 0);
+    double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
+    double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
+    uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
+    T* R = reinterpret_cast<T*>(smem_raw + L.R);
+    T* RF = reinterpret_cast<T*>(smem_raw + L.RF);
+    if constexpr (sizeof(T) == 8) {
+        pad_copy(R, reinterpret_cast<const T*>(KT.Rd), KT.R_rows, KB, lenp);
+        if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFd), KT.RF_rows, 1, lenFp);
+    } else {
+        pad_copy(R, reinterpret_cast<const T*>(KT.Rf), KT.R_rows, KB, lenp);
+        if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFf), KT.RF_rows, 1, lenFp);
+    }
+    const int kstride = KB;  // one component block, one point
+
+    BlockWalk W;
+    W.init(a.n_blocks);
+    int64_t b = W.take();
+    if (b < 0) return;  // uniform over the workgroup
+
+    auto load_records = [&](const BlockMeta& Mx, PairRegs<KB, T>& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
+        P.tet = hp ? a.pair_tet[gp] : 0;
+        P.ibf = hp ? a.pair_ib[gp] : 0;
+        const uint32_t* pw =
+            reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * nf);
+#pragma unroll
+        for (int w = 0; w < PW; ++w) P.posw[w] = (hp && 2 * w < nf) ? pw[w] : 0u;
+    };
+    auto load_k = [&](const BlockMeta& Mx, PairRegs<KB, T>& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const T* Kt = Kg + (int64_t)P.tet * a.rowlen;
+#pragma unroll
+        for (int j = 0; j < KB; ++j) P.k[j] = hp ? Kt[j] : T(0);
+        P.kf = (hp && a.has_rhs) ? Kt[kstride] : T(0);
+    };
+
+    BlockMeta M = a.meta[b];
+    PairRegs<KB, T> cur, nxt;
+    load_records(M, cur);
+    load_k(M, cur);
+    int64_t bn = W.take();
+    BlockMeta Mn{};
+    if (bn >= 0) Mn = a.meta[bn];
+    __syncthreads();  // reference tensors in LDS
+
+    int nonfinite = 0;
+    for (;;) {
+        // slot starts of b (overlap phase A), records of the next block
+        block_copy(sst, a.sstart + M.s0, (int)M.ns);
+        if (bn >= 0) load_records(Mn, nxt);
+        const int64_t bnn = W.take();
+        BlockMeta Mnn{};
+        if (bnn >= 0) Mnn = a.meta[bnn];
+
+        // phase A of b
+        if ((int)threadIdx.x < M.np) {
+            const int ib = cur.ibf & 63;
+            const bool dir = (cur.ibf & kPairDirichlet) != 0;
+            const T* Rb = R + (ib % nfB0) * lenp;
+#pragma unroll
+            for (int ia0 = 0; ia0 < 2 * PW; ++ia0) {
+                const uint32_t w = cur.posw[ia0 >> 1];
+                const int ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                if (ia0 < nfA0 && ps != kPosDropped) {
+                    const T* Rr = Rb + nfB0 * ia0 * lenp;
+                    T acc = 0;
+#pragma unroll
+                    for (int j = 0; j < KB; ++j) acc += cur.k[j] * Rr[j];
+                    const double v = dir ? 1.0 : (double)acc;
+                    nonfinite += isfinite(v) ? 0 : 1;
+                    vals[ps] = v;
+                }
+            }
+            if (a.has_rhs) {
+                const T acc = cur.kf * RF[(ib % nfB0) * lenFp];
+                const double fv = dir ? 0.0 : (double)acc;
+                nonfinite += isfinite(fv) ? 0 : 1;
+                fvals[threadIdx.x] = fv;
+            }
+        }
+        // geometry rows of the next block (its records have arrived by now)
+        if (bn >= 0) load_k(Mn, nxt);
+        __syncthreads();
+
+        // phase B of b
+        const int ns = M.ns, nv = M.nv, nr = M.nr, np = M.np;
+        for (int u = threadIdx.x; u < ns; u += kBlock) {
+            const int cb = sst[u];
+            const int ce = u + 1 < ns ? sst[u + 1] : nv;
+            double sum = 0.0;
+            int c = cb;
+            for (; c + 4 <= ce; c += 4) {
+                const double v0 = vals[c], v1 = vals[c + 1], v2 = vals[c + 2], v3 = vals[c + 3];
+                sum += v0;
+                sum += v1;
+                sum += v2;
+                sum += v3;
+            }
+            for (; c < ce; ++c) sum += vals[c];
+            a.val[M.s0 + u] = sum;
+        }
+        if (a.has_rhs) {
+            for (int u = threadIdx.x; u < nr; u += kBlock) {
+                const int pb = a.pstart[M.r0 + u];
+                const int pe = u + 1 < nr ? a.pstart[M.r0 + u + 1] : np;
+                double sum = 0.0;
+                for (int p = pb; p < pe; ++p) sum += fvals[p];
+                a.rhs[M.r0 + u] = sum;
+            }
+        }
+        __syncthreads();
+        if (bn < 0) break;
+        cur = nxt;
+        M = Mn;
+        Mn = Mnn;
+        b = bn;
+        bn = bnn;
     }
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }

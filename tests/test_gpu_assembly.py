@@ -200,3 +200,27 @@ def test_plain_baseline_matches_row_blocks(fem, dirichlet):
     s = Asm.setup()
     S = {"row_ptr": s["row_ptr"]}
     compare(v2.cpu().numpy(), r2.cpu().numpy(), S, v1.cpu().numpy(), r1.cpu().numpy(), tol=1e-12)
+
+
+@pytest.mark.parametrize("fem,dirichlet,n", [(2, 1, 4), (3, 1, 16), (4, 0, 6), (3, 0, 24)])
+def test_persistent_and_per_block_kernels_bitwise_equal(fem, dirichlet, n):
+    """The persistent row-block pass (default for scalar spaces; at n=16/24
+    every workgroup walks several blocks) and the one-workgroup-per-block pass
+    sum in the same order: bitwise equal values and rhs, FP64 and FP32."""
+    rng = np.random.default_rng(n + fem)
+    coords, tets = af.kuhn_cube(n, 0.2, 6)
+    order = 4 if fem == 4 else 2
+    D = aniso(rng, len(tets))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem), order=order, tensor_type=af.TENSOR_SYMMETRIC,
+                       tensor_storage=af.COEF_PER_TET, rhs_order=order, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_PER_TET, dirichlet=dirichlet)
+    Dp = dev(T.pack_sym(D, 3).T.copy())
+    f = dev(rng.uniform(0.5, 2.0, (1, len(tets))))
+    for prec in (af.PREC_FP64, af.PREC_FP32):
+        Asm.set_persistent(True)
+        v1, r1, st1 = Asm.assemble(D=Dp, F=f, precision=prec)
+        Asm.set_persistent(False)
+        v2, r2, st2 = Asm.assemble(D=Dp, F=f, precision=prec)
+        assert st1.code == 0 and st2.code == 0
+        assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
+        assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
