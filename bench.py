@@ -195,7 +195,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="sub-cubes per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true", help="only the headline assembly (no --sections)")
+    ap.add_argument("--sections", default="precision,pcg,batched,elasticity",
+                    help="comma list of extra sections: precision, pcg, batched, elasticity")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--learned-train", type=int, default=2000, help="training systems (learned preconditioner)")
     ap.add_argument("--learned-eval", type=int, default=200, help="held-out systems (learned preconditioner)")
@@ -303,13 +305,14 @@ def main():
         "phase_ms": {"geometry": tm[0], "row_blocks": tm[1], "launches": nlaunch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_kernel<3,3,1,1,double,true>",
+                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_persistent<3,3,double>",
                      "ms_per_launch": ms_pass_events,
                      "algorithmic_bytes_per_pass": alg_bytes,
                      "traffic_source": traffic_src},
     }
 
-    if not args.skip_extras:
+    sections = set() if args.skip_extras else {x.strip() for x in args.sections.split(",") if x.strip()}
+    if "precision" in sections:
         sec32 = timed(af.PREC_FP32, max(5, args.steps // 2), 2)
         out["fp32_contraction"] = {"ms_per_step": sec32 * 1e3, "element_matrices_per_s": n_tets / sec32,
                                    "speedup_vs_fp64": sec64 / sec32}
@@ -355,6 +358,7 @@ def main():
             "fp32_speedup_vs_rowblock_fp64": sec64 / sec32,
             "fp32_element_rel_frobenius_error_max": res["fp32_element_rel_frobenius_error"]["max"]}
         del xyz
+    if "pcg" in sections:
         # Jacobi-PCG on the 16^3 P1 Poisson system (rel-res 1e-10)
         c1, t1 = af.kuhn_cube(16, 0.2, 1)
         A1 = af.Assembler(c1, t1, af.Op(af.GRAD, af.FEM_P1), dirichlet=af.DIRICHLET_BOUNDARY)
@@ -391,7 +395,9 @@ def main():
             "all_converged": bool(all(r["ok"] for r in rows)),
             "note": "wall time of each solve call (learned includes the network's factor), FP64 true residual"}
         del net, grf
+    if "batched" in sections:
         out["batched_uneven_256"] = bench_batched(args, torch)
+    if "elasticity" in sections:
         out["elasticity_p2vec_48"] = bench_elasticity(args, torch)
 
     if not args.no_cpu_baseline:

@@ -299,6 +299,14 @@ int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int32_t* col_id
                         const double* G, const double* GT, const double* b, double* x, double rtol, int64_t maxit,
                         anifem_pcg_result* res, void* stream);
 
+/* PCG with an explicit SPD preconditioner matrix M (z = M r; device CSR
+ * m_row_ptr / m_col_idx / m_val, any pattern), for example M = G G^T from
+ * anifem_gnn_precond.  Stopping rule and reporting as for anifem_pcg_jacobi.
+ * No reference counterpart. */
+int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* val,
+                      const int64_t* m_row_ptr, const int32_t* m_col_idx, const double* m_val, const double* b,
+                      double* x, double rtol, int64_t maxit, anifem_pcg_result* res, void* stream);
+
 /* ---- learned preconditioner ----------------------------------------------
  * No reference counterpart.  A 3-layer message-passing network (32 hidden
  * units, BF16 weights, FP32 accumulation) reads an assembled SPD matrix A and
@@ -322,6 +330,12 @@ int anifem_gnn_factor(anifem_gnn* ctx, const double* a, double* G, double* GT, v
 /* device views of the context's pattern and last factor */
 int anifem_gnn_views(const anifem_gnn* ctx, int64_t* n, const int64_t** row_ptr, const int32_t** col_idx,
                      const double** G, const double** GT);
+/* M = G G^T of the last factor kept in the context (anifem_gnn_factor with
+ * G = NULL), as a device CSR matrix owned by the context.  Its pattern and
+ * term lists are built by anifem_gnn_create; ANIFEM_ERR_NOMEM when M has more
+ * than 2^30 terms and was not built. */
+int anifem_gnn_precond(anifem_gnn* ctx, int64_t* m_nnz, const int64_t** m_row_ptr, const int32_t** m_col_idx,
+                       const double** m_val, void* stream);
 /* forward + loss + gradient for matrix values a; probes: device n x 4 FP32
  * (row-major) or NULL for Rademacher probes from seed; loss (host) may be NULL */
 int anifem_gnn_loss_grad(anifem_gnn* ctx, const double* a, const float* probes, uint64_t seed, float* loss,
@@ -337,10 +351,11 @@ typedef struct anifem_pcg_learned_result {
     int32_t fallback;            /* 1: Jacobi-PCG was used */
     int32_t reserved;
 } anifem_pcg_learned_result;
-/* The network's factor for A (values val on the context's pattern), flexible
- * PCG with it; when its FP64 true residual exceeds rtol (1% slack for the
- * recursive-residual drift) or it fails, Jacobi-PCG from the same initial
- * guess. */
+/* The network's factor G for A (values val on the context's pattern), then
+ * PCG with M = G G^T formed explicitly (anifem_pcg_matrix; flexible PCG with
+ * G and G^T when M was not built); when its FP64 true residual exceeds rtol
+ * (1% slack for the recursive-residual drift) or it fails, Jacobi-PCG from the
+ * same initial guess. */
 int anifem_pcg_learned(anifem_gnn* ctx, const double* val, const double* b, double* x, double rtol, int64_t maxit,
                        anifem_pcg_learned_result* res, void* stream);
 

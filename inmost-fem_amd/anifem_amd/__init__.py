@@ -133,6 +133,8 @@ SIGNATURES = {
     "anifem_batch_run": (_I32, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
                                    ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_pcg_matrix": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
+                                 ctypes.POINTER(anifem_pcg_result), _VP]),
     "anifem_gnn_n_params": (_I32, []),
     "anifem_gnn_create": (_I32, [ctypes.POINTER(_VP), _I64, _VP, _VP, _VP]),
     "anifem_gnn_destroy": (_I32, [_VP]),
@@ -141,6 +143,7 @@ SIGNATURES = {
     "anifem_gnn_get_grad": (_I32, [_VP, _VP]),
     "anifem_gnn_factor": (_I32, [_VP, _VP, _VP, _VP, _VP]),
     "anifem_gnn_views": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_gnn_precond": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "anifem_gnn_loss_grad": (_I32, [_VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
     "anifem_gnn_adam_step": (_I32, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _VP]),
     "anifem_pcg_learned": (_I32, [_VP, _VP, _VP, _VP, _D, _I64, ctypes.POINTER(anifem_pcg_learned_result), _VP]),
@@ -418,6 +421,29 @@ def pcg_factored(n, row_ptr, col_idx, val, G, GT, b, x, rtol=1e-8, maxit=100000,
     return res, rc == OK
 
 
+def copy_to_host(addr, count, dtype):
+    """numpy copy of `count` elements of `dtype` at a device address (for the
+    device views the library returns: G, M = G G^T)."""
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the HIP runtime libanifem_gpu.so links
+    out = np.empty(count, dtype)
+    if hip.hipDeviceSynchronize() != 0 or hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(addr),
+                                                         ctypes.c_size_t(out.nbytes), 2) != 0:
+        raise AnifemError(ERR_HIP, "hipMemcpy")
+    return out
+
+
+def pcg_matrix(n, row_ptr, col_idx, val, m_row_ptr, m_col_idx, m_val, b, x, rtol=1e-8, maxit=100000,
+               stream=None):
+    """PCG with an explicit SPD preconditioner matrix M (z = M r) on device
+    CSR; returns (result, converged)."""
+    res = anifem_pcg_result()
+    a = [p if isinstance(p, int) else _ptr(p) for p in (row_ptr, col_idx, val, m_row_ptr, m_col_idx, m_val)]
+    rc = lib().anifem_pcg_matrix(n, *a, _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(res), _stream_handle(stream))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        raise AnifemError(rc, "anifem_pcg_matrix")
+    return res, rc == OK
+
+
 def grf_sample(xyz, ell, sigma, n_features=512, seed=0, lognormal=True, out=None, stream=None):
     """Gaussian random field (or its exponential) at device points xyz (n x 3, float64)."""
     import torch
@@ -490,8 +516,17 @@ class LearnedPreconditioner:
                "anifem_gnn_views")
         return G.value, GT.value
 
+    def precond(self, stream=None):
+        """M = G G^T of the last factor kept in the context (factor(a) with no
+        G / GT): (m_nnz, row_ptr, col_idx, val) device addresses."""
+        m = ctypes.c_int64()
+        rp, ci, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().anifem_gnn_precond(self._h, ctypes.byref(m), ctypes.byref(rp), ctypes.byref(ci),
+                                        ctypes.byref(v), _stream_handle(stream)), "anifem_gnn_precond")
+        return m.value, rp.value, ci.value, v.value
+
     def solve(self, a, b, x, rtol=1e-8, maxit=100000, stream=None):
-        """Learned-preconditioned flexible PCG with Jacobi-PCG fallback."""
+        """Learned-preconditioned PCG (M = G G^T) with Jacobi-PCG fallback."""
         res = anifem_pcg_learned_result()
         rc = lib().anifem_pcg_learned(self._h, _ptr(a), _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(res),
                                       _stream_handle(stream))

@@ -169,6 +169,18 @@ __global__ void transpose_vals_kernel(int64_t nnz, const int32_t* tslot, const T
     if (s < nnz) vt[s] = v[tslot[s]];
 }
 
+// M(e) = sum over the entry's terms of G_ik G_jk, in ascending k (the two
+// mirror entries (i,j), (j,i) list the same products in the same order, so M
+// is bitwise symmetric)
+__global__ void form_m_kernel(int64_t m_nnz, const int64_t* tp, const int32_t* ta, const int32_t* tb, const double* G,
+                              double* mv) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= m_nnz) return;
+    double s = 0;
+    for (int64_t t = tp[e]; t < tp[e + 1]; ++t) s = fma(G[ta[t]], G[tb[t]], s);
+    mv[e] = s;
+}
+
 // ---- loss and its gradient --------------------------------------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -426,6 +438,12 @@ struct anifem_gnn {
     float *theta = nullptr, *pw = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
     float *partial = nullptr, *loss = nullptr;
     int* scatter = nullptr;  // head gradient index map
+    // M = G G^T as an explicit matrix: its CSR pattern, and per entry the slot
+    // pairs (G_ik, G_jk) of its terms in ascending k
+    int64_t m_nnz = 0;
+    int64_t *m_rp = nullptr, *m_tp = nullptr;
+    int32_t *m_ci = nullptr, *m_ta = nullptr, *m_tb = nullptr;
+    double* m_val = nullptr;
     int64_t nchunk = 0, adam_t = 0;
     std::vector<void*> bufs;
 };
@@ -527,6 +545,63 @@ int anifem_gnn_create(anifem_gnn** out, int64_t n, const int64_t* row_ptr, const
         if (it == e || *it != row[q]) return ANIFEM_ERR_ARG;
         tslot[q] = (int32_t)(it - col_idx);
     }
+    // Symbolic M = G G^T.  G has the lower triangle of the pattern: its column
+    // k holds slots (i, k), i >= k (the transposes of row k's slots (k, i)).
+    // Row i of M gathers column k's entries for every (i, k) in G.
+    std::vector<int64_t> cptr(n + 1, 0);
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t c = 0;
+        for (int64_t q = row_ptr[k]; q < row_ptr[k + 1]; ++q) c += col_idx[q] >= k;
+        cptr[k + 1] = cptr[k] + c;
+    }
+    std::vector<int32_t> cslot(cptr[n]), crow(cptr[n]);
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t o = cptr[k];
+        for (int64_t q = row_ptr[k]; q < row_ptr[k + 1]; ++q)
+            if (col_idx[q] >= k) {
+                cslot[o] = tslot[q];
+                crow[o++] = col_idx[q];
+            }
+    }
+    int64_t n_terms = 0;
+    for (int64_t k = 0; k < n; ++k) n_terms += (cptr[k + 1] - cptr[k]) * (cptr[k + 1] - cptr[k]);
+    constexpr int64_t kMaxTerms = int64_t(1) << 30;  // beyond this M is not formed
+    std::vector<int64_t> m_rp, m_tp;
+    std::vector<int32_t> m_ci, m_ta, m_tb;
+    if (n_terms <= kMaxTerms) {
+        m_rp.assign(n + 1, 0);
+        m_ta.reserve(n_terms);
+        m_tb.reserve(n_terms);
+        m_tp.push_back(0);
+        struct Term {
+            int32_t j, ta, tb;
+        };
+        std::vector<Term> terms;
+        for (int64_t i = 0; i < n; ++i) {
+            terms.clear();
+            for (int64_t q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+                const int32_t k = col_idx[q];
+                if (k > i) break;  // columns ascend: (i, k) in G for k <= i, in ascending k
+                for (int64_t o = cptr[k]; o < cptr[k + 1]; ++o) terms.push_back({crow[o], (int32_t)q, cslot[o]});
+            }
+            std::stable_sort(terms.begin(), terms.end(), [](const Term& a, const Term& b) { return a.j < b.j; });
+            for (size_t u = 0; u < terms.size(); ++u) {
+                if (u == 0 || terms[u].j != terms[u - 1].j) {
+                    if (u) m_tp.push_back((int64_t)m_ta.size());
+                    m_ci.push_back(terms[u].j);
+                }
+                m_ta.push_back(terms[u].ta);
+                m_tb.push_back(terms[u].tb);
+            }
+            if (!terms.empty()) m_tp.push_back((int64_t)m_ta.size());
+            m_rp[i + 1] = (int64_t)m_ci.size();
+        }
+        g->m_nnz = (int64_t)m_ci.size();
+        if (g->m_nnz >= INT32_MAX) {
+            g->m_nnz = 0;
+            m_ci.clear();
+        }
+    }
     anifem_gnn* p = g.get();
     int rc = 0;
 #define ALLOC(ptr, cnt) \
@@ -575,11 +650,26 @@ int anifem_gnn_create(anifem_gnn** out, int64_t n, const int64_t* row_ptr, const
     g->nchunk = (n + CHUNK - 1) / CHUNK;
     ALLOC(partial, g->nchunk * (H * H + H) + 1024);
     ALLOC(loss, 1);
+    if (g->m_nnz > 0) {
+        ALLOC(m_rp, n + 1);
+        ALLOC(m_tp, g->m_nnz + 1);
+        ALLOC(m_ci, g->m_nnz);
+        ALLOC(m_ta, m_ta.size());
+        ALLOC(m_tb, m_tb.size());
+        ALLOC(m_val, g->m_nnz);
+    }
 #undef ALLOC
     ANIFEM_HIP_CHECK(hipMemcpyAsync(g->rp, row_ptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     ANIFEM_HIP_CHECK(hipMemcpyAsync(g->ci, col_idx, nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
     ANIFEM_HIP_CHECK(hipMemcpyAsync(g->row, row.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
     ANIFEM_HIP_CHECK(hipMemcpyAsync(g->tslot, tslot.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (g->m_nnz > 0) {
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_rp, m_rp.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_tp, m_tp.data(), m_tp.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_ci, m_ci.data(), m_ci.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_ta, m_ta.data(), m_ta.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_tb, m_tb.data(), m_tb.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->theta, 0, NPARAM * sizeof(float), s));
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->pw, 0, NPARAM * sizeof(float), s));
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->grad, 0, NPARAM * sizeof(float), s));
@@ -640,6 +730,21 @@ int anifem_gnn_views(const anifem_gnn* g, int64_t* n, const int64_t** row_ptr, c
     if (col_idx) *col_idx = g->ci;
     if (G) *G = g->G64;
     if (GT) *GT = g->GT64;
+    return ANIFEM_OK;
+}
+
+int anifem_gnn_precond(anifem_gnn* g, int64_t* m_nnz, const int64_t** m_row_ptr, const int32_t** m_col_idx,
+                       const double** m_val, void* stream) {
+    if (!g) return ANIFEM_ERR_ARG;
+    if (g->m_nnz <= 0) return ANIFEM_ERR_NOMEM;  // M was not formed (more than 2^30 terms)
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(form_m_kernel, dim3(nb(g->m_nnz)), dim3(NT), 0, s, g->m_nnz, g->m_tp, g->m_ta, g->m_tb,
+                       (const double*)g->G64, g->m_val);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    if (m_nnz) *m_nnz = g->m_nnz;
+    if (m_row_ptr) *m_row_ptr = g->m_rp;
+    if (m_col_idx) *m_col_idx = g->m_ci;
+    if (m_val) *m_val = g->m_val;
     return ANIFEM_OK;
 }
 

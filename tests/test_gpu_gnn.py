@@ -1,5 +1,5 @@
 """Learned preconditioner on the GPU (anifem_gnn_*, anifem_pcg_factored,
-anifem_pcg_learned, anifem_grf_sample) against the float64 CPU restatement
+anifem_pcg_matrix, anifem_pcg_learned, anifem_grf_sample) against the float64 CPU restatement
 tests/gnn_oracle.py and against Jacobi-PCG."""
 import numpy as np
 import pytest
@@ -96,6 +96,61 @@ def test_zero_heads_match_jacobi_pcg():
     assert abs(rf.iterations - rj.iterations) <= 1
     assert rf.true_relres <= 1e-10 * 1.01
     assert torch.max(torch.abs(x - xj)).item() <= 1e-8 * torch.max(torch.abs(xj)).item()
+
+
+def test_precond_matrix_is_G_GT():
+    A, s, val, _ = gpu_system(n=5, seed=8)
+    net = af.LearnedPreconditioner(s["row_ptr"], s["col_idx"])
+    net.set_params(random_params(net, 9))
+    G_addr, _ = net.factor(val)
+    m_nnz, mrp, mci, mv = net.precond()
+    n = A.ndof
+    M_rp = af.copy_to_host(mrp, n + 1, np.int64)
+    M_ci = af.copy_to_host(mci, m_nnz, np.int32)
+    M_v = af.copy_to_host(mv, m_nnz, np.float64)
+    G = af.copy_to_host(G_addr, A.nnz, np.float64)
+    g = O.Graph(s["row_ptr"], s["col_idx"], val.cpu().numpy())
+    Gd = np.zeros((n, n))
+    Gd[g.row, g.ci] = G
+    Md = Gd @ Gd.T
+    # pattern: exactly the pairs (i, j) that share a column k of G's pattern
+    assert np.all(G[g.row >= g.ci] != 0)
+    S = np.zeros((n, n), np.int64)
+    S[g.row[g.row >= g.ci], g.ci[g.row >= g.ci]] = 1
+    P = (S @ S.T) > 0
+    Mh = np.zeros((n, n))
+    seen = np.zeros((n, n), bool)
+    for i in range(n):
+        cols = M_ci[M_rp[i]:M_rp[i + 1]]
+        assert np.all(np.diff(cols) > 0)
+        Mh[i, cols] = M_v[M_rp[i]:M_rp[i + 1]]
+        seen[i, cols] = True
+    np.testing.assert_array_equal(seen, P)
+    assert np.max(np.abs(Mh - Md)) <= 1e-14 * np.max(np.abs(Md))
+    np.testing.assert_array_equal(Mh, Mh.T)  # bitwise symmetric
+
+
+def test_pcg_matrix_matches_factored():
+    import torch
+    A, s, val, rhs = gpu_system(n=8, seed=10)
+    net = af.LearnedPreconditioner(s["row_ptr"], s["col_idx"])
+    net.set_params(random_params(net, 11, head=0.05))
+    G, GT = net.factor(val)
+    m_nnz, mrp, mci, mv = net.precond()
+    rp, ci = A.csr()
+    x = torch.zeros(A.ndof, dtype=torch.float64, device="cuda")
+    rf, okf = af.pcg_factored(A.ndof, rp, ci, val, G, GT, rhs, x, rtol=1e-10)
+    xf = x.clone()
+    x.zero_()
+    rm, okm = af.pcg_matrix(A.ndof, rp, ci, val, mrp, mci, mv, rhs, x, rtol=1e-10)
+    assert okf and okm
+    assert abs(rm.iterations - rf.iterations) <= 2, (rm.iterations, rf.iterations)
+    assert rm.true_relres <= 1.01e-10
+    assert torch.max(torch.abs(x - xf)).item() <= 1e-8 * torch.max(torch.abs(xf)).item()
+    # repeated solves are bitwise identical
+    x2 = torch.zeros_like(x)
+    af.pcg_matrix(A.ndof, rp, ci, val, mrp, mci, mv, rhs, x2, rtol=1e-10)
+    assert torch.equal(x, x2)
 
 
 def test_training_reduces_loss_and_iterations():

@@ -51,3 +51,61 @@ def test_pcg_zero_rhs_and_maxit():
     b = torch.ones_like(b)
     res, ok = af.pcg_jacobi(Asm.ndof, rp, ci, val, b, x, rtol=1e-14, maxit=3)
     assert not ok and res.iterations == 3
+
+
+def test_pcg_matrix_with_diagonal_m_is_jacobi():
+    """M = diag(a_ii)^-1 as an explicit matrix: the same method as Jacobi-PCG."""
+    import torch
+    coords, tets = af.kuhn_cube(10, 0.2, 3)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), order=2, tensor_type=af.TENSOR_NULL,
+                       tensor_storage=af.COEF_CONST, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
+    val, rhs, st = Asm.assemble(D=None, F=torch.ones(1, dtype=torch.float64, device="cuda"))
+    assert st.code == 0
+    rp, ci = Asm.csr()
+    s = Asm.setup()
+    row = np.repeat(np.arange(Asm.ndof), np.diff(s["row_ptr"]))
+    vh = val.cpu().numpy()
+    diag = np.zeros(Asm.ndof)
+    isd = row == s["col_idx"]
+    diag[row[isd]] = vh[isd]
+    mrp = torch.arange(Asm.ndof + 1, dtype=torch.int64, device="cuda")
+    mci = torch.arange(Asm.ndof, dtype=torch.int32, device="cuda")
+    mv = torch.from_numpy(1.0 / diag).cuda()
+    x = torch.zeros(Asm.ndof, dtype=torch.float64, device="cuda")
+    rj, okj = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-10, maxit=5000)
+    xj = x.clone()
+    x.zero_()
+    rm, okm = af.pcg_matrix(Asm.ndof, rp, ci, val, mrp, mci, mv, rhs, x, rtol=1e-10, maxit=5000)
+    assert okj and okm
+    assert abs(rm.iterations - rj.iterations) <= 1, (rm.iterations, rj.iterations)
+    assert rm.true_relres <= 1.01e-10 and rj.true_relres <= 1.01e-10
+    assert torch.max(torch.abs(x - xj)).item() <= 1e-8 * torch.max(torch.abs(xj)).item()
+    # maxit bounds and the zero right-hand side
+    x.zero_()
+    rm, okm = af.pcg_matrix(Asm.ndof, rp, ci, val, mrp, mci, mv, rhs, x, rtol=1e-14, maxit=5)
+    assert not okm and rm.iterations == 5
+    x.zero_()
+    rm, okm = af.pcg_matrix(Asm.ndof, rp, ci, val, mrp, mci, mv, torch.zeros_like(rhs), x, rtol=1e-8)
+    assert okm and rm.iterations == 0
+
+
+def test_pcg_iteration_counts_across_poll_chunks():
+    """Convergence inside any iteration of a poll chunk reports that iteration
+    (maxit just below / at the converged count)."""
+    import torch
+    coords, tets = af.kuhn_cube(8, 0.2, 4)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P1), dirichlet=af.DIRICHLET_BOUNDARY,
+                       rhs_tensor_type=af.TENSOR_SCALAR, rhs_tensor_storage=af.COEF_CONST)
+    val, rhs, _ = Asm.assemble(D=None, F=torch.ones(1, dtype=torch.float64, device="cuda"))
+    rp, ci = Asm.csr()
+    x = torch.zeros(Asm.ndof, dtype=torch.float64, device="cuda")
+    r0, ok0 = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9)
+    assert ok0
+    k = r0.iterations
+    x.zero_()
+    r1, ok1 = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9, maxit=k)
+    assert ok1 and r1.iterations == k
+    x.zero_()
+    r2, ok2 = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9, maxit=k - 1)
+    assert not ok2 and r2.iterations == k - 1
