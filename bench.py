@@ -165,7 +165,7 @@ def bench_elasticity(args, torch):
     rhs = torch.empty(A.ndof, dtype=torch.float64, device="cuda")
     _, _, st = A.assemble(D=D, F=g, val=val, rhs=rhs)
     assert st.code == 0
-    ref = val.clone()
+    ref, rhs64 = val.clone(), rhs.clone()
     res = {"n_tets": n, "ndof": A.ndof, "nnz": A.nnz, "setup_s": setup_s}
     for prec, name in [(af.PREC_FP64, "fp64"), (af.PREC_FP32, "fp32")]:
         ms = timed_events(lambda: A.assemble(D=D, F=g, precision=prec, val=val, rhs=rhs, status=False), 10, 2,
@@ -174,7 +174,29 @@ def bench_elasticity(args, torch):
     A.assemble(D=D, F=g, precision=af.PREC_FP32, val=val, rhs=rhs, status=False)
     torch.cuda.synchronize()
     res["fp32_max_entry_error_rel_to_max"] = float((val - ref).abs().max() / ref.abs().max())
-    del A, val, rhs, ref
+    # element matrices (30 x 30) of every tet: FP32 contraction against FP64
+    xyz = torch.from_numpy(np.ascontiguousarray(coords[tets].reshape(n, 12).T)).to("cuda")
+    op = af.Op(af.GRAD, af.FEM_P2, 3)
+    E64 = af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyz, af.PREC_FP64)
+    E32 = af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyz, af.PREC_FP32)
+    err = (E32.double() - E64).flatten(1).norm(dim=1) / E64.flatten(1).norm(dim=1)
+    res["fp32_element_rel_frobenius_error"] = {"max": err.max().item(), "median": err.median().item()}
+    del E64, E32, err, xyz
+    # displacements: Jacobi-PCG to rel-res 1e-10 on the FP64-assembled system
+    # and on the system (matrix and load) assembled with the FP32 contraction
+    rp, ci = A.csr()
+    u64 = torch.zeros(A.ndof, dtype=torch.float64, device="cuda")
+    u32 = torch.zeros_like(u64)
+    t0 = time.perf_counter()
+    r64, ok64 = af.pcg_jacobi(A.ndof, rp, ci, ref, rhs64, u64, rtol=1e-10, maxit=100000)
+    r32, ok32 = af.pcg_jacobi(A.ndof, rp, ci, val, rhs, u32, rtol=1e-10, maxit=100000)
+    torch.cuda.synchronize()
+    res["displacement"] = {
+        "rtol": 1e-10, "converged": bool(ok64 and ok32), "iterations_fp64": r64.iterations,
+        "iterations_fp32": r32.iterations, "solve_seconds_both": time.perf_counter() - t0,
+        "fp32_rel_l2_error": float((u32 - u64).norm() / u64.norm()),
+        "fp32_max_error_rel_to_max": float((u32 - u64).abs().max() / u64.abs().max())}
+    del A, val, rhs, ref, rhs64, u64, u32
     return res
 
 
@@ -387,6 +409,7 @@ def main():
             "loss_last_100": float(losses[-100:].mean()), "eval_systems": len(rows), "rtol": 1e-8,
             "jacobi_ms_mean": float(jm.mean()), "learned_ms_mean": float(lm.mean()),
             "speedup_mean": float(jm.mean() / lm.mean()),
+            "factor_ms_mean": float(np.mean([r["factor_ms"] for r in rows])),
             "jacobi_iterations_mean": float(np.mean([r["jacobi_it"] for r in rows])),
             "learned_iterations_mean": float(np.mean([r["learned_it"] for r in rows])),
             "learned_faster_fraction": float(np.mean(lm < jm)),

@@ -299,7 +299,8 @@ int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int32_t* col_id
                         const double* G, const double* GT, const double* b, double* x, double rtol, int64_t maxit,
                         anifem_pcg_result* res, void* stream);
 
-/* PCG with an explicit SPD preconditioner matrix M (z = M r; device CSR
+/* Flexible PCG (Polak-Ribiere beta) with an explicit SPD preconditioner
+ * matrix M (z = M r; device CSR
  * m_row_ptr / m_col_idx / m_val, any pattern), for example M = G G^T from
  * anifem_gnn_precond.  Stopping rule and reporting as for anifem_pcg_jacobi.
  * No reference counterpart. */
@@ -352,8 +353,8 @@ typedef struct anifem_pcg_learned_result {
     int32_t reserved;
 } anifem_pcg_learned_result;
 /* The network's factor G for A (values val on the context's pattern), then
- * PCG with M = G G^T formed explicitly (anifem_pcg_matrix; flexible PCG with
- * G and G^T when M was not built); when its FP64 true residual exceeds rtol
+ * flexible PCG with M = G G^T formed explicitly (anifem_pcg_matrix; with G
+ * and G^T applied in turn, anifem_pcg_factored, when M was not built); when its FP64 true residual exceeds rtol
  * (1% slack for the recursive-residual drift) or it fails, Jacobi-PCG from the
  * same initial guess. */
 int anifem_pcg_learned(anifem_gnn* ctx, const double* val, const double* b, double* x, double rtol, int64_t maxit,

@@ -1,6 +1,6 @@
 """Learned preconditioner workflow on the GPU: Gaussian-random-field diffusion
 systems produced by this assembler, training of the message-passing network
-(anifem_gnn_* kernels) and flexible PCG against Jacobi-PCG.
+(anifem_gnn_* kernels) and PCG with M = G G^T against Jacobi-PCG.
 
 System family (one mesh, many coefficients): P1 diffusion on the unit cube
 split into n^3 Kuhn sub-cubes (jittered), kappa = exp(g) per tet with g a
@@ -69,8 +69,9 @@ def train(systems, seeds, lr=1e-3, epochs=1, init_seed=0, log=None):
 
 
 def compare(systems, net, seeds, rtol=1e-8, maxit=20000):
-    """Per held-out seed: Jacobi-PCG and learned FPCG times (ms, synchronized
-    wall clock of the solve calls), iterations, true residuals, fallbacks."""
+    """Per held-out seed: Jacobi-PCG and learned PCG times (ms, synchronized
+    wall clock of the solve calls), iterations, true residuals, fallbacks, and
+    the time of the network's factor plus M = G G^T alone (factor_ms)."""
     import torch
     val = torch.empty(systems.asm.nnz, dtype=torch.float64, device=systems.dev)
     rhs = torch.empty(systems.n, dtype=torch.float64, device=systems.dev)
@@ -90,7 +91,13 @@ def compare(systems, net, seeds, rtol=1e-8, maxit=20000):
         rl, okl = net.solve(val, rhs, x, rtol=rtol, maxit=maxit)
         torch.cuda.synchronize()
         tl = (time.perf_counter() - t) * 1e3
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        net.factor(val)
+        net.precond()
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t) * 1e3
         rows.append(dict(seed=int(sd), jacobi_ms=tj, jacobi_it=rj.iterations, jacobi_true=rj.true_relres,
                          learned_ms=tl, learned_it=rl.iterations, learned_true=rl.true_relres,
-                         fallback=rl.fallback, ok=bool(okj and okl)))
+                         fallback=rl.fallback, factor_ms=tf, ok=bool(okj and okl)))
     return rows

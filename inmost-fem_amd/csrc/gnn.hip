@@ -34,7 +34,7 @@ constexpr int NL = 3;        // message-passing layers
 constexpr int P = 4;         // probe vectors per loss evaluation
 constexpr int NT = 256;      // threads per workgroup
 constexpr int NODES = NT / H;  // nodes per workgroup in node x feature kernels
-constexpr int CHUNK = 512;     // nodes per partial sum of a weight gradient
+constexpr int CHUNK = 128;     // nodes per partial sum of a weight gradient
 
 // flat parameter layout (tests/gnn_oracle.py PARAM_SHAPES)
 constexpr int OFF_WIN = 0, OFF_BIN = OFF_WIN + H * FIN;
@@ -101,6 +101,25 @@ __global__ void input_kernel(int64_t n, const float* x, const float* pw, float* 
     h0[i] = fmaxf(z, 0.0f);
 }
 
+// sum_{s in [s0, s1)} at_s in[ci_s][k] added to acc in ascending s, with four
+// gathers in flight at a time
+__device__ __forceinline__ float gather_rows(float acc, int64_t s0, int64_t s1, const int32_t* ci, const float* at,
+                                             const float* in, int k) {
+    int64_t s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        const int32_t c0 = ci[s], c1 = ci[s + 1], c2 = ci[s + 2], c3 = ci[s + 3];
+        const float a0 = at[s], a1 = at[s + 1], a2 = at[s + 2], a3 = at[s + 3];
+        const float v0 = in[(int64_t)c0 * H + k], v1 = in[(int64_t)c1 * H + k];
+        const float v2 = in[(int64_t)c2 * H + k], v3 = in[(int64_t)c3 * H + k];
+        acc += a0 * v0;
+        acc += a1 * v1;
+        acc += a2 * v2;
+        acc += a3 * v3;
+    }
+    for (; s < s1; ++s) acc += at[s] * in[(int64_t)ci[s] * H + k];
+    return acc;
+}
+
 // out[r][k] = sum_{s in row r} at_s in[ci_s][k] (+ add[r][k])
 __global__ void spmm_kernel(int64_t n, const int64_t* rp, const int32_t* ci, const float* at, const float* in,
                             const float* add, float* out) {
@@ -108,31 +127,63 @@ __global__ void spmm_kernel(int64_t n, const int64_t* rp, const int32_t* ci, con
     if (i >= n * H) return;
     const int64_t r = i / H;
     const int k = (int)(i % H);
-    float acc = add ? add[i] : 0.0f;
-    for (int64_t s = rp[r]; s < rp[r + 1]; ++s) acc += at[s] * in[(int64_t)ci[s] * H + k];
-    out[i] = acc;
+    out[i] = gather_rows(add ? add[i] : 0.0f, rp[r], rp[r + 1], ci, at, in, k);
 }
 
-// z = W_s h + W_m m + b; hn = relu(z) + h   (NODES nodes per workgroup)
-__global__ __launch_bounds__(NT) void layer_kernel(int64_t n, const float* h, const float* m, const float* pw, int l,
-                                                   float* z, float* hn) {
-    __shared__ float Ws[H * H], Wm[H * H], hs[NODES][H], ms[NODES][H];
+// One message-passing layer for LN nodes per workgroup:
+//   m = At h;  z = W_s h + W_m m + b;  hn = relu(z) + h
+// Lane (g, k) = (threadIdx / H, threadIdx % H) owns feature k of nodes
+// g, g + 8, ...  The weights sit in LDS transposed (Wt[j][k] = W[k][j]), so a
+// wave's 32 lanes of one node read 32 consecutive words; h and m rows are
+// broadcasts.
+constexpr int LN = 32;
+__global__ __launch_bounds__(NT) void mp_layer_kernel(int64_t n, const int64_t* rp, const int32_t* ci,
+                                                      const float* at, const float* h, const float* pw, int l,
+                                                      float* m, float* z, float* hn) {
+    __shared__ float Wst[H][H + 1], Wmt[H][H + 1], hs[LN][H], ms[LN][H];
     for (int i = threadIdx.x; i < H * H; i += NT) {
-        Ws[i] = pw[off_ws(l) + i];
-        Wm[i] = pw[off_wm(l) + i];
+        const int k = i / H, j = i % H;
+        Wst[j][k] = pw[off_ws(l) + i];
+        Wmt[j][k] = pw[off_wm(l) + i];
     }
-    const int ln = threadIdx.x / H, k = threadIdx.x % H;
-    const int64_t r = (int64_t)blockIdx.x * NODES + ln;
-    hs[ln][k] = r < n ? h[r * H + k] : 0.0f;
-    ms[ln][k] = r < n ? m[r * H + k] : 0.0f;
+    constexpr int G = NT / H, PER = LN / G;
+    const int k = threadIdx.x % H, g = threadIdx.x / H;
+    const int64_t base = (int64_t)blockIdx.x * LN;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int ln = g + G * t;
+        const int64_t r = base + ln;
+        float hv = 0, mv = 0;
+        if (r < n) {
+            hv = h[r * H + k];
+            mv = gather_rows(0.0f, rp[r], rp[r + 1], ci, at, h, k);
+            m[r * H + k] = mv;
+        }
+        hs[ln][k] = hv;
+        ms[ln][k] = mv;
+    }
     __syncthreads();
-    if (r >= n) return;
-    float acc = pw[off_b(l) + k];
-#pragma unroll 8
-    for (int j = 0; j < H; ++j) acc += Ws[k * H + j] * hs[ln][j] + Wm[k * H + j] * ms[ln][j];
-    z[r * H + k] = acc;
-    hn[r * H + k] = fmaxf(acc, 0.0f) + hs[ln][k];
+    const float b = pw[off_b(l) + k];
+    float acc[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) acc[t] = b;
+#pragma unroll 4
+    for (int j = 0; j < H; ++j) {
+        const float ws = Wst[j][k], wm = Wmt[j][k];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) acc[t] += ws * hs[g + G * t][j] + wm * ms[g + G * t][j];
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int ln = g + G * t;
+        const int64_t r = base + ln;
+        if (r < n) {
+            z[r * H + k] = acc[t];
+            hn[r * H + k] = fmaxf(acc[t], 0.0f) + hs[ln][k];
+        }
+    }
 }
+
 
 __global__ void head_edge_kernel(int64_t nnz, const int32_t* row, const int32_t* ci, const float* at, const float* h,
                                  const float* pw, float* Lv) {
@@ -331,18 +382,28 @@ __global__ void relu_mask_kernel(int64_t n, const float* dh, const float* z, flo
 // with sums, of A[r][k]: partial[blk][k*KB + j], partial[blk][H*KB + k].
 __global__ __launch_bounds__(NT) void outer_partial_kernel(int64_t n, const float* A, const float* B, int KB,
                                                            int sums, float* partial) {
+    // the chunk's rows of A (CHUNK x H) and B (CHUNK x KB, KB <= H) in LDS;
+    // lane e = k * KB + j then reads A[r][k] as a broadcast and B[r][j] from
+    // consecutive words
+    __shared__ float As[CHUNK][H], Bs[CHUNK][H];
     const int64_t r0 = (int64_t)blockIdx.x * CHUNK;
-    const int64_t r1 = std::min<int64_t>(n, r0 + CHUNK);
+    const int rows = (int)std::min<int64_t>(CHUNK, n - r0);
+    for (int i = threadIdx.x; i < CHUNK * H; i += NT) {
+        const int r = i / H, c = i % H;
+        As[r][c] = r < rows ? A[(r0 + r) * H + c] : 0.0f;
+        if (c < KB) Bs[r][c] = r < rows ? B[(r0 + r) * KB + c] : 0.0f;
+    }
+    __syncthreads();
     const int len = H * KB + (sums ? H : 0);
     float* out = partial + (int64_t)blockIdx.x * len;
     for (int e = threadIdx.x; e < len; e += NT) {
         float acc = 0;
         if (e < H * KB) {
             const int k = e / KB, j = e % KB;
-            for (int64_t r = r0; r < r1; ++r) acc += A[r * H + k] * B[r * KB + j];
+            for (int r = 0; r < rows; ++r) acc += As[r][k] * Bs[r][j];
         } else {
             const int k = e - H * KB;
-            for (int64_t r = r0; r < r1; ++r) acc += A[r * H + k];
+            for (int r = 0; r < rows; ++r) acc += As[r][k];
         }
         out[e] = acc;
     }
@@ -459,24 +520,24 @@ int dalloc(anifem_gnn* g, T** p, size_t count) {
 
 unsigned nb(int64_t work, int threads = NT) { return (unsigned)std::max<int64_t>(1, (work + threads - 1) / threads); }
 
-// forward through the network: G32 / G64 (if non-null) and GT32
-int forward(anifem_gnn* g, const double* a, double* G64, hipStream_t s) {
+// forward through the network: G32 / G64 (if non-null) and, for the loss,
+// GT32
+int forward(anifem_gnn* g, const double* a, double* G64, bool gt32, hipStream_t s) {
     const int64_t n = g->n, nnz = g->nnz;
     hipLaunchKernelGGL(diag_kernel, dim3(nb(n)), dim3(NT), 0, s, n, g->rp, g->ci, a, g->dinv, g->logd);
     hipLaunchKernelGGL(feat_kernel, dim3(nb(n)), dim3(NT), 0, s, n, g->rp, g->ci, a, g->dinv, g->logd, g->at, g->a32,
                        g->x);
     hipLaunchKernelGGL(input_kernel, dim3(nb(n * H)), dim3(NT), 0, s, n, g->x, g->pw, g->z0, g->h[0]);
-    for (int l = 0; l < NL; ++l) {
-        hipLaunchKernelGGL(spmm_kernel, dim3(nb(n * H)), dim3(NT), 0, s, n, g->rp, g->ci, g->at, g->h[l],
-                           (const float*)nullptr, g->m[l]);
-        hipLaunchKernelGGL(layer_kernel, dim3(nb(n, NODES)), dim3(NT), 0, s, n, g->h[l], g->m[l], g->pw, l, g->z[l],
-                           g->h[l + 1]);
-    }
+    for (int l = 0; l < NL; ++l)
+        hipLaunchKernelGGL(mp_layer_kernel, dim3(nb(n, LN)), dim3(NT), 0, s, n, g->rp, g->ci, g->at, g->h[l], g->pw,
+                           l, g->m[l], g->z[l], g->h[l + 1]);
     hipLaunchKernelGGL(head_edge_kernel, dim3(nb(nnz)), dim3(NT), 0, s, nnz, g->row, g->ci, g->at, g->h[NL], g->pw,
                        g->Lv);
     hipLaunchKernelGGL(head_node_kernel, dim3(nb(n)), dim3(NT), 0, s, n, g->rp, g->ci, g->h[NL], g->pw, g->dinv, g->Lv,
                        g->Dh, g->G32, G64);
-    hipLaunchKernelGGL(transpose_vals_kernel<float>, dim3(nb(nnz)), dim3(NT), 0, s, nnz, g->tslot, g->G32, g->GT32);
+    if (gt32)
+        hipLaunchKernelGGL(transpose_vals_kernel<float>, dim3(nb(nnz)), dim3(NT), 0, s, nnz, g->tslot, g->G32,
+                           g->GT32);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -714,7 +775,7 @@ int anifem_gnn_get_grad(const anifem_gnn* g, float* grad) {
 int anifem_gnn_factor(anifem_gnn* g, const double* a, double* G, double* GT, void* stream) {
     if (!g || !a) return ANIFEM_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = forward(g, a, G ? G : g->G64, s);
+    int rc = forward(g, a, G ? G : g->G64, false, s);
     if (rc) return rc;
     hipLaunchKernelGGL(transpose_vals_kernel<double>, dim3(nb(g->nnz)), dim3(NT), 0, s, g->nnz, g->tslot,
                        (const double*)(G ? G : g->G64), GT ? GT : g->GT64);
@@ -753,7 +814,7 @@ int anifem_gnn_loss_grad(anifem_gnn* g, const double* a, const float* probes, ui
     if (!g || !a) return ANIFEM_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t n = g->n, nnz = g->nnz;
-    int rc = forward(g, a, nullptr, s);
+    int rc = forward(g, a, nullptr, true, s);
     if (rc) return rc;
     if (probes)
         ANIFEM_HIP_CHECK(hipMemcpyAsync(g->Z, probes, n * P * sizeof(float), hipMemcpyDeviceToDevice, s));

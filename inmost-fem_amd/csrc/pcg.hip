@@ -162,12 +162,35 @@ __device__ double block_bcast(double a) {
 
 __device__ double block_total(const double* part, int nparts) { return block_bcast(thread_partial(part, nparts)); }
 
+// NV block sums at once, every thread receiving the totals: one barrier pair
+// for all of them.  Wave trees then waves 0..3 in order, as block_sum.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV]) {
+    __shared__ double sh[NV][kThreads / 64];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[i][w] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double r = 0;
+        for (int j = 0; j < kThreads / 64; ++j) r += sh[i][j];
+        v[i] = r;
+    }
+    __syncthreads();
+}
+
 // ---- two launches per iteration ----------------------------------------
 // Iteration k runs dir_spmv(k) then update(k):
-//   dir_spmv(k): test r_k.r_k (done, it = k); beta = rz_k / rz_(k-1);
+//   dir_spmv(k): test r_k.r_k (done, it = k); beta = rz_k / rz_(k-1), or
+//                (rz_k - r_k.z_(k-1)) / rz_(k-1) (flexible, Polak-Ribiere);
 //                p_k = z_k + beta p_(k-1) (p_0 = z_0); Ap_k = A p_k; partials p_k.Ap_k
 //   update(k):   alpha = rz_k / p_k.Ap_k; x += alpha p_k; r_(k+1) = r_k - alpha Ap_k;
-//                z_(k+1) = M r_(k+1); partials r.z, r.r
+//                z_(k+1) = M r_(k+1); partials r.z, r.r (and r.z_old if flexible)
 // A row of A p_k needs p_k at its neighbours, which other workgroups write in the
 // same launch, so every reader forms p_k = fma(beta, p_(k-1), z_k) itself from
 // the previous buffer (p alternates between two buffers); likewise r_(k+1) for
@@ -183,8 +206,8 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
                                                              const double* __restrict__ v, const double* z,
                                                              const double* p_prev, double* p, double* Ap,
                                                              double* partA, const double* partU, int nU,
-                                                             PcgState* st, int64_t k, int check_only,
-                                                             int* hdone) {
+                                                             int flexible, PcgState* st, int64_t k,
+                                                             int check_only, int* hdone) {
     const bool first = k == 0;
     const int64_t rows_per_block = kThreads / L;
     const int lane = threadIdx.x % L;
@@ -212,11 +235,16 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
         zr = z[row0];
         if (!first) pr = p_prev[row0];
     }
-    const double rr_part = first ? 0.0 : thread_partial(partU + kRedBlocks, nU);
-    const double rz_part = first ? 0.0 : thread_partial(partU, nU);
+    double tot[3] = {0.0, 0.0, 0.0};  // r.r, r.z, r.z_old
+    if (!first) {
+        tot[0] = thread_partial(partU + kRedBlocks, nU);
+        tot[1] = thread_partial(partU, nU);
+        if (flexible) tot[2] = thread_partial(partU + 2 * kRedBlocks, nU);
+    }
     if (st->done) return;
+    if (!first) block_sums(tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const double rr = first ? st->rr : block_bcast(rr_part);
+    const double rr = first ? st->rr : tot[0];
     if (rr <= st->tol2 || check_only) {
         if (lead) {
             st->rr = rr;
@@ -230,8 +258,9 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
     }
     double beta = 0;
     if (!first) {
-        const double rz = block_bcast(rz_part);
-        beta = rz / st->rzb[(k - 1) & 1];
+        // Polak-Ribiere (flexible) for a preconditioner that may not be exactly linear
+        const double rz = tot[1];
+        beta = (rz - tot[2]) / st->rzb[(k - 1) & 1];
         if (lead) st->rzb[k & 1] = rz;
     }
     if (lead) {
@@ -264,8 +293,9 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
             dot += s * pk;
         }
     }
-    const double b = block_sum(dot);
-    if (threadIdx.x == 0) partA[blockIdx.x] = b;
+    double out[1] = {dot};
+    block_sums(out);
+    if (threadIdx.x == 0) partA[blockIdx.x] = out[0];
 }
 
 // Jacobi update: z = Dinv r (row-local, r updated in place)
@@ -282,9 +312,10 @@ __global__ __launch_bounds__(kThreads) void update_jacobi_kernel(int64_t n, doub
         r0 = r[i0];
         d0 = dinv[i0];
     }
-    const double pap_part = thread_partial(partA, nA);
+    double pap[1] = {thread_partial(partA, nA)};
     if (st->done) return;
-    const double alpha = st->rzb[k & 1] / block_bcast(pap_part);
+    block_sums(pap);
+    const double alpha = st->rzb[k & 1] / pap[0];
     double rz = 0, rr = 0;
     for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const bool pre = i == i0;
@@ -296,10 +327,11 @@ __global__ __launch_bounds__(kThreads) void update_jacobi_kernel(int64_t n, doub
         rz += ri * zi;
         rr += ri * ri;
     }
-    const double s0 = block_sum(rz), s1 = block_sum(rr);
+    double out[2] = {rz, rr};
+    block_sums(out);
     if (threadIdx.x == 0) {
-        partU[blockIdx.x] = s0;
-        partU[kRedBlocks + blockIdx.x] = s1;
+        partU[blockIdx.x] = out[0];
+        partU[kRedBlocks + blockIdx.x] = out[1];
     }
 }
 
@@ -332,17 +364,19 @@ __global__ __launch_bounds__(kThreads) void update_matrix_kernel(int64_t n, cons
             er[j] = r_prev[c];
         }
     }
-    double xr = 0, pr = 0, ar = 0, rr0 = 0;
+    double xr = 0, pr = 0, ar = 0, rr0 = 0, zo0 = 0;
     if (row0 < n && lane == 0) {
         xr = x[row0];
         pr = p[row0];
         ar = Ap[row0];
         rr0 = r_prev[row0];
+        zo0 = z[row0];
     }
-    const double pap_part = thread_partial(partA, nA);
+    double pap[1] = {thread_partial(partA, nA)};
     if (st->done) return;
-    const double alpha = st->rzb[k & 1] / block_bcast(pap_part);
-    double rz = 0, rr = 0;
+    block_sums(pap);
+    const double alpha = st->rzb[k & 1] / pap[0];
+    double rz = 0, rr = 0, rzo = 0;
     for (int64_t base = (int64_t)blockIdx.x * rows_per_block; base < n; base += (int64_t)gridDim.x * rows_per_block) {
         const int64_t row = base + threadIdx.x / L;
         const bool pre = row == row0;
@@ -363,16 +397,19 @@ __global__ __launch_bounds__(kThreads) void update_matrix_kernel(int64_t n, cons
         if (row < n && lane == 0) {
             const double ri = fma(-alpha, pre ? ar : Ap[row], pre ? rr0 : r_prev[row]);
             x[row] = fma(alpha, pre ? pr : p[row], pre ? xr : x[row]);
+            rzo += ri * (pre ? zo0 : z[row]);
             r[row] = ri;
             z[row] = s;
             rz += ri * s;
             rr += ri * ri;
         }
     }
-    const double s0 = block_sum(rz), s1 = block_sum(rr);
+    double out[3] = {rz, rr, rzo};
+    block_sums(out);
     if (threadIdx.x == 0) {
-        partU[blockIdx.x] = s0;
-        partU[kRedBlocks + blockIdx.x] = s1;
+        partU[blockIdx.x] = out[0];
+        partU[kRedBlocks + blockIdx.x] = out[1];
+        partU[2 * kRedBlocks + blockIdx.x] = out[2];
     }
 }
 
@@ -567,10 +604,12 @@ void gt_dispatch(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* ci, c
     return launch_gt<32>(n, rp, ci, v, r, t, partB, nB, st, k, hdone, s);
 }
 
-// Lanes per row from a matrix's average row length.
+// Lanes per row for the two-launch iteration: about six entries per lane, so
+// that a row's operands fit the kPre prefetch registers and a workgroup
+// covers many rows (fewer partial sums for the next kernel to read).
 int lanes_per_row(int64_t n, int64_t nnz) {
     const double avg = n > 0 ? (double)nnz / n : 0;
-    return avg <= 12 ? 4 : avg <= 24 ? 8 : avg <= 48 ? 16 : 32;
+    return avg <= 12 ? 2 : avg <= 24 ? 4 : avg <= 48 ? 8 : avg <= 96 ? 16 : 32;
 }
 
 int red_grid(int64_t n, int L) {
@@ -579,12 +618,13 @@ int red_grid(int64_t n, int L) {
 }
 
 void launch_dir(int L, int grid, int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* z,
-                const double* p_prev, double* p, double* Ap, double* partA, const double* partU, int nU, PcgState* st,
-                int64_t k, int check_only, int* hdone, hipStream_t s) {
+                const double* p_prev, double* p, double* Ap, double* partA, const double* partU, int nU, int flexible,
+                PcgState* st, int64_t k, int check_only, int* hdone, hipStream_t s) {
 #define ANIFEM_DIR(LL)                                                                                               \
     hipLaunchKernelGGL(dir_spmv_kernel<LL>, dim3((unsigned)grid), dim3(kThreads), 0, s, n, rp, ci, v, z, p_prev, p, \
-                       Ap, partA, partU, nU, st, k, check_only, hdone)
-    if (L == 4) ANIFEM_DIR(4);
+                       Ap, partA, partU, nU, flexible, st, k, check_only, hdone)
+    if (L == 2) ANIFEM_DIR(2);
+    else if (L == 4) ANIFEM_DIR(4);
     else if (L == 8) ANIFEM_DIR(8);
     else if (L == 16) ANIFEM_DIR(16);
     else ANIFEM_DIR(32);
@@ -597,7 +637,8 @@ void launch_update_matrix(int L, int grid, int64_t n, const int64_t* mrp, const 
 #define ANIFEM_UPD(LL)                                                                                             \
     hipLaunchKernelGGL(update_matrix_kernel<LL>, dim3((unsigned)grid), dim3(kThreads), 0, s, n, mrp, mci, mv, x, \
                        r_prev, r, z, p, Ap, partA, nA, partU, st, k)
-    if (L == 4) ANIFEM_UPD(4);
+    if (L == 2) ANIFEM_UPD(2);
+    else if (L == 4) ANIFEM_UPD(4);
     else if (L == 8) ANIFEM_UPD(8);
     else if (L == 16) ANIFEM_UPD(16);
     else ANIFEM_UPD(32);
@@ -720,8 +761,8 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
     int rc = run_iterations(
         maxit, st,
         [&](int64_t k, int check_only, int* hdone) {
-            launch_dir(L, nD, n, row_ptr, col_idx, val, z, pb[(k + 1) & 1], pb[k & 1], Ap, partA, partU, (int)vb, st,
-                       k, check_only, hdone, s);
+            launch_dir(L, nD, n, row_ptr, col_idx, val, z, pb[(k + 1) & 1], pb[k & 1], Ap, partA, partU, (int)vb, 0,
+                       st, k, check_only, hdone, s);
             if (!check_only)
                 hipLaunchKernelGGL(update_jacobi_kernel, dim3(vb), dim3(kThreads), 0, s, n, x, r, z, pb[k & 1], Ap,
                                    dinv, partA, nD, partU, st, k);
@@ -732,9 +773,10 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
     return rc;
 }
 
-// PCG with an explicit SPD preconditioner matrix M (z = M r), for example
-// M = G G^T of the learned factor.  Same stopping rule and reporting as
-// anifem_pcg_jacobi; two launches per iteration.
+// Flexible PCG (Polak-Ribiere beta) with an explicit SPD preconditioner
+// matrix M (z = M r), for example M = G G^T of the learned factor.  Same
+// stopping rule and reporting as anifem_pcg_jacobi; two launches per
+// iteration.
 extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_t* col_idx, const double* val,
                                  const int64_t* m_row_ptr, const int32_t* m_col_idx, const double* m_val,
                                  const double* b, double* x, double rtol, int64_t maxit, anifem_pcg_result* res,
@@ -750,11 +792,11 @@ extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     double* work = nullptr;
     const size_t nv = (size_t)n;
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 3 * kRedBlocks) + sizeof(PcgState)));
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 4 * kRedBlocks) + sizeof(PcgState)));
     double *rb[2] = {work, work + nv}, *z = work + 2 * nv, *pb[2] = {z + nv, z + 2 * nv}, *Ap = z + 3 * nv,
            *part = Ap + nv;
-    PcgState* st = reinterpret_cast<PcgState*>(part + 3 * kRedBlocks);
-    double *partA = part, *partU = part + kRedBlocks;
+    PcgState* st = reinterpret_cast<PcgState*>(part + 4 * kRedBlocks);
+    double *partA = part, *partU = part + kRedBlocks;  // p.Ap; (r.z, r.r, r.z_old)
     const unsigned vb = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, kRedBlocks);
     // r0 = b - A x0, z0 = M r0
     spmv_dispatch(n, nnz, row_ptr, col_idx, val, x, Ap, nullptr, nullptr, s);
@@ -767,8 +809,8 @@ extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_
     int rc = run_iterations(
         maxit, st,
         [&](int64_t k, int check_only, int* hdone) {
-            launch_dir(L, nD, n, row_ptr, col_idx, val, z, pb[(k + 1) & 1], pb[k & 1], Ap, partA, partU, nU, st, k,
-                       check_only, hdone, s);
+            launch_dir(L, nD, n, row_ptr, col_idx, val, z, pb[(k + 1) & 1], pb[k & 1], Ap, partA, partU, nU, 1, st,
+                       k, check_only, hdone, s);
             if (!check_only)
                 launch_update_matrix(LM, nU, n, m_row_ptr, m_col_idx, m_val, x, rb[k & 1], rb[(k + 1) & 1], z,
                                      pb[k & 1], Ap, partA, nD, partU, st, k, s);
