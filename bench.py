@@ -181,6 +181,17 @@ def bench_elasticity(args, torch):
     E32 = af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyz, af.PREC_FP32)
     err = (E32.double() - E64).flatten(1).norm(dim=1) / E64.flatten(1).norm(dim=1)
     res["fp32_element_rel_frobenius_error"] = {"max": err.max().item(), "median": err.median().item()}
+    af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyz, af.PREC_FP16X3, out=E32)
+    err = (E32.double() - E64).flatten(1).norm(dim=1) / E64.flatten(1).norm(dim=1)
+    res["fp16x3_mfma_element_rel_frobenius_error"] = {"max": err.max().item(), "median": err.median().item()}
+    elem = {}
+    for prec, name, outp in [(af.PREC_FP64, "fp64", E64), (af.PREC_FP32, "fp32", E32),
+                             (af.PREC_FP16X3, "fp16x3_mfma", E32)]:
+        ms = timed_events(lambda: af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyz, prec,
+                                                    out=outp), 5, 1, torch)
+        bytes_ = n * (96 + 16 + 900 * outp.element_size())
+        elem[name] = {"ms": ms, "element_matrices_per_s": n / ms * 1e3, "GB_per_s": bytes_ / ms / 1e6}
+    res["element_only_30x30"] = elem
     del E64, E32, err, xyz
     # displacements: Jacobi-PCG to rel-res 1e-10 on the FP64-assembled system
     # and on the system (matrix and load) assembled with the FP32 contraction
@@ -342,7 +353,7 @@ def main():
         xyz = torch.from_numpy(np.ascontiguousarray(coords[tets].reshape(n_tets, 12).T)).to(dev)
         op = af.Op(af.GRAD, af.FEM_P2)
         res = {}
-        for prec, name in [(af.PREC_FP64, "fp64"), (af.PREC_FP32, "fp32")]:
+        for prec, name in [(af.PREC_FP64, "fp64"), (af.PREC_FP32, "fp32"), (af.PREC_FP16X3, "fp16x3_mfma")]:
             dt = torch.float64 if prec == af.PREC_FP64 else torch.float32
             A = torch.empty((n_tets, 10, 10), dtype=dt, device=dev)
             for _ in range(2):
@@ -369,6 +380,11 @@ def main():
         err = (A32.double() - A64).flatten(1).norm(dim=1) / A64.flatten(1).norm(dim=1)
         res["fp32_element_rel_frobenius_error"] = {"max": err.max().item(), "median": err.median().item(),
                                                    "p99.9": torch.quantile(err.float()[::7], 0.999).item()}
+        af.fem3dtet_batch(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, 2, xyz, af.PREC_FP16X3, out=A32)
+        err = (A32.double() - A64).flatten(1).norm(dim=1) / A64.flatten(1).norm(dim=1)
+        res["fp16x3_mfma_element_rel_frobenius_error"] = {
+            "max": err.max().item(), "median": err.median().item(),
+            "p99.9": torch.quantile(err.float()[::7], 0.999).item()}
         del A64, A32, err
         out["element_only"] = res
         # the plain FP64 method on the GPU (reference algorithm per tet, FP64
@@ -378,7 +394,10 @@ def main():
             "plain_fp64_kernel_ms": plain_ms, "rowblock_fp64_ms": sec64 * 1e3, "rowblock_fp32_ms": sec32 * 1e3,
             "fp32_speedup_vs_plain_fp64": plain_ms / (sec32 * 1e3),
             "fp32_speedup_vs_rowblock_fp64": sec64 / sec32,
-            "fp32_element_rel_frobenius_error_max": res["fp32_element_rel_frobenius_error"]["max"]}
+            "fp32_element_rel_frobenius_error_max": res["fp32_element_rel_frobenius_error"]["max"],
+            "fp16x3_mfma_element_ms": res["fp16x3_mfma"]["ms"],
+            "fp16x3_mfma_element_rel_frobenius_error_max": res["fp16x3_mfma_element_rel_frobenius_error"]["max"],
+            "fp16x3_mfma_element_speedup_vs_plain_fp64": plain_ms / res["fp16x3_mfma"]["ms"]}
         del xyz
     if "pcg" in sections:
         # Jacobi-PCG on the 16^3 P1 Poisson system (rel-res 1e-10)

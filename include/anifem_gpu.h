@@ -72,8 +72,13 @@ typedef struct anifem_tensor {
 /* Arithmetic of the element contraction.  Geometry (Jacobian, inverse,
  * determinant) is always FP64; global CSR / RHS accumulation is always FP64. */
 enum {
-    ANIFEM_PREC_FP64 = 0, /* FP64 contraction, FP64 element output */
-    ANIFEM_PREC_FP32 = 1  /* FP32 contraction, FP32 element output */
+    ANIFEM_PREC_FP64 = 0,  /* FP64 contraction, FP64 element output */
+    ANIFEM_PREC_FP32 = 1,  /* FP32 contraction, FP32 element output */
+    /* anifem_fem3dtet_batch only, tensors constant over each tet: R and the
+     * per-tet geometry tensor, each scaled to [-1, 1], split into FP16 hi + lo
+     * and contracted on MFMA (three products, FP32 accumulation), FP32 element
+     * output.  No reference counterpart (the reference is FP64 throughout). */
+    ANIFEM_PREC_FP16X3 = 2
 };
 
 /* ---- element level ------------------------------------------------------
@@ -85,7 +90,8 @@ enum {
  *           xyz[(3*v + c)*n_tets + r]  (the Tetras XY0..XY3 columns)
  *   A_out : device, n_tets matrices of nfB x nfA, column-major, one after the
  *           other (the reference's fused layout A[ib + nfB*(ia + nfA*r)]);
- *           double for ANIFEM_PREC_FP64, float for ANIFEM_PREC_FP32.
+ *           double for ANIFEM_PREC_FP64, float for ANIFEM_PREC_FP32 and
+ *           ANIFEM_PREC_FP16X3.
  * Errors mirror the reference's throws: ANIFEM_ERR_ORDER (order > 6),
  * ANIFEM_ERR_DIMS (NULL/SCALAR with dimA != dimB), ANIFEM_ERR_ARG. */
 int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,

@@ -29,7 +29,7 @@ IDEN, GRAD = 1, 2
 FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
 TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
 COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_FP16X3 = 0, 1, 2
 DIRICHLET_NONE, DIRICHLET_BOUNDARY, DIRICHLET_DOF_MASK = 0, 1, 2
 
 _ERRNAMES = {ERR_NONFINITE: "non-finite local entry", ERR_CONNECTIVITY: "broken connectivity",

@@ -39,6 +39,11 @@ struct RefTensorDev {
     int KB = 0;  // dA*dB
     int q = 0;   // points (per-point tensors) or 1
     const double* w = nullptr;  // quadrature weights on the device (q values)
+    // FP16 hi + lo split of R / r16_scale for the MFMA contraction (q == 1 and
+    // KB <= 16 only): [2][e0pad][16] halves, row e0, column kk (zero padded)
+    const void* r16 = nullptr;
+    float r16_scale = 1.0f;
+    int e0pad = 0;
 };
 int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out);
 

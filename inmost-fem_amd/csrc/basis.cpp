@@ -179,6 +179,15 @@ void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_
         }
 }
 
+// hi = fp16(x), lo = fp16(x - hi) of n floats: out[0..n) hi, out[n..2n) lo
+__global__ void split_f16_kernel(const float* x, int64_t n, _Float16* out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const _Float16 h = (_Float16)x[i];
+    out[i] = h;
+    out[n + i] = (_Float16)(x[i] - (float)h);
+}
+
 int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, int, int, int>, RefTensorDev> cache;
@@ -213,6 +222,29 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
     ANIFEM_HIP_CHECK(hipMemcpy(d32, Rf.data(), Rf.size() * sizeof(float), hipMemcpyHostToDevice));
     t.r64 = static_cast<const double*>(d64);
     t.r32 = static_cast<const float*>(d32);
+    if (!per_point && t.KB <= 16) {
+        // R / max|R| in [-1, 1], rows padded to 16 columns and to a multiple of 16 rows
+        double mx = 0;
+        for (double v : R) mx = std::max(mx, std::fabs(v));
+        if (mx == 0) mx = 1;
+        t.e0pad = (t.E0 + 15) / 16 * 16;
+        t.r16_scale = (float)mx;
+        std::vector<float> xs((size_t)t.e0pad * 16, 0.0f);
+        for (int e = 0; e < t.E0; ++e)
+            for (int kk = 0; kk < t.KB; ++kk) xs[(size_t)e * 16 + kk] = (float)(R[(size_t)e * t.KB + kk] / mx);
+        void* dx = nullptr;
+        void* d16 = nullptr;
+        const int64_t nx = (int64_t)xs.size();
+        ANIFEM_HIP_CHECK(hipMalloc(&dx, nx * sizeof(float)));
+        ANIFEM_HIP_CHECK(hipMalloc(&d16, 2 * nx * sizeof(_Float16)));
+        ANIFEM_HIP_CHECK(hipMemcpy(dx, xs.data(), nx * sizeof(float), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(split_f16_kernel, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, 0,
+                           static_cast<const float*>(dx), nx, static_cast<_Float16*>(d16));
+        ANIFEM_HIP_CHECK(hipGetLastError());
+        ANIFEM_HIP_CHECK(hipDeviceSynchronize());
+        ANIFEM_HIP_CHECK(hipFree(dx));
+        t.r16 = d16;
+    }
     cache[key] = t;
     *out = t;
     return ANIFEM_OK;

@@ -36,6 +36,9 @@ int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_t
     a.Rd = R.r64;
     a.Rf = R.r32;
     a.wq = R.w;
+    a.r16 = R.r16;
+    a.r16_scale = R.r16_scale;
+    a.e0pad = R.e0pad;
     return ANIFEM_OK;
 }
 
@@ -79,6 +82,40 @@ int launch_element(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a
     return ANIFEM_ERR_ARG;
 }
 
+template <int DA0, int DB0, int NCA, int NCB>
+static int launch_mfma_one(const ElemArgs& a, hipStream_t s) {
+    if (NCA * NCB == 1) {
+        const int64_t blocks = (a.n + kMfmaTets - 1) / kMfmaTets;
+        hipLaunchKernelGGL((element_mfma_kernel<DA0, DB0, NCA, NCB>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    } else {
+        const int64_t blocks = (a.n + kMfmaVecTets - 1) / kMfmaVecTets;
+        hipLaunchKernelGGL((element_mfma_vec_kernel<DA0, DB0, NCA, NCB>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                           a);
+    }
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
+template <int DA0, int DB0>
+static int launch_mfma_nc(int ncA, int ncB, const ElemArgs& a, hipStream_t s) {
+    if (ncA == 1 && ncB == 1) return launch_mfma_one<DA0, DB0, 1, 1>(a, s);
+    if (ncA == 3 && ncB == 3) return launch_mfma_one<DA0, DB0, 3, 3>(a, s);
+    if (ncA == 1 && ncB == 3) return launch_mfma_one<DA0, DB0, 1, 3>(a, s);
+    if (ncA == 3 && ncB == 1) return launch_mfma_one<DA0, DB0, 3, 1>(a, s);
+    return ANIFEM_ERR_ARG;
+}
+
+// FP16 hi + lo MFMA contraction: tensors constant over each tet only
+int launch_element_mfma(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a, hipStream_t s) {
+    if (a.q != 1 || !a.r16) return ANIFEM_ERR_ARG;
+    const int dA = ref_dim(opA.op), dB = ref_dim(opB.op);
+    if (dA == 3 && dB == 3) return launch_mfma_nc<3, 3>(opA.ncomp, opB.ncomp, a, s);
+    if (dA == 1 && dB == 1) return launch_mfma_nc<1, 1>(opA.ncomp, opB.ncomp, a, s);
+    if (dA == 3 && dB == 1) return launch_mfma_nc<3, 1>(opA.ncomp, opB.ncomp, a, s);
+    if (dA == 1 && dB == 3) return launch_mfma_nc<1, 3>(opA.ncomp, opB.ncomp, a, s);
+    return ANIFEM_ERR_ARG;
+}
+
 }  // namespace anifem
 
 extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
@@ -96,5 +133,6 @@ extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (precision == ANIFEM_PREC_FP64) return launch_element<double>(opA, opB, a, 256, s);
     if (precision == ANIFEM_PREC_FP32) return launch_element<float>(opA, opB, a, 256, s);
+    if (precision == ANIFEM_PREC_FP16X3) return launch_element_mfma(opA, opB, a, s);
     return ANIFEM_ERR_ARG;
 }

@@ -48,6 +48,9 @@ struct ElemArgs {
     const float* Rf;
     const double* wq;  // quadrature weights (per-point K)
     void* out;         // batch mode: element matrices
+    const void* r16;   // FP16 hi + lo split of R / r16_scale ([2][e0pad][16])
+    float r16_scale;
+    int e0pad;
 };
 
 __device__ __forceinline__ void tet_geometry(const double P[4][3], double PSI[9], double& det) {
@@ -94,6 +97,38 @@ __device__ __forceinline__ double tensor_at(const ElemArgs& a, int64_t cr, int n
     return 0.0;
 }
 
+// K block (cb, ca) at point n of one tet, scaled by s:
+// Kb[b*DA0 + a] = s * sum_{k0, j0} TB(k0, b) D(k0 + DB0 cb, j0 + DA0 ca) TA(j0, a),
+// T = PSI^T for GRAD, 1 for IDEN.
+template <int DA0, int DB0, typename KT>
+__device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, const double PSI[9], double s, int n,
+                                               int cb, int ca, KT* Kb) {
+    double Dblk[DB0][DA0];
+    for (int k0 = 0; k0 < DB0; ++k0)
+        for (int j0 = 0; j0 < DA0; ++j0) Dblk[k0][j0] = tensor_at(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
+    double M[DB0][DA0];  // (D * TA)
+    for (int k0 = 0; k0 < DB0; ++k0)
+        for (int a0 = 0; a0 < DA0; ++a0) {
+            double acc = 0;
+            if (DA0 == 3) {
+                for (int j0 = 0; j0 < 3; ++j0) acc += Dblk[k0][j0] * PSI[a0 + 3 * j0];
+            } else {
+                acc = Dblk[k0][0];
+            }
+            M[k0][a0] = acc;
+        }
+    for (int b0 = 0; b0 < DB0; ++b0)
+        for (int a0 = 0; a0 < DA0; ++a0) {
+            double acc = 0;
+            if (DB0 == 3) {
+                for (int k0 = 0; k0 < 3; ++k0) acc += PSI[b0 + 3 * k0] * M[k0][a0];
+            } else {
+                acc = M[0][a0];
+            }
+            Kb[b0 * DA0 + a0] = (KT)(s * acc);
+        }
+}
+
 // K[(cb*NCA + ca)*q*KB + n*KB + b*DA0 + a] for one tet.
 template <int DA0, int DB0, int NCA, int NCB, typename KT>
 __device__ __forceinline__ void geometry_tensor(const ElemArgs& a, int64_t cr, const double PSI[9], double mes,
@@ -102,34 +137,8 @@ __device__ __forceinline__ void geometry_tensor(const ElemArgs& a, int64_t cr, c
     for (int n = 0; n < a.q; ++n) {
         const double s = (a.q > 1) ? mes * a.wq[n] : mes;
         for (int cb = 0; cb < NCB; ++cb)
-            for (int ca = 0; ca < NCA; ++ca) {
-                double Dblk[DB0][DA0];
-                for (int k0 = 0; k0 < DB0; ++k0)
-                    for (int j0 = 0; j0 < DA0; ++j0) Dblk[k0][j0] = tensor_at(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
-                // K(b, a) = s * sum_{k0, j0} TB(k0, b) D(k0, j0) TA(j0, a); T = PSI^T for GRAD
-                double M[DB0][DA0];  // (D * TA)
-                for (int k0 = 0; k0 < DB0; ++k0)
-                    for (int a0 = 0; a0 < DA0; ++a0) {
-                        double acc = 0;
-                        if (DA0 == 3) {
-                            for (int j0 = 0; j0 < 3; ++j0) acc += Dblk[k0][j0] * PSI[a0 + 3 * j0];
-                        } else {
-                            acc = Dblk[k0][0];
-                        }
-                        M[k0][a0] = acc;
-                    }
-                KT* Kb = K + ((cb * NCA + ca) * a.q + n) * KB;
-                for (int b0 = 0; b0 < DB0; ++b0)
-                    for (int a0 = 0; a0 < DA0; ++a0) {
-                        double acc = 0;
-                        if (DB0 == 3) {
-                            for (int k0 = 0; k0 < 3; ++k0) acc += PSI[b0 + 3 * k0] * M[k0][a0];
-                        } else {
-                            acc = M[0][a0];
-                        }
-                        Kb[b0 * DA0 + a0] = (KT)(s * acc);
-                    }
-            }
+            for (int ca = 0; ca < NCA; ++ca)
+                geometry_block<DA0, DB0, KT>(a, cr, PSI, s, n, cb, ca, K + ((cb * NCA + ca) * a.q + n) * KB);
     }
 }
 
@@ -197,6 +206,201 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
                 for (int j = 0; j < a.q * KB; ++j) acc += Kt[j] * Rn[j];
             }
             static_cast<T*>(a.out)[(tile0 + t) * E + e] = acc;
+        }
+    }
+}
+
+// ---- split-precision MFMA contraction (ANIFEM_PREC_FP16X3) -----------------
+// For a tensor constant over each tet the element matrices of a block of tets
+// are one product per component block (cb, ca):
+//   C[e0][t] = sum_kk R[e0][kk] K_t[kk],  e0 = ib0 + nfB0 ia0 < E0, kk < KB <= 16,
+// run on v_mfma_f32_16x16x16f16 (16 entries x 16 tets x 16 kk per instruction).
+// R / max|R| and K_t / max|K_t| lie in [-1, 1]; each is split as hi = fp16(x),
+// lo = fp16(x - hi) (22 significant bits), and C = Rl Kh + Rh Kl + Rh Kh in
+// FP32 (the Rl Kl term, below 2^-22 of |R||K|, is dropped), then scaled back.
+// Operand layouts of the 16x16x16 instruction, lane l (j = l % 16, g = l / 16):
+//   A (16 x 16): row j, columns 4g..4g+3;  B (16 x 16): rows 4g..4g+3, column j;
+//   D (16 x 16): rows 4g..4g+3, column j.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaTets = 256;  // tets per workgroup: 64 per wavefront, in 4 groups of 16
+
+template <int DA0, int DB0, int NCA, int NCB>
+__global__ __launch_bounds__(kBlock) void element_mfma_kernel(ElemArgs a) {
+    constexpr int KB = DA0 * DB0, NB = NCA * NCB;
+    static_assert(KB <= 16, "the 16-wide kk dimension holds K");
+    // one component block at a time: K_t / max|K_t| of that block, hi and lo
+    __shared__ __align__(16) _Float16 Kh[kMfmaTets][16], Kl[kMfmaTets][16];
+    __shared__ float ks[kMfmaTets];
+    const int64_t tile0 = (int64_t)blockIdx.x * kMfmaTets;
+    const int ntile = (int)((a.n - tile0) < kMfmaTets ? (a.n - tile0) : kMfmaTets);
+    // FP64 geometry of this lane's tet, kept in registers across the blocks
+    const int t = threadIdx.x;
+    const int64_t r = tile0 + t;
+    double PSI[9], mes = 0;
+    if (t < ntile) {
+        double P[4][3], det;
+        load_tet<false>(a, r, P);
+        tet_geometry(P, PSI, det);
+        mes = fabs(det) / 6.0;
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64, j = l % 16, g = l / 16;
+    const int E = a.E, nfB0 = a.nfB0, nfA0 = a.nfA0, nfB = nfB0 * NCB;
+    const int E0 = nfA0 * nfB0;
+    // a lane's 4 entries are 16 contiguous, aligned bytes
+    const bool vec4 = NB == 1 && (E0 % 4) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 15) == 0;
+    const _Float16* Rh = static_cast<const _Float16*>(a.r16);
+    const _Float16* Rl = Rh + (int64_t)a.e0pad * 16;
+    for (int b = 0; b < NB; ++b) {
+        const int cb = b / NCA, ca = b % NCA;
+        {
+            double Kb[KB];
+            double mx = 0;
+            if (t < ntile) {
+                geometry_block<DA0, DB0, double>(a, r, PSI, mes, 0, cb, ca, Kb);
+                for (int i = 0; i < KB; ++i) mx = fmax(mx, fabs(Kb[i]));
+            } else {
+                for (int i = 0; i < KB; ++i) Kb[i] = 0.0;
+            }
+            const double sc = mx > 0 ? mx : 1.0;
+            ks[t] = (float)sc;
+            for (int kk = 0; kk < 16; ++kk) {
+                const float x = kk < KB ? (float)(Kb[kk < KB ? kk : 0] / sc) : 0.0f;
+                const _Float16 h = (_Float16)x;
+                Kh[t][kk] = h;
+                Kl[t][kk] = (_Float16)(x - (float)h);
+            }
+        }
+        __syncthreads();
+        // wavefront w: tets 64w + 16q + j, q = 0..3
+        half4_t bh[4], bl[4];
+        float sc[4];
+        bool live[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int tl = 64 * w + 16 * q + j;
+            bh[q] = *reinterpret_cast<const half4_t*>(&Kh[tl][4 * g]);
+            bl[q] = *reinterpret_cast<const half4_t*>(&Kl[tl][4 * g]);
+            sc[q] = ks[tl] * a.r16_scale;
+            live[q] = tl < ntile;
+        }
+        for (int et = 0; et < a.e0pad / 16; ++et) {
+            const int row = et * 16 + j;
+            const half4_t ah = *reinterpret_cast<const half4_t*>(Rh + row * 16 + 4 * g);
+            const half4_t al = *reinterpret_cast<const half4_t*>(Rl + row * 16 + 4 * g);
+            const int e00 = et * 16 + 4 * g;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[q], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[q], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[q], c, 0, 0, 0);
+                if (!live[q]) continue;
+                float* out = static_cast<float*>(a.out) + (tile0 + 64 * w + 16 * q + j) * E;
+                if (vec4) {
+                    if (e00 < E0) {
+                        const float4_t o = {c[0] * sc[q], c[1] * sc[q], c[2] * sc[q], c[3] * sc[q]};
+                        *reinterpret_cast<float4_t*>(out + e00) = o;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int e0 = e00 + v;
+                        if (e0 < E0) {
+                            const int ib0 = e0 % nfB0, ia0 = e0 / nfB0;
+                            out[(ib0 + nfB0 * cb) + nfB * (ia0 + nfA0 * ca)] = c[v] * sc[q];
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();  // Kh / Kl / ks are rewritten for the next block
+    }
+}
+
+// Vector spaces (NCA * NCB > 1): 64 tets per workgroup, all component blocks
+// of K_t (each scaled by its own max|K|) split into LDS, four lanes per tet.
+// The MFMA tiles then walk the output by columns: tile h of column
+// ia = ia0 + nfA0 ca covers rows ib = 16h .. 16h + 15, whose component rows
+// cb = ib / nfB0 are one k-step each (the A rows of other components are zero),
+// so every column of every tet is written once, contiguously.
+constexpr int kMfmaVecTets = 64;
+
+template <int DA0, int DB0, int NCA, int NCB>
+__global__ __launch_bounds__(kBlock) void element_mfma_vec_kernel(ElemArgs a) {
+    constexpr int KB = DA0 * DB0, NB = NCA * NCB;
+    static_assert(KB <= 16, "the 16-wide kk dimension holds K");
+    __shared__ __align__(16) _Float16 Kh[NB][kMfmaVecTets][16], Kl[NB][kMfmaVecTets][16];
+    __shared__ float ks[NB][kMfmaVecTets];
+    const int64_t tile0 = (int64_t)blockIdx.x * kMfmaVecTets;
+    const int ntile = (int)((a.n - tile0) < kMfmaVecTets ? (a.n - tile0) : kMfmaVecTets);
+    {
+        const int t = threadIdx.x / 4, part = threadIdx.x % 4;
+        const int64_t r = tile0 + t;
+        double PSI[9], mes = 0;
+        if (t < ntile) {
+            double P[4][3], det;
+            load_tet<false>(a, r, P);
+            tet_geometry(P, PSI, det);
+            mes = fabs(det) / 6.0;
+        }
+        for (int b = part; b < NB; b += 4) {
+            double Kb[KB];
+            double mx = 0;
+            if (t < ntile) {
+                geometry_block<DA0, DB0, double>(a, r, PSI, mes, 0, b / NCA, b % NCA, Kb);
+                for (int i = 0; i < KB; ++i) mx = fmax(mx, fabs(Kb[i]));
+            } else {
+                for (int i = 0; i < KB; ++i) Kb[i] = 0.0;
+            }
+            const double sc = mx > 0 ? mx : 1.0;
+            ks[b][t] = (float)sc;
+            for (int kk = 0; kk < 16; ++kk) {
+                const float x = kk < KB ? (float)(Kb[kk < KB ? kk : 0] / sc) : 0.0f;
+                const _Float16 h = (_Float16)x;
+                Kh[b][t][kk] = h;
+                Kl[b][t][kk] = (_Float16)(x - (float)h);
+            }
+        }
+    }
+    __syncthreads();
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64, j = l % 16, g = l / 16;
+    const int tl = 16 * w + j;  // this lane's tet (B operand column and output)
+    const bool live = tl < ntile;
+    const int nfB0 = a.nfB0, nfA0 = a.nfA0, nfB = nfB0 * NCB, nfA = nfA0 * NCA;
+    const _Float16* Rh = static_cast<const _Float16*>(a.r16);
+    const _Float16* Rl = Rh + (int64_t)a.e0pad * 16;
+    float* out = static_cast<float*>(a.out) + (tile0 + tl) * a.E;
+    const int ntiles = (nfB + 15) / 16;
+    const half4_t zero4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+    for (int ia = 0; ia < nfA; ++ia) {
+        const int ca = ia / nfA0, ia0 = ia % nfA0;
+        for (int h = 0; h < ntiles; ++h) {
+            const int ibA = 16 * h + j;  // A operand row of this lane
+            const int cb_lo = (16 * h) / nfB0;
+            const int cb_hi = min(NCB - 1, (16 * h + 15) / nfB0);
+            float4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int cb = cb_lo; cb <= cb_hi; ++cb) {
+                const int b = cb * NCA + ca;
+                half4_t ah = zero4, al = zero4;
+                if (ibA < nfB && ibA / nfB0 == cb) {
+                    const int e0 = ibA % nfB0 + nfB0 * ia0;
+                    ah = *reinterpret_cast<const half4_t*>(Rh + e0 * 16 + 4 * g);
+                    al = *reinterpret_cast<const half4_t*>(Rl + e0 * 16 + 4 * g);
+                }
+                const half4_t bh = *reinterpret_cast<const half4_t*>(&Kh[b][tl][4 * g]);
+                const half4_t bl = *reinterpret_cast<const half4_t*>(&Kl[b][tl][4 * g]);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
+            }
+            if (!live) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ib = 16 * h + 4 * g + v;  // D row of this lane's value v
+                if (ib < nfB) out[ib + nfB * ia] = c[v] * ks[(ib / nfB0) * NCA + ca][tl] * a.r16_scale;
+            }
         }
     }
 }

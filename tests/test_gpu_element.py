@@ -1,7 +1,8 @@
 """GPU element matrices (anifem_fem3dtet_batch) against the reference fixtures
 and the FP64 oracle.  FP64 mode: <= 1e-12 relative Frobenius per element.
 FP32 mode (FP64 geometry, FP32 contraction): <= 5e-6 relative Frobenius per
-element, the bound stated for the fast mode."""
+element.  FP16X3 mode (FP16 hi + lo split on MFMA, FP32 accumulation): <= 1e-6
+relative Frobenius per element, the bound stated for the fast mode."""
 import numpy as np
 import pytest
 
@@ -12,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
 FP32_TOL = 5e-6
+FP16X3_TOL = 1e-6
 
 
 def _torch():
@@ -65,7 +67,13 @@ def test_element_vs_reference_fixture(case):
         storage, Dprod = af.COEF_PER_POINT, product_coef(tt, af.COEF_PER_POINT, Dpp, n, dA, dB, q)
     else:
         storage, Dprod = af.COEF_PER_TET, product_coef(tt, af.COEF_PER_TET, D, n, dA, dB)
-    for prec, tol in [(af.PREC_FP64, FP64_TOL), (af.PREC_FP32, FP32_TOL)]:
+    precs = [(af.PREC_FP64, FP64_TOL), (af.PREC_FP32, FP32_TOL)]
+    if storage == af.COEF_PER_TET:
+        precs.append((af.PREC_FP16X3, FP16X3_TOL))
+    else:  # the MFMA mode takes tensors constant over each tet only
+        with pytest.raises(af.AnifemError):
+            run_product(opA, opB, tt, storage, Dprod, order, xyz, af.PREC_FP16X3)
+    for prec, tol in precs:
         A = run_product(opA, opB, tt, storage, Dprod, order, xyz, prec)
         err = T.rel_frob(A, Aref)
         print(f"{case} prec={prec} max rel Frobenius error {err:.3e}", flush=True)
@@ -74,6 +82,8 @@ def test_element_vs_reference_fixture(case):
         Dl = dev(np.ascontiguousarray(z["lame"].T))
         A = run_product(opA, opB, af.TENSOR_LAME, af.COEF_PER_TET, Dl, order, xyz)
         assert T.rel_frob(A, Aref) <= FP64_TOL
+        A = run_product(opA, opB, af.TENSOR_LAME, af.COEF_PER_TET, Dl, order, xyz, af.PREC_FP16X3)
+        assert T.rel_frob(A, Aref) <= FP16X3_TOL
 
 
 PAIRS = [(2, 2, 1, 2, 2, 1), (2, 3, 1, 2, 3, 1), (2, 4, 1, 2, 4, 1), (2, 3, 3, 2, 3, 3), (1, 1, 1, 1, 3, 1),
@@ -133,6 +143,62 @@ def test_element_sliver_and_large_batch():
     sel[-1] = n - 1
     A_or = T.orc_elem(2, 3, 1, 2, 3, 1, af.TENSOR_SCALAR, 1, 2, xyz[sel], D[sel])
     assert T.rel_frob(A[sel], A_or) <= FP64_TOL
+
+
+def test_fp16x3_mfma_high_contrast_and_slivers():
+    """The split-precision MFMA mode against the FP64 mode on the cases that
+    stress it: P2^3 elasticity with E log-uniform in [1, 1e3] and nu up to
+    0.45 on jittered Kuhn tets, P2 / P3 anisotropic diffusion (eigenvalues
+    1e-2 .. 1e2) on near-slivers, and the IDEN / mixed operator pairs."""
+    rng = np.random.default_rng(11)
+    coords, tets = af.kuhn_cube(6, 0.2, 5)
+    xyz = coords[tets].reshape(len(tets), 12)
+    n = len(xyz)
+    E = 10 ** rng.uniform(0, 3, n)
+    nu = rng.uniform(0.2, 0.45, n)
+    lame = np.stack([E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))])
+    v2 = af.Op(af.GRAD, af.FEM_P2, 3)
+    A64 = run_product(v2, v2, af.TENSOR_LAME, af.COEF_PER_TET, dev(lame), 2, xyz)
+    A16 = run_product(v2, v2, af.TENSOR_LAME, af.COEF_PER_TET, dev(lame), 2, xyz, af.PREC_FP16X3)
+    err = T.rel_frob(A16, A64)
+    print(f"P2^3 lame fp16x3 max rel Frobenius error {err:.3e}", flush=True)
+    assert err <= FP16X3_TOL
+    # near-slivers with anisotropic SPD tensors
+    m = 4000
+    base = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0.3, 0.3, 1e-3], dtype=float)
+    xs = base[None, :] + rng.uniform(-0.05, 0.05, (m, 12)) * np.array([1.0, 1.0, 0.0] * 4)  # altitude 1e-3
+    Q = np.linalg.qr(rng.normal(size=(m, 3, 3)))[0]
+    lam = 10 ** rng.uniform(-2, 2, (m, 3))
+    Dm = np.einsum("rij,rj,rkj->rik", Q, lam, Q)
+    Dp = dev(np.ascontiguousarray(T.pack_sym(Dm.reshape(m, 9), 3).T))
+    for fem in (af.FEM_P1, af.FEM_P2, af.FEM_P3):
+        op = af.Op(af.GRAD, fem)
+        order = 4 if fem == af.FEM_P3 else 2
+        A64 = run_product(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, order, xs)
+        A16 = run_product(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, order, xs, af.PREC_FP16X3)
+        err = T.rel_frob(A16, A64)
+        print(f"fem {fem} sliver fp16x3 max rel Frobenius error {err:.3e}", flush=True)
+        assert err <= FP16X3_TOL, (fem, err)
+    # IDEN pairs: mass matrix and the load vector shape (IDEN P0 against IDEN P2)
+    kap = dev(rng.uniform(0.1, 10, (1, n)))
+    for opA, opB in [(af.Op(af.IDEN, af.FEM_P2), af.Op(af.IDEN, af.FEM_P2)),
+                     (af.Op(af.IDEN, af.FEM_P0), af.Op(af.IDEN, af.FEM_P2)),
+                     (af.Op(af.GRAD, af.FEM_P2), af.Op(af.IDEN, af.FEM_P1, 3))]:
+        A64 = run_product(opA, opB, af.TENSOR_SCALAR, af.COEF_PER_TET, kap, 2, xyz)
+        A16 = run_product(opA, opB, af.TENSOR_SCALAR, af.COEF_PER_TET, kap, 2, xyz, af.PREC_FP16X3)
+        err = T.rel_frob(A16, A64)
+        assert err <= FP16X3_TOL, (opA, opB, err)
+    # a ragged tail: tets beyond the last full 64-tet workgroup are written, nothing else
+    torch = _torch()
+    op = af.Op(af.GRAD, af.FEM_P2)
+    k = 130
+    out = torch.full((k + 5, 10, 10), 7.0, dtype=torch.float32, device="cuda")
+    af.fem3dtet_batch(op, op, af.TENSOR_NULL, af.COEF_CONST, None, 2, dev(T.soa_xyz(xyz[:k])), af.PREC_FP16X3,
+                      out=out[:k])
+    torch.cuda.synchronize()
+    assert torch.all(out[k:] == 7.0)
+    ref = run_product(op, op, af.TENSOR_NULL, af.COEF_CONST, None, 2, xyz[:k])
+    assert T.rel_frob(out[:k].cpu().numpy().reshape(k, -1).astype(np.float64), ref) <= FP16X3_TOL
 
 
 def test_empty_batch_and_errors():
