@@ -193,6 +193,25 @@ def bench_elasticity(args, torch):
         elem[name] = {"ms": ms, "element_matrices_per_s": n / ms * 1e3, "GB_per_s": bytes_ / ms / 1e6}
     res["element_only_30x30"] = elem
     del E64, E32, err, xyz
+    # the FP16 hi + lo MFMA mode end to end: element matrices and load vectors
+    # on the reordered tets, scattered through the row-block lists
+    xyzr = torch.from_numpy(np.ascontiguousarray(coords[A.tets()].reshape(n, 12).T)).to("cuda")
+    p0, pv = af.Op(af.IDEN, af.FEM_P0), af.Op(af.IDEN, af.FEM_P2, 3)
+    Ae = torch.empty((n, 30, 30), dtype=torch.float32, device="cuda")
+    Fe = torch.empty((n, 1, 30), dtype=torch.float32, device="cuda")
+    val16 = torch.empty_like(val)
+    rhs16 = torch.empty_like(rhs)
+
+    def mfma_pass():
+        af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyzr, af.PREC_FP16X3, out=Ae)
+        af.fem3dtet_batch(p0, pv, af.TENSOR_GENERAL, af.COEF_CONST, g, 2, xyzr, af.PREC_FP16X3, out=Fe)
+        A.assemble_elements(Ae, Fe.view(n, 30), val=val16, rhs=rhs16, status=False)
+
+    ms16 = timed_events(mfma_pass, 5, 1, torch)
+    res["fp16x3_mfma_elements_then_scatter"] = {
+        "ms_per_pass": ms16, "element_matrices_per_s": n / ms16 * 1e3,
+        "max_entry_error_rel_to_max": float((val16 - ref).abs().max() / ref.abs().max())}
+    del Ae, Fe, xyzr
     # displacements: Jacobi-PCG to rel-res 1e-10 on the FP64-assembled system
     # and on the system (matrix and load) assembled with the FP32 contraction
     rp, ci = A.csr()
@@ -201,13 +220,18 @@ def bench_elasticity(args, torch):
     t0 = time.perf_counter()
     r64, ok64 = af.pcg_jacobi(A.ndof, rp, ci, ref, rhs64, u64, rtol=1e-10, maxit=100000)
     r32, ok32 = af.pcg_jacobi(A.ndof, rp, ci, val, rhs, u32, rtol=1e-10, maxit=100000)
+    u16 = torch.zeros_like(u64)
+    r16, ok16 = af.pcg_jacobi(A.ndof, rp, ci, val16, rhs16, u16, rtol=1e-10, maxit=100000)
     torch.cuda.synchronize()
     res["displacement"] = {
-        "rtol": 1e-10, "converged": bool(ok64 and ok32), "iterations_fp64": r64.iterations,
-        "iterations_fp32": r32.iterations, "solve_seconds_both": time.perf_counter() - t0,
+        "rtol": 1e-10, "converged": bool(ok64 and ok32 and ok16), "iterations_fp64": r64.iterations,
+        "iterations_fp32": r32.iterations, "iterations_fp16x3_mfma": r16.iterations,
+        "solve_seconds_all": time.perf_counter() - t0,
         "fp32_rel_l2_error": float((u32 - u64).norm() / u64.norm()),
-        "fp32_max_error_rel_to_max": float((u32 - u64).abs().max() / u64.abs().max())}
-    del A, val, rhs, ref, rhs64, u64, u32
+        "fp32_max_error_rel_to_max": float((u32 - u64).abs().max() / u64.abs().max()),
+        "fp16x3_mfma_rel_l2_error": float((u16 - u64).norm() / u64.norm()),
+        "fp16x3_mfma_max_error_rel_to_max": float((u16 - u64).abs().max() / u64.abs().max())}
+    del A, val, rhs, ref, rhs64, u64, u32, u16, val16, rhs16
     return res
 
 
@@ -385,7 +409,25 @@ def main():
         res["fp16x3_mfma_element_rel_frobenius_error"] = {
             "max": err.max().item(), "median": err.median().item(),
             "p99.9": torch.quantile(err.float()[::7], 0.999).item()}
-        del A64, A32, err
+        # FP16X3 (MFMA) elements on the reordered tets, scattered through the row blocks
+        xyzr = torch.from_numpy(np.ascontiguousarray(coords[Asm.tets()].reshape(n_tets, 12).T)).to(dev)
+        p0, p2 = af.Op(af.IDEN, af.FEM_P0), af.Op(af.IDEN, af.FEM_P2)
+        Fe = torch.empty((n_tets, 1, 10), dtype=torch.float32, device=dev)
+        val16 = torch.empty_like(val)
+        rhs16 = torch.empty_like(rhs)
+
+        def mfma_pass():
+            af.fem3dtet_batch(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, Dp, 2, xyzr, af.PREC_FP16X3, out=A32)
+            af.fem3dtet_batch(p0, p2, af.TENSOR_SCALAR, af.COEF_CONST, f, 2, xyzr, af.PREC_FP16X3, out=Fe)
+            Asm.assemble_elements(A32, Fe.view(n_tets, 10), val=val16, rhs=rhs16, status=False)
+
+        ms16 = timed_events(mfma_pass, 5, 1, torch)
+        Asm.assemble(D=Dp, F=f, val=val, rhs=rhs, status=False, stream=stream)
+        torch.cuda.synchronize()
+        res["fp16x3_mfma_elements_then_scatter"] = {
+            "ms_per_pass": ms16, "element_matrices_per_s": n_tets / ms16 * 1e3,
+            "max_entry_error_rel_to_max": float((val16 - val).abs().max() / val.abs().max())}
+        del A64, A32, err, Fe, val16, rhs16, xyzr
         out["element_only"] = res
         # the plain FP64 method on the GPU (reference algorithm per tet, FP64
         # atomic scatter) against the mixed-precision row-block assembly

@@ -224,6 +224,25 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
  * block.  Both sum in the same order: the results are bitwise identical. */
 int anifem_assembly_set_persistent(anifem_assembly* ctx, int on);
 
+/* Assembly of caller-supplied element matrices, the drop-in for
+ * AssemblerT::Assemble with an arbitrary local function (the MatFuncWrap
+ * plugin, func_wrap.h:96-187, whose result is the dense column-major local
+ * matrix plus the local load vector, func_wrap.inl:138-141), evaluated for all
+ * cells beforehand.  A_elem: device, n_tets matrices of nf x nf, column-major
+ * (entry (ib, ia) at A_elem[t*nf*nf + ib + nf*ia]); F_elem: device, n_tets
+ * vectors of nf, or NULL (rhs is then zeroed); both in the caller's tet order
+ * and in the local DOF order of the tet's nodes as anifem_assembly_tets
+ * returns them (after the orientation reorder, ordering.inl:8-26, as the
+ * reference hands the reordered nodes to the local function).  elem_precision:
+ * ANIFEM_PREC_FP64 (double) or ANIFEM_PREC_FP32 (float) element values; the
+ * CSR and rhs accumulate in FP64, with element-level homogeneous Dirichlet and
+ * the same deterministic order as anifem_assembly_run. */
+int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const void* F_elem, int elem_precision,
+                                 double* csr_val, double* rhs, anifem_status* status, void* stream);
+/* Host copy of the tets' node lists after the orientation reorder, caller's
+ * tet order (n_tets x 4). */
+int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets);
+
 /* The plain FP64 method as a measurement baseline: one lane per tet follows
  * the reference's algorithm step by step (U = PSI^T Ghat and DU = w|T| D U at
  * each quadrature point, A = V^T DU; core.inl:218-367), element-level

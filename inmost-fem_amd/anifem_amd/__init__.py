@@ -118,6 +118,8 @@ SIGNATURES = {
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
     "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_run_elements": (_I32, [_VP, _VP, _VP, _I32, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_tets": (_I32, [_VP, _VP]),
     "anifem_assembly_run_plain": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _VP,
                                          ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_set_persistent": (_I32, [_VP, _I32]),
@@ -376,6 +378,33 @@ class Assembler:
                                              _stream_handle(stream))
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run_plain")
+        return val, rhs, (st if status else None)
+
+    def tets(self):
+        """(n_tets, 4) int32 node lists after the orientation reorder: the
+        local node order element matrices for assemble_elements refer to."""
+        out = np.zeros((self.n_tets, 4), np.int32)
+        _check(lib().anifem_assembly_tets(self._h, _ptr(out)), "anifem_assembly_tets")
+        return out
+
+    def assemble_elements(self, A_elem, F_elem=None, val=None, rhs=None, status=True, stream=None):
+        """Scatter caller-supplied element matrices (n_tets, nf, nf) in the
+        column-major layout of fem3dtet_batch (float64 or float32) and load
+        vectors (n_tets, nf) into the FP64 CSR (anifem_assembly_run_elements)."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if val is None:
+            val = torch.empty(self.nnz, dtype=torch.float64, device=dev)
+        if rhs is None and F_elem is not None:
+            rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
+        prec = PREC_FP64 if A_elem.dtype == torch.float64 else PREC_FP32
+        if A_elem.dtype not in (torch.float64, torch.float32) or (F_elem is not None and F_elem.dtype != A_elem.dtype):
+            raise AnifemError(ERR_ARG, "element arrays must be float64 or float32, both of one type")
+        st = anifem_status()
+        rc = lib().anifem_assembly_run_elements(self._h, _ptr(A_elem), _ptr(F_elem), prec, _ptr(val), _ptr(rhs),
+                                                ctypes.byref(st) if status else None, _stream_handle(stream))
+        if rc not in (OK, ERR_NONFINITE):
+            raise AnifemError(rc, "anifem_assembly_run_elements")
         return val, rhs, (st if status else None)
 
     def set_persistent(self, on=True):

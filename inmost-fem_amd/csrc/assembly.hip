@@ -494,6 +494,69 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     return ANIFEM_OK;
 }
 
+int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets) {
+    if (!ctx || !tets) return ANIFEM_ERR_ARG;
+    std::memcpy(tets, ctx->S.tets.data(), ctx->S.tets.size() * sizeof(int32_t));
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const void* F_elem, int elem_precision,
+                                 double* csr_val, double* rhs, anifem_status* status, void* stream) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only || !A_elem || !csr_val) return ANIFEM_ERR_ARG;
+    if (elem_precision != ANIFEM_PREC_FP64 && elem_precision != ANIFEM_PREC_FP32) return ANIFEM_ERR_ARG;
+    const AssemblySetup& S = ctx->S;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    BlockArgs a{};
+    a.has_rhs = (F_elem && rhs) ? 1 : 0;
+    a.n_kinds = 1;
+    a.kinds[0] = KindTable{nullptr, nullptr, nullptr, nullptr, 0, 0, S.nf, 0, 0, 0};
+    a.n_blocks = S.n_blocks;
+    a.meta = ctx->d_blk_meta;
+    a.blk_row0 = ctx->d_blk_row0;
+    a.blk_s0 = ctx->d_blk_s0;
+    a.blk_p0 = ctx->d_blk_p0;
+    a.blk_v0 = ctx->d_blk_v0;
+    a.pair_tet = ctx->d_pair_tet;
+    a.pair_ib = ctx->d_pair_ib;
+    a.pair_pos = ctx->d_pair_pos;
+    a.sstart = ctx->d_sstart;
+    a.pstart = ctx->d_pstart;
+    a.val = csr_val;
+    a.rhs = rhs;
+    a.status = ctx->d_status;
+    a.max_values = (int)S.max_block_values;
+    a.max_pairs = (int)S.max_block_pairs;
+    a.max_slots = (int)S.max_block_slots;
+    // a rhs without load vectors is zero
+    if (rhs && !a.has_rhs && (rc = fill(rhs, S.ndof, s))) return rc;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    if (S.n_blocks > 0) {
+        const size_t lds = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0).total;
+        const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
+        if (elem_precision == ANIFEM_PREC_FP64)
+            hipLaunchKernelGGL(row_block_elements_kernel<double>, dim3(grid), dim3(kBlock), lds, s, a,
+                               static_cast<const double*>(A_elem), static_cast<const double*>(F_elem));
+        else
+            hipLaunchKernelGGL(row_block_elements_kernel<float>, dim3(grid), dim3(kBlock), lds, s, a,
+                               static_cast<const float*>(A_elem), static_cast<const float*>(F_elem));
+        ANIFEM_HIP_CHECK(hipGetLastError());
+    }
+    if (status) {
+        int h[4] = {0, 0, 0, 0};
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+        status->n_nonfinite = h[0];
+        status->n_degenerate = 0;
+        status->code = h[0] ? ANIFEM_ERR_NONFINITE : ANIFEM_OK;
+        status->reserved = 0;
+        return status->code;
+    }
+    return ANIFEM_OK;
+}
+
 int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, double* csr_val,
                               double* rhs, anifem_status* status, void* stream) {
     using namespace anifem;

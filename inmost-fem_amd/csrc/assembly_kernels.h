@@ -161,6 +161,40 @@ __host__ __device__ inline BlockLds block_lds_layout(int max_values, int max_pai
     return L;
 }
 
+// Phase B of a row block: one lane per slot adds the slot's entries in list
+// (ascending tet) order and stores the CSR value once; one lane per row does
+// the same for its pairs' load-vector entries.
+__device__ __forceinline__ void row_block_finish(const BlockArgs& a, int32_t r0, int nr, int64_t s0, int ns, int np,
+                                                 int nv, const double* vals, const double* fvals,
+                                                 const uint16_t* sst) {
+    for (int u = threadIdx.x; u < ns; u += kBlock) {
+        const int cb = sst[u];
+        const int ce = u + 1 < ns ? sst[u + 1] : nv;
+        double sum = 0.0;
+        int c = cb;
+        // loads issued four at a time, added in order
+        for (; c + 4 <= ce; c += 4) {
+            const double v0 = vals[c], v1 = vals[c + 1], v2 = vals[c + 2], v3 = vals[c + 3];
+            sum += v0;
+            sum += v1;
+            sum += v2;
+            sum += v3;
+        }
+        for (; c < ce; ++c) sum += vals[c];
+        a.val[s0 + u] = sum;
+    }
+    // one lane per row: its pairs' load-vector entries, ascending tet order
+    if (a.has_rhs) {
+        for (int u = threadIdx.x; u < nr; u += kBlock) {
+            const int pb = a.pstart[r0 + u];
+            const int pe = u + 1 < nr ? a.pstart[r0 + u + 1] : np;
+            double sum = 0.0;
+            for (int p = pb; p < pe; ++p) sum += fvals[p];
+            a.rhs[r0 + u] = sum;
+        }
+    }
+}
+
 // RS: the reference tensors are staged in LDS (else read from global memory
 // when they do not fit); fixed per instantiation so that every load has a
 // known address space.
@@ -302,33 +336,55 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     }
     __syncthreads();
 
-    // phase B: one lane per slot, entries added in list (ascending tet) order
-    for (int u = threadIdx.x; u < ns; u += kBlock) {
-        const int cb = sst[u];
-        const int ce = u + 1 < ns ? sst[u + 1] : nv;
-        double sum = 0.0;
-        int c = cb;
-        // loads issued four at a time, added in order
-        for (; c + 4 <= ce; c += 4) {
-            const double v0 = vals[c], v1 = vals[c + 1], v2 = vals[c + 2], v3 = vals[c + 3];
-            sum += v0;
-            sum += v1;
-            sum += v2;
-            sum += v3;
+    row_block_finish(a, r0, nr, s0, ns, np, nv, vals, fvals, sst);
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
+// Row-block pass over caller-supplied element matrices (the local matrices a
+// MatFuncWrap returns, func_wrap.h:96-187): Ae holds n_tets nf x nf matrices,
+// column-major (entry (ib, ia) at Ae[t*nf*nf + ib + nf*ia]), Fe n_tets load
+// vectors of nf (or null), in each tet's local DOF order after the node
+// reorder.  Phase A reads the pair's row of its tet's matrix instead of forming
+// it; Dirichlet rows and phase B are those of row_block_kernel.
+template <typename TE>
+__global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a, const TE* __restrict__ Ae,
+                                                                    const TE* __restrict__ Fe) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0);
+    double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
+    double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
+    uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
+    const int64_t per_xcd = (a.n_blocks + 7) / 8;
+    const int64_t k = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (k >= a.n_blocks) return;  // uniform over the workgroup
+    const BlockMeta M = a.meta[k];
+    const int nf = a.kinds[M.kind].nf;
+    const bool has_pair = (int)threadIdx.x < M.np;
+    block_copy(sst, a.sstart + M.s0, M.ns);
+    int nonfinite = 0;
+    if (has_pair) {
+        const int64_t gp = M.p0 + threadIdx.x;
+        const int64_t qp = M.q0 + (int64_t)threadIdx.x * nf;
+        const int64_t tet = a.pair_tet[gp];
+        const int ibf = a.pair_ib[gp];
+        const int ib = ibf & 63;
+        const bool dir = (ibf & kPairDirichlet) != 0;
+        const TE* At = Ae + tet * nf * nf + ib;
+        for (int ia = 0; ia < nf; ++ia) {
+            const int ps = a.pair_pos[qp + ia];
+            if (ps == kPosDropped) continue;
+            const double v = dir ? 1.0 : (double)At[(int64_t)nf * ia];
+            nonfinite += isfinite(v) ? 0 : 1;
+            vals[ps] = v;
         }
-        for (; c < ce; ++c) sum += vals[c];
-        a.val[s0 + u] = sum;
-    }
-    // one lane per row: its pairs' load-vector entries, ascending tet order
-    if (a.has_rhs) {
-        for (int u = threadIdx.x; u < nr; u += kBlock) {
-            const int pb = a.pstart[r0 + u];
-            const int pe = u + 1 < nr ? a.pstart[r0 + u + 1] : np;
-            double sum = 0.0;
-            for (int p = pb; p < pe; ++p) sum += fvals[p];
-            a.rhs[r0 + u] = sum;
+        if (a.has_rhs) {
+            const double fv = dir ? 0.0 : (double)Fe[tet * nf + ib];
+            nonfinite += isfinite(fv) ? 0 : 1;
+            fvals[threadIdx.x] = fv;
         }
     }
+    __syncthreads();
+    row_block_finish(a, M.r0, M.nr, M.s0, M.ns, M.np, M.nv, vals, fvals, sst);
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
