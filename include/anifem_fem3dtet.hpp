@@ -86,7 +86,7 @@ struct TensorView {
     const double* data;
 };
 
-enum Precision { FP64 = ANIFEM_PREC_FP64, FP32 = ANIFEM_PREC_FP32 };
+enum Precision { FP64 = ANIFEM_PREC_FP64, FP32 = ANIFEM_PREC_FP32, FP16X3 = ANIFEM_PREC_FP16X3 };
 
 // Checks a view against the traits: a fixed TensorType must match, and a
 // constant tensor must use constant storage.
@@ -165,6 +165,20 @@ public:
         if (F) tF = anifem_tensor{F->type, F->storage, F->data};
         return anifem_assembly_run(ctx_, &tD, F ? &tF : nullptr, nullptr, precision, csr_val, rhs, st, stream);
     }
+    // Assemble from element matrices computed by the caller (the reference's
+    // local-function path): A_elem n_tets x nf x nf column-major, F_elem
+    // n_tets x nf or null, FP64 (double) or FP32 (float) values, in the local
+    // order of the reordered nodes (Tets()).
+    template <typename TE>
+    int AssembleElements(const TE* A_elem, const TE* F_elem, double* csr_val, double* rhs,
+                         anifem_status* st = nullptr, void* stream = nullptr) {
+        static_assert(sizeof(TE) == 8 || sizeof(TE) == 4, "double or float element values");
+        if (!ctx_) return status_ ? status_ : ANIFEM_ERR_ARG;
+        return anifem_assembly_run_elements(ctx_, A_elem, F_elem, sizeof(TE) == 8 ? FP64 : FP32, csr_val, rhs, st,
+                                            stream);
+    }
+    // Host copy of the tets' nodes after the orientation reorder (n_tets x 4)
+    int Tets(int32_t* tets) const { return ctx_ ? anifem_assembly_tets(ctx_, tets) : ANIFEM_ERR_ARG; }
     anifem_assembly* handle() const { return ctx_; }
 
 private:

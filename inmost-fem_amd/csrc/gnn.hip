@@ -420,15 +420,25 @@ __global__ __launch_bounds__(NT) void colsum_partial_kernel(int64_t n, const flo
     }
 }
 
-// out[e] = sum_b partial[b][e] in block order (e < len); scatter[e] maps e
-// to its gradient index (or consecutive from `base` when null)
-__global__ void reduce_partials_kernel(int nblk, int len, const float* partial, float* grad, int base,
-                                       const int* scatter) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
+// grad[base + e] = sum_b partial[b * stride + e] for e < count: 32 entries per
+// workgroup, eight lanes per entry each adding the blocks b = sub (mod 8) in
+// order, then the eight subsums in order (a fixed summation order)
+constexpr int kRedSub = NT / 32;
+__global__ __launch_bounds__(NT) void reduce_partials_kernel(int nblk, int stride, int count, const float* partial,
+                                                             float* grad, int base) {
+    __shared__ float sh[kRedSub][32];
+    const int el = threadIdx.x % 32, sub = threadIdx.x / 32;
+    const int e = blockIdx.x * 32 + el;
     float acc = 0;
-    for (int b = 0; b < nblk; ++b) acc += partial[(int64_t)b * len + e];
-    grad[scatter ? scatter[e] : base + e] = acc;
+    if (e < count)
+        for (int b = sub; b < nblk; b += kRedSub) acc += partial[(int64_t)b * stride + e];
+    sh[sub][el] = acc;
+    __syncthreads();
+    if (sub == 0 && e < count) {
+        float t = 0;
+        for (int k = 0; k < kRedSub; ++k) t += sh[k][el];
+        grad[base + e] = t;
+    }
 }
 
 __global__ void loss_total_kernel(int nblk, const float* partial, float scale, float* loss) {
@@ -498,7 +508,6 @@ struct anifem_gnn {
     // parameters
     float *theta = nullptr, *pw = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
     float *partial = nullptr, *loss = nullptr;
-    int* scatter = nullptr;  // head gradient index map
     // M = G G^T as an explicit matrix: its CSR pattern, and per entry the slot
     // pairs (G_ik, G_jk) of its terms in ascending k
     int64_t m_nnz = 0;
@@ -519,6 +528,13 @@ int dalloc(anifem_gnn* g, T** p, size_t count) {
 }
 
 unsigned nb(int64_t work, int threads = NT) { return (unsigned)std::max<int64_t>(1, (work + threads - 1) / threads); }
+
+// entries [0, count) of the chunk partials (row stride `stride`) summed into grad[base ..]
+void reduce_partials(hipStream_t s, int64_t nblk, int stride, int count, const float* partial, float* grad,
+                     int base) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nb(count, 32)), dim3(NT), 0, s, (int)nblk, stride, count, partial,
+                       grad, base);
+}
 
 // forward through the network: G32 / G64 (if non-null) and, for the loss,
 // GT32
@@ -837,16 +853,12 @@ int anifem_gnn_loss_grad(anifem_gnn* g, const double* a, const float* probes, ui
     hipLaunchKernelGGL(head_back_kernel, dim3(nb(n * H)), dim3(NT), 0, s, n, g->rp, g->ci, g->tslot, g->dG, g->Lv,
                        g->Dh, g->at, g->h[NL], g->pw, g->dh, g->gwo, g->gwd, g->gsc);
     hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->gwo, H, g->partial);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(H), 0, s, (int)g->nchunk, H, g->partial, g->grad,
-                       OFF_WO, (const int*)nullptr);
+    reduce_partials(s, g->nchunk, H, H, g->partial, g->grad, OFF_WO);
     hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->gwd, H, g->partial);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(H), 0, s, (int)g->nchunk, H, g->partial, g->grad,
-                       OFF_WD, (const int*)nullptr);
+    reduce_partials(s, g->nchunk, H, H, g->partial, g->grad, OFF_WD);
     hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->gsc, 2, g->partial);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1), 0, s, (int)g->nchunk, 2, g->partial, g->grad,
-                       OFF_CO, (const int*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1), 0, s, (int)g->nchunk, 2, g->partial + 1, g->grad,
-                       OFF_CD, (const int*)nullptr);
+    reduce_partials(s, g->nchunk, 2, 1, g->partial, g->grad, OFF_CO);
+    reduce_partials(s, g->nchunk, 2, 1, g->partial + 1, g->grad, OFF_CD);
     // message-passing layers, last to first
     for (int l = NL - 1; l >= 0; --l) {
         hipLaunchKernelGGL(layer_back_kernel, dim3(nb(n, NODES)), dim3(NT), 0, s, n, g->dh, g->z[l], g->pw, l, g->dz,
@@ -854,14 +866,11 @@ int anifem_gnn_loss_grad(anifem_gnn* g, const double* a, const float* probes, ui
         // dW_s = dz^T h, dW_m = dz^T m, db = sum dz
         hipLaunchKernelGGL(outer_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->dz, g->h[l], H, 1,
                            g->partial);
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nb(H * H)), dim3(NT), 0, s, (int)g->nchunk, H * H + H,
-                           g->partial, g->grad, off_ws(l), (const int*)nullptr);
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(H), 0, s, (int)g->nchunk, H * H + H,
-                           g->partial + H * H, g->grad, off_b(l), (const int*)nullptr);
+        reduce_partials(s, g->nchunk, H * H + H, H * H, g->partial, g->grad, off_ws(l));
+        reduce_partials(s, g->nchunk, H * H + H, H, g->partial + H * H, g->grad, off_b(l));
         hipLaunchKernelGGL(outer_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->dz, g->m[l], H, 0,
                            g->partial);
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nb(H * H)), dim3(NT), 0, s, (int)g->nchunk, H * H,
-                           g->partial, g->grad, off_wm(l), (const int*)nullptr);
+        reduce_partials(s, g->nchunk, H * H, H * H, g->partial, g->grad, off_wm(l));
         // dh_prev = tmp + At dm
         hipLaunchKernelGGL(spmm_kernel, dim3(nb(n * H)), dim3(NT), 0, s, n, g->rp, g->ci, g->at, g->dm, g->tmp,
                            g->dh);
@@ -870,8 +879,7 @@ int anifem_gnn_loss_grad(anifem_gnn* g, const double* a, const float* probes, ui
     hipLaunchKernelGGL(relu_mask_kernel, dim3(nb(n * H)), dim3(NT), 0, s, n, g->dh, g->z0, g->dz);
     hipLaunchKernelGGL(outer_partial_kernel, dim3((unsigned)g->nchunk), dim3(NT), 0, s, n, g->dz, g->x, FIN, 1,
                        g->partial);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(NT), 0, s, (int)g->nchunk, H * FIN + H, g->partial,
-                       g->grad, OFF_WIN, (const int*)nullptr);
+    reduce_partials(s, g->nchunk, H * FIN + H, H * FIN + H, g->partial, g->grad, OFF_WIN);
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (loss) {
         ANIFEM_HIP_CHECK(hipMemcpyAsync(loss, g->loss, sizeof(float), hipMemcpyDeviceToHost, s));

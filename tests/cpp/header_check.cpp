@@ -40,6 +40,11 @@ int main() {
     const TensorView null_const{TENSOR_NULL, ANIFEM_COEF_CONST, nullptr};
     CHECK((fem3Dtet<GP2, Operator<IDEN, FemFix<FEM_P2>>, DfuncTraits<TENSOR_NULL, true>>(buf, 1, null_const, buf, 2) ==
            ANIFEM_ERR_DIMS));
+    // a context that could not be created reports its status from every call
+    Assembler<GP2, GP2> bad(0, nullptr, 0, nullptr, 2, sym_tet, 2, null_const, ANIFEM_DIRICHLET_NONE);
+    CHECK(bad.status() != ANIFEM_OK);
+    CHECK(bad.AssembleElements<double>(nullptr, nullptr, nullptr, nullptr) == bad.status());
+    CHECK(bad.AssembleElements<float>(nullptr, nullptr, nullptr, nullptr) == bad.status());
     std::printf("header ok\n");
     return 0;
 }
