@@ -154,6 +154,14 @@ typedef struct anifem_status {
  * uploads. */
 int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
                            void* stream);
+/* anifem_assembly_create with explicit row-block budgets (CSR slots, element
+ * entries and (row, tet) pairs per block; at most 65535, 65535 and 256).  The
+ * default chooses them by the entries per pair: 2048 / 2048 / 256 up to 10,
+ * 4096 / 4096 / 256 up to 20, 4096 / 8192 / 256 above.  Returns ANIFEM_ERR_DIMS
+ * when one row alone exceeds them.  No reference counterpart. */
+int anifem_assembly_create_budget(anifem_assembly** ctx, const anifem_mesh_desc* mesh,
+                                  const anifem_problem_desc* prob, int max_slots, int max_values, int max_pairs,
+                                  void* stream);
 int anifem_assembly_destroy(anifem_assembly* ctx);
 /* Host-only variant: builds the same setup without touching the device
  * (inspection of the enumeration, pattern, slot map and row blocks through

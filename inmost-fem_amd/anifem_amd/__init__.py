@@ -105,6 +105,8 @@ SIGNATURES = {
     "anifem_op_dim": (_I32, [anifem_op]),
     "anifem_quadrature_npoints": (_I32, [_I32]),
     "anifem_quadrature": (_I32, [_I32, _VP, _VP]),
+    "anifem_assembly_create_budget": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
+                                             ctypes.POINTER(anifem_problem_desc), _I32, _I32, _I32, _VP]),
     "anifem_assembly_create": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
                                       ctypes.POINTER(anifem_problem_desc), _VP]),
     "anifem_assembly_create_host": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
@@ -275,7 +277,7 @@ class Assembler:
     def __init__(self, coords, tets, trial: Op, test: Op = None, order=2, tensor_type=TENSOR_NULL,
                  tensor_storage=COEF_CONST, rhs_order=None, rhs_tensor_type=TENSOR_SCALAR,
                  rhs_tensor_storage=COEF_CONST, dirichlet=DIRICHLET_NONE, dirichlet_dof_mask=None, host_only=False,
-                 stream=None):
+                 budget=None, stream=None):
         self._coords = np.ascontiguousarray(coords, dtype=np.float64)
         self._tets = np.ascontiguousarray(tets, dtype=np.int32)
         test = test or trial
@@ -291,6 +293,10 @@ class Assembler:
         L = lib()
         if host_only:
             rc = L.anifem_assembly_create_host(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob))
+        elif budget is not None:  # (max_slots, max_values, max_pairs) per row block
+            rc = L.anifem_assembly_create_budget(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob),
+                                                 int(budget[0]), int(budget[1]), int(budget[2]),
+                                                 _stream_handle(stream))
         else:
             rc = L.anifem_assembly_create(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob),
                                           _stream_handle(stream))

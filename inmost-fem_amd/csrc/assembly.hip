@@ -233,13 +233,14 @@ __global__ __launch_bounds__(128) void plain_kernel(anifem::ElemArgs m, anifem::
 }
 
 int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-               const anifem::BlockCaps& caps = anifem::BlockCaps()) {
+               const anifem::BlockCaps* caps = nullptr) {
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
-    return build_setup(*mesh, *prob, ctx->S, caps);
+    const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
+    return build_setup(*mesh, *prob, ctx->S, caps ? *caps : default_caps(nf));
 }
 
 }  // namespace
@@ -251,7 +252,19 @@ int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, co
 
 extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
                                       const anifem_problem_desc* prob, void* stream) {
-    return anifem::assembly_create_caps(out, mesh, prob, anifem::BlockCaps(), stream);
+    if (!prob || !anifem::op_valid(prob->test)) return ANIFEM_ERR_ARG;
+    const int nf = anifem::nf_scalar(prob->test.fem) * prob->test.ncomp;
+    return anifem::assembly_create_caps(out, mesh, prob, anifem::default_caps(nf), stream);
+}
+
+extern "C" int anifem_assembly_create_budget(anifem_assembly** out, const anifem_mesh_desc* mesh,
+                                             const anifem_problem_desc* prob, int max_slots, int max_values,
+                                             int max_pairs, void* stream) {
+    anifem::BlockCaps c;
+    c.max_slots = max_slots;
+    c.max_values = max_values;
+    c.max_pairs = max_pairs;
+    return anifem::assembly_create_caps(out, mesh, prob, c, stream);
 }
 
 // Assembly context with given row-block budgets (the batch autotuner's trials).
@@ -260,7 +273,7 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     using namespace anifem;
     if (!out) return ANIFEM_ERR_ARG;
     std::unique_ptr<anifem_assembly> ctx(new anifem_assembly());
-    int rc = build_host(ctx.get(), mesh, prob, caps);
+    int rc = build_host(ctx.get(), mesh, prob, &caps);
     if (rc) return rc;
     AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -20,6 +20,20 @@ struct BlockCaps {
     int max_values = 2048;  // element entries per block (<= 65535)
     int max_pairs = 256;    // (row, incident tet) pairs per block (<= workgroup size)
 };
+// Default budgets by entries per (row, tet) pair: a block of 256 pairs of nf
+// entries needs about 256 nf values, so larger element matrices get larger
+// value / slot budgets (measured: P3 at 4096 / 4096, P2^3 at 8192 values).
+inline BlockCaps default_caps(int nf) {
+    BlockCaps c;
+    if (nf > 20) {
+        c.max_slots = 4096;
+        c.max_values = 8192;
+    } else if (nf > 10) {
+        c.max_slots = 4096;
+        c.max_values = 4096;
+    }
+    return c;
+}
 constexpr uint16_t kPosDropped = 0xFFFF;  // pair_pos of an entry the pattern drops
 constexpr uint8_t kPairDirichlet = 0x80;  // pair_ib flag: the pair's row is Dirichlet
 
