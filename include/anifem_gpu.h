@@ -303,6 +303,10 @@ int anifem_batch_info(const anifem_batch* ctx, int mesh, int64_t* tet_offset, in
 int anifem_batch_csr(const anifem_batch* ctx, int mesh, const int64_t** row_ptr, const int32_t** col_idx);
 /* chosen budgets {slots, entries, pairs} per block and the timed candidates (ms; -1: not applicable) */
 int anifem_batch_tuning(const anifem_batch* ctx, int fem, int32_t* caps, double* ms);
+/* Row-block kernel choice of the batch's graph (default 1: the persistent
+ * pass, element kinds restaged per workgroup range; 0: one workgroup per
+ * block).  Both sum in the same order: the results are bitwise identical. */
+int anifem_batch_set_persistent(anifem_batch* b, int on);
 /* kappa: device, all tets in mesh order; f: device, one FP64 value; val (total
  * nnz) and rhs (total ndof): device FP64, overwritten */
 int anifem_batch_run(anifem_batch* ctx, const double* kappa, const double* f, double* val, double* rhs,

@@ -134,6 +134,7 @@ SIGNATURES = {
     "anifem_batch_info": (_I32, [_VP, _I32, _VP, _VP, _VP, _VP, _VP]),
     "anifem_batch_csr": (_I32, [_VP, _I32, _VP, _VP]),
     "anifem_batch_tuning": (_I32, [_VP, _I32, _VP, _VP]),
+    "anifem_batch_set_persistent": (_I32, [_VP, _I32]),
     "anifem_batch_run": (_I32, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
                                    ctypes.POINTER(anifem_pcg_result), _VP]),
@@ -608,6 +609,9 @@ class Batch:
         ms = np.zeros(4)
         _check(lib().anifem_batch_tuning(self._h, fem, _ptr(caps), _ptr(ms)), "anifem_batch_tuning")
         return caps, ms
+
+    def set_persistent(self, on=True):
+        _check(lib().anifem_batch_set_persistent(self._h, 1 if on else 0), "anifem_batch_set_persistent")
 
     def run(self, kappa, f, val=None, rhs=None, status=True, stream=None):
         import torch

@@ -122,7 +122,7 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     // persistent pass for scalar spaces with one carried point
     constexpr bool scalar_space = NCA == 1 && NCB == 1;
     if (scalar_space && ctx->persistent && staged && len == DA0 * DB0 && (!a.has_rhs || lenF == 1) &&
-        a.kinds[0].nf <= 20 && a.n_kinds == 1) {
+        a.kinds[0].nf <= 20) {
         int occ = 0, n_cu = 0;
         ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, T>), kBlock, lds));

@@ -423,8 +423,9 @@ struct PairRegs {
 };
 
 // Persistent row-block pass for scalar spaces with one carried point (the
-// matrix tensor constant over each tet): each workgroup stages the reference
-// tensors once and walks its blocks; while block b is evaluated, the next
+// matrix tensor constant over each tet) and elements of at most 20 entries
+// per row: each workgroup stages the reference tensors of its blocks' element
+// kind (restaged when the kind changes) and walks its blocks; while block b is evaluated, the next
 // block's pair records and value positions are in flight, and its geometry
 // rows are requested once those have arrived, so the gather latency hides
 // behind phases A and B.  Same arithmetic and summation order as
@@ -438,22 +439,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lenp = pad16<T>(KB);
     const int lenFp = a.has_rhs ? pad16<T>(1) : 0;
-    const KindTable& KT = a.kinds[0];
-    const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0;
-    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T), KT.R_rows * lenp,
-                                        a.has_rhs ? KT.RF_rows * lenFp : 0);
+    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T), a.max_R_rows * lenp,
+                                        a.has_rhs ? a.max_RF_rows * lenFp : 0);
     double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
     double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
     uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
     T* R = reinterpret_cast<T*>(smem_raw + L.R);
     T* RF = reinterpret_cast<T*>(smem_raw + L.RF);
-    if constexpr (sizeof(T) == 8) {
-        pad_copy(R, reinterpret_cast<const T*>(KT.Rd), KT.R_rows, KB, lenp);
-        if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFd), KT.RF_rows, 1, lenFp);
-    } else {
-        pad_copy(R, reinterpret_cast<const T*>(KT.Rf), KT.R_rows, KB, lenp);
-        if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFf), KT.RF_rows, 1, lenFp);
-    }
+    // reference tensors of one element kind in LDS (a workgroup's contiguous
+    // range of the processing order rarely crosses kinds: they are restaged
+    // only when it does)
+    auto stage = [&](int kind) {
+        const KindTable& KT = a.kinds[kind];
+        if constexpr (sizeof(T) == 8) {
+            pad_copy(R, reinterpret_cast<const T*>(KT.Rd), KT.R_rows, KB, lenp);
+            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFd), KT.RF_rows, 1, lenFp);
+        } else {
+            pad_copy(R, reinterpret_cast<const T*>(KT.Rf), KT.R_rows, KB, lenp);
+            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFf), KT.RF_rows, 1, lenFp);
+        }
+    };
     const int kstride = KB;  // one component block, one point
 
     BlockWalk W;
@@ -462,6 +467,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     if (b < 0) return;  // uniform over the workgroup
 
     auto load_records = [&](const BlockMeta& Mx, PairRegs<KB, T>& P) {
+        const int nf = a.kinds[Mx.kind].nf;
         const bool hp = (int)threadIdx.x < Mx.np;
         const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
         P.tet = hp ? a.pair_tet[gp] : 0;
@@ -480,6 +486,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     };
 
     BlockMeta M = a.meta[b];
+    int staged_kind = (int)M.kind;
+    stage(staged_kind);
     PairRegs<KB, T> cur, nxt;
     load_records(M, cur);
     load_k(M, cur);
@@ -490,6 +498,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 
     int nonfinite = 0;
     for (;;) {
+        if ((int)M.kind != staged_kind) {  // uniform: the previous block ended with a barrier
+            staged_kind = (int)M.kind;
+            stage(staged_kind);
+            __syncthreads();
+        }
+        const int nfA0 = a.kinds[staged_kind].nfA0, nfB0 = a.kinds[staged_kind].nfB0;
         // slot starts of b (overlap phase A), records of the next block
         block_copy(sst, a.sstart + M.s0, (int)M.ns);
         if (bn >= 0) load_records(Mn, nxt);

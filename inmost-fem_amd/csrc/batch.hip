@@ -75,6 +75,7 @@ struct anifem_batch {
     anifem::BlockCaps caps[kKinds];
     double tune_ms[kKinds][kCandidates] = {};
     bool kind_used[kKinds] = {false, false, false};
+    bool persistent = true;  // persistent row-block pass (else one workgroup per block)
     // device
     double* d_coords = nullptr;
     int32_t* d_tetnodes = nullptr;
@@ -225,6 +226,17 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
                               : block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0).total;
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
 
+    // persistent row-block pass (element kinds restaged per workgroup range):
+    // occupancy x CUs workgroups, whole XCD groups
+    int64_t pgrid = 0;
+    if (staged && b->persistent) {
+        int occ = 0, n_cu = 0;
+        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, reinterpret_cast<const void*>(&row_block_persistent<3, 3, double>), kBlock, lds));
+        ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+        pgrid = std::min<int64_t>((int64_t)std::max(occ, 1) * std::max(n_cu, 1), b->n_blocks);
+        pgrid = std::max<int64_t>(8, pgrid / 8 * 8);
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -236,7 +248,10 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
                        b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
-    if (staged)
+    if (pgrid > 0)
+        hipLaunchKernelGGL((row_block_persistent<3, 3, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+                           (const double*)b->d_K);
+    else if (staged)
         hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else
@@ -428,6 +443,18 @@ int anifem_batch_tuning(const anifem_batch* b, int fem, int32_t* caps, double* m
     }
     if (ms)
         for (int c = 0; c < kCandidates; ++c) ms[c] = b->tune_ms[k][c];
+    return ANIFEM_OK;
+}
+
+int anifem_batch_set_persistent(anifem_batch* b, int on) {
+    if (!b) return ANIFEM_ERR_ARG;
+    if (b->persistent != (on != 0)) {
+        b->persistent = on != 0;
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;  // captured again on the next run
+        b->graph = nullptr;
+    }
     return ANIFEM_OK;
 }
 

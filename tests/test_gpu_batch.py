@@ -78,3 +78,22 @@ def test_autotuner_reports_budgets():
         k = [i for i, c in enumerate([(2048, 1024, 128), (2048, 2048, 256), (4096, 4096, 256),
                                       (1024, 1024, 256)]) if c == tuple(int(x) for x in caps)][0]
         assert ms[k] == ms[ms > 0].min()
+
+
+def test_batch_persistent_and_per_block_bitwise_equal():
+    """The persistent row-block pass (element kinds restaged as a workgroup's
+    block range crosses meshes of another order) against one workgroup per
+    block: identical values and rhs."""
+    import torch
+    meshes, kappas, kappa = build(SPECS, seed=3)
+    f = torch.ones(1, dtype=torch.float64, device="cuda")
+    B = af.Batch(meshes, autotune=False)
+    v1, r1, st = B.run(kappa, f)
+    assert st.code == 0
+    v1, r1 = v1.clone(), r1.clone()
+    B.set_persistent(False)
+    v2, r2, _ = B.run(kappa, f)
+    assert torch.equal(v1, v2) and torch.equal(r1, r2)
+    B.set_persistent(True)
+    v3, r3, _ = B.run(kappa, f)
+    assert torch.equal(v1, v3) and torch.equal(r1, r3)
