@@ -328,6 +328,11 @@ int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_t* col_idx,
                       const double* b, double* x, double rtol, int64_t maxit, anifem_pcg_result* res,
                       void* stream);
 
+/* The solvers keep their device workspaces per host thread and device
+ * (grown on demand, reused by later calls, so a solve allocates nothing once
+ * its sizes have been seen); this frees the calling thread's workspaces. */
+int anifem_pcg_release(void);
+
 /* Flexible PCG (Polak-Ribiere beta) with a factored preconditioner
  * M^-1 = G G^T: G and GT are device FP64 values of G and of G^T on the CSR
  * pattern of A (which must be symmetric).  Stopping rule and reporting as for

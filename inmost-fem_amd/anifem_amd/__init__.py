@@ -138,6 +138,7 @@ SIGNATURES = {
     "anifem_batch_run": (_I32, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
                                    ctypes.POINTER(anifem_pcg_result), _VP]),
+    "anifem_pcg_release": (_I32, []),
     "anifem_pcg_matrix": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
                                  ctypes.POINTER(anifem_pcg_result), _VP]),
     "anifem_gnn_n_params": (_I32, []),

@@ -712,6 +712,41 @@ int run_iterations(int64_t maxit, const PcgState* st, F&& enqueue, hipStream_t s
     return rc;
 }
 
+// Device workspaces of the solvers, per host thread, device and slot: grown
+// on demand and kept for the next call, so a solve allocates nothing once its
+// sizes have been seen (anifem_pcg_release frees them).  Every solver returns
+// after synchronizing its stream, so a workspace is idle between calls.
+struct WorkEntry {
+    int dev, slot;
+    void* p;
+    size_t bytes;
+};
+constexpr int kWorkEntries = 16;
+thread_local WorkEntry tl_work[kWorkEntries];
+thread_local int tl_work_used = 0;
+
+int workspace(int slot, size_t bytes, void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ANIFEM_ERR_HIP;
+    WorkEntry* e = nullptr;
+    for (int i = 0; i < tl_work_used; ++i)
+        if (tl_work[i].dev == dev && tl_work[i].slot == slot) e = &tl_work[i];
+    if (!e) {
+        if (tl_work_used == kWorkEntries) return ANIFEM_ERR_NOMEM;
+        e = &tl_work[tl_work_used++];
+        *e = WorkEntry{dev, slot, nullptr, 0};
+    }
+    if (e->bytes < bytes) {
+        if (e->p) (void)hipFree(e->p);
+        e->p = nullptr;
+        e->bytes = 0;
+        if (hipMalloc(&e->p, bytes) != hipSuccess) return ANIFEM_ERR_NOMEM;
+        e->bytes = bytes;
+    }
+    *out = e->p;
+    return ANIFEM_OK;
+}
+
 // Final report: the true residual ||b - A x|| from x, and the state's counters.
 int finish(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* ci, const double* v, const double* b,
            const double* x, double* Ax, double* part, const PcgState* st, anifem_pcg_result* res, hipStream_t s) {
@@ -747,7 +782,10 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     double* work = nullptr;
     const size_t nv = (size_t)n;
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 3 * kRedBlocks) + sizeof(PcgState)));
+    {
+        const int wrc = workspace(0, sizeof(double) * (6 * nv + 3 * kRedBlocks) + sizeof(PcgState), (void**)&work);
+        if (wrc) return wrc;
+    }
     double *r = work, *z = r + nv, *pb[2] = {z + nv, z + 2 * nv}, *Ap = z + 3 * nv, *dinv = Ap + nv, *part = dinv + nv;
     PcgState* st = reinterpret_cast<PcgState*>(part + 3 * kRedBlocks);
     double *partA = part, *partU = part + kRedBlocks;  // p.Ap; (r.z, r.r)
@@ -769,7 +807,7 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
         },
         s);
     if (rc == ANIFEM_OK) rc = finish(n, nnz, row_ptr, col_idx, val, b, x, Ap, part, st, res, s);
-    (void)hipFree(work);
+    if (rc != ANIFEM_OK) (void)hipStreamSynchronize(s);  // the workspace is idle on return
     return rc;
 }
 
@@ -792,7 +830,10 @@ extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     double* work = nullptr;
     const size_t nv = (size_t)n;
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 4 * kRedBlocks) + sizeof(PcgState)));
+    {
+        const int wrc = workspace(0, sizeof(double) * (6 * nv + 4 * kRedBlocks) + sizeof(PcgState), (void**)&work);
+        if (wrc) return wrc;
+    }
     double *rb[2] = {work, work + nv}, *z = work + 2 * nv, *pb[2] = {z + nv, z + 2 * nv}, *Ap = z + 3 * nv,
            *part = Ap + nv;
     PcgState* st = reinterpret_cast<PcgState*>(part + 4 * kRedBlocks);
@@ -817,7 +858,7 @@ extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_
         },
         s);
     if (rc == ANIFEM_OK) rc = finish(n, nnz, row_ptr, col_idx, val, b, x, Ap, part, st, res, s);
-    (void)hipFree(work);
+    if (rc != ANIFEM_OK) (void)hipStreamSynchronize(s);  // the workspace is idle on return
     return rc;
 }
 
@@ -836,7 +877,10 @@ extern "C" int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int3
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     double* work = nullptr;
     const size_t nv = (size_t)n;
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&work, sizeof(double) * (6 * nv + 5 * kRedBlocks) + sizeof(PcgState)));
+    {
+        const int wrc = workspace(0, sizeof(double) * (6 * nv + 5 * kRedBlocks) + sizeof(PcgState), (void**)&work);
+        if (wrc) return wrc;
+    }
     double *r = work, *t = r + nv, *za = t + nv, *zb = za + nv, *p = zb + nv, *Ap = p + nv, *part = Ap + nv;
     PcgState* st = reinterpret_cast<PcgState*>(part + 5 * kRedBlocks);
     double* partB = part + 2 * kRedBlocks;  // update partials (r.r)
@@ -866,7 +910,7 @@ extern "C" int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int3
         },
         s);
     if (rc == ANIFEM_OK) rc = finish(n, nnz, row_ptr, col_idx, val, b, x, Ap, part, st, res, s);
-    (void)hipFree(work);
+    if (rc != ANIFEM_OK) (void)hipStreamSynchronize(s);  // the workspace is idle on return
     return rc;
 }
 
@@ -886,7 +930,8 @@ extern "C" int anifem_pcg_learned(anifem_gnn* g, const double* val, const double
     int rc = anifem_gnn_views(g, &n, &rp, &ci, &G, &GT);
     if (rc) return rc;
     double* x0 = nullptr;
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&x0, n * sizeof(double)));
+    rc = workspace(1, n * sizeof(double), (void**)&x0);
+    if (rc) return rc;
     anifem_pcg_learned_result out{};
     if (hipMemcpyAsync(x0, x, n * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) rc = ANIFEM_ERR_HIP;
     if (!rc) rc = anifem_gnn_factor(g, val, nullptr, nullptr, stream);
@@ -923,7 +968,15 @@ extern "C" int anifem_pcg_learned(anifem_gnn* g, const double* val, const double
             if (!rc) rc = rc2;
         }
     }
-    (void)hipFree(x0);
+    if (rc != ANIFEM_OK) (void)hipStreamSynchronize(s);
     if (res) *res = out;
     return rc;
+}
+
+extern "C" int anifem_pcg_release(void) {
+    using namespace anifem;
+    for (int i = 0; i < tl_work_used; ++i)
+        if (tl_work[i].p) (void)hipFree(tl_work[i].p);
+    tl_work_used = 0;
+    return ANIFEM_OK;
 }

@@ -109,3 +109,27 @@ def test_pcg_iteration_counts_across_poll_chunks():
     x.zero_()
     r2, ok2 = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9, maxit=k - 1)
     assert not ok2 and r2.iterations == k - 1
+
+
+def test_pcg_workspace_reuse_and_release():
+    """Repeated solves reuse the cached workspace (bitwise-equal results), a
+    larger system grows it, and after anifem_pcg_release the next solve
+    allocates afresh."""
+    import torch
+    out = []
+    for n in (6, 9, 6):
+        coords, tets = af.kuhn_cube(n, 0.2, 5)
+        Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P1), dirichlet=af.DIRICHLET_BOUNDARY,
+                           rhs_tensor_type=af.TENSOR_SCALAR, rhs_tensor_storage=af.COEF_CONST)
+        val, rhs, _ = Asm.assemble(D=None, F=torch.ones(1, dtype=torch.float64, device="cuda"))
+        rp, ci = Asm.csr()
+        x = torch.zeros(Asm.ndof, dtype=torch.float64, device="cuda")
+        r, ok = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9)
+        assert ok
+        out.append(x.cpu().numpy())
+    np.testing.assert_array_equal(out[0], out[2])
+    assert af.lib().anifem_pcg_release() == af.OK
+    x.zero_()
+    r, ok = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-9)
+    assert ok
+    np.testing.assert_array_equal(x.cpu().numpy(), out[2])
