@@ -370,6 +370,13 @@ def main():
 
     sections = set() if args.skip_extras else {x.strip() for x in args.sections.split(",") if x.strip()}
     if "precision" in sections:
+        # the coloured scatter (greedy colouring, one launch per colour) beside the row blocks
+        Asm.set_scatter(af.SCATTER_COLOURED)
+        n_col, _ = Asm.colours()
+        sec_col = timed(af.PREC_FP64, max(5, args.steps // 2), 2)
+        Asm.set_scatter(af.SCATTER_ROW_BLOCKS)
+        out["coloured_scatter_fp64"] = {"ms_per_step": sec_col * 1e3, "n_colours": n_col,
+                                        "row_blocks_speedup": sec_col / sec64}
         sec32 = timed(af.PREC_FP32, max(5, args.steps // 2), 2)
         out["fp32_contraction"] = {"ms_per_step": sec32 * 1e3, "element_matrices_per_s": n_tets / sec32,
                                    "speedup_vs_fp64": sec64 / sec32}

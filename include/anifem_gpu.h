@@ -232,6 +232,21 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
  * block.  Both sum in the same order: the results are bitwise identical. */
 int anifem_assembly_set_persistent(anifem_assembly* ctx, int on);
 
+/* Scatter method of anifem_assembly_run (no reference counterpart: the
+ * reference scatters with one thread, assembler.inl:381-481).
+ *   ANIFEM_SCATTER_ROW_BLOCKS (default): owner-computes row blocks, each value
+ *     written once, sums in the reference's ascending tet order.
+ *   ANIFEM_SCATTER_COLOURED: first-fit greedy colouring of the tets (no two
+ *     tets of a colour share a node), one launch per colour adding each entry
+ *     to its slot; deterministic (colour order), no atomics; values equal to
+ *     the row blocks' up to the order of the additions.
+ * The colouring is built on the first switch to COLOURED. */
+enum { ANIFEM_SCATTER_ROW_BLOCKS = 0, ANIFEM_SCATTER_COLOURED = 1 };
+int anifem_assembly_set_scatter(anifem_assembly* ctx, int method);
+/* the colouring (after set_scatter(COLOURED)): count and, if colour is not
+ * NULL, the colour of each tet (host, n_tets) */
+int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int32_t* colour);
+
 /* Assembly of caller-supplied element matrices, the drop-in for
  * AssemblerT::Assemble with an arbitrary local function (the MatFuncWrap
  * plugin, func_wrap.h:96-187, whose result is the dense column-major local

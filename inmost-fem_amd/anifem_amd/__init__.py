@@ -30,6 +30,7 @@ FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
 TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
 COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_FP16X3 = 0, 1, 2
+SCATTER_ROW_BLOCKS, SCATTER_COLOURED = 0, 1
 DIRICHLET_NONE, DIRICHLET_BOUNDARY, DIRICHLET_DOF_MASK = 0, 1, 2
 
 _ERRNAMES = {ERR_NONFINITE: "non-finite local entry", ERR_CONNECTIVITY: "broken connectivity",
@@ -122,6 +123,8 @@ SIGNATURES = {
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_run_elements": (_I32, [_VP, _VP, _VP, _I32, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_tets": (_I32, [_VP, _VP]),
+    "anifem_assembly_set_scatter": (_I32, [_VP, _I32]),
+    "anifem_assembly_colours": (_I32, [_VP, _VP, _VP]),
     "anifem_assembly_run_plain": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _VP,
                                          ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_set_persistent": (_I32, [_VP, _I32]),
@@ -417,6 +420,17 @@ class Assembler:
 
     def set_persistent(self, on=True):
         _check(lib().anifem_assembly_set_persistent(self._h, 1 if on else 0), "set_persistent")
+
+    def set_scatter(self, method):
+        """SCATTER_ROW_BLOCKS (default) or SCATTER_COLOURED."""
+        _check(lib().anifem_assembly_set_scatter(self._h, method), "anifem_assembly_set_scatter")
+
+    def colours(self):
+        """(n_colours, colour per tet) of the coloured scatter."""
+        n = ctypes.c_int32()
+        col = np.zeros(self.n_tets, np.int32)
+        _check(lib().anifem_assembly_colours(self._h, ctypes.byref(n), _ptr(col)), "anifem_assembly_colours")
+        return n.value, col
 
     def enable_timers(self, on=True):
         _check(lib().anifem_assembly_enable_timers(self._h, 1 if on else 0), "enable_timers")

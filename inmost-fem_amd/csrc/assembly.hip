@@ -51,6 +51,11 @@ struct anifem_assembly {
     bool host_only = false;           // built by anifem_assembly_create_host
     bool timers_on = false;
     bool persistent = true;           // persistent row-block pass where it applies
+    // coloured scatter (anifem_assembly_set_scatter): tets by colour, in
+    // ascending tet order within a colour; colour c is [colour_ptr[c], colour_ptr[c+1])
+    bool coloured = false;
+    int32_t* d_col_tets = nullptr;
+    std::vector<int64_t> colour_ptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     anifem_timers timers{};
 };
@@ -78,6 +83,87 @@ template <typename T>
 int upload(T** dst, const T* src, size_t count, hipStream_t s) {
     ANIFEM_HIP_CHECK(hipMalloc((void**)dst, std::max<size_t>(count, 1) * sizeof(T)));
     if (count) ANIFEM_HIP_CHECK(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return ANIFEM_OK;
+}
+
+// One colour of the coloured scatter: one lane per (tet of the colour, entry
+// e = ib + nf*ia).  The entry is formed from the tet's geometry row as in the
+// row-block pass and added to its CSR slot; tets of one colour share no node,
+// hence no DOF, so no two lanes of a launch touch one value and the colours
+// run one launch after another: no atomics, a fixed (colour) order.  Element-
+// level Dirichlet as applyDir (dc_on_dof.h:38-45): a kept entry of a
+// Dirichlet row is its diagonal, 1; the row's load-vector entry is 0.
+template <int DA0, int DB0, int NCA, int NCB, typename T>
+__global__ __launch_bounds__(anifem::kBlock) void colour_scatter_kernel(anifem::BlockArgs a, const T* __restrict__ Kg,
+                                                                        const int32_t* __restrict__ ctets, int64_t n_c,
+                                                                        const int32_t* __restrict__ slot,
+                                                                        const int32_t* __restrict__ dofs,
+                                                                        const uint8_t* __restrict__ isdir) {
+    using namespace anifem;
+    const KindTable& KT = a.kinds[0];
+    const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0, E = nf * nf;
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (i >= n_c * E) return;
+    const int64_t t = ctets[i / E];
+    const int e = (int)(i % E);
+    const int ib = e % nf, ia = e / nf;
+    const int ib0 = ib % nfB0, cb = ib / nfB0, ia0 = ia % nfA0, ca = ia / nfA0;
+    const int len = a.m.q * DA0 * DB0;
+    const T* R = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.Rd : (const void*)KT.Rf);
+    const T* Kb = Kg + t * a.rowlen + (cb * NCA + ca) * len;
+    const T* Rr = R + (ib0 + nfB0 * ia0) * len;
+    T acc = 0;
+    for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
+    const int32_t dof = dofs[t * nf + ib];
+    const bool dir = isdir[dof] != 0;
+    int bad = 0;
+    const int32_t sl = slot[t * E + e];
+    if (sl >= 0) {
+        const double v = dir ? 1.0 : (double)acc;
+        bad += isfinite(v) ? 0 : 1;
+        a.val[sl] += v;
+    }
+    if (a.has_rhs && ia == 0) {
+        const int lenF = a.f.q;
+        const T* RF = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.RFd : (const void*)KT.RFf);
+        const T* KF = Kg + t * a.rowlen + NCA * NCB * len + cb * lenF;
+        T fa = 0;
+        for (int j = 0; j < lenF; ++j) fa += KF[j] * RF[ib0 * lenF + j];
+        const double fv = dir ? 0.0 : (double)fa;
+        bad += isfinite(fv) ? 0 : 1;
+        a.rhs[dof] += fv;
+    }
+    if (bad) atomicAdd(&a.status[0], bad);
+}
+
+template <int DA0, int DB0, int NCA, int NCB, typename T>
+int launch_coloured(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, hipStream_t s) {
+    using namespace anifem;
+    const AssemblySetup& S = ctx->S;
+    int rc;
+    if ((rc = fill(a.val, S.nnz, s))) return rc;
+    if (a.has_rhs && (rc = fill(a.rhs, S.ndof, s))) return rc;
+    const int64_t E = (int64_t)S.nf * S.nf;
+    for (size_t c = 0; c + 1 < ctx->colour_ptr.size(); ++c) {
+        const int64_t n_c = ctx->colour_ptr[c + 1] - ctx->colour_ptr[c];
+        const int64_t blocks = (n_c * E + kBlock - 1) / kBlock;
+        if (!ctx->d_slot || !ctx->d_dofs || !ctx->d_isdir || !ctx->d_col_tets) return ANIFEM_ERR_HIP;
+        hipLaunchKernelGGL((colour_scatter_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                           a, Kg, ctx->d_col_tets + ctx->colour_ptr[c], n_c, ctx->d_slot, ctx->d_dofs, ctx->d_isdir);
+    }
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
+// Element-to-slot map, element DOFs and Dirichlet flags on the device, for the
+// paths that scatter per element (plain baseline, coloured scatter); uploaded
+// once, each array on its own.
+int upload_maps(anifem_assembly* ctx, hipStream_t s) {
+    const anifem::AssemblySetup& S = ctx->S;
+    int rc;
+    if (!ctx->d_slot && (rc = upload(&ctx->d_slot, S.slot.data(), S.slot.size(), s))) return rc;
+    if (!ctx->d_dofs && (rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
+    if (!ctx->d_isdir && (rc = upload(&ctx->d_isdir, S.is_dir.data(), S.is_dir.size(), s))) return rc;
     return ANIFEM_OK;
 }
 
@@ -118,6 +204,7 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
                        S.n_tets);
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
+    if (ctx->coloured) return launch_coloured<DA0, DB0, NCA, NCB, T>(ctx, a, Kg, s);
     if (S.n_blocks == 0) return ANIFEM_OK;
     // persistent pass for scalar spaces with one carried point
     constexpr bool scalar_space = NCA == 1 && NCB == 1;
@@ -232,6 +319,37 @@ __global__ __launch_bounds__(128) void plain_kernel(anifem::ElemArgs m, anifem::
     if (bad) atomicAdd(status, bad);
 }
 
+// First-fit greedy colouring of the tets in ascending order: two tets
+// conflict when they share a node (then they may share a DOF; tets with no
+// common node share none).  Returns the number of colours.
+int greedy_colour(int64_t n_nodes, int64_t n_tets, const int32_t* tets, std::vector<int32_t>& colour) {
+    std::vector<int64_t> ptr((size_t)n_nodes + 1, 0);
+    for (int64_t k = 0; k < 4 * n_tets; ++k) ptr[(size_t)tets[k] + 1]++;
+    for (int64_t i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
+    std::vector<int32_t> adj((size_t)(4 * n_tets));
+    for (int64_t t = 0; t < n_tets; ++t)
+        for (int k = 0; k < 4; ++k) adj[(size_t)fillp[(size_t)tets[4 * t + k]]++] = (int32_t)t;
+    colour.assign((size_t)n_tets, -1);
+    std::vector<uint64_t> used;
+    int ncol = 0;
+    for (int64_t t = 0; t < n_tets; ++t) {
+        used.assign((size_t)ncol / 64 + 1, 0);
+        for (int k = 0; k < 4; ++k) {
+            const int32_t v = tets[4 * t + k];
+            for (int64_t q = ptr[v]; q < ptr[v + 1]; ++q) {
+                const int32_t c = colour[(size_t)adj[(size_t)q]];
+                if (c >= 0) used[(size_t)c >> 6] |= uint64_t(1) << (c & 63);
+            }
+        }
+        int c = 0;
+        while ((size_t)(c >> 6) < used.size() && (used[(size_t)c >> 6] >> (c & 63)) & 1) ++c;
+        colour[(size_t)t] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+
 int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
                const anifem::BlockCaps* caps = nullptr) {
     using namespace anifem;
@@ -328,7 +446,7 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
         void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_meta, ctx->d_blk_s0, ctx->d_blk_p0,
                         ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
-                        ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab};
+                        ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -395,6 +513,48 @@ int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, i
     cp(pair_pos, S.pair_pos);
     cp(sstart, S.sstart);
     cp(pstart, S.pstart);
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_set_scatter(anifem_assembly* ctx, int method) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only) return ANIFEM_ERR_ARG;
+    if (method == ANIFEM_SCATTER_ROW_BLOCKS) {
+        ctx->coloured = false;
+        return ANIFEM_OK;
+    }
+    if (method != ANIFEM_SCATTER_COLOURED) return ANIFEM_ERR_ARG;
+    if (!ctx->d_col_tets) {
+        const AssemblySetup& S = ctx->S;
+        std::vector<int32_t> colour;
+        const int nc = greedy_colour(S.n_nodes, S.n_tets, S.tets.data(), colour);
+        if (nc < 0) return ANIFEM_ERR_NOMEM;
+        ctx->colour_ptr.assign((size_t)nc + 1, 0);
+        for (int64_t t = 0; t < S.n_tets; ++t) ctx->colour_ptr[(size_t)colour[t] + 1]++;
+        for (int c = 0; c < nc; ++c) ctx->colour_ptr[c + 1] += ctx->colour_ptr[c];
+        std::vector<int64_t> fillp(ctx->colour_ptr.begin(), ctx->colour_ptr.end() - 1);
+        std::vector<int32_t> ct((size_t)S.n_tets);
+        for (int64_t t = 0; t < S.n_tets; ++t) ct[(size_t)fillp[(size_t)colour[t]]++] = (int32_t)t;
+        hipStream_t s = nullptr;
+        int rc;
+        if ((rc = upload_maps(ctx, s))) return rc;
+        if ((rc = upload(&ctx->d_col_tets, ct.data(), ct.size(), s))) return rc;
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    ctx->coloured = true;
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int32_t* colour) {
+    if (!ctx || !n_colours) return ANIFEM_ERR_ARG;
+    if (ctx->colour_ptr.empty()) return ANIFEM_ERR_ARG;  // anifem_assembly_set_scatter(COLOURED) first
+    *n_colours = (int32_t)ctx->colour_ptr.size() - 1;
+    if (colour) {
+        std::vector<int32_t> ct((size_t)ctx->S.n_tets);
+        ANIFEM_HIP_CHECK(hipMemcpy(ct.data(), ctx->d_col_tets, ct.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (size_t c = 0; c + 1 < ctx->colour_ptr.size(); ++c)
+            for (int64_t k = ctx->colour_ptr[c]; k < ctx->colour_ptr[c + 1]; ++k) colour[ct[(size_t)k]] = (int32_t)c;
+    }
     return ANIFEM_OK;
 }
 
@@ -589,10 +749,10 @@ int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, cons
     const int q = quadrature(P.order, &pq, &wq);
     const int qf = quadrature(P.rhs_order, &pf, &wf);
     if (q < 0 || qf < 0) return ANIFEM_ERR_ORDER;
-    if (!ctx->d_slot) {
-        if ((rc = upload(&ctx->d_slot, S.slot.data(), S.slot.size(), s))) return rc;
-        if ((rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
-        if ((rc = upload(&ctx->d_isdir, S.is_dir.data(), S.is_dir.size(), s))) return rc;
+    if ((rc = upload_maps(ctx, s))) return rc;
+    // the reference tables of this path are built on its first call, apart
+    // from the element maps (which the coloured scatter may have uploaded)
+    if (!ctx->d_plain_tab) {
         std::vector<double> g, v, tab;
         reference_table(ANIFEM_GRAD, S.fem, q, pq, g);
         reference_table(ANIFEM_IDEN, S.fem, qf, pf, v);
@@ -602,6 +762,7 @@ int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, cons
         tab.insert(tab.end(), wf, wf + qf);
         if ((rc = upload(&ctx->d_plain_tab, tab.data(), tab.size(), s))) return rc;
     }
+    if (!ctx->d_plain_tab || !ctx->d_slot || !ctx->d_dofs || !ctx->d_isdir) return ANIFEM_ERR_HIP;
     const double* gtab = ctx->d_plain_tab;
     const double* wqd = gtab + (size_t)q * nf * 3;
     const double* vtab = wqd + q;

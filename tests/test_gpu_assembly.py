@@ -287,3 +287,74 @@ def test_assemble_elements_from_batch_matches_fused():
         print(f"precision {prec}: elements-then-scatter vs fused, max entry error {ev:.2e}, rhs {er:.2e}")
         assert ev <= tol and er <= tol, (prec, ev, er)
     del torch
+
+
+@pytest.mark.parametrize("fem,nc,tt", [(af.FEM_P2, 1, af.TENSOR_SYMMETRIC), (af.FEM_P1, 3, af.TENSOR_LAME),
+                                       (af.FEM_P3, 1, af.TENSOR_SCALAR)])
+def test_coloured_scatter_matches_row_blocks(fem, nc, tt):
+    """The coloured scatter (greedy colouring, one launch per colour) against
+    the row-block pass: the oracle's first-fit colouring, conflict-free
+    colours, values equal up to the order of the additions, repeat runs
+    bitwise identical."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(31 + fem)
+    coords, tets = af.kuhn_cube(5, 0.2, 12)
+    n = len(tets)
+    order = 4 if fem == af.FEM_P3 else 2
+    if tt == af.TENSOR_SYMMETRIC:
+        D = dev(T.pack_sym(aniso(rng, n), 3).T.copy())
+    elif tt == af.TENSOR_LAME:
+        D = dev(np.stack([rng.uniform(1, 10, n), rng.uniform(1, 10, n)]))
+    else:
+        D = dev(10 ** rng.uniform(-1, 1, (1, n)))
+    rhs_tt = af.TENSOR_GENERAL if nc == 3 else af.TENSOR_SCALAR
+    F = dev(np.array([0.0, 0.0, -1.0])) if nc == 3 else dev(np.ones(1))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), order=order, tensor_type=tt,
+                       tensor_storage=af.COEF_PER_TET, rhs_order=order, rhs_tensor_type=rhs_tt,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
+    v_rb, r_rb, st = Asm.assemble(D=D, F=F)
+    assert st.code == 0
+    v_rb, r_rb = v_rb.cpu().numpy(), r_rb.cpu().numpy()
+    Asm.set_scatter(af.SCATTER_COLOURED)
+    ncol, col = Asm.colours()
+    col_or = np.zeros(n, np.int32)
+    tr = np.ascontiguousarray(Asm.tets())
+    assert T.oracle().orc_greedy_colour(ctypes.c_long(len(coords)), ctypes.c_long(n), T.P(tr), T.P(col_or)) == ncol
+    np.testing.assert_array_equal(col, col_or)
+    for c in range(ncol):  # no node in two tets of one colour
+        nodes = tr[col == c].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+    v_c, r_c, st = Asm.assemble(D=D, F=F)
+    assert st.code == 0
+    s = Asm.setup()
+    S = {"row_ptr": s["row_ptr"]}
+    compare(v_c.cpu().numpy(), r_c.cpu().numpy(), S, v_rb, r_rb, tol=1e-13)
+    v_c2, r_c2, _ = Asm.assemble(D=D, F=F)
+    assert torch.equal(v_c, v_c2) and torch.equal(r_c, r_c2)
+    Asm.set_scatter(af.SCATTER_ROW_BLOCKS)
+    v_b, _, _ = Asm.assemble(D=D, F=F)
+    np.testing.assert_array_equal(v_b.cpu().numpy(), v_rb)
+
+
+def test_plain_baseline_after_coloured_scatter():
+    """The plain baseline and the coloured scatter share the element maps of a
+    context: the plain path builds its own tables even when the coloured
+    scatter uploaded the maps first (regression: its tables were tied to the
+    maps' upload)."""
+    rng = np.random.default_rng(41)
+    coords, tets = af.kuhn_cube(4, 0.2, 13)
+    n = len(tets)
+    D = dev(T.pack_sym(aniso(rng, n), 3).T.copy())
+    F = dev(np.ones(1))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), order=2, tensor_type=af.TENSOR_SYMMETRIC,
+                       tensor_storage=af.COEF_PER_TET, rhs_order=2, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
+    v_rb, r_rb, _ = Asm.assemble(D=D, F=F)
+    Asm.set_scatter(af.SCATTER_COLOURED)
+    Asm.assemble(D=D, F=F)
+    Asm.set_scatter(af.SCATTER_ROW_BLOCKS)
+    v_p, r_p, st = Asm.assemble_plain(D=D, F=F)
+    assert st.code == 0
+    S = {"row_ptr": Asm.setup()["row_ptr"]}
+    compare(v_p.cpu().numpy(), r_p.cpu().numpy(), S, v_rb.cpu().numpy(), r_rb.cpu().numpy(), tol=1e-12)
