@@ -58,30 +58,47 @@ def aniso_packed(rng, n):
 
 def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
     """Reference element core (oracle/_ref, compiled from the AniFem++ sources)
-    on a bounded sample of the workload's tets, element-level homogeneous
-    Dirichlet (dc_on_dof.h:38-45), then the oracle's sequential cell-order
-    scatter; single thread on the host.  Falls back to the oracle's own
-    element restatement where the reference build is absent."""
+    on the workload's tets, element-level homogeneous Dirichlet
+    (dc_on_dof.h:38-45), then the oracle's sequential cell-order scatter.  The
+    element matrices are computed once on one host thread and once split over
+    min(16, host CPUs) threads (contiguous tet chunks; the reference's
+    assembler evaluates cells independently, assembler.inl:381); the scatter
+    is sequential.  Falls back to the oracle's own element restatement where
+    the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
     import anifem_testlib as T
 
     ref = T.ref_lib()
     kind = "reference" if ref is not None else "port"
     n_all = len(tets_reordered)
     n = n_all
-    xyz = coords[tets_reordered[:n]].reshape(n, 12)
+    xyz = np.ascontiguousarray(coords[tets_reordered[:n]].reshape(n, 12))
     Df = np.ascontiguousarray(Dfull[:n].transpose(0, 2, 1).reshape(n, 9))
-    t0 = time.perf_counter()
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def elements(lo, hi, A):
+        if ref is not None:
+            rc = ref.ref_fem3dtet(1, 3, 0, 2, ctypes.c_long(hi - lo), xyz[lo:hi].ctypes.data_as(dp),
+                                  Df[lo:hi].ctypes.data_as(dp), ctypes.c_long(9), A[lo:hi].ctypes.data_as(dp))
+            assert rc == 0
+        else:
+            A[lo:hi] = T.orc_elem(2, 3, 1, 2, 3, 1, 3, 1, 2, xyz[lo:hi], Df[lo:hi])
+
     if ref is not None:
-        A = np.zeros(n * 100)
-        dp = ctypes.POINTER(ctypes.c_double)
         ref.ref_fem3dtet.restype = ctypes.c_int
-        rc = ref.ref_fem3dtet(1, 3, 0, 2, ctypes.c_long(n), xyz.ctypes.data_as(dp), Df.ctypes.data_as(dp),
-                              ctypes.c_long(9), A.ctypes.data_as(dp))
-        assert rc == 0
-        A = A.reshape(n, 100)
-    else:
-        A = T.orc_elem(2, 3, 1, 2, 3, 1, 3, 1, 2, xyz, Df)
+    A = np.zeros((n, 100))
+    t0 = time.perf_counter()
+    elements(0, n, A)
+    t_one = time.perf_counter() - t0
+    threads = max(1, min(16, os.cpu_count() or 1))
+    A_mt = np.zeros((n, 100))
+    cuts = np.linspace(0, n, threads + 1).astype(int)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda k: elements(int(cuts[k]), int(cuts[k + 1]), A_mt), range(threads)))
+    t_mt = time.perf_counter() - t0
+    assert np.array_equal(A, A_mt)
     t1 = time.perf_counter()
     # applyDir with value 0 on every flagged local DOF: zero row and column, unit diagonal
     dmask = is_dir[dofs[:n]].astype(bool)  # (n, 10) local flags
@@ -92,12 +109,14 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
     rc, _, _ = T.oracle_scatter(n, 10, np.ascontiguousarray(dofs[:n]), np.ascontiguousarray(slot[:n]), A, None, nnz,
                                 ndof)
     assert rc == 0, rc
-    t2 = time.perf_counter()
-    rate = n / (t2 - t0)
-    return {"value": rate, "unit": "element matrices/s", "cores": 1, "kind": kind,
+    t_sc = time.perf_counter() - t1
+    rate = n / (t_mt + t_sc)
+    return {"value": rate, "unit": "element matrices/s", "cores": threads, "kind": kind,
+            "single_thread_value": n / (t_one + t_sc),
             "sample": f"all {n} of {n_all} tets: P2 GRAD-GRAD element matrices with per-tet SPD D "
-                      f"({'reference fem3Dtet compiled from the AniFem++ sources' if ref else 'oracle restatement'}, "
-                      f"{t1 - t0:.2f}s) + oracle sequential scatter into the same CSR ({t2 - t1:.2f}s)"}
+                      f"({'reference fem3Dtet compiled from the AniFem++ sources' if ref else 'oracle restatement'}: "
+                      f"{t_mt:.2f}s on {threads} threads, {t_one:.2f}s on one) + oracle sequential scatter into the "
+                      f"same CSR ({t_sc:.2f}s)"}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "assembly_p2_aniso_64cube_fp64_traffic.txt")

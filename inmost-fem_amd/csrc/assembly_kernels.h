@@ -118,7 +118,14 @@ __device__ __forceinline__ void pad_copy(V* __restrict__ dst, const V* __restric
 // values), stored to Kg[r*rowlen ...] in the T precision of the contraction.
 template <int DA0, int DB0, int NCA, int NCB, typename T>
 __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __restrict__ Kg, int64_t n_tets) {
-    const int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    // short rows (scalar spaces, one carried point) are assembled in LDS and
+    // stored by the workgroup as one contiguous range: coalesced stores
+    // instead of one strided store per value and lane
+    constexpr int kStageRow = 16;
+    __shared__ T stage[NCA * NCB == 1 ? kBlock * kStageRow : 1];
+    const bool staged = NCA * NCB == 1 && a.rowlen <= kStageRow;
+    const int64_t r0 = blockIdx.x * (int64_t)kBlock;
+    const int64_t r = r0 + threadIdx.x;
     int degenerate = 0;
     if (r < n_tets) {
         const int kstride = NCA * NCB * a.m.q * DA0 * DB0;
@@ -127,8 +134,16 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
         tet_geometry(P, PSI, det);
         degenerate = !(fabs(det) > 0.0 && isfinite(det));
         const double mes = fabs(det) / 6.0;
-        geometry_tensor<DA0, DB0, NCA, NCB, T>(a.m, r, PSI, mes, Kg + r * a.rowlen);
-        if (a.has_rhs) geometry_tensor<1, 1, 1, NCB, T>(a.f, r, PSI, mes, Kg + r * a.rowlen + kstride);
+        T* row = staged ? stage + threadIdx.x * a.rowlen : Kg + r * a.rowlen;
+        geometry_tensor<DA0, DB0, NCA, NCB, T>(a.m, r, PSI, mes, row);
+        if (a.has_rhs) geometry_tensor<1, 1, 1, NCB, T>(a.f, r, PSI, mes, row + kstride);
+    }
+    if (staged) {  // uniform over the workgroup
+        __syncthreads();
+        const int64_t nt = n_tets - r0 < kBlock ? n_tets - r0 : kBlock;
+        const int nv = (int)nt * a.rowlen;
+        T* dst = Kg + r0 * a.rowlen;
+        for (int i = threadIdx.x; i < nv; i += kBlock) dst[i] = stage[i];
     }
     // one counter update per wavefront
     const unsigned long long ballot = __ballot(degenerate);
