@@ -167,6 +167,23 @@ int upload_maps(anifem_assembly* ctx, hipStream_t s) {
     return ANIFEM_OK;
 }
 
+// Persistent row-block pass: occupancy x CUs workgroups, whole XCD groups
+// (surplus workgroups exit at once).
+template <int DA0, int DB0, int PW, typename T>
+int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
+    using namespace anifem;
+    int occ = 0, n_cu = 0;
+    ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T>), kBlock, lds));
+    ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
+    G = std::min<int64_t>(G, ctx->S.n_blocks);
+    G = std::max<int64_t>(8, G / 8 * 8);
+    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a, Kg);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
 template <int DA0, int DB0, int NCA, int NCB, typename T>
 int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     using namespace anifem;
@@ -210,17 +227,11 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     constexpr bool scalar_space = NCA == 1 && NCB == 1;
     if (scalar_space && ctx->persistent && staged && len == DA0 * DB0 && (!a.has_rhs || lenF == 1) &&
         a.kinds[0].nf <= 20) {
-        int occ = 0, n_cu = 0;
-        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, T>), kBlock, lds));
-        ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
-        G = std::min<int64_t>(G, S.n_blocks);
-        G = std::max<int64_t>(8, G / 8 * 8);  // whole XCD groups; surplus workgroups exit at once
-        hipLaunchKernelGGL((row_block_persistent<DA0, DB0, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a,
-                           (const T*)Kg);
-        ANIFEM_HIP_CHECK(hipGetLastError());
-        return ANIFEM_OK;
+        // position words per pair: the shortest instantiation that holds nf
+        const int nf = a.kinds[0].nf;
+        if (nf <= 4) return launch_persistent<DA0, DB0, 2, T>(ctx, a, Kg, lds, s);
+        if (nf <= 10) return launch_persistent<DA0, DB0, 5, T>(ctx, a, Kg, lds, s);
+        return launch_persistent<DA0, DB0, 10, T>(ctx, a, Kg, lds, s);
     }
     // 8 equal XCD ranges of the processing order (padding workgroups exit)
     const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
@@ -410,6 +421,9 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     if ((rc = upload(&ctx->d_pair_tet, S.pair_tet.data(), S.pair_tet.size(), s))) return rc;
     if ((rc = upload(&ctx->d_pair_ib, S.pair_ib.data(), S.pair_ib.size(), s))) return rc;
     if ((rc = upload(&ctx->d_pair_pos, S.pair_pos.data(), S.pair_pos.size(), s))) return rc;
+    // one entry of padding: the persistent pass reads slot starts as aligned
+    // 32-bit words, the last of which may reach one entry past the array
+    S.sstart.push_back(0);
     if ((rc = upload(&ctx->d_sstart, S.sstart.data(), S.sstart.size(), s))) return rc;
     if ((rc = upload(&ctx->d_pstart, S.pstart.data(), S.pstart.size(), s))) return rc;
     if ((rc = upload(&ctx->d_row_ptr, S.row_ptr.data(), S.row_ptr.size(), s))) return rc;

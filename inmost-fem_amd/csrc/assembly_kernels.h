@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
 // LDS layout of one row-block workgroup, in bytes from the start (all parts
 // 16-byte aligned): entry values | load-vector values | R | R_F | slot starts.
 struct BlockLds {
-    size_t vals, fvals, R, RF, sst, total;
+    size_t vals, fvals, R, RF, sst, pst, total;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -171,7 +171,9 @@ __host__ __device__ inline BlockLds block_lds_layout(int max_values, int max_pai
     L.RF = o;
     o = align16(o + (size_t)elem * RF_vals);
     L.sst = o;
-    o = align16(o + 2 * (size_t)max_slots);
+    o = align16(o + 2 * (size_t)max_slots + 4);  // + one word: slot starts may be staged from an odd entry
+    L.pst = o;
+    o = align16(o + 4 * (size_t)max_pairs);
     L.total = o;
     return L;
 }
@@ -426,31 +428,46 @@ struct BlockWalk {
     }
 };
 
-// One lane's pair of a block: its tet's geometry row (K block and load-vector
-// factor) and its value positions as 16-bit pairs.
-template <int KB, typename T>
+// One lane's share of a block's records: its pair's tet, geometry row (K block
+// and load-vector factor) and value positions as 16-bit pairs (PW words), and
+// SW 32-bit words of the block's slot starts and one row's first pair, which
+// are moved to LDS when the block's turn comes.
+template <int KB, int PW, int SW, typename T>
 struct PairRegs {
     int32_t tet;
     int ibf;
-    uint32_t posw[10];
+    uint32_t posw[PW];
     T k[KB];
     T kf;
+    uint32_t sw[SW > 0 ? SW : 1];
+    int pst;
 };
 
+// Slot starts of a block as aligned 32-bit words: the words covering
+// sstart[s0 .. s0+ns) (the last may reach one entry past the array, which the
+// upload pads).
+__device__ __forceinline__ int slot_words(int64_t s0, int ns) { return (int)(((s0 & 1) + ns + 1) >> 1); }
+
 // Persistent row-block pass for scalar spaces with one carried point (the
-// matrix tensor constant over each tet) and elements of at most 20 entries
+// matrix tensor constant over each tet) and elements of at most 2*PW entries
 // per row: each workgroup stages the reference tensors of its blocks' element
-// kind (restaged when the kind changes) and walks its blocks; while block b is evaluated, the next
-// block's pair records and value positions are in flight, and its geometry
-// rows are requested once those have arrived, so the gather latency hides
-// behind phases A and B.  Same arithmetic and summation order as
-// row_block_kernel: the results are bitwise identical.
-template <int DA0, int DB0, typename T>
-// Five waves per SIMD: 96 VGPRs without spills (measured faster than the
-// unconstrained 99 VGPRs / four waves, and than spilling to six or eight).
+// kind (restaged when the kind changes) and walks its blocks; while block b is
+// evaluated, the next block's pair records, value positions, slot starts and
+// row starts are in flight, and its geometry rows are requested once the
+// records have arrived, so the gather latency hides behind phases A and B.
+// Same arithmetic and summation order as row_block_kernel: the results are
+// bitwise identical.
+template <int DA0, int DB0, int PW, typename T>
+// Five waves per SIMD (the LDS of five 30 KB workgroups fills a CU): at most
+// 96 VGPRs, measured faster than four unconstrained waves and than spilling
+// to six or eight.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void row_block_persistent(BlockArgs a, const T* __restrict__ Kg) {
     constexpr int KB = DA0 * DB0;
-    constexpr int PW = 10;
+    // slot starts carried in registers for blocks of at most 2*SW*kBlock - 2
+    // slots (the registers freed by the shorter position lists of P1 and P2);
+    // else copied to LDS when the block's turn comes
+    constexpr int SW = PW <= 5 ? 5 : 0;
+    const bool sw_regs = SW > 0 && a.max_slots <= 2 * SW * kBlock - 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lenp = pad16<T>(KB);
     const int lenFp = a.has_rhs ? pad16<T>(1) : 0;
@@ -459,6 +476,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
     double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
     uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
+    int* pstl = reinterpret_cast<int*>(smem_raw + L.pst);
     T* R = reinterpret_cast<T*>(smem_raw + L.R);
     T* RF = reinterpret_cast<T*>(smem_raw + L.RF);
     // reference tensors of one element kind in LDS (a workgroup's contiguous
@@ -481,7 +499,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     int64_t b = W.take();
     if (b < 0) return;  // uniform over the workgroup
 
-    auto load_records = [&](const BlockMeta& Mx, PairRegs<KB, T>& P) {
+    using Regs = PairRegs<KB, PW, SW, T>;
+    auto load_records = [&](const BlockMeta& Mx, Regs& P) {
         const int nf = a.kinds[Mx.kind].nf;
         const bool hp = (int)threadIdx.x < Mx.np;
         const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
@@ -492,18 +511,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 #pragma unroll
         for (int w = 0; w < PW; ++w) P.posw[w] = (hp && 2 * w < nf) ? pw[w] : 0u;
     };
-    auto load_k = [&](const BlockMeta& Mx, PairRegs<KB, T>& P) {
+    auto load_k = [&](const BlockMeta& Mx, Regs& P) {
         const bool hp = (int)threadIdx.x < Mx.np;
         const T* Kt = Kg + (int64_t)P.tet * a.rowlen;
 #pragma unroll
         for (int j = 0; j < KB; ++j) P.k[j] = hp ? Kt[j] : T(0);
         P.kf = (hp && a.has_rhs) ? Kt[kstride] : T(0);
+        // slot and row starts, needed only at the block's phase B
+        if constexpr (SW > 0) {
+            if (sw_regs) {
+                const uint32_t* sg = reinterpret_cast<const uint32_t*>(a.sstart) + (Mx.s0 >> 1);
+                const int nw = slot_words(Mx.s0, (int)Mx.ns);
+#pragma unroll
+                for (int w = 0; w < SW; ++w) {
+                    const int i = w * kBlock + (int)threadIdx.x;
+                    P.sw[w] = i < nw ? sg[i] : 0u;
+                }
+            }
+        }
+        P.pst = (a.has_rhs && (int)threadIdx.x < Mx.nr) ? (int)a.pstart[Mx.r0 + threadIdx.x] : 0;
     };
 
     BlockMeta M = a.meta[b];
     int staged_kind = (int)M.kind;
     stage(staged_kind);
-    PairRegs<KB, T> cur, nxt;
+    Regs cur, nxt;
     load_records(M, cur);
     load_k(M, cur);
     int64_t bn = W.take();
@@ -519,8 +551,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
             __syncthreads();
         }
         const int nfA0 = a.kinds[staged_kind].nfA0, nfB0 = a.kinds[staged_kind].nfB0;
-        // slot starts of b (overlap phase A), records of the next block
-        block_copy(sst, a.sstart + M.s0, (int)M.ns);
+        // slot and row starts of b to LDS (read in phase B), records of the
+        // next block
+        const uint16_t* sstb = sst;
+        bool sst_copied = false;
+        if constexpr (SW > 0) {
+            if (sw_regs) {
+                uint32_t* s32 = reinterpret_cast<uint32_t*>(sst);
+                const int nw = slot_words(M.s0, (int)M.ns);
+#pragma unroll
+                for (int w = 0; w < SW; ++w) {
+                    const int i = w * kBlock + (int)threadIdx.x;
+                    if (i < nw) s32[i] = cur.sw[w];
+                }
+                sstb = sst + (M.s0 & 1);
+                sst_copied = true;
+            }
+        }
+        if (!sst_copied) block_copy(sst, a.sstart + M.s0, (int)M.ns);
+        if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
         if (bn >= 0) load_records(Mn, nxt);
         const int64_t bnn = W.take();
         BlockMeta Mnn{};
@@ -553,14 +602,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
             }
         }
         // geometry rows of the next block (its records have arrived by now)
+        __builtin_amdgcn_sched_barrier(0);
         if (bn >= 0) load_k(Mn, nxt);
         __syncthreads();
 
         // phase B of b
         const int ns = M.ns, nv = M.nv, nr = M.nr, np = M.np;
         for (int u = threadIdx.x; u < ns; u += kBlock) {
-            const int cb = sst[u];
-            const int ce = u + 1 < ns ? sst[u + 1] : nv;
+            const int cb = sstb[u];
+            const int ce = u + 1 < ns ? sstb[u + 1] : nv;
             double sum = 0.0;
             int c = cb;
             for (; c + 4 <= ce; c += 4) {
@@ -575,8 +625,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         }
         if (a.has_rhs) {
             for (int u = threadIdx.x; u < nr; u += kBlock) {
-                const int pb = a.pstart[M.r0 + u];
-                const int pe = u + 1 < nr ? a.pstart[M.r0 + u + 1] : np;
+                const int pb = pstl[u];
+                const int pe = u + 1 < nr ? pstl[u + 1] : np;
                 double sum = 0.0;
                 for (int p = pb; p < pe; ++p) sum += fvals[p];
                 a.rhs[M.r0 + u] = sum;

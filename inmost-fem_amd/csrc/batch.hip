@@ -228,11 +228,18 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
 
     // persistent row-block pass (element kinds restaged per workgroup range):
     // occupancy x CUs workgroups, whole XCD groups
+    // position words per pair: the shortest instantiation that holds the
+    // largest element of the kinds in use
+    int max_nf = 0;
+    for (int k = 0; k < kKinds; ++k)
+        if (b->kind_used[k]) max_nf = std::max(max_nf, a.kinds[k].nf);
+    const void* pkernel = max_nf <= 4    ? reinterpret_cast<const void*>(&row_block_persistent<3, 3, 2, double>)
+                          : max_nf <= 10 ? reinterpret_cast<const void*>(&row_block_persistent<3, 3, 5, double>)
+                                         : reinterpret_cast<const void*>(&row_block_persistent<3, 3, 10, double>);
     int64_t pgrid = 0;
     if (staged && b->persistent) {
         int occ = 0, n_cu = 0;
-        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, reinterpret_cast<const void*>(&row_block_persistent<3, 3, double>), kBlock, lds));
+        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pkernel, kBlock, lds));
         ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
         pgrid = std::min<int64_t>((int64_t)std::max(occ, 1) * std::max(n_cu, 1), b->n_blocks);
         pgrid = std::max<int64_t>(8, pgrid / 8 * 8);
@@ -248,8 +255,14 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
                        b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
-    if (pgrid > 0)
-        hipLaunchKernelGGL((row_block_persistent<3, 3, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+    if (pgrid > 0 && max_nf <= 4)
+        hipLaunchKernelGGL((row_block_persistent<3, 3, 2, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+                           (const double*)b->d_K);
+    else if (pgrid > 0 && max_nf <= 10)
+        hipLaunchKernelGGL((row_block_persistent<3, 3, 5, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+                           (const double*)b->d_K);
+    else if (pgrid > 0)
+        hipLaunchKernelGGL((row_block_persistent<3, 3, 10, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else if (staged)
         hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
@@ -382,6 +395,7 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     if ((rc = upload_b(p, &p->d_pair_tet, pair_tet, s))) return rc;
     if ((rc = upload_b(p, &p->d_pair_ib, pair_ib, s))) return rc;
     if ((rc = upload_b(p, &p->d_pair_pos, pair_pos, s))) return rc;
+    sstart.push_back(0);  // padding: slot starts are read as aligned 32-bit words
     if ((rc = upload_b(p, &p->d_sstart, sstart, s))) return rc;
     if ((rc = upload_b(p, &p->d_pstart, pstart, s))) return rc;
     if ((rc = upload_b(p, &p->d_row_ptr, row_ptr, s))) return rc;
