@@ -381,7 +381,7 @@ def main():
         "phase_ms": {"geometry": tm[0], "row_blocks": tm[1], "launches": nlaunch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_persistent<3,3,double>",
+                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_persistent<3,3,5,double>",
                      "ms_per_launch": ms_pass_events,
                      "algorithmic_bytes_per_pass": alg_bytes,
                      "traffic_source": traffic_src},
