@@ -184,6 +184,24 @@ int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T*
     return ANIFEM_OK;
 }
 
+// Persistent row-block pass for vector spaces (NC components of NF0 scalar
+// DOFs on both sides): occupancy x CUs workgroups, whole XCD groups.
+template <int DA0, int DB0, int NC, int NF0, typename T>
+int launch_persistent_vec(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
+    using namespace anifem;
+    int occ = 0, n_cu = 0;
+    ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void*>(&row_block_persistent_vec<DA0, DB0, NC, NF0, T>), kBlock, lds));
+    ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
+    G = std::min<int64_t>(G, ctx->S.n_blocks);
+    G = std::max<int64_t>(8, G / 8 * 8);
+    hipLaunchKernelGGL((row_block_persistent_vec<DA0, DB0, NC, NF0, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a,
+                       Kg);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
 template <int DA0, int DB0, int NCA, int NCB, typename T>
 int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     using namespace anifem;
@@ -232,6 +250,18 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
         if (nf <= 4) return launch_persistent<DA0, DB0, 2, T>(ctx, a, Kg, lds, s);
         if (nf <= 10) return launch_persistent<DA0, DB0, 5, T>(ctx, a, Kg, lds, s);
         return launch_persistent<DA0, DB0, 10, T>(ctx, a, Kg, lds, s);
+    }
+    // persistent pass for 3-component vector spaces of P1 or P2 components:
+    // the reference tensors staged in LDS whenever one workgroup fits a CU
+    // (vector blocks hold one or two workgroups per CU either way)
+    if constexpr (NCA == 3 && NCB == 3) {
+        const KindTable& K0 = a.kinds[0];
+        const size_t lds_v = lds_of(true);
+        if (ctx->persistent && lds_v <= (size_t)max_lds && a.max_R_rows * len < 65536 && len == DA0 * DB0 &&
+            (!a.has_rhs || lenF == 1) && K0.nfA0 == K0.nfB0) {
+            if (K0.nfA0 == 4) return launch_persistent_vec<DA0, DB0, 3, 4, T>(ctx, a, (const T*)Kg, lds_v, s);
+            if (K0.nfA0 == 10) return launch_persistent_vec<DA0, DB0, 3, 10, T>(ctx, a, (const T*)Kg, lds_v, s);
+        }
     }
     // 8 equal XCD ranges of the processing order (padding workgroups exit)
     const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));

@@ -643,4 +643,194 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
+// Persistent row-block pass for vector spaces FemVec<NC, FemFix<..>> x
+// FemVec<NC, ..> (NF0 scalar DOFs per component on both sides, one carried
+// point): the scheme of row_block_persistent with a pair's NC component
+// blocks (cb, 0..NC-1) of its tet's geometry row in registers.  Vector
+// elements fill a CU's LDS with one or two workgroups, so the next block's
+// records, geometry rows, slot starts and row starts are all carried in
+// registers (no occupancy cap on the register count), which is what hides
+// the gather latency.  Same arithmetic and summation order as
+// row_block_kernel: the results are bitwise identical.
+template <int DA0, int DB0, int NC, int NF0, typename T>
+__global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, const T* __restrict__ Kg) {
+    constexpr int KB = DA0 * DB0;
+    constexpr int NF = NC * NF0;       // entries per pair
+    constexpr int PW = NF / 2;         // position words per pair
+    static_assert(NF % 2 == 0, "positions are read as 32-bit words");
+    constexpr int SW = 9;              // slot-start words per lane: blocks of at most 4606 slots
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lenp = pad16<T>(KB);
+    const int lenFp = a.has_rhs ? pad16<T>(1) : 0;
+    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T), a.max_R_rows * lenp,
+                                        a.has_rhs ? a.max_RF_rows * lenFp : 0);
+    double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
+    double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
+    uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
+    int* pstl = reinterpret_cast<int*>(smem_raw + L.pst);
+    T* R = reinterpret_cast<T*>(smem_raw + L.R);
+    T* RF = reinterpret_cast<T*>(smem_raw + L.RF);
+    const bool sw_regs = a.max_slots <= 2 * SW * kBlock - 2;
+    const int kstride = NC * NC * KB;  // NC x NC component blocks, one point
+
+    BlockWalk W;
+    W.init(a.n_blocks);
+    int64_t b = W.take();
+    if (b < 0) return;  // uniform over the workgroup
+
+    {  // reference tensors of the (single) element kind
+        const KindTable& KT = a.kinds[0];
+        if constexpr (sizeof(T) == 8) {
+            pad_copy(R, reinterpret_cast<const T*>(KT.Rd), KT.R_rows, KB, lenp);
+            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFd), KT.RF_rows, 1, lenFp);
+        } else {
+            pad_copy(R, reinterpret_cast<const T*>(KT.Rf), KT.R_rows, KB, lenp);
+            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFf), KT.RF_rows, 1, lenFp);
+        }
+    }
+
+    struct Regs {
+        int32_t tet;
+        int ibf;
+        uint32_t posw[PW];
+        T k[NC][KB];
+        T kf;
+        uint32_t sw[SW];
+        int pst;
+    };
+    auto load_records = [&](const BlockMeta& Mx, Regs& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
+        P.tet = hp ? a.pair_tet[gp] : 0;
+        P.ibf = hp ? a.pair_ib[gp] : 0;
+        // NF even: a pair's positions start at an even entry (q0 = p0 * NF)
+        const uint32_t* pw =
+            reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * NF);
+#pragma unroll
+        for (int w = 0; w < PW; ++w) P.posw[w] = hp ? pw[w] : 0u;
+    };
+    auto load_k = [&](const BlockMeta& Mx, Regs& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const int cb = (P.ibf & 63) / NF0;
+        const T* Kt = Kg + (int64_t)P.tet * a.rowlen;
+#pragma unroll
+        for (int ca = 0; ca < NC; ++ca)
+#pragma unroll
+            for (int j = 0; j < KB; ++j) P.k[ca][j] = hp ? Kt[(cb * NC + ca) * KB + j] : T(0);
+        P.kf = (hp && a.has_rhs) ? Kt[kstride + cb] : T(0);
+        if (sw_regs) {
+            const uint32_t* sg = reinterpret_cast<const uint32_t*>(a.sstart) + (Mx.s0 >> 1);
+            const int nw = slot_words(Mx.s0, (int)Mx.ns);
+#pragma unroll
+            for (int w = 0; w < SW; ++w) {
+                const int i = w * kBlock + (int)threadIdx.x;
+                P.sw[w] = i < nw ? sg[i] : 0u;
+            }
+        }
+        P.pst = (a.has_rhs && (int)threadIdx.x < Mx.nr) ? (int)a.pstart[Mx.r0 + threadIdx.x] : 0;
+    };
+
+    BlockMeta M = a.meta[b];
+    Regs cur, nxt;
+    load_records(M, cur);
+    load_k(M, cur);
+    int64_t bn = W.take();
+    BlockMeta Mn{};
+    if (bn >= 0) Mn = a.meta[bn];
+    __syncthreads();  // reference tensors in LDS
+
+    int nonfinite = 0;
+    for (;;) {
+        // slot and row starts of b to LDS (read in phase B), records of the
+        // next block
+        const uint16_t* sstb = sst;
+        if (sw_regs) {
+            uint32_t* s32 = reinterpret_cast<uint32_t*>(sst);
+            const int nw = slot_words(M.s0, (int)M.ns);
+#pragma unroll
+            for (int w = 0; w < SW; ++w) {
+                const int i = w * kBlock + (int)threadIdx.x;
+                if (i < nw) s32[i] = cur.sw[w];
+            }
+            sstb = sst + (M.s0 & 1);
+        } else {
+            block_copy(sst, a.sstart + M.s0, (int)M.ns);
+        }
+        if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
+        if (bn >= 0) load_records(Mn, nxt);
+        const int64_t bnn = W.take();
+        BlockMeta Mnn{};
+        if (bnn >= 0) Mnn = a.meta[bnn];
+
+        // phase A of b: entries (ib, ca*NF0 + ia0) of the pair's row
+        if ((int)threadIdx.x < M.np) {
+            const int ib = cur.ibf & 63;
+            const int ib0 = ib % NF0;
+            const bool dir = (cur.ibf & kPairDirichlet) != 0;
+            const T* Rb = R + ib0 * lenp;
+#pragma unroll
+            for (int ca = 0; ca < NC; ++ca)
+#pragma unroll
+                for (int ia0 = 0; ia0 < NF0; ++ia0) {
+                    const int e = ca * NF0 + ia0;
+                    const uint32_t w = cur.posw[e >> 1];
+                    const int ps = (int)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    if (ps != kPosDropped) {
+                        const T* Rr = Rb + NF0 * ia0 * lenp;
+                        T acc = 0;
+#pragma unroll
+                        for (int j = 0; j < KB; ++j) acc += cur.k[ca][j] * Rr[j];
+                        const double v = dir ? 1.0 : (double)acc;
+                        nonfinite += isfinite(v) ? 0 : 1;
+                        vals[ps] = v;
+                    }
+                }
+            if (a.has_rhs) {
+                const T acc = cur.kf * RF[ib0 * lenFp];
+                const double fv = dir ? 0.0 : (double)acc;
+                nonfinite += isfinite(fv) ? 0 : 1;
+                fvals[threadIdx.x] = fv;
+            }
+        }
+        // geometry rows, slot and row starts of the next block
+        if (bn >= 0) load_k(Mn, nxt);
+        __syncthreads();
+
+        // phase B of b
+        const int ns = M.ns, nv = M.nv, nr = M.nr, np = M.np;
+        for (int u = threadIdx.x; u < ns; u += kBlock) {
+            const int cb = sstb[u];
+            const int ce = u + 1 < ns ? sstb[u + 1] : nv;
+            double sum = 0.0;
+            int c = cb;
+            for (; c + 4 <= ce; c += 4) {
+                const double v0 = vals[c], v1 = vals[c + 1], v2 = vals[c + 2], v3 = vals[c + 3];
+                sum += v0;
+                sum += v1;
+                sum += v2;
+                sum += v3;
+            }
+            for (; c < ce; ++c) sum += vals[c];
+            a.val[M.s0 + u] = sum;
+        }
+        if (a.has_rhs) {
+            for (int u = threadIdx.x; u < nr; u += kBlock) {
+                const int pb = pstl[u];
+                const int pe = u + 1 < nr ? pstl[u + 1] : np;
+                double sum = 0.0;
+                for (int p = pb; p < pe; ++p) sum += fvals[p];
+                a.rhs[M.r0 + u] = sum;
+            }
+        }
+        __syncthreads();
+        if (bn < 0) break;
+        cur = nxt;
+        M = Mn;
+        Mn = Mnn;
+        b = bn;
+        bn = bnn;
+    }
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
 }  // namespace anifem

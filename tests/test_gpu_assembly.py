@@ -226,6 +226,36 @@ def test_persistent_and_per_block_kernels_bitwise_equal(fem, dirichlet, n):
         assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
 
 
+@pytest.mark.parametrize("fem,dirichlet,n,rhs_storage", [(af.FEM_P2, 1, 12, af.COEF_PER_TET),
+                                                          (af.FEM_P1, 0, 16, af.COEF_CONST),
+                                                          (af.FEM_P2, 0, 3, af.COEF_CONST)])
+def test_persistent_vector_and_per_block_kernels_bitwise_equal(fem, dirichlet, n, rhs_storage):
+    """Vector spaces (FemVec<3, P1|P2>, Lame tensor): the persistent pass with
+    component blocks in registers (at n=12/16 every workgroup walks several
+    blocks) and the one-workgroup-per-block pass sum in the same order: bitwise
+    equal values and rhs, FP64 and FP32."""
+    rng = np.random.default_rng(n + fem)
+    coords, tets = af.kuhn_cube(n, 0.2, 8)
+    nt = len(tets)
+    E = 10 ** rng.uniform(0, 3, nt)
+    nu = rng.uniform(0.2, 0.45, nt)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, 3), order=2 if fem == af.FEM_P2 else 1,
+                       tensor_type=af.TENSOR_LAME, tensor_storage=af.COEF_PER_TET,
+                       rhs_tensor_type=af.TENSOR_GENERAL, rhs_tensor_storage=rhs_storage, dirichlet=dirichlet)
+    g = rng.normal(size=(3, nt)) if rhs_storage == af.COEF_PER_TET else np.array([0.0, 0.0, -1.0])
+    Dp, gp = dev(np.stack([lam, mu])), dev(g)
+    for prec in (af.PREC_FP64, af.PREC_FP32):
+        Asm.set_persistent(True)
+        v1, r1, st1 = Asm.assemble(D=Dp, F=gp, precision=prec)
+        Asm.set_persistent(False)
+        v2, r2, st2 = Asm.assemble(D=Dp, F=gp, precision=prec)
+        assert st1.code == 0 and st2.code == 0
+        assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
+        assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
+
+
 @pytest.mark.parametrize("fem,nc", [(af.FEM_P2, 1), (af.FEM_P1, 3), (af.FEM_P3, 1)])
 def test_assemble_elements_matches_oracle_scatter(fem, nc):
     """Caller-supplied element matrices (random, FP64 and FP32 values) through
