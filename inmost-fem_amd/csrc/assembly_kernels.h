@@ -757,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
             block_copy(sst, a.sstart + M.s0, (int)M.ns);
         }
         if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
-        if (bn >= 0) load_records(Mn, nxt);
+        load_records(Mn, nxt);  // past the last block: Mn has no pairs, nothing is read
         const int64_t bnn = W.take();
         BlockMeta Mnn{};
         if (bnn >= 0) Mnn = a.meta[bnn];
@@ -768,23 +768,27 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
             const int ib0 = ib % NF0;
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + ib0 * lenp;
+            // one reference row per ia0, shared by the NC component entries
 #pragma unroll
-            for (int ca = 0; ca < NC; ++ca)
+            for (int ia0 = 0; ia0 < NF0; ++ia0) {
+                T r[KB];
 #pragma unroll
-                for (int ia0 = 0; ia0 < NF0; ++ia0) {
+                for (int j = 0; j < KB; ++j) r[j] = Rb[NF0 * ia0 * lenp + j];
+#pragma unroll
+                for (int ca = 0; ca < NC; ++ca) {
                     const int e = ca * NF0 + ia0;
                     const uint32_t w = cur.posw[e >> 1];
                     const int ps = (int)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
                     if (ps != kPosDropped) {
-                        const T* Rr = Rb + NF0 * ia0 * lenp;
                         T acc = 0;
 #pragma unroll
-                        for (int j = 0; j < KB; ++j) acc += cur.k[ca][j] * Rr[j];
+                        for (int j = 0; j < KB; ++j) acc += cur.k[ca][j] * r[j];
                         const double v = dir ? 1.0 : (double)acc;
                         nonfinite += isfinite(v) ? 0 : 1;
                         vals[ps] = v;
                     }
                 }
+            }
             if (a.has_rhs) {
                 const T acc = cur.kf * RF[ib0 * lenFp];
                 const double fv = dir ? 0.0 : (double)acc;
@@ -793,7 +797,8 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
             }
         }
         // geometry rows, slot and row starts of the next block
-        if (bn >= 0) load_k(Mn, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+        load_k(Mn, nxt);
         __syncthreads();
 
         // phase B of b
