@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         }
         if (!sst_copied) block_copy(sst, a.sstart + M.s0, (int)M.ns);
         if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
-        if (bn >= 0) load_records(Mn, nxt);
+        load_records(Mn, nxt);  // past the last block: Mn has no pairs, nothing is read
         const int64_t bnn = W.take();
         BlockMeta Mnn{};
         if (bnn >= 0) Mnn = a.meta[bnn];
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         }
         // geometry rows of the next block (its records have arrived by now)
         __builtin_amdgcn_sched_barrier(0);
-        if (bn >= 0) load_k(Mn, nxt);
+        load_k(Mn, nxt);
         __syncthreads();
 
         // phase B of b
