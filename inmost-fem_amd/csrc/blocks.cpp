@@ -120,16 +120,20 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
             }
         }
     });
-    // processing order: by the mean id of the blocks' tets, so that blocks
+    // processing order: by the largest id of the blocks' tets, so that blocks
     // sharing tets (node rows and the edge / face rows around them, far apart
-    // in row order) run close in time and find the tets' geometry rows in L2
+    // in row order) run close in time and find the tets' geometry rows in L2.
+    // A node row's tets lie in a few separate id ranges (the cells around the
+    // node); the block runs with the last of them, while the earlier ones are
+    // still cached.  (A replay of the pass through a 4 MB LRU cache per XCD
+    // fetches each K row about twice with this key, four times with the mean
+    // id.)
     {
-        std::vector<double> key(S.n_blocks);
+        std::vector<int64_t> key(S.n_blocks);
         for (int64_t b = 0; b < S.n_blocks; ++b) {
-            double sum = 0;
-            for (int64_t p = S.blk_p0[b]; p < S.blk_p0[b + 1]; ++p) sum += S.pair_tet[p];
-            const int64_t np = S.blk_p0[b + 1] - S.blk_p0[b];
-            key[b] = np ? sum / (double)np : 0.0;
+            int64_t mx = -1;
+            for (int64_t p = S.blk_p0[b]; p < S.blk_p0[b + 1]; ++p) mx = std::max<int64_t>(mx, S.pair_tet[p]);
+            key[b] = mx;
         }
         S.blk_order.resize(S.n_blocks);
         for (int64_t b = 0; b < S.n_blocks; ++b) S.blk_order[b] = (int32_t)b;
