@@ -365,41 +365,44 @@ __global__ __launch_bounds__(kBlock) void element_mfma_vec_kernel(ElemArgs a) {
         }
     }
     __syncthreads();
+    // D = K^T R^T: rows are tets, columns entries ib, so that for each of
+    // its four values a lane group of 16 stores 16 consecutive entries of one
+    // tet's column (64-byte runs)
     const int w = threadIdx.x / 64, l = threadIdx.x % 64, j = l % 16, g = l / 16;
-    const int tl = 16 * w + j;  // this lane's tet (B operand column and output)
-    const bool live = tl < ntile;
+    const int tl = 16 * w + j;  // the tet whose K row this lane supplies (A operand row)
     const int nfB0 = a.nfB0, nfA0 = a.nfA0, nfB = nfB0 * NCB, nfA = nfA0 * NCA;
     const _Float16* Rh = static_cast<const _Float16*>(a.r16);
     const _Float16* Rl = Rh + (int64_t)a.e0pad * 16;
-    float* out = static_cast<float*>(a.out) + (tile0 + tl) * a.E;
     const int ntiles = (nfB + 15) / 16;
     const half4_t zero4 = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
     for (int ia = 0; ia < nfA; ++ia) {
         const int ca = ia / nfA0, ia0 = ia % nfA0;
         for (int h = 0; h < ntiles; ++h) {
-            const int ibA = 16 * h + j;  // A operand row of this lane
+            const int ibB = 16 * h + j;  // B operand column (entry) of this lane
             const int cb_lo = (16 * h) / nfB0;
             const int cb_hi = min(NCB - 1, (16 * h + 15) / nfB0);
             float4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int cb = cb_lo; cb <= cb_hi; ++cb) {
                 const int b = cb * NCA + ca;
-                half4_t ah = zero4, al = zero4;
-                if (ibA < nfB && ibA / nfB0 == cb) {
-                    const int e0 = ibA % nfB0 + nfB0 * ia0;
-                    ah = *reinterpret_cast<const half4_t*>(Rh + e0 * 16 + 4 * g);
-                    al = *reinterpret_cast<const half4_t*>(Rl + e0 * 16 + 4 * g);
+                half4_t rh = zero4, rl = zero4;
+                if (ibB < nfB && ibB / nfB0 == cb) {
+                    const int e0 = ibB % nfB0 + nfB0 * ia0;
+                    rh = *reinterpret_cast<const half4_t*>(Rh + e0 * 16 + 4 * g);
+                    rl = *reinterpret_cast<const half4_t*>(Rl + e0 * 16 + 4 * g);
                 }
-                const half4_t bh = *reinterpret_cast<const half4_t*>(&Kh[b][tl][4 * g]);
-                const half4_t bl = *reinterpret_cast<const half4_t*>(&Kl[b][tl][4 * g]);
-                c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
+                const half4_t kh = *reinterpret_cast<const half4_t*>(&Kh[b][tl][4 * g]);
+                const half4_t kl = *reinterpret_cast<const half4_t*>(&Kl[b][tl][4 * g]);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(kh, rl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(kl, rh, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16f16(kh, rh, c, 0, 0, 0);
             }
-            if (!live) continue;
+            if (ibB >= nfB) continue;
+            const int kb = (ibB / nfB0) * NCA + ca;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int ib = 16 * h + 4 * g + v;  // D row of this lane's value v
-                if (ib < nfB) out[ib + nfB * ia] = c[v] * ks[(ib / nfB0) * NCA + ca][tl] * a.r16_scale;
+                const int tv = 16 * w + 4 * g + v;  // D row of this lane's value v
+                if (tv < ntile)
+                    static_cast<float*>(a.out)[(tile0 + tv) * a.E + ibB + nfB * ia] = c[v] * ks[kb][tv] * a.r16_scale;
             }
         }
     }
