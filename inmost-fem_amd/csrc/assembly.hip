@@ -391,15 +391,18 @@ int greedy_colour(int64_t n_nodes, int64_t n_tets, const int32_t* tets, std::vec
     return ncol;
 }
 
+// The setup of a context: its enumeration phase on the device for a device
+// context, on the host for a host-only one (identical results).
 int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-               const anifem::BlockCaps* caps = nullptr) {
+               const anifem::BlockCaps* caps = nullptr, bool on_device = false, void* stream = nullptr) {
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
     const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
-    return build_setup(*mesh, *prob, ctx->S, caps ? *caps : default_caps(nf));
+    const BlockCaps c = caps ? *caps : default_caps(nf);
+    return on_device ? build_setup_device(*mesh, *prob, ctx->S, c, stream) : build_setup(*mesh, *prob, ctx->S, c);
 }
 
 }  // namespace
@@ -432,7 +435,7 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     using namespace anifem;
     if (!out) return ANIFEM_ERR_ARG;
     std::unique_ptr<anifem_assembly> ctx(new anifem_assembly());
-    int rc = build_host(ctx.get(), mesh, prob, &caps);
+    int rc = build_host(ctx.get(), mesh, prob, &caps, true, stream);
     if (rc) return rc;
     AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);

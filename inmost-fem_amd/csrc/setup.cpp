@@ -106,8 +106,7 @@ void order_permutation(const int64_t g[4], unsigned char perm[4]) {
 
 }  // namespace
 
-int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
-                const BlockCaps& caps) {
+int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
     const int fem = prob.trial.fem;
     if (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
     if (prob.test.fem != fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
@@ -156,6 +155,13 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
     });
     if (bad.load()) return ANIFEM_ERR_CONNECTIVITY;
     S.n_degenerate = nonpos.load();
+    return ANIFEM_OK;
+}
+
+int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
+    const int fem = S.fem;
+    const int nf0 = nf_scalar(fem);
+    const int64_t NT = S.n_tets, NN = S.n_nodes;
 
     // edges and faces by first appearance
     std::vector<int64_t> eid, fid;
@@ -252,16 +258,9 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
         return ANIFEM_ERR_ARG;
     }
 
-    // DOF -> tet adjacency
-    std::vector<int64_t> aptr(S.ndof + 1, 0);
-    for (int64_t i = 0; i < (int64_t)S.dofs.size(); ++i) aptr[S.dofs[i] + 1]++;
-    for (int64_t d = 0; d < S.ndof; ++d) aptr[d + 1] += aptr[d];
-    std::vector<int32_t> adj(aptr[S.ndof]);
-    {
-        std::vector<int64_t> fill(aptr.begin(), aptr.end() - 1);
-        for (int64_t t = 0; t < NT; ++t)
-            for (int i = 0; i < S.nf; ++i) adj[fill[S.dofs[(size_t)S.nf * t + i]]++] = (int32_t)t;
-    }
+    std::vector<int64_t> aptr;
+    std::vector<int32_t> adj;
+    dof_adjacency(S, aptr, adj);
 
     // CSR pattern: per row, sort + unique of the DOFs of its tets
     S.row_ptr.assign(S.ndof + 1, 0);
@@ -313,6 +312,27 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
                 }
         }
     });
+    return ANIFEM_OK;
+}
+
+void dof_adjacency(const AssemblySetup& S, std::vector<int64_t>& aptr, std::vector<int32_t>& adj) {
+    aptr.assign(S.ndof + 1, 0);
+    for (int64_t i = 0; i < (int64_t)S.dofs.size(); ++i) aptr[S.dofs[i] + 1]++;
+    for (int64_t d = 0; d < S.ndof; ++d) aptr[d + 1] += aptr[d];
+    adj.resize(aptr[S.ndof]);
+    std::vector<int64_t> fill(aptr.begin(), aptr.end() - 1);
+    for (int64_t t = 0; t < S.n_tets; ++t)
+        for (int i = 0; i < S.nf; ++i) adj[fill[S.dofs[(size_t)S.nf * t + i]]++] = (int32_t)t;
+}
+
+int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
+                const BlockCaps& caps) {
+    int rc = setup_prepare(mesh, prob, S);
+    if (!rc) rc = setup_enumerate_host(prob, S);
+    if (rc) return rc;
+    std::vector<int64_t> aptr;
+    std::vector<int32_t> adj;
+    dof_adjacency(S, aptr, adj);
     return build_blocks(S, aptr, adj, caps);
 }
 

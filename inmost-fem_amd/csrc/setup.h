@@ -79,6 +79,17 @@ struct AssemblySetup {
 
 int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
                 const BlockCaps& caps = BlockCaps());
+// The phases of build_setup: argument checks and the node reorder; the
+// enumeration, Dirichlet flags, CSR pattern and slot map (host or device, with
+// identical results); the DOF -> tet adjacency the row blocks are cut from.
+int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S);
+int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S);
+int setup_enumerate_device(const anifem_problem_desc& prob, AssemblySetup& S, void* stream);
+void dof_adjacency(const AssemblySetup& S, std::vector<int64_t>& aptr, std::vector<int32_t>& adj);
+// build_setup with the enumeration phase on the current device (stream-ordered
+// on `stream`); the host phase runs instead for meshes beyond its index ranges.
+int build_setup_device(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,
+                       const BlockCaps& caps, void* stream);
 int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj,
                  const BlockCaps& caps);
 
