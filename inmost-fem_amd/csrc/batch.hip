@@ -21,7 +21,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <atomic>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "anifem_common.h"
@@ -332,52 +334,99 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     std::vector<uint8_t> pair_ib;
     std::vector<uint16_t> pair_pos, sstart, pstart;
     std::vector<int64_t> row_ptr;
+    // per-mesh setups on host threads, each with a stream of its own for the
+    // device phase
+    std::vector<AssemblySetup> Ss(n_meshes);
+    {
+        std::vector<int> rcs(n_meshes, ANIFEM_OK);
+        std::atomic<int> next{0};
+        const int W = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (int w = 0; w < std::min(W, n_meshes); ++w)
+            th.emplace_back([&] {
+                hipStream_t ws = nullptr;
+                const bool ok = hipSetDevice(b->device) == hipSuccess && hipStreamCreate(&ws) == hipSuccess;
+                for (;;) {
+                    const int j = next.fetch_add(1);
+                    if (j >= n_meshes) break;
+                    if (!ok) {
+                        rcs[j] = ANIFEM_ERR_HIP;
+                        continue;
+                    }
+                    const anifem_batch_mesh& mj = meshes[j];
+                    anifem_mesh_desc mdj{mj.n_nodes, mj.n_tets, mj.coords, mj.tets};
+                    rcs[j] = build_setup_device(mdj, problem_of(mj.fem, dirichlet), Ss[j], b->caps[kind_of(mj.fem)], ws);
+                    // what the concatenation does not read
+                    std::vector<int32_t>().swap(Ss[j].slot);
+                    std::vector<int32_t>().swap(Ss[j].dofs);
+                }
+                if (ws) (void)hipStreamDestroy(ws);
+            });
+        for (auto& x : th) x.join();
+        for (int rc : rcs)
+            if (rc) return rc;
+    }
+    // offsets of each mesh in the concatenated arrays
     b->tet_off.assign(1, 0);
     b->node_off.assign(1, 0);
     b->dof_off.assign(1, 0);
     b->nnz_off.assign(1, 0);
     b->rp_off.assign(1, 0);
-    int64_t pair_off = 0, pos_off = 0;
+    std::vector<int64_t> pair_off(1, 0), pos_off(1, 0), meta_off(1, 0);
     for (int i = 0; i < n_meshes; ++i) {
-        const anifem_batch_mesh& m = meshes[i];
-        const int k = kind_of(m.fem);
-        anifem_mesh_desc md{m.n_nodes, m.n_tets, m.coords, m.tets};
-        const anifem_problem_desc prob = problem_of(m.fem, dirichlet);
-        AssemblySetup S;
-        int rc = build_setup(md, prob, S, b->caps[k]);
-        if (rc) return rc;
-        const int64_t t0 = b->tet_off.back(), n0 = b->node_off.back(), d0 = b->dof_off.back(),
-                      z0 = b->nnz_off.back();
-        coords.insert(coords.end(), m.coords, m.coords + 3 * m.n_nodes);
-        for (int64_t q = 0; q < 4 * S.n_tets; ++q) tets.push_back((int32_t)(S.tets[q] + n0));
-        for (const BlockMeta& M0 : S.blk_meta) {
-            BlockMeta M = M0;
-            M.s0 += z0;
-            M.q0 = pos_off + (M0.p0) * S.nf;
-            M.p0 += pair_off;
-            M.r0 += (int32_t)d0;
-            M.kind = (uint32_t)k;
-            meta.push_back(M);
-        }
-        for (int32_t t : S.pair_tet) pair_tet.push_back((int32_t)(t + t0));
-        pair_ib.insert(pair_ib.end(), S.pair_ib.begin(), S.pair_ib.end());
-        pair_pos.insert(pair_pos.end(), S.pair_pos.begin(), S.pair_pos.end());
-        sstart.insert(sstart.end(), S.sstart.begin(), S.sstart.end());
-        pstart.insert(pstart.end(), S.pstart.begin(), S.pstart.end());
-        row_ptr.insert(row_ptr.end(), S.row_ptr.begin(), S.row_ptr.end());
-        col_idx.insert(col_idx.end(), S.col_idx.begin(), S.col_idx.end());
+        const AssemblySetup& S = Ss[i];
+        b->tet_off.push_back(b->tet_off.back() + S.n_tets);
+        b->node_off.push_back(b->node_off.back() + S.n_nodes);
+        b->dof_off.push_back(b->dof_off.back() + S.ndof);
+        b->nnz_off.push_back(b->nnz_off.back() + S.nnz);
+        b->rp_off.push_back(b->rp_off.back() + S.ndof + 1);
+        pair_off.push_back(pair_off.back() + (int64_t)S.pair_tet.size());
+        pos_off.push_back(pos_off.back() + (int64_t)S.pair_pos.size());
+        meta_off.push_back(meta_off.back() + (int64_t)S.blk_meta.size());
         b->max_values = std::max<int>(b->max_values, (int)S.max_block_values);
         b->max_pairs = std::max<int>(b->max_pairs, (int)S.max_block_pairs);
         b->max_slots = std::max<int>(b->max_slots, (int)S.max_block_slots);
-        pair_off += (int64_t)S.pair_tet.size();
-        pos_off += (int64_t)S.pair_pos.size();
-        b->tet_off.push_back(t0 + S.n_tets);
-        b->node_off.push_back(n0 + S.n_nodes);
-        b->dof_off.push_back(d0 + S.ndof);
-        b->nnz_off.push_back(z0 + S.nnz);
-        b->rp_off.push_back(b->rp_off.back() + S.ndof + 1);
         if (b->tet_off.back() >= INT32_MAX || b->dof_off.back() >= INT32_MAX) return ANIFEM_ERR_ARG;
     }
+    coords.resize(3 * (size_t)b->node_off.back());
+    tets.resize(4 * (size_t)b->tet_off.back());
+    meta.resize(meta_off.back());
+    pair_tet.resize(pair_off.back());
+    pair_ib.resize(pair_off.back());
+    pair_pos.resize(pos_off.back());
+    sstart.resize(b->nnz_off.back());
+    pstart.resize(b->dof_off.back());
+    row_ptr.resize(b->rp_off.back());
+    col_idx.resize(b->nnz_off.back());
+    // each mesh fills its ranges (ids shifted by the mesh's offsets), then
+    // releases its setup
+    host_parallel_for(n_meshes, 1, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            AssemblySetup& S = Ss[i];
+            const anifem_batch_mesh& m = meshes[i];
+            const int k = kind_of(m.fem);
+            const int64_t t0 = b->tet_off[i], n0 = b->node_off[i], d0 = b->dof_off[i], z0 = b->nnz_off[i];
+            std::copy(m.coords, m.coords + 3 * m.n_nodes, coords.begin() + 3 * n0);
+            for (int64_t q = 0; q < 4 * S.n_tets; ++q) tets[4 * t0 + q] = (int32_t)(S.tets[q] + n0);
+            for (size_t u = 0; u < S.blk_meta.size(); ++u) {
+                BlockMeta M = S.blk_meta[u];
+                M.q0 = pos_off[i] + M.p0 * S.nf;
+                M.s0 += z0;
+                M.p0 += pair_off[i];
+                M.r0 += (int32_t)d0;
+                M.kind = (uint32_t)k;
+                meta[meta_off[i] + u] = M;
+            }
+            for (size_t u = 0; u < S.pair_tet.size(); ++u) pair_tet[pair_off[i] + u] = (int32_t)(S.pair_tet[u] + t0);
+            std::copy(S.pair_ib.begin(), S.pair_ib.end(), pair_ib.begin() + pair_off[i]);
+            std::copy(S.pair_pos.begin(), S.pair_pos.end(), pair_pos.begin() + pos_off[i]);
+            std::copy(S.sstart.begin(), S.sstart.end(), sstart.begin() + z0);
+            std::copy(S.pstart.begin(), S.pstart.end(), pstart.begin() + d0);
+            std::copy(S.row_ptr.begin(), S.row_ptr.end(), row_ptr.begin() + b->rp_off[i]);
+            std::copy(S.col_idx.begin(), S.col_idx.end(), col_idx.begin() + z0);
+            S = AssemblySetup();
+        }
+    });
     b->n_tets = b->tet_off.back();
     b->n_nodes = b->node_off.back();
     b->ndof = b->dof_off.back();
