@@ -379,6 +379,7 @@ def main():
                    "dirichlet": "homogeneous, boundary"},
         "nnz_per_s": Asm.nnz / sec64,
         "phase_ms": {"geometry": tm[0], "row_blocks": tm[1], "launches": nlaunch},
+        "setup_s": setup_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "assembly pass: geometry_kernel<3,3,1,1,double> + row_block_persistent<3,3,5,double>",
