@@ -97,6 +97,26 @@ enum {
 int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
                           const double* xyz, int precision, void* A_out, void* stream);
 
+/* Batched face integrals over face `face` (0..3: the face through vertices
+ * face, face+1, face+2 mod 4) of each tet:
+ *   dot_normal = 0: fem3Dface<OpA, OpB>(XYZ, face, Dfnc, A, order)
+ *     A_r(ib, ia) = int_{f_r} (D OpA(phi_ia)) . OpB(phi_ib) ds
+ *     (anifem++/fem/operations/int_face.h:15-20, int_face.inl:12-44);
+ *   dot_normal = 1: fem3DfaceN<OpA, OpB>(XYZ, face, Dfnc, A, order)
+ *     A_r(ib, ia) = int_{f_r} ((D OpA(phi_ia)) . n) . OpB(phi_ib) ds with n the
+ *     unit normal pointing out of the tet; D has 3*dimB rows, row d + 3k
+ *     pairing normal component d with output component k
+ *     (int_face.h:32-37, int_face.inl:46-82, core.h:176-195).
+ * The triangle rule of `order` (0..6) on the face; per-point coefficients are
+ * given at the points anifem_face_quadrature returns.  Layouts, errors and
+ * precisions as anifem_fem3dtet_batch, FP64 and FP32 only. */
+int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int face, int dot_normal,
+                           int64_t n_tets, const double* xyz, int precision, void* A_out, void* stream);
+/* Triangle rule of `order` placed on face `face`, as tet barycentric points
+ * p[4q] (coordinate of the opposite vertex 0) and weights w[q] summing to 1;
+ * returns q (or <0). */
+int anifem_face_quadrature(int order, int face, double* p, double* w);
+
 /* Number of basis functions (Nfa) and operator dimension (Dim) of an operator
  * (the Nfa/Dim typedefs of poly_*.h and operators.h:127-155).  <0 if invalid. */
 int anifem_op_nfa(anifem_op op);

@@ -106,6 +106,9 @@ SIGNATURES = {
     "anifem_op_dim": (_I32, [anifem_op]),
     "anifem_quadrature_npoints": (_I32, [_I32]),
     "anifem_quadrature": (_I32, [_I32, _VP, _VP]),
+    "anifem_fem3dface_batch": (_I32, [anifem_op, anifem_op, ctypes.POINTER(anifem_tensor), _I32, _I32, _I32, _I64,
+                                      _VP, _I32, _VP, _VP]),
+    "anifem_face_quadrature": (_I32, [_I32, _I32, _VP, _VP]),
     "anifem_assembly_create_budget": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
                                              ctypes.POINTER(anifem_problem_desc), _I32, _I32, _I32, _VP]),
     "anifem_assembly_create": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
@@ -260,6 +263,35 @@ def fem3dtet_batch(opA: Op, opB: Op, ttype, storage, D, order, xyz, precision=PR
     rc = lib().anifem_fem3dtet_batch(opA.c(), opB.c(), ctypes.byref(t), order, n, _ptr(xyz), precision, _ptr(out),
                                      _stream_handle(stream))
     _check(rc, "anifem_fem3dtet_batch")
+    return out
+
+
+def face_quadrature(order, face):
+    """Triangle rule of `order` on face `face` of the tet: (p (q, 4) barycentric,
+    w (q,) summing to 1) -- the points of per-point face coefficients."""
+    p = np.zeros(4 * 12)
+    w = np.zeros(12)
+    q = lib().anifem_face_quadrature(order, face, _ptr(p), _ptr(w))
+    if q < 0:
+        raise AnifemError(q, "face_quadrature")
+    return p[:4 * q].reshape(q, 4), w[:q]
+
+
+def fem3dface_batch(opA: Op, opB: Op, ttype, storage, D, order, face, xyz, dot_normal=False, precision=PREC_FP64,
+                    out=None, stream=None):
+    """Face integrals over face `face` of n tets (anifem_fem3dface_batch):
+    fem3Dface, or fem3DfaceN with dot_normal (D then has 3*dimB rows).
+    Layouts as fem3dtet_batch."""
+    import torch
+    n = xyz.shape[1]
+    nfa, nfb = opA.nfa, opB.nfa
+    dt = torch.float64 if precision == PREC_FP64 else torch.float32
+    if out is None:
+        out = torch.empty((n, nfa, nfb), dtype=dt, device=xyz.device)
+    t = anifem_tensor(ttype, storage, _ptr(D))
+    rc = lib().anifem_fem3dface_batch(opA.c(), opB.c(), ctypes.byref(t), order, face, 1 if dot_normal else 0, n,
+                                      _ptr(xyz), precision, _ptr(out), _stream_handle(stream))
+    _check(rc, "anifem_fem3dface_batch")
     return out
 
 

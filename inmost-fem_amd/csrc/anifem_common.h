@@ -28,7 +28,11 @@ void reference_table(int op, int fem, int q, const double* p, std::vector<double
 //   per_point == false:  R[rr*KB + b*dA + a] = sum_n w_n V_b(n, ib) U_a(n, ia)
 //   per_point == true :  R[(rr*q + n)*KB + b*dA + a] = V_b(n, ib) U_a(n, ia)
 // with rr = ib + nfB0*ia and KB = dA*dB.
-void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R);
+// face < 0: volume rule; face 0..3: the triangle rule on that face (face integrals)
+void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R,
+                      int face = -1);
+int triangle_quadrature(int order, const double** p, const double** w);
+int rule_points(int order, int face, std::vector<double>& pts, std::vector<double>& wts);
 
 // Device-resident copies of reference tensors, built lazily per (pair, order,
 // per_point) and kept for the life of the process.
@@ -45,7 +49,8 @@ struct RefTensorDev {
     float r16_scale = 1.0f;
     int e0pad = 0;
 };
-int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out);
+int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out,
+                   int face = -1);
 
 // Symmetric-tensor packing: D(i,j), i<=j, at i + j*(j+1)/2.
 inline int sym_index(int i, int j) { return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }

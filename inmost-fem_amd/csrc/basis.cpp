@@ -20,6 +20,7 @@
 
 #include "anifem_common.h"
 #include "tet_rules.inc"
+#include "tri_rules.inc"
 
 namespace anifem {
 
@@ -51,6 +52,46 @@ int quadrature(int order, const double** p, const double** w) {
         case 24: *p = ANIFEM_TET_Q24_P, *w = ANIFEM_TET_Q24_W; break;
         default: return -1;
     }
+    return q;
+}
+
+int triangle_quadrature(int order, const double** p, const double** w) {
+    if (order < 0 || order > ANIFEM_TRI_MAX_ORDER) return -1;
+    int q = ANIFEM_TRI_NPTS_BY_ORDER[order];
+    switch (q) {
+        case 1: *p = ANIFEM_TRI_Q1_P, *w = ANIFEM_TRI_Q1_W; break;
+        case 3: *p = ANIFEM_TRI_Q3_P, *w = ANIFEM_TRI_Q3_W; break;
+        case 6: *p = ANIFEM_TRI_Q6_P, *w = ANIFEM_TRI_Q6_W; break;
+        case 7: *p = ANIFEM_TRI_Q7_P, *w = ANIFEM_TRI_Q7_W; break;
+        case 12: *p = ANIFEM_TRI_Q12_P, *w = ANIFEM_TRI_Q12_W; break;
+        default: return -1;
+    }
+    return q;
+}
+
+// Points of the rule for `order` as tet barycentric coordinates: the volume
+// rule (face < 0), or the triangle rule on face `face` (vertices face,
+// face+1, face+2 mod 4; the fourth coordinate 0), placed as in
+// int_face.inl:31-36.
+int rule_points(int order, int face, std::vector<double>& pts, std::vector<double>& wts) {
+    const double *p, *w;
+    if (face < 0) {
+        const int q = quadrature(order, &p, &w);
+        if (q < 0) return q;
+        pts.assign(p, p + 4 * q);
+        wts.assign(w, w + q);
+        return q;
+    }
+    if (face > 3) return -1;
+    const int q = triangle_quadrature(order, &p, &w);
+    if (q < 0) return q;
+    pts.assign((size_t)4 * q, 0.0);
+    for (int n = 0; n < q; ++n) {
+        pts[4 * n + face] = p[3 * n + 0];
+        pts[4 * n + (face + 1) % 4] = p[3 * n + 1];
+        pts[4 * n + (face + 2) % 4] = p[3 * n + 2];
+    }
+    wts.assign(w, w + q);
     return q;
 }
 
@@ -157,9 +198,12 @@ void reference_table(int op, int fem, int q, const double* p, std::vector<double
     }
 }
 
-void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R) {
-    const double *p, *w;
-    int q = quadrature(order, &p, &w);
+void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, std::vector<double>& R,
+                      int face) {
+    std::vector<double> pts, wts;
+    int q = rule_points(order, face, pts, wts);
+    const double* p = pts.data();
+    const double* w = wts.data();
     std::vector<double> TA, TB;
     reference_table(opA, femA, q, p, TA);
     reference_table(opB, femB, q, p, TB);
@@ -188,12 +232,12 @@ __global__ void split_f16_kernel(const float* x, int64_t n, _Float16* out) {
     out[n + i] = (_Float16)(x[i] - (float)h);
 }
 
-int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out) {
+int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out, int face) {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, int, int, int>, RefTensorDev> cache;
+    static std::map<std::tuple<int, int, int, int, int, int, int, int>, RefTensorDev> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return ANIFEM_ERR_HIP;
-    auto key = std::make_tuple(dev, opA, femA, opB, femB, order, (int)per_point);
+    auto key = std::make_tuple(dev, opA, femA, opB, femB, order, (int)per_point, face);
     std::lock_guard<std::mutex> lock(mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
@@ -201,12 +245,12 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
         return ANIFEM_OK;
     }
     std::vector<double> R;
-    reference_tensor(opA, femA, opB, femB, order, per_point, R);
+    reference_tensor(opA, femA, opB, femB, order, per_point, R, face);
     std::vector<float> Rf(R.begin(), R.end());
     RefTensorDev t;
-    const double* pp;
-    const double* ww;
-    const int nq = quadrature(order, &pp, &ww);
+    std::vector<double> pts, wts;
+    const int nq = rule_points(order, face, pts, wts);
+    const double* ww = wts.data();
     t.q = per_point ? nq : 1;
     t.E0 = nf_scalar(femA) * nf_scalar(femB);
     t.KB = ref_dim(opA) * ref_dim(opB);

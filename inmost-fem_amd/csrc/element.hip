@@ -9,15 +9,22 @@ namespace anifem {
 
 // Operator-dependent part of ElemArgs; validates the pair like the reference's
 // compile-time / runtime checks (int_tet.inl:94-104, diff_tensor.h:313-315).
-int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a) {
+// Element arguments of a volume integral (face < 0) or of the integral over
+// one face, plain (fem3Dface) or of (D OpA(u)) . n (fem3DfaceN, dot_normal):
+// there D has 3 dimB rows, and a.dimB is that row count (the tensor's layout).
+static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, int face,
+                        int dot_normal, ElemArgs& a) {
     if (!op_valid(opA) || !op_valid(opB) || !D) return ANIFEM_ERR_ARG;
-    const double *p, *w;
-    int q = quadrature(order, &p, &w);
+    if (face > 3 || (face < 0 && dot_normal)) return ANIFEM_ERR_ARG;
+    std::vector<double> pts, wts;
+    int q = rule_points(order, face, pts, wts);
     if (q < 0) return ANIFEM_ERR_ORDER;
+    a.face = face < 0 ? -1 : face;
+    a.dot_normal = dot_normal ? 1 : 0;
     a.nfA0 = nf_scalar(opA.fem);
     a.nfB0 = nf_scalar(opB.fem);
     a.dimA = ref_dim(opA.op) * opA.ncomp;
-    a.dimB = ref_dim(opB.op) * opB.ncomp;
+    a.dimB = ref_dim(opB.op) * opB.ncomp * (dot_normal ? 3 : 1);
     a.E = a.nfA0 * opA.ncomp * a.nfB0 * opB.ncomp;
     a.ttype = D->type;
     a.storage = D->storage;
@@ -30,7 +37,7 @@ int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_t
     if (D->type != ANIFEM_TENSOR_NULL && !D->data) return ANIFEM_ERR_ARG;
     if (D->storage < ANIFEM_COEF_CONST || D->storage > ANIFEM_COEF_PER_POINT) return ANIFEM_ERR_ARG;
     RefTensorDev R;
-    int rc = get_ref_tensor(opA.op, opA.fem, opB.op, opB.fem, order, D->storage == ANIFEM_COEF_PER_POINT, &R);
+    int rc = get_ref_tensor(opA.op, opA.fem, opB.op, opB.fem, order, D->storage == ANIFEM_COEF_PER_POINT, &R, a.face);
     if (rc) return rc;
     a.q = R.q;
     a.Rd = R.r64;
@@ -40,6 +47,10 @@ int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_t
     a.r16_scale = R.r16_scale;
     a.e0pad = R.e0pad;
     return ANIFEM_OK;
+}
+
+int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a) {
+    return prepare_args(opA, opB, D, order, -1, 0, a);
 }
 
 // Tets per workgroup: as many as keep the LDS geometry tensors <= 32 KiB
@@ -135,4 +146,32 @@ extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_
     if (precision == ANIFEM_PREC_FP32) return launch_element<float>(opA, opB, a, 256, s);
     if (precision == ANIFEM_PREC_FP16X3) return launch_element_mfma(opA, opB, a, s);
     return ANIFEM_ERR_ARG;
+}
+
+extern "C" int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int face,
+                                      int dot_normal, int64_t n_tets, const double* xyz, int precision, void* A_out,
+                                      void* stream) {
+    using namespace anifem;
+    if (n_tets < 0 || (n_tets > 0 && (!xyz || !A_out)) || face < 0 || face > 3) return ANIFEM_ERR_ARG;
+    ElemArgs a{};
+    int rc = prepare_args(opA, opB, D, order, face, dot_normal, a);
+    if (rc) return rc;
+    if (n_tets == 0) return ANIFEM_OK;
+    a.n = n_tets;
+    a.coef_n = n_tets;
+    a.xyz = xyz;
+    a.out = A_out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (precision == ANIFEM_PREC_FP64) return launch_element<double>(opA, opB, a, 256, s);
+    if (precision == ANIFEM_PREC_FP32) return launch_element<float>(opA, opB, a, 256, s);
+    return ANIFEM_ERR_ARG;  // the MFMA mode computes volume integrals only
+}
+
+extern "C" int anifem_face_quadrature(int order, int face, double* p, double* w) {
+    std::vector<double> pts, wts;
+    const int q = anifem::rule_points(order, face, pts, wts);
+    if (q < 0 || face < 0 || face > 3) return q < 0 ? ANIFEM_ERR_ORDER : ANIFEM_ERR_ARG;
+    if (p) std::copy(pts.begin(), pts.end(), p);
+    if (w) std::copy(wts.begin(), wts.end(), w);
+    return q;
 }

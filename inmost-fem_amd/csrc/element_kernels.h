@@ -35,6 +35,8 @@ struct ElemArgs {
     int ttype, storage;
     int dimA, dimB;    // full operator dims (ncomp * ref dim)
     int p0_bcast;      // IDEN P0 trial against a vector test: D column replicated
+    int face;          // -1: volume integral; 0..3: integral over that face (int_face.inl)
+    int dot_normal;    // face integral of (D OpA(u)) . n against OpB (fem3DfaceN): D has 3 dimB rows
     int tile;          // tets per workgroup
     // geometry source: either SoA coordinates (batch mode) or mesh gather
     const double* xyz;        // 12 x n
@@ -97,15 +99,45 @@ __device__ __forceinline__ double tensor_at(const ElemArgs& a, int64_t cr, int n
     return 0.0;
 }
 
+// Area of face f of a tet (vertices f, f+1, f+2 mod 4) and, when nrm is given,
+// its unit normal pointing away from the fourth vertex (geometry.h:67-93).
+__device__ __forceinline__ double face_measure(const double P[4][3], int f, double* nrm) {
+    const double* p0 = P[f];
+    const double* p1 = P[(f + 1) % 4];
+    const double* p2 = P[(f + 2) % 4];
+    const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double v[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const double len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (nrm) {
+        const double* p3 = P[(f + 3) % 4];
+        const double h = c[0] * (p3[0] - p0[0]) + c[1] * (p3[1] - p0[1]) + c[2] * (p3[2] - p0[2]);
+        const double sgn = h > 0 ? -1.0 : 1.0;
+        for (int i = 0; i < 3; ++i) nrm[i] = sgn * c[i] / len;
+    }
+    return len / 2;
+}
+
 // K block (cb, ca) at point n of one tet, scaled by s:
 // Kb[b*DA0 + a] = s * sum_{k0, j0} TB(k0, b) D(k0 + DB0 cb, j0 + DA0 ca) TA(j0, a),
-// T = PSI^T for GRAD, 1 for IDEN.
+// T = PSI^T for GRAD, 1 for IDEN.  With a face normal nrm, D(k, j) is the
+// normal contraction sum_d D(d + 3k, j) nrm[d] of the 3 dimB x dimA tensor
+// (core.h:176-195).
 template <int DA0, int DB0, typename KT>
 __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, const double PSI[9], double s, int n,
-                                               int cb, int ca, KT* Kb) {
+                                               int cb, int ca, KT* Kb, const double* nrm = nullptr) {
     double Dblk[DB0][DA0];
     for (int k0 = 0; k0 < DB0; ++k0)
-        for (int j0 = 0; j0 < DA0; ++j0) Dblk[k0][j0] = tensor_at(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
+        for (int j0 = 0; j0 < DA0; ++j0) {
+            if (nrm) {
+                const int k = k0 + DB0 * cb;
+                double v = 0;
+                for (int d = 0; d < 3; ++d) v += tensor_at(a, cr, n, d + 3 * k, j0 + DA0 * ca) * nrm[d];
+                Dblk[k0][j0] = v;
+            } else {
+                Dblk[k0][j0] = tensor_at(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
+            }
+        }
     double M[DB0][DA0];  // (D * TA)
     for (int k0 = 0; k0 < DB0; ++k0)
         for (int a0 = 0; a0 < DA0; ++a0) {
@@ -132,13 +164,13 @@ __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, co
 // K[(cb*NCA + ca)*q*KB + n*KB + b*DA0 + a] for one tet.
 template <int DA0, int DB0, int NCA, int NCB, typename KT>
 __device__ __forceinline__ void geometry_tensor(const ElemArgs& a, int64_t cr, const double PSI[9], double mes,
-                                                KT* K) {
+                                                KT* K, const double* nrm = nullptr) {
     constexpr int KB = DA0 * DB0;
     for (int n = 0; n < a.q; ++n) {
         const double s = (a.q > 1) ? mes * a.wq[n] : mes;
         for (int cb = 0; cb < NCB; ++cb)
             for (int ca = 0; ca < NCA; ++ca)
-                geometry_block<DA0, DB0, KT>(a, cr, PSI, s, n, cb, ca, K + ((cb * NCA + ca) * a.q + n) * KB);
+                geometry_block<DA0, DB0, KT>(a, cr, PSI, s, n, cb, ca, K + ((cb * NCA + ca) * a.q + n) * KB, nrm);
     }
 }
 
@@ -168,11 +200,13 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
 
     for (int t = threadIdx.x; t < ntile; t += kBlock) {
         const int64_t r = tile0 + t;
-        double P[4][3], PSI[9], det;
+        double P[4][3], PSI[9], det, nrm[3];
         load_tet<false>(a, r, P);
         tet_geometry(P, PSI, det);
-        const double mes = fabs(det) / 6.0;
-        geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride);
+        // volume |det|/6, or the face's area (int_face.inl:40-43, 79-84)
+        const double mes = a.face < 0 ? fabs(det) / 6.0 : face_measure(P, a.face, a.dot_normal ? nrm : nullptr);
+        geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride,
+                                               a.face >= 0 && a.dot_normal ? nrm : nullptr);
     }
     __syncthreads();
 

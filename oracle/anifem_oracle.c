@@ -456,6 +456,149 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
     return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Face integrals: fem3Dface / fem3DfaceN (int_face.inl:12-82)                */
+
+/* Gauss-Legendre nodes and weights on [0, 1] (Newton on P_m, long double). */
+static void orc_gauss01(int m, ld* x, ld* w) {
+    const ld pi = 3.141592653589793238462643383279502884L;
+    for (int i = 0; i < m; ++i) {
+        ld z = cosl(pi * (i + 0.75L) / (m + 0.5L)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            ld p0 = 1, p1 = z;
+            for (int k = 2; k <= m; ++k) {
+                ld p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = m * (z * p1 - p0) / (z * z - 1);
+            ld dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        x[i] = (1 - z) / 2;
+        w[i] = 1 / ((1 - z * z) * dp * dp);
+    }
+}
+
+/* The oracle's own triangle rule: a collapsed (Duffy) product of 6-point
+ * Gauss rules, exact for degree <= 10 on the triangle, placed on face `face`
+ * (barycentric coordinates face, face+1, face+2 mod 4 as int_face.inl:31-36;
+ * the opposite vertex 0).  p[4*36], w[36] summing to 1; returns 36. */
+int orc_face_quadrature(int face, double* p, double* w) {
+    enum { M = 6 };
+    ld x[M], wx[M];
+    orc_gauss01(M, x, wx);
+    int q = 0;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j, ++q) {
+            ld l1 = x[i] * (1 - x[j]), l2 = x[j], l0 = 1 - l1 - l2;
+            if (p) {
+                for (int k = 0; k < 4; ++k) p[4 * q + k] = 0;
+                p[4 * q + face] = (double)l0;
+                p[4 * q + (face + 1) % 4] = (double)l1;
+                p[4 * q + (face + 2) % 4] = (double)l2;
+            }
+            if (w) w[q] = (double)(2 * wx[i] * wx[j] * (1 - x[j]));
+        }
+    return q;
+}
+
+/* Area of the triangle p0 p1 p2 (geometry.h:67-72) and, for face_normal
+ * (geometry.h:76-93), the unit normal pointing away from p3. */
+static double orc_face_geom(const double* p0, const double* p1, const double* p2, const double* p3, double* nrm) {
+    double a[3], b[3], c[3];
+    for (int i = 0; i < 3; ++i) a[i] = p1[i] - p0[i], b[i] = p2[i] - p0[i];
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+    double len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (nrm) {
+        double h = 0;
+        for (int i = 0; i < 3; ++i) h += c[i] * (p3[i] - p0[i]);
+        for (int i = 0; i < 3; ++i) nrm[i] = (h > 0 ? -c[i] : c[i]) / len;
+    }
+    return len / 2;
+}
+
+/* fem3Dface (dot_normal = 0) / fem3DfaceN (dot_normal = 1) over face `face`
+ * of n tets, with the oracle's own rule (orc_face_quadrature); per-point D
+ * (coef_mode 2) is given at those points.  In the normal form D has
+ * 3*dimB rows and DU is contracted with the normal as core.h:176-195. */
+int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int tensor_type, int coef_mode, int face,
+                  int dot_normal, long n, const double* xyz, const double* D, double* A) {
+    int nfA = orc_nfa(femA, ncA), nfB = orc_nfa(femB, ncB);
+    if (nfA <= 0 || nfB <= 0 || face < 0 || face > 3) return -1;
+    if ((opA == ORC_GRAD && femA == ORC_P0) || (opB == ORC_GRAD && femB == ORC_P0)) return -1;
+    double XYL[4 * 36], WG[36];
+    int q = orc_face_quadrature(face, XYL, WG);
+    int dimA = orc_dim(opA, ncA), dimB = orc_dim(opB, ncB);
+    int nD = dot_normal ? 3 : 1, dimBt = dimB * nD;
+    int ddim = tensor_type == ORC_TENSOR_NULL ? 0 : (tensor_type == ORC_TENSOR_SCALAR ? 1 : dimA * dimBt);
+    int bcast = nfA == 1 && dimA == 1;
+    if ((tensor_type == ORC_TENSOR_NULL || tensor_type == ORC_TENSOR_SCALAR) && dimA != dimBt && !bcast) return -3;
+    double* U = malloc(sizeof(double) * (size_t)dimA * q * nfA);
+    double* V = malloc(sizeof(double) * (size_t)dimB * q * nfB);
+    double* DU = malloc(sizeof(double) * (size_t)dimBt * q * nfA);
+    for (long r = 0; r < n; ++r) {
+        const double* P = xyz + 12 * r;
+        double XYP[12], PSI[9], nrm[3];
+        for (int i = 0; i < 3; ++i) XYP[i] = 0;
+        for (int l = 0; l < 3; ++l)
+            for (int i = 0; i < 3; ++i) XYP[3 * (l + 1) + i] = P[3 * (l + 1) + i] - P[i];
+        orc_inverse3x3(XYP + 3, PSI);
+        const double* x0 = XYP + 3 * face;
+        const double* x1 = XYP + 3 * ((face + 1) % 4);
+        const double* x2 = XYP + 3 * ((face + 2) % 4);
+        const double* x3 = XYP + 3 * ((face + 3) % 4);
+        double mes = orc_face_geom(x0, x1, x2, x3, dot_normal ? nrm : NULL);
+        orc_apply_op(opA, femA, ncA, q, XYL, PSI, U);
+        orc_apply_op(opB, femB, ncB, q, XYL, PSI, V);
+        for (int n_ = 0; n_ < q; ++n_) {
+            const double* Dn = coef_mode == ORC_COEF_CONST      ? D
+                               : coef_mode == ORC_COEF_PER_TET ? D + (size_t)ddim * r
+                                                               : D + (size_t)ddim * ((size_t)q * r + n_);
+            double wv = WG[n_] * mes;
+            for (int i = 0; i < nfA; ++i) {
+                const double* u = U + dimA * (n_ + q * i);
+                double* du = DU + dimBt * (n_ + q * i);
+                for (int k = 0; k < dimBt; ++k) {
+                    double s;
+                    if (tensor_type == ORC_TENSOR_NULL) s = dimA == dimBt ? u[k] : u[0];
+                    else if (tensor_type == ORC_TENSOR_SCALAR) s = Dn[0] * (dimA == dimBt ? u[k] : u[0]);
+                    else {
+                        s = 0;
+                        for (int j = 0; j < dimA; ++j) s += Dn[k + dimBt * j] * u[j];
+                    }
+                    du[k] = wv * s;
+                }
+            }
+        }
+        /* normal contraction: DU_k = sum_d DU_{d + 3k} n_d, packed in place */
+        if (dot_normal)
+            for (int i = 0; i < nfA; ++i)
+                for (int n_ = 0; n_ < q; ++n_) {
+                    const double* src = DU + dimBt * (n_ + q * i);
+                    double tmp[9];
+                    for (int k = 0; k < dimB; ++k)
+                        tmp[k] = src[3 * k] * nrm[0] + src[3 * k + 1] * nrm[1] + src[3 * k + 2] * nrm[2];
+                    for (int k = 0; k < dimB; ++k) DU[k + dimB * (n_ + q * i)] = tmp[k];
+                }
+        double* Ar = A + (size_t)nfA * nfB * r;
+        int nrow = q * dimB;
+        for (int ia = 0; ia < nfA; ++ia)
+            for (int ib = 0; ib < nfB; ++ib) {
+                double s = 0;
+                for (int j = 0; j < nrow; ++j) s += DU[j + nrow * ia] * V[j + nrow * ib];
+                Ar[ib + nfB * ia] = s;
+            }
+    }
+    free(U);
+    free(V);
+    free(DU);
+    return 0;
+}
+
 void orc_apply_dir(int nf, double* A, double* F, int k, double bc) {
     for (int i = 0; i < nf; ++i) F[i] -= A[i + nf * k] * bc;
     F[k] = bc;

@@ -32,6 +32,7 @@ enum { ORC_COEF_CONST = 0, ORC_COEF_PER_TET = 1, ORC_COEF_PER_POINT = 2 };
  * Pinned by: polynomial exactness (tests/fem/quadrature_formulas_test.cpp:7-45
  * restated) and the reference rules of oracle/_ref where that library exists. */
 int orc_quadrature(int order, double* p, double* w);
+int orc_face_quadrature(int face, double* p, double* w);
 
 /* inverse3x3 (anifem++/fem/geometry.h:25-45): column-major inverse, returns det. */
 double orc_inverse3x3(const double* m, double* inv);
@@ -49,6 +50,21 @@ double orc_inverse3x3(const double* m, double* inv);
  * known-answer tables (tests/fem/operations/int_tet_test.cpp:191-500). */
 int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int tensor_type, int coef_mode,
                  int order, long n, const double* xyz, const double* D, double* A);
+
+/* Face integrals over face `face` (vertices face, face+1, face+2 mod 4) of n
+ * tetrahedra: fem3Dface (dot_normal 0) and fem3DfaceN (dot_normal 1: D has
+ * 3*dimB rows, DU contracted with the outward unit normal) of
+ * anifem++/fem/operations/int_face.inl:12-82, core.h:176-195.  The oracle's own
+ * triangle rule (orc_face_quadrature: 36-point collapsed Gauss, degree 10)
+ * replaces the reference's order-selected rule; every integrand on the path is
+ * a polynomial of lower degree, so the integrals agree to rounding.  PER_POINT
+ * D is given at the orc_face_quadrature points.  Layouts and codes as
+ * orc_fem3dtet.
+ * Pinned by: reference fixtures (tests/golden/face_*.npz from the compiled
+ * reference's fem3Dface / fem3DfaceN) and closed forms (face mass
+ * area/12 (1 + delta_ij) for P1, int_f phi_i = area/3). */
+int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int tensor_type, int coef_mode, int face,
+                  int dot_normal, long n, const double* xyz, const double* D, double* A);
 
 /* Number of basis functions / operator dimension for (op, fem, ncomp). */
 int orc_nfa(int fem, int ncomp);

@@ -91,9 +91,93 @@ int dispatch_tensor(int tt, int per_point, int order, long n, const double* xyz,
     return -3;
 }
 
+// Face integrals over face `face` of each tet: fem3Dface, or fem3DfaceN when
+// DN (anifem++/fem/operations/int_face.inl:12-82).
+template <typename OpA, typename OpB, int TT, int SPARSITY, bool DN, bool XDEP = false>
+int run_face_case(int order, int face, long n, const double* xyz, const double* D, long ddim, double* A) {
+    constexpr long nfa = OpA::Nfa::value, nfb = OpB::Nfa::value;
+    double buf[nfa * nfb > 1 ? nfa * nfb : 1];
+    for (long r = 0; r < n; ++r) {
+        const double* p = xyz + 12 * r;
+        Tetra<const double> XYZ(p, p + 3, p + 6, p + 9);
+        DenseMatrix<double> Am(buf, nfb, nfa, nfa * nfb);
+        TetData td{D ? D + ddim * r : nullptr, ddim};
+        if (DN)
+            fem3DfaceN<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, face, StoredTensor<TT, XDEP>(), Am, order, &td);
+        else
+            fem3Dface<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, face, StoredTensor<TT, XDEP>(), Am, order, &td);
+        std::memcpy(A + nfa * nfb * r, buf, sizeof(double) * nfa * nfb);
+    }
+    return 0;
+}
+
+template <typename OpA, typename OpB, bool DN>
+int dispatch_face(int tt, int per_point, int order, int face, long n, const double* xyz, const double* D, long ddim,
+                  double* A) {
+    if (per_point == 2) {
+        switch (tt) {
+            case TENSOR_SCALAR:
+                return run_face_case<OpA, OpB, TENSOR_SCALAR, PerPoint, DN, true>(order, face, n, xyz, D, ddim, A);
+            case TENSOR_GENERAL:
+                return run_face_case<OpA, OpB, TENSOR_GENERAL, PerPoint, DN, true>(order, face, n, xyz, D, ddim, A);
+        }
+        return -3;
+    }
+    switch (tt) {
+        case TENSOR_NULL: return run_face_case<OpA, OpB, TENSOR_NULL, TENSOR_NULL, DN>(order, face, n, xyz, D, ddim, A);
+        case TENSOR_SCALAR:
+            return run_face_case<OpA, OpB, TENSOR_SCALAR, TENSOR_SCALAR, DN>(order, face, n, xyz, D, ddim, A);
+        case TENSOR_GENERAL:
+            return run_face_case<OpA, OpB, TENSOR_GENERAL, TENSOR_GENERAL, DN>(order, face, n, xyz, D, ddim, A);
+    }
+    return -3;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Number of points of the reference triangle rule; fills p[3*q], w[q].
+int ref_triangle_quadrature(int order, double* p, double* w) {
+    try {
+        auto f = triangle_quadrature_formulas(order);
+        int q = f.GetNumPoints();
+        if (p) std::memcpy(p, f.p, sizeof(double) * 3 * q);
+        if (w) std::memcpy(w, f.w, sizeof(double) * q);
+        return q;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Face pair codes (OpA, OpB, dot_normal): 0=(IDEN P1, IDEN P1); 1=(IDEN P2, IDEN P2); 2=(IDEN P3, IDEN P3);
+// 3=(IDEN P0, IDEN P2); 4=(IDEN P0, IDEN V2); 5=(GRAD P2, IDEN P2, normal); 6=(GRAD V2, IDEN V2, normal);
+// 7=(GRAD P1, GRAD P1); 8=(GRAD P3, IDEN P3, normal).  Layouts as ref_fem3dtet.
+int ref_fem3dface(int pair, int tensor_type, int per_point, int order, int face, long n, const double* xyz,
+                  const double* D, long ddim, double* A) {
+    using P0 = FemFix<FEM_P0>;
+    using P1 = FemFix<FEM_P1>;
+    using P2 = FemFix<FEM_P2>;
+    using P3 = FemFix<FEM_P3>;
+    using V2 = FemVec<3, FEM_P2>;
+    try {
+        switch (pair) {
+            case 0: return dispatch_face<Operator<IDEN, P1>, Operator<IDEN, P1>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 1: return dispatch_face<Operator<IDEN, P2>, Operator<IDEN, P2>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 2: return dispatch_face<Operator<IDEN, P3>, Operator<IDEN, P3>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 3: return dispatch_face<Operator<IDEN, P0>, Operator<IDEN, P2>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 4: return dispatch_face<Operator<IDEN, P0>, Operator<IDEN, V2>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 5: return dispatch_face<Operator<GRAD, P2>, Operator<IDEN, P2>, true>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 6: return dispatch_face<Operator<GRAD, V2>, Operator<IDEN, V2>, true>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 7: return dispatch_face<Operator<GRAD, P1>, Operator<GRAD, P1>, false>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            case 8: return dispatch_face<Operator<GRAD, P3>, Operator<IDEN, P3>, true>(tensor_type, per_point, order, face, n, xyz, D, ddim, A);
+            default: break;
+        }
+    } catch (const std::exception&) {
+        return -2;
+    }
+    return -1;
+}
 
 // Number of points of the reference rule; fills p[4*q], w[q] when non-null.
 int ref_quadrature(int order, double* p, double* w) {

@@ -72,6 +72,43 @@ def orc_elem(opA, femA, ncA, opB, femB, ncB, ttype, mode, order, xyz, D):
     return A.reshape(n, -1)
 
 
+def orc_face_quadrature(face):
+    p = np.zeros(4 * 36)
+    w = np.zeros(36)
+    q = oracle().orc_face_quadrature(face, P(p), P(w))
+    return p[:4 * q].reshape(q, 4), w[:q]
+
+
+def orc_face(opA, femA, ncA, opB, femB, ncB, ttype, mode, face, dot_normal, xyz, D):
+    """Oracle face integrals (fem3Dface / fem3DfaceN): (n, nfB*nfA)."""
+    n = len(xyz)
+    A = np.zeros(n * nfa(femA, ncA) * nfa(femB, ncB))
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    D = None if D is None else np.ascontiguousarray(D, dtype=np.float64)
+    rc = oracle().orc_fem3dface(opA, femA, ncA, opB, femB, ncB, ttype, mode, face, 1 if dot_normal else 0,
+                                ctypes.c_long(n), P(xyz), P(D), P(A))
+    if rc != 0:
+        raise RuntimeError(f"oracle fem3dface rc={rc}")
+    return A.reshape(n, -1)
+
+
+def points_physical(xyz, p):
+    """Physical points (n, q, 3) of barycentric points p (q, 4)."""
+    P0 = xyz[:, 0:3]
+    X = np.repeat(P0[:, None, :], len(p), axis=1).copy()
+    for l in range(3):
+        X += p[None, :, l + 1, None] * (xyz[:, None, 3 * (l + 1):3 * (l + 2)] - P0[:, None, :])
+    return X
+
+
+def face_golden_cases():
+    out = []
+    for f in sorted(glob.glob(os.path.join(GOLDEN_DIR, "face_*.npz"))):
+        z = np.load(f)
+        out.append((os.path.basename(f)[5:-4], {k: z[k] for k in z.files}))
+    return out
+
+
 def physical_points(xyz, order):
     """Physical quadrature points (n, q, 3) of the rule (core.inl:250-270)."""
     p, _ = orc_quadrature(order)

@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Write the golden element-matrix fixtures (tests/golden/elem_*.npz).
+"""Write the golden element-matrix fixtures (tests/golden/elem_*.npz) and
+face-integral fixtures (tests/golden/face_*.npz).
 
 Runs in the development container only: it loads oracle/_ref/libanifem_ref.so,
 the reference AniFem++ element core compiled from its own sources by
@@ -110,6 +111,62 @@ def cases(rng):
     return out
 
 
+# face pair codes (ref_fem3dface) -> (opA, femA, ncA, opB, femB, ncB, dot_normal)
+FACE_PAIRS = {
+    0: (1, 2, 1, 1, 2, 1, 0), 1: (1, 3, 1, 1, 3, 1, 0), 2: (1, 4, 1, 1, 4, 1, 0), 3: (1, 1, 1, 1, 3, 1, 0),
+    4: (1, 1, 1, 1, 3, 3, 0), 5: (2, 3, 1, 1, 3, 1, 1), 6: (2, 3, 3, 1, 3, 3, 1), 7: (2, 2, 1, 2, 2, 1, 0),
+    8: (2, 4, 1, 1, 4, 1, 1),
+}
+
+
+def face_cases(rng):
+    """(name, face pair, tensor_type, per_point, order, xyz, D (n, ddim))."""
+    n = 32
+    xyz = kuhn_tets(rng, n)
+    out = [
+        ("mass_p1_null", 0, 1, 0, 2, xyz, np.zeros((n, 1))),
+        ("mass_p2_scalar", 1, 2, 0, 4, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))),
+        ("mass_p3_null", 2, 1, 0, 6, xyz, np.zeros((n, 1))),
+        ("load_p2", 3, 2, 0, 2, xyz, rng.uniform(0.5, 2, (n, 1))),
+        ("traction_vec_p2", 4, 4, 0, 2, xyz, rng.normal(size=(n, 3))),
+        ("flux_p2_normal", 5, 2, 0, 3, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))),
+        ("grad_p1_general", 7, 4, 0, 1, xyz, np.array([aniso_tensor(rng).ravel(order="F") for _ in range(n)])),
+        ("flux_p3_normal_general", 8, 4, 0, 5, xyz, rng.normal(size=(n, 9))),
+        ("pp_mass_p2_scalar_xdep", 1, 2, 2, 6, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))),
+    ]
+    ne = 8
+    E = 10.0 ** rng.uniform(0, 3, ne)
+    nu = rng.uniform(0.2, 0.45, ne)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.array([lame_tensor(lam[r], mu[r]).ravel(order="F") for r in range(ne)])
+    out.append(("traction_lame_normal", 6, 4, 0, 3, xyz[:ne], D))
+    return out
+
+
+def write_faces(L, rng):
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.ref_fem3dface.restype = ctypes.c_int
+    for name, pair, tt, pp, order, xyz, D in face_cases(rng):
+        oa, fa, ca, ob, fb, cb, dn = FACE_PAIRS[pair]
+        nfa, nfb = NF[fa] * ca, NF[fb] * cb
+        n = len(xyz)
+        xyz = np.ascontiguousarray(xyz)
+        D = np.ascontiguousarray(D)
+        A = np.zeros((4, n * nfa * nfb))
+        for face in range(4):
+            Af = np.zeros(n * nfa * nfb)
+            rc = L.ref_fem3dface(pair, tt, pp, order, face, ctypes.c_long(n), xyz.ctypes.data_as(dp),
+                                 D.ctypes.data_as(dp), ctypes.c_long(D.shape[1]), Af.ctypes.data_as(dp))
+            if rc != 0:
+                sys.exit(f"face {name}: reference harness returned {rc}")
+            A[face] = Af
+        meta = np.array([pair, tt, pp, order, oa, fa, ca, ob, fb, cb, dn], dtype=np.int64)
+        np.savez_compressed(os.path.join(HERE, f"face_{name}.npz"), meta=meta, xyz=xyz, D=D,
+                            A=A.reshape(4, n, nfa * nfb))
+        print(f"face {name}: {n} tets x 4 faces, {nfb}x{nfa}")
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: build it with `make -C oracle`")
@@ -136,6 +193,7 @@ def main():
             arrays["lame"] = extra
         np.savez_compressed(os.path.join(HERE, f"elem_{name}.npz"), **arrays)
         print(f"{name}: {n} tets, {nfb}x{nfa}")
+    write_faces(L, np.random.default_rng(20260202))
 
 
 if __name__ == "__main__":

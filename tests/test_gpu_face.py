@@ -1,0 +1,95 @@
+"""GPU face integrals (anifem_fem3dface_batch: fem3Dface and fem3DfaceN of
+anifem++/fem/operations/int_face.inl) against the fixtures of the reference
+element core compiled here and against the FP64 oracle.  FP64 mode: <= 1e-12
+relative Frobenius per element; FP32 mode: <= 5e-6."""
+import numpy as np
+import pytest
+
+import anifem_amd as af
+import anifem_testlib as T
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 5e-6
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def run_face(opA, opB, tt, storage, D, order, face, xyz, dn, prec=af.PREC_FP64):
+    import torch
+    out = af.fem3dface_batch(opA, opB, tt, storage, D, order, face, dev(T.soa_xyz(xyz)), dot_normal=dn,
+                             precision=prec)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(len(xyz), -1).astype(np.float64)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in T.face_golden_cases()])
+def test_face_vs_reference_fixture(case):
+    z = dict(T.face_golden_cases())[case]
+    pair, tt, pp, order, oa, fa, ca, ob, fb, cb, dn = [int(v) for v in z["meta"]]
+    xyz, D = z["xyz"], z["D"]
+    n = len(xyz)
+    opA, opB = af.Op(oa, fa, ca), af.Op(ob, fb, cb)
+    for face in range(4):
+        Aref = z["A"][face]
+        if pp == 2:
+            # per-point tensor: the reference's field evaluated at this
+            # product's face points (polynomial integrand: rule-independent)
+            p, _ = af.face_quadrature(order, face)
+            m = T.xdep_field(T.points_physical(xyz, p))
+            Dp = dev(np.ascontiguousarray((m[:, :, None] * D[:, None, :]).transpose(2, 1, 0)))
+            storage = af.COEF_PER_POINT
+        elif tt == af.TENSOR_NULL:
+            Dp, storage = None, af.COEF_CONST
+        else:
+            Dp, storage = dev(np.ascontiguousarray(D.T)), af.COEF_PER_TET
+        for prec, tol in [(af.PREC_FP64, FP64_TOL), (af.PREC_FP32, FP32_TOL)]:
+            A = run_face(opA, opB, tt, storage, Dp, order, face, xyz, dn, prec)
+            err = T.rel_frob(A, Aref)
+            assert err <= tol, (case, face, prec, err)
+
+
+@pytest.mark.parametrize("face", [0, 1, 2, 3])
+def test_face_symmetric_tensor_and_const_storage_vs_oracle(face):
+    """SYMMETRIC per-tet tensors (packed upper triangle) and a CONST tensor,
+    which the fixtures do not cover, against the oracle."""
+    rng = np.random.default_rng(face)
+    xyz = rng.normal(size=(40, 12))
+    n = len(xyz)
+    Dfull = np.zeros((n, 9))
+    for r in range(n):
+        G = rng.normal(size=(3, 3))
+        Dfull[r] = (G @ G.T + np.eye(3)).ravel(order="F")
+    op = af.Op(af.GRAD, af.FEM_P2)
+    A = run_face(op, op, af.TENSOR_SYMMETRIC, af.COEF_PER_TET, dev(T.pack_sym(Dfull, 3).T.copy()), 4, face, xyz, False)
+    Ao = T.orc_face(2, 3, 1, 2, 3, 1, af.TENSOR_GENERAL, 1, face, 0, xyz, Dfull)
+    assert T.rel_frob(A, Ao) <= FP64_TOL
+    # CONST GENERAL 3 x 3 tensor in the normal form: ((D grad u) . n) v
+    Dc = rng.normal(size=9)
+    A = run_face(af.Op(af.GRAD, af.FEM_P3), af.Op(af.IDEN, af.FEM_P3), af.TENSOR_GENERAL, af.COEF_CONST, dev(Dc), 5,
+                 face, xyz, True)
+    Ao = T.orc_face(2, 4, 1, 1, 4, 1, af.TENSOR_GENERAL, 0, face, 1, xyz, Dc)
+    assert T.rel_frob(A, Ao) <= FP64_TOL
+
+
+def test_face_errors():
+    import torch
+    op = af.Op(af.IDEN, af.FEM_P1)
+    xyz = dev(T.soa_xyz(np.random.default_rng(0).normal(size=(4, 12))))
+    out = torch.empty((4, 4, 4), dtype=torch.float64, device="cuda")
+    t = af.anifem_tensor(af.TENSOR_NULL, af.COEF_CONST, None)
+    L = af.lib()
+    import ctypes
+    call = lambda face, dn, order, prec: L.anifem_fem3dface_batch(op.c(), op.c(), ctypes.byref(t), order, face, dn, 4,
+                                                                   af._ptr(xyz), prec, af._ptr(out), None)
+    assert call(0, 0, 2, af.PREC_FP64) == 0
+    assert call(4, 0, 2, af.PREC_FP64) == af.ERR_ARG
+    assert call(-1, 0, 2, af.PREC_FP64) == af.ERR_ARG
+    assert call(0, 0, 7, af.PREC_FP64) == af.ERR_ORDER
+    assert call(0, 0, 2, af.PREC_FP16X3) == af.ERR_ARG
+    # the normal form of a NULL tensor needs 3 dimB == dimA
+    assert call(0, 1, 2, af.PREC_FP64) == af.ERR_DIMS

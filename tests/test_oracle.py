@@ -160,6 +160,92 @@ def test_golden_fixtures_vs_oracle():
         assert err < 1e-13, (name, err)
 
 
+def test_face_fixtures_vs_oracle():
+    """Face integrals (fem3Dface / fem3DfaceN) of the reference element core
+    compiled here, on every face: the oracle with its own triangle rule
+    agrees to rounding."""
+    cases = T.face_golden_cases()
+    assert len(cases) >= 10
+    for name, z in cases:
+        pair, tt, pp, order, oa, fa, ca, ob, fb, cb, dn = [int(v) for v in z["meta"]]
+        xyz, D = z["xyz"], z["D"]
+        for face in range(4):
+            if pp == 2:
+                pts, _ = T.orc_face_quadrature(face)
+                m = T.xdep_field(T.points_physical(xyz, pts))
+                Dp = (m[:, :, None] * D[:, None, :]).reshape(len(xyz), -1)
+                A = T.orc_face(oa, fa, ca, ob, fb, cb, tt, PERPT, face, dn, xyz, Dp)
+            else:
+                A = T.orc_face(oa, fa, ca, ob, fb, cb, tt, PERTET, face, dn, xyz, D if tt != NULL else None)
+            err = T.rel_frob(A, z["A"][face])
+            assert err < 1e-13, (name, face, err)
+
+
+def test_face_closed_forms():
+    """P1 face mass = area/12 (1 + delta_ij) on the face's vertices and 0 on the
+    opposite one; int_f phi_i = area/3; in fem3DfaceN with D = I the P1
+    gradients sum to zero, so every row of (grad u . n, v) sums to 0 over the
+    trial DOFs."""
+    rng = np.random.default_rng(5)
+    xyz = rng.normal(size=(6, 12))
+    for face in range(4):
+        Pv = xyz.reshape(6, 4, 3)
+        idx = [face, (face + 1) % 4, (face + 2) % 4]
+        a = np.cross(Pv[:, idx[1]] - Pv[:, idx[0]], Pv[:, idx[2]] - Pv[:, idx[0]])
+        area = np.linalg.norm(a, axis=1) / 2
+        M = T.orc_face(IDEN, P1, 1, IDEN, P1, 1, NULL, CONST, face, 0, xyz, None).reshape(6, 4, 4)
+        for r in range(6):
+            ref = np.zeros((4, 4))
+            for i in idx:
+                for j in idx:
+                    ref[i, j] = area[r] / 12 * (2 if i == j else 1)
+            assert np.allclose(M[r], ref, atol=1e-14 * area[r])
+        F = T.orc_face(IDEN, P0, 1, IDEN, P1, 1, NULL, CONST, face, 0, xyz, None).reshape(6, 4)
+        for r in range(6):
+            ref = np.zeros(4)
+            ref[idx] = area[r] / 3
+            assert np.allclose(F[r], ref, atol=1e-14 * area[r])
+        # sum_j grad(lambda_j) = 0: every row of (grad u . n, v) sums to 0 over u's DOFs
+        G = T.orc_face(GRAD, P1, 1, IDEN, P1, 1, SCALAR, CONST, face, 1, xyz, np.ones(1)).reshape(6, 4, 4)
+        assert np.abs(G.sum(axis=1)).max() < 1e-12 * np.abs(G).max()
+
+
+def test_face_rule_exactness_and_reference_rules():
+    """The product's triangle rules (anifem_face_quadrature) integrate every
+    barycentric monomial on the face up to their degree; the reference's
+    rules of the same order (where its core is built here) integrate the same
+    monomials to the same values."""
+    import anifem_amd as af
+    R = T.ref_lib()
+    for order, deg in [(1, 1), (2, 2), (3, 3), (4, 4), (5, 5), (6, 6)]:
+        for face in range(4):
+            p, w = af.face_quadrature(order, face)
+            assert abs(w.sum() - 1) < 1e-15
+            assert np.all(p[:, (face + 3) % 4] == 0)
+            L = p[:, [face, (face + 1) % 4, (face + 2) % 4]]
+            assert np.all(L > 0) and np.all(w > 0)
+            for s_ in range(deg + 1):
+                for a in range(s_ + 1):
+                    for b in range(s_ - a + 1):
+                        al = (a, b, s_ - a - b)
+                        f = math.factorial
+                        exact = 2 * f(al[0]) * f(al[1]) * f(al[2]) / f(s_ + 2)
+                        val = np.sum(w * np.prod(L ** np.array(al), axis=1))
+                        assert abs(val - exact) < 1e-15 * max(1, exact), (order, al)
+        if R is not None:
+            q = R.ref_triangle_quadrature(order, None, None)
+            pr, wr = np.zeros(3 * q), np.zeros(q)
+            R.ref_triangle_quadrature(order, T.P(pr), T.P(wr))
+            pr = pr.reshape(q, 3)
+            for s_ in range(deg + 1):
+                for a in range(s_ + 1):
+                    for b in range(s_ - a + 1):
+                        al = np.array((a, b, s_ - a - b))
+                        f = math.factorial
+                        exact = 2 * f(al[0]) * f(al[1]) * f(al[2]) / f(s_ + 2)
+                        assert abs(np.sum(wr * np.prod(pr ** al, axis=1)) - exact) < 1e-14
+
+
 def _mesh_system(n, fem, dirichlet):
     import anifem_amd as af
     coords, tets = af.kuhn_cube(n, 0.2, 11)
