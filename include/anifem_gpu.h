@@ -282,6 +282,16 @@ int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int3
  * the same deterministic order as anifem_assembly_run. */
 int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const void* F_elem, int elem_precision,
                                  double* csr_val, double* rhs, anifem_status* status, void* stream);
+/* AssembleRHS (anifem++/inmost_interface/assembler.inl:497-561): the global
+ * right-hand side from per-cell load vectors F_elem (n_tets x nf, FP64 or FP32,
+ * local DOF order after the node reorder), with element-level homogeneous
+ * Dirichlet (a Dirichlet row receives 0).  accumulate = 0 stores the sum,
+ * accumulate = 1 adds it: rhs <- rhs + assembled, as the reference.  Each row
+ * sums its cells' entries in ascending cell order (deterministic; no atomics).
+ * Boundary load terms (Neumann) enter as the face integrals of
+ * anifem_fem3dface_batch added into the load vector of the face's cell. */
+int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_precision, double* rhs, int accumulate,
+                            anifem_status* status, void* stream);
 /* Host copy of the tets' node lists after the orientation reorder, caller's
  * tet order (n_tets x 4). */
 int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets);

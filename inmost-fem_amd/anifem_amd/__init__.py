@@ -126,6 +126,7 @@ SIGNATURES = {
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_run_elements": (_I32, [_VP, _VP, _VP, _I32, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_tets": (_I32, [_VP, _VP]),
+    "anifem_assembly_run_rhs": (_I32, [_VP, _VP, _I32, _VP, _I32, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_set_scatter": (_I32, [_VP, _I32]),
     "anifem_assembly_colours": (_I32, [_VP, _VP, _VP]),
     "anifem_assembly_run_plain": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _VP,
@@ -449,6 +450,22 @@ class Assembler:
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run_elements")
         return val, rhs, (st if status else None)
+
+    def assemble_rhs(self, F_elem, rhs=None, accumulate=False, status=True, stream=None):
+        """AssembleRHS: the rhs from per-cell load vectors (n_tets, nf), float64
+        or float32 (anifem_assembly_run_rhs); accumulate adds to rhs."""
+        import torch
+        if rhs is None:
+            rhs = torch.zeros(self.ndof, dtype=torch.float64, device=F_elem.device)
+        if F_elem.dtype not in (torch.float64, torch.float32):
+            raise AnifemError(ERR_ARG, "load vectors must be float64 or float32")
+        prec = PREC_FP64 if F_elem.dtype == torch.float64 else PREC_FP32
+        st = anifem_status()
+        rc = lib().anifem_assembly_run_rhs(self._h, _ptr(F_elem), prec, _ptr(rhs), 1 if accumulate else 0,
+                                           ctypes.byref(st) if status else None, _stream_handle(stream))
+        if rc not in (OK, ERR_NONFINITE):
+            raise AnifemError(rc, "anifem_assembly_run_rhs")
+        return rhs, (st if status else None)
 
     def set_persistent(self, on=True):
         _check(lib().anifem_assembly_set_persistent(self._h, 1 if on else 0), "set_persistent")

@@ -714,6 +714,50 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     return ANIFEM_OK;
 }
 
+int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_precision, double* rhs, int accumulate,
+                            anifem_status* status, void* stream) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only || !F_elem || !rhs) return ANIFEM_ERR_ARG;
+    if (elem_precision != ANIFEM_PREC_FP64 && elem_precision != ANIFEM_PREC_FP32) return ANIFEM_ERR_ARG;
+    const AssemblySetup& S = ctx->S;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BlockArgs a{};
+    a.has_rhs = 1;
+    a.n_kinds = 1;
+    a.kinds[0] = KindTable{nullptr, nullptr, nullptr, nullptr, 0, 0, S.nf, 0, 0, 0};
+    a.n_blocks = S.n_blocks;
+    a.meta = ctx->d_blk_meta;
+    a.pair_tet = ctx->d_pair_tet;
+    a.pair_ib = ctx->d_pair_ib;
+    a.pstart = ctx->d_pstart;
+    a.rhs = rhs;
+    a.status = ctx->d_status;
+    if (S.max_block_pairs > kBlock) return ANIFEM_ERR_DIMS;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    if (S.n_blocks > 0) {
+        const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
+        if (elem_precision == ANIFEM_PREC_FP64)
+            hipLaunchKernelGGL(row_block_rhs_kernel<double>, dim3(grid), dim3(kBlock), 0, s, a,
+                               static_cast<const double*>(F_elem), accumulate ? 1 : 0);
+        else
+            hipLaunchKernelGGL(row_block_rhs_kernel<float>, dim3(grid), dim3(kBlock), 0, s, a,
+                               static_cast<const float*>(F_elem), accumulate ? 1 : 0);
+        ANIFEM_HIP_CHECK(hipGetLastError());
+    }
+    if (status) {
+        int h[4] = {0, 0, 0, 0};
+        ANIFEM_HIP_CHECK(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
+        status->n_nonfinite = h[0];
+        status->n_degenerate = 0;
+        status->code = h[0] ? ANIFEM_ERR_NONFINITE : ANIFEM_OK;
+        status->reserved = 0;
+        return status->code;
+    }
+    return ANIFEM_OK;
+}
+
 int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets) {
     if (!ctx || !tets) return ANIFEM_ERR_ARG;
     std::memcpy(tets, ctx->S.tets.data(), ctx->S.tets.size() * sizeof(int32_t));

@@ -405,6 +405,38 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a,
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
+// Right-hand side only (AssembleRHS, assembler.inl:497-561): one workgroup per
+// block; each pair's load-vector entry (0 on a Dirichlet row, applyDir) goes
+// to LDS, then one lane per row adds its pairs' entries in ascending tet order
+// and stores (accumulate = 0) or adds the sum to rhs (rhs <- rhs + assembled).
+template <typename TE>
+__global__ __launch_bounds__(kBlock) void row_block_rhs_kernel(BlockArgs a, const TE* __restrict__ Fe, int accumulate) {
+    __shared__ double fv[kBlock];
+    const int64_t per_xcd = (a.n_blocks + 7) / 8;
+    const int64_t k = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (k >= a.n_blocks) return;  // uniform over the workgroup
+    const BlockMeta M = a.meta[k];
+    const int nf = a.kinds[M.kind].nf;
+    int nonfinite = 0;
+    if ((int)threadIdx.x < M.np) {
+        const int64_t gp = M.p0 + threadIdx.x;
+        const int64_t tet = a.pair_tet[gp];
+        const int ibf = a.pair_ib[gp];
+        const double v = (ibf & kPairDirichlet) ? 0.0 : (double)Fe[tet * nf + (ibf & 63)];
+        nonfinite += isfinite(v) ? 0 : 1;
+        fv[threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < M.nr; u += kBlock) {
+        const int pb = a.pstart[M.r0 + u];
+        const int pe = u + 1 < M.nr ? a.pstart[M.r0 + u + 1] : M.np;
+        double sum = 0.0;
+        for (int p = pb; p < pe; ++p) sum += fv[p];
+        a.rhs[M.r0 + u] = accumulate ? a.rhs[M.r0 + u] + sum : sum;
+    }
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
 }  // namespace anifem
 
 namespace anifem {

@@ -93,3 +93,72 @@ def test_face_errors():
     assert call(0, 0, 2, af.PREC_FP16X3) == af.ERR_ARG
     # the normal form of a NULL tensor needs 3 dimB == dimA
     assert call(0, 1, 2, af.PREC_FP64) == af.ERR_DIMS
+
+
+def test_assemble_rhs_matches_elements_path_and_accumulates():
+    """AssembleRHS (anifem_assembly_run_rhs): the rhs of anifem_assembly_run_elements
+    with zero matrices, bit for bit; accumulate adds to the given rhs; Dirichlet
+    rows receive 0."""
+    import torch
+    rng = np.random.default_rng(3)
+    coords, tets = af.kuhn_cube(4, 0.2, 2)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), dirichlet=af.DIRICHLET_BOUNDARY)
+    n, nf = Asm.n_tets, Asm.nf
+    F = torch.from_numpy(rng.normal(size=(n, nf))).cuda()
+    rhs1, st = Asm.assemble_rhs(F)
+    assert st.code == 0
+    _, rhs2, _ = Asm.assemble_elements(torch.zeros((n, nf, nf), dtype=torch.float64, device="cuda"), F)
+    assert torch.equal(rhs1, rhs2)
+    rhs0 = torch.from_numpy(rng.normal(size=Asm.ndof)).cuda()
+    rhs3, _ = Asm.assemble_rhs(F, rhs=rhs0.clone(), accumulate=True)
+    assert torch.equal(rhs3, rhs0 + rhs1)
+    dr = torch.from_numpy(Asm.setup()["dirichlet"].astype(bool)).cuda()
+    assert torch.all(rhs1[dr] == 0)
+    rhs32, _ = Asm.assemble_rhs(F.float())
+    assert float((rhs32 - rhs1).abs().max() / rhs1.abs().max()) < 1e-6
+
+
+def boundary_items(tets):
+    """(tet, local face) pairs of the faces that belong to one tet, local face k
+    through vertices k, k+1, k+2 mod 4 of the given (reordered) node lists."""
+    n = len(tets)
+    keys = np.stack([np.sort(tets[:, [k, (k + 1) % 4, (k + 2) % 4]], axis=1) for k in range(4)], axis=1)
+    flat = keys.reshape(-1, 3)
+    _, inv, cnt = np.unique(flat, axis=0, return_inverse=True, return_counts=True)
+    once = cnt[inv.ravel()] == 1
+    idx = np.nonzero(once)[0]
+    return idx // 4, idx % 4
+
+
+def test_neumann_boundary_load_via_face_integrals():
+    """A Neumann load g on the boundary of the unit cube: face integrals of
+    IDEN P0 x IDEN P2 (fem3Dface) over the boundary faces, added into their
+    cells' load vectors, assembled with AssembleRHS.  The total is g |dOmega| =
+    6 g, and the rhs equals the oracle's face integrals scattered in cell order."""
+    import torch
+    coords, tets = af.kuhn_cube(4, 0.2, 5)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), dirichlet=af.DIRICHLET_NONE)
+    n, nf = Asm.n_tets, Asm.nf
+    rt = Asm.tets()  # node lists after the reorder: the local DOF order of the cells
+    xyz = coords[rt].reshape(n, 12)
+    bt, bf = boundary_items(rt)
+    g = 1.75
+    F = np.zeros((n, nf))
+    Fo = np.zeros((n, nf))
+    p0, p2 = af.Op(af.IDEN, af.FEM_P0), af.Op(af.IDEN, af.FEM_P2)
+    for k in range(4):
+        sel = bt[bf == k]
+        if len(sel) == 0:
+            continue
+        Ff = run_face(p0, p2, af.TENSOR_SCALAR, af.COEF_CONST, dev(np.array([g])), 2, k, xyz[sel], False)
+        np.add.at(F, sel, Ff)
+        Fof = T.orc_face(1, 1, 1, 1, 3, 1, af.TENSOR_SCALAR, 0, k, 0, xyz[sel], np.array([g]))
+        np.add.at(Fo, sel, Fof)
+    rhs, st = Asm.assemble_rhs(torch.from_numpy(F).cuda())
+    assert st.code == 0
+    rhs = rhs.cpu().numpy()
+    assert abs(rhs.sum() - 6 * g) < 1e-12 * 6 * g
+    S = Asm.setup()
+    _, _, r_or = T.oracle_scatter(n, nf, S["dofs"], S["slot"], np.zeros((n, nf * nf)), Fo, len(S["col_idx"]),
+                                  Asm.ndof)
+    assert np.abs(rhs - r_or).max() <= 1e-12 * np.abs(r_or).max()
