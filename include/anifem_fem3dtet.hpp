@@ -109,6 +109,27 @@ inline int fem3Dtet(const double* xyz, int64_t n_tets, const TensorView& D, void
     return anifem_fem3dtet_batch(OpA::c(), OpB::c(), &t, order, n_tets, xyz, precision, A_out, stream);
 }
 
+// Batched fem3Dface / fem3DfaceN (int_face.h:15-45): integrals over face
+// `face_num` (through vertices face_num, face_num+1, face_num+2 mod 4) of each
+// tet; in fem3DfaceN the tensor has 3*dimB rows and is contracted with the
+// outward unit normal.  Layouts as fem3Dtet; FP64 or FP32.
+template <typename OpA, typename OpB, typename FuncTraits = DfuncTraits<>>
+inline int fem3Dface(const double* xyz, int64_t n_tets, int face_num, const TensorView& D, void* A_out,
+                     int order = 5, Precision precision = FP64, void* stream = nullptr) {
+    const int rc = check_view<FuncTraits>(D);
+    if (rc) return rc;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dface_batch(OpA::c(), OpB::c(), &t, order, face_num, 0, n_tets, xyz, precision, A_out, stream);
+}
+template <typename OpA, typename OpB, typename FuncTraits = DfuncTraits<>>
+inline int fem3DfaceN(const double* xyz, int64_t n_tets, int face_num, const TensorView& D, void* A_out,
+                      int order = 5, Precision precision = FP64, void* stream = nullptr) {
+    const int rc = check_view<FuncTraits>(D);
+    if (rc) return rc;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dface_batch(OpA::c(), OpB::c(), &t, order, face_num, 1, n_tets, xyz, precision, A_out, stream);
+}
+
 // Nfa / Dim of an operator (poly_*.h, operators.h:127-155)
 template <typename Op>
 inline int nfa() {
@@ -176,6 +197,17 @@ public:
         if (!ctx_) return status_ ? status_ : ANIFEM_ERR_ARG;
         return anifem_assembly_run_elements(ctx_, A_elem, F_elem, sizeof(TE) == 8 ? FP64 : FP32, csr_val, rhs, st,
                                             stream);
+    }
+    // AssembleRHS(rhs) (assembler.inl:497-561): rhs from per-cell load vectors
+    // F_elem (n_tets x nf), stored (accumulate = false) or added (rhs <- rhs +
+    // assembled, the reference's semantics).
+    template <typename TE>
+    int AssembleRHS(const TE* F_elem, double* rhs, bool accumulate = true, anifem_status* st = nullptr,
+                    void* stream = nullptr) {
+        static_assert(sizeof(TE) == 8 || sizeof(TE) == 4, "double or float load vectors");
+        if (!ctx_) return status_ ? status_ : ANIFEM_ERR_ARG;
+        return anifem_assembly_run_rhs(ctx_, F_elem, sizeof(TE) == 8 ? FP64 : FP32, rhs, accumulate ? 1 : 0, st,
+                                       stream);
     }
     // Host copy of the tets' nodes after the orientation reorder (n_tets x 4)
     int Tets(int32_t* tets) const { return ctx_ ? anifem_assembly_tets(ctx_, tets) : ANIFEM_ERR_ARG; }

@@ -40,11 +40,18 @@ int main() {
     const TensorView null_const{TENSOR_NULL, ANIFEM_COEF_CONST, nullptr};
     CHECK((fem3Dtet<GP2, Operator<IDEN, FemFix<FEM_P2>>, DfuncTraits<TENSOR_NULL, true>>(buf, 1, null_const, buf, 2) ==
            ANIFEM_ERR_DIMS));
+    // face integrals: the triangle rules stop at order 6; faces are 0..3
+    using IP2 = Operator<IDEN, FemFix<FEM_P2>>;
+    CHECK((fem3Dface<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 0, null_const, buf, 7) == ANIFEM_ERR_ORDER));
+    CHECK((fem3Dface<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 4, null_const, buf, 2) == ANIFEM_ERR_ARG));
+    // the normal form of an identity needs 3 dimB == dimA (GRAD against IDEN: 3 == 3)
+    CHECK((fem3DfaceN<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 0, null_const, buf, 2) == ANIFEM_ERR_DIMS));
     // a context that could not be created reports its status from every call
     Assembler<GP2, GP2> bad(0, nullptr, 0, nullptr, 2, sym_tet, 2, null_const, ANIFEM_DIRICHLET_NONE);
     CHECK(bad.status() != ANIFEM_OK);
     CHECK(bad.AssembleElements<double>(nullptr, nullptr, nullptr, nullptr) == bad.status());
     CHECK(bad.AssembleElements<float>(nullptr, nullptr, nullptr, nullptr) == bad.status());
+    CHECK(bad.AssembleRHS<double>(nullptr, nullptr) == bad.status());
     std::printf("header ok\n");
     return 0;
 }
