@@ -112,6 +112,14 @@ int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, 
  * precisions as anifem_fem3dtet_batch, FP64 and FP32 only. */
 int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int face, int dot_normal,
                            int64_t n_tets, const double* xyz, int precision, void* A_out, void* stream);
+/* Batched fem3DapplyL<Op>(XYZ, XYL, dofs, opU) (anifem++/fem/operations/
+ * eval.h:13-18, eval.inl:84-100, core.inl:370-405): Op(u) at q barycentric
+ * points xyl (host, 4q, the same for every tet) of each tet, u given by its
+ * per-tet DOF values dofs (device, n_tets x nfa: dofs[r*nfa + i]).  opU
+ * (device) receives n_tets blocks of dim*q values: opU[r*dim*q + k + dim*n]
+ * (for a vector space, component k + 3c of GRAD is du_c/dx_k). */
+int anifem_fem3dapply_batch(anifem_op op, const double* xyl, int q, int64_t n_tets, const double* xyz,
+                            const double* dofs, double* opU, void* stream);
 /* Triangle rule of `order` placed on face `face`, as tet barycentric points
  * p[4q] (coordinate of the opposite vertex 0) and weights w[q] summing to 1;
  * returns q (or <0). */
@@ -292,6 +300,14 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
  * anifem_fem3dface_batch added into the load vector of the face's cell. */
 int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_precision, double* rhs, int accumulate,
                             anifem_status* status, void* stream);
+/* Evaluation of a global FE vector u (device, ndof; e.g. the PCG solution) in
+ * every cell: op = ANIFEM_IDEN or ANIFEM_GRAD of the context's trial space at
+ * q barycentric points xyl (host, 4q) of each cell, layout as
+ * anifem_fem3dapply_batch, caller's cell order.  The cell's DOF values are
+ * gathered through the context's DOF map (NATURAL numbering, P3 edge
+ * orientation), the step after the solve of examples/tutorials/ex1.cpp:167-168. */
+int anifem_assembly_eval(anifem_assembly* ctx, int op, const double* xyl, int q, const double* u, double* opU,
+                         void* stream);
 /* Host copy of the tets' node lists after the orientation reorder, caller's
  * tet order (n_tets x 4). */
 int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets);

@@ -109,6 +109,8 @@ SIGNATURES = {
     "anifem_fem3dface_batch": (_I32, [anifem_op, anifem_op, ctypes.POINTER(anifem_tensor), _I32, _I32, _I32, _I64,
                                       _VP, _I32, _VP, _VP]),
     "anifem_face_quadrature": (_I32, [_I32, _I32, _VP, _VP]),
+    "anifem_fem3dapply_batch": (_I32, [anifem_op, _VP, _I32, _I64, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_eval": (_I32, [_VP, _I32, _VP, _I32, _VP, _VP, _VP]),
     "anifem_assembly_create_budget": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
                                              ctypes.POINTER(anifem_problem_desc), _I32, _I32, _I32, _VP]),
     "anifem_assembly_create": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
@@ -296,6 +298,52 @@ def fem3dface_batch(opA: Op, opB: Op, ttype, storage, D, order, face, xyz, dot_n
     return out
 
 
+def fem3dapply_batch(op: Op, xyl, xyz, dofs, out=None, stream=None):
+    """Op(u) at the barycentric points xyl (q, 4) of n tets with per-tet DOF
+    values dofs (device (n, nfa)) (anifem_fem3dapply_batch): (n, q, dim)."""
+    import torch
+    xyl = np.ascontiguousarray(xyl, dtype=np.float64)
+    q, n = len(xyl), xyz.shape[1]
+    if out is None:
+        out = torch.empty((n, q, op.dim), dtype=torch.float64, device=xyz.device)
+    _check(lib().anifem_fem3dapply_batch(op.c(), _ptr(xyl), q, n, _ptr(xyz), _ptr(dofs), _ptr(out),
+                                         _stream_handle(stream)), "anifem_fem3dapply_batch")
+    return out
+
+
+def write_vtu(path, coords, tets, point_data=None, cell_data=None):
+    """Unstructured-grid VTK file (XML .vtu, ASCII) of a tetrahedral mesh with
+    per-node and per-cell fields (arrays of n or (n, k) values), the output
+    step after the solve (the reference saves through its mesh library,
+    examples/tutorials/ex1.cpp:167-168)."""
+    coords = np.asarray(coords, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+
+    def arr(name, a):
+        a = np.asarray(a, dtype=np.float64)
+        k = 1 if a.ndim == 1 else a.shape[1]
+        body = " ".join(repr(float(v)) for v in a.ravel())
+        return (f'<DataArray type="Float64" Name="{name}" NumberOfComponents="{k}" format="ascii">'
+                f"{body}</DataArray>\n")
+
+    lines = ['<?xml version="1.0"?>\n',
+             '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">\n<UnstructuredGrid>\n',
+             f'<Piece NumberOfPoints="{len(coords)}" NumberOfCells="{len(tets)}">\n']
+    lines.append("<PointData>\n" + "".join(arr(k, v) for k, v in (point_data or {}).items()) + "</PointData>\n")
+    lines.append("<CellData>\n" + "".join(arr(k, v) for k, v in (cell_data or {}).items()) + "</CellData>\n")
+    lines.append("<Points>\n" + arr("Points", coords) + "</Points>\n")
+    lines.append("<Cells>\n")
+    lines.append('<DataArray type="Int64" Name="connectivity" format="ascii">' +
+                 " ".join(map(str, tets.ravel())) + "</DataArray>\n")
+    lines.append('<DataArray type="Int64" Name="offsets" format="ascii">' +
+                 " ".join(map(str, 4 * np.arange(1, len(tets) + 1))) + "</DataArray>\n")
+    lines.append('<DataArray type="UInt8" Name="types" format="ascii">' + " ".join(["10"] * len(tets)) +
+                 "</DataArray>\n")  # VTK_TETRA
+    lines.append("</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
+    with open(path, "w") as fh:
+        fh.write("".join(lines))
+
+
 # ---- meshes ----------------------------------------------------------------
 def kuhn_cube(n, jitter=0.2, seed=0):
     """Jittered Kuhn cube: (coords (N,3) float64, tets (6n^3, 4) int32)."""
@@ -466,6 +514,30 @@ class Assembler:
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run_rhs")
         return rhs, (st if status else None)
+
+    def node_values(self, u):
+        """Per-node values (n_nodes, ncomp) of a global vector u (host or
+        device): the vertex DOFs, which come first in each component's block of
+        the NATURAL numbering (global_enumerator.cpp:562-777)."""
+        u = np.asarray(u.cpu() if hasattr(u, "cpu") else u)
+        nc = self.trial.ncomp
+        nscal = self.ndof // nc
+        nn = len(self._coords)
+        return np.stack([u[c * nscal:c * nscal + nn] for c in range(nc)], axis=1)
+
+    def evaluate(self, u, xyl, op=GRAD, out=None, stream=None):
+        """op (IDEN or GRAD) of the trial space's global vector u (device,
+        ndof) at the barycentric points xyl (q, 4) of every cell
+        (anifem_assembly_eval): (n_tets, q, dim)."""
+        import torch
+        xyl = np.ascontiguousarray(xyl, dtype=np.float64)
+        q = len(xyl)
+        dim = (3 if op == GRAD else 1) * self.trial.ncomp
+        if out is None:
+            out = torch.empty((self.n_tets, q, dim), dtype=torch.float64, device=u.device)
+        _check(lib().anifem_assembly_eval(self._h, op, _ptr(xyl), q, _ptr(u), _ptr(out), _stream_handle(stream)),
+               "anifem_assembly_eval")
+        return out
 
     def set_persistent(self, on=True):
         _check(lib().anifem_assembly_set_persistent(self._h, 1 if on else 0), "set_persistent")

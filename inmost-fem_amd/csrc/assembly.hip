@@ -20,6 +20,7 @@
 
 namespace anifem {
 int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a);
+int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s);
 }  // namespace anifem
 
 struct anifem_assembly {
@@ -756,6 +757,24 @@ int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_p
         return status->code;
     }
     return ANIFEM_OK;
+}
+
+int anifem_assembly_eval(anifem_assembly* ctx, int op, const double* xyl, int q, const double* u, double* opU,
+                         void* stream) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only || !u || !opU || (op != ANIFEM_IDEN && op != ANIFEM_GRAD)) return ANIFEM_ERR_ARG;
+    const AssemblySetup& S = ctx->S;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    if (!ctx->d_dofs && (rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
+    ApplyArgs a{};
+    a.n = S.n_tets;
+    a.coords = ctx->d_coords;
+    a.tetnodes = ctx->d_tetnodes;
+    a.dofmap = ctx->d_dofs;
+    a.u = u;
+    a.out = opU;
+    return launch_apply(anifem_op{op, ctx->prob.trial.fem, ctx->prob.trial.ncomp}, xyl, q, a, true, s);
 }
 
 int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets) {

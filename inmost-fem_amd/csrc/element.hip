@@ -167,6 +167,47 @@ extern "C" int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem
     return ANIFEM_ERR_ARG;  // the MFMA mode computes volume integrals only
 }
 
+namespace anifem {
+// Reference table of op at q barycentric points, on the device, and the
+// launch of apply_kernel (the table is freed after the kernel has run).
+int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s) {
+    if (!op_valid(op) || !xyl || q < 1 || q > 96) return ANIFEM_ERR_ARG;
+    std::vector<double> tab;
+    reference_table(op.op, op.fem, q, xyl, tab);
+    a.grad = op.op == ANIFEM_GRAD ? 1 : 0;
+    a.nf0 = nf_scalar(op.fem);
+    a.ncomp = op.ncomp;
+    a.q = q;
+    if (a.n == 0) return ANIFEM_OK;
+    double* dtab = nullptr;
+    ANIFEM_HIP_CHECK(hipMalloc(&dtab, tab.size() * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        a.tab = dtab;
+        const int64_t blocks = (a.n * q + kBlock - 1) / kBlock;
+        if (mesh) hipLaunchKernelGGL(apply_kernel<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL(apply_kernel<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dtab);
+    ANIFEM_HIP_CHECK(e);
+    return ANIFEM_OK;
+}
+}  // namespace anifem
+
+extern "C" int anifem_fem3dapply_batch(anifem_op op, const double* xyl, int q, int64_t n_tets, const double* xyz,
+                                       const double* dofs, double* opU, void* stream) {
+    using namespace anifem;
+    if (n_tets < 0 || (n_tets > 0 && (!xyz || !dofs || !opU))) return ANIFEM_ERR_ARG;
+    ApplyArgs a{};
+    a.n = n_tets;
+    a.xyz = xyz;
+    a.dofs = dofs;
+    a.out = opU;
+    return launch_apply(op, xyl, q, a, false, static_cast<hipStream_t>(stream));
+}
+
 extern "C" int anifem_face_quadrature(int order, int face, double* p, double* w) {
     std::vector<double> pts, wts;
     const int q = anifem::rule_points(order, face, pts, wts);

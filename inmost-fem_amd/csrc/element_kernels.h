@@ -244,6 +244,56 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     }
 }
 
+// ---- solution evaluation: fem3DapplyL (eval.inl:84-100, core.inl:370-405) ----
+struct ApplyArgs {
+    int grad;              // GRAD (else IDEN)
+    int nf0, ncomp, q;     // scalar basis size, components, points per tet
+    const double* tab;     // reference values / gradients of the scalar basis: tab[(n*nf0 + i)*d + a]
+    int64_t n;             // tets
+    const double* xyz;     // batch: 12 x n SoA coordinates
+    const double* coords;  // mesh: node coordinates (n_nodes x 3) ...
+    const int32_t* tetnodes;  // ... and 4 x n SoA node lists
+    const double* dofs;    // batch: per-tet DOF values, dofs[r*nf + i]
+    const int32_t* dofmap; // mesh: per-tet global DOF ids, dofmap[r*nf + i] ...
+    const double* u;       // ... into the global vector u
+    double* out;           // opU[r*dim*q + k + dim*n]
+};
+
+// One lane per (tet, point): Op(u)(x_n) = sum_i U(k, n, i) dof_i, with
+// U = PSI^T Ghat for GRAD (component k + 3c = du_c/dx_k for a vector space).
+template <bool MESH>
+__global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
+    const int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (idx >= a.n * a.q) return;
+    const int64_t r = idx / a.q;
+    const int pt = (int)(idx % a.q);
+    const int nf = a.nf0 * a.ncomp, dim = (a.grad ? 3 : 1) * a.ncomp;
+    double PSI[9];
+    if (a.grad) {
+        double P[4][3], det;
+        for (int v = 0; v < 4; ++v)
+            for (int c = 0; c < 3; ++c)
+                P[v][c] = MESH ? a.coords[3 * (int64_t)a.tetnodes[(int64_t)v * a.n + r] + c]
+                               : a.xyz[(int64_t)(3 * v + c) * a.n + r];
+        tet_geometry(P, PSI, det);
+    }
+    double* o = a.out + r * dim * a.q + (int64_t)dim * pt;
+    for (int c = 0; c < a.ncomp; ++c) {
+        double g[3] = {0.0, 0.0, 0.0};
+        const int d = a.grad ? 3 : 1;
+        for (int i = 0; i < a.nf0; ++i) {
+            const int64_t e = r * nf + c * a.nf0 + i;
+            const double dv = MESH ? a.u[a.dofmap[e]] : a.dofs[e];
+            for (int j = 0; j < d; ++j) g[j] += a.tab[(pt * a.nf0 + i) * d + j] * dv;
+        }
+        if (a.grad) {
+            for (int k = 0; k < 3; ++k) o[k + 3 * c] = PSI[0 + 3 * k] * g[0] + PSI[1 + 3 * k] * g[1] + PSI[2 + 3 * k] * g[2];
+        } else {
+            o[c] = g[0];
+        }
+    }
+}
+
 // ---- split-precision MFMA contraction (ANIFEM_PREC_FP16X3) -----------------
 // For a tensor constant over each tet the element matrices of a block of tets
 // are one product per component block (cb, ca):

@@ -599,6 +599,31 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
     return 0;
 }
 
+/* fem3DapplyL (eval.inl:84-100, core.inl:370-405): Op(u) at q barycentric
+ * points xyl of each tet, opU = V dofs with V[k + dim*n, i] the operator table
+ * of orc_apply_op; out[r*dim*q + k + dim*n]. */
+int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n, const double* xyz,
+                   const double* dofs, double* out) {
+    int nf = orc_nfa(fem, ncomp), dim = orc_dim(op, ncomp);
+    if (nf <= 0 || q <= 0 || (op == ORC_GRAD && fem == ORC_P0)) return -1;
+    double* U = malloc(sizeof(double) * (size_t)dim * q * nf);
+    for (long r = 0; r < n; ++r) {
+        const double* P = xyz + 12 * r;
+        double XYP[9], PSI[9];
+        for (int l = 0; l < 3; ++l)
+            for (int i = 0; i < 3; ++i) XYP[i + 3 * l] = P[3 * (l + 1) + i] - P[i];
+        orc_inverse3x3(XYP, PSI);
+        orc_apply_op(op, fem, ncomp, q, xyl, PSI, U);
+        for (int j = 0; j < dim * q; ++j) {
+            double s = 0;
+            for (int i = 0; i < nf; ++i) s += U[j + dim * q * i] * dofs[(size_t)nf * r + i];
+            out[(size_t)dim * q * r + j] = s;
+        }
+    }
+    free(U);
+    return 0;
+}
+
 void orc_apply_dir(int nf, double* A, double* F, int k, double bc) {
     for (int i = 0; i < nf; ++i) F[i] -= A[i + nf * k] * bc;
     F[k] = bc;

@@ -66,6 +66,15 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
 int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int tensor_type, int coef_mode, int face,
                   int dot_normal, long n, const double* xyz, const double* D, double* A);
 
+/* fem3DapplyL (anifem++/fem/operations/eval.inl:84-100, core.inl:370-405):
+ * Op(u) at q barycentric points xyl (4q) of n tets with per-tet DOF values
+ * dofs (n x nfa); out (n x dim*q) receives out[r*dim*q + k + dim*n].
+ * Pinned by: fixtures of the compiled reference's fem3DapplyL
+ * (tests/golden/apply_*.npz) and the reproduction of polynomials of the
+ * space's degree from their nodal interpolants. */
+int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n, const double* xyz,
+                   const double* dofs, double* out);
+
 /* Number of basis functions / operator dimension for (op, fem, ncomp). */
 int orc_nfa(int fem, int ncomp);
 int orc_dim(int op, int ncomp);

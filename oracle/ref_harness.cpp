@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
+#include <vector>
 
 using namespace Ani;
 
@@ -133,6 +134,21 @@ int dispatch_face(int tt, int per_point, int order, int face, long n, const doub
     return -3;
 }
 
+template <typename Op>
+int run_apply(int q, const double* xyl, long n, const double* xyz, const double* dofs, double* out) {
+    constexpr long nfa = Op::Nfa::value, dim = Op::Dim::value;
+    std::vector<double> L(xyl, xyl + 4 * q), dv(nfa), o(dim * q);
+    for (long r = 0; r < n; ++r) {
+        const double* p = xyz + 12 * r;
+        Tetra<const double> XYZ(p, p + 3, p + 6, p + 9);
+        std::copy(dofs + nfa * r, dofs + nfa * (r + 1), dv.begin());
+        DenseMatrix<double> D(dv.data(), nfa, 1, nfa), O(o.data(), dim * q, 1, dim * q);
+        fem3DapplyL<Op>(XYZ, ArrayView<double>(L.data(), 4 * q), D, O);
+        std::copy(o.begin(), o.end(), out + dim * q * r);
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -148,6 +164,31 @@ int ref_triangle_quadrature(int order, double* p, double* w) {
     } catch (...) {
         return -1;
     }
+}
+
+// fem3DapplyL (anifem++/fem/operations/eval.inl:84-100) per tet.  Op codes:
+// 0 IDEN P1, 1 IDEN P2, 2 GRAD P2, 3 IDEN P3, 4 GRAD P3, 5 IDEN V2, 6 GRAD V2.
+// xyl: 4q barycentric points; dofs n x nfa; out n x dim*q.
+int ref_fem3dapply(int code, int q, const double* xyl, long n, const double* xyz, const double* dofs, double* out) {
+    using P1 = FemFix<FEM_P1>;
+    using P2 = FemFix<FEM_P2>;
+    using P3 = FemFix<FEM_P3>;
+    using V2 = FemVec<3, FEM_P2>;
+    try {
+        switch (code) {
+            case 0: return run_apply<Operator<IDEN, P1>>(q, xyl, n, xyz, dofs, out);
+            case 1: return run_apply<Operator<IDEN, P2>>(q, xyl, n, xyz, dofs, out);
+            case 2: return run_apply<Operator<GRAD, P2>>(q, xyl, n, xyz, dofs, out);
+            case 3: return run_apply<Operator<IDEN, P3>>(q, xyl, n, xyz, dofs, out);
+            case 4: return run_apply<Operator<GRAD, P3>>(q, xyl, n, xyz, dofs, out);
+            case 5: return run_apply<Operator<IDEN, V2>>(q, xyl, n, xyz, dofs, out);
+            case 6: return run_apply<Operator<GRAD, V2>>(q, xyl, n, xyz, dofs, out);
+            default: break;
+        }
+    } catch (const std::exception&) {
+        return -2;
+    }
+    return -1;
 }
 
 // Face pair codes (OpA, OpB, dot_normal): 0=(IDEN P1, IDEN P1); 1=(IDEN P2, IDEN P2); 2=(IDEN P3, IDEN P3);

@@ -101,6 +101,28 @@ def points_physical(xyz, p):
     return X
 
 
+def orc_apply(op, fem, nc, xyl, xyz, dofs):
+    """Oracle fem3DapplyL: (n, q, dim)."""
+    n, q = len(xyz), len(xyl)
+    d = dim(op, nc)
+    out = np.zeros(n * q * d)
+    xyl = np.ascontiguousarray(xyl, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    dofs = np.ascontiguousarray(dofs, dtype=np.float64)
+    rc = oracle().orc_fem3dapply(op, fem, nc, q, P(xyl), ctypes.c_long(n), P(xyz), P(dofs), P(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle fem3dapply rc={rc}")
+    return out.reshape(n, q, d)
+
+
+def apply_golden_cases():
+    out = []
+    for f in sorted(glob.glob(os.path.join(GOLDEN_DIR, "apply_*.npz"))):
+        z = np.load(f)
+        out.append((os.path.basename(f)[6:-4], {k: z[k] for k in z.files}))
+    return out
+
+
 def face_golden_cases():
     out = []
     for f in sorted(glob.glob(os.path.join(GOLDEN_DIR, "face_*.npz"))):

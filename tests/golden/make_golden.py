@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
-"""Write the golden element-matrix fixtures (tests/golden/elem_*.npz) and
-face-integral fixtures (tests/golden/face_*.npz).
+"""Write the golden element-matrix fixtures (tests/golden/elem_*.npz),
+face-integral fixtures (tests/golden/face_*.npz) and solution-evaluation
+fixtures (tests/golden/apply_*.npz).
 
 Runs in the development container only: it loads oracle/_ref/libanifem_ref.so,
 the reference AniFem++ element core compiled from its own sources by
@@ -167,6 +168,32 @@ def write_faces(L, rng):
         print(f"face {name}: {n} tets x 4 faces, {nfb}x{nfa}")
 
 
+# fem3DapplyL op codes (ref_fem3dapply) -> (op, fem, ncomp)
+APPLY_OPS = {0: (1, 2, 1), 1: (1, 3, 1), 2: (2, 3, 1), 3: (1, 4, 1), 4: (2, 4, 1), 5: (1, 3, 3), 6: (2, 3, 3)}
+
+
+def write_apply(L, rng):
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.ref_fem3dapply.restype = ctypes.c_int
+    n, q = 24, 7
+    xyz = np.ascontiguousarray(kuhn_tets(rng, n))
+    lam = rng.dirichlet(np.ones(4), q)  # interior barycentric points
+    xyl = np.ascontiguousarray(lam)
+    for code, (op, fem, nc) in APPLY_OPS.items():
+        nf = NF[fem] * nc
+        dim = (3 if op == 2 else 1) * nc
+        dofs = np.ascontiguousarray(rng.normal(size=(n, nf)))
+        out = np.zeros(n * dim * q)
+        rc = L.ref_fem3dapply(code, q, xyl.ctypes.data_as(dp), ctypes.c_long(n), xyz.ctypes.data_as(dp),
+                              dofs.ctypes.data_as(dp), out.ctypes.data_as(dp))
+        if rc != 0:
+            sys.exit(f"apply {code}: reference harness returned {rc}")
+        meta = np.array([code, op, fem, nc], dtype=np.int64)
+        np.savez_compressed(os.path.join(HERE, f"apply_{code}.npz"), meta=meta, xyz=xyz, xyl=xyl, dofs=dofs,
+                            out=out.reshape(n, q, dim))
+        print(f"apply {code}: {n} tets x {q} points, dim {dim}")
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: build it with `make -C oracle`")
@@ -194,6 +221,7 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"elem_{name}.npz"), **arrays)
         print(f"{name}: {n} tets, {nfb}x{nfa}")
     write_faces(L, np.random.default_rng(20260202))
+    write_apply(L, np.random.default_rng(20260303))
 
 
 if __name__ == "__main__":

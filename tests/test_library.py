@@ -79,3 +79,25 @@ def test_cpp_header_mirrors_reference_tags(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "header ok" in out.stdout
+
+
+def test_write_vtu_round_trip(tmp_path):
+    """The VTU writer: an XML unstructured grid whose points, connectivity,
+    offsets, cell types and fields read back as written."""
+    import xml.etree.ElementTree as ET
+    coords, tets = af.kuhn_cube(2, 0.2, 1)
+    u = np.sin(coords[:, 0]) + coords[:, 1] ** 2
+    g = np.random.default_rng(0).normal(size=(len(tets), 3))
+    path = tmp_path / "mesh.vtu"
+    af.write_vtu(str(path), coords, tets, point_data={"u": u}, cell_data={"g": g})
+    root = ET.parse(str(path)).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfPoints")) == len(coords) and int(piece.get("NumberOfCells")) == len(tets)
+    arrays = {a.get("Name"): a for a in root.iter("DataArray")}
+    rd = lambda name: np.array(arrays[name].text.split(), dtype=np.float64)
+    assert np.array_equal(rd("Points").reshape(-1, 3), coords)
+    assert np.array_equal(rd("connectivity").reshape(-1, 4), tets)
+    assert np.array_equal(rd("offsets"), 4 * np.arange(1, len(tets) + 1))
+    assert np.all(rd("types") == 10)
+    assert np.array_equal(rd("u"), u) and np.array_equal(rd("g").reshape(-1, 3), g)
+    assert arrays["g"].get("NumberOfComponents") == "3"

@@ -246,6 +246,53 @@ def test_face_rule_exactness_and_reference_rules():
                         assert abs(np.sum(wr * np.prod(pr ** al, axis=1)) - exact) < 1e-14
 
 
+def test_apply_fixtures_vs_oracle():
+    """fem3DapplyL of the reference element core compiled here (IDEN/GRAD of
+    P1-P3 and of FemVec<3, P2>) at interior points: the oracle agrees."""
+    cases = T.apply_golden_cases()
+    assert len(cases) == 7
+    for name, z in cases:
+        code, op, fem, nc = [int(v) for v in z["meta"]]
+        out = T.orc_apply(op, fem, nc, z["xyl"], z["xyz"], z["dofs"])
+        err = np.abs(out - z["out"]).max() / np.abs(z["out"]).max()
+        assert err < 1e-13, (name, err)
+
+
+@pytest.mark.parametrize("fem,deg", [(P1, 1), (P2, 2), (P3, 3)])
+def test_apply_reproduces_polynomials(fem, deg):
+    """The nodal interpolant of a polynomial of the space's degree is the
+    polynomial: IDEN gives f and GRAD gives grad f at any point (the P1-P3
+    bases are Lagrange bases on the vertices, the points I + (J - I)/k of each
+    edge (P3: the node next to I first) and the face centroids)."""
+    rng = np.random.default_rng(fem)
+    c = rng.normal(size=10)
+    f = lambda x: (c[0] + c[1] * x[..., 0] + c[2] * x[..., 1] + c[3] * x[..., 2] +
+                   (deg >= 2) * (c[4] * x[..., 0] * x[..., 1] + c[5] * x[..., 2] ** 2) +
+                   (deg >= 3) * (c[6] * x[..., 0] * x[..., 1] * x[..., 2] + c[7] * x[..., 1] ** 3))
+
+    def grad(x, h=1e-6):
+        return np.stack([(f(x + h * e) - f(x - h * e)) / (2 * h) for e in np.eye(3)], axis=-1)
+    P = rng.normal(size=(4, 3))
+    E = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    nodes = [P[i] for i in range(4)]
+    if fem == P2:
+        nodes += [(P[i] + P[j]) / 2 for i, j in E]
+    if fem == P3:
+        for i, j in E:
+            nodes += [(2 * P[i] + P[j]) / 3, (P[i] + 2 * P[j]) / 3]
+        for k in range(4):
+            ids = [[0, 1, 2], [1, 2, 3], [0, 2, 3], [0, 1, 3]][k]
+            nodes.append(P[ids].mean(axis=0))
+    dofs = f(np.array(nodes))[None, :]
+    xyl = rng.dirichlet(np.ones(4), 6)
+    X = xyl @ P
+    xyz = P.reshape(1, 12)
+    val = T.orc_apply(IDEN, fem, 1, xyl, xyz, dofs)[0, :, 0]
+    assert np.allclose(val, f(X), rtol=1e-12, atol=1e-12)
+    g = T.orc_apply(GRAD, fem, 1, xyl, xyz, dofs)[0]
+    assert np.allclose(g, grad(X), rtol=1e-7, atol=1e-7)
+
+
 def _mesh_system(n, fem, dirichlet):
     import anifem_amd as af
     coords, tets = af.kuhn_cube(n, 0.2, 11)
