@@ -28,7 +28,15 @@ namespace anifem_gpu {
 
 // Finite elements and operators (numbering of operators.h:25-45)
 enum FiniteElement { FEM_P0 = ANIFEM_FEM_P0, FEM_P1 = ANIFEM_FEM_P1, FEM_P2 = ANIFEM_FEM_P2, FEM_P3 = ANIFEM_FEM_P3 };
-enum OperatorType { IDEN = ANIFEM_IDEN, GRAD = ANIFEM_GRAD };
+enum OperatorType {
+    IDEN = ANIFEM_IDEN,
+    GRAD = ANIFEM_GRAD,
+    DIV = ANIFEM_DIV,
+    CURL = ANIFEM_CURL,
+    DUDX = ANIFEM_DUDX,
+    DUDY = ANIFEM_DUDY,
+    DUDZ = ANIFEM_DUDZ
+};
 
 // FemFix<OP> / FemVec<DIM, OP> (operators.h:57-67)
 template <int OP>
@@ -56,7 +64,9 @@ struct SpaceOf<FemVec<DIM, OP>> {
 // Operator<OPERATOR, FEMTYPE> (operators.h:50-51,127-155)
 template <int OPERATOR, typename FEMTYPE>
 struct Operator {
-    static_assert(OPERATOR == IDEN || OPERATOR == GRAD, "GPU operators: IDEN and GRAD");
+    static_assert(OPERATOR == IDEN || OPERATOR == GRAD || OPERATOR == DIV || OPERATOR == CURL || OPERATOR == DUDX ||
+                      OPERATOR == DUDY || OPERATOR == DUDZ,
+                  "operators of operators.h:37-45");
     static constexpr anifem_op c() {
         return anifem_op{OPERATOR, detail::SpaceOf<FEMTYPE>::fem, detail::SpaceOf<FEMTYPE>::ncomp};
     }

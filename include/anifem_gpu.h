@@ -30,11 +30,22 @@ extern "C" {
 
 /* ---- operators: Operator<OP, FemFix<FEM>> or Operator<OP, FemVec<ncomp, FEM>>
  * Numbering follows the reference enums (anifem++/fem/operators.h:25-45). */
-enum { ANIFEM_IDEN = 1, ANIFEM_GRAD = 2 };
+/* Operators (the reference's OperatorType, operators.h:37-45).  DIV and CURL
+ * act on 3-component vector spaces, DUDX/DUDY/DUDZ on any space (one partial
+ * derivative per component); all three are formed from the gradient. */
+enum {
+    ANIFEM_IDEN = 1,
+    ANIFEM_GRAD = 2,
+    ANIFEM_DIV = 3,
+    ANIFEM_CURL = 4,
+    ANIFEM_DUDX = 6,
+    ANIFEM_DUDY = 7,
+    ANIFEM_DUDZ = 8
+};
 enum { ANIFEM_FEM_P0 = 1, ANIFEM_FEM_P1 = 2, ANIFEM_FEM_P2 = 3, ANIFEM_FEM_P3 = 4 };
 
 typedef struct anifem_op {
-    int32_t op;    /* ANIFEM_IDEN or ANIFEM_GRAD */
+    int32_t op;    /* ANIFEM_IDEN, ANIFEM_GRAD, ANIFEM_DIV, ANIFEM_CURL, ANIFEM_DUDX..DUDZ */
     int32_t fem;   /* ANIFEM_FEM_P0 .. ANIFEM_FEM_P3 */
     int32_t ncomp; /* 1 for FemFix, >1 for FemVec<ncomp, fem> (operators.h:63-67,127-155) */
 } anifem_op;

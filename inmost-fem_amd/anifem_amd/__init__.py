@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libanifem_gpu.so")
 # ---- enums (include/anifem_gpu.h) -----------------------------------------
 OK, ERR_NONFINITE, ERR_CONNECTIVITY, ERR_ARG, ERR_ORDER, ERR_DIMS, ERR_HIP, ERR_NOMEM, ERR_NOT_CONVERGED = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
-IDEN, GRAD = 1, 2
+IDEN, GRAD, DIV, CURL, DUDX, DUDY, DUDZ = 1, 2, 3, 4, 6, 7, 8
 FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
 TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
 COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2

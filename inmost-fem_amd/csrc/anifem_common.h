@@ -12,7 +12,8 @@ namespace anifem {
 
 // Scalar finite-element space data (poly_0.h .. poly_3.h of the reference).
 int nf_scalar(int fem);          // 1, 4, 10, 20 for P0..P3; -1 otherwise
-int ref_dim(int op);             // 1 for IDEN, 3 for GRAD; -1 otherwise
+int ref_dim(int op);             // per component: 1 for IDEN, 3 for the gradient-based operators; -1 otherwise
+int op_dim(const anifem_op& op);  // the operator's dimension (Dim of operators.h)
 bool op_valid(const anifem_op& op);
 
 // Quadrature rule for `order` (generated table, tet_rules.inc); returns q or -1.

@@ -34,11 +34,33 @@ int nf_scalar(int fem) {
     return -1;
 }
 
-int ref_dim(int op) { return op == ANIFEM_IDEN ? 1 : (op == ANIFEM_GRAD ? 3 : -1); }
+int ref_dim(int op) {
+    switch (op) {
+        case ANIFEM_IDEN: return 1;
+        case ANIFEM_GRAD:
+        case ANIFEM_DIV:
+        case ANIFEM_CURL:
+        case ANIFEM_DUDX:
+        case ANIFEM_DUDY:
+        case ANIFEM_DUDZ: return 3;
+        default: return -1;
+    }
+}
+
+int op_dim(const anifem_op& op) {
+    switch (op.op) {
+        case ANIFEM_IDEN: return op.ncomp;
+        case ANIFEM_GRAD: return 3 * op.ncomp;
+        case ANIFEM_DIV: return 1;
+        case ANIFEM_CURL: return 3;
+        default: return op.ncomp;  // one partial derivative per component
+    }
+}
 
 bool op_valid(const anifem_op& op) {
     if (ref_dim(op.op) < 0 || nf_scalar(op.fem) < 0 || op.ncomp < 1 || op.ncomp > 3) return false;
-    if (op.op == ANIFEM_GRAD && op.fem == ANIFEM_FEM_P0) return false;
+    if (op.op != ANIFEM_IDEN && op.fem == ANIFEM_FEM_P0) return false;
+    if ((op.op == ANIFEM_DIV || op.op == ANIFEM_CURL) && op.ncomp != 3) return false;
     return true;
 }
 
@@ -299,7 +321,7 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
 extern "C" {
 
 int anifem_op_nfa(anifem_op op) { return anifem::op_valid(op) ? anifem::nf_scalar(op.fem) * op.ncomp : -1; }
-int anifem_op_dim(anifem_op op) { return anifem::op_valid(op) ? anifem::ref_dim(op.op) * op.ncomp : -1; }
+int anifem_op_dim(anifem_op op) { return anifem::op_valid(op) ? anifem::op_dim(op) : -1; }
 
 int anifem_quadrature_npoints(int order) {
     const double *p, *w;

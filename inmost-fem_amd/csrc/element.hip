@@ -23,8 +23,14 @@ static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem
     a.dot_normal = dot_normal ? 1 : 0;
     a.nfA0 = nf_scalar(opA.fem);
     a.nfB0 = nf_scalar(opB.fem);
-    a.dimA = ref_dim(opA.op) * opA.ncomp;
-    a.dimB = ref_dim(opB.op) * opB.ncomp * (dot_normal ? 3 : 1);
+    // DIV / CURL / DUDX..DUDZ run as the gradient with a signed selector in
+    // the tensor (element_kernels.h: op_map); the normal form takes them on
+    // the trial side only
+    if (dot_normal && opB.op != ANIFEM_IDEN && opB.op != ANIFEM_GRAD) return ANIFEM_ERR_ARG;
+    a.selA = opA.op;
+    a.selB = opB.op;
+    a.dimA = op_dim(opA);
+    a.dimB = op_dim(opB) * (dot_normal ? 3 : 1);
     a.E = a.nfA0 * opA.ncomp * a.nfB0 * opB.ncomp;
     a.ttype = D->type;
     a.storage = D->storage;
@@ -172,6 +178,7 @@ namespace anifem {
 // launch of apply_kernel (the table is freed after the kernel has run).
 int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s) {
     if (!op_valid(op) || !xyl || q < 1 || q > 96) return ANIFEM_ERR_ARG;
+    if (op.op != ANIFEM_IDEN && op.op != ANIFEM_GRAD) return ANIFEM_ERR_ARG;
     std::vector<double> tab;
     reference_table(op.op, op.fem, q, xyl, tab);
     a.grad = op.op == ANIFEM_GRAD ? 1 : 0;

@@ -35,6 +35,7 @@ struct ElemArgs {
     int ttype, storage;
     int dimA, dimB;    // full operator dims (ncomp * ref dim)
     int p0_bcast;      // IDEN P0 trial against a vector test: D column replicated
+    int selA, selB;    // operator codes: DIV / CURL / DUDX..DUDZ select from the gradient (op_map)
     int face;          // -1: volume integral; 0..3: integral over that face (int_face.inl)
     int dot_normal;    // face integral of (D OpA(u)) . n against OpB (fem3DfaceN): D has 3 dimB rows
     int tile;          // tets per workgroup
@@ -99,6 +100,39 @@ __device__ __forceinline__ double tensor_at(const ElemArgs& a, int64_t cr, int n
     return 0.0;
 }
 
+// Component of a gradient-based operator that gradient entry g = d + 3c
+// (d/dx_d of component c) feeds, with its sign, or -1 (operators.h:321-400):
+// DIV: div u = sum_c du_c/dx_c; CURL: (curl u)_i = eps_{idc} du_c/dx_d;
+// DUDX..DUDZ: du_c/dx_{0,1,2}; GRAD and IDEN: g itself.
+__device__ __forceinline__ int op_map(int op, int g, int& sign) {
+    sign = 1;
+    const int d = g % 3, c = g / 3;
+    switch (op) {
+        case ANIFEM_DIV: return d == c ? 0 : -1;
+        case ANIFEM_CURL: {
+            if (d == c) return -1;
+            const int i = 3 - d - c;
+            sign = (d == (i + 1) % 3) ? 1 : -1;
+            return i;
+        }
+        case ANIFEM_DUDX:
+        case ANIFEM_DUDY:
+        case ANIFEM_DUDZ: return d == op - ANIFEM_DUDX ? c : -1;
+        default: return g;
+    }
+}
+
+// Tensor entry between gradient entries (or values) k of the test side and j
+// of the trial side: the operators' selectors applied to D.
+__device__ __forceinline__ double tensor_sel(const ElemArgs& a, int64_t cr, int n, int k, int j) {
+    if (a.selA <= ANIFEM_GRAD && a.selB <= ANIFEM_GRAD) return tensor_at(a, cr, n, k, j);
+    int sA, sB;
+    const int mA = op_map(a.selA, j, sA), mB = op_map(a.selB, k, sB);
+    if (mA < 0 || mB < 0) return 0.0;
+    const double v = tensor_at(a, cr, n, mB, mA);
+    return sA * sB > 0 ? v : -v;
+}
+
 // Area of face f of a tet (vertices f, f+1, f+2 mod 4) and, when nrm is given,
 // its unit normal pointing away from the fourth vertex (geometry.h:67-93).
 __device__ __forceinline__ double face_measure(const double P[4][3], int f, double* nrm) {
@@ -132,10 +166,10 @@ __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, co
             if (nrm) {
                 const int k = k0 + DB0 * cb;
                 double v = 0;
-                for (int d = 0; d < 3; ++d) v += tensor_at(a, cr, n, d + 3 * k, j0 + DA0 * ca) * nrm[d];
+                for (int d = 0; d < 3; ++d) v += tensor_sel(a, cr, n, d + 3 * k, j0 + DA0 * ca) * nrm[d];
                 Dblk[k0][j0] = v;
             } else {
-                Dblk[k0][j0] = tensor_at(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
+                Dblk[k0][j0] = tensor_sel(a, cr, n, k0 + DB0 * cb, j0 + DA0 * ca);
             }
         }
     double M[DB0][DA0];  // (D * TA)

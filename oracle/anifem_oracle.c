@@ -270,7 +270,24 @@ int orc_nfa(int fem, int ncomp) {
     return nf < 0 ? nf : nf * ncomp;
 }
 
-int orc_dim(int op, int ncomp) { return (op == ORC_GRAD ? 3 : 1) * ncomp; }
+int orc_dim(int op, int ncomp) {
+    switch (op) {
+        case ORC_IDEN: return ncomp;
+        case ORC_GRAD: return 3 * ncomp;
+        case ORC_DIV: return 1;
+        case ORC_CURL: return 3;
+        default: return ncomp;
+    }
+}
+
+/* operator code valid on (fem, ncomp): operators.h:37-45, :321-400 */
+static int orc_op_ok(int op, int fem, int ncomp) {
+    if (op != ORC_IDEN && op != ORC_GRAD && op != ORC_DIV && op != ORC_CURL && (op < ORC_DUDX || op > ORC_DUDZ))
+        return 0;
+    if (op != ORC_IDEN && fem == ORC_P0) return 0;
+    if ((op == ORC_DIV || op == ORC_CURL) && ncomp != 3) return 0;
+    return 1;
+}
 
 static const unsigned char ORC_EI[6] = {0, 0, 0, 1, 1, 2}, ORC_EJ[6] = {1, 2, 3, 2, 3, 3};
 static const unsigned char ORC_IPF[16] = {0, 1, 2, 3, 1, 2, 3, 0, 0, 2, 3, 1, 0, 1, 3, 2};
@@ -345,8 +362,8 @@ static void orc_refgrad(int fem, const double* L, double* g) {
 
 /* Full (FemVec-expanded) operator table U[k + dim*(n + q*i)] for one tet. */
 static void orc_apply_op(int op, int fem, int ncomp, int q, const double* XYL, const double* PSI, double* U) {
-    int nf0 = orc_nf_scalar(fem), dim0 = (op == ORC_GRAD ? 3 : 1);
-    int dim = dim0 * ncomp, nf = nf0 * ncomp;
+    int nf0 = orc_nf_scalar(fem), dim0 = (op == ORC_IDEN ? 1 : 3);
+    int dim = orc_dim(op, ncomp), nf = nf0 * ncomp;
     memset(U, 0, sizeof(double) * (size_t)dim * q * nf);
     double tab[3 * 20];
     for (int n = 0; n < q; ++n) {
@@ -372,9 +389,25 @@ static void orc_apply_op(int op, int fem, int ncomp, int q, const double* XYL, c
             }
         }
         for (int c = 0; c < ncomp; ++c)
-            for (int i = 0; i < nf0; ++i)
-                for (int k = 0; k < dim0; ++k)
-                    U[(k + dim0 * c) + dim * (n + q * (i + nf0 * c))] = tab[k + dim0 * i];
+            for (int i = 0; i < nf0; ++i) {
+                double* u = U + dim * (n + q * (i + nf0 * c));
+                const double* t = tab + dim0 * i;
+                switch (op) {
+                    case ORC_IDEN: u[c] = t[0]; break;
+                    case ORC_GRAD:
+                        for (int k = 0; k < 3; ++k) u[k + 3 * c] = t[k];
+                        break;
+                    case ORC_DIV: u[0] = t[c]; break; /* operators.h:321-349 */
+                    case ORC_CURL:                    /* operators.h:356-400: (curl u)_i = eps_idc du_c/dx_d */
+                        for (int d = 0; d < 3; ++d)
+                            if (d != c) {
+                                int ic = 3 - d - c;
+                                u[ic] = (d == (ic + 1) % 3) ? t[d] : -t[d];
+                            }
+                        break;
+                    default: u[c] = t[op - ORC_DUDX]; /* operators.h:286-316 */
+                }
+            }
     }
 }
 
@@ -384,9 +417,7 @@ static void orc_apply_op(int op, int fem, int ncomp, int q, const double* XYL, c
 int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int tensor_type, int coef_mode,
                  int order, long n, const double* xyz, const double* D, double* A) {
     int nfA = orc_nfa(femA, ncA), nfB = orc_nfa(femB, ncB);
-    if (nfA <= 0 || nfB <= 0 || (opA != ORC_IDEN && opA != ORC_GRAD) || (opB != ORC_IDEN && opB != ORC_GRAD))
-        return -1;
-    if ((opA == ORC_GRAD && femA == ORC_P0) || (opB == ORC_GRAD && femB == ORC_P0)) return -1;
+    if (nfA <= 0 || nfB <= 0 || !orc_op_ok(opA, femA, ncA) || !orc_op_ok(opB, femB, ncB)) return -1;
     double XYL[4 * 24], WG[24];
     int q = orc_quadrature(order, XYL, WG);
     if (q <= 0) return -2;
@@ -529,7 +560,7 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
                   int dot_normal, long n, const double* xyz, const double* D, double* A) {
     int nfA = orc_nfa(femA, ncA), nfB = orc_nfa(femB, ncB);
     if (nfA <= 0 || nfB <= 0 || face < 0 || face > 3) return -1;
-    if ((opA == ORC_GRAD && femA == ORC_P0) || (opB == ORC_GRAD && femB == ORC_P0)) return -1;
+    if (!orc_op_ok(opA, femA, ncA) || !orc_op_ok(opB, femB, ncB)) return -1;
     double XYL[4 * 36], WG[36];
     int q = orc_face_quadrature(face, XYL, WG);
     int dimA = orc_dim(opA, ncA), dimB = orc_dim(opB, ncB);
@@ -605,7 +636,7 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
 int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n, const double* xyz,
                    const double* dofs, double* out) {
     int nf = orc_nfa(fem, ncomp), dim = orc_dim(op, ncomp);
-    if (nf <= 0 || q <= 0 || (op == ORC_GRAD && fem == ORC_P0)) return -1;
+    if (nf <= 0 || q <= 0 || !orc_op_ok(op, fem, ncomp)) return -1;
     double* U = malloc(sizeof(double) * (size_t)dim * q * nf);
     for (long r = 0; r < n; ++r) {
         const double* P = xyz + 12 * r;

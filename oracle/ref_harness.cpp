@@ -233,7 +233,9 @@ int ref_quadrature(int order, double* p, double* w) {
     }
 }
 
-// Pair codes (OpA, OpB): 0=(GRAD P1, GRAD P1); 1=(GRAD P2, GRAD P2); 2=(GRAD P3, GRAD P3); 3=(GRAD V1, GRAD V1); 4=(GRAD V2, GRAD V2); 5=(IDEN P0, IDEN P1); 6=(IDEN P0, IDEN P2); 7=(IDEN P0, IDEN P3); 8=(IDEN P0, IDEN V2); 9=(IDEN P1, IDEN P1); 10=(IDEN P2, IDEN P2); 11=(IDEN P3, IDEN P3); 12=(GRAD P3, GRAD V1); 13=(IDEN P0, IDEN V1); 14=(GRAD P2, IDEN P2).
+// Pair codes (OpA, OpB): 0=(GRAD P1, GRAD P1); 1=(GRAD P2, GRAD P2); 2=(GRAD P3, GRAD P3); 3=(GRAD V1, GRAD V1); 4=(GRAD V2, GRAD V2); 5=(IDEN P0, IDEN P1); 6=(IDEN P0, IDEN P2); 7=(IDEN P0, IDEN P3); 8=(IDEN P0, IDEN V2); 9=(IDEN P1, IDEN P1); 10=(IDEN P2, IDEN P2); 11=(IDEN P3, IDEN P3); 12=(GRAD P3, GRAD V1); 13=(IDEN P0, IDEN V1); 14=(GRAD P2, IDEN P2);
+// 15=(DIV V2, DIV V2); 16=(DIV V2, IDEN P1); 17=(CURL V1, CURL V1); 18=(CURL V2, IDEN V2);
+// 19=(DUDX P2, DUDY P2); 20=(DUDZ P1, IDEN P1).
 //  xyz is n x 12 (P0,P1,P2,P3 per tet), D is n x ddim.
 // A receives n element matrices, each nfB x nfA column-major.
 int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, const double* xyz, const double* D,
@@ -261,6 +263,12 @@ int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, co
             case 12: return dispatch_tensor<Operator<GRAD, P3>, Operator<GRAD, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 13: return dispatch_tensor<Operator<IDEN, P0>, Operator<IDEN, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 14: return dispatch_tensor<Operator<GRAD, P2>, Operator<IDEN, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 15: return dispatch_tensor<Operator<DIV, V2>, Operator<DIV, V2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 16: return dispatch_tensor<Operator<DIV, V2>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 17: return dispatch_tensor<Operator<CURL, V1>, Operator<CURL, V1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 18: return dispatch_tensor<Operator<CURL, V2>, Operator<IDEN, V2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 19: return dispatch_tensor<Operator<DUDX, P2>, Operator<DUDY, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 20: return dispatch_tensor<Operator<DUDZ, P1>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             default: break;
         }
     } catch (const std::exception&) {

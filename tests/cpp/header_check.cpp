@@ -27,6 +27,10 @@ int main() {
     CHECK(nfa<GP2>() == 10 && dim<GP2>() == 3);
     CHECK(nfa<IP3>() == 20 && dim<IP3>() == 1);
     CHECK(nfa<GV2>() == 30 && dim<GV2>() == 9);
+    // DIV / CURL of FemVec<3, P2>, DUDX of P2 (operators.h:286-400)
+    CHECK((nfa<Operator<DIV, FemVec<3, FEM_P2>>>() == 30 && dim<Operator<DIV, FemVec<3, FEM_P2>>>() == 1));
+    CHECK((dim<Operator<CURL, FemVec<3, FEM_P2>>>() == 3 && dim<Operator<DUDX, FemFix<FEM_P2>>>() == 1));
+    CHECK((dim<Operator<DIV, FemFix<FEM_P2>>>() < 0));
     // traits against views
     const TensorView sym_tet{TENSOR_SYMMETRIC, ANIFEM_COEF_PER_TET, nullptr};
     CHECK((check_view<DfuncTraits<TENSOR_SYMMETRIC, false>>(sym_tet) == ANIFEM_OK));

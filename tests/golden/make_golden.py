@@ -26,6 +26,8 @@ PAIRS = {
     4: (2, 3, 3, 2, 3, 3), 5: (1, 1, 1, 1, 2, 1), 6: (1, 1, 1, 1, 3, 1), 7: (1, 1, 1, 1, 4, 1),
     8: (1, 1, 1, 1, 3, 3), 9: (1, 2, 1, 1, 2, 1), 10: (1, 3, 1, 1, 3, 1), 11: (1, 4, 1, 1, 4, 1),
     12: (2, 4, 1, 2, 2, 3), 13: (1, 1, 1, 1, 2, 3), 14: (2, 3, 1, 1, 3, 1),
+    15: (3, 3, 3, 3, 3, 3), 16: (3, 3, 3, 1, 2, 1), 17: (4, 2, 3, 4, 2, 3), 18: (4, 3, 3, 1, 3, 3),
+    19: (6, 3, 1, 7, 3, 1), 20: (8, 2, 1, 1, 2, 1),
 }
 NF = {1: 1, 2: 4, 3: 10, 4: 20}
 
@@ -110,6 +112,20 @@ def cases(rng):
     out.append(("grad_p2_iden_p2_general", 14, 4, 0, 2, xyz, rng.normal(size=(n, 3))))
     out.append(("vec_p1_general", 3, 4, 0, 2, xyz[:32], rng.normal(size=(32, 81))))
     return out
+
+
+def op_cases(rng):
+    """DIV / CURL / DUDX..DUDZ operators (operators.h:286-400)."""
+    n = 32
+    xyz = kuhn_tets(rng, n)
+    return [
+        ("div_v2_div_v2_scalar", 15, 2, 0, 2, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))),
+        ("div_v2_iden_p1_null", 16, 1, 0, 2, xyz, np.zeros((n, 1))),
+        ("curl_v1_curl_v1_scalar", 17, 2, 0, 2, xyz, 10.0 ** rng.uniform(-1, 1, (n, 1))),
+        ("curl_v2_iden_v2_general", 18, 4, 0, 3, xyz, rng.normal(size=(n, 9))),
+        ("dudx_p2_dudy_p2_scalar", 19, 2, 0, 2, xyz, rng.uniform(0.5, 2, (n, 1))),
+        ("dudz_p1_iden_p1_null", 20, 1, 0, 2, xyz, np.zeros((n, 1))),
+    ]
 
 
 # face pair codes (ref_fem3dface) -> (opA, femA, ncA, opB, femB, ncB, dot_normal)
@@ -201,7 +217,7 @@ def main():
     L.ref_fem3dtet.restype = ctypes.c_int
     dp = ctypes.POINTER(ctypes.c_double)
     rng = np.random.default_rng(20260101)
-    for case in cases(rng):
+    for case in cases(rng) + op_cases(np.random.default_rng(20260404)):
         name, pair, tt, pp, order, xyz, D = case[:7]
         extra = case[7] if len(case) > 7 else None
         oa, fa, ca, ob, fb, cb = PAIRS[pair]

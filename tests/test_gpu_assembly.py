@@ -140,6 +140,31 @@ def test_elasticity_p2vec_lame_assembly():
     compare(val.cpu().numpy(), rhs.cpu().numpy(), S, v_or, r_or)
 
 
+@pytest.mark.parametrize("op,fem,nc,tt", [(af.DIV, af.FEM_P2, 3, af.TENSOR_SCALAR), (af.CURL, af.FEM_P1, 3, af.TENSOR_NULL),
+                                          (af.DUDX, af.FEM_P2, 1, af.TENSOR_SCALAR)])
+def test_derivative_operator_assembly_vs_oracle(op, fem, nc, tt):
+    """grad-div, curl-curl and d/dx-d/dx terms (operators.h:286-400) through
+    the fused assembly (persistent passes for scalar and vector spaces)
+    against the oracle's element matrices, applyDir and sequential scatter."""
+    rng = np.random.default_rng(op)
+    coords, tets = af.kuhn_cube(4, 0.2, 6)
+    n = len(tets)
+    Asm = af.Assembler(coords, tets, af.Op(op, fem, nc), order=2, tensor_type=tt, tensor_storage=af.COEF_PER_TET,
+                       dirichlet=af.DIRICHLET_BOUNDARY)
+    kap = rng.uniform(0.5, 2.0, n)
+    val, _, st = Asm.assemble(D=dev(kap) if tt != af.TENSOR_NULL else None)
+    assert st.code == 0
+    setup = Asm.setup()
+    S = T.oracle_assembly(coords, tets, fem, nc, dirichlet_mask=setup["dirichlet"])
+    xyz = coords[S["tets"]].reshape(n, 12)
+    A = T.orc_elem(op, fem, nc, op, fem, nc, tt, 1, 2, xyz, kap[:, None] if tt != af.TENSOR_NULL else None)
+    nf = T.nfa(fem, nc)
+    A, _ = T.apply_dir_elementwise(A, np.zeros((n, nf)), S["dofs"], setup["dirichlet"])
+    rc, v_or, _ = T.oracle_scatter(n, nf, S["dofs"], S["slot"], A, None, len(S["col_idx"]), S["ndof"])
+    assert rc == 0
+    compare(val.cpu().numpy(), None, S, v_or, None)
+
+
 def test_nonfinite_status():
     coords, tets = af.kuhn_cube(3, 0.2, 1)
     n = len(tets)
