@@ -224,7 +224,7 @@ def bench_elasticity(args, torch):
     def mfma_pass():
         af.fem3dtet_batch(op, op, af.TENSOR_LAME, af.COEF_PER_TET, D, 2, xyzr, af.PREC_FP16X3, out=Ae)
         af.fem3dtet_batch(p0, pv, af.TENSOR_GENERAL, af.COEF_CONST, g, 2, xyzr, af.PREC_FP16X3, out=Fe)
-        A.assemble_elements(Ae, Fe.view(n, 30), val=val16, rhs=rhs16, status=False)
+        A.assemble_elements(Ae, Fe.view(n, 30), val=val16, rhs=rhs16, status=False, symmetric=True)
 
     ms16 = timed_events(mfma_pass, 5, 1, torch)
     res["fp16x3_mfma_elements_then_scatter"] = {

@@ -301,6 +301,10 @@ int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int3
  * the same deterministic order as anifem_assembly_run. */
 int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const void* F_elem, int elem_precision,
                                  double* csr_val, double* rhs, anifem_status* status, void* stream);
+/* Flag for elem_precision: the element matrices are symmetric (the caller's
+ * promise); each row is read as the matching column, contiguous in the
+ * column-major layout, i.e. the scatter uses A(ia, ib) for A(ib, ia). */
+enum { ANIFEM_ELEM_SYMMETRIC = 0x100 };
 /* AssembleRHS (anifem++/inmost_interface/assembler.inl:497-561): the global
  * right-hand side from per-cell load vectors F_elem (n_tets x nf, FP64 or FP32,
  * local DOF order after the node reorder), with element-level homogeneous

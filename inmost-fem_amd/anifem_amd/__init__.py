@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libanifem_gpu.so")
 OK, ERR_NONFINITE, ERR_CONNECTIVITY, ERR_ARG, ERR_ORDER, ERR_DIMS, ERR_HIP, ERR_NOMEM, ERR_NOT_CONVERGED = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
 IDEN, GRAD, DIV, CURL, DUDX, DUDY, DUDZ = 1, 2, 3, 4, 6, 7, 8
+ELEM_SYMMETRIC = 0x100
 FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
 TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
 COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2
@@ -479,7 +480,7 @@ class Assembler:
         _check(lib().anifem_assembly_tets(self._h, _ptr(out)), "anifem_assembly_tets")
         return out
 
-    def assemble_elements(self, A_elem, F_elem=None, val=None, rhs=None, status=True, stream=None):
+    def assemble_elements(self, A_elem, F_elem=None, val=None, rhs=None, status=True, stream=None, symmetric=False):
         """Scatter caller-supplied element matrices (n_tets, nf, nf) in the
         column-major layout of fem3dtet_batch (float64 or float32) and load
         vectors (n_tets, nf) into the FP64 CSR (anifem_assembly_run_elements)."""
@@ -493,6 +494,8 @@ class Assembler:
         if A_elem.dtype not in (torch.float64, torch.float32) or (F_elem is not None and F_elem.dtype != A_elem.dtype):
             raise AnifemError(ERR_ARG, "element arrays must be float64 or float32, both of one type")
         st = anifem_status()
+        if symmetric:
+            prec |= ELEM_SYMMETRIC
         rc = lib().anifem_assembly_run_elements(self._h, _ptr(A_elem), _ptr(F_elem), prec, _ptr(val), _ptr(rhs),
                                                 ctypes.byref(st) if status else None, _stream_handle(stream))
         if rc not in (OK, ERR_NONFINITE):

@@ -787,6 +787,8 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
                                  double* csr_val, double* rhs, anifem_status* status, void* stream) {
     using namespace anifem;
     if (!ctx || ctx->host_only || !A_elem || !csr_val) return ANIFEM_ERR_ARG;
+    const bool sym = (elem_precision & ANIFEM_ELEM_SYMMETRIC) != 0;
+    elem_precision &= ~ANIFEM_ELEM_SYMMETRIC;
     if (elem_precision != ANIFEM_PREC_FP64 && elem_precision != ANIFEM_PREC_FP32) return ANIFEM_ERR_ARG;
     const AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -819,12 +821,18 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
     if (S.n_blocks > 0) {
         const size_t lds = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0).total;
         const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
-        if (elem_precision == ANIFEM_PREC_FP64)
-            hipLaunchKernelGGL(row_block_elements_kernel<double>, dim3(grid), dim3(kBlock), lds, s, a,
-                               static_cast<const double*>(A_elem), static_cast<const double*>(F_elem));
+        const double* A64 = static_cast<const double*>(A_elem);
+        const double* F64 = static_cast<const double*>(F_elem);
+        const float* A32 = static_cast<const float*>(A_elem);
+        const float* F32 = static_cast<const float*>(F_elem);
+        if (elem_precision == ANIFEM_PREC_FP64 && sym)
+            hipLaunchKernelGGL((row_block_elements_kernel<double, true>), dim3(grid), dim3(kBlock), lds, s, a, A64, F64);
+        else if (elem_precision == ANIFEM_PREC_FP64)
+            hipLaunchKernelGGL((row_block_elements_kernel<double, false>), dim3(grid), dim3(kBlock), lds, s, a, A64, F64);
+        else if (sym)
+            hipLaunchKernelGGL((row_block_elements_kernel<float, true>), dim3(grid), dim3(kBlock), lds, s, a, A32, F32);
         else
-            hipLaunchKernelGGL(row_block_elements_kernel<float>, dim3(grid), dim3(kBlock), lds, s, a,
-                               static_cast<const float*>(A_elem), static_cast<const float*>(F_elem));
+            hipLaunchKernelGGL((row_block_elements_kernel<float, false>), dim3(grid), dim3(kBlock), lds, s, a, A32, F32);
         ANIFEM_HIP_CHECK(hipGetLastError());
     }
     if (status) {

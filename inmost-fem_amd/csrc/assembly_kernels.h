@@ -363,7 +363,9 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
 // vectors of nf (or null), in each tet's local DOF order after the node
 // reorder.  Phase A reads the pair's row of its tet's matrix instead of forming
 // it; Dirichlet rows and phase B are those of row_block_kernel.
-template <typename TE>
+// SYM: the caller's matrices are symmetric; the pair's row is read as the
+// matching column, which is contiguous in the column-major layout.
+template <typename TE, bool SYM>
 __global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a, const TE* __restrict__ Ae,
                                                                     const TE* __restrict__ Fe) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -386,13 +388,28 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a,
         const int ibf = a.pair_ib[gp];
         const int ib = ibf & 63;
         const bool dir = (ibf & kPairDirichlet) != 0;
-        const TE* At = Ae + tet * nf * nf + ib;
-        for (int ia = 0; ia < nf; ++ia) {
-            const int ps = a.pair_pos[qp + ia];
-            if (ps == kPosDropped) continue;
-            const double v = dir ? 1.0 : (double)At[(int64_t)nf * ia];
-            nonfinite += isfinite(v) ? 0 : 1;
-            vals[ps] = v;
+        const TE* At = Ae + tet * nf * nf + (SYM ? (int64_t)nf * ib : ib);
+        const int64_t st = SYM ? 1 : nf;
+        // eight entries at a time: their positions and values are requested
+        // together (every value is inside the tet's matrix, dropped or not),
+        // then the kept ones are stored
+        constexpr int U = 8;
+        for (int ia0 = 0; ia0 < nf; ia0 += U) {
+            int ps[U];
+            TE v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int ia = ia0 + j < nf ? ia0 + j : nf - 1;
+                ps[j] = ia0 + j < nf ? (int)a.pair_pos[qp + ia] : kPosDropped;
+                v[j] = At[st * ia];
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (ps[j] == kPosDropped) continue;
+                const double x = dir ? 1.0 : (double)v[j];
+                nonfinite += isfinite(x) ? 0 : 1;
+                vals[ps[j]] = x;
+            }
         }
         if (a.has_rhs) {
             const double fv = dir ? 0.0 : (double)Fe[tet * nf + ib];

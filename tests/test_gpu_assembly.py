@@ -313,6 +313,28 @@ def test_assemble_elements_matches_oracle_scatter(fem, nc):
     assert st.code == af.ERR_NONFINITE and st.n_nonfinite >= 1
 
 
+@pytest.mark.parametrize("fem,nc", [(af.FEM_P2, 1), (af.FEM_P2, 3)])
+def test_assemble_elements_symmetric_reads_columns(fem, nc):
+    """ANIFEM_ELEM_SYMMETRIC: exactly symmetric element matrices give the same
+    CSR values bit for bit when the rows are read as columns; for a
+    non-symmetric input the flag assembles the transposes."""
+    import torch
+    rng = np.random.default_rng(nc)
+    coords, tets = af.kuhn_cube(3, 0.2, 9)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY)
+    n, nf = Asm.n_tets, Asm.nf
+    B = rng.normal(size=(n, nf, nf))
+    for dt in (torch.float64, torch.float32):
+        S = torch.from_numpy(B + B.transpose(0, 2, 1)).to("cuda", dt)
+        v1, _, _ = Asm.assemble_elements(S)
+        v2, _, _ = Asm.assemble_elements(S, symmetric=True)
+        assert torch.equal(v1, v2)
+        N = torch.from_numpy(B).to("cuda", dt)
+        v3, _, _ = Asm.assemble_elements(N, symmetric=True)
+        v4, _, _ = Asm.assemble_elements(N.transpose(1, 2).contiguous())
+        assert torch.equal(v3, v4)
+
+
 def test_assemble_elements_from_batch_matches_fused():
     """Element matrices from fem3dtet_batch on the reordered tets, scattered by
     assemble_elements, against the fused assembly: FP64 elements agree to
