@@ -783,6 +783,51 @@ int anifem_assembly_tets(const anifem_assembly* ctx, int32_t* tets) {
     return ANIFEM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Persistent element scatter for the common element sizes (occupancy x CUs
+// workgroups, whole XCD groups), else one workgroup per block.
+template <typename TE, int NF, bool SYM>
+int launch_elements_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const TE* A, const TE* F, size_t lds,
+                               hipStream_t s) {
+    using namespace anifem;
+    int occ = 0, n_cu = 0;
+    ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ, reinterpret_cast<const void*>(&row_block_elements_persistent<TE, NF, SYM>), kBlock, lds));
+    ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
+    G = std::min<int64_t>(G, ctx->S.n_blocks);
+    G = std::max<int64_t>(8, G / 8 * 8);
+    hipLaunchKernelGGL((row_block_elements_persistent<TE, NF, SYM>), dim3((unsigned)G), dim3(kBlock), lds, s, a, A, F);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
+template <typename TE, bool SYM>
+int launch_elements(anifem_assembly* ctx, const anifem::BlockArgs& a, const TE* A, const TE* F, size_t lds,
+                    unsigned grid, hipStream_t s) {
+    using namespace anifem;
+    if (ctx->persistent) {
+        switch (ctx->S.nf) {
+            case 4: return launch_elements_persistent<TE, 4, SYM>(ctx, a, A, F, lds, s);
+            case 10: return launch_elements_persistent<TE, 10, SYM>(ctx, a, A, F, lds, s);
+            case 12: return launch_elements_persistent<TE, 12, SYM>(ctx, a, A, F, lds, s);
+            case 20: return launch_elements_persistent<TE, 20, SYM>(ctx, a, A, F, lds, s);
+            case 30: return launch_elements_persistent<TE, 30, SYM>(ctx, a, A, F, lds, s);
+            default: break;
+        }
+    }
+    hipLaunchKernelGGL((row_block_elements_kernel<TE, SYM>), dim3(grid), dim3(kBlock), lds, s, a, A, F);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const void* F_elem, int elem_precision,
                                  double* csr_val, double* rhs, anifem_status* status, void* stream) {
     using namespace anifem;
@@ -825,15 +870,11 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
         const double* F64 = static_cast<const double*>(F_elem);
         const float* A32 = static_cast<const float*>(A_elem);
         const float* F32 = static_cast<const float*>(F_elem);
-        if (elem_precision == ANIFEM_PREC_FP64 && sym)
-            hipLaunchKernelGGL((row_block_elements_kernel<double, true>), dim3(grid), dim3(kBlock), lds, s, a, A64, F64);
-        else if (elem_precision == ANIFEM_PREC_FP64)
-            hipLaunchKernelGGL((row_block_elements_kernel<double, false>), dim3(grid), dim3(kBlock), lds, s, a, A64, F64);
-        else if (sym)
-            hipLaunchKernelGGL((row_block_elements_kernel<float, true>), dim3(grid), dim3(kBlock), lds, s, a, A32, F32);
-        else
-            hipLaunchKernelGGL((row_block_elements_kernel<float, false>), dim3(grid), dim3(kBlock), lds, s, a, A32, F32);
-        ANIFEM_HIP_CHECK(hipGetLastError());
+        if (elem_precision == ANIFEM_PREC_FP64 && sym) rc = launch_elements<double, true>(ctx, a, A64, F64, lds, grid, s);
+        else if (elem_precision == ANIFEM_PREC_FP64) rc = launch_elements<double, false>(ctx, a, A64, F64, lds, grid, s);
+        else if (sym) rc = launch_elements<float, true>(ctx, a, A32, F32, lds, grid, s);
+        else rc = launch_elements<float, false>(ctx, a, A32, F32, lds, grid, s);
+        if (rc) return rc;
     }
     if (status) {
         int h[4] = {0, 0, 0, 0};

@@ -887,4 +887,154 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
+// Persistent pass over caller-supplied element matrices (NF entries per
+// pair): the walk of row_block_persistent_vec with the next block's pair
+// records in flight during phase A and its matrix rows, load-vector entries,
+// slot and row starts requested after it.  Phase A only places values, so
+// the results are bitwise those of row_block_elements_kernel.
+template <typename TE, int NF, bool SYM>
+__global__ __launch_bounds__(kBlock) void row_block_elements_persistent(BlockArgs a, const TE* __restrict__ Ae,
+                                                                        const TE* __restrict__ Fe) {
+    constexpr int PW = NF / 2;
+    constexpr int SW = 9;
+    static_assert(NF % 2 == 0, "positions are read as 32-bit words");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0);
+    double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
+    double* fvals = reinterpret_cast<double*>(smem_raw + L.fvals);
+    uint16_t* sst = reinterpret_cast<uint16_t*>(smem_raw + L.sst);
+    int* pstl = reinterpret_cast<int*>(smem_raw + L.pst);
+    const bool sw_regs = a.max_slots <= 2 * SW * kBlock - 2;
+
+    BlockWalk W;
+    W.init(a.n_blocks);
+    int64_t b = W.take();
+    if (b < 0) return;  // uniform over the workgroup
+
+    struct Regs {
+        int32_t tet;
+        int ibf;
+        uint32_t posw[PW];
+        TE v[NF];
+        TE fv;
+        uint32_t sw[SW];
+        int pst;
+    };
+    auto load_records = [&](const BlockMeta& Mx, Regs& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
+        P.tet = hp ? a.pair_tet[gp] : 0;
+        P.ibf = hp ? a.pair_ib[gp] : 0;
+        const uint32_t* pw =
+            reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * NF);
+#pragma unroll
+        for (int w = 0; w < PW; ++w) P.posw[w] = hp ? pw[w] : 0u;
+    };
+    auto load_rows = [&](const BlockMeta& Mx, Regs& P) {
+        const bool hp = (int)threadIdx.x < Mx.np;
+        const int ib = P.ibf & 63;
+        const TE* At = Ae + (int64_t)P.tet * NF * NF + (SYM ? (int64_t)NF * ib : ib);
+#pragma unroll
+        for (int e = 0; e < NF; ++e) P.v[e] = hp ? At[(SYM ? 1 : NF) * e] : TE(0);
+        P.fv = (hp && a.has_rhs) ? Fe[(int64_t)P.tet * NF + ib] : TE(0);
+        if (sw_regs) {
+            const uint32_t* sg = reinterpret_cast<const uint32_t*>(a.sstart) + (Mx.s0 >> 1);
+            const int nw = slot_words(Mx.s0, (int)Mx.ns);
+#pragma unroll
+            for (int w = 0; w < SW; ++w) {
+                const int i = w * kBlock + (int)threadIdx.x;
+                P.sw[w] = i < nw ? sg[i] : 0u;
+            }
+        }
+        P.pst = (a.has_rhs && (int)threadIdx.x < Mx.nr) ? (int)a.pstart[Mx.r0 + threadIdx.x] : 0;
+    };
+
+    BlockMeta M = a.meta[b];
+    Regs cur, nxt;
+    load_records(M, cur);
+    load_rows(M, cur);
+    int64_t bn = W.take();
+    BlockMeta Mn{};
+    if (bn >= 0) Mn = a.meta[bn];
+
+    int nonfinite = 0;
+    for (;;) {
+        const uint16_t* sstb = sst;
+        if (sw_regs) {
+            uint32_t* s32 = reinterpret_cast<uint32_t*>(sst);
+            const int nw = slot_words(M.s0, (int)M.ns);
+#pragma unroll
+            for (int w = 0; w < SW; ++w) {
+                const int i = w * kBlock + (int)threadIdx.x;
+                if (i < nw) s32[i] = cur.sw[w];
+            }
+            sstb = sst + (M.s0 & 1);
+        } else {
+            block_copy(sst, a.sstart + M.s0, (int)M.ns);
+        }
+        if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
+        load_records(Mn, nxt);  // past the last block: Mn has no pairs, nothing is read
+        const int64_t bnn = W.take();
+        BlockMeta Mnn{};
+        if (bnn >= 0) Mnn = a.meta[bnn];
+
+        // phase A of b: the pair's row placed at its listed positions
+        if ((int)threadIdx.x < M.np) {
+            const bool dir = (cur.ibf & kPairDirichlet) != 0;
+#pragma unroll
+            for (int e = 0; e < NF; ++e) {
+                const uint32_t w = cur.posw[e >> 1];
+                const int ps = (int)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+                if (ps != kPosDropped) {
+                    const double v = dir ? 1.0 : (double)cur.v[e];
+                    nonfinite += isfinite(v) ? 0 : 1;
+                    vals[ps] = v;
+                }
+            }
+            if (a.has_rhs) {
+                const double fv = dir ? 0.0 : (double)cur.fv;
+                nonfinite += isfinite(fv) ? 0 : 1;
+                fvals[threadIdx.x] = fv;
+            }
+        }
+        load_rows(Mn, nxt);
+        __syncthreads();
+
+        // phase B of b
+        const int ns = M.ns, nv = M.nv, nr = M.nr, np = M.np;
+        for (int u = threadIdx.x; u < ns; u += kBlock) {
+            const int cb = sstb[u];
+            const int ce = u + 1 < ns ? sstb[u + 1] : nv;
+            double sum = 0.0;
+            int c = cb;
+            for (; c + 4 <= ce; c += 4) {
+                const double v0 = vals[c], v1 = vals[c + 1], v2 = vals[c + 2], v3 = vals[c + 3];
+                sum += v0;
+                sum += v1;
+                sum += v2;
+                sum += v3;
+            }
+            for (; c < ce; ++c) sum += vals[c];
+            a.val[M.s0 + u] = sum;
+        }
+        if (a.has_rhs) {
+            for (int u = threadIdx.x; u < nr; u += kBlock) {
+                const int pb = pstl[u];
+                const int pe = u + 1 < nr ? pstl[u + 1] : np;
+                double sum = 0.0;
+                for (int p = pb; p < pe; ++p) sum += fvals[p];
+                a.rhs[M.r0 + u] = sum;
+            }
+        }
+        __syncthreads();
+        if (bn < 0) break;
+        cur = nxt;
+        M = Mn;
+        Mn = Mnn;
+        b = bn;
+        bn = bnn;
+    }
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
 }  // namespace anifem
