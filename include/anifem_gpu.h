@@ -43,10 +43,29 @@ enum {
     ANIFEM_DUDZ = 8
 };
 enum { ANIFEM_FEM_P0 = 1, ANIFEM_FEM_P1 = 2, ANIFEM_FEM_P2 = 3, ANIFEM_FEM_P3 = 4 };
+/* The other FemFix spaces of the reference (operators.h:26-35; spaces/*.h),
+ * same codes.  They are accepted by the element-level entry points
+ * (anifem_fem3dtet_batch in FP64 / FP32, anifem_fem3dface_batch;
+ * anifem_fem3dapply_batch for CR1, B2, B4), not by the global assembly:
+ *   CR1 Crouzeix-Raviart, 1 - 3 lambda_{(i+3)%4} per face (crouzeix_raviart_1.h)
+ *   B2  edge bubbles 4 lambda_i lambda_j, edges 01,02,03,12,13,23 (bubble_2.h)
+ *   B4  cell bubble 256 lambda_0 lambda_1 lambda_2 lambda_3 (bubble_4.h)
+ *   RT0 Raviart-Thomas, one vector field per face: IDEN, DIV (CURL = 0)
+ *       (raviant_thomas_0.h)
+ *   ND0 Nedelec, one vector field per edge: IDEN, CURL (DIV = 0)
+ *       (nedelec_0.h)
+ * RT0 and ND0 take ncomp = 1; their GRAD is not provided. */
+enum {
+    ANIFEM_FEM_RT0 = 21,
+    ANIFEM_FEM_ND0 = 25,
+    ANIFEM_FEM_CR1 = 31,
+    ANIFEM_FEM_B2 = 1002,
+    ANIFEM_FEM_B4 = 1004
+};
 
 typedef struct anifem_op {
     int32_t op;    /* ANIFEM_IDEN, ANIFEM_GRAD, ANIFEM_DIV, ANIFEM_CURL, ANIFEM_DUDX..DUDZ */
-    int32_t fem;   /* ANIFEM_FEM_P0 .. ANIFEM_FEM_P3 */
+    int32_t fem;   /* ANIFEM_FEM_P0 .. ANIFEM_FEM_P3, or one of the spaces above */
     int32_t ncomp; /* 1 for FemFix, >1 for FemVec<ncomp, fem> (operators.h:63-67,127-155) */
 } anifem_op;
 

@@ -28,6 +28,8 @@ OK, ERR_NONFINITE, ERR_CONNECTIVITY, ERR_ARG, ERR_ORDER, ERR_DIMS, ERR_HIP, ERR_
 IDEN, GRAD, DIV, CURL, DUDX, DUDY, DUDZ = 1, 2, 3, 4, 6, 7, 8
 ELEM_SYMMETRIC = 0x100
 FEM_P0, FEM_P1, FEM_P2, FEM_P3 = 1, 2, 3, 4
+# element-level spaces (fem3dtet_batch, fem3dface_batch; not the global assembly)
+FEM_RT0, FEM_ND0, FEM_CR1, FEM_B2, FEM_B4 = 21, 25, 31, 1002, 1004
 TENSOR_NULL, TENSOR_SCALAR, TENSOR_SYMMETRIC, TENSOR_GENERAL, TENSOR_LAME = 1, 2, 3, 4, 101
 COEF_CONST, COEF_PER_TET, COEF_PER_POINT = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_FP16X3 = 0, 1, 2

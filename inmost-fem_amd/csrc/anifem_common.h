@@ -10,16 +10,26 @@
 
 namespace anifem {
 
-// Scalar finite-element space data (poly_0.h .. poly_3.h of the reference).
-int nf_scalar(int fem);          // 1, 4, 10, 20 for P0..P3; -1 otherwise
-int ref_dim(int op);             // per component: 1 for IDEN, 3 for the gradient-based operators; -1 otherwise
+// Finite-element space data (poly_0.h .. poly_3.h and spaces/*.h of the reference).
+int nf_scalar(int fem);          // basis functions per component: 1, 4, 10, 20 for P0..P3, ...; -1 otherwise
+bool fem_lagrange(int fem);      // P0..P3: the spaces of the global assembly
+bool fem_vector(int fem);        // RT0, ND0: vector-valued bases, one component
+// per component: 1 for IDEN of a scalar space, 3 for the gradient-based
+// operators; for RT0 / ND0 the reference field's dimension; -1 otherwise
+int ref_dim(int op, int fem);
 int op_dim(const anifem_op& op);  // the operator's dimension (Dim of operators.h)
 bool op_valid(const anifem_op& op);
+// How the physical values of OP(phi_i) follow from the reference table:
+//   map   0: U = PSI^T Uh (gradients, ND0 values) or U = Uh (dimension 1);
+//         1: U = J Uh (RT0 values, ND0 curls: the Piola map without 1/det J)
+//   scale 0: none; 1: |face i| / (|det J| / 2) (RT0); 2: |edge i| (ND0 values);
+//         3: |edge i| / det J (ND0 curls), a per-tet factor of basis function i
+void space_map(int op, int fem, int& map, int& scale);
 
 // Quadrature rule for `order` (generated table, tet_rules.inc); returns q or -1.
 int quadrature(int order, const double** p, const double** w);
 
-// Reference-space operator table of a scalar space at the rule's points:
+// Reference-space operator table of a space at the rule's points:
 // tab[(n*nf + i)*d + a] = component a (d = ref_dim) of OP(phi_i) at point n,
 // with gradients taken in the reference tetrahedron's coordinates
 // (x1, x2, x3) = (l1, l2, l3).

@@ -278,7 +278,8 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
 
 template <typename T>
 int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, hipStream_t s) {
-    const int dA = anifem::ref_dim(ctx->prob.trial.op), dB = anifem::ref_dim(ctx->prob.test.op);
+    const int dA = anifem::ref_dim(ctx->prob.trial.op, ctx->prob.trial.fem),
+              dB = anifem::ref_dim(ctx->prob.test.op, ctx->prob.test.fem);
     const int nc = ctx->prob.trial.ncomp;
     if (dA == 3 && dB == 3 && nc == 1) return launch_blocks<3, 3, 1, 1, T>(ctx, a, s);
     if (dA == 3 && dB == 3 && nc == 3) return launch_blocks<3, 3, 3, 3, T>(ctx, a, s);
@@ -399,6 +400,8 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
+    // the global assembly numbers the DOFs of P0..P3 spaces
+    if (!fem_lagrange(prob->trial.fem) || !fem_lagrange(prob->test.fem)) return ANIFEM_ERR_ARG;
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
     const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
@@ -415,7 +418,7 @@ int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, co
 
 extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
                                       const anifem_problem_desc* prob, void* stream) {
-    if (!prob || !anifem::op_valid(prob->test)) return ANIFEM_ERR_ARG;
+    if (!prob || !anifem::op_valid(prob->test) || !anifem::fem_lagrange(prob->test.fem)) return ANIFEM_ERR_ARG;
     const int nf = anifem::nf_scalar(prob->test.fem) * prob->test.ncomp;
     return anifem::assembly_create_caps(out, mesh, prob, anifem::default_caps(nf), stream);
 }

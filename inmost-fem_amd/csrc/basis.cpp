@@ -30,11 +30,27 @@ int nf_scalar(int fem) {
         case ANIFEM_FEM_P1: return 4;
         case ANIFEM_FEM_P2: return 10;
         case ANIFEM_FEM_P3: return 20;
+        case ANIFEM_FEM_CR1: return 4;
+        case ANIFEM_FEM_B2: return 6;
+        case ANIFEM_FEM_B4: return 1;
+        case ANIFEM_FEM_RT0: return 4;
+        case ANIFEM_FEM_ND0: return 6;
     }
     return -1;
 }
 
-int ref_dim(int op) {
+bool fem_lagrange(int fem) { return fem >= ANIFEM_FEM_P0 && fem <= ANIFEM_FEM_P3; }
+bool fem_vector(int fem) { return fem == ANIFEM_FEM_RT0 || fem == ANIFEM_FEM_ND0; }
+
+int ref_dim(int op, int fem) {
+    if (fem_vector(fem)) {
+        switch (op) {
+            case ANIFEM_IDEN:
+            case ANIFEM_CURL: return 3;
+            case ANIFEM_DIV: return 1;
+            default: return -1;
+        }
+    }
     switch (op) {
         case ANIFEM_IDEN: return 1;
         case ANIFEM_GRAD:
@@ -48,6 +64,7 @@ int ref_dim(int op) {
 }
 
 int op_dim(const anifem_op& op) {
+    if (fem_vector(op.fem)) return ref_dim(op.op, op.fem);
     switch (op.op) {
         case ANIFEM_IDEN: return op.ncomp;
         case ANIFEM_GRAD: return 3 * op.ncomp;
@@ -58,10 +75,28 @@ int op_dim(const anifem_op& op) {
 }
 
 bool op_valid(const anifem_op& op) {
-    if (ref_dim(op.op) < 0 || nf_scalar(op.fem) < 0 || op.ncomp < 1 || op.ncomp > 3) return false;
+    if (ref_dim(op.op, op.fem) < 0 || nf_scalar(op.fem) < 0 || op.ncomp < 1 || op.ncomp > 3) return false;
+    if (fem_vector(op.fem)) return op.ncomp == 1;
     if (op.op != ANIFEM_IDEN && op.fem == ANIFEM_FEM_P0) return false;
     if ((op.op == ANIFEM_DIV || op.op == ANIFEM_CURL) && op.ncomp != 3) return false;
     return true;
+}
+
+void space_map(int op, int fem, int& map, int& scale) {
+    map = 0;
+    scale = 0;
+    if (fem == ANIFEM_FEM_RT0) {
+        // RT0: phi_i = c_i (x - x_{(i+3)%4}) = c_i J (xi - xi_{(i+3)%4}),
+        // div phi_i = 3 c_i (raviant_thomas_0.h:35-47, :136-160); curl = 0
+        if (op == ANIFEM_IDEN) map = 1, scale = 1;
+        else if (op == ANIFEM_DIV) scale = 1;
+    } else if (fem == ANIFEM_FEM_ND0) {
+        // ND0 on edge (a, b): phi = |e| (l_a grad l_b - l_b grad l_a)
+        // = |e| PSI^T (l_a gh_b - l_b gh_a); curl phi = 2 |e| grad l_a x grad l_b
+        // = (|e| / det J) J (2 gh_a x gh_b) (nedelec_0.h:35-53, :100-135); div = 0
+        if (op == ANIFEM_IDEN) scale = 2;
+        else if (op == ANIFEM_CURL) map = 1, scale = 3;
+    }
 }
 
 int quadrature(int order, const double** p, const double** w) {
@@ -160,6 +195,14 @@ void basis_values(int fem, const double* L, double* phi) {
             }
             for (int f = 0; f < 4; ++f) phi[16 + f] = 27.0 * L[kFace[f][0]] * L[kFace[f][1]] * L[kFace[f][2]];
             break;
+        case ANIFEM_FEM_CR1:
+            // face i (vertices i, i+1, i+2): 1 - 3 l_{(i+3)%4}
+            for (int i = 0; i < 4; ++i) phi[i] = 1.0 - 3.0 * L[(i + 3) % 4];
+            break;
+        case ANIFEM_FEM_B2:
+            for (int e = 0; e < 6; ++e) phi[e] = 4.0 * L[kEdgeA[e]] * L[kEdgeB[e]];
+            break;
+        case ANIFEM_FEM_B4: phi[0] = 256.0 * L[0] * L[1] * L[2] * L[3]; break;
     }
 }
 
@@ -198,22 +241,76 @@ void basis_gradients(int fem, const double* L, double* g) {
                         27.0 * (dlam(A, a) * L[B] * L[C] + L[A] * dlam(B, a) * L[C] + L[A] * L[B] * dlam(C, a));
             }
             break;
+        case ANIFEM_FEM_CR1:
+            for (int i = 0; i < 4; ++i)
+                for (int a = 0; a < 3; ++a) g[i * 3 + a] = -3.0 * dlam((i + 3) % 4, a);
+            break;
+        case ANIFEM_FEM_B2:
+            for (int e = 0; e < 6; ++e) {
+                int I = kEdgeA[e], J = kEdgeB[e];
+                for (int a = 0; a < 3; ++a) g[e * 3 + a] = 4.0 * (L[I] * dlam(J, a) + L[J] * dlam(I, a));
+            }
+            break;
+        case ANIFEM_FEM_B4:
+            for (int a = 0; a < 3; ++a) {
+                double s = 0;
+                for (int v = 0; v < 4; ++v) {
+                    double prod = dlam(v, a);
+                    for (int w = 0; w < 4; ++w)
+                        if (w != v) prod *= L[w];
+                    s += prod;
+                }
+                g[a] = 256.0 * s;
+            }
+            break;
         default:
             for (int a = 0; a < 3; ++a) g[a] = 0.0;
+    }
+}
+
+// Reference fields of the vector-valued spaces (the part of OP(phi_i) that
+// space_map's map and scale turn into physical values), d components each:
+//   RT0 IDEN: xi - xi_{(i+3)%4} (xi_a = l_{a+1}); DIV: 3; CURL: 0
+//   ND0 IDEN on edge (a, b): l_a gh_b - l_b gh_a; CURL: 2 gh_a x gh_b; DIV: 0
+void vector_field(int op, int fem, const double* L, double* u) {
+    if (fem == ANIFEM_FEM_RT0) {
+        for (int i = 0; i < 4; ++i) {
+            const int v = (i + 3) % 4;
+            if (op == ANIFEM_IDEN)
+                for (int a = 0; a < 3; ++a) u[i * 3 + a] = L[a + 1] - (v == a + 1 ? 1.0 : 0.0);
+            else if (op == ANIFEM_DIV) u[i] = 3.0;
+            else
+                for (int a = 0; a < 3; ++a) u[i * 3 + a] = 0.0;
+        }
+    } else {
+        for (int e = 0; e < 6; ++e) {
+            const int A = kEdgeA[e], B = kEdgeB[e];
+            if (op == ANIFEM_IDEN) {
+                for (int c = 0; c < 3; ++c) u[e * 3 + c] = L[A] * dlam(B, c) - L[B] * dlam(A, c);
+            } else if (op == ANIFEM_CURL) {
+                for (int c = 0; c < 3; ++c) {
+                    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                    u[e * 3 + c] = 2.0 * (dlam(A, c1) * dlam(B, c2) - dlam(A, c2) * dlam(B, c1));
+                }
+            } else {
+                u[e] = 0.0;
+            }
+        }
     }
 }
 
 }  // namespace
 
 void reference_table(int op, int fem, int q, const double* p, std::vector<double>& tab) {
-    int nf = nf_scalar(fem), d = ref_dim(op);
+    int nf = nf_scalar(fem), d = ref_dim(op, fem);
     tab.assign((size_t)q * nf * d, 0.0);
     double buf[60];
     for (int n = 0; n < q; ++n) {
         // the reference recomputes l0 = 1 - l1 - l2 - l3 for stability
         // (quadrature_formulas.h:195-200); the generated table already satisfies it
         const double* L = p + 4 * n;
-        if (op == ANIFEM_IDEN) basis_values(fem, L, buf);
+        if (fem_vector(fem)) vector_field(op, fem, L, buf);
+        else if (op == ANIFEM_IDEN) basis_values(fem, L, buf);
         else basis_gradients(fem, L, buf);
         for (int i = 0; i < nf; ++i)
             for (int a = 0; a < d; ++a) tab[((size_t)n * nf + i) * d + a] = buf[i * d + a];
@@ -229,7 +326,7 @@ void reference_tensor(int opA, int femA, int opB, int femB, int order, bool per_
     std::vector<double> TA, TB;
     reference_table(opA, femA, q, p, TA);
     reference_table(opB, femB, q, p, TB);
-    int nfA = nf_scalar(femA), nfB = nf_scalar(femB), dA = ref_dim(opA), dB = ref_dim(opB);
+    int nfA = nf_scalar(femA), nfB = nf_scalar(femB), dA = ref_dim(opA, femA), dB = ref_dim(opB, femB);
     int KB = dA * dB, E0 = nfA * nfB, qq = per_point ? q : 1;
     R.assign((size_t)E0 * qq * KB, 0.0);
     for (int ia = 0; ia < nfA; ++ia)
@@ -275,7 +372,7 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
     const double* ww = wts.data();
     t.q = per_point ? nq : 1;
     t.E0 = nf_scalar(femA) * nf_scalar(femB);
-    t.KB = ref_dim(opA) * ref_dim(opB);
+    t.KB = ref_dim(opA, femA) * ref_dim(opB, femB);
     void* d64 = nullptr;
     void* d32 = nullptr;
     void* dw = nullptr;

@@ -27,8 +27,12 @@ static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem
     // the tensor (element_kernels.h: op_map); the normal form takes them on
     // the trial side only
     if (dot_normal && opB.op != ANIFEM_IDEN && opB.op != ANIFEM_GRAD) return ANIFEM_ERR_ARG;
-    a.selA = opA.op;
-    a.selB = opB.op;
+    // RT0 / ND0: the operator is the space's own (no selector); their values
+    // follow from the reference fields through space_map's map and factors
+    a.selA = fem_vector(opA.fem) ? ANIFEM_IDEN : opA.op;
+    a.selB = fem_vector(opB.fem) ? ANIFEM_IDEN : opB.op;
+    space_map(opA.op, opA.fem, a.mapA, a.scA);
+    space_map(opB.op, opB.fem, a.mapB, a.scB);
     a.dimA = op_dim(opA);
     a.dimB = op_dim(opB) * (dot_normal ? 3 : 1);
     a.E = a.nfA0 * opA.ncomp * a.nfB0 * opB.ncomp;
@@ -71,7 +75,9 @@ template <int DA0, int DB0, int NCA, int NCB, typename T>
 static int launch_one(ElemArgs a, int cap, hipStream_t s) {
     const int kstride = NCA * NCB * a.q * DA0 * DB0;
     a.tile = choose_tile(kstride, (int)sizeof(T), cap);
-    const size_t lds = (size_t)a.tile * kstride * sizeof(T);
+    size_t lds = (size_t)a.tile * kstride * sizeof(T);
+    if (a.scA | a.scB)
+        lds = element_scale_offset(a.tile, kstride, (int)sizeof(T)) + (size_t)a.tile * (a.nfA0 + a.nfB0) * sizeof(double);
     if (lds > 64 * 1024) return ANIFEM_ERR_ARG;
     const int64_t blocks = (a.n + a.tile - 1) / a.tile;
     if (blocks == 0) return ANIFEM_OK;
@@ -91,7 +97,7 @@ static int launch_nc(int ncA, int ncB, const ElemArgs& a, int cap, hipStream_t s
 
 template <typename T>
 int launch_element(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a, int cap, hipStream_t s) {
-    const int dA = ref_dim(opA.op), dB = ref_dim(opB.op);
+    const int dA = ref_dim(opA.op, opA.fem), dB = ref_dim(opB.op, opB.fem);
     if (dA == 3 && dB == 3) return launch_nc<T, 3, 3>(opA.ncomp, opB.ncomp, a, cap, s);
     if (dA == 1 && dB == 1) return launch_nc<T, 1, 1>(opA.ncomp, opB.ncomp, a, cap, s);
     if (dA == 3 && dB == 1) return launch_nc<T, 3, 1>(opA.ncomp, opB.ncomp, a, cap, s);
@@ -125,7 +131,8 @@ static int launch_mfma_nc(int ncA, int ncB, const ElemArgs& a, hipStream_t s) {
 // FP16 hi + lo MFMA contraction: tensors constant over each tet only
 int launch_element_mfma(const anifem_op& opA, const anifem_op& opB, const ElemArgs& a, hipStream_t s) {
     if (a.q != 1 || !a.r16) return ANIFEM_ERR_ARG;
-    const int dA = ref_dim(opA.op), dB = ref_dim(opB.op);
+    if (a.mapA | a.mapB | a.scA | a.scB) return ANIFEM_ERR_ARG;  // RT0 / ND0: FP64 and FP32 modes only
+    const int dA = ref_dim(opA.op, opA.fem), dB = ref_dim(opB.op, opB.fem);
     if (dA == 3 && dB == 3) return launch_mfma_nc<3, 3>(opA.ncomp, opB.ncomp, a, s);
     if (dA == 1 && dB == 1) return launch_mfma_nc<1, 1>(opA.ncomp, opB.ncomp, a, s);
     if (dA == 3 && dB == 1) return launch_mfma_nc<3, 1>(opA.ncomp, opB.ncomp, a, s);
@@ -178,7 +185,7 @@ namespace anifem {
 // launch of apply_kernel (the table is freed after the kernel has run).
 int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s) {
     if (!op_valid(op) || !xyl || q < 1 || q > 96) return ANIFEM_ERR_ARG;
-    if (op.op != ANIFEM_IDEN && op.op != ANIFEM_GRAD) return ANIFEM_ERR_ARG;
+    if ((op.op != ANIFEM_IDEN && op.op != ANIFEM_GRAD) || fem_vector(op.fem)) return ANIFEM_ERR_ARG;
     std::vector<double> tab;
     reference_table(op.op, op.fem, q, xyl, tab);
     a.grad = op.op == ANIFEM_GRAD ? 1 : 0;

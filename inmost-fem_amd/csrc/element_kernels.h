@@ -36,6 +36,8 @@ struct ElemArgs {
     int dimA, dimB;    // full operator dims (ncomp * ref dim)
     int p0_bcast;      // IDEN P0 trial against a vector test: D column replicated
     int selA, selB;    // operator codes: DIV / CURL / DUDX..DUDZ select from the gradient (op_map)
+    int mapA, mapB;    // 0: T = PSI^T (or 1 for dimension 1); 1: T = J (Piola map of RT0 values, ND0 curls)
+    int scA, scB;      // per-DOF factor of the basis (dof_scale): 0 none, 1 RT0, 2 ND0 values, 3 ND0 curls
     int face;          // -1: volume integral; 0..3: integral over that face (int_face.inl)
     int dot_normal;    // face integral of (D OpA(u)) . n against OpB (fem3DfaceN): D has 3 dimB rows
     int tile;          // tets per workgroup
@@ -75,6 +77,34 @@ __device__ __forceinline__ void tet_geometry(const double P[4][3], double PSI[9]
     PSI[6] = (m[3] * m[7] - m[6] * m[4]) * rd;
     PSI[7] = (m[6] * m[1] - m[0] * m[7]) * rd;
     PSI[8] = (m[0] * m[4] - m[3] * m[1]) * rd;
+}
+
+// JT[a + 3j] = J(j, a) = P[a+1][j] - P[0][j]: the map of contravariant
+// (Piola) reference fields, x = P0 + J xi.
+__device__ __forceinline__ void tet_jacobian_t(const double P[4][3], double JT[9]) {
+    for (int a = 0; a < 3; ++a)
+        for (int j = 0; j < 3; ++j) JT[a + 3 * j] = P[a + 1][j] - P[0][j];
+}
+
+// Per-DOF factor of basis function i of the vector-valued spaces:
+// 1: RT0, |face i| / (|det J| / 2), face i through vertices i, i+1, i+2
+//    (raviant_thomas_0.h:35-45); 2: ND0 values, |edge i| (nedelec_0.h:37-46);
+// 3: ND0 curls, |edge i| / det J (nedelec_0.h:111-122); edges 01,02,03,12,13,23.
+__device__ __forceinline__ double dof_scale(int kind, const double P[4][3], double det, int i) {
+    if (kind == 1) {
+        const double* p0 = P[i];
+        const double* p1 = P[(i + 1) % 4];
+        const double* p2 = P[(i + 2) % 4];
+        const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const double v[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        return sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) / fabs(det);
+    }
+    const int ea = i < 3 ? 0 : (i < 5 ? 1 : 2);
+    const int eb = i < 3 ? i + 1 : (i == 3 ? 2 : 3);
+    const double d0 = P[eb][0] - P[ea][0], d1 = P[eb][1] - P[ea][1], d2 = P[eb][2] - P[ea][2];
+    const double len = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    return kind == 3 ? len / det : len;
 }
 
 // Value D(k, j) of the dimB x dimA tensor at (tet, point), k row, j column.
@@ -154,12 +184,15 @@ __device__ __forceinline__ double face_measure(const double P[4][3], int f, doub
 
 // K block (cb, ca) at point n of one tet, scaled by s:
 // Kb[b*DA0 + a] = s * sum_{k0, j0} TB(k0, b) D(k0 + DB0 cb, j0 + DA0 ca) TA(j0, a),
-// T = PSI^T for GRAD, 1 for IDEN.  With a face normal nrm, D(k, j) is the
-// normal contraction sum_d D(d + 3k, j) nrm[d] of the 3 dimB x dimA tensor
-// (core.h:176-195).
+// T = PSI^T for GRAD, 1 for IDEN, J for the Piola-mapped fields (map 1, with
+// JT given).  With a face normal nrm, D(k, j) is the normal contraction
+// sum_d D(d + 3k, j) nrm[d] of the 3 dimB x dimA tensor (core.h:176-195).
 template <int DA0, int DB0, typename KT>
 __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, const double PSI[9], double s, int n,
-                                               int cb, int ca, KT* Kb, const double* nrm = nullptr) {
+                                               int cb, int ca, KT* Kb, const double* nrm = nullptr,
+                                               const double* JT = nullptr) {
+    const double* TAm = (JT && a.mapA) ? JT : PSI;
+    const double* TBm = (JT && a.mapB) ? JT : PSI;
     double Dblk[DB0][DA0];
     for (int k0 = 0; k0 < DB0; ++k0)
         for (int j0 = 0; j0 < DA0; ++j0) {
@@ -177,7 +210,7 @@ __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, co
         for (int a0 = 0; a0 < DA0; ++a0) {
             double acc = 0;
             if (DA0 == 3) {
-                for (int j0 = 0; j0 < 3; ++j0) acc += Dblk[k0][j0] * PSI[a0 + 3 * j0];
+                for (int j0 = 0; j0 < 3; ++j0) acc += Dblk[k0][j0] * TAm[a0 + 3 * j0];
             } else {
                 acc = Dblk[k0][0];
             }
@@ -187,7 +220,7 @@ __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, co
         for (int a0 = 0; a0 < DA0; ++a0) {
             double acc = 0;
             if (DB0 == 3) {
-                for (int k0 = 0; k0 < 3; ++k0) acc += PSI[b0 + 3 * k0] * M[k0][a0];
+                for (int k0 = 0; k0 < 3; ++k0) acc += TBm[b0 + 3 * k0] * M[k0][a0];
             } else {
                 acc = M[0][a0];
             }
@@ -198,13 +231,14 @@ __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, co
 // K[(cb*NCA + ca)*q*KB + n*KB + b*DA0 + a] for one tet.
 template <int DA0, int DB0, int NCA, int NCB, typename KT>
 __device__ __forceinline__ void geometry_tensor(const ElemArgs& a, int64_t cr, const double PSI[9], double mes,
-                                                KT* K, const double* nrm = nullptr) {
+                                                KT* K, const double* nrm = nullptr, const double* JT = nullptr) {
     constexpr int KB = DA0 * DB0;
     for (int n = 0; n < a.q; ++n) {
         const double s = (a.q > 1) ? mes * a.wq[n] : mes;
         for (int cb = 0; cb < NCB; ++cb)
             for (int ca = 0; ca < NCA; ++ca)
-                geometry_block<DA0, DB0, KT>(a, cr, PSI, s, n, cb, ca, K + ((cb * NCA + ca) * a.q + n) * KB, nrm);
+                geometry_block<DA0, DB0, KT>(a, cr, PSI, s, n, cb, ca, K + ((cb * NCA + ca) * a.q + n) * KB, nrm,
+                                             JT);
     }
 }
 
@@ -221,8 +255,15 @@ __device__ __forceinline__ void load_tet(const ElemArgs& a, int64_t r, double P[
     }
 }
 
+// LDS byte offset of the per-DOF factors that follow the K tensors of a tile.
+__host__ __device__ inline size_t element_scale_offset(int tile, int kstride, int elem_bytes) {
+    return ((size_t)tile * kstride * elem_bytes + 15) / 16 * 16;
+}
+
 // Element matrices of a tile of tets, stored in T precision to a.out
 // (n_tets matrices of nfB x nfA, column-major, one after the other).
+// Vector-valued bases (RT0, ND0) carry a per-tet factor per basis function,
+// applied to the contracted entry: A(ib, ia) = sB_ib sA_ia sum_kk K R.
 template <int DA0, int DB0, int NCA, int NCB, typename T>
 __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     constexpr int KB = DA0 * DB0;
@@ -231,16 +272,25 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     const int kstride = NCA * NCB * a.q * KB;
     const int64_t tile0 = (int64_t)blockIdx.x * a.tile;
     const int ntile = (int)((a.n - tile0) < a.tile ? (a.n - tile0) : a.tile);
+    const bool scaled = (a.scA | a.scB) != 0;
+    const int ns = a.nfA0 + a.nfB0;
+    double* Ss = reinterpret_cast<double*>(smem_raw + element_scale_offset(a.tile, kstride, (int)sizeof(T)));
 
     for (int t = threadIdx.x; t < ntile; t += kBlock) {
         const int64_t r = tile0 + t;
-        double P[4][3], PSI[9], det, nrm[3];
+        double P[4][3], PSI[9], det, nrm[3], JT[9];
         load_tet<false>(a, r, P);
         tet_geometry(P, PSI, det);
+        tet_jacobian_t(P, JT);
         // volume |det|/6, or the face's area (int_face.inl:40-43, 79-84)
         const double mes = a.face < 0 ? fabs(det) / 6.0 : face_measure(P, a.face, a.dot_normal ? nrm : nullptr);
         geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride,
-                                               a.face >= 0 && a.dot_normal ? nrm : nullptr);
+                                               a.face >= 0 && a.dot_normal ? nrm : nullptr, JT);
+        if (scaled) {
+            double* St = Ss + (int64_t)t * ns;
+            for (int i = 0; i < a.nfA0; ++i) St[i] = a.scA ? dof_scale(a.scA, P, det, i) : 1.0;
+            for (int i = 0; i < a.nfB0; ++i) St[a.nfA0 + i] = a.scB ? dof_scale(a.scB, P, det, i) : 1.0;
+        }
     }
     __syncthreads();
 
@@ -272,6 +322,10 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
             } else {
                 const T* Rn = R + (int64_t)rr * a.q * KB;
                 for (int j = 0; j < a.q * KB; ++j) acc += Kt[j] * Rn[j];
+            }
+            if (scaled) {
+                const double* St = Ss + (int64_t)t * ns;
+                acc = (T)((double)acc * (St[ia0] * St[a.nfA0 + ib0]));
             }
             static_cast<T*>(a.out)[(tile0 + t) * E + e] = acc;
         }

@@ -28,7 +28,6 @@
 namespace anifem {
 
 int nf_scalar(int fem);
-int ref_dim(int op);
 int quadrature(int order, const double** p, const double** w);
 
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn) {

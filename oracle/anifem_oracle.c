@@ -6,7 +6,9 @@
  * Element part: a restatement of the reference's per-tet algorithm
  *   geometry   core.inl:218-270, geometry.h:25-45
  *   operators  poly_0.h:62-81, poly_1.h:12-133, poly_2.h:12-102, poly_3.h:12-119,
- *              operators.h:127-155 (FemVec block structure)
+ *              operators.h:127-155 (FemVec block structure); spaces/
+ *              crouzeix_raviart_1.h, bubble_2.h, bubble_4.h,
+ *              raviant_thomas_0.h, nedelec_0.h (physical-space formulas)
  *   D*U        diff_tensor.h:62-97 (DFunc wrapper), :280-995 (applyDU)
  *   V^T*DU     core.inl:29-60 (fusive_AT_mul_B), :278-367
  * in the same data layouts (U[k + dim*(n + q*i)], A[ib + nfB*ia]) and the same
@@ -261,8 +263,22 @@ static int orc_nf_scalar(int fem) {
         case ORC_P1: return 4;
         case ORC_P2: return 10;
         case ORC_P3: return 20;
+        case ORC_CR1: return 4;
+        case ORC_B2: return 6;
+        case ORC_B4: return 1;
+        case ORC_RT0: return 4;
+        case ORC_ND0: return 6;
     }
     return -1;
+}
+
+static int orc_is_vector(int fem) { return fem == ORC_RT0 || fem == ORC_ND0; }
+
+/* operator dimension on (fem, ncomp): raviant_thomas_0.h and nedelec_0.h
+ * declare Dim 3 for IDEN and CURL, 1 for DIV */
+static int orc_opdim(int op, int fem, int ncomp) {
+    if (orc_is_vector(fem)) return op == ORC_DIV ? 1 : 3;
+    return orc_dim(op, ncomp);
 }
 
 int orc_nfa(int fem, int ncomp) {
@@ -282,6 +298,7 @@ int orc_dim(int op, int ncomp) {
 
 /* operator code valid on (fem, ncomp): operators.h:37-45, :321-400 */
 static int orc_op_ok(int op, int fem, int ncomp) {
+    if (orc_is_vector(fem)) return ncomp == 1 && (op == ORC_IDEN || op == ORC_DIV || op == ORC_CURL);
     if (op != ORC_IDEN && op != ORC_GRAD && op != ORC_DIV && op != ORC_CURL && (op < ORC_DUDX || op > ORC_DUDZ))
         return 0;
     if (op != ORC_IDEN && fem == ORC_P0) return 0;
@@ -313,6 +330,15 @@ static void orc_iden(int fem, const double* L, double* phi) {
             }
             for (int f = 0; f < 4; ++f)
                 phi[16 + f] = 27 * L[ORC_IPF[4 * f]] * L[ORC_IPF[4 * f + 1]] * L[ORC_IPF[4 * f + 2]];
+            break;
+        case ORC_CR1: /* crouzeix_raviart_1.h:34-37: 1 - 3 l_{IPF3[i]}, IPF3 = {3, 0, 1, 2} */
+            for (int i = 0; i < 4; ++i) phi[i] = 1 - 3 * L[(i + 3) % 4];
+            break;
+        case ORC_B2: /* bubble_2.h:31-34 */
+            for (int e = 0; e < 6; ++e) phi[e] = 4 * L[ORC_EI[e]] * L[ORC_EJ[e]];
+            break;
+        case ORC_B4: /* bubble_4.h:30-33 */
+            phi[0] = 256 * L[0] * L[1] * L[2] * L[3];
             break;
     }
 }
@@ -357,11 +383,102 @@ static void orc_refgrad(int fem, const double* L, double* g) {
                                                s1 * s2 * ORC_GP1[d + 3 * c]);
             }
             break;
+        case ORC_CR1: /* crouzeix_raviart_1.h:61-73: -3 grad l_{IPF3[i]} */
+            for (int i = 0; i < 4; ++i)
+                for (int d = 0; d < 3; ++d) g[d + 3 * i] = -3 * ORC_GP1[d + 3 * ((i + 3) % 4)];
+            break;
+        case ORC_B2: /* bubble_2.h: 4 (l_I grad l_J + l_J grad l_I) */
+            for (int e = 0; e < 6; ++e)
+                for (int d = 0; d < 3; ++d)
+                    g[d + 3 * e] = 4 * (ORC_GP1[d + 3 * ORC_EI[e]] * L[ORC_EJ[e]] + L[ORC_EI[e]] * ORC_GP1[d + 3 * ORC_EJ[e]]);
+            break;
+        case ORC_B4: { /* bubble_4.h:70-80: 256 sum_v grad l_v prod_{w != v} l_w */
+            double l[4] = {L[1] * L[2] * L[3], L[0] * L[2] * L[3], L[0] * L[1] * L[3], L[0] * L[1] * L[2]};
+            for (int d = 0; d < 3; ++d)
+                g[d] = 256 * (ORC_GP1[d] * l[0] + ORC_GP1[d + 3] * l[1] + ORC_GP1[d + 6] * l[2] + ORC_GP1[d + 9] * l[3]);
+            break;
+        }
+    }
+}
+
+/* (p0 - p2) x (p1 - p2) (geometry.h:55-63) */
+static void orc_cross3(const double* p0, const double* p1, const double* p2, double* c) {
+    double a[3] = {p0[0] - p2[0], p0[1] - p2[1], p0[2] - p2[2]}, b[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = -a[0] * b[2] + a[2] * b[0];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Operator tables of the vector-valued spaces for one tet with vertices X[3v + j]
+ * and det J = det: U[k + dim*(n + q*i)].
+ *   RT0 IDEN raviant_thomas_0.h:35-47, DIV :136-160, CURL :190-196 (zero)
+ *   ND0 IDEN nedelec_0.h:35-53, CURL :100-135, DIV :74-78 (zero)            */
+static void orc_vector_op(int op, int fem, int q, const double* XYL, const double* PSI, const double* X, double det,
+                          double* U) {
+    static const unsigned char IPR[24] = {0, 1, 2, 3, 0, 2, 3, 1, 0, 3, 1, 2, 1, 2, 0, 3, 1, 3, 2, 0, 2, 3, 0, 1};
+    int dim = orc_opdim(op, fem, 1), nf = orc_nf_scalar(fem);
+    memset(U, 0, sizeof(double) * (size_t)dim * q * nf);
+    double V[4][3]; /* V[v][j] = d lambda_v / d x_j */
+    for (int v = 0; v < 4; ++v)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += ORC_GP1[k + 3 * v] * PSI[k + 3 * j];
+            V[v][j] = s;
+        }
+    if (fem == ORC_RT0) {
+        if (op == ORC_CURL) return;
+        for (int i = 0; i < 4; ++i) {
+            const unsigned char* ip = ORC_IPF + 4 * i;
+            const double *x0 = X + 3 * ip[0], *x1 = X + 3 * ip[1], *x2 = X + 3 * ip[2], *x3 = X + 3 * ip[3];
+            double c[3];
+            orc_cross3(x0, x1, x2, c);
+            double sqr = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) / 2 / (fabs(det) / 2);
+            if (op == ORC_IDEN) {
+                for (int n = 0; n < q; ++n)
+                    for (int j = 0; j < 3; ++j)
+                        U[j + 3 * (n + q * i)] = sqr * (XYL[ip[0] + 4 * n] * (x0[j] - x3[j]) +
+                                                        XYL[ip[1] + 4 * n] * (x1[j] - x3[j]) +
+                                                        XYL[ip[2] + 4 * n] * (x2[j] - x3[j]));
+            } else {
+                double s = 0;
+                for (int j = 0; j < 3; ++j)
+                    s += V[ip[0]][j] * (x0[j] - x3[j]) + V[ip[1]][j] * (x1[j] - x3[j]) + V[ip[2]][j] * (x2[j] - x3[j]);
+                for (int n = 0; n < q; ++n) U[n + q * i] = sqr * s;
+            }
+        }
+        return;
+    }
+    if (op == ORC_DIV) return;
+    for (int i = 0; i < 6; ++i) {
+        const unsigned char* ip = IPR + 4 * i;
+        const double *a = X + 3 * ip[0], *b = X + 3 * ip[1];
+        double len = sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+        double xyn[3], xym[3];
+        orc_cross3(X + 3 * ip[2], X + 3 * ip[3], X + 3 * ip[0], xyn);
+        orc_cross3(X + 3 * ip[2], X + 3 * ip[3], X + 3 * ip[1], xym);
+        double sqr = len / det;
+        if (op == ORC_IDEN) {
+            for (int n = 0; n < q; ++n)
+                for (int k = 0; k < 3; ++k)
+                    U[k + 3 * (n + q * i)] = sqr * (xyn[k] * XYL[ip[0] + 4 * n] + xym[k] * XYL[ip[1] + 4 * n]);
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                double v = sqr * ((V[ip[0]][k1] * xyn[k2] + V[ip[1]][k1] * xym[k2]) -
+                                  (V[ip[0]][k2] * xyn[k1] + V[ip[1]][k2] * xym[k1]));
+                for (int n = 0; n < q; ++n) U[k + 3 * (n + q * i)] = v;
+            }
+        }
     }
 }
 
 /* Full (FemVec-expanded) operator table U[k + dim*(n + q*i)] for one tet. */
-static void orc_apply_op(int op, int fem, int ncomp, int q, const double* XYL, const double* PSI, double* U) {
+static void orc_apply_op(int op, int fem, int ncomp, int q, const double* XYL, const double* PSI, const double* X,
+                         double det, double* U) {
+    if (orc_is_vector(fem)) {
+        orc_vector_op(op, fem, q, XYL, PSI, X, det, U);
+        return;
+    }
     int nf0 = orc_nf_scalar(fem), dim0 = (op == ORC_IDEN ? 1 : 3);
     int dim = orc_dim(op, ncomp), nf = nf0 * ncomp;
     memset(U, 0, sizeof(double) * (size_t)dim * q * nf);
@@ -421,7 +538,7 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
     double XYL[4 * 24], WG[24];
     int q = orc_quadrature(order, XYL, WG);
     if (q <= 0) return -2;
-    int dimA = orc_dim(opA, ncA), dimB = orc_dim(opB, ncB);
+    int dimA = orc_opdim(opA, femA, ncA), dimB = orc_opdim(opB, femB, ncB);
     int ddim = tensor_type == ORC_TENSOR_NULL ? 0 : (tensor_type == ORC_TENSOR_SCALAR ? 1 : dimA * dimB);
     if ((tensor_type == ORC_TENSOR_NULL || tensor_type == ORC_TENSOR_SCALAR) && dimA != dimB &&
         !(nfA == 1 && dimA == 1))
@@ -437,8 +554,8 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
             for (int i = 0; i < 3; ++i) XYP[i + 3 * l] = P[3 * (l + 1) + i] - P[i];
         double det = orc_inverse3x3(XYP, PSI);
         double mes = fabs(det) / 6;
-        orc_apply_op(opA, femA, ncA, q, XYL, PSI, U);
-        orc_apply_op(opB, femB, ncB, q, XYL, PSI, V);
+        orc_apply_op(opA, femA, ncA, q, XYL, PSI, P, det, U);
+        orc_apply_op(opB, femB, ncB, q, XYL, PSI, P, det, V);
         for (int n_ = 0; n_ < q; ++n_) {
             const double* Dn = NULL;
             if (coef_mode == ORC_COEF_CONST) Dn = D;
@@ -563,7 +680,7 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
     if (!orc_op_ok(opA, femA, ncA) || !orc_op_ok(opB, femB, ncB)) return -1;
     double XYL[4 * 36], WG[36];
     int q = orc_face_quadrature(face, XYL, WG);
-    int dimA = orc_dim(opA, ncA), dimB = orc_dim(opB, ncB);
+    int dimA = orc_opdim(opA, femA, ncA), dimB = orc_opdim(opB, femB, ncB);
     int nD = dot_normal ? 3 : 1, dimBt = dimB * nD;
     int ddim = tensor_type == ORC_TENSOR_NULL ? 0 : (tensor_type == ORC_TENSOR_SCALAR ? 1 : dimA * dimBt);
     int bcast = nfA == 1 && dimA == 1;
@@ -577,14 +694,14 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
         for (int i = 0; i < 3; ++i) XYP[i] = 0;
         for (int l = 0; l < 3; ++l)
             for (int i = 0; i < 3; ++i) XYP[3 * (l + 1) + i] = P[3 * (l + 1) + i] - P[i];
-        orc_inverse3x3(XYP + 3, PSI);
+        double det = orc_inverse3x3(XYP + 3, PSI);
         const double* x0 = XYP + 3 * face;
         const double* x1 = XYP + 3 * ((face + 1) % 4);
         const double* x2 = XYP + 3 * ((face + 2) % 4);
         const double* x3 = XYP + 3 * ((face + 3) % 4);
         double mes = orc_face_geom(x0, x1, x2, x3, dot_normal ? nrm : NULL);
-        orc_apply_op(opA, femA, ncA, q, XYL, PSI, U);
-        orc_apply_op(opB, femB, ncB, q, XYL, PSI, V);
+        orc_apply_op(opA, femA, ncA, q, XYL, PSI, XYP, det, U);
+        orc_apply_op(opB, femB, ncB, q, XYL, PSI, XYP, det, V);
         for (int n_ = 0; n_ < q; ++n_) {
             const double* Dn = coef_mode == ORC_COEF_CONST      ? D
                                : coef_mode == ORC_COEF_PER_TET ? D + (size_t)ddim * r
@@ -635,7 +752,7 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
  * of orc_apply_op; out[r*dim*q + k + dim*n]. */
 int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n, const double* xyz,
                    const double* dofs, double* out) {
-    int nf = orc_nfa(fem, ncomp), dim = orc_dim(op, ncomp);
+    int nf = orc_nfa(fem, ncomp), dim = orc_opdim(op, fem, ncomp);
     if (nf <= 0 || q <= 0 || !orc_op_ok(op, fem, ncomp)) return -1;
     double* U = malloc(sizeof(double) * (size_t)dim * q * nf);
     for (long r = 0; r < n; ++r) {
@@ -643,8 +760,8 @@ int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n,
         double XYP[9], PSI[9];
         for (int l = 0; l < 3; ++l)
             for (int i = 0; i < 3; ++i) XYP[i + 3 * l] = P[3 * (l + 1) + i] - P[i];
-        orc_inverse3x3(XYP, PSI);
-        orc_apply_op(op, fem, ncomp, q, xyl, PSI, U);
+        double det = orc_inverse3x3(XYP, PSI);
+        orc_apply_op(op, fem, ncomp, q, xyl, PSI, P, det, U);
         for (int j = 0; j < dim * q; ++j) {
             double s = 0;
             for (int i = 0; i < nf; ++i) s += U[j + dim * q * i] * dofs[(size_t)nf * r + i];

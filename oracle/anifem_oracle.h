@@ -21,6 +21,8 @@ extern "C" {
  * (anifem++/fem/operators.h:25-45): IDEN=1, GRAD=2; FEM_P0=1 .. FEM_P3=4. */
 enum { ORC_IDEN = 1, ORC_GRAD = 2, ORC_DIV = 3, ORC_CURL = 4, ORC_DUDX = 6, ORC_DUDY = 7, ORC_DUDZ = 8 };
 enum { ORC_P0 = 1, ORC_P1 = 2, ORC_P2 = 3, ORC_P3 = 4 };
+/* operators.h:26-35 */
+enum { ORC_RT0 = 21, ORC_ND0 = 25, ORC_CR1 = 31, ORC_B2 = 1002, ORC_B4 = 1004 };
 /* Tensor types (anifem++/fem/diff_tensor.h:17-22). */
 enum { ORC_TENSOR_NULL = 1, ORC_TENSOR_SCALAR = 2, ORC_TENSOR_SYMMETRIC = 3, ORC_TENSOR_GENERAL = 4 };
 /* Coefficient storage: one tensor for all tets, one per tet, one per (tet, point). */

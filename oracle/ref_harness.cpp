@@ -235,7 +235,11 @@ int ref_quadrature(int order, double* p, double* w) {
 
 // Pair codes (OpA, OpB): 0=(GRAD P1, GRAD P1); 1=(GRAD P2, GRAD P2); 2=(GRAD P3, GRAD P3); 3=(GRAD V1, GRAD V1); 4=(GRAD V2, GRAD V2); 5=(IDEN P0, IDEN P1); 6=(IDEN P0, IDEN P2); 7=(IDEN P0, IDEN P3); 8=(IDEN P0, IDEN V2); 9=(IDEN P1, IDEN P1); 10=(IDEN P2, IDEN P2); 11=(IDEN P3, IDEN P3); 12=(GRAD P3, GRAD V1); 13=(IDEN P0, IDEN V1); 14=(GRAD P2, IDEN P2);
 // 15=(DIV V2, DIV V2); 16=(DIV V2, IDEN P1); 17=(CURL V1, CURL V1); 18=(CURL V2, IDEN V2);
-// 19=(DUDX P2, DUDY P2); 20=(DUDZ P1, IDEN P1).
+// 19=(DUDX P2, DUDY P2); 20=(DUDZ P1, IDEN P1);
+// 21=(IDEN RT0, IDEN RT0); 22=(DIV RT0, DIV RT0); 23=(DIV RT0, IDEN P0); 24=(IDEN ND0, IDEN ND0);
+// 25=(CURL ND0, CURL ND0); 26=(CURL ND0, IDEN RT0); 27=(GRAD CR1, GRAD CR1); 28=(IDEN CR1, IDEN P1);
+// 29=(GRAD B2, GRAD B2); 30=(IDEN B4, IDEN B4); 31=(GRAD B4, GRAD P1); 32=(IDEN P1, IDEN ND0);
+// 33=(GRAD B4, GRAD B4).
 //  xyz is n x 12 (P0,P1,P2,P3 per tet), D is n x ddim.
 // A receives n element matrices, each nfB x nfA column-major.
 int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, const double* xyz, const double* D,
@@ -246,6 +250,11 @@ int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, co
     using P3 = FemFix<FEM_P3>;
     using V1 = FemVec<3, FEM_P1>;
     using V2 = FemVec<3, FEM_P2>;
+    using RT0 = FemFix<FEM_RT0>;
+    using ND0 = FemFix<FEM_ND0>;
+    using CR1 = FemFix<FEM_CR1>;
+    using B2 = FemFix<FEM_B2>;
+    using B4 = FemFix<FEM_B4>;
     try {
         switch (pair) {
             case 0: return dispatch_tensor<Operator<GRAD, P1>, Operator<GRAD, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
@@ -269,6 +278,19 @@ int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, co
             case 18: return dispatch_tensor<Operator<CURL, V2>, Operator<IDEN, V2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 19: return dispatch_tensor<Operator<DUDX, P2>, Operator<DUDY, P2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 20: return dispatch_tensor<Operator<DUDZ, P1>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 21: return dispatch_tensor<Operator<IDEN, RT0>, Operator<IDEN, RT0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 22: return dispatch_tensor<Operator<DIV, RT0>, Operator<DIV, RT0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 23: return dispatch_tensor<Operator<DIV, RT0>, Operator<IDEN, P0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 24: return dispatch_tensor<Operator<IDEN, ND0>, Operator<IDEN, ND0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 25: return dispatch_tensor<Operator<CURL, ND0>, Operator<CURL, ND0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 26: return dispatch_tensor<Operator<CURL, ND0>, Operator<IDEN, RT0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 27: return dispatch_tensor<Operator<GRAD, CR1>, Operator<GRAD, CR1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 28: return dispatch_tensor<Operator<IDEN, CR1>, Operator<IDEN, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 29: return dispatch_tensor<Operator<GRAD, B2>, Operator<GRAD, B2>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 30: return dispatch_tensor<Operator<IDEN, B4>, Operator<IDEN, B4>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 31: return dispatch_tensor<Operator<GRAD, B4>, Operator<GRAD, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 32: return dispatch_tensor<Operator<IDEN, P1>, Operator<IDEN, ND0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 33: return dispatch_tensor<Operator<GRAD, B4>, Operator<GRAD, B4>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             default: break;
         }
     } catch (const std::exception&) {

@@ -14,7 +14,7 @@ ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libanifem_ref.so")
 GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
 
-NF = {1: 1, 2: 4, 3: 10, 4: 20}
+NF = {1: 1, 2: 4, 3: 10, 4: 20, 21: 4, 25: 6, 31: 4, 1002: 6, 1004: 1}
 
 _oracle = None
 

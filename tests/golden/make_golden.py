@@ -28,8 +28,13 @@ PAIRS = {
     12: (2, 4, 1, 2, 2, 3), 13: (1, 1, 1, 1, 2, 3), 14: (2, 3, 1, 1, 3, 1),
     15: (3, 3, 3, 3, 3, 3), 16: (3, 3, 3, 1, 2, 1), 17: (4, 2, 3, 4, 2, 3), 18: (4, 3, 3, 1, 3, 3),
     19: (6, 3, 1, 7, 3, 1), 20: (8, 2, 1, 1, 2, 1),
+    21: (1, 21, 1, 1, 21, 1), 22: (3, 21, 1, 3, 21, 1), 23: (3, 21, 1, 1, 1, 1), 24: (1, 25, 1, 1, 25, 1),
+    25: (4, 25, 1, 4, 25, 1), 26: (4, 25, 1, 1, 21, 1), 27: (2, 31, 1, 2, 31, 1), 28: (1, 31, 1, 1, 2, 1),
+    29: (2, 1002, 1, 2, 1002, 1), 30: (1, 1004, 1, 1, 1004, 1), 31: (2, 1004, 1, 2, 2, 1), 32: (1, 2, 1, 1, 25, 1),
+    33: (2, 1004, 1, 2, 1004, 1),
 }
-NF = {1: 1, 2: 4, 3: 10, 4: 20}
+# basis functions per component: P0..P3, RT0, ND0, CR1, B2, B4
+NF = {1: 1, 2: 4, 3: 10, 4: 20, 21: 4, 25: 6, 31: 4, 1002: 6, 1004: 1}
 
 
 def kuhn_tets(rng, n_tets, n=16, jitter=0.2):
@@ -128,6 +133,29 @@ def op_cases(rng):
     ]
 
 
+def space_cases(rng):
+    """RT0, ND0, CR1, B2 and B4 (spaces/*.h)."""
+    n = 32
+    xyz = kuhn_tets(rng, n)
+    pos = lambda: 10.0 ** rng.uniform(-1, 1, (n, 1))
+    spd = lambda: np.stack([aniso_tensor(rng).ravel(order="F") for _ in range(n)])
+    return [
+        ("sp_rt0_rt0_general", 21, 4, 0, 2, xyz, rng.normal(size=(n, 9))),
+        ("sp_rt0_rt0_xdep", 21, 4, 2, 2, xyz, rng.normal(size=(n, 9))),
+        ("sp_divrt0_divrt0_scalar", 22, 2, 0, 1, xyz, pos()),
+        ("sp_divrt0_p0_null", 23, 1, 0, 1, xyz, np.zeros((n, 1))),
+        ("sp_nd0_nd0_general", 24, 4, 0, 2, xyz, rng.normal(size=(n, 9))),
+        ("sp_curlnd0_curlnd0_scalar", 25, 2, 0, 1, xyz, pos()),
+        ("sp_curlnd0_rt0_null", 26, 1, 0, 2, xyz, np.zeros((n, 1))),
+        ("sp_cr1_cr1_scalar", 27, 2, 0, 1, xyz, pos()),
+        ("sp_cr1_p1_null", 28, 1, 0, 2, xyz, np.zeros((n, 1))),
+        ("sp_b2_b2_general", 29, 4, 0, 2, xyz, rng.normal(size=(n, 9))),
+        ("sp_b4_b4_null", 30, 1, 0, 6, xyz, np.zeros((n, 1))),
+        ("sp_b4_b4_aniso", 33, 4, 0, 6, xyz, spd()),
+        ("sp_p1_nd0_general", 32, 4, 0, 2, xyz, rng.normal(size=(n, 3))),
+    ]
+
+
 # face pair codes (ref_fem3dface) -> (opA, femA, ncA, opB, femB, ncB, dot_normal)
 FACE_PAIRS = {
     0: (1, 2, 1, 1, 2, 1, 0), 1: (1, 3, 1, 1, 3, 1, 0), 2: (1, 4, 1, 1, 4, 1, 0), 3: (1, 1, 1, 1, 3, 1, 0),
@@ -216,8 +244,16 @@ def main():
     L = ctypes.CDLL(REF)
     L.ref_fem3dtet.restype = ctypes.c_int
     dp = ctypes.POINTER(ctypes.c_double)
+    # groups to write (default all): elem (with the operator and space cases), faces, apply;
+    # "spaces" writes the space cases only
+    groups = set(sys.argv[1:]) or {"elem", "faces", "apply"}
     rng = np.random.default_rng(20260101)
-    for case in cases(rng) + op_cases(np.random.default_rng(20260404)):
+    todo = []
+    if "elem" in groups:
+        todo += cases(rng) + op_cases(np.random.default_rng(20260404))
+    if "elem" in groups or "spaces" in groups:
+        todo += space_cases(np.random.default_rng(20260505))
+    for case in todo:
         name, pair, tt, pp, order, xyz, D = case[:7]
         extra = case[7] if len(case) > 7 else None
         oa, fa, ca, ob, fb, cb = PAIRS[pair]
@@ -236,8 +272,10 @@ def main():
             arrays["lame"] = extra
         np.savez_compressed(os.path.join(HERE, f"elem_{name}.npz"), **arrays)
         print(f"{name}: {n} tets, {nfb}x{nfa}")
-    write_faces(L, np.random.default_rng(20260202))
-    write_apply(L, np.random.default_rng(20260303))
+    if "faces" in groups:
+        write_faces(L, np.random.default_rng(20260202))
+    if "apply" in groups:
+        write_apply(L, np.random.default_rng(20260303))
 
 
 if __name__ == "__main__":

@@ -68,7 +68,11 @@ def test_element_vs_reference_fixture(case):
     else:
         storage, Dprod = af.COEF_PER_TET, product_coef(tt, af.COEF_PER_TET, D, n, dA, dB)
     precs = [(af.PREC_FP64, FP64_TOL), (af.PREC_FP32, FP32_TOL)]
-    if storage == af.COEF_PER_TET:
+    if fa in (af.FEM_RT0, af.FEM_ND0) or fb in (af.FEM_RT0, af.FEM_ND0):
+        # the MFMA mode takes the spaces whose values are a fixed map of reference values
+        with pytest.raises(af.AnifemError):
+            run_product(opA, opB, tt, storage, Dprod, order, xyz, af.PREC_FP16X3)
+    elif storage == af.COEF_PER_TET:
         precs.append((af.PREC_FP16X3, FP16X3_TOL))
     else:  # the MFMA mode takes tensors constant over each tet only
         with pytest.raises(af.AnifemError):
@@ -88,7 +92,10 @@ def test_element_vs_reference_fixture(case):
 
 PAIRS = [(2, 2, 1, 2, 2, 1), (2, 3, 1, 2, 3, 1), (2, 4, 1, 2, 4, 1), (2, 3, 3, 2, 3, 3), (1, 1, 1, 1, 3, 1),
          (1, 1, 1, 1, 4, 3), (1, 3, 1, 1, 3, 1), (2, 4, 1, 2, 2, 3), (2, 3, 1, 1, 3, 1), (1, 2, 1, 2, 2, 1),
-         (1, 1, 1, 1, 2, 3)]
+         (1, 1, 1, 1, 2, 3),
+         # RT0 values and divergences, ND0 values and curls, CR1, B2, B4 (spaces/*.h)
+         (1, 21, 1, 1, 21, 1), (3, 21, 1, 1, 1, 1), (1, 25, 1, 1, 25, 1), (4, 25, 1, 1, 21, 1),
+         (2, 31, 1, 2, 31, 1), (2, 1002, 1, 1, 1002, 1), (1, 1004, 1, 1, 2, 1)]
 
 
 @pytest.mark.parametrize("pair", PAIRS)

@@ -210,6 +210,64 @@ def test_face_closed_forms():
         assert np.abs(G.sum(axis=1)).max() < 1e-12 * np.abs(G).max()
 
 
+RT0, ND0, CR1, B2, B4 = 21, 25, 31, 1002, 1004
+DIV, CURL = 3, 4
+
+
+def tet_measures(xyz):
+    """|T| and the areas of faces k (vertices k, k+1, k+2 mod 4), k = 0..3."""
+    Pv = xyz.reshape(-1, 4, 3)
+    vol = np.abs(np.linalg.det(np.stack([Pv[:, 1] - Pv[:, 0], Pv[:, 2] - Pv[:, 0], Pv[:, 3] - Pv[:, 0]], -1))) / 6
+    areas = []
+    for k in range(4):
+        a, b, c = Pv[:, k], Pv[:, (k + 1) % 4], Pv[:, (k + 2) % 4]
+        areas.append(np.linalg.norm(np.cross(b - a, c - a), axis=1) / 2)
+    return vol, np.stack(areas, 1)
+
+
+def test_space_closed_forms():
+    """The other spaces against what their definitions fix: RT0 normal
+    components phi_i . n = delta_ij on face j (face fluxes delta_ij |f_j|) and
+    int div phi_i = |f_i|; ND0 curls are divergence-free in the mean,
+    int curl phi_i = |e_i| * 2 int grad l_a x grad l_b, and their curl-curl
+    matrix annihilates gradients (the ND0 DOFs of grad l_v); CR1 face means
+    phi_i = delta_ij on face j; int B2 = |T| / 5; int B4 = 256 |T| / 840."""
+    rng = np.random.default_rng(17)
+    n = 8
+    xyz = rng.normal(size=(n, 12))
+    vol, area = tet_measures(xyz)
+    for face in range(4):
+        F = T.orc_face(IDEN, RT0, 1, IDEN, P0, 1, NULL, CONST, face, 1, xyz, None).reshape(n, 4)
+        ref = np.zeros((n, 4))
+        ref[:, face] = area[:, face]
+        assert np.abs(F - ref).max() <= 1e-13 * area.max()
+        C = T.orc_face(IDEN, CR1, 1, IDEN, P0, 1, NULL, CONST, face, 0, xyz, None).reshape(n, 4)
+        assert np.abs(C - ref).max() <= 1e-13 * area.max()
+    Dv = T.orc_elem(DIV, RT0, 1, IDEN, P0, 1, NULL, CONST, 1, xyz, None).reshape(n, 4)
+    assert np.abs(Dv - area).max() <= 1e-13 * area.max()
+    b2 = T.orc_elem(IDEN, B2, 1, IDEN, P0, 1, NULL, CONST, 2, xyz, None).reshape(n, 6)
+    assert np.abs(b2 - vol[:, None] / 5).max() <= 1e-14 * vol.max()
+    b4 = T.orc_elem(IDEN, B4, 1, IDEN, P0, 1, NULL, CONST, 4, xyz, None).reshape(n)
+    assert np.abs(b4 - 256 * vol / 840).max() <= 1e-14 * vol.max()
+    # ND0: phi_i . t integrates to |e_i| along edge i (t from a to b) and to 0
+    # along the others, so grad l_v = sum_i g_i phi_i with
+    # g_i = (l_v(b) - l_v(a)) / |e_i|; the curl-curl matrix maps g to 0 and the
+    # mass matrix gives |grad l_v|^2 |T|
+    E = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    Pv = xyz.reshape(n, 4, 3)
+    elen = np.stack([np.linalg.norm(Pv[:, b] - Pv[:, a], axis=1) for a, b in E], 1)
+    K = T.orc_elem(CURL, ND0, 1, CURL, ND0, 1, NULL, CONST, 1, xyz, None).reshape(n, 6, 6)
+    M = T.orc_elem(IDEN, ND0, 1, IDEN, ND0, 1, NULL, CONST, 2, xyz, None).reshape(n, 6, 6)
+    for v in range(4):
+        g = np.array([(1.0 if b == v else 0.0) - (1.0 if a == v else 0.0) for a, b in E])[None, :] / elen
+        assert np.abs(np.einsum("rij,rj->ri", K, g)).max() <= 1e-12 * np.abs(K).max()
+        J = np.stack([Pv[:, 1] - Pv[:, 0], Pv[:, 2] - Pv[:, 0], Pv[:, 3] - Pv[:, 0]], -1)
+        gh = np.array([-1.0, -1.0, -1.0]) if v == 0 else np.eye(3)[v - 1]
+        grad = np.linalg.solve(np.transpose(J, (0, 2, 1)), np.broadcast_to(gh, (n, 3))[..., None])[..., 0]
+        norm2 = np.einsum("ri,rij,rj->r", g, M, g)
+        assert np.allclose(norm2, (grad ** 2).sum(1) * vol, rtol=1e-12)
+
+
 def test_face_rule_exactness_and_reference_rules():
     """The product's triangle rules (anifem_face_quadrature) integrate every
     barycentric monomial on the face up to their degree; the reference's
