@@ -27,7 +27,18 @@
 namespace anifem_gpu {
 
 // Finite elements and operators (numbering of operators.h:25-45)
-enum FiniteElement { FEM_P0 = ANIFEM_FEM_P0, FEM_P1 = ANIFEM_FEM_P1, FEM_P2 = ANIFEM_FEM_P2, FEM_P3 = ANIFEM_FEM_P3 };
+enum FiniteElement {
+    FEM_P0 = ANIFEM_FEM_P0,
+    FEM_P1 = ANIFEM_FEM_P1,
+    FEM_P2 = ANIFEM_FEM_P2,
+    FEM_P3 = ANIFEM_FEM_P3,
+    // element-level only (operators.h:26-35; spaces/*.h)
+    FEM_RT0 = ANIFEM_FEM_RT0,
+    FEM_ND0 = ANIFEM_FEM_ND0,
+    FEM_CR1 = ANIFEM_FEM_CR1,
+    FEM_B2 = ANIFEM_FEM_B2,
+    FEM_B4 = ANIFEM_FEM_B4
+};
 enum OperatorType {
     IDEN = ANIFEM_IDEN,
     GRAD = ANIFEM_GRAD,
