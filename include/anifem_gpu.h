@@ -45,8 +45,9 @@ enum {
 enum { ANIFEM_FEM_P0 = 1, ANIFEM_FEM_P1 = 2, ANIFEM_FEM_P2 = 3, ANIFEM_FEM_P3 = 4 };
 /* The other FemFix spaces of the reference (operators.h:26-35; spaces/*.h),
  * same codes.  They are accepted by the element-level entry points
- * (anifem_fem3dtet_batch in FP64 / FP32, anifem_fem3dface_batch;
- * anifem_fem3dapply_batch for CR1, B2, B4), not by the global assembly:
+ * (anifem_fem3dtet_batch in FP64 / FP32, anifem_fem3dface_batch,
+ * anifem_fem3dapply_batch, anifem_fem3dapplyx_batch), not by the global
+ * assembly:
  *   CR1 Crouzeix-Raviart, 1 - 3 lambda_{(i+3)%4} per face (crouzeix_raviart_1.h)
  *   B2  edge bubbles 4 lambda_i lambda_j, edges 01,02,03,12,13,23 (bubble_2.h)
  *   B4  cell bubble 256 lambda_0 lambda_1 lambda_2 lambda_3 (bubble_4.h)
@@ -147,9 +148,17 @@ int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D,
  * points xyl (host, 4q, the same for every tet) of each tet, u given by its
  * per-tet DOF values dofs (device, n_tets x nfa: dofs[r*nfa + i]).  opU
  * (device) receives n_tets blocks of dim*q values: opU[r*dim*q + k + dim*n]
- * (for a vector space, component k + 3c of GRAD is du_c/dx_k). */
+ * (for a vector space, component k + 3c of GRAD is du_c/dx_k).  Operators:
+ * IDEN and GRAD of P0..P3, CR1, B2, B4; IDEN, DIV and CURL of RT0 and ND0. */
 int anifem_fem3dapply_batch(anifem_op op, const double* xyl, int q, int64_t n_tets, const double* xyz,
                             const double* dofs, double* opU, void* stream);
+/* Batched fem3DapplyX<Op>(XYZ, X, dofs, opU) (eval.h:26-31, eval.inl:103-125):
+ * Op(u) at q physical points per tet, X (device, n_tets x q x 3:
+ * X[(r*q + n)*3 + c]); their barycentric coordinates are formed on the device
+ * from the inverse Jacobian as getBaryCoord does (core.inl:120-137).  Other
+ * arguments, layouts and operators as anifem_fem3dapply_batch. */
+int anifem_fem3dapplyx_batch(anifem_op op, int q, int64_t n_tets, const double* xyz, const double* X,
+                             const double* dofs, double* opU, void* stream);
 /* Triangle rule of `order` placed on face `face`, as tet barycentric points
  * p[4q] (coordinate of the opposite vertex 0) and weights w[q] summing to 1;
  * returns q (or <0). */

@@ -113,6 +113,7 @@ SIGNATURES = {
                                       _VP, _I32, _VP, _VP]),
     "anifem_face_quadrature": (_I32, [_I32, _I32, _VP, _VP]),
     "anifem_fem3dapply_batch": (_I32, [anifem_op, _VP, _I32, _I64, _VP, _VP, _VP, _VP]),
+    "anifem_fem3dapplyx_batch": (_I32, [anifem_op, _I32, _I64, _VP, _VP, _VP, _VP, _VP]),
     "anifem_assembly_eval": (_I32, [_VP, _I32, _VP, _I32, _VP, _VP, _VP]),
     "anifem_assembly_create_budget": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
                                              ctypes.POINTER(anifem_problem_desc), _I32, _I32, _I32, _VP]),
@@ -307,10 +308,26 @@ def fem3dapply_batch(op: Op, xyl, xyz, dofs, out=None, stream=None):
     import torch
     xyl = np.ascontiguousarray(xyl, dtype=np.float64)
     q, n = len(xyl), xyz.shape[1]
+    if op.dim < 0:
+        raise AnifemError(ERR_ARG, "anifem_fem3dapply_batch: invalid operator")
     if out is None:
         out = torch.empty((n, q, op.dim), dtype=torch.float64, device=xyz.device)
     _check(lib().anifem_fem3dapply_batch(op.c(), _ptr(xyl), q, n, _ptr(xyz), _ptr(dofs), _ptr(out),
                                          _stream_handle(stream)), "anifem_fem3dapply_batch")
+    return out
+
+
+def fem3dapplyx_batch(op: Op, X, xyz, dofs, out=None, stream=None):
+    """Op(u) at q physical points per tet, X device (n, q, 3), with per-tet DOF
+    values dofs (device (n, nfa)) (anifem_fem3dapplyx_batch): (n, q, dim)."""
+    import torch
+    n, q = X.shape[0], X.shape[1]
+    if op.dim < 0:
+        raise AnifemError(ERR_ARG, "anifem_fem3dapplyx_batch: invalid operator")
+    if out is None:
+        out = torch.empty((n, q, op.dim), dtype=torch.float64, device=X.device)
+    _check(lib().anifem_fem3dapplyx_batch(op.c(), q, n, _ptr(xyz), _ptr(X), _ptr(dofs), _ptr(out),
+                                          _stream_handle(stream)), "anifem_fem3dapplyx_batch")
     return out
 
 

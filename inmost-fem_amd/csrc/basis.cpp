@@ -19,6 +19,7 @@
 #include <tuple>
 
 #include "anifem_common.h"
+#include "basis_eval.h"
 #include "tet_rules.inc"
 #include "tri_rules.inc"
 
@@ -162,144 +163,6 @@ int tensor_ddim(int type, int dimA, int dimB) {
     }
     return -1;
 }
-
-namespace {
-
-// Edges of the local tetrahedron in the order 01,02,03,12,13,23 and the P3
-// face node triples {0,1,2},{1,2,3},{2,3,0},{3,0,1}
-// (docs: local numbering; poly_2.h:75, poly_3.h:83).
-const int kEdgeA[6] = {0, 0, 0, 1, 1, 2};
-const int kEdgeB[6] = {1, 2, 3, 2, 3, 3};
-const int kFace[4][3] = {{0, 1, 2}, {1, 2, 3}, {2, 3, 0}, {3, 0, 1}};
-
-// d lambda_i / d x_a in reference coordinates (lambda_0 = 1 - x1 - x2 - x3).
-inline double dlam(int i, int a) { return i == 0 ? -1.0 : (i - 1 == a ? 1.0 : 0.0); }
-
-void basis_values(int fem, const double* L, double* phi) {
-    switch (fem) {
-        case ANIFEM_FEM_P0: phi[0] = 1.0; break;
-        case ANIFEM_FEM_P1:
-            for (int i = 0; i < 4; ++i) phi[i] = L[i];
-            break;
-        case ANIFEM_FEM_P2:
-            for (int i = 0; i < 4; ++i) phi[i] = L[i] * (2.0 * L[i] - 1.0);
-            for (int e = 0; e < 6; ++e) phi[4 + e] = 4.0 * L[kEdgeA[e]] * L[kEdgeB[e]];
-            break;
-        case ANIFEM_FEM_P3:
-            for (int i = 0; i < 4; ++i) phi[i] = 0.5 * L[i] * (3.0 * L[i] - 1.0) * (3.0 * L[i] - 2.0);
-            for (int e = 0; e < 6; ++e) {
-                // two DOFs per edge (I, J): the one near I first (poly_3.h:39-44)
-                double a = L[kEdgeA[e]], b = L[kEdgeB[e]];
-                phi[4 + 2 * e] = 4.5 * a * (3.0 * a - 1.0) * b;
-                phi[5 + 2 * e] = 4.5 * a * (3.0 * b - 1.0) * b;
-            }
-            for (int f = 0; f < 4; ++f) phi[16 + f] = 27.0 * L[kFace[f][0]] * L[kFace[f][1]] * L[kFace[f][2]];
-            break;
-        case ANIFEM_FEM_CR1:
-            // face i (vertices i, i+1, i+2): 1 - 3 l_{(i+3)%4}
-            for (int i = 0; i < 4; ++i) phi[i] = 1.0 - 3.0 * L[(i + 3) % 4];
-            break;
-        case ANIFEM_FEM_B2:
-            for (int e = 0; e < 6; ++e) phi[e] = 4.0 * L[kEdgeA[e]] * L[kEdgeB[e]];
-            break;
-        case ANIFEM_FEM_B4: phi[0] = 256.0 * L[0] * L[1] * L[2] * L[3]; break;
-    }
-}
-
-// g[i*3 + a] = d phi_i / d x_a (reference coordinates)
-void basis_gradients(int fem, const double* L, double* g) {
-    switch (fem) {
-        case ANIFEM_FEM_P1:
-            for (int i = 0; i < 4; ++i)
-                for (int a = 0; a < 3; ++a) g[i * 3 + a] = dlam(i, a);
-            break;
-        case ANIFEM_FEM_P2:
-            for (int i = 0; i < 4; ++i)
-                for (int a = 0; a < 3; ++a) g[i * 3 + a] = (4.0 * L[i] - 1.0) * dlam(i, a);
-            for (int e = 0; e < 6; ++e) {
-                int I = kEdgeA[e], J = kEdgeB[e];
-                for (int a = 0; a < 3; ++a) g[(4 + e) * 3 + a] = 4.0 * (L[I] * dlam(J, a) + L[J] * dlam(I, a));
-            }
-            break;
-        case ANIFEM_FEM_P3:
-            for (int i = 0; i < 4; ++i) {
-                double s = L[i];
-                for (int a = 0; a < 3; ++a) g[i * 3 + a] = ((13.5 * s - 9.0) * s + 1.0) * dlam(i, a);
-            }
-            for (int e = 0; e < 6; ++e) {
-                int I = kEdgeA[e], J = kEdgeB[e];
-                double s = L[I], t = L[J];
-                for (int a = 0; a < 3; ++a) {
-                    g[(4 + 2 * e) * 3 + a] = 4.5 * ((6.0 * s - 1.0) * t * dlam(I, a) + (3.0 * s - 1.0) * s * dlam(J, a));
-                    g[(5 + 2 * e) * 3 + a] = 4.5 * ((6.0 * t - 1.0) * s * dlam(J, a) + (3.0 * t - 1.0) * t * dlam(I, a));
-                }
-            }
-            for (int f = 0; f < 4; ++f) {
-                int A = kFace[f][0], B = kFace[f][1], C = kFace[f][2];
-                for (int a = 0; a < 3; ++a)
-                    g[(16 + f) * 3 + a] =
-                        27.0 * (dlam(A, a) * L[B] * L[C] + L[A] * dlam(B, a) * L[C] + L[A] * L[B] * dlam(C, a));
-            }
-            break;
-        case ANIFEM_FEM_CR1:
-            for (int i = 0; i < 4; ++i)
-                for (int a = 0; a < 3; ++a) g[i * 3 + a] = -3.0 * dlam((i + 3) % 4, a);
-            break;
-        case ANIFEM_FEM_B2:
-            for (int e = 0; e < 6; ++e) {
-                int I = kEdgeA[e], J = kEdgeB[e];
-                for (int a = 0; a < 3; ++a) g[e * 3 + a] = 4.0 * (L[I] * dlam(J, a) + L[J] * dlam(I, a));
-            }
-            break;
-        case ANIFEM_FEM_B4:
-            for (int a = 0; a < 3; ++a) {
-                double s = 0;
-                for (int v = 0; v < 4; ++v) {
-                    double prod = dlam(v, a);
-                    for (int w = 0; w < 4; ++w)
-                        if (w != v) prod *= L[w];
-                    s += prod;
-                }
-                g[a] = 256.0 * s;
-            }
-            break;
-        default:
-            for (int a = 0; a < 3; ++a) g[a] = 0.0;
-    }
-}
-
-// Reference fields of the vector-valued spaces (the part of OP(phi_i) that
-// space_map's map and scale turn into physical values), d components each:
-//   RT0 IDEN: xi - xi_{(i+3)%4} (xi_a = l_{a+1}); DIV: 3; CURL: 0
-//   ND0 IDEN on edge (a, b): l_a gh_b - l_b gh_a; CURL: 2 gh_a x gh_b; DIV: 0
-void vector_field(int op, int fem, const double* L, double* u) {
-    if (fem == ANIFEM_FEM_RT0) {
-        for (int i = 0; i < 4; ++i) {
-            const int v = (i + 3) % 4;
-            if (op == ANIFEM_IDEN)
-                for (int a = 0; a < 3; ++a) u[i * 3 + a] = L[a + 1] - (v == a + 1 ? 1.0 : 0.0);
-            else if (op == ANIFEM_DIV) u[i] = 3.0;
-            else
-                for (int a = 0; a < 3; ++a) u[i * 3 + a] = 0.0;
-        }
-    } else {
-        for (int e = 0; e < 6; ++e) {
-            const int A = kEdgeA[e], B = kEdgeB[e];
-            if (op == ANIFEM_IDEN) {
-                for (int c = 0; c < 3; ++c) u[e * 3 + c] = L[A] * dlam(B, c) - L[B] * dlam(A, c);
-            } else if (op == ANIFEM_CURL) {
-                for (int c = 0; c < 3; ++c) {
-                    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-                    u[e * 3 + c] = 2.0 * (dlam(A, c1) * dlam(B, c2) - dlam(A, c2) * dlam(B, c1));
-                }
-            } else {
-                u[e] = 0.0;
-            }
-        }
-    }
-}
-
-}  // namespace
 
 void reference_table(int op, int fem, int q, const double* p, std::vector<double>& tab) {
     int nf = nf_scalar(fem), d = ref_dim(op, fem);

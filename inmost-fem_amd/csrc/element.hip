@@ -181,27 +181,47 @@ extern "C" int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem
 }
 
 namespace anifem {
-// Reference table of op at q barycentric points, on the device, and the
-// launch of apply_kernel (the table is freed after the kernel has run).
-int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s) {
-    if (!op_valid(op) || !xyl || q < 1 || q > 96) return ANIFEM_ERR_ARG;
-    if ((op.op != ANIFEM_IDEN && op.op != ANIFEM_GRAD) || fem_vector(op.fem)) return ANIFEM_ERR_ARG;
-    std::vector<double> tab;
-    reference_table(op.op, op.fem, q, xyl, tab);
-    a.grad = op.op == ANIFEM_GRAD ? 1 : 0;
+// Operator-dependent part of ApplyArgs: IDEN / GRAD of the scalar spaces,
+// IDEN / DIV / CURL of RT0 and ND0.
+static int prepare_apply(const anifem_op& op, int q, ApplyArgs& a) {
+    if (!op_valid(op) || q < 1 || q > 96) return ANIFEM_ERR_ARG;
+    if (!fem_vector(op.fem) && op.op != ANIFEM_IDEN && op.op != ANIFEM_GRAD) return ANIFEM_ERR_ARG;
+    int m = 0, sc = 0;
+    space_map(op.op, op.fem, m, sc);
+    a.d = ref_dim(op.op, op.fem);
+    a.map = m == 1 ? 2 : ((op.op == ANIFEM_GRAD || (op.fem == ANIFEM_FEM_ND0 && op.op == ANIFEM_IDEN)) ? 1 : 0);
+    a.scale = sc;
+    a.op = op.op;
+    a.fem = op.fem;
     a.nf0 = nf_scalar(op.fem);
     a.ncomp = op.ncomp;
     a.q = q;
+    return ANIFEM_OK;
+}
+
+static int run_apply(ApplyArgs a, bool mesh, hipStream_t s) {
+    const int64_t blocks = (a.n * a.q + kBlock - 1) / kBlock;
+    if (mesh) hipLaunchKernelGGL(apply_kernel<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(apply_kernel<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError() == hipSuccess ? ANIFEM_OK : ANIFEM_ERR_HIP;
+}
+
+// Reference table of op at q barycentric points, on the device, and the
+// launch of apply_kernel (the table is freed after the kernel has run).
+int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s) {
+    if (!xyl) return ANIFEM_ERR_ARG;
+    int rc = prepare_apply(op, q, a);
+    if (rc) return rc;
     if (a.n == 0) return ANIFEM_OK;
+    std::vector<double> tab;
+    reference_table(op.op, op.fem, q, xyl, tab);
     double* dtab = nullptr;
     ANIFEM_HIP_CHECK(hipMalloc(&dtab, tab.size() * sizeof(double)));
     hipError_t e = hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         a.tab = dtab;
-        const int64_t blocks = (a.n * q + kBlock - 1) / kBlock;
-        if (mesh) hipLaunchKernelGGL(apply_kernel<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL(apply_kernel<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
-        e = hipGetLastError();
+        a.X = nullptr;
+        if (run_apply(a, mesh, s)) e = hipErrorLaunchFailure;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(dtab);
@@ -209,6 +229,23 @@ int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, boo
     return ANIFEM_OK;
 }
 }  // namespace anifem
+
+extern "C" int anifem_fem3dapplyx_batch(anifem_op op, int q, int64_t n_tets, const double* xyz, const double* X,
+                                        const double* dofs, double* opU, void* stream) {
+    using namespace anifem;
+    if (n_tets < 0 || (n_tets > 0 && (!xyz || !X || !dofs || !opU))) return ANIFEM_ERR_ARG;
+    ApplyArgs a{};
+    int rc = prepare_apply(op, q, a);
+    if (rc) return rc;
+    if (n_tets == 0) return ANIFEM_OK;
+    a.n = n_tets;
+    a.xyz = xyz;
+    a.X = X;
+    a.tab = nullptr;
+    a.dofs = dofs;
+    a.out = opU;
+    return run_apply(a, false, static_cast<hipStream_t>(stream));
+}
 
 extern "C" int anifem_fem3dapply_batch(anifem_op op, const double* xyl, int q, int64_t n_tets, const double* xyz,
                                        const double* dofs, double* opU, void* stream) {

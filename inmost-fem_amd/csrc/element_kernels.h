@@ -21,6 +21,7 @@
 #include <cstdint>
 
 #include "../../include/anifem_gpu.h"
+#include "basis_eval.h"
 
 namespace anifem {
 
@@ -334,9 +335,13 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
 
 // ---- solution evaluation: fem3DapplyL (eval.inl:84-100, core.inl:370-405) ----
 struct ApplyArgs {
-    int grad;              // GRAD (else IDEN)
-    int nf0, ncomp, q;     // scalar basis size, components, points per tet
-    const double* tab;     // reference values / gradients of the scalar basis: tab[(n*nf0 + i)*d + a]
+    int d;                 // reference components per basis function and component (1 or 3)
+    int map;               // 0: as tabulated; 1: PSI^T (gradients, ND0 values); 2: J (RT0 values, ND0 curls)
+    int scale;             // per-DOF factor (dof_scale kind), 0 none
+    int op, fem;           // operator and space: the basis is evaluated on the device in X mode
+    int nf0, ncomp, q;     // basis size per component, components, points per tet
+    const double* tab;     // L mode: reference table at the common points, tab[(n*nf0 + i)*d + a]
+    const double* X;       // X mode: physical points X[(r*q + n)*3 + c] (tab unused)
     int64_t n;             // tets
     const double* xyz;     // batch: 12 x n SoA coordinates
     const double* coords;  // mesh: node coordinates (n_nodes x 3) ...
@@ -347,37 +352,56 @@ struct ApplyArgs {
     double* out;           // opU[r*dim*q + k + dim*n]
 };
 
-// One lane per (tet, point): Op(u)(x_n) = sum_i U(k, n, i) dof_i, with
-// U = PSI^T Ghat for GRAD (component k + 3c = du_c/dx_k for a vector space).
+// One lane per (tet, point): Op(u)(x_n) = sum_i s_i T U_i(x_n) dof_i with the
+// reference values U_i tabulated (fem3DapplyL, barycentric points common to
+// all tets) or evaluated here at the barycentric coordinates of the tet's own
+// physical point (fem3DapplyX: l_{k+1} = sum_j PSI(k, j) (x_j - P0_j),
+// l_0 = 1 - l_1 - l_2 - l_3, core.inl:120-137); T = 1, PSI^T or J and s_i the
+// per-DOF factor of RT0 / ND0 (component k + 3c = du_c/dx_k for GRAD of a
+// vector space).
 template <bool MESH>
 __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
     const int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (idx >= a.n * a.q) return;
     const int64_t r = idx / a.q;
     const int pt = (int)(idx % a.q);
-    const int nf = a.nf0 * a.ncomp, dim = (a.grad ? 3 : 1) * a.ncomp;
-    double PSI[9];
-    if (a.grad) {
-        double P[4][3], det;
+    const int nf = a.nf0 * a.ncomp, d = a.d, dim = d * a.ncomp;
+    double P[4][3], PSI[9], JT[9], det = 1.0;
+    if (a.map != 0 || a.scale != 0 || a.X) {
         for (int v = 0; v < 4; ++v)
             for (int c = 0; c < 3; ++c)
                 P[v][c] = MESH ? a.coords[3 * (int64_t)a.tetnodes[(int64_t)v * a.n + r] + c]
                                : a.xyz[(int64_t)(3 * v + c) * a.n + r];
         tet_geometry(P, PSI, det);
+        tet_jacobian_t(P, JT);
+    }
+    double loc[60];
+    const double* T = a.tab ? a.tab + (int64_t)pt * a.nf0 * d : loc;
+    if (a.X) {
+        const double* x = a.X + (r * a.q + pt) * 3;
+        const double dr[3] = {x[0] - P[0][0], x[1] - P[0][1], x[2] - P[0][2]};
+        double L[4];
+        for (int i = 0; i < 3; ++i) L[i + 1] = PSI[i] * dr[0] + PSI[i + 3] * dr[1] + PSI[i + 6] * dr[2];
+        L[0] = 1.0 - L[1] - L[2] - L[3];
+        if (a.fem == ANIFEM_FEM_RT0 || a.fem == ANIFEM_FEM_ND0) vector_field(a.op, a.fem, L, loc);
+        else if (a.op == ANIFEM_GRAD) basis_gradients(a.fem, L, loc);
+        else basis_values(a.fem, L, loc);
     }
     double* o = a.out + r * dim * a.q + (int64_t)dim * pt;
     for (int c = 0; c < a.ncomp; ++c) {
         double g[3] = {0.0, 0.0, 0.0};
-        const int d = a.grad ? 3 : 1;
         for (int i = 0; i < a.nf0; ++i) {
             const int64_t e = r * nf + c * a.nf0 + i;
-            const double dv = MESH ? a.u[a.dofmap[e]] : a.dofs[e];
-            for (int j = 0; j < d; ++j) g[j] += a.tab[(pt * a.nf0 + i) * d + j] * dv;
+            double dv = MESH ? a.u[a.dofmap[e]] : a.dofs[e];
+            if (a.scale) dv *= dof_scale(a.scale, P, det, i);
+            for (int j = 0; j < d; ++j) g[j] += T[i * d + j] * dv;
         }
-        if (a.grad) {
+        if (a.map == 1) {
             for (int k = 0; k < 3; ++k) o[k + 3 * c] = PSI[0 + 3 * k] * g[0] + PSI[1 + 3 * k] * g[1] + PSI[2 + 3 * k] * g[2];
+        } else if (a.map == 2) {
+            for (int k = 0; k < 3; ++k) o[k + 3 * c] = JT[0 + 3 * k] * g[0] + JT[1 + 3 * k] * g[1] + JT[2 + 3 * k] * g[2];
         } else {
-            o[c] = g[0];
+            for (int j = 0; j < d; ++j) o[j + d * c] = g[j];
         }
     }
 }

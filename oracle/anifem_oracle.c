@@ -772,6 +772,38 @@ int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n,
     return 0;
 }
 
+/* fem3DapplyX (eval.inl:103-125): q physical points per tet X[(r*q + n)*3 + c],
+ * their barycentric coordinates from getBaryCoord (core.inl:120-137), then
+ * fem3DapplyL at them; out as orc_fem3dapply. */
+int orc_fem3dapplyx(int op, int fem, int ncomp, int q, long n, const double* xyz, const double* X,
+                    const double* dofs, double* out) {
+    int nf = orc_nfa(fem, ncomp), dim = orc_opdim(op, fem, ncomp);
+    if (nf <= 0 || q <= 0 || !orc_op_ok(op, fem, ncomp)) return -1;
+    double* XYL = malloc(sizeof(double) * 4 * (size_t)q);
+    for (long r = 0; r < n; ++r) {
+        const double* P = xyz + 12 * r;
+        double XYP[9], PSI[9];
+        for (int l = 0; l < 3; ++l)
+            for (int i = 0; i < 3; ++i) XYP[i + 3 * l] = P[3 * (l + 1) + i] - P[i];
+        orc_inverse3x3(XYP, PSI);
+        for (int m = 0; m < q; ++m) {
+            const double* x = X + 3 * ((size_t)q * r + m);
+            double dr[3] = {x[0] - P[0], x[1] - P[1], x[2] - P[2]};
+            for (int i = 0; i < 3; ++i) XYL[4 * m + i + 1] = 0;
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) XYL[4 * m + i + 1] += PSI[i + 3 * j] * dr[j];
+            XYL[4 * m] = 1 - XYL[4 * m + 1] - XYL[4 * m + 2] - XYL[4 * m + 3];
+        }
+        int rc = orc_fem3dapply(op, fem, ncomp, q, XYL, 1, P, dofs + (size_t)nf * r, out + (size_t)dim * q * r);
+        if (rc) {
+            free(XYL);
+            return rc;
+        }
+    }
+    free(XYL);
+    return 0;
+}
+
 void orc_apply_dir(int nf, double* A, double* F, int k, double bc) {
     for (int i = 0; i < nf; ++i) F[i] -= A[i + nf * k] * bc;
     F[k] = bc;

@@ -76,6 +76,9 @@ int orc_fem3dface(int opA, int femA, int ncA, int opB, int femB, int ncB, int te
  * space's degree from their nodal interpolants. */
 int orc_fem3dapply(int op, int fem, int ncomp, int q, const double* xyl, long n, const double* xyz,
                    const double* dofs, double* out);
+/* fem3DapplyX: q physical points per tet, X[(r*q + n)*3 + c]. */
+int orc_fem3dapplyx(int op, int fem, int ncomp, int q, long n, const double* xyz, const double* X,
+                    const double* dofs, double* out);
 
 /* Number of basis functions / operator dimension for (op, fem, ncomp). */
 int orc_nfa(int fem, int ncomp);

@@ -149,6 +149,61 @@ int run_apply(int q, const double* xyl, long n, const double* xyz, const double*
     return 0;
 }
 
+template <typename Op>
+int run_applyx(int q, long n, const double* xyz, const double* X, const double* dofs, double* out) {
+    constexpr long nfa = Op::Nfa::value, dim = Op::Dim::value;
+    std::vector<double> dv(nfa), o(dim * q);
+    for (long r = 0; r < n; ++r) {
+        const double* p = xyz + 12 * r;
+        Tetra<const double> XYZ(p, p + 3, p + 6, p + 9);
+        std::copy(dofs + nfa * r, dofs + nfa * (r + 1), dv.begin());
+        fem3DapplyX<Op>(XYZ, ArrayView<const double>(X + 3 * q * r, 3 * q), ArrayView<double>(dv.data(), nfa),
+                        ArrayView<double>(o.data(), dim * q));
+        std::copy(o.begin(), o.end(), out + dim * q * r);
+    }
+    return 0;
+}
+
+template <typename Op>
+int run_apply_any(bool at_x, int q, const double* pts, long n, const double* xyz, const double* dofs, double* out) {
+    return at_x ? run_applyx<Op>(q, n, xyz, pts, dofs, out) : run_apply<Op>(q, pts, n, xyz, dofs, out);
+}
+
+int apply_switch(int code, bool at_x, int q, const double* pts, long n, const double* xyz, const double* dofs,
+                 double* out) {
+    using P1 = FemFix<FEM_P1>;
+    using P2 = FemFix<FEM_P2>;
+    using P3 = FemFix<FEM_P3>;
+    using V2 = FemVec<3, FEM_P2>;
+    using RT0 = FemFix<FEM_RT0>;
+    using ND0 = FemFix<FEM_ND0>;
+    using CR1 = FemFix<FEM_CR1>;
+    using B2 = FemFix<FEM_B2>;
+    using B4 = FemFix<FEM_B4>;
+    try {
+        switch (code) {
+            case 0: return run_apply_any<Operator<IDEN, P1>>(at_x, q, pts, n, xyz, dofs, out);
+            case 1: return run_apply_any<Operator<IDEN, P2>>(at_x, q, pts, n, xyz, dofs, out);
+            case 2: return run_apply_any<Operator<GRAD, P2>>(at_x, q, pts, n, xyz, dofs, out);
+            case 3: return run_apply_any<Operator<IDEN, P3>>(at_x, q, pts, n, xyz, dofs, out);
+            case 4: return run_apply_any<Operator<GRAD, P3>>(at_x, q, pts, n, xyz, dofs, out);
+            case 5: return run_apply_any<Operator<IDEN, V2>>(at_x, q, pts, n, xyz, dofs, out);
+            case 6: return run_apply_any<Operator<GRAD, V2>>(at_x, q, pts, n, xyz, dofs, out);
+            case 7: return run_apply_any<Operator<IDEN, RT0>>(at_x, q, pts, n, xyz, dofs, out);
+            case 8: return run_apply_any<Operator<DIV, RT0>>(at_x, q, pts, n, xyz, dofs, out);
+            case 9: return run_apply_any<Operator<IDEN, ND0>>(at_x, q, pts, n, xyz, dofs, out);
+            case 10: return run_apply_any<Operator<CURL, ND0>>(at_x, q, pts, n, xyz, dofs, out);
+            case 11: return run_apply_any<Operator<IDEN, CR1>>(at_x, q, pts, n, xyz, dofs, out);
+            case 12: return run_apply_any<Operator<GRAD, B2>>(at_x, q, pts, n, xyz, dofs, out);
+            case 13: return run_apply_any<Operator<GRAD, B4>>(at_x, q, pts, n, xyz, dofs, out);
+            default: break;
+        }
+    } catch (const std::exception&) {
+        return -2;
+    }
+    return -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -167,28 +222,16 @@ int ref_triangle_quadrature(int order, double* p, double* w) {
 }
 
 // fem3DapplyL (anifem++/fem/operations/eval.inl:84-100) per tet.  Op codes:
-// 0 IDEN P1, 1 IDEN P2, 2 GRAD P2, 3 IDEN P3, 4 GRAD P3, 5 IDEN V2, 6 GRAD V2.
+// 0 IDEN P1, 1 IDEN P2, 2 GRAD P2, 3 IDEN P3, 4 GRAD P3, 5 IDEN V2, 6 GRAD V2,
+// 7 IDEN RT0, 8 DIV RT0, 9 IDEN ND0, 10 CURL ND0, 11 IDEN CR1, 12 GRAD B2, 13 GRAD B4.
 // xyl: 4q barycentric points; dofs n x nfa; out n x dim*q.
 int ref_fem3dapply(int code, int q, const double* xyl, long n, const double* xyz, const double* dofs, double* out) {
-    using P1 = FemFix<FEM_P1>;
-    using P2 = FemFix<FEM_P2>;
-    using P3 = FemFix<FEM_P3>;
-    using V2 = FemVec<3, FEM_P2>;
-    try {
-        switch (code) {
-            case 0: return run_apply<Operator<IDEN, P1>>(q, xyl, n, xyz, dofs, out);
-            case 1: return run_apply<Operator<IDEN, P2>>(q, xyl, n, xyz, dofs, out);
-            case 2: return run_apply<Operator<GRAD, P2>>(q, xyl, n, xyz, dofs, out);
-            case 3: return run_apply<Operator<IDEN, P3>>(q, xyl, n, xyz, dofs, out);
-            case 4: return run_apply<Operator<GRAD, P3>>(q, xyl, n, xyz, dofs, out);
-            case 5: return run_apply<Operator<IDEN, V2>>(q, xyl, n, xyz, dofs, out);
-            case 6: return run_apply<Operator<GRAD, V2>>(q, xyl, n, xyz, dofs, out);
-            default: break;
-        }
-    } catch (const std::exception&) {
-        return -2;
-    }
-    return -1;
+    return apply_switch(code, false, q, xyl, n, xyz, dofs, out);
+}
+
+// fem3DapplyX at q physical points per tet, X[(r*q + n)*3 + c]; codes as ref_fem3dapply.
+int ref_fem3dapplyx(int code, int q, const double* X, long n, const double* xyz, const double* dofs, double* out) {
+    return apply_switch(code, true, q, X, n, xyz, dofs, out);
 }
 
 // Face pair codes (OpA, OpB, dot_normal): 0=(IDEN P1, IDEN P1); 1=(IDEN P2, IDEN P2); 2=(IDEN P3, IDEN P3);

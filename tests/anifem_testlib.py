@@ -48,7 +48,9 @@ def nfa(fem, nc=1):
     return NF[fem] * nc
 
 
-def dim(op, nc=1):
+def dim(op, nc=1, fem=None):
+    if fem in (21, 25):  # RT0, ND0: IDEN and CURL 3, DIV 1
+        return 1 if op == 3 else 3
     return (3 if op == 2 else 1) * nc
 
 
@@ -104,7 +106,7 @@ def points_physical(xyz, p):
 def orc_apply(op, fem, nc, xyl, xyz, dofs):
     """Oracle fem3DapplyL: (n, q, dim)."""
     n, q = len(xyz), len(xyl)
-    d = dim(op, nc)
+    d = dim(op, nc, fem)
     out = np.zeros(n * q * d)
     xyl = np.ascontiguousarray(xyl, dtype=np.float64)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -113,6 +115,28 @@ def orc_apply(op, fem, nc, xyl, xyz, dofs):
     if rc != 0:
         raise RuntimeError(f"oracle fem3dapply rc={rc}")
     return out.reshape(n, q, d)
+
+
+def orc_applyx(op, fem, nc, X, xyz, dofs):
+    """Oracle fem3DapplyX at per-tet physical points X (n, q, 3): (n, q, dim)."""
+    n, q = X.shape[0], X.shape[1]
+    d = dim(op, nc, fem)
+    out = np.zeros(n * q * d)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    dofs = np.ascontiguousarray(dofs, dtype=np.float64)
+    rc = oracle().orc_fem3dapplyx(op, fem, nc, q, ctypes.c_long(n), P(xyz), P(X), P(dofs), P(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle fem3dapplyx rc={rc}")
+    return out.reshape(n, q, d)
+
+
+def applyx_golden_cases():
+    out = []
+    for f in sorted(glob.glob(os.path.join(GOLDEN_DIR, "applyx_*.npz"))):
+        z = np.load(f)
+        out.append((os.path.basename(f)[7:-4], {k: z[k] for k in z.files}))
+    return out
 
 
 def apply_golden_cases():

@@ -212,30 +212,48 @@ def write_faces(L, rng):
         print(f"face {name}: {n} tets x 4 faces, {nfb}x{nfa}")
 
 
-# fem3DapplyL op codes (ref_fem3dapply) -> (op, fem, ncomp)
-APPLY_OPS = {0: (1, 2, 1), 1: (1, 3, 1), 2: (2, 3, 1), 3: (1, 4, 1), 4: (2, 4, 1), 5: (1, 3, 3), 6: (2, 3, 3)}
+# fem3DapplyL / fem3DapplyX op codes (ref_fem3dapply, ref_fem3dapplyx) -> (op, fem, ncomp)
+APPLY_OPS = {0: (1, 2, 1), 1: (1, 3, 1), 2: (2, 3, 1), 3: (1, 4, 1), 4: (2, 4, 1), 5: (1, 3, 3), 6: (2, 3, 3),
+             7: (1, 21, 1), 8: (3, 21, 1), 9: (1, 25, 1), 10: (4, 25, 1), 11: (1, 31, 1), 12: (2, 1002, 1),
+             13: (2, 1004, 1)}
 
 
-def write_apply(L, rng):
+def op_dim(op, fem, nc):
+    if fem in (21, 25):
+        return 1 if op == 3 else 3
+    return (3 if op == 2 else 1) * nc
+
+
+def write_apply(L, rng, codes, at_x=False):
     dp = ctypes.POINTER(ctypes.c_double)
-    L.ref_fem3dapply.restype = ctypes.c_int
+    fn = L.ref_fem3dapplyx if at_x else L.ref_fem3dapply
+    fn.restype = ctypes.c_int
     n, q = 24, 7
     xyz = np.ascontiguousarray(kuhn_tets(rng, n))
     lam = rng.dirichlet(np.ones(4), q)  # interior barycentric points
     xyl = np.ascontiguousarray(lam)
-    for code, (op, fem, nc) in APPLY_OPS.items():
+    if at_x:
+        # per tet its own q physical points (inside and, for the last two, outside the tet)
+        lam_t = rng.dirichlet(np.ones(4), (n, q))
+        lam_t[:, -2:] = lam_t[:, -2:] * 1.5 - 0.125
+        X = np.ascontiguousarray(np.einsum("rqv,rvc->rqc", lam_t, xyz.reshape(n, 4, 3)))
+    for code in codes:
+        op, fem, nc = APPLY_OPS[code]
         nf = NF[fem] * nc
-        dim = (3 if op == 2 else 1) * nc
+        dim = op_dim(op, fem, nc)
         dofs = np.ascontiguousarray(rng.normal(size=(n, nf)))
         out = np.zeros(n * dim * q)
-        rc = L.ref_fem3dapply(code, q, xyl.ctypes.data_as(dp), ctypes.c_long(n), xyz.ctypes.data_as(dp),
-                              dofs.ctypes.data_as(dp), out.ctypes.data_as(dp))
+        pts = X if at_x else xyl
+        rc = fn(code, q, pts.ctypes.data_as(dp), ctypes.c_long(n), xyz.ctypes.data_as(dp), dofs.ctypes.data_as(dp),
+                out.ctypes.data_as(dp))
         if rc != 0:
             sys.exit(f"apply {code}: reference harness returned {rc}")
         meta = np.array([code, op, fem, nc], dtype=np.int64)
-        np.savez_compressed(os.path.join(HERE, f"apply_{code}.npz"), meta=meta, xyz=xyz, xyl=xyl, dofs=dofs,
-                            out=out.reshape(n, q, dim))
-        print(f"apply {code}: {n} tets x {q} points, dim {dim}")
+        name = f"applyx_{code}.npz" if at_x else f"apply_{code}.npz"
+        extra = dict(X=X) if at_x else dict(xyl=xyl)
+        np.savez_compressed(os.path.join(HERE, name), meta=meta, xyz=xyz, dofs=dofs, out=out.reshape(n, q, dim),
+                            **extra)
+        print(f"{name}: {n} tets x {q} points, dim {dim}")
 
 
 def main():
@@ -245,7 +263,7 @@ def main():
     L.ref_fem3dtet.restype = ctypes.c_int
     dp = ctypes.POINTER(ctypes.c_double)
     # groups to write (default all): elem (with the operator and space cases), faces, apply;
-    # "spaces" writes the space cases only
+    # "spaces" writes the cases of the RT0 / ND0 / CR1 / B2 / B4 spaces and fem3DapplyX only
     groups = set(sys.argv[1:]) or {"elem", "faces", "apply"}
     rng = np.random.default_rng(20260101)
     todo = []
@@ -275,7 +293,10 @@ def main():
     if "faces" in groups:
         write_faces(L, np.random.default_rng(20260202))
     if "apply" in groups:
-        write_apply(L, np.random.default_rng(20260303))
+        write_apply(L, np.random.default_rng(20260303), range(7))
+    if "apply" in groups or "spaces" in groups:
+        write_apply(L, np.random.default_rng(20260606), range(7, 14))
+        write_apply(L, np.random.default_rng(20260707), range(14), at_x=True)
 
 
 if __name__ == "__main__":

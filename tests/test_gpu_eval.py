@@ -28,6 +28,35 @@ def test_apply_batch_vs_reference_fixture(case):
     assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max(), case
 
 
+@pytest.mark.parametrize("case", [c[0] for c in T.applyx_golden_cases()])
+def test_applyx_batch_vs_reference_fixture(case):
+    """anifem_fem3dapplyx_batch (fem3DapplyX) at the fixtures' per-tet physical
+    points against the compiled reference."""
+    z = dict(T.applyx_golden_cases())[case]
+    code, op, fem, nc = [int(v) for v in z["meta"]]
+    out = af.fem3dapplyx_batch(af.Op(op, fem, nc), dev(z["X"]), dev(T.soa_xyz(z["xyz"])), dev(z["dofs"]))
+    out = out.cpu().numpy()
+    ref = z["out"]
+    assert out.shape == ref.shape
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max(), case
+
+
+def test_applyx_matches_applyl_at_common_points():
+    """At physical points that are the same barycentric points in every tet,
+    the X form equals the L form (up to the rounding of the barycentric
+    coordinates) for P3 gradients and ND0 curls."""
+    rng = np.random.default_rng(9)
+    n, q = 200, 6
+    xyz = rng.normal(size=(n, 12))
+    xyl = rng.dirichlet(np.ones(4), q)
+    X = T.points_physical(xyz, xyl)
+    for op in (af.Op(af.GRAD, af.FEM_P3), af.Op(af.CURL, af.FEM_ND0), af.Op(af.IDEN, af.FEM_RT0)):
+        dofs = dev(rng.normal(size=(n, op.nfa)))
+        a = af.fem3dapply_batch(op, xyl, dev(T.soa_xyz(xyz)), dofs).cpu().numpy()
+        b = af.fem3dapplyx_batch(op, dev(X), dev(T.soa_xyz(xyz)), dofs).cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
+
+
 def quadratic(c):
     f = lambda x: (c[0] + c[1] * x[..., 0] + c[2] * x[..., 1] + c[3] * x[..., 2] + c[4] * x[..., 0] * x[..., 1] +
                    c[5] * x[..., 2] ** 2 + c[6] * x[..., 1] * x[..., 2])

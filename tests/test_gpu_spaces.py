@@ -87,8 +87,8 @@ def test_apply_polynomial_spaces_vs_oracle(op, fem):
 
 
 def test_refused_cases():
-    """GRAD of RT0 / ND0, vector copies of them, the MFMA mode, their
-    evaluation and the global assembly of any space beyond P0..P3 return
+    """GRAD of RT0 / ND0 (elements and evaluation), vector copies of them, the
+    MFMA mode and the global assembly of any space beyond P0..P3 return
     ANIFEM_ERR_ARG."""
     import torch
     L = af.lib()
@@ -105,7 +105,7 @@ def test_refused_cases():
     assert call(af.Op(af.IDEN, af.FEM_ND0, 3), af.Op(af.IDEN, af.FEM_ND0, 3), af.PREC_FP64) == af.ERR_ARG
     assert af.Op(af.GRAD, af.FEM_ND0).nfa == -1
     with pytest.raises(af.AnifemError):
-        af.fem3dapply_batch(rt, np.full((1, 4), 0.25), xyz, dev(np.zeros((4, 4))))
+        af.fem3dapply_batch(af.Op(af.GRAD, af.FEM_RT0), np.full((1, 4), 0.25), xyz, dev(np.zeros((4, 4))))
     coords, tets = af.kuhn_cube(2, 0.1, 1)
     for fem in (af.FEM_CR1, af.FEM_B2, af.FEM_RT0):
         op = af.Op(af.IDEN if fem == af.FEM_RT0 else af.GRAD, fem)

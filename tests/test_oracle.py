@@ -306,14 +306,28 @@ def test_face_rule_exactness_and_reference_rules():
 
 def test_apply_fixtures_vs_oracle():
     """fem3DapplyL of the reference element core compiled here (IDEN/GRAD of
-    P1-P3 and of FemVec<3, P2>) at interior points: the oracle agrees."""
+    P1-P3 and of FemVec<3, P2>; RT0 values and divergences, ND0 values and
+    curls, CR1, B2, B4) at interior points: the oracle agrees."""
     cases = T.apply_golden_cases()
-    assert len(cases) == 7
+    assert len(cases) == 14
     for name, z in cases:
         code, op, fem, nc = [int(v) for v in z["meta"]]
         out = T.orc_apply(op, fem, nc, z["xyl"], z["xyz"], z["dofs"])
         err = np.abs(out - z["out"]).max() / np.abs(z["out"]).max()
         assert err < 1e-13, (name, err)
+
+
+def test_applyx_fixtures_vs_oracle():
+    """fem3DapplyX of the compiled reference (per-tet physical points, two of
+    them outside the tet: the polynomial extension is evaluated there): the
+    oracle's getBaryCoord + fem3DapplyL agrees, every operator code."""
+    cases = T.applyx_golden_cases()
+    assert len(cases) == 14
+    for name, z in cases:
+        code, op, fem, nc = [int(v) for v in z["meta"]]
+        out = T.orc_applyx(op, fem, nc, z["X"], z["xyz"], z["dofs"])
+        err = np.abs(out - z["out"]).max() / np.abs(z["out"]).max()
+        assert err < 1e-12, (name, err)
 
 
 @pytest.mark.parametrize("fem,deg", [(P1, 1), (P2, 2), (P3, 3)])
