@@ -151,6 +151,20 @@ inline int fem3DfaceN(const double* xyz, int64_t n_tets, int face_num, const Ten
     return anifem_fem3dface_batch(OpA::c(), OpB::c(), &t, order, face_num, 1, n_tets, xyz, precision, A_out, stream);
 }
 
+// Batched fem3DapplyL / fem3DapplyX (eval.h:13-31): Op(u) at q barycentric
+// points common to all tets (xyl, host) or at q physical points per tet
+// (X, device, X[(r*q + n)*3 + c]); dofs n x nfa, opU n x q x dim (device).
+template <typename Op>
+inline int fem3DapplyL(const double* xyz, int64_t n_tets, const double* xyl, int q, const double* dofs, double* opU,
+                       void* stream = nullptr) {
+    return anifem_fem3dapply_batch(Op::c(), xyl, q, n_tets, xyz, dofs, opU, stream);
+}
+template <typename Op>
+inline int fem3DapplyX(const double* xyz, int64_t n_tets, const double* X, int q, const double* dofs, double* opU,
+                       void* stream = nullptr) {
+    return anifem_fem3dapplyx_batch(Op::c(), q, n_tets, xyz, X, dofs, opU, stream);
+}
+
 // Nfa / Dim of an operator (poly_*.h, operators.h:127-155)
 template <typename Op>
 inline int nfa() {
