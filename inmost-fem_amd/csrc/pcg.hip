@@ -213,23 +213,28 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
     const int lane = threadIdx.x % L;
     // loads that do not depend on beta: the first row pass's entries and gathers
     const int64_t row0 = (int64_t)blockIdx.x * rows_per_block + threadIdx.x / L;
-    int64_t q0 = 0, q1 = 0;
-    if (row0 < n) {
-        q0 = rp[row0];
-        q1 = rp[row0 + 1];
-    }
+    // a row's first kPre entries per lane: all loads issued before any is used
     double ev[kPre], ez[kPre], ep[kPre];
-#pragma unroll
-    for (int j = 0; j < kPre; ++j) {
-        const int64_t q = q0 + lane + (int64_t)j * L;
-        ev[j] = ez[j] = ep[j] = 0;
-        if (q < q1) {
-            const int32_t c = ci[q];
-            ev[j] = v[q];
-            ez[j] = z[c];
-            if (!first) ep[j] = p_prev[c];
+    auto load_row = [&](int64_t row, int64_t& qa, int64_t& qb) {
+        qa = qb = 0;
+        if (row < n) {
+            qa = rp[row];
+            qb = rp[row + 1];
         }
-    }
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int64_t q = qa + lane + (int64_t)j * L;
+            ev[j] = ez[j] = ep[j] = 0;
+            if (q < qb) {
+                const int32_t c = ci[q];
+                ev[j] = v[q];
+                ez[j] = z[c];
+                if (!first) ep[j] = p_prev[c];
+            }
+        }
+    };
+    int64_t q0, q1;
+    load_row(row0, q0, q1);
     double zr = 0, pr = 0;
     if (row0 < n && lane == 0) {
         zr = z[row0];
@@ -271,14 +276,13 @@ __global__ __launch_bounds__(kThreads) void dir_spmv_kernel(int64_t n, const int
     for (int64_t base = (int64_t)blockIdx.x * rows_per_block; base < n; base += (int64_t)gridDim.x * rows_per_block) {
         const int64_t row = base + threadIdx.x / L;
         const bool pre = row == row0;  // the first pass, loaded above
+        if (!pre) load_row(row, q0, q1);
         double s = 0;
-        if (pre) {
 #pragma unroll
-            for (int j = 0; j < kPre; ++j) s += ev[j] * (first ? ez[j] : fma(beta, ep[j], ez[j]));
-        }
+        for (int j = 0; j < kPre; ++j) s += ev[j] * (first ? ez[j] : fma(beta, ep[j], ez[j]));
         if (row < n) {
-            const int64_t qs = pre ? q0 + lane + (int64_t)kPre * L : rp[row] + lane;
-            const int64_t qe = rp[row + 1];
+            const int64_t qs = q0 + lane + (int64_t)kPre * L;
+            const int64_t qe = q1;
             for (int64_t q = qs; q < qe; q += L) {
                 const int32_t c = ci[q];
                 s += v[q] * (first ? z[c] : fma(beta, p_prev[c], z[c]));
@@ -617,6 +621,33 @@ int red_grid(int64_t n, int L) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, kRedBlocks));
 }
 
+// A grid-stride launch of no more workgroups than are resident at once: a
+// second, partial round would leave most CUs idle while it runs.
+int resident_grid(const void* kernel, int grid) {
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kThreads, 0) == hipSuccess && occ > 0 && ncu > 0)
+        grid = (int)std::min<int64_t>(grid, (int64_t)occ * ncu);
+    return std::max(grid, 1);
+}
+
+const void* dir_kernel_of(int L) {
+    if (L == 2) return reinterpret_cast<const void*>(&dir_spmv_kernel<2>);
+    if (L == 4) return reinterpret_cast<const void*>(&dir_spmv_kernel<4>);
+    if (L == 8) return reinterpret_cast<const void*>(&dir_spmv_kernel<8>);
+    if (L == 16) return reinterpret_cast<const void*>(&dir_spmv_kernel<16>);
+    return reinterpret_cast<const void*>(&dir_spmv_kernel<32>);
+}
+
+const void* update_matrix_kernel_of(int L) {
+    if (L == 2) return reinterpret_cast<const void*>(&update_matrix_kernel<2>);
+    if (L == 4) return reinterpret_cast<const void*>(&update_matrix_kernel<4>);
+    if (L == 8) return reinterpret_cast<const void*>(&update_matrix_kernel<8>);
+    if (L == 16) return reinterpret_cast<const void*>(&update_matrix_kernel<16>);
+    return reinterpret_cast<const void*>(&update_matrix_kernel<32>);
+}
+
 void launch_dir(int L, int grid, int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* z,
                 const double* p_prev, double* p, double* Ap, double* partA, const double* partU, int nU, int flexible,
                 PcgState* st, int64_t k, int check_only, int* hdone, hipStream_t s) {
@@ -795,7 +826,7 @@ extern "C" int anifem_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_
     spmv_dispatch(n, nnz, row_ptr, col_idx, val, x, Ap, nullptr, nullptr, s);
     hipLaunchKernelGGL(init_kernel, dim3(vb), dim3(kThreads), 0, s, n, b, Ap, dinv, r, z, pb[1], part);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, (int)vb, 0, st, rtol);
-    const int L = lanes_per_row(n, nnz), nD = red_grid(n, L);
+    const int L = lanes_per_row(n, nnz), nD = resident_grid(dir_kernel_of(L), red_grid(n, L));
     int rc = run_iterations(
         maxit, st,
         [&](int64_t k, int check_only, int* hdone) {
@@ -845,8 +876,8 @@ extern "C" int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, (int)vb, 3, st, rtol);
     const int pz = spmv_dispatch(n, mnnz, m_row_ptr, m_col_idx, m_val, rb[0], z, part, st, s);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, s, part, pz, 5, st, rtol);
-    const int L = lanes_per_row(n, nnz), nD = red_grid(n, L);
-    const int LM = lanes_per_row(n, mnnz), nU = red_grid(n, LM);
+    const int L = lanes_per_row(n, nnz), nD = resident_grid(dir_kernel_of(L), red_grid(n, L));
+    const int LM = lanes_per_row(n, mnnz), nU = resident_grid(update_matrix_kernel_of(LM), red_grid(n, LM));
     int rc = run_iterations(
         maxit, st,
         [&](int64_t k, int check_only, int* hdone) {
