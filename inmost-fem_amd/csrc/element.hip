@@ -81,7 +81,12 @@ static int launch_one(ElemArgs a, int cap, hipStream_t s) {
     if (lds > 64 * 1024) return ANIFEM_ERR_ARG;
     const int64_t blocks = (a.n + a.tile - 1) / a.tile;
     if (blocks == 0) return ANIFEM_OK;
-    hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), lds, s, a);
+    if (a.mapA | a.mapB | a.scA | a.scB)
+        hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T, true>), dim3((unsigned)blocks), dim3(kBlock), lds, s,
+                           a);
+    else
+        hipLaunchKernelGGL((element_kernel<DA0, DB0, NCA, NCB, T, false>), dim3((unsigned)blocks), dim3(kBlock), lds,
+                           s, a);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }

@@ -263,9 +263,11 @@ __host__ __device__ inline size_t element_scale_offset(int tile, int kstride, in
 
 // Element matrices of a tile of tets, stored in T precision to a.out
 // (n_tets matrices of nfB x nfA, column-major, one after the other).
-// Vector-valued bases (RT0, ND0) carry a per-tet factor per basis function,
-// applied to the contracted entry: A(ib, ia) = sB_ib sA_ia sum_kk K R.
-template <int DA0, int DB0, int NCA, int NCB, typename T>
+// PIOLA (RT0, ND0 on either side): the J map of geometry_block, and a per-tet
+// factor per basis function applied to the contracted entry,
+// A(ib, ia) = sB_ib sA_ia sum_kk K R; a separate instantiation keeps the
+// registers of the other spaces' kernel unchanged.
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool PIOLA = false>
 __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     constexpr int KB = DA0 * DB0;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -273,20 +275,26 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
     const int kstride = NCA * NCB * a.q * KB;
     const int64_t tile0 = (int64_t)blockIdx.x * a.tile;
     const int ntile = (int)((a.n - tile0) < a.tile ? (a.n - tile0) : a.tile);
-    const bool scaled = (a.scA | a.scB) != 0;
+    const bool scaled = PIOLA && (a.scA | a.scB) != 0;
     const int ns = a.nfA0 + a.nfB0;
     double* Ss = reinterpret_cast<double*>(smem_raw + element_scale_offset(a.tile, kstride, (int)sizeof(T)));
 
     for (int t = threadIdx.x; t < ntile; t += kBlock) {
         const int64_t r = tile0 + t;
-        double P[4][3], PSI[9], det, nrm[3], JT[9];
+        double P[4][3], PSI[9], det, nrm[3];
         load_tet<false>(a, r, P);
         tet_geometry(P, PSI, det);
-        tet_jacobian_t(P, JT);
         // volume |det|/6, or the face's area (int_face.inl:40-43, 79-84)
         const double mes = a.face < 0 ? fabs(det) / 6.0 : face_measure(P, a.face, a.dot_normal ? nrm : nullptr);
-        geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride,
-                                               a.face >= 0 && a.dot_normal ? nrm : nullptr, JT);
+        if constexpr (PIOLA) {
+            double JT[9];
+            tet_jacobian_t(P, JT);
+            geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride,
+                                                   a.face >= 0 && a.dot_normal ? nrm : nullptr, JT);
+        } else {
+            geometry_tensor<DA0, DB0, NCA, NCB, T>(a, r, PSI, mes, Ks + (int64_t)t * kstride,
+                                                   a.face >= 0 && a.dot_normal ? nrm : nullptr);
+        }
         if (scaled) {
             double* St = Ss + (int64_t)t * ns;
             for (int i = 0; i < a.nfA0; ++i) St[i] = a.scA ? dof_scale(a.scA, P, det, i) : 1.0;
@@ -324,9 +332,11 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
                 const T* Rn = R + (int64_t)rr * a.q * KB;
                 for (int j = 0; j < a.q * KB; ++j) acc += Kt[j] * Rn[j];
             }
-            if (scaled) {
-                const double* St = Ss + (int64_t)t * ns;
-                acc = (T)((double)acc * (St[ia0] * St[a.nfA0 + ib0]));
+            if constexpr (PIOLA) {
+                if (scaled) {
+                    const double* St = Ss + (int64_t)t * ns;
+                    acc = (T)((double)acc * (St[ia0] * St[a.nfA0 + ib0]));
+                }
             }
             static_cast<T*>(a.out)[(tile0 + t) * E + e] = acc;
         }
