@@ -279,8 +279,8 @@ def main():
     ap.add_argument("--learned-eval", type=int, default=200, help="held-out systems (learned preconditioner)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; runs on several GPUs are not provided "
+              "(the owner-computes row blocks shard by owned rows with one ghost layer of tets)", file=sys.stderr)
         sys.exit(2)
 
     import torch

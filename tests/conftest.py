@@ -14,7 +14,7 @@ def pytest_configure(config):
 
 def pytest_collection_modifyitems(config, items):
     # GPU tests are skipped where no HIP device is present (the CPU suite runs
-    # with -m "not gpu"; on the GPU box the device is required and present).
+    # with -m "not gpu"; on an MI355X the device is required and present).
     try:
         import torch
         have_gpu = torch.cuda.is_available()
