@@ -198,7 +198,7 @@ __device__ __forceinline__ void row_block_finish(const BlockArgs& a, int32_t r0,
             sum += v3;
         }
         for (; c < ce; ++c) sum += vals[c];
-        a.val[s0 + u] = sum;
+        __builtin_nontemporal_store(sum, &a.val[s0 + u]);  // written once, never re-read by the pass
     }
     // one lane per row: its pairs' load-vector entries, ascending tet order
     if (a.has_rhs) {
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                 sum += v3;
             }
             for (; c < ce; ++c) sum += vals[c];
-            a.val[M.s0 + u] = sum;
+            __builtin_nontemporal_store(sum, &a.val[M.s0 + u]);  // written once, never re-read by the pass
         }
         if (a.has_rhs) {
             for (int u = threadIdx.x; u < nr; u += kBlock) {
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
                 sum += v3;
             }
             for (; c < ce; ++c) sum += vals[c];
-            a.val[M.s0 + u] = sum;
+            __builtin_nontemporal_store(sum, &a.val[M.s0 + u]);  // written once, never re-read by the pass
         }
         if (a.has_rhs) {
             for (int u = threadIdx.x; u < nr; u += kBlock) {
@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_persistent(BlockArg
                 sum += v3;
             }
             for (; c < ce; ++c) sum += vals[c];
-            a.val[M.s0 + u] = sum;
+            __builtin_nontemporal_store(sum, &a.val[M.s0 + u]);  // written once, never re-read by the pass
         }
         if (a.has_rhs) {
             for (int u = threadIdx.x; u < nr; u += kBlock) {
