@@ -285,10 +285,16 @@ int anifem_assembly_coords(anifem_assembly* ctx, double** coords);
  * sum of its element contributions in ascending tet order, the order of the
  * reference's element loop; there are no atomics on values and repeated runs
  * are bitwise identical.
- * D and F use the problem's tensor kinds; F may be NULL (no rhs; rhs may be
- * NULL too).  dir_values (device, ndof, may be NULL = homogeneous) gives the
- * Dirichlet values.  If status is non-NULL the call synchronizes the stream
- * and fills it; otherwise it returns after enqueueing. */
+ * D and F use the problem's tensor kinds; F may be NULL (the load vectors
+ * are then zero; rhs may be NULL too).  dir_values (device FP64, ndof, may be
+ * NULL = homogeneous) gives the value of every Dirichlet DOF (the others are
+ * ignored); each cell then applies applyDir with those values
+ * (dc_on_dof.h:38-45, per cell as ex1.cpp:100-105): its load vector loses
+ * A(:, k) bc_k for each Dirichlet local DOF k and takes bc_k on row k, so a
+ * Dirichlet row's rhs is (incident tets) x bc_k.  The matrix does not depend
+ * on the values.  Values need the row-block scatter (ANIFEM_ERR_ARG with the
+ * coloured one).  If status is non-NULL the call synchronizes the stream and
+ * fills it; otherwise it returns after enqueueing. */
 int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
                         const double* dir_values, int precision, double* csr_val, double* rhs,
                         anifem_status* status, void* stream);

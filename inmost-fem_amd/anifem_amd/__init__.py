@@ -456,20 +456,23 @@ class Assembler:
         _check(lib().anifem_assembly_csr(self._h, ctypes.byref(rp), ctypes.byref(ci)), "csr")
         return rp.value, ci.value
 
-    def assemble(self, D=None, F=None, precision=PREC_FP64, val=None, rhs=None, status=True, stream=None):
-        """Assemble into device FP64 val (nnz) and rhs (ndof); returns (val, rhs, status)."""
+    def assemble(self, D=None, F=None, precision=PREC_FP64, val=None, rhs=None, status=True, stream=None,
+                 dir_values=None):
+        """Assemble into device FP64 val (nnz) and rhs (ndof); returns (val,
+        rhs, status).  dir_values: device FP64 (ndof) Dirichlet values of the
+        flagged DOFs (None: homogeneous), applied per cell as applyDir does."""
         import torch
         dev = torch.device("cuda", torch.cuda.current_device())
         if val is None:
             val = torch.empty(self.nnz, dtype=torch.float64, device=dev)
-        if rhs is None and F is not None:
+        if rhs is None and (F is not None or dir_values is not None):
             rhs = torch.empty(self.ndof, dtype=torch.float64, device=dev)
         tD = anifem_tensor(self.tensor_type, self.tensor_storage, _ptr(D))
         tF = anifem_tensor(self.rhs_tensor_type, self.rhs_tensor_storage, _ptr(F)) if F is not None else None
         st = anifem_status()
-        rc = lib().anifem_assembly_run(self._h, ctypes.byref(tD), ctypes.byref(tF) if tF is not None else None, None,
-                                       precision, _ptr(val), _ptr(rhs), ctypes.byref(st) if status else None,
-                                       _stream_handle(stream))
+        rc = lib().anifem_assembly_run(self._h, ctypes.byref(tD), ctypes.byref(tF) if tF is not None else None,
+                                       _ptr(dir_values), precision, _ptr(val), _ptr(rhs),
+                                       ctypes.byref(st) if status else None, _stream_handle(stream))
         if rc not in (OK, ERR_NONFINITE):
             raise AnifemError(rc, "anifem_assembly_run")
         return val, rhs, (st if status else None)
@@ -761,6 +764,22 @@ class Batch:
         v = [ctypes.c_int64() for _ in range(5)]
         _check(lib().anifem_batch_info(self._h, i, *[ctypes.byref(x) for x in v]), "anifem_batch_info")
         return dict(zip(["tet_offset", "dof_offset", "nnz_offset", "ndof", "nnz"], [x.value for x in v]))
+
+    def csr(self, i):
+        """Host copies (row_ptr, col_idx) of mesh i's own CSR pattern."""
+        inf = self.info(i)
+        rp, ci = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().anifem_batch_csr(self._h, i, ctypes.byref(rp), ctypes.byref(ci)), "anifem_batch_csr")
+        return copy_to_host(rp.value, inf["ndof"] + 1, np.int64), copy_to_host(ci.value, inf["nnz"], np.int32)
+
+    def csr_all(self):
+        """Host copies (row_ptr, col_idx) of every mesh's pattern, concatenated:
+        mesh i's row_ptr occupies ndof_i + 1 entries starting at 0, and its
+        columns are its own DOF numbers."""
+        rp, ci = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().anifem_batch_csr(self._h, 0, ctypes.byref(rp), ctypes.byref(ci)), "anifem_batch_csr")
+        return (copy_to_host(rp.value, self.ndof + self.n_meshes, np.int64),
+                copy_to_host(ci.value, self.nnz, np.int32))
 
     def tuning(self, fem):
         caps = np.zeros(3, np.int32)

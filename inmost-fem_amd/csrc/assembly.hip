@@ -57,8 +57,12 @@ struct anifem_assembly {
     bool coloured = false;
     int32_t* d_col_tets = nullptr;
     std::vector<int64_t> colour_ptr;
+    // inhomogeneous Dirichlet values: the rows they change (LiftRow records)
+    anifem::LiftRow* d_lift = nullptr;
+    int64_t n_lift = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     anifem_timers timers{};
+    const double* run_bc = nullptr;   // the running call's Dirichlet values (device), or null
 };
 
 namespace {
@@ -276,17 +280,33 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     return ANIFEM_OK;
 }
 
+// The row-block pass, then the rows an inhomogeneous Dirichlet condition
+// changes (when the call has Dirichlet values and a rhs).
+template <int DA0, int DB0, int NCA, int NCB, typename T>
+int launch_blocks_lift(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
+    using namespace anifem;
+    int rc = launch_blocks<DA0, DB0, NCA, NCB, T>(ctx, a, s);
+    if (rc || !ctx->run_bc || !a.rhs || ctx->n_lift == 0) return rc;
+    const int len = a.m.q * DA0 * DB0, lenF = a.has_rhs ? a.f.q : 0;
+    a.rowlen = pad16<T>(NCA * NCB * len + (a.has_rhs ? NCB * lenF : 0));  // as launch_blocks sets it
+    const int64_t blocks = (ctx->n_lift + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((dirichlet_lift_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
+                       static_cast<const T*>(ctx->d_kbuf), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->run_bc);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
 template <typename T>
 int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, hipStream_t s) {
     const int dA = anifem::ref_dim(ctx->prob.trial.op, ctx->prob.trial.fem),
               dB = anifem::ref_dim(ctx->prob.test.op, ctx->prob.test.fem);
     const int nc = ctx->prob.trial.ncomp;
-    if (dA == 3 && dB == 3 && nc == 1) return launch_blocks<3, 3, 1, 1, T>(ctx, a, s);
-    if (dA == 3 && dB == 3 && nc == 3) return launch_blocks<3, 3, 3, 3, T>(ctx, a, s);
-    if (dA == 1 && dB == 1 && nc == 1) return launch_blocks<1, 1, 1, 1, T>(ctx, a, s);
-    if (dA == 1 && dB == 1 && nc == 3) return launch_blocks<1, 1, 3, 3, T>(ctx, a, s);
-    if (dA == 3 && dB == 1 && nc == 1) return launch_blocks<3, 1, 1, 1, T>(ctx, a, s);
-    if (dA == 1 && dB == 3 && nc == 1) return launch_blocks<1, 3, 1, 1, T>(ctx, a, s);
+    if (dA == 3 && dB == 3 && nc == 1) return launch_blocks_lift<3, 3, 1, 1, T>(ctx, a, s);
+    if (dA == 3 && dB == 3 && nc == 3) return launch_blocks_lift<3, 3, 3, 3, T>(ctx, a, s);
+    if (dA == 1 && dB == 1 && nc == 1) return launch_blocks_lift<1, 1, 1, 1, T>(ctx, a, s);
+    if (dA == 1 && dB == 1 && nc == 3) return launch_blocks_lift<1, 1, 3, 3, T>(ctx, a, s);
+    if (dA == 3 && dB == 1 && nc == 1) return launch_blocks_lift<3, 1, 1, 1, T>(ctx, a, s);
+    if (dA == 1 && dB == 3 && nc == 1) return launch_blocks_lift<1, 3, 1, 1, T>(ctx, a, s);
     return ANIFEM_ERR_ARG;
 }
 
@@ -466,6 +486,26 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     if ((rc = upload(&ctx->d_row_ptr, S.row_ptr.data(), S.row_ptr.size(), s))) return rc;
     if ((rc = upload(&ctx->d_col_idx, S.col_idx.data(), S.col_idx.size(), s))) return rc;
     ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_status, 4 * sizeof(int)));
+    // rows an inhomogeneous Dirichlet condition changes: Dirichlet rows and
+    // rows with a dropped (Dirichlet-column) entry in one of their pairs
+    {
+        std::vector<LiftRow> lift;
+        const int nf = S.nf;
+        for (int64_t b = 0; b < S.n_blocks; ++b) {
+            const int32_t r0 = S.blk_row0[b], r1 = S.blk_row0[b + 1];
+            for (int32_t r = r0; r < r1; ++r) {
+                const int64_t pb = S.blk_p0[b] + S.pstart[r];
+                const int64_t pe = r + 1 < r1 ? S.blk_p0[b] + S.pstart[r + 1] : S.blk_p0[b + 1];
+                bool lifted = S.is_dir[r] != 0;
+                for (int64_t p = pb; p < pe && !lifted; ++p)
+                    for (int ia = 0; ia < nf && !lifted; ++ia) lifted = S.pair_pos[(size_t)p * nf + ia] == kPosDropped;
+                if (lifted) lift.push_back(LiftRow{pb, r, (int32_t)(pe - pb)});
+            }
+        }
+        ctx->n_lift = (int64_t)lift.size();
+        if ((rc = upload(&ctx->d_lift, lift.data(), lift.size(), s))) return rc;
+        if (ctx->n_lift && (rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
+    }
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     // the device holds the pair lists; the host keeps the rest for inspection
     std::vector<int32_t>().swap(S.pair_tet);
@@ -497,7 +537,8 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
         void* bufs[] = {ctx->d_coords,   ctx->d_tetnodes, ctx->d_blk_row0, ctx->d_blk_meta, ctx->d_blk_s0, ctx->d_blk_p0,
                         ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
-                        ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets};
+                        ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets,
+                        ctx->d_lift};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -640,7 +681,9 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
                         anifem_status* status, void* stream) {
     using namespace anifem;
     if (!ctx || ctx->host_only || !D || !csr_val) return ANIFEM_ERR_ARG;
-    if (dir_values) return ANIFEM_ERR_ARG;  // inhomogeneous Dirichlet values: not supported
+    // Dirichlet values are applied by the row-block scatter (the coloured
+    // scatter resolves homogeneous conditions only)
+    if (dir_values && ctx->coloured) return ANIFEM_ERR_ARG;
     if (precision != ANIFEM_PREC_FP64 && precision != ANIFEM_PREC_FP32) return ANIFEM_ERR_ARG;
     if (D->type != ctx->prob.tensor_type || D->storage != ctx->prob.tensor_storage) return ANIFEM_ERR_ARG;
     const bool do_rhs = F && rhs;
@@ -692,7 +735,9 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     if (rhs && !do_rhs && (rc = fill(rhs, S.ndof, s))) return rc;
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
     ANIFEM_HIP_CHECK(hipGetLastError());
+    ctx->run_bc = dir_values;
     rc = precision == ANIFEM_PREC_FP64 ? dispatch_blocks<double>(ctx, a, s) : dispatch_blocks<float>(ctx, a, s);
+    ctx->run_bc = nullptr;
     if (rc) return rc;
     if (ctx->timers_on) {
         ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[2], s));

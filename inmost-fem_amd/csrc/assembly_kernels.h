@@ -357,6 +357,78 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }
 
+// Rows whose load-vector entries an inhomogeneous Dirichlet condition changes:
+// a Dirichlet row, or a row with a Dirichlet column in one of its cells.  Its
+// pairs (the row's incident tets, ascending) are the global pair range
+// [p0, p0 + np).
+struct LiftRow {
+    int64_t p0;
+    int32_t row;
+    int32_t np;
+};
+
+// Inhomogeneous Dirichlet values (dc_on_dof.h:38-45, applied per cell as in
+// ex1.cpp:100-105): after the row-block pass, one lane per lifted row forms
+// the row's rhs again with each cell's load-vector entry corrected as applyDir
+// leaves it, then sums the cells' entries in ascending tet order (the order of
+// the pass and of the reference's element loop):
+//   * Dirichlet row k: F(k) = bc_k in every cell, so the row gets the sum of
+//     bc_k over its incident tets;
+//   * other row i: F(i) = F0(i) - A(i, k1) bc_k1 - A(i, k2) bc_k2 - ... over the
+//     cell's Dirichlet DOFs in local order; applyDir on an earlier DOF zeroes
+//     only its own row and column, so every A(i, k) is the original element
+//     entry.  Those entries are exactly the ones the pattern dropped
+//     (positions kPosDropped), formed here from the tet's geometry row and the
+//     reference tensor as phase A forms the kept ones.
+// Rows that no Dirichlet DOF touches keep the pass's values.
+template <int DA0, int DB0, int NCA, int NCB, typename T>
+__global__ __launch_bounds__(kBlock) void dirichlet_lift_kernel(BlockArgs a, const T* __restrict__ Kg,
+                                                                const LiftRow* __restrict__ lift, int64_t n_lift,
+                                                                const int32_t* __restrict__ dofs,
+                                                                const double* __restrict__ bc) {
+    constexpr int KB = DA0 * DB0;
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (i >= n_lift) return;
+    const KindTable& KT = a.kinds[0];
+    const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0;
+    const int len = a.m.q * KB, lenF = a.has_rhs ? a.f.q : 0;
+    const int kstride = NCA * NCB * len;
+    const T* R = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.Rd : (const void*)KT.Rf);
+    const T* RF = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.RFd : (const void*)KT.RFf);
+    const LiftRow L = lift[i];
+    const double bc_row = bc[L.row];
+    double sum = 0.0;
+    int nonfinite = 0;
+    for (int64_t gp = L.p0; gp < L.p0 + L.np; ++gp) {
+        const int ibf = a.pair_ib[gp];
+        double fv;
+        if (ibf & kPairDirichlet) {
+            fv = bc_row;
+        } else {
+            const int64_t tet = a.pair_tet[gp];
+            const int ib = ibf & 63, ib0 = ib % nfB0, cb = ib / nfB0;
+            const T* Kt = Kg + tet * a.rowlen;
+            T f0 = 0;
+            for (int j = 0; j < lenF; ++j) f0 += Kt[kstride + cb * lenF + j] * RF[ib0 * lenF + j];
+            fv = (double)f0;
+            const uint16_t* pos = a.pair_pos + gp * nf;
+            for (int ia = 0; ia < nf; ++ia) {
+                if (pos[ia] != kPosDropped) continue;
+                const int ca = ia / nfA0, ia0 = ia % nfA0;
+                const T* Kb = Kt + (cb * NCA + ca) * len;
+                const T* Rr = R + (ib0 + nfB0 * ia0) * len;
+                T acc = 0;
+                for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
+                fv -= (double)acc * bc[dofs[tet * nf + ia]];
+            }
+        }
+        nonfinite += isfinite(fv) ? 0 : 1;
+        sum += fv;
+    }
+    a.rhs[L.row] = sum;
+    if (nonfinite) atomicAdd(&a.status[0], nonfinite);
+}
+
 // Row-block pass over caller-supplied element matrices (the local matrices a
 // MatFuncWrap returns, func_wrap.h:96-187): Ae holds n_tets nf x nf matrices,
 // column-major (entry (ib, ia) at Ae[t*nf*nf + ib + nf*ia]), Fe n_tets load

@@ -6,10 +6,10 @@
 //   * NATURAL global enumeration: (component, geometric type {node, edge,
 //     face}, element id, dof on element) (global_enumerator.h:392-404,
 //     global_enumerator.cpp:562-777);
-//   * P3 edge DOF pairs ordered by global node id: the pair keeps its local
-//     order when perm[I] < perm[J] with perm = createOrderPermutation(global
-//     node ids) (ordering.inl:137-151, tetdofmap.inl:98-105, applied in
-//     fill_assemble_templates, assembler.inl:140-184);
+//   * P3 edge DOF pairs ordered by global node id: the pair of local edge
+//     (I, J) keeps its local order when gid(I) < gid(J), the reference's
+//     vertex-rank rule (ordering.inl:137-151, tetdofmap.inl:98-105, applied
+//     in fill_assemble_templates, assembler.inl:140-184);
 //   * CSR pattern: every DOF pair sharing a tet plus the diagonal
 //     (assembler.inl:114-136), minus the off-diagonals of Dirichlet rows and
 //     columns, whose element entries applyDir makes exactly zero and the drop
@@ -87,21 +87,13 @@ int64_t number_by_first_appearance(const std::vector<Key>& keys, std::vector<int
     return nu;
 }
 
-// createOrderPermutation (ordering.inl:137-151)
-void order_permutation(const int64_t g[4], unsigned char perm[4]) {
-    unsigned char i = 0, j = 0;
-    for (int k = 1; k < 4; ++k) {
-        if (g[k] < g[i]) i = (unsigned char)k;
-        else if (g[k] > g[j]) j = (unsigned char)k;
-    }
-    perm[i] = 0;
-    perm[j] = 3;
-    unsigned char qid = (unsigned char)(i + j - 1 + (std::min(i, j) > 0 ? 1 : 0));
-    static const unsigned char lookup[12] = {2, 3, 1, 3, 1, 2, 0, 3, 0, 2, 0, 1};
-    i = lookup[2 * qid], j = lookup[2 * qid + 1];
-    if (g[i] < g[j]) perm[i] = 1, perm[j] = 2;
-    else perm[i] = 2, perm[j] = 1;
-}
+// P3 edge DOF pair order.  The reference ranks the four vertices of a cell by
+// global node id (createOrderPermutation, ordering.inl:137-151) and keeps the
+// pair of local edge (I, J) in local order when rank(I) < rank(J)
+// (index_on_reorderd_elem, tetdofmap.inl:98-105).  For distinct ids the ranks
+// compare as the ids do, so the pair keeps its order exactly when the edge's
+// first local vertex carries the smaller global node id.
+inline bool edge_pair_kept(const int32_t* v, int k) { return v[kEdge[k][0]] < v[kEdge[k][1]]; }
 
 }  // namespace
 
@@ -208,12 +200,9 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
             if (fem == ANIFEM_FEM_P2)
                 for (int k = 0; k < 6; ++k) loc[4 + k] = NN + eid[6 * t + k];
             if (fem == ANIFEM_FEM_P3) {
-                int64_t g[4] = {v[0], v[1], v[2], v[3]};
-                unsigned char perm[4];
-                order_permutation(g, perm);
                 for (int k = 0; k < 6; ++k) {
                     int64_t base = NN + 2 * eid[6 * t + k];
-                    bool keep = perm[kEdge[k][0]] < perm[kEdge[k][1]];
+                    bool keep = edge_pair_kept(v, k);
                     loc[4 + 2 * k] = base + (keep ? 0 : 1);
                     loc[5 + 2 * k] = base + (keep ? 1 : 0);
                 }

@@ -123,22 +123,9 @@ __global__ void mult_kernel(const int32_t* __restrict__ start, const int32_t* __
     mult[rankpos[first[u]]] = end - start[u];
 }
 
-// createOrderPermutation (ordering.inl:137-151)
-__device__ void order_permutation_d(const int64_t g[4], unsigned char perm[4]) {
-    unsigned char i = 0, j = 0;
-    for (int k = 1; k < 4; ++k) {
-        if (g[k] < g[i]) i = (unsigned char)k;
-        else if (g[k] > g[j]) j = (unsigned char)k;
-    }
-    perm[i] = 0;
-    perm[j] = 3;
-    const unsigned char qid = (unsigned char)(i + j - 1 + ((i < j ? i : j) > 0 ? 1 : 0));
-    const unsigned char lookup[12] = {2, 3, 1, 3, 1, 2, 0, 3, 0, 2, 0, 1};
-    i = lookup[2 * qid], j = lookup[2 * qid + 1];
-    if (g[i] < g[j]) perm[i] = 1, perm[j] = 2;
-    else perm[i] = 2, perm[j] = 1;
-}
-
+// P3 edge DOF pair order: the pair of local edge (I, J) keeps its local order
+// when gid(I) < gid(J), which is what the reference's vertex ranking by global
+// node id gives (ordering.inl:137-151, tetdofmap.inl:98-105).
 __global__ void dofs_kernel(const int32_t* __restrict__ tets, int64_t NT, int64_t NN, const int32_t* __restrict__ eid,
                             const int32_t* __restrict__ fid, int64_t n_edges, int fem, int nf0, int ncomp,
                             int64_t nscal, int32_t* dofs) {
@@ -150,12 +137,9 @@ __global__ void dofs_kernel(const int32_t* __restrict__ tets, int64_t NT, int64_
     if (fem == ANIFEM_FEM_P2)
         for (int k = 0; k < 6; ++k) loc[4 + k] = NN + eid[6 * t + k];
     if (fem == ANIFEM_FEM_P3) {
-        const int64_t g[4] = {v[0], v[1], v[2], v[3]};
-        unsigned char perm[4];
-        order_permutation_d(g, perm);
         for (int k = 0; k < 6; ++k) {
             const int64_t base = NN + 2 * (int64_t)eid[6 * t + k];
-            const bool keep = perm[kEdgeD[k][0]] < perm[kEdgeD[k][1]];
+            const bool keep = v[kEdgeD[k][0]] < v[kEdgeD[k][1]];
             loc[4 + 2 * k] = base + (keep ? 0 : 1);
             loc[5 + 2 * k] = base + (keep ? 1 : 0);
         }

@@ -812,6 +812,24 @@ void orc_apply_dir(int nf, double* A, double* F, int k, double bc) {
     A[k + nf * k] = 1.0;
 }
 
+/* Element-level Dirichlet on every cell, as the reference's local function
+ * applies it (ex1.cpp:100-105): applyDir on each flagged local DOF k in
+ * ascending local order, with value bc[dofs[k]] (0 when bc is NULL). */
+void orc_apply_dir_cells(long n_tets, int nf, const int32_t* dofs, const uint8_t* is_dir, const double* bc,
+                         double* A, double* F) {
+    double Fs[64];
+    for (long t = 0; t < n_tets; ++t) {
+        double* At = A + (size_t)nf * nf * t;
+        double* Ft = F ? F + (size_t)nf * t : Fs;
+        if (!F)
+            for (int i = 0; i < nf; ++i) Fs[i] = 0.0;
+        for (int k = 0; k < nf; ++k) {
+            const int32_t d = dofs[(size_t)nf * t + k];
+            if (is_dir[d]) orc_apply_dir(nf, At, Ft, k, bc ? bc[d] : 0.0);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Assembly-side restatement                                                  */
 
@@ -902,19 +920,15 @@ static long orc_number_entities(long n_tets, const int32_t* tets, int ne, int nv
     return nu;
 }
 
-/* createOrderPermutation (ordering.inl:137-151) */
-static void orc_order_perm(const long* g, unsigned char* perm) {
-    unsigned char i = 0, j = 0;
-    for (int k = 1; k < 4; ++k)
-        if (g[k] < g[i]) i = (unsigned char)k;
-        else if (g[k] > g[j]) j = (unsigned char)k;
-    perm[i] = 0;
-    perm[j] = 3;
-    unsigned char qid = (unsigned char)(i + j - 1 + ((i < j ? i : j) > 0 ? 1 : 0));
-    static const unsigned char lookup[12] = {2, 3, 1, 3, 1, 2, 0, 3, 0, 2, 0, 1};
-    i = lookup[2 * qid], j = lookup[2 * qid + 1];
-    if (g[i] < g[j]) perm[i] = 1, perm[j] = 2;
-    else perm[i] = 2, perm[j] = 1;
+/* Vertex ranks by global node id: rank[k] = number of vertices of the cell
+ * whose id is smaller than g[k] (the permutation createOrderPermutation
+ * returns, ordering.inl:137-151, for the distinct ids of a valid cell). */
+static void orc_vertex_rank(const long* g, unsigned char* rank) {
+    for (int k = 0; k < 4; ++k) {
+        unsigned char r = 0;
+        for (int m = 0; m < 4; ++m) r = (unsigned char)(r + (g[m] < g[k]));
+        rank[k] = r;
+    }
 }
 
 long orc_enumerate(int fem, int ncomp, long n_nodes, long n_tets, const int32_t* tets, int32_t* dofs, long* n_edges,
@@ -944,7 +958,7 @@ long orc_enumerate(int fem, int ncomp, long n_nodes, long n_tets, const int32_t*
         const int32_t* v = tets + 4 * t;
         long gl[4] = {v[0], v[1], v[2], v[3]};
         unsigned char perm[4] = {0, 1, 2, 3};
-        if (fem == ORC_P3) orc_order_perm(gl, perm);
+        if (fem == ORC_P3) orc_vertex_rank(gl, perm);
         long loc[20];
         for (int i = 0; i < 4; ++i) loc[i] = v[i];
         if (fem == ORC_P2)
@@ -1088,6 +1102,45 @@ int orc_scatter(long n_tets, int nf, const int32_t* dofs, const int64_t* slot, c
         }
     }
     return 0;
+}
+
+/* The scatter of orc_scatter parallel over colours (the CPU baseline's
+ * multi-threaded form): tets of one colour share no node, hence no DOF, so a
+ * colour's tets are added by nthreads threads without conflicts, colour after
+ * colour.  colour_ptr (n_colours + 1) and colour_tets list each colour's tets
+ * in ascending order.  Values equal orc_scatter's up to the order of the
+ * additions into each slot. */
+int orc_scatter_coloured(int n_colours, const int64_t* colour_ptr, const int32_t* colour_tets, int nf,
+                         const int32_t* dofs, const int64_t* slot, const double* A, const double* F, double drop_val,
+                         int nthreads, double* val, double* rhs) {
+    int bad = 0;
+    for (int c = 0; c < n_colours; ++c) {
+        const long n_c = (long)(colour_ptr[c + 1] - colour_ptr[c]);
+        const int32_t* ct = colour_tets + colour_ptr[c];
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(| : bad)
+        for (long k = 0; k < n_c; ++k) {
+            const long t = ct[k];
+            for (int i = 0; i < nf; ++i) {
+                if (F) {
+                    const double f = F[(size_t)nf * t + i];
+                    rhs[dofs[(size_t)nf * t + i]] += f;
+                    bad |= isfinite(f) ? 0 : 1;
+                }
+                for (int j = 0; j < nf; ++j) {
+                    const double a = A[(size_t)nf * nf * t + i + nf * j];
+                    const int64_t s = slot[(size_t)nf * nf * t + i + nf * j];
+                    bad |= isfinite(a) ? 0 : 1;
+                    if (fabs(a) > drop_val) {
+                        if (s < 0)
+                            bad |= 2;
+                        else
+                            val[s] += a;
+                    }
+                }
+            }
+        }
+    }
+    return bad & 2 ? -3 : (bad ? -1 : 0);
 }
 
 void orc_spmv(long nrow, const int64_t* row_ptr, const int32_t* col_idx, const double* val, const double* x,

@@ -88,6 +88,12 @@ int orc_dim(int op, int ncomp);
  * nf x nf matrix A and vector F. */
 void orc_apply_dir(int nf, double* A, double* F, int k, double bc);
 
+/* applyDir on every flagged local DOF of every cell, in ascending local order
+ * (ex1.cpp:100-105): A n_tets column-major nf x nf matrices, F n_tets vectors
+ * of nf (may be NULL), bc the Dirichlet value of each global DOF (NULL: 0). */
+void orc_apply_dir_cells(long n_tets, int nf, const int32_t* dofs, const uint8_t* is_dir, const double* bc,
+                         double* A, double* F);
+
 /* ---- global assembly (assembler.inl, ordering.inl, global_enumerator.cpp) ---- */
 
 /* Local node reorder of the assembler (ordering.inl:8-26), applied in place to
@@ -129,6 +135,13 @@ int orc_greedy_colour(long n_nodes, long n_tets, const int32_t* tets, int32_t* c
  * slot. */
 int orc_scatter(long n_tets, int nf, const int32_t* dofs, const int64_t* slot, const double* A, const double* F,
                 double drop_val, double* val, double* rhs);
+
+/* orc_scatter parallel over colours (no two tets of a colour share a node):
+ * colour c holds colour_tets[colour_ptr[c] .. colour_ptr[c+1]).  Same return
+ * codes; values equal orc_scatter's up to the order of the additions. */
+int orc_scatter_coloured(int n_colours, const int64_t* colour_ptr, const int32_t* colour_tets, int nf,
+                         const int32_t* dofs, const int64_t* slot, const double* A, const double* F, double drop_val,
+                         int nthreads, double* val, double* rhs);
 
 /* y = A x for CSR A. */
 void orc_spmv(long nrow, const int64_t* row_ptr, const int32_t* col_idx, const double* val, const double* x,

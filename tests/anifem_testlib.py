@@ -258,18 +258,55 @@ def oracle_scatter(n_tets, nf, dofs, slot, A, F, nnz, ndof, drop_val=1e-100):
     return rc, val, rhs
 
 
-def apply_dir_elementwise(A, F, dofs, is_dir):
-    """Element-level homogeneous Dirichlet through the oracle's applyDir."""
-    O = oracle()
+def apply_dir_elementwise(A, F, dofs, is_dir, bc=None):
+    """Element-level Dirichlet through the oracle's applyDir on every flagged
+    local DOF of every cell (bc: value per global DOF, None = homogeneous)."""
     n, nf = dofs.shape
-    A = A.copy()
-    F = F.copy()
-    for r in range(n):
-        a = np.ascontiguousarray(A[r])
-        f = np.ascontiguousarray(F[r])
-        for k in range(nf):
-            if is_dir[dofs[r, k]]:
-                O.orc_apply_dir(nf, P(a), P(f), k, ctypes.c_double(0.0))
-        A[r] = a
-        F[r] = f
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    F = None if F is None else np.ascontiguousarray(F, dtype=np.float64).copy()
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    is_dir = np.ascontiguousarray(is_dir, dtype=np.uint8)
+    bc = None if bc is None else np.ascontiguousarray(bc, dtype=np.float64)
+    oracle().orc_apply_dir_cells(ctypes.c_long(n), nf, P(dofs), P(is_dir), P(bc), P(A), P(F))
     return A, F
+
+
+def shuffled_mesh(coords, tets, seed):
+    """The mesh with relabelled nodes, shuffled cells and a shuffled local
+    vertex order in every cell: exercises the node reorder (ordering.inl:8-26)
+    and the P3 edge orientation by global node id (ordering.inl:137-151)."""
+    rng = np.random.default_rng(seed)
+    relabel = rng.permutation(len(coords))
+    inv = np.empty_like(relabel)
+    inv[relabel] = np.arange(len(coords))
+    coords2 = np.ascontiguousarray(coords[inv])
+    tets2 = relabel[tets]
+    tets2 = tets2[rng.permutation(len(tets2))]
+    for r in range(len(tets2)):
+        tets2[r] = tets2[r][rng.permutation(4)]
+    return coords2, np.ascontiguousarray(tets2, dtype=np.int32)
+
+
+def lagrange_nodes(fem):
+    """Barycentric location (nf, 4) of every local DOF of a Lagrange space,
+    read off the oracle's basis (pinned by the compiled reference's
+    fem3DapplyL fixtures): DOF i sits at the lattice point where phi_i = 1 and
+    every other basis function vanishes."""
+    p = {2: 1, 3: 2, 4: 3}[fem]
+    lat = np.array([(a, b, c, p - a - b - c) for a in range(p + 1) for b in range(p + 1 - a)
+                    for c in range(p + 1 - a - b)], dtype=np.float64) / p
+    nf = NF[fem]
+    xyz = np.tile(np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], dtype=np.float64), (nf, 1))
+    phi = orc_apply(1, fem, 1, lat, xyz, np.eye(nf))[:, :, 0]  # phi[i, point]
+    out = np.zeros((nf, 4))
+    for i in range(nf):
+        j = int(np.argmax(phi[i]))
+        assert abs(phi[i, j] - 1) < 1e-12 and np.abs(np.delete(phi[:, j], i)).max() < 1e-12
+        out[i] = lat[j]
+    return out
+
+
+def dof_positions(coords, rtets, dofs, nodes):
+    """Physical position of every global DOF seen from every cell: (n_tets, nf, 3)."""
+    P = coords[rtets]  # (n, 4, 3)
+    return np.einsum("iv,tvc->tic", nodes, P)

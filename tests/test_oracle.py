@@ -442,3 +442,60 @@ def test_planted_mistake_is_caught():
     A = T.orc_elem(GRAD, P3, 1, GRAD, P1, 3, GEN, PERPT, K["order"], xyz, D.reshape(1, -1))
     Aexp = np.array(K["A_times_scale"], dtype=float) / K["scale"]
     assert np.linalg.norm(A.reshape(20, 12).T - Aexp) > 1e-3
+
+
+def test_apply_dir_cells_inhomogeneous():
+    """orc_apply_dir_cells (applyDir on every flagged DOF of every cell, as
+    ex1.cpp:100-105) against orc_apply_dir applied one DOF at a time in a
+    shuffled order: applyDir on one DOF zeroes only its row and column, so the
+    order does not matter, and the result has the closed form
+    F(i) = F0(i) - sum_k A0(i, k) bc_k off the Dirichlet rows, F(k) = bc_k."""
+    rng = np.random.default_rng(4)
+    n, nf, ndof = 30, 10, 40
+    dofs = np.stack([rng.choice(ndof, nf, replace=False) for _ in range(n)]).astype(np.int32)
+    is_dir = (rng.random(ndof) < 0.3).astype(np.uint8)
+    bc = rng.normal(size=ndof)
+    A0 = rng.normal(size=(n, nf * nf))
+    F0 = rng.normal(size=(n, nf))
+    A, F = T.apply_dir_elementwise(A0, F0, dofs, is_dir, bc)
+    O = T.oracle()
+    for t in range(n):
+        a = A0[t].copy()
+        f = F0[t].copy()
+        for k in rng.permutation(nf):
+            if is_dir[dofs[t, k]]:
+                O.orc_apply_dir(nf, T.P(a), T.P(f), int(k), ctypes.c_double(bc[dofs[t, k]]))
+        np.testing.assert_allclose(A[t], a, rtol=0, atol=0)
+        np.testing.assert_allclose(F[t], f, rtol=1e-14, atol=1e-14)
+        M0 = A0[t].reshape(nf, nf, order="F")
+        d = is_dir[dofs[t]].astype(bool)
+        expect = F0[t] - M0[:, d] @ bc[dofs[t][d]]
+        expect[d] = bc[dofs[t][d]]
+        np.testing.assert_allclose(F[t], expect, rtol=1e-13, atol=1e-13)
+        M = A[t].reshape(nf, nf, order="F")
+        assert np.all(M[d][:, ~d] == 0) and np.all(M[~d][:, d] == 0)
+        np.testing.assert_array_equal(M[np.ix_(d, d)], np.eye(int(d.sum())))
+
+
+def test_coloured_scatter_equals_sequential():
+    """orc_scatter_coloured (the CPU baseline's threaded scatter) adds the same
+    contributions as orc_scatter, colour by colour."""
+    import anifem_amd as af
+    coords, tets = af.kuhn_cube(4, 0.2, 2)
+    S = T.oracle_assembly(coords, tets, P2, 1)
+    n = len(tets)
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(n, 100))
+    F = rng.normal(size=(n, 10))
+    rc, v1, r1 = T.oracle_scatter(n, 10, S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
+    col = np.zeros(n, np.int32)
+    tt = np.ascontiguousarray(S["tets"])
+    nc = T.oracle().orc_greedy_colour(ctypes.c_long(len(coords)), ctypes.c_long(n), T.P(tt), T.P(col))
+    order = np.argsort(col, kind="stable").astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=nc))]).astype(np.int64)
+    v2, r2 = np.zeros_like(v1), np.zeros_like(r1)
+    rc2 = T.oracle().orc_scatter_coloured(nc, T.P(ptr), T.P(order), 10, T.P(S["dofs"]), T.P(S["slot"]), T.P(A),
+                                          T.P(F), ctypes.c_double(1e-100), 4, T.P(v2), T.P(r2))
+    assert rc == 0 and rc2 == 0
+    np.testing.assert_allclose(v2, v1, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(r2, r1, rtol=1e-13, atol=1e-13)

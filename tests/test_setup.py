@@ -9,20 +9,8 @@ import anifem_testlib as T
 
 
 def _shuffled_mesh(n, seed):
-    """Kuhn cube with relabelled nodes, shuffled tets and shuffled local vertex
-    order: exercises the node reorder (ordering.inl:8-26) and the P3 edge
-    orientation by global node id (ordering.inl:137-151)."""
     coords, tets = af.kuhn_cube(n, 0.2, seed)
-    rng = np.random.default_rng(seed)
-    relabel = rng.permutation(len(coords))
-    inv = np.empty_like(relabel)
-    inv[relabel] = np.arange(len(coords))
-    coords2 = coords[inv]
-    tets2 = relabel[tets]
-    tets2 = tets2[rng.permutation(len(tets2))]
-    for r in range(len(tets2)):
-        tets2[r] = tets2[r][rng.permutation(4)]
-    return coords2, tets2.astype(np.int32)
+    return T.shuffled_mesh(coords, tets, seed)
 
 
 def _boundary_mask(A, coords):
