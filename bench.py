@@ -56,67 +56,111 @@ def aniso_packed(rng, n):
     return np.stack([D[:, i, j] for i, j in idx]), D
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
-    """Reference element core (oracle/_ref, compiled from the AniFem++ sources)
-    on the workload's tets, element-level homogeneous Dirichlet
-    (dc_on_dof.h:38-45), then the oracle's sequential cell-order scatter.  The
-    element matrices are computed once on one host thread and once split over
-    min(16, host CPUs) threads (contiguous tet chunks; the reference's
-    assembler evaluates cells independently, assembler.inl:381); the scatter
-    is sequential.  Falls back to the oracle's own element restatement where
-    the reference build is absent."""
+    """The reference path on the host cores, whole workload: the reference
+    element core (oracle/_ref, compiled from the AniFem++ sources) forms every
+    element matrix and load vector, element-level homogeneous Dirichlet
+    (dc_on_dof.h:38-45) follows per cell, then the scatter into the same CSR.
+    Multi-threaded form: the cells split into contiguous chunks over all host
+    CPUs (the reference evaluates cells independently, assembler.inl:381),
+    then a colour-parallel scatter (no two cells of a colour share a node).
+    Single-thread form: one thread for the cells, then the oracle's
+    sequential cell-order scatter (assembler.inl:397-481), whose values are
+    returned for the parity check.  Falls back to the oracle's own element
+    restatement where the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from concurrent.futures import ThreadPoolExecutor
     import anifem_testlib as T
 
     ref = T.ref_lib()
     kind = "reference" if ref is not None else "port"
-    n_all = len(tets_reordered)
-    n = n_all
-    xyz = np.ascontiguousarray(coords[tets_reordered[:n]].reshape(n, 12))
-    Df = np.ascontiguousarray(Dfull[:n].transpose(0, 2, 1).reshape(n, 9))
+    O = T.oracle()
+    n = len(tets_reordered)
+    xyz = np.ascontiguousarray(coords[tets_reordered].reshape(n, 12))
+    Df = np.ascontiguousarray(Dfull.transpose(0, 2, 1).reshape(n, 9))
+    one = np.ones(1)
     dp = ctypes.POINTER(ctypes.c_double)
-
-    def elements(lo, hi, A):
-        if ref is not None:
-            rc = ref.ref_fem3dtet(1, 3, 0, 2, ctypes.c_long(hi - lo), xyz[lo:hi].ctypes.data_as(dp),
-                                  Df[lo:hi].ctypes.data_as(dp), ctypes.c_long(9), A[lo:hi].ctypes.data_as(dp))
-            assert rc == 0
-        else:
-            A[lo:hi] = T.orc_elem(2, 3, 1, 2, 3, 1, 3, 1, 2, xyz[lo:hi], Df[lo:hi])
-
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    is_dir = np.ascontiguousarray(is_dir, dtype=np.uint8)
     if ref is not None:
         ref.ref_fem3dtet.restype = ctypes.c_int
-    A = np.zeros((n, 100))
+
+    def cells(lo, hi, A, F):
+        if ref is not None:
+            # (GRAD P2, GRAD P2) with a per-tet symmetric tensor; (IDEN P0, IDEN P2) with f = 1
+            assert ref.ref_fem3dtet(1, 3, 0, 2, ctypes.c_long(hi - lo), xyz[lo:hi].ctypes.data_as(dp),
+                                    Df[lo:hi].ctypes.data_as(dp), ctypes.c_long(9), A[lo:hi].ctypes.data_as(dp)) == 0
+            assert ref.ref_fem3dtet(6, 2, 0, 2, ctypes.c_long(hi - lo), xyz[lo:hi].ctypes.data_as(dp),
+                                    one.ctypes.data_as(dp), ctypes.c_long(0), F[lo:hi].ctypes.data_as(dp)) == 0
+        else:
+            A[lo:hi] = T.orc_elem(2, 3, 1, 2, 3, 1, 3, 1, 2, xyz[lo:hi], Df[lo:hi])
+            F[lo:hi] = T.orc_elem(1, 1, 1, 1, 3, 1, 2, 0, 2, xyz[lo:hi], np.ones((1, 1)))
+        O.orc_apply_dir_cells(ctypes.c_long(hi - lo), 10, T.P(dofs[lo:hi]), T.P(is_dir), None, T.P(A[lo:hi]),
+                              T.P(F[lo:hi]))
+
+    def zeros(*shape):
+        # touched before any timed region: first-touch page faults of a fresh
+        # allocation are not part of the path
+        a = np.empty(shape)
+        a.fill(0.0)
+        return a
+
+    # one thread: cells, then the sequential scatter
+    A1, F1 = zeros(n, 100), zeros(n, 10)
     t0 = time.perf_counter()
-    elements(0, n, A)
-    t_one = time.perf_counter() - t0
-    threads = max(1, min(16, os.cpu_count() or 1))
-    A_mt = np.zeros((n, 100))
-    cuts = np.linspace(0, n, threads + 1).astype(int)
+    cells(0, n, A1, F1)
+    t_cells_1 = time.perf_counter() - t0
+    val_seq, rhs_seq = zeros(nnz), zeros(ndof)
+    t0 = time.perf_counter()
+    rc = O.orc_scatter(ctypes.c_long(n), 10, T.P(dofs), T.P(slot), T.P(A1), T.P(F1), ctypes.c_double(1e-100),
+                       T.P(val_seq), T.P(rhs_seq))
+    t_scatter_1 = time.perf_counter() - t0
+    assert rc == 0, rc
+    # all host CPUs: cells in chunks, colour-parallel scatter (the colouring
+    # is setup, like the CSR pattern, and is not timed)
+    threads = max(1, os.cpu_count() or 1)
+    col = np.zeros(n, np.int32)
+    tt = np.ascontiguousarray(tets_reordered, dtype=np.int32)
+    n_col = O.orc_greedy_colour(ctypes.c_long(len(coords)), ctypes.c_long(n), T.P(tt), T.P(col))
+    col_tets = np.argsort(col, kind="stable").astype(np.int32)
+    col_ptr = np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n_col))]).astype(np.int64)
+    A, F = zeros(n, 100), zeros(n, 10)
+    cuts = np.linspace(0, n, 4 * threads + 1).astype(int)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        list(ex.map(lambda k: elements(int(cuts[k]), int(cuts[k + 1]), A_mt), range(threads)))
-    t_mt = time.perf_counter() - t0
-    assert np.array_equal(A, A_mt)
-    t1 = time.perf_counter()
-    # applyDir with value 0 on every flagged local DOF: zero row and column, unit diagonal
-    dmask = is_dir[dofs[:n]].astype(bool)  # (n, 10) local flags
-    A3 = A.reshape(n, 10, 10)  # [tet, col ia, row ib] (column-major per tet)
-    A3[dmask[:, :, None] | dmask[:, None, :]] = 0.0
-    idx = np.nonzero(dmask)
-    A3[idx[0], idx[1], idx[1]] = 1.0
-    rc, _, _ = T.oracle_scatter(n, 10, np.ascontiguousarray(dofs[:n]), np.ascontiguousarray(slot[:n]), A, None, nnz,
-                                ndof)
+        list(ex.map(lambda k: cells(int(cuts[k]), int(cuts[k + 1]), A, F), range(4 * threads)))
+    t_cells_mt = time.perf_counter() - t0
+    assert np.array_equal(A, A1) and np.array_equal(F, F1)
+    val_col, rhs_col = zeros(nnz), zeros(ndof)
+    t0 = time.perf_counter()
+    rc = O.orc_scatter_coloured(int(n_col), T.P(col_ptr), T.P(col_tets), 10, T.P(dofs), T.P(slot), T.P(A), T.P(F),
+                                ctypes.c_double(1e-100), threads, T.P(val_col), T.P(rhs_col))
+    t_scatter_mt = time.perf_counter() - t0
     assert rc == 0, rc
-    t_sc = time.perf_counter() - t1
-    rate = n / (t_mt + t_sc)
-    return {"value": rate, "unit": "element matrices/s", "cores": threads, "kind": kind,
-            "single_thread_value": n / (t_one + t_sc),
-            "sample": f"all {n} of {n_all} tets: P2 GRAD-GRAD element matrices with per-tet SPD D "
-                      f"({'reference fem3Dtet compiled from the AniFem++ sources' if ref else 'oracle restatement'}: "
-                      f"{t_mt:.2f}s on {threads} threads, {t_one:.2f}s on one) + oracle sequential scatter into the "
-                      f"same CSR ({t_sc:.2f}s)"}
+    scale = max(np.abs(val_seq).max(), 1e-300)
+    col_err = float(np.abs(val_col - val_seq).max() / scale)
+    res = {"value": n / (t_cells_mt + t_scatter_mt), "unit": "element matrices/s", "cores": threads,
+           "kind": kind, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+           "single_thread_value": n / (t_cells_1 + t_scatter_1),
+           "seconds": {"cells_all_cpus": t_cells_mt, "scatter_coloured_all_cpus": t_scatter_mt,
+                       "cells_one_thread": t_cells_1, "scatter_sequential": t_scatter_1},
+           "colours": int(n_col), "coloured_vs_sequential_max_err_rel_max": col_err,
+           "sample": f"all {n} tets: P2 GRAD-GRAD element matrices with per-tet SPD D and unit-load vectors "
+                     f"({'reference fem3Dtet compiled from the AniFem++ sources' if ref else 'oracle restatement'}), "
+                     f"element-level Dirichlet, scatter into the same CSR: {threads} threads "
+                     f"({t_cells_mt:.2f}s cells + {t_scatter_mt:.2f}s coloured scatter, {n_col} colours); "
+                     f"one thread {t_cells_1:.2f}s cells + {t_scatter_1:.2f}s sequential scatter"}
+    return res, val_seq, rhs_seq
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "assembly_p2_aniso_64cube_fp64_traffic.txt")
@@ -518,8 +562,24 @@ def main():
         # the assembler's node reorder (ordering.inl:8-26) before the element calls
         tt = np.ascontiguousarray(tets.copy())
         T.oracle().orc_reorder_nodes(ctypes.c_long(len(tt)), T.P(tt), T.P(coords))
-        out["cpu_baseline"] = cpu_baseline(coords, tt, Dfull, s["dofs"], s["slot"], s["dirichlet"], Asm.nnz,
-                                           Asm.ndof)
+        out["cpu_baseline"], val_ref, rhs_ref = cpu_baseline(coords, tt, Dfull, s["dofs"], s["slot"],
+                                                             s["dirichlet"], Asm.nnz, Asm.ndof)
+        # full-size parity: the timed FP64 pass against the reference path's
+        # result (reference element core + sequential scatter), entry error
+        # relative to the row's largest entry
+        Asm.assemble(D=Dp, F=f, val=val, rhs=rhs, status=False, stream=stream)
+        torch.cuda.synchronize()
+        v = val.cpu().numpy()
+        r = rhs.cpu().numpy()
+        rp = s["row_ptr"]
+        rowmax = np.maximum.reduceat(np.abs(val_ref), rp[:-1])
+        ent = float((np.abs(v - val_ref) / np.repeat(rowmax, np.diff(rp))).max())
+        rerr = float(np.abs(r - rhs_ref).max() / np.abs(rhs_ref).max())
+        out["parity_full_size"] = {
+            "against": "reference fem3Dtet element matrices and load vectors + applyDir + sequential scatter",
+            "fp64_max_entry_err_rel_row_max": ent, "fp64_max_rhs_err_rel_max": rerr,
+            "bound": 1e-12, "ok": bool(ent <= 1e-12 and rerr <= 1e-12)}
+        del val_ref, rhs_ref, v, r
     print(json.dumps(out), flush=True)
 
 
