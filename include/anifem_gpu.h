@@ -299,6 +299,27 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
                         const double* dir_values, int precision, double* csr_val, double* rhs,
                         anifem_status* status, void* stream);
 
+/* anifem_assembly_run without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The geometry rows of the pass, which anifem_assembly_run keeps in a buffer
+ * the context allocates on its first run, live in a fixed scratch array of
+ * the library instead: 128 MiB (134,217,728 bytes) per device, held by every
+ * device the library runs on until the process ends.  A problem needs
+ * n_tets x rowlen x (8 in FP64, 4 in FP32) bytes, rowlen being the tensor
+ * values per tet padded to 16 bytes (P2 with a per-tet tensor and a load:
+ * 80 bytes per tet in FP64, 48 in FP32, so up to 1,677,721 / 2,796,202 tets);
+ * a larger one returns ANIFEM_ERR_DIMS before anything is launched.  The
+ * reference tensors of the problem are built at anifem_assembly_create; the
+ * coloured scatter is refused (ANIFEM_ERR_ARG).  Arguments, results and
+ * status as anifem_assembly_run (the status read-back synchronizes the
+ * stream, as there).  All calls on one device share its scratch array:
+ * calls that overlap in time on one device (other streams, other threads)
+ * race and must be serialized by the caller. */
+int anifem_assembly_run_noalloc(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
+                                const double* dir_values, int precision, double* csr_val, double* rhs,
+                                anifem_status* status, void* stream);
+
 /* Row-block kernel choice (default 1): a persistent pass (each workgroup
  * walks many blocks and prefetches the next one's data) where it applies
  * (scalar spaces, tensor constant over each tet); 0 forces one workgroup per

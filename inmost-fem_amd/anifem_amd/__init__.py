@@ -130,6 +130,8 @@ SIGNATURES = {
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
     "anifem_assembly_run": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP, _I32, _VP,
                                    _VP, ctypes.POINTER(anifem_status), _VP]),
+    "anifem_assembly_run_noalloc": (_I32, [_VP, ctypes.POINTER(anifem_tensor), ctypes.POINTER(anifem_tensor), _VP,
+                                           _I32, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_run_elements": (_I32, [_VP, _VP, _VP, _I32, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_assembly_tets": (_I32, [_VP, _VP]),
     "anifem_assembly_run_rhs": (_I32, [_VP, _VP, _I32, _VP, _I32, ctypes.POINTER(anifem_status), _VP]),

@@ -62,6 +62,10 @@ struct RefTensorDev {
 };
 int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out,
                    int face = -1);
+// Set by entry points that must not allocate device memory: get_ref_tensor
+// then only looks tables up and fails with ANIFEM_ERR_ARG where one would
+// have to be built.
+extern thread_local bool g_ref_lookup_only;
 
 // Symmetric-tensor packing: D(i,j), i<=j, at i + j*(j+1)/2.
 inline int sym_index(int i, int j) { return i <= j ? i + j * (j + 1) / 2 : j + i * (i + 1) / 2; }

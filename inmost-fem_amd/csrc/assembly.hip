@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "anifem_common.h"
 #include "assembly_kernels.h"
@@ -63,9 +64,38 @@ struct anifem_assembly {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     anifem_timers timers{};
     const double* run_bc = nullptr;   // the running call's Dirichlet values (device), or null
+    void* run_scratch = nullptr;      // no-allocation call: this device's scratch array, else null
+    void* run_kg = nullptr;           // geometry rows of the running call
 };
 
 namespace {
+
+// Scratch of anifem_assembly_run_noalloc: the geometry rows of one pass live
+// in this array instead of a buffer the context allocates.  Every device the
+// library runs on holds its own copy for the life of the process; host code
+// uses the device address from scratch_address(), never the array's name.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// This device's address of `scratch`, looked up once per device (with that
+// device current) and kept.
+int scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static void* addr[256] = {};
+    if (device < 0 || device >= 256) return ANIFEM_ERR_ARG;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "anifem: hipGetSymbolAddress(scratch): %s\n", hipGetErrorString(e));
+            return ANIFEM_ERR_HIP;
+        }
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return ANIFEM_OK;
+}
 
 __global__ void fill_kernel(double* p, int64_t n, double v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -229,16 +259,24 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     if (a.max_pairs > kBlock) return ANIFEM_ERR_DIMS;
     const size_t lds = lds_of(staged);
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
-    // geometry rows of every tet (kept on the device between runs)
+    // geometry rows of every tet: in the no-allocation call's scratch array,
+    // else in a buffer the context keeps between runs
     const size_t need = (size_t)S.n_tets * a.rowlen * sizeof(T);
-    if (ctx->kbuf_bytes < need) {
-        if (ctx->d_kbuf) ANIFEM_HIP_CHECK(hipFree(ctx->d_kbuf));
-        ctx->d_kbuf = nullptr;
-        ctx->kbuf_bytes = 0;
-        ANIFEM_HIP_CHECK(hipMalloc(&ctx->d_kbuf, need));
-        ctx->kbuf_bytes = need;
+    T* Kg;
+    if (ctx->run_scratch) {
+        if (need > kScratchBytes) return ANIFEM_ERR_DIMS;
+        Kg = static_cast<T*>(ctx->run_scratch);
+    } else {
+        if (ctx->kbuf_bytes < need) {
+            if (ctx->d_kbuf) ANIFEM_HIP_CHECK(hipFree(ctx->d_kbuf));
+            ctx->d_kbuf = nullptr;
+            ctx->kbuf_bytes = 0;
+            ANIFEM_HIP_CHECK(hipMalloc(&ctx->d_kbuf, need));
+            ctx->kbuf_bytes = need;
+        }
+        Kg = static_cast<T*>(ctx->d_kbuf);
     }
-    T* Kg = static_cast<T*>(ctx->d_kbuf);
+    ctx->run_kg = Kg;
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
                        S.n_tets);
@@ -291,7 +329,7 @@ int launch_blocks_lift(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s)
     a.rowlen = pad16<T>(NCA * NCB * len + (a.has_rhs ? NCB * lenF : 0));  // as launch_blocks sets it
     const int64_t blocks = (ctx->n_lift + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((dirichlet_lift_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
-                       static_cast<const T*>(ctx->d_kbuf), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->run_bc);
+                       static_cast<const T*>(ctx->run_kg), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->run_bc);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -464,6 +502,22 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
     ANIFEM_HIP_CHECK(hipGetDevice(&ctx->device));
+    // set-up for anifem_assembly_run_noalloc on this device: the scratch
+    // array's address, and the reference tensors of the problem's pairs
+    // (built here, so that the run only looks them up; a pair the run would
+    // refuse is left for the run to report)
+    {
+        void* sp = nullptr;
+        if ((rc = scratch_address(ctx->device, &sp))) return rc;
+        RefTensorDev R;
+        std::vector<double> pts, wts;
+        const bool pp = prob->tensor_storage == ANIFEM_COEF_PER_POINT;
+        if (rule_points(prob->order, -1, pts, wts) > 0)
+            (void)get_ref_tensor(prob->trial.op, prob->trial.fem, prob->test.op, prob->test.fem, prob->order, pp, &R);
+        const bool ppf = prob->rhs_tensor_storage == ANIFEM_COEF_PER_POINT;
+        if (rule_points(prob->rhs_order, -1, pts, wts) > 0)
+            (void)get_ref_tensor(ANIFEM_IDEN, ANIFEM_FEM_P0, ANIFEM_IDEN, prob->test.fem, prob->rhs_order, ppf, &R);
+    }
     const int64_t NT = S.n_tets;
     std::vector<int32_t> tn(4 * NT);
     for (int64_t t = 0; t < NT; ++t)
@@ -676,9 +730,14 @@ int anifem_assembly_coords(anifem_assembly* ctx, double** coords) {
     return ANIFEM_OK;
 }
 
-int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
-                        const double* dir_values, int precision, double* csr_val, double* rhs,
-                        anifem_status* status, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// anifem_assembly_run, and its no-allocation form (scratch: this device's
+// scratch array, which then holds the geometry rows).
+int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, const double* dir_values,
+                 int precision, double* csr_val, double* rhs, anifem_status* status, void* stream, void* scratch) {
     using namespace anifem;
     if (!ctx || ctx->host_only || !D || !csr_val) return ANIFEM_ERR_ARG;
     // Dirichlet values are applied by the row-block scatter (the coloured
@@ -736,8 +795,11 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
     ANIFEM_HIP_CHECK(hipGetLastError());
     ctx->run_bc = dir_values;
+    ctx->run_scratch = scratch;
     rc = precision == ANIFEM_PREC_FP64 ? dispatch_blocks<double>(ctx, a, s) : dispatch_blocks<float>(ctx, a, s);
     ctx->run_bc = nullptr;
+    ctx->run_scratch = nullptr;
+    ctx->run_kg = nullptr;
     if (rc) return rc;
     if (ctx->timers_on) {
         ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[2], s));
@@ -761,6 +823,32 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
         return status->code;
     }
     return ANIFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
+                        const double* dir_values, int precision, double* csr_val, double* rhs,
+                        anifem_status* status, void* stream) {
+    return assembly_run(ctx, D, F, dir_values, precision, csr_val, rhs, status, stream, nullptr);
+}
+
+int anifem_assembly_run_noalloc(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F,
+                                const double* dir_values, int precision, double* csr_val, double* rhs,
+                                anifem_status* status, void* stream) {
+    using namespace anifem;
+    if (!ctx || ctx->host_only) return ANIFEM_ERR_ARG;
+    // the coloured scatter builds device tables on first use
+    if (ctx->coloured) return ANIFEM_ERR_ARG;
+    void* sp = nullptr;
+    int rc = scratch_address(ctx->device, &sp);  // looked up at create: no HIP call here
+    if (rc) return rc;
+    g_ref_lookup_only = true;
+    rc = assembly_run(ctx, D, F, dir_values, precision, csr_val, rhs, status, stream, sp);
+    g_ref_lookup_only = false;
+    return rc;
 }
 
 int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_precision, double* rhs, int accumulate,

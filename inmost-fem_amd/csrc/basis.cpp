@@ -214,6 +214,8 @@ __global__ void split_f16_kernel(const float* x, int64_t n, _Float16* out) {
     out[n + i] = (_Float16)(x[i] - (float)h);
 }
 
+thread_local bool g_ref_lookup_only = false;
+
 int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out, int face) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int, int, int, int, int, int>, RefTensorDev> cache;
@@ -226,6 +228,7 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
         *out = it->second;
         return ANIFEM_OK;
     }
+    if (g_ref_lookup_only) return ANIFEM_ERR_ARG;  // building the table would allocate
     std::vector<double> R;
     reference_tensor(opA, femA, opB, femB, order, per_point, R, face);
     std::vector<float> Rf(R.begin(), R.end());
