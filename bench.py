@@ -549,6 +549,20 @@ def main():
             "max_true_relres": float(max(max(r["learned_true"], r["jacobi_true"]) for r in rows)),
             "all_converged": bool(all(r["ok"] for r in rows)),
             "note": "wall time of each solve call (learned includes the network's factor), FP64 true residual"}
+        # the non-learned factor on the same pattern (FSAI), both factors in
+        # the same PCG with M^-1 = G G^T applied in turn
+        from anifem_amd.learned import compare_fsai
+        frows = compare_fsai(grf, net, np.arange(1_000_000, 1_000_000 + min(20, args.learned_eval)), rtol=1e-8)
+        out["pcg_learned_vs_jacobi_grf32"]["vs_fsai_same_pattern"] = {
+            "systems": len(frows),
+            "fsai_iterations_mean": float(np.mean([r["fsai_it"] for r in frows])),
+            "learned_iterations_mean": float(np.mean([r["learned_it"] for r in frows])),
+            "fsai_solve_ms_mean": float(np.mean([r["fsai_ms"] for r in frows])),
+            "learned_solve_ms_mean": float(np.mean([r["learned_ms"] for r in frows])),
+            "fsai_host_setup_ms_mean": float(np.mean([r["fsai_setup_ms"] for r in frows])),
+            "all_converged": bool(all(r["ok"] for r in frows)),
+            "note": "anifem_pcg_factored with each factor; the learned time includes the network's factor, the "
+                    "FSAI factor is built on the host outside the timed solve"}
         del net, grf
     if "batched" in sections:
         out["batched_uneven_256"] = bench_batched(args, torch)

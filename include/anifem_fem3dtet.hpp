@@ -212,14 +212,17 @@ public:
     int csr(const int64_t** row_ptr, const int32_t** col_idx) const {
         return ctx_ ? anifem_assembly_csr(ctx_, row_ptr, col_idx) : ANIFEM_ERR_ARG;
     }
-    // Assemble(A, b): device csr_val (nnz) and rhs (ndof) are overwritten.
+    // Assemble(A, b): device csr_val (nnz) and rhs (ndof) are overwritten;
+    // dir_values (device, ndof, or null = homogeneous) are the Dirichlet
+    // values applyDir imposes per cell (dc_on_dof.h:38-45).
     int Assemble(const TensorView& D, const TensorView* F, double* csr_val, double* rhs,
-                 anifem_status* st = nullptr, Precision precision = FP64, void* stream = nullptr) {
+                 anifem_status* st = nullptr, Precision precision = FP64, void* stream = nullptr,
+                 const double* dir_values = nullptr) {
         if (!ctx_) return status_ ? status_ : ANIFEM_ERR_ARG;
         const anifem_tensor tD{D.type, D.storage, D.data};
         anifem_tensor tF{};
         if (F) tF = anifem_tensor{F->type, F->storage, F->data};
-        return anifem_assembly_run(ctx_, &tD, F ? &tF : nullptr, nullptr, precision, csr_val, rhs, st, stream);
+        return anifem_assembly_run(ctx_, &tD, F ? &tF : nullptr, dir_values, precision, csr_val, rhs, st, stream);
     }
     // Assemble from element matrices computed by the caller (the reference's
     // local-function path): A_elem n_tets x nf x nf column-major, F_elem

@@ -173,6 +173,17 @@ int anifem_quadrature_npoints(int order);
 /* Quadrature rule (barycentric p[4q], weights w[q] summing to 1), host arrays. */
 int anifem_quadrature(int order, double* p, double* w);
 
+/* Physical quadrature points X_n = P0 + sum_l lambda_{l+1,n} (P_{l+1} - P0)
+ * of the volume rule `order` (the anifem_quadrature points) in every tet, on
+ * the device: the reference's internalFem3DtetGeomInit_XYG (core.inl:250-270),
+ * where it evaluates the coefficient functor (diff_tensor.h:147-153).
+ *   xyz : device, 12 x n_tets SoA as anifem_fem3dtet_batch
+ *   X   : device, 3 x q x n_tets: coordinate c of point n of tet r at
+ *         X[(c*q + n)*n_tets + r], the layout of PER_POINT coefficients, so
+ *         that a coefficient evaluated at X component-wise is a PER_POINT
+ *         tensor value without any reordering. */
+int anifem_quadrature_points_batch(int order, int64_t n_tets, const double* xyz, double* X, void* stream);
+
 /* ---- whole-mesh assembly --------------------------------------------------
  * The reference assembles cell by cell through a virtual plugin
  * (MatFuncWrap::operator(), inmost_interface/func_wrap.h:96-187) called from
@@ -274,6 +285,12 @@ int anifem_assembly_blocks(const anifem_assembly* ctx, anifem_block_layout* out)
 int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, int64_t* blk_p0, int64_t* blk_v0,
                                 int32_t* pair_tet, uint8_t* pair_ib, uint16_t* pair_pos, uint16_t* sstart,
                                 uint16_t* pstart);
+/* The physical quadrature points of the context's tets, in the caller's tet
+ * order and the local node order after the orientation reorder
+ * (ordering.inl:8-26): the points at which anifem_assembly_run expects
+ * PER_POINT coefficients, as the reference evaluates them at XYG of the
+ * reordered cell.  Layout as anifem_quadrature_points_batch. */
+int anifem_assembly_quadrature_points(const anifem_assembly* ctx, int order, double* X, void* stream);
 /* Device node coordinates (n_nodes x 3) used by run; the caller may update
  * them in place (moving mesh) between runs. */
 int anifem_assembly_coords(anifem_assembly* ctx, double** coords);
@@ -307,8 +324,10 @@ int anifem_assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anif
  * the library instead: 128 MiB (134,217,728 bytes) per device, held by every
  * device the library runs on until the process ends.  A problem needs
  * n_tets x rowlen x (8 in FP64, 4 in FP32) bytes, rowlen being the tensor
- * values per tet padded to 16 bytes (P2 with a per-tet tensor and a load:
- * 80 bytes per tet in FP64, 48 in FP32, so up to 1,677,721 / 2,796,202 tets);
+ * values per tet padded to 16 bytes (GRAD-GRAD of a scalar space with a load:
+ * 64 / 32 bytes per tet in FP64 / FP32 for a symmetric tensor, whose K is
+ * kept packed, so up to 2,097,152 / 4,194,304 tets; 80 / 48 bytes for a
+ * general one);
  * a larger one returns ANIFEM_ERR_DIMS before anything is launched.  The
  * reference tensors of the problem are built at anifem_assembly_create; the
  * coloured scatter is refused (ANIFEM_ERR_ARG).  Arguments, results and

@@ -109,6 +109,8 @@ SIGNATURES = {
     "anifem_op_dim": (_I32, [anifem_op]),
     "anifem_quadrature_npoints": (_I32, [_I32]),
     "anifem_quadrature": (_I32, [_I32, _VP, _VP]),
+    "anifem_quadrature_points_batch": (_I32, [_I32, _I64, _VP, _VP, _VP]),
+    "anifem_assembly_quadrature_points": (_I32, [_VP, _I32, _VP, _VP]),
     "anifem_fem3dface_batch": (_I32, [anifem_op, anifem_op, ctypes.POINTER(anifem_tensor), _I32, _I32, _I32, _I64,
                                       _VP, _I32, _VP, _VP]),
     "anifem_face_quadrature": (_I32, [_I32, _I32, _VP, _VP]),
@@ -275,6 +277,19 @@ def fem3dtet_batch(opA: Op, opB: Op, ttype, storage, D, order, xyz, precision=PR
     return out
 
 
+def quadrature_points_batch(order, xyz, out=None, stream=None):
+    """Physical points of the volume rule in every tet (device): xyz the
+    (12, n) SoA vertices; returns (3, q, n), the PER_POINT coefficient layout."""
+    import torch
+    n = xyz.shape[1]
+    q = len(quadrature(order)[1])
+    if out is None:
+        out = torch.empty((3, q, n), dtype=torch.float64, device=xyz.device)
+    _check(lib().anifem_quadrature_points_batch(order, n, _ptr(xyz), _ptr(out), _stream_handle(stream)),
+           "anifem_quadrature_points_batch")
+    return out
+
+
 def face_quadrature(order, face):
     """Triangle rule of `order` on face `face` of the tet: (p (q, 4) barycentric,
     w (q,) summing to 1) -- the points of per-point face coefficients."""
@@ -391,6 +406,7 @@ class Assembler:
         test = test or trial
         self.trial, self.test = trial, test
         self.tensor_type, self.tensor_storage = tensor_type, tensor_storage
+        self.order = order
         self.rhs_tensor_type, self.rhs_tensor_storage = rhs_tensor_type, rhs_tensor_storage
         self._mask = None if dirichlet_dof_mask is None else np.ascontiguousarray(dirichlet_dof_mask, dtype=np.uint8)
         mesh = anifem_mesh_desc(len(self._coords), len(self._tets), _ptr(self._coords), _ptr(self._tets))
@@ -564,6 +580,19 @@ class Assembler:
             out = torch.empty((self.n_tets, q, dim), dtype=torch.float64, device=u.device)
         _check(lib().anifem_assembly_eval(self._h, op, _ptr(xyl), q, _ptr(u), _ptr(out), _stream_handle(stream)),
                "anifem_assembly_eval")
+        return out
+
+    def quadrature_points(self, order=None, out=None, stream=None):
+        """Physical points (3, q, n_tets) of the volume rule in every tet of the
+        context (device): where PER_POINT coefficients are expected."""
+        import torch
+        order = self.order if order is None else order
+        q = len(quadrature(order)[1])
+        if out is None:
+            out = torch.empty((3, q, self.n_tets), dtype=torch.float64,
+                              device=torch.device("cuda", torch.cuda.current_device()))
+        _check(lib().anifem_assembly_quadrature_points(self._h, order, _ptr(out), _stream_handle(stream)),
+               "anifem_assembly_quadrature_points")
         return out
 
     def set_persistent(self, on=True):

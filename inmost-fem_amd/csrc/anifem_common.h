@@ -59,7 +59,15 @@ struct RefTensorDev {
     const void* r16 = nullptr;
     float r16_scale = 1.0f;
     int e0pad = 0;
+    // 3 x 3 tensors constant over the tet (KB == 9, q == 1): R with the
+    // columns of K(a, b) and K(b, a) summed, for a symmetric K packed as
+    // K00 K01 K11 K02 K12 K22 (the contraction sum_kk K R then runs over 6
+    // terms); E0 x 6, FP64 and FP32
+    const double* rs64 = nullptr;
+    const float* rs32 = nullptr;
 };
+// Packed index of K(a, b) = K(b, a) in the symmetric layout above.
+__host__ __device__ inline int sym_pack(int a, int b) { return a <= b ? a + b * (b + 1) / 2 : b + a * (a + 1) / 2; }
 int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_point, RefTensorDev* out,
                    int face = -1);
 // Set by entry points that must not allocate device memory: get_ref_tensor

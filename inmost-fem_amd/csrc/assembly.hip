@@ -21,6 +21,8 @@
 
 namespace anifem {
 int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a);
+int launch_physical_points(int order, int64_t n, const double* xyz, const double* coords, const int32_t* tetnodes,
+                           int64_t nodes_stride, double* X, hipStream_t s);
 int launch_apply(const anifem_op& op, const double* xyl, int q, ApplyArgs a, bool mesh, hipStream_t s);
 }  // namespace anifem
 
@@ -38,6 +40,8 @@ struct anifem_assembly {
     int32_t* d_pair_tet = nullptr;
     uint8_t* d_pair_ib = nullptr;
     uint16_t* d_pair_pos = nullptr;
+    uint32_t* d_pair_pos8 = nullptr;  // compact positions (compact_positions), when they fit
+    int pos8_words = 0;
     uint16_t* d_sstart = nullptr;
     uint16_t* d_pstart = nullptr;
     int64_t* d_row_ptr = nullptr;
@@ -61,6 +65,15 @@ struct anifem_assembly {
     // inhomogeneous Dirichlet values: the rows they change (LiftRow records)
     anifem::LiftRow* d_lift = nullptr;
     int64_t n_lift = 0;
+    // spaces assembled through element matrices (Piola-mapped RT0 / ND0 and
+    // one-DOF spaces): orientation signs, the cells' vertices and the element
+    // matrices and load vectors of the last run
+    bool elem_path = false;
+    int8_t* d_sign = nullptr;
+    double* d_xyz = nullptr;
+    void* d_ebuf = nullptr;
+    void* d_fbuf = nullptr;
+    size_t ebuf_bytes = 0, fbuf_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     anifem_timers timers{};
     const double* run_bc = nullptr;   // the running call's Dirichlet values (device), or null
@@ -204,19 +217,28 @@ int upload_maps(anifem_assembly* ctx, hipStream_t s) {
 
 // Persistent row-block pass: occupancy x CUs workgroups, whole XCD groups
 // (surplus workgroups exit at once).
-template <int DA0, int DB0, int PW, typename T>
-int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
+template <int DA0, int DB0, int PW, typename T, bool P8>
+int launch_persistent_p(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
     using namespace anifem;
     int occ = 0, n_cu = 0;
     ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T>), kBlock, lds));
+        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T, P8>), kBlock, lds));
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
     G = std::min<int64_t>(G, ctx->S.n_blocks);
     G = std::max<int64_t>(8, G / 8 * 8);
-    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a, Kg);
+    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T, P8>), dim3((unsigned)G), dim3(kBlock), lds, s, a, Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
+}
+
+// The compact positions when the context has them in this instantiation's
+// word count, else the 16-bit positions (identical values either way).
+template <int DA0, int DB0, int PW, typename T>
+int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
+    if (ctx->d_pair_pos8 && ctx->pos8_words == (2 + 2 * PW + 3) / 4)
+        return launch_persistent_p<DA0, DB0, PW, T, true>(ctx, a, Kg, lds, s);
+    return launch_persistent_p<DA0, DB0, PW, T, false>(ctx, a, Kg, lds, s);
 }
 
 // Persistent row-block pass for vector spaces (NC components of NF0 scalar
@@ -237,11 +259,16 @@ int launch_persistent_vec(anifem_assembly* ctx, const anifem::BlockArgs& a, cons
     return ANIFEM_OK;
 }
 
-template <int DA0, int DB0, int NCA, int NCB, typename T>
+// SYM: K symmetric (the same 3-dimensional operator on both sides of a scalar
+// space, a NULL / SCALAR / SYMMETRIC tensor constant over each tet): the
+// geometry rows hold K packed to 6 values and the row-block passes run as
+// for 6 x 1 blocks against the column-summed R of the kind tables.
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM>
 int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     using namespace anifem;
+    constexpr int KDA = SYM ? 6 : DA0, KDB = SYM ? 1 : DB0;  // contraction block of the row-block passes
     const AssemblySetup& S = ctx->S;
-    const int len = a.m.q * DA0 * DB0, lenF = a.has_rhs ? a.f.q : 0;
+    const int len = a.m.q * KDA * KDB, lenF = a.has_rhs ? a.f.q : 0;
     const int kstride = NCA * NCB * len;
     a.rowlen = pad16<T>(kstride + (a.has_rhs ? NCB * lenF : 0));
     int max_lds = 0;
@@ -278,21 +305,21 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     }
     ctx->run_kg = Kg;
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
+    hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T, SYM>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
                        S.n_tets);
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
-    if (ctx->coloured) return launch_coloured<DA0, DB0, NCA, NCB, T>(ctx, a, Kg, s);
+    if (ctx->coloured) return launch_coloured<KDA, KDB, NCA, NCB, T>(ctx, a, Kg, s);
     if (S.n_blocks == 0) return ANIFEM_OK;
     // persistent pass for scalar spaces with one carried point
     constexpr bool scalar_space = NCA == 1 && NCB == 1;
-    if (scalar_space && ctx->persistent && staged && len == DA0 * DB0 && (!a.has_rhs || lenF == 1) &&
+    if (scalar_space && ctx->persistent && staged && len == KDA * KDB && (!a.has_rhs || lenF == 1) &&
         a.kinds[0].nf <= 20) {
         // position words per pair: the shortest instantiation that holds nf
         const int nf = a.kinds[0].nf;
-        if (nf <= 4) return launch_persistent<DA0, DB0, 2, T>(ctx, a, Kg, lds, s);
-        if (nf <= 10) return launch_persistent<DA0, DB0, 5, T>(ctx, a, Kg, lds, s);
-        return launch_persistent<DA0, DB0, 10, T>(ctx, a, Kg, lds, s);
+        if (nf <= 4) return launch_persistent<KDA, KDB, 2, T>(ctx, a, Kg, lds, s);
+        if (nf <= 10) return launch_persistent<KDA, KDB, 5, T>(ctx, a, Kg, lds, s);
+        return launch_persistent<KDA, KDB, 10, T>(ctx, a, Kg, lds, s);
     }
     // persistent pass for 3-component vector spaces of P1 or P2 components:
     // the reference tensors staged in LDS whenever one workgroup fits a CU
@@ -300,7 +327,7 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     if constexpr (NCA == 3 && NCB == 3) {
         const KindTable& K0 = a.kinds[0];
         const size_t lds_v = lds_of(true);
-        if (ctx->persistent && lds_v <= (size_t)max_lds && a.max_R_rows * len < 65536 && len == DA0 * DB0 &&
+        if (ctx->persistent && lds_v <= (size_t)max_lds && a.max_R_rows * len < 65536 && len == KDA * KDB &&
             (!a.has_rhs || lenF == 1) && K0.nfA0 == K0.nfB0) {
             if (K0.nfA0 == 4) return launch_persistent_vec<DA0, DB0, 3, 4, T>(ctx, a, (const T*)Kg, lds_v, s);
             if (K0.nfA0 == 10) return launch_persistent_vec<DA0, DB0, 3, 10, T>(ctx, a, (const T*)Kg, lds_v, s);
@@ -309,10 +336,10 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     // 8 equal XCD ranges of the processing order (padding workgroups exit)
     const unsigned grid = (unsigned)(8 * ((S.n_blocks + 7) / 8));
     if (staged)
-        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, true>), dim3(grid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_kernel<KDA, KDB, NCA, NCB, T, true>), dim3(grid), dim3(kBlock), lds, s, a,
                            (const T*)Kg);
     else
-        hipLaunchKernelGGL((row_block_kernel<DA0, DB0, NCA, NCB, T, false>), dim3(grid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_kernel<KDA, KDB, NCA, NCB, T, false>), dim3(grid), dim3(kBlock), lds, s, a,
                            (const T*)Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
@@ -320,25 +347,27 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
 
 // The row-block pass, then the rows an inhomogeneous Dirichlet condition
 // changes (when the call has Dirichlet values and a rhs).
-template <int DA0, int DB0, int NCA, int NCB, typename T>
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM = false>
 int launch_blocks_lift(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     using namespace anifem;
-    int rc = launch_blocks<DA0, DB0, NCA, NCB, T>(ctx, a, s);
+    constexpr int KDA = SYM ? 6 : DA0, KDB = SYM ? 1 : DB0;
+    int rc = launch_blocks<DA0, DB0, NCA, NCB, T, SYM>(ctx, a, s);
     if (rc || !ctx->run_bc || !a.rhs || ctx->n_lift == 0) return rc;
-    const int len = a.m.q * DA0 * DB0, lenF = a.has_rhs ? a.f.q : 0;
+    const int len = a.m.q * KDA * KDB, lenF = a.has_rhs ? a.f.q : 0;
     a.rowlen = pad16<T>(NCA * NCB * len + (a.has_rhs ? NCB * lenF : 0));  // as launch_blocks sets it
     const int64_t blocks = (ctx->n_lift + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((dirichlet_lift_kernel<DA0, DB0, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
+    hipLaunchKernelGGL((dirichlet_lift_kernel<KDA, KDB, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
                        static_cast<const T*>(ctx->run_kg), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->run_bc);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
 
 template <typename T>
-int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, hipStream_t s) {
+int dispatch_blocks(anifem_assembly* ctx, const anifem::BlockArgs& a, bool sym, hipStream_t s) {
     const int dA = anifem::ref_dim(ctx->prob.trial.op, ctx->prob.trial.fem),
               dB = anifem::ref_dim(ctx->prob.test.op, ctx->prob.test.fem);
     const int nc = ctx->prob.trial.ncomp;
+    if (dA == 3 && dB == 3 && nc == 1 && sym) return launch_blocks_lift<3, 3, 1, 1, T, true>(ctx, a, s);
     if (dA == 3 && dB == 3 && nc == 1) return launch_blocks_lift<3, 3, 1, 1, T>(ctx, a, s);
     if (dA == 3 && dB == 3 && nc == 3) return launch_blocks_lift<3, 3, 3, 3, T>(ctx, a, s);
     if (dA == 1 && dB == 1 && nc == 1) return launch_blocks_lift<1, 1, 1, 1, T>(ctx, a, s);
@@ -458,8 +487,8 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
-    // the global assembly numbers the DOFs of P0..P3 spaces
-    if (!fem_lagrange(prob->trial.fem) || !fem_lagrange(prob->test.fem)) return ANIFEM_ERR_ARG;
+    // the global assembly numbers the DOFs of the spaces of space_layout
+    if (!space_layout(prob->trial.fem).valid || !space_layout(prob->test.fem).valid) return ANIFEM_ERR_ARG;
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
     const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
@@ -532,6 +561,12 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     if ((rc = upload(&ctx->d_pair_tet, S.pair_tet.data(), S.pair_tet.size(), s))) return rc;
     if ((rc = upload(&ctx->d_pair_ib, S.pair_ib.data(), S.pair_ib.size(), s))) return rc;
     if ((rc = upload(&ctx->d_pair_pos, S.pair_pos.data(), S.pair_pos.size(), s))) return rc;
+    {
+        std::vector<uint32_t> p8;
+        ctx->pos8_words = compact_positions(S, p8);
+        if (ctx->pos8_words && (rc = upload(&ctx->d_pair_pos8, p8.data(), p8.size(), s))) return rc;
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));  // p8 is a host temporary
+    }
     // one entry of padding: the persistent pass reads slot starts as aligned
     // 32-bit words, the last of which may reach one entry past the array
     S.sstart.push_back(0);
@@ -540,6 +575,10 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     if ((rc = upload(&ctx->d_row_ptr, S.row_ptr.data(), S.row_ptr.size(), s))) return rc;
     if ((rc = upload(&ctx->d_col_idx, S.col_idx.data(), S.col_idx.size(), s))) return rc;
     ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_status, 4 * sizeof(int)));
+    // spaces whose elements need the Piola map and per-DOF factors (RT0,
+    // ND0) or have one DOF (P0, B4) are assembled from element matrices
+    ctx->elem_path = fem_vector(prob->trial.fem) || (S.nf & 1);
+    if (!S.sign.empty() && (rc = upload(&ctx->d_sign, S.sign.data(), S.sign.size(), s))) return rc;
     // rows an inhomogeneous Dirichlet condition changes: Dirichlet rows and
     // rows with a dropped (Dirichlet-column) entry in one of their pairs
     {
@@ -592,7 +631,8 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
                         ctx->d_blk_v0,   ctx->d_pair_tet, ctx->d_pair_ib,  ctx->d_pair_pos, ctx->d_sstart,
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
                         ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets,
-                        ctx->d_lift};
+                        ctx->d_lift,     ctx->d_pair_pos8, ctx->d_sign,   ctx->d_xyz,      ctx->d_ebuf,
+                        ctx->d_fbuf};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -724,6 +764,12 @@ int anifem_assembly_timers(const anifem_assembly* ctx, anifem_timers* out) {
     return ANIFEM_OK;
 }
 
+int anifem_assembly_quadrature_points(const anifem_assembly* ctx, int order, double* X, void* stream) {
+    if (!ctx || ctx->host_only) return ANIFEM_ERR_ARG;
+    return anifem::launch_physical_points(order, ctx->S.n_tets, nullptr, ctx->d_coords, ctx->d_tetnodes,
+                                          ctx->S.n_tets, X, static_cast<hipStream_t>(stream));
+}
+
 int anifem_assembly_coords(anifem_assembly* ctx, double** coords) {
     if (!ctx || ctx->host_only || !coords) return ANIFEM_ERR_ARG;
     *coords = ctx->d_coords;
@@ -733,6 +779,92 @@ int anifem_assembly_coords(anifem_assembly* ctx, double** coords) {
 }  // extern "C"
 
 namespace {
+
+// The cells' vertices as the 12 x n SoA of the element batch.
+__global__ void gather_xyz_kernel(const double* __restrict__ coords, const int32_t* __restrict__ tetnodes, int64_t n,
+                                  double* __restrict__ xyz) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int v = 0; v < 4; ++v) {
+        const int64_t node = tetnodes[v * n + r];
+        for (int c = 0; c < 3; ++c) xyz[(int64_t)(3 * v + c) * n + r] = coords[3 * node + c];
+    }
+}
+
+// Global basis = sign x local basis on oriented DOFs (assembler.inl:154-164):
+// A(ib, ia) *= s_ib s_ia, F(ib) *= s_ib, one lane per matrix entry.
+template <typename T>
+__global__ void apply_signs_kernel(int64_t n, int nf, const int8_t* __restrict__ sign, T* A, T* F) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t E = (int64_t)nf * nf;
+    if (i >= n * E) return;
+    const int64_t r = i / E;
+    const int e = (int)(i - r * E), ib = e % nf, ia = e / nf;
+    const int8_t* sg = sign + r * nf;
+    const int s = sg[ib] * sg[ia];
+    if (s < 0) A[i] = -A[i];
+    if (F && ia == 0 && sg[ib] < 0) F[r * nf + ib] = -F[r * nf + ib];
+}
+
+template <typename T>
+int grow(void** buf, size_t* have, size_t need) {
+    if (*have >= need) return ANIFEM_OK;
+    if (*buf) ANIFEM_HIP_CHECK(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+    ANIFEM_HIP_CHECK(hipMalloc(buf, need));
+    *have = need;
+    return ANIFEM_OK;
+}
+
+// Assembly through element matrices, for the spaces whose elements the
+// fused pass does not form (RT0 / ND0: Piola map, per-DOF factors and
+// orientation signs; one-DOF spaces): the element batch (anifem_fem3dtet_batch,
+// the reference's runtime fem3Dtet over any operator pair, int_tet.inl:2-27)
+// on the reordered cells, the signs of oriented DOFs, then the row-block
+// scatter of element matrices (anifem_assembly_run_elements) with
+// element-level Dirichlet.
+int run_via_elements(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, int precision,
+                     double* csr_val, double* rhs, anifem_status* status, hipStream_t s) {
+    using namespace anifem;
+    const AssemblySetup& S = ctx->S;
+    const int64_t n = S.n_tets;
+    const int nf = S.nf;
+    const size_t esz = precision == ANIFEM_PREC_FP64 ? 8 : 4;
+    int rc;
+    size_t xyz_bytes = ctx->d_xyz ? (size_t)12 * n * sizeof(double) : 0;
+    void* xp = ctx->d_xyz;
+    if ((rc = grow<double>(&xp, &xyz_bytes, (size_t)12 * n * sizeof(double)))) return rc;
+    ctx->d_xyz = static_cast<double*>(xp);
+    if ((rc = grow<double>(&ctx->d_ebuf, &ctx->ebuf_bytes, (size_t)n * nf * nf * esz))) return rc;
+    const bool do_rhs = F && rhs;
+    if (do_rhs && (rc = grow<double>(&ctx->d_fbuf, &ctx->fbuf_bytes, (size_t)n * nf * esz))) return rc;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(gather_xyz_kernel, dim3(blocks), dim3(256), 0, s, ctx->d_coords, ctx->d_tetnodes, n,
+                       ctx->d_xyz);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    if ((rc = anifem_fem3dtet_batch(ctx->prob.trial, ctx->prob.test, D, ctx->prob.order, n, ctx->d_xyz, precision,
+                                    ctx->d_ebuf, s)))
+        return rc;
+    if (do_rhs) {
+        const anifem_op p0{ANIFEM_IDEN, ANIFEM_FEM_P0, 1};
+        const anifem_op tb{ANIFEM_IDEN, ctx->prob.test.fem, ctx->prob.test.ncomp};
+        if ((rc = anifem_fem3dtet_batch(p0, tb, F, ctx->prob.rhs_order, n, ctx->d_xyz, precision, ctx->d_fbuf, s)))
+            return rc;
+    }
+    if (ctx->d_sign) {
+        const unsigned sb = (unsigned)((n * nf * nf + 255) / 256);
+        if (precision == ANIFEM_PREC_FP64)
+            hipLaunchKernelGGL(apply_signs_kernel<double>, dim3(sb), dim3(256), 0, s, n, nf, ctx->d_sign,
+                               static_cast<double*>(ctx->d_ebuf), do_rhs ? static_cast<double*>(ctx->d_fbuf) : nullptr);
+        else
+            hipLaunchKernelGGL(apply_signs_kernel<float>, dim3(sb), dim3(256), 0, s, n, nf, ctx->d_sign,
+                               static_cast<float*>(ctx->d_ebuf), do_rhs ? static_cast<float*>(ctx->d_fbuf) : nullptr);
+        ANIFEM_HIP_CHECK(hipGetLastError());
+    }
+    return anifem_assembly_run_elements(ctx, ctx->d_ebuf, do_rhs ? ctx->d_fbuf : nullptr, precision, csr_val, rhs,
+                                        status, s);
+}
 
 // anifem_assembly_run, and its no-allocation form (scratch: this device's
 // scratch array, which then holds the geometry rows).
@@ -748,6 +880,12 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     const bool do_rhs = F && rhs;
     if (do_rhs && (F->type != ctx->prob.rhs_tensor_type || F->storage != ctx->prob.rhs_tensor_storage))
         return ANIFEM_ERR_ARG;
+    if (ctx->elem_path) {
+        // element matrices in buffers of the context: no Dirichlet values, no
+        // no-allocation form, no coloured scatter
+        if (dir_values || scratch || ctx->coloured) return ANIFEM_ERR_ARG;
+        return run_via_elements(ctx, D, F, precision, csr_val, rhs, status, static_cast<hipStream_t>(stream));
+    }
     const AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
@@ -767,8 +905,17 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     }
     a.has_rhs = do_rhs ? 1 : 0;
     a.n_kinds = 1;
-    a.kinds[0] = KindTable{a.m.Rd, a.m.Rf, a.f.Rd, a.f.Rf, a.m.nfA0, a.m.nfB0, S.nf, a.m.nfA0 * a.m.nfB0,
-                           do_rhs ? a.f.nfB0 : 0, 0};
+    // K symmetric: the same 3-dimensional operator of one scalar space on both
+    // sides (same selector, no Piola map) and a symmetric tensor constant over
+    // each tet
+    const anifem_op& tr = ctx->prob.trial;
+    const anifem_op& te = ctx->prob.test;
+    const bool sym = tr.op == te.op && tr.fem == te.fem && tr.ncomp == 1 && te.ncomp == 1 && a.m.Rds &&
+                     a.m.q == 1 && ref_dim(tr.op, tr.fem) == 3 && !(a.m.mapA | a.m.mapB | a.m.scA | a.m.scB) &&
+                     (D->type == ANIFEM_TENSOR_NULL || D->type == ANIFEM_TENSOR_SCALAR ||
+                      D->type == ANIFEM_TENSOR_SYMMETRIC);
+    a.kinds[0] = KindTable{sym ? a.m.Rds : a.m.Rd, sym ? a.m.Rfs : a.m.Rf, a.f.Rd, a.f.Rf, a.m.nfA0, a.m.nfB0, S.nf,
+                           a.m.nfA0 * a.m.nfB0, do_rhs ? a.f.nfB0 : 0, 0};
     a.max_R_rows = a.kinds[0].R_rows;
     a.max_RF_rows = a.kinds[0].RF_rows;
     a.n_blocks = S.n_blocks;
@@ -780,6 +927,7 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     a.pair_tet = ctx->d_pair_tet;
     a.pair_ib = ctx->d_pair_ib;
     a.pair_pos = ctx->d_pair_pos;
+    a.pair_pos8 = ctx->d_pair_pos8;
     a.sstart = ctx->d_sstart;
     a.pstart = ctx->d_pstart;
     a.val = csr_val;
@@ -796,7 +944,7 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     ANIFEM_HIP_CHECK(hipGetLastError());
     ctx->run_bc = dir_values;
     ctx->run_scratch = scratch;
-    rc = precision == ANIFEM_PREC_FP64 ? dispatch_blocks<double>(ctx, a, s) : dispatch_blocks<float>(ctx, a, s);
+    rc = precision == ANIFEM_PREC_FP64 ? dispatch_blocks<double>(ctx, a, sym, s) : dispatch_blocks<float>(ctx, a, sym, s);
     ctx->run_bc = nullptr;
     ctx->run_scratch = nullptr;
     ctx->run_kg = nullptr;
@@ -898,7 +1046,10 @@ int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_p
 int anifem_assembly_eval(anifem_assembly* ctx, int op, const double* xyl, int q, const double* u, double* opU,
                          void* stream) {
     using namespace anifem;
-    if (!ctx || ctx->host_only || !u || !opU || (op != ANIFEM_IDEN && op != ANIFEM_GRAD)) return ANIFEM_ERR_ARG;
+    if (!ctx || ctx->host_only || !u || !opU) return ANIFEM_ERR_ARG;
+    // IDEN / GRAD, and DIV / CURL of the vector-valued spaces
+    if (op != ANIFEM_IDEN && op != ANIFEM_GRAD && !(fem_vector(ctx->prob.trial.fem) && op != ANIFEM_GRAD))
+        return ANIFEM_ERR_ARG;
     const AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
@@ -908,6 +1059,7 @@ int anifem_assembly_eval(anifem_assembly* ctx, int op, const double* xyl, int q,
     a.coords = ctx->d_coords;
     a.tetnodes = ctx->d_tetnodes;
     a.dofmap = ctx->d_dofs;
+    a.sign = ctx->d_sign;
     a.u = u;
     a.out = opU;
     return launch_apply(anifem_op{op, ctx->prob.trial.fem, ctx->prob.trial.ncomp}, xyl, q, a, true, s);
@@ -987,6 +1139,7 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
     a.pair_tet = ctx->d_pair_tet;
     a.pair_ib = ctx->d_pair_ib;
     a.pair_pos = ctx->d_pair_pos;
+    a.pair_pos8 = ctx->d_pair_pos8;
     a.sstart = ctx->d_sstart;
     a.pstart = ctx->d_pstart;
     a.val = csr_val;

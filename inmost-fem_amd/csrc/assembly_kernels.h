@@ -57,6 +57,7 @@ struct BlockArgs {
     const int32_t* pair_tet;
     const uint8_t* pair_ib;     // local row index | kPairDirichlet
     const uint16_t* pair_pos;   // nf per pair: value-list position or kPosDropped
+    const uint32_t* pair_pos8;  // compact positions (compact_positions), or null
     const uint16_t* sstart;     // per slot: first value, relative to its block
     const uint16_t* pstart;     // per row: first pair, relative to its block
     double* val;
@@ -116,8 +117,12 @@ __device__ __forceinline__ void pad_copy(V* __restrict__ dst, const V* __restric
 // |T| (core.inl:218-248), then the geometry tensors of the matrix (NCA*NCB
 // blocks of q*DA0*DB0 values) and of the load vector (NCB blocks of qF
 // values), stored to Kg[r*rowlen ...] in the T precision of the contraction.
-template <int DA0, int DB0, int NCA, int NCB, typename T>
+// SYM (3 x 3 tensors of a symmetric K, one component, one carried point): the
+// matrix part of the row is K packed to its 6 distinct values (sym_pack),
+// which the row-block passes contract with the column-summed R.
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM = false>
 __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __restrict__ Kg, int64_t n_tets) {
+    static_assert(!SYM || (DA0 == 3 && DB0 == 3 && NCA == 1 && NCB == 1), "packed K: 3 x 3 scalar blocks");
     // short rows (scalar spaces, one carried point) are assembled in LDS and
     // stored by the workgroup as one contiguous range: coalesced stores
     // instead of one strided store per value and lane
@@ -128,14 +133,23 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
     const int64_t r = r0 + threadIdx.x;
     int degenerate = 0;
     if (r < n_tets) {
-        const int kstride = NCA * NCB * a.m.q * DA0 * DB0;
+        const int kstride = SYM ? 6 : NCA * NCB * a.m.q * DA0 * DB0;
         double P[4][3], PSI[9], det;
         load_tet<true>(a.m, r, P);
         tet_geometry(P, PSI, det);
         degenerate = !(fabs(det) > 0.0 && isfinite(det));
         const double mes = fabs(det) / 6.0;
         T* row = staged ? stage + threadIdx.x * a.rowlen : Kg + r * a.rowlen;
-        geometry_tensor<DA0, DB0, NCA, NCB, T>(a.m, r, PSI, mes, row);
+        if constexpr (SYM) {
+            double K9[9];  // K(b, a) at b*3 + a
+            geometry_block<3, 3, double>(a.m, r, PSI, mes, 0, 0, 0, K9);
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int c = 0; c <= b; ++c) row[sym_pack(c, b)] = (T)K9[b * 3 + c];
+        } else {
+            geometry_tensor<DA0, DB0, NCA, NCB, T>(a.m, r, PSI, mes, row);
+        }
         if (a.has_rhs) geometry_tensor<1, 1, 1, NCB, T>(a.f, r, PSI, mes, row + kstride);
     }
     if (staged) {  // uniform over the workgroup
@@ -578,12 +592,15 @@ __device__ __forceinline__ int slot_words(int64_t s0, int ns) { return (int)(((s
 // records have arrived, so the gather latency hides behind phases A and B.
 // Same arithmetic and summation order as row_block_kernel: the results are
 // bitwise identical.
-template <int DA0, int DB0, int PW, typename T>
+// P8: the value positions come from the compact list (a 16-bit base and one
+// byte per entry, P8 words per pair) instead of 16-bit positions.
+template <int DA0, int DB0, int PW, typename T, bool P8 = false>
 // Five waves per SIMD (the LDS of five 30 KB workgroups fills a CU): at most
 // 96 VGPRs, measured faster than four unconstrained waves and than spilling
 // to six or eight.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void row_block_persistent(BlockArgs a, const T* __restrict__ Kg) {
     constexpr int KB = DA0 * DB0;
+    constexpr int NPW = P8 ? (2 + 2 * PW + 3) / 4 : PW;  // position words per pair
     // slot starts carried in registers for blocks of at most 2*SW*kBlock - 2
     // slots (the registers freed by the shorter position lists of P1 and P2);
     // else copied to LDS when the block's turn comes
@@ -620,17 +637,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     int64_t b = W.take();
     if (b < 0) return;  // uniform over the workgroup
 
-    using Regs = PairRegs<KB, PW, SW, T>;
+    using Regs = PairRegs<KB, NPW, SW, T>;
     auto load_records = [&](const BlockMeta& Mx, Regs& P) {
         const int nf = a.kinds[Mx.kind].nf;
         const bool hp = (int)threadIdx.x < Mx.np;
         const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
         P.tet = hp ? a.pair_tet[gp] : 0;
         P.ibf = hp ? a.pair_ib[gp] : 0;
-        const uint32_t* pw =
-            reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * nf);
+        if constexpr (P8) {
+            const uint32_t* pw = a.pair_pos8 + gp * NPW;
 #pragma unroll
-        for (int w = 0; w < PW; ++w) P.posw[w] = (hp && 2 * w < nf) ? pw[w] : 0u;
+            for (int w = 0; w < NPW; ++w) P.posw[w] = hp ? pw[w] : 0u;
+        } else {
+            const uint32_t* pw =
+                reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * nf);
+#pragma unroll
+            for (int w = 0; w < PW; ++w) P.posw[w] = (hp && 2 * w < nf) ? pw[w] : 0u;
+        }
     };
     auto load_k = [&](const BlockMeta& Mx, Regs& P) {
         const bool hp = (int)threadIdx.x < Mx.np;
@@ -701,10 +724,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
             const int ib = cur.ibf & 63;
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + (ib % nfB0) * lenp;
+            const int base8 = (int)(cur.posw[0] & 0xFFFFu);
 #pragma unroll
             for (int ia0 = 0; ia0 < 2 * PW; ++ia0) {
-                const uint32_t w = cur.posw[ia0 >> 1];
-                const int ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                int ps;
+                if constexpr (P8) {
+                    // byte 2 + ia0 of the pair's words: offset from the base, 0xFF dropped
+                    const uint32_t o = (cur.posw[(2 + ia0) >> 2] >> (((2 + ia0) & 3) * 8)) & 0xFFu;
+                    ps = o == 0xFFu ? (int)kPosDropped : base8 + (int)o;
+                } else {
+                    const uint32_t w = cur.posw[ia0 >> 1];
+                    ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                }
                 if (ia0 < nfA0 && ps != kPosDropped) {
                     const T* Rr = Rb + nfB0 * ia0 * lenp;
                     T acc = 0;

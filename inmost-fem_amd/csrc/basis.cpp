@@ -251,6 +251,22 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
     ANIFEM_HIP_CHECK(hipMemcpy(d32, Rf.data(), Rf.size() * sizeof(float), hipMemcpyHostToDevice));
     t.r64 = static_cast<const double*>(d64);
     t.r32 = static_cast<const float*>(d32);
+    if (!per_point && t.KB == 9) {
+        // columns kk = b*3 + a of K(b, a), summed onto the packed K(a, b)
+        std::vector<double> Rs((size_t)t.E0 * 6, 0.0);
+        for (int e = 0; e < t.E0; ++e)
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) Rs[(size_t)e * 6 + sym_pack(a, b)] += R[(size_t)e * 9 + b * 3 + a];
+        std::vector<float> Rsf(Rs.begin(), Rs.end());
+        void* ds64 = nullptr;
+        void* ds32 = nullptr;
+        ANIFEM_HIP_CHECK(hipMalloc(&ds64, Rs.size() * sizeof(double)));
+        ANIFEM_HIP_CHECK(hipMalloc(&ds32, Rsf.size() * sizeof(float)));
+        ANIFEM_HIP_CHECK(hipMemcpy(ds64, Rs.data(), Rs.size() * sizeof(double), hipMemcpyHostToDevice));
+        ANIFEM_HIP_CHECK(hipMemcpy(ds32, Rsf.data(), Rsf.size() * sizeof(float), hipMemcpyHostToDevice));
+        t.rs64 = static_cast<const double*>(ds64);
+        t.rs32 = static_cast<const float*>(ds32);
+    }
     if (!per_point && t.KB <= 16) {
         // R / max|R| in [-1, 1], rows padded to 16 columns and to a multiple of 16 rows
         double mx = 0;

@@ -184,7 +184,9 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         if ((rc = prepare_elem_args(anifem_op{ANIFEM_IDEN, ANIFEM_FEM_P0, 1}, anifem_op{ANIFEM_IDEN, fem, 1}, &F,
                                     order_of(fem), ef)))
             return rc;
-        a.kinds[k] = KindTable{em.Rd, em.Rf, ef.Rd, ef.Rf, em.nfA0, em.nfB0, em.nfB0, em.nfA0 * em.nfB0, ef.nfB0, 0};
+        // kappa PSI^T PSI is symmetric: K packed to 6 values, the column-summed R
+        if (!em.Rds) return ANIFEM_ERR_ARG;
+        a.kinds[k] = KindTable{em.Rds, em.Rfs, ef.Rd, ef.Rf, em.nfA0, em.nfB0, em.nfB0, em.nfA0 * em.nfB0, ef.nfB0, 0};
         a.max_R_rows = std::max(a.max_R_rows, a.kinds[k].R_rows);
         a.max_RF_rows = std::max(a.max_RF_rows, a.kinds[k].RF_rows);
         if (first < 0) {
@@ -202,7 +204,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     a.m.pos0 = 0;
     a.f.coef_n = b->n_tets;
     a.has_rhs = 1;
-    a.rowlen = pad16<double>(9 + 1);
+    a.rowlen = pad16<double>(6 + 1);
     a.n_blocks = b->n_blocks;
     a.meta = b->d_meta;
     a.pair_tet = b->d_pair_tet;
@@ -220,7 +222,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     int max_lds = 0;
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
     const size_t lds_staged =
-        block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, a.max_R_rows * pad16<double>(1 * 9),
+        block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, a.max_R_rows * pad16<double>(6),
                          a.max_RF_rows * pad16<double>(1))
             .total;
     const bool staged = lds_staged <= (size_t)max_lds / 2;
@@ -235,9 +237,9 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     int max_nf = 0;
     for (int k = 0; k < kKinds; ++k)
         if (b->kind_used[k]) max_nf = std::max(max_nf, a.kinds[k].nf);
-    const void* pkernel = max_nf <= 4    ? reinterpret_cast<const void*>(&row_block_persistent<3, 3, 2, double>)
-                          : max_nf <= 10 ? reinterpret_cast<const void*>(&row_block_persistent<3, 3, 5, double>)
-                                         : reinterpret_cast<const void*>(&row_block_persistent<3, 3, 10, double>);
+    const void* pkernel = max_nf <= 4    ? reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double>)
+                          : max_nf <= 10 ? reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double>)
+                                         : reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double>);
     int64_t pgrid = 0;
     if (staged && b->persistent) {
         int occ = 0, n_cu = 0;
@@ -254,23 +256,23 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     ANIFEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, s, b->d_status);
     const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
+    hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
                        b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
     if (pgrid > 0 && max_nf <= 4)
-        hipLaunchKernelGGL((row_block_persistent<3, 3, 2, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_persistent<6, 1, 2, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else if (pgrid > 0 && max_nf <= 10)
-        hipLaunchKernelGGL((row_block_persistent<3, 3, 5, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_persistent<6, 1, 5, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else if (pgrid > 0)
-        hipLaunchKernelGGL((row_block_persistent<3, 3, 10, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_persistent<6, 1, 10, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else if (staged)
-        hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     else
-        hipLaunchKernelGGL((row_block_kernel<3, 3, 1, 1, double, false>), dim3(grid), dim3(kBlock), lds, s, a,
+        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, false>), dim3(grid), dim3(kBlock), lds, s, a,
                            (const double*)b->d_K);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
@@ -451,7 +453,7 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     if ((rc = upload_b(p, &p->d_col_idx, col_idx, s))) return rc;
     ANIFEM_HIP_CHECK(hipMalloc((void**)&p->d_status, 4 * sizeof(int)));
     p->bufs.push_back(p->d_status);
-    ANIFEM_HIP_CHECK(hipMalloc((void**)&p->d_K, (size_t)p->n_tets * pad16<double>(10) * sizeof(double)));
+    ANIFEM_HIP_CHECK(hipMalloc((void**)&p->d_K, (size_t)p->n_tets * pad16<double>(6 + 1) * sizeof(double)));
     p->bufs.push_back(p->d_K);
     ANIFEM_HIP_CHECK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));

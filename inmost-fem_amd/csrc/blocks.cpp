@@ -163,4 +163,39 @@ int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::
     return ANIFEM_OK;
 }
 
+int compact_positions(const AssemblySetup& S, std::vector<uint32_t>& out) {
+    const int nf = S.nf;
+    const int words = (2 + nf + 3) / 4;
+    const int64_t NP = (int64_t)S.pair_tet.size();
+    out.assign((size_t)NP * words, 0u);
+    std::atomic<bool> ok{true};
+    host_parallel_for(NP, 1 << 14, [&](int64_t b, int64_t e) {
+        for (int64_t p = b; p < e && ok.load(std::memory_order_relaxed); ++p) {
+            const uint16_t* pos = &S.pair_pos[(size_t)p * nf];
+            int lo = 0xFFFF, hi = -1;
+            for (int ia = 0; ia < nf; ++ia)
+                if (pos[ia] != kPosDropped) {
+                    lo = std::min<int>(lo, pos[ia]);
+                    hi = std::max<int>(hi, pos[ia]);
+                }
+            if (hi < 0 || hi - lo >= 0xFF) {
+                ok = false;
+                return;
+            }
+            uint8_t bytes[4 * 16] = {};
+            bytes[0] = (uint8_t)(lo & 0xFF);
+            bytes[1] = (uint8_t)(lo >> 8);
+            for (int ia = 0; ia < nf; ++ia) bytes[2 + ia] = pos[ia] == kPosDropped ? 0xFF : (uint8_t)(pos[ia] - lo);
+            for (int w = 0; w < words; ++w)
+                out[(size_t)p * words + w] = (uint32_t)bytes[4 * w] | (uint32_t)bytes[4 * w + 1] << 8 |
+                                             (uint32_t)bytes[4 * w + 2] << 16 | (uint32_t)bytes[4 * w + 3] << 24;
+        }
+    });
+    if (!ok.load()) {
+        std::vector<uint32_t>().swap(out);
+        return 0;
+    }
+    return words;
+}
+
 }  // namespace anifem

@@ -52,6 +52,8 @@ static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem
     a.q = R.q;
     a.Rd = R.r64;
     a.Rf = R.r32;
+    a.Rds = R.rs64;
+    a.Rfs = R.rs32;
     a.wq = R.w;
     a.r16 = R.r16;
     a.r16_scale = R.r16_scale;
@@ -145,7 +147,65 @@ int launch_element_mfma(const anifem_op& opA, const anifem_op& opB, const ElemAr
     return ANIFEM_ERR_ARG;
 }
 
+// Physical quadrature points of every tet (internalFem3DtetGeomInit_XYG,
+// core.inl:250-270): X_n = P0 + sum_l lambda_{l+1,n} (P_{l+1} - P0), one lane
+// per tet, stored X[(c*q + n)*n + r] (coalesced over tets; the layout of
+// per-point coefficient arrays).
+struct RulePoints {
+    double lam[24][3];  // lambda_1..3 of each point
+    int q;
+};
+
+template <bool MESH>
+__global__ __launch_bounds__(kBlock) void physical_points_kernel(ElemArgs a, RulePoints rp, double* __restrict__ X) {
+    const int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (r >= a.n) return;
+    double P[4][3];
+    load_tet<MESH>(a, r, P);
+    for (int n = 0; n < rp.q; ++n)
+        for (int c = 0; c < 3; ++c) {
+            double x = P[0][c];
+            for (int l = 0; l < 3; ++l) x += rp.lam[n][l] * (P[l + 1][c] - P[0][c]);
+            X[((int64_t)c * rp.q + n) * a.n + r] = x;
+        }
+}
+
+// Physical points of the volume rule `order` for n tets, from SoA vertices
+// (xyz) or from a mesh (coords + SoA tet nodes, tets [pos0, pos0 + n)).
+int launch_physical_points(int order, int64_t n, const double* xyz, const double* coords, const int32_t* tetnodes,
+                           int64_t nodes_stride, double* X, hipStream_t s) {
+    const double *p, *w;
+    const int q = quadrature(order, &p, &w);
+    if (q < 0) return ANIFEM_ERR_ORDER;
+    if (q > 24 || n < 0 || !X || (!xyz && (!coords || !tetnodes))) return ANIFEM_ERR_ARG;
+    RulePoints rp{};
+    rp.q = q;
+    for (int k = 0; k < q; ++k)
+        for (int l = 0; l < 3; ++l) rp.lam[k][l] = p[4 * k + l + 1];
+    ElemArgs a{};
+    a.n = n;
+    a.xyz = xyz;
+    a.coords = coords;
+    a.tetnodes = tetnodes;
+    a.nodes_stride = nodes_stride;
+    a.pos0 = 0;
+    if (n == 0) return ANIFEM_OK;
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    if (xyz)
+        hipLaunchKernelGGL(physical_points_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, a, rp, X);
+    else
+        hipLaunchKernelGGL(physical_points_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, a, rp, X);
+    ANIFEM_HIP_CHECK(hipGetLastError());
+    return ANIFEM_OK;
+}
+
 }  // namespace anifem
+
+extern "C" int anifem_quadrature_points_batch(int order, int64_t n_tets, const double* xyz, double* X, void* stream) {
+    if (!xyz) return ANIFEM_ERR_ARG;
+    return anifem::launch_physical_points(order, n_tets, xyz, nullptr, nullptr, 0, X,
+                                          static_cast<hipStream_t>(stream));
+}
 
 extern "C" int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
                                      const double* xyz, int precision, void* A_out, void* stream) {

@@ -52,6 +52,8 @@ struct ElemArgs {
     const double* D;
     const double* Rd;
     const float* Rf;
+    const double* Rds;  // R for a symmetric K in packed form (RefTensorDev::rs64), or null
+    const float* Rfs;
     const double* wq;  // quadrature weights (per-point K)
     void* out;         // batch mode: element matrices
     const void* r16;   // FP16 hi + lo split of R / r16_scale ([2][e0pad][16])
@@ -358,6 +360,7 @@ struct ApplyArgs {
     const int32_t* tetnodes;  // ... and 4 x n SoA node lists
     const double* dofs;    // batch: per-tet DOF values, dofs[r*nf + i]
     const int32_t* dofmap; // mesh: per-tet global DOF ids, dofmap[r*nf + i] ...
+    const int8_t* sign;    // ... their orientation signs (oriented spaces), or null ...
     const double* u;       // ... into the global vector u
     double* out;           // opU[r*dim*q + k + dim*n]
 };
@@ -403,6 +406,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
         for (int i = 0; i < a.nf0; ++i) {
             const int64_t e = r * nf + c * a.nf0 + i;
             double dv = MESH ? a.u[a.dofmap[e]] : a.dofs[e];
+            if (MESH && a.sign) dv *= (double)a.sign[e];
             if (a.scale) dv *= dof_scale(a.scale, P, det, i);
             for (int j = 0; j < d; ++j) g[j] += T[i * d + j] * dv;
         }

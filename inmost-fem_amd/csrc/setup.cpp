@@ -97,10 +97,49 @@ inline bool edge_pair_kept(const int32_t* v, int k) { return v[kEdge[k][0]] < v[
 
 }  // namespace
 
+SpaceLayout space_layout(int fem) {
+    SpaceLayout L;
+    L.valid = true;
+    switch (fem) {
+        case ANIFEM_FEM_P0: L.cell = 1; break;                           // poly_0.h:86-87
+        case ANIFEM_FEM_P1: L.node = 1; break;                           // poly_1.h:135-137
+        case ANIFEM_FEM_P2: L.node = 1, L.edge = 1; break;               // poly_2.h:104-107
+        case ANIFEM_FEM_P3: L.node = 1, L.edge = 2, L.face = 1; break;   // poly_3.h:122-125
+        case ANIFEM_FEM_RT0: L.face = 1, L.face_oriented = true; break;  // raviant_thomas_0.h:202-203
+        case ANIFEM_FEM_ND0: L.edge = 1, L.edge_oriented = true; break;  // nedelec_0.h:186-187
+        case ANIFEM_FEM_CR1: L.face = 1; break;                          // crouzeix_raviart_1.h:90-91
+        case ANIFEM_FEM_B2: L.edge = 1; break;                           // bubble_2.h:97-98
+        case ANIFEM_FEM_B4: L.cell = 1; break;                           // bubble_4.h:90-91
+        default: L.valid = false;
+    }
+    return L;
+}
+
+namespace {
+// Orientation sign of a cell's local edge k or face k for its global basis
+// function.  An edge (I, J) is positive when gid(I) < gid(J).  A face k is
+// positive when the global-id ranks of the vertices k, k+1, k+2, k+3 (mod 4),
+// read in that order, descend exactly once: the reference's test on the node
+// permutation (isPositivePermutationOrient, ordering.inl:152-172).
+inline int8_t edge_sign(const int32_t* v, int k) { return v[kEdge[k][0]] < v[kEdge[k][1]] ? 1 : -1; }
+inline int8_t face_sign(const int32_t* v, int k) {
+    int rank[4];
+    for (int a = 0; a < 4; ++a) {
+        rank[a] = 0;
+        for (int b = 0; b < 4; ++b) rank[a] += v[b] < v[a];
+    }
+    int descents = 0;
+    for (int j = 0; j < 3; ++j) descents += rank[(k + j) % 4] > rank[(k + j + 1) % 4];
+    return descents == 1 ? 1 : -1;
+}
+}  // namespace
+
 int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
     const int fem = prob.trial.fem;
-    if (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
+    if (!space_layout(fem).valid) return ANIFEM_ERR_ARG;
     if (prob.test.fem != fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
+    // vector spaces FemVec<3, P1..P3> only
+    if (prob.trial.ncomp != 1 && (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3)) return ANIFEM_ERR_ARG;
     S.fem = fem;
     S.ncomp = prob.trial.ncomp;
     const int nf0 = nf_scalar(fem);
@@ -153,12 +192,13 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
     const int fem = S.fem;
     const int nf0 = nf_scalar(fem);
     const int64_t NT = S.n_tets, NN = S.n_nodes;
+    const SpaceLayout L = space_layout(fem);
 
     // edges and faces by first appearance
     std::vector<int64_t> eid, fid;
     std::vector<int32_t> fmult;
-    const bool need_edges = fem >= ANIFEM_FEM_P2 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
-    const bool need_faces = fem >= ANIFEM_FEM_P3 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    const bool need_edges = L.edge > 0 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    const bool need_faces = L.face > 0 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
     if (need_edges) {
         std::vector<uint64_t> keys(6 * NT);
         host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
@@ -185,31 +225,45 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
         S.n_faces = number_by_first_appearance(keys, fid, &fmult);
     }
 
-    // NATURAL enumeration
-    const int64_t per_edge = fem == ANIFEM_FEM_P2 ? 1 : (fem == ANIFEM_FEM_P3 ? 2 : 0);
-    const int64_t per_face = fem == ANIFEM_FEM_P3 ? 1 : 0;
-    const int64_t nscal = NN + per_edge * S.n_edges + per_face * S.n_faces;
+    // NATURAL enumeration: nodes, edges, faces, cells (global_enumerator.h:392-404)
+    const int64_t per_edge = L.edge, per_face = L.face;
+    const int64_t edge_base = NN * L.node, face_base = edge_base + per_edge * S.n_edges;
+    const int64_t cell_base = face_base + per_face * S.n_faces;
+    const int64_t nscal = cell_base + (int64_t)L.cell * NT;
     S.ndof = nscal * S.ncomp;
     if (S.ndof >= INT32_MAX) return ANIFEM_ERR_ARG;
     S.dofs.resize((size_t)S.nf * NT);
+    const bool oriented = L.edge_oriented || L.face_oriented;
+    if (oriented) S.sign.assign((size_t)S.nf * NT, 1);
+    else std::vector<int8_t>().swap(S.sign);
     host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
         for (int64_t t = b; t < e; ++t) {
             const int32_t* v = &S.tets[4 * t];
             int64_t loc[20];
-            for (int i = 0; i < 4; ++i) loc[i] = v[i];
-            if (fem == ANIFEM_FEM_P2)
-                for (int k = 0; k < 6; ++k) loc[4 + k] = NN + eid[6 * t + k];
-            if (fem == ANIFEM_FEM_P3) {
-                for (int k = 0; k < 6; ++k) {
-                    int64_t base = NN + 2 * eid[6 * t + k];
-                    bool keep = edge_pair_kept(v, k);
-                    loc[4 + 2 * k] = base + (keep ? 0 : 1);
-                    loc[5 + 2 * k] = base + (keep ? 1 : 0);
+            int8_t sg[20];
+            int i = 0;
+            for (int k = 0; k < 4 && L.node; ++k, ++i) loc[i] = v[k], sg[i] = 1;
+            for (int k = 0; k < 6 && per_edge; ++k) {
+                const int64_t base = edge_base + per_edge * eid[(size_t)6 * t + k];
+                if (per_edge == 1) {
+                    loc[i] = base;
+                    sg[i++] = L.edge_oriented ? edge_sign(v, k) : 1;
+                } else if (per_edge == 2) {
+                    const bool keep = edge_pair_kept(v, k);
+                    loc[i] = base + (keep ? 0 : 1), sg[i++] = 1;
+                    loc[i] = base + (keep ? 1 : 0), sg[i++] = 1;
                 }
-                for (int k = 0; k < 4; ++k) loc[16 + k] = NN + 2 * S.n_edges + fid[4 * t + k];
             }
+            for (int k = 0; k < 4 && per_face; ++k, ++i) {
+                loc[i] = face_base + fid[(size_t)4 * t + k];
+                sg[i] = L.face_oriented ? face_sign(v, k) : 1;
+            }
+            if (L.cell) loc[i] = cell_base + t, sg[i++] = 1;
             for (int c = 0; c < S.ncomp; ++c)
-                for (int i = 0; i < nf0; ++i) S.dofs[(size_t)S.nf * t + nf0 * c + i] = (int32_t)(c * nscal + loc[i]);
+                for (int j = 0; j < nf0; ++j) {
+                    S.dofs[(size_t)S.nf * t + nf0 * c + j] = (int32_t)(c * nscal + loc[j]);
+                    if (oriented) S.sign[(size_t)S.nf * t + nf0 * c + j] = sg[j];
+                }
         }
     });
 
@@ -237,10 +291,10 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
                 }
             }
         for (int c = 0; c < S.ncomp; ++c) {
-            for (int64_t i = 0; i < NN; ++i) S.is_dir[c * nscal + i] = snode[i];
+            for (int64_t i = 0; i < NN && L.node; ++i) S.is_dir[c * nscal + i] = snode[i];
             for (int64_t i = 0; i < S.n_edges; ++i)
-                for (int s = 0; s < per_edge; ++s) S.is_dir[c * nscal + NN + per_edge * i + s] = sedge[i];
-            for (int64_t i = 0; i < S.n_faces && per_face; ++i) S.is_dir[c * nscal + NN + per_edge * S.n_edges + i] = sface[i];
+                for (int s = 0; s < per_edge; ++s) S.is_dir[c * nscal + edge_base + per_edge * i + s] = sedge[i];
+            for (int64_t i = 0; i < S.n_faces && per_face; ++i) S.is_dir[c * nscal + face_base + i] = sface[i];
         }
     } else if (prob.dirichlet != ANIFEM_DIRICHLET_NONE) {
         return ANIFEM_ERR_ARG;

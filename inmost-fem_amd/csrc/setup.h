@@ -56,6 +56,7 @@ struct AssemblySetup {
     int64_t ndof = 0, nnz = 0;
     std::vector<int32_t> tets;        // n_tets x 4 after the node reorder (caller's order)
     std::vector<int32_t> dofs;        // n_tets x nf global DOFs (caller's order)
+    std::vector<int8_t> sign;         // n_tets x nf orientation signs of oriented DOFs (RT0, ND0), else empty
     std::vector<uint8_t> is_dir;      // ndof
     std::vector<int64_t> row_ptr;     // ndof + 1
     std::vector<int32_t> col_idx;     // nnz
@@ -92,6 +93,24 @@ int build_setup_device(const anifem_mesh_desc& mesh, const anifem_problem_desc& 
                        const BlockCaps& caps, void* stream);
 int build_blocks(AssemblySetup& S, const std::vector<int64_t>& aptr, const std::vector<int32_t>& adj,
                  const BlockCaps& caps);
+// Compact value positions: per pair, the smallest kept position as 16 bits
+// and each entry's offset from it as one byte (0xFF: dropped), little endian
+// in ceil((2 + nf) / 4) 32-bit words.  Returns the words per pair, or 0 (out
+// empty) when some pair's kept positions span 255 or more.
+int compact_positions(const AssemblySetup& S, std::vector<uint32_t>& out);
+
+// DOF layout of a space over the entities of a cell (the DofSymmetries of
+// spaces/poly_*.h and spaces/*.h): DOFs per node, edge, face and cell, and
+// whether the edge / face DOFs are oriented (global basis = sign x local
+// basis, the sign following the global node ids: assembler.inl:154-164,
+// ordering.inl:152-172).  The local DOF order is nodes, edges (01, 02, 03,
+// 12, 13, 23), faces ({k, k+1, k+2 mod 4}), cell.
+struct SpaceLayout {
+    int node = 0, edge = 0, face = 0, cell = 0;
+    bool edge_oriented = false, face_oriented = false;
+    bool valid = false;
+};
+SpaceLayout space_layout(int fem);
 
 // Parallel loop over [0, n) on the host.
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn);

@@ -165,6 +165,27 @@ def test_training_reduces_loss_and_iterations():
     assert np.mean([r["learned_it"] for r in rows]) < 0.8 * np.mean([r["jacobi_it"] for r in rows])
 
 
+def test_learned_factor_against_fsai_on_the_same_pattern():
+    """The trained network's factor and the non-learned FSAI factor on the
+    same lower pattern, in the same PCG with M^-1 = G G^T: both converge to
+    the tolerance by the FP64 true residual, FSAI needs fewer iterations than
+    Jacobi, and the comparison is reported."""
+    from anifem_amd.learned import GrfSystems, train, compare_fsai, compare
+    sysm = GrfSystems(n=10, ell=0.1, sigma=1.5)
+    net, _, _ = train(sysm, np.arange(300), lr=3e-3, epochs=2)
+    seeds = np.arange(20_000, 20_005)
+    rows = compare_fsai(sysm, net, seeds, rtol=1e-8)
+    jac = compare(sysm, net, seeds, rtol=1e-8)
+    for r, j in zip(rows, jac):
+        assert r["ok"]
+        assert r["fsai_true"] <= 1.01e-8 and r["learned_true"] <= 1.01e-8
+        assert r["fsai_it"] < j["jacobi_it"]
+    f_it = np.mean([r["fsai_it"] for r in rows])
+    l_it = np.mean([r["learned_it"] for r in rows])
+    j_it = np.mean([j["jacobi_it"] for j in jac])
+    print(f"iterations: Jacobi {j_it:.1f}, FSAI {f_it:.1f}, learned {l_it:.1f}")
+
+
 def test_fallback_to_jacobi_when_the_factor_fails():
     import torch
     A, s, val, rhs = gpu_system(n=5, seed=6)

@@ -38,6 +38,32 @@ def test_pcg_poisson_c1_shape():
     assert 0.05 < xg.max() < 0.06
 
 
+def test_pcg_large_system_three_launch_iteration():
+    """P2 Poisson on the 32^3 cube (274,625 DOFs, 7.7 M stored entries): large
+    enough for the iteration whose SpMV gathers one vector per entry.  Against
+    the oracle's CG: iterations within 2, solution within 1e-8, and the FP64
+    true residual at the tolerance."""
+    import torch
+    coords, tets = af.kuhn_cube(32, 0.2, 2)
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), order=2, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
+    assert Asm.nnz >= 1 << 22
+    val, rhs, st = Asm.assemble(F=torch.ones(1, dtype=torch.float64, device="cuda"))
+    assert st.code == 0
+    rp, ci = Asm.csr()
+    x = torch.zeros(Asm.ndof, dtype=torch.float64, device="cuda")
+    res, ok = af.pcg_jacobi(Asm.ndof, rp, ci, val, rhs, x, rtol=1e-10, maxit=20000)
+    assert ok and res.true_relres <= 1.5e-10
+    s = Asm.setup()
+    vh, bh = val.cpu().numpy(), rhs.cpu().numpy()
+    xo = np.zeros(Asm.ndof)
+    rr = ctypes.c_double()
+    it = T.oracle().orc_jacobi_pcg(ctypes.c_long(Asm.ndof), T.P(s["row_ptr"]), T.P(s["col_idx"]), T.P(vh), T.P(bh),
+                                   T.P(xo), ctypes.c_double(1e-10), ctypes.c_long(20000), ctypes.byref(rr))
+    assert it > 0 and abs(res.iterations - it) <= 2, (res.iterations, it)
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
 def test_pcg_zero_rhs_and_maxit():
     import torch
     coords, tets = af.kuhn_cube(4, 0.2, 2)

@@ -40,16 +40,22 @@ def _free_bytes():
     return free.value
 
 
-def _problem(n, seed):
+def _problem(n, seed, general=False):
+    """P2 diffusion with a per-tet SPD tensor, stored packed (SYMMETRIC) or in
+    full (GENERAL: the geometry rows are then not packed, 80 B per tet)."""
     import torch
     rng = np.random.default_rng(seed)
     coords, tets = af.kuhn_cube(n, 0.2, seed)
-    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), order=2, tensor_type=af.TENSOR_SYMMETRIC,
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P2), order=2,
+                       tensor_type=af.TENSOR_GENERAL if general else af.TENSOR_SYMMETRIC,
                        tensor_storage=af.COEF_PER_TET, rhs_order=2, rhs_tensor_type=af.TENSOR_SCALAR,
                        rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_BOUNDARY)
     m = rng.normal(size=(len(tets), 3, 3))
     Dm = np.einsum("rij,rkj->rik", m, m) + np.eye(3)
-    Dp = np.stack([Dm[:, i, j] for i, j in [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]])
+    if general:
+        Dp = np.stack([Dm[:, i, j] for j in range(3) for i in range(3)])  # column-major
+    else:
+        Dp = np.stack([Dm[:, i, j] for i, j in [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]])
     return Asm, torch.from_numpy(np.ascontiguousarray(Dp)).cuda(), torch.full((1,), 1.25, dtype=torch.float64,
                                                                               device="cuda")
 
@@ -80,10 +86,11 @@ def test_noalloc_matches_run_on_two_calls_second_smaller():
 
 def test_noalloc_refuses_beyond_the_scratch():
     import torch
-    # P2 FP64 geometry rows take 80 bytes per tet: 66^3 Kuhn tets need 138 MB
+    # P2 FP64 geometry rows of a GENERAL tensor take 80 bytes per tet: 66^3
+    # Kuhn tets need 138 MB
     n = 66
     assert 6 * n ** 3 * 80 > SCRATCH_BYTES
-    Asm, D, F = _problem(n, 3)
+    Asm, D, F = _problem(n, 3, general=True)
     val = torch.empty(Asm.nnz, dtype=torch.float64, device="cuda")
     rhs = torch.empty(Asm.ndof, dtype=torch.float64, device="cuda")
     rc, _ = _noalloc(Asm, D, F, val, rhs)
