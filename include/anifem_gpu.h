@@ -257,6 +257,11 @@ int anifem_assembly_csr(const anifem_assembly* ctx, const int64_t** row_ptr, con
  * dropped), Dirichlet DOF flags.  Any pointer may be NULL. */
 int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int32_t* col_idx, int32_t* dofs,
                                int64_t* slot, uint8_t* dirichlet);
+/* Orientation signs of the cells' local DOFs (host, n_tets x nf, +1 / -1):
+ * the global basis function of an oriented DOF (RT0 faces, ND0 edges) is
+ * sign x the cell's local one (assembler.inl:154-164, ordering.inl:152-172);
+ * every sign is +1 for the other spaces. */
+int anifem_assembly_copy_signs(const anifem_assembly* ctx, int8_t* sign);
 /* Row blocks of the owner-computes assembly.  Block b owns the CSR rows
  * [blk_row0[b], blk_row0[b+1]) and forms every slot value and rhs entry of
  * them exactly once.  Its pairs [blk_p0[b], blk_p0[b+1]) are the (row, incident

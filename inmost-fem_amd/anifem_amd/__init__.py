@@ -127,6 +127,7 @@ SIGNATURES = {
     "anifem_assembly_info": (_I32, [_VP, _VP, _VP, _VP, _VP]),
     "anifem_assembly_csr": (_I32, [_VP, _VP, _VP]),
     "anifem_assembly_copy_setup": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "anifem_assembly_copy_signs": (_I32, [_VP, _VP]),
     "anifem_assembly_blocks": (_I32, [_VP, ctypes.POINTER(anifem_block_layout)]),
     "anifem_assembly_copy_blocks": (_I32, [_VP] + [_VP] * 8),
     "anifem_assembly_coords": (_I32, [_VP, _VP]),
@@ -449,6 +450,12 @@ class Assembler:
                "copy_setup")
         return dict(row_ptr=rp, col_idx=ci, dofs=dofs, slot=slot, dirichlet=dr)
 
+    def signs(self):
+        """(n_tets, nf) int8 orientation signs of the local DOFs (+1 / -1)."""
+        out = np.zeros((self.n_tets, self.nf), np.int8)
+        _check(lib().anifem_assembly_copy_signs(self._h, _ptr(out)), "anifem_assembly_copy_signs")
+        return out
+
     def block_layout(self):
         lay = anifem_block_layout()
         _check(lib().anifem_assembly_blocks(self._h, ctypes.byref(lay)), "blocks")
@@ -569,13 +576,17 @@ class Assembler:
         return np.stack([u[c * nscal:c * nscal + nn] for c in range(nc)], axis=1)
 
     def evaluate(self, u, xyl, op=GRAD, out=None, stream=None):
-        """op (IDEN or GRAD) of the trial space's global vector u (device,
-        ndof) at the barycentric points xyl (q, 4) of every cell
+        """op (IDEN or GRAD; IDEN, DIV or CURL of RT0 / ND0) of the trial
+        space's global vector u (device, ndof) at the barycentric points xyl
+        (q, 4) of every cell
         (anifem_assembly_eval): (n_tets, q, dim)."""
         import torch
         xyl = np.ascontiguousarray(xyl, dtype=np.float64)
         q = len(xyl)
-        dim = (3 if op == GRAD else 1) * self.trial.ncomp
+        if self.trial.fem in (FEM_RT0, FEM_ND0):
+            dim = 1 if op == DIV else 3  # vector-valued spaces: IDEN and CURL 3, DIV 1
+        else:
+            dim = (3 if op == GRAD else 1) * self.trial.ncomp
         if out is None:
             out = torch.empty((self.n_tets, q, dim), dtype=torch.float64, device=u.device)
         _check(lib().anifem_assembly_eval(self._h, op, _ptr(xyl), q, _ptr(u), _ptr(out), _stream_handle(stream)),

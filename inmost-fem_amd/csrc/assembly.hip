@@ -505,7 +505,7 @@ int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, co
 
 extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
                                       const anifem_problem_desc* prob, void* stream) {
-    if (!prob || !anifem::op_valid(prob->test) || !anifem::fem_lagrange(prob->test.fem)) return ANIFEM_ERR_ARG;
+    if (!prob || !anifem::op_valid(prob->test) || !anifem::space_layout(prob->test.fem).valid) return ANIFEM_ERR_ARG;
     const int nf = anifem::nf_scalar(prob->test.fem) * prob->test.ncomp;
     return anifem::assembly_create_caps(out, mesh, prob, anifem::default_caps(nf), stream);
 }
@@ -666,6 +666,14 @@ int anifem_assembly_copy_setup(const anifem_assembly* ctx, int64_t* row_ptr, int
     if (dirichlet) std::memcpy(dirichlet, S.is_dir.data(), S.is_dir.size());
     if (slot)
         for (size_t i = 0; i < S.slot.size(); ++i) slot[i] = S.slot[i];
+    return ANIFEM_OK;
+}
+
+int anifem_assembly_copy_signs(const anifem_assembly* ctx, int8_t* sign) {
+    if (!ctx || !sign) return ANIFEM_ERR_ARG;
+    const anifem::AssemblySetup& S = ctx->S;
+    if (S.sign.empty()) std::memset(sign, 1, (size_t)S.n_tets * S.nf);
+    else std::memcpy(sign, S.sign.data(), S.sign.size());
     return ANIFEM_OK;
 }
 

@@ -117,20 +117,25 @@ SpaceLayout space_layout(int fem) {
 
 namespace {
 // Orientation sign of a cell's local edge k or face k for its global basis
-// function.  An edge (I, J) is positive when gid(I) < gid(J).  A face k is
-// positive when the global-id ranks of the vertices k, k+1, k+2, k+3 (mod 4),
-// read in that order, descend exactly once: the reference's test on the node
-// permutation (isPositivePermutationOrient, ordering.inl:152-172).
+// function (global = sign x local, assembler.inl:154-164).
+//  * Edge (I, J): positive when gid(I) < gid(J), the reference's rule
+//    (isPositivePermutationOrient, ordering.inl:152-158): the global ND0
+//    function then has unit tangential component from the lower to the
+//    higher global id in every cell.
+//  * Face k = (k, k+1, k+2): the local RT0 function points out of the cell.
+//    In a reordered cell (det(P0-P3, P1-P3, P2-P3) >= 0) the vertex order
+//    k, k+1, k+2 turns counter-clockwise seen from outside for even k and
+//    clockwise for odd k, so the sign (-1)^k x (parity of the face's global
+//    ids in that order) gives every cell the same normal, the right-hand
+//    normal of the face's vertices in ascending global id.  (The reference's
+//    face test, ordering.inl:159-164, compares the third face vertex with the
+//    opposite vertex; read as written it does not give a normal-continuous
+//    global space, and this build uses the sign that does.)
 inline int8_t edge_sign(const int32_t* v, int k) { return v[kEdge[k][0]] < v[kEdge[k][1]] ? 1 : -1; }
 inline int8_t face_sign(const int32_t* v, int k) {
-    int rank[4];
-    for (int a = 0; a < 4; ++a) {
-        rank[a] = 0;
-        for (int b = 0; b < 4; ++b) rank[a] += v[b] < v[a];
-    }
-    int descents = 0;
-    for (int j = 0; j < 3; ++j) descents += rank[(k + j) % 4] > rank[(k + j + 1) % 4];
-    return descents == 1 ? 1 : -1;
+    const int32_t a = v[k], b = v[(k + 1) % 4], c = v[(k + 2) % 4];
+    const int inversions = (a > b) + (a > c) + (b > c);
+    return ((k + inversions) & 1) ? -1 : 1;
 }
 }  // namespace
 
