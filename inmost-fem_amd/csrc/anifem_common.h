@@ -26,7 +26,9 @@ bool op_valid(const anifem_op& op);
 //         3: |edge i| / det J (ND0 curls), a per-tet factor of basis function i
 void space_map(int op, int fem, int& map, int& scale);
 
-// Quadrature rule for `order` (generated table, tet_rules.inc); returns q or -1.
+// Quadrature rule for `order` (0..kTetMaxOrder; generated table tet_rules.inc
+// up to order 6, conical Gauss-Jacobi products above); returns q or -1.
+constexpr int kTetMaxOrder = 20;
 int quadrature(int order, const double** p, const double** w);
 
 // Reference-space operator table of a space at the rule's points:

@@ -13,6 +13,7 @@
 // (summed over n with the weights when D is constant over the tet).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -100,12 +101,112 @@ void space_map(int op, int fem, int& map, int& scale) {
     }
 }
 
+// Conical product rule on the tet for orders above the tabulated ones: with
+// x = u, y = (1 - u) v, z = (1 - u)(1 - v) s the volume element is
+// (1 - u)^2 (1 - v) du dv ds, so Gauss-Jacobi rules on [0, 1] for the weights
+// (1 - u)^2, (1 - v) and 1 with m = order/2 + 1 points each integrate every
+// polynomial of total degree <= order exactly (Stroud's collapsed product).
+// Nodes are the zeros of the monic orthogonal polynomial of degree m for the
+// weight (1 - t)^alpha, found by Newton's method with deflation on the
+// three-term recurrence; weights by the Christoffel sum
+// w_i = mu0 / sum_{k<m} p_k(t_i)^2 / (b_1 ... b_k).  Points are listed with u
+// outermost and s innermost, each in ascending order; the weights are scaled
+// to sum to 1 (mean-value form).  The reference serves these orders with
+// fully symmetric rules of fewer points (quadrature_formulas.cpp:602-1489);
+// both integrate polynomials of degree <= order exactly.
+static void gauss_jacobi01(int m, int alpha, std::vector<long double>& t, std::vector<long double>& w) {
+    // monic Jacobi recurrence on [-1, 1] for (1 - x)^alpha:
+    // p_{k+1} = (x - a_k) p_k - b_k p_{k-1}
+    std::vector<long double> ak(m + 1), bk(m + 1);
+    const long double al = alpha;
+    for (int k = 0; k <= m; ++k) {
+        const long double s = 2 * k + al;
+        ak[k] = (k == 0) ? -al / (al + 2) : -al * al / (s * (s + 2));
+        bk[k] = (k == 0) ? 0 : 4.0L * k * (k + al) * k * (k + al) / (s * s * (s + 1) * (s - 1));
+    }
+    auto eval = [&](long double x, long double& p, long double& dp) {
+        long double p0 = 1, p1 = x - ak[0], d0 = 0, d1 = 1;
+        for (int k = 1; k < m; ++k) {
+            const long double p2 = (x - ak[k]) * p1 - bk[k] * p0;
+            const long double d2 = (x - ak[k]) * d1 + p1 - bk[k] * d0;
+            p0 = p1, p1 = p2, d0 = d1, d1 = d2;
+        }
+        p = p1, dp = d1;
+    };
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> x;
+    for (int i = 0; i < m; ++i) {
+        long double z = std::cos(pi * (i + 0.75L) / (m + 0.5L));
+        for (int it = 0; it < 200; ++it) {
+            long double p, dp;
+            eval(z, p, dp);
+            long double defl = 0;
+            for (long double r : x) defl += 1.0L / (z - r);
+            const long double dz = p / (dp - p * defl);
+            z -= dz;
+            if (std::fabs(dz) < 1e-19L) break;
+        }
+        x.push_back(z);
+    }
+    std::sort(x.begin(), x.end());
+    const long double mu0 = std::pow(2.0L, al + 1) / (al + 1);  // int_{-1}^{1} (1 - x)^alpha
+    t.resize(m);
+    w.resize(m);
+    for (int i = 0; i < m; ++i) {
+        long double p0 = 1, p1 = x[i] - ak[0], sum = 1, nrm = 1;
+        for (int k = 1; k < m; ++k) {
+            nrm *= bk[k];
+            sum += p1 * p1 / nrm;
+            const long double p2 = (x[i] - ak[k]) * p1 - bk[k] * p0;
+            p0 = p1, p1 = p2;
+        }
+        t[i] = (1 + x[i]) / 2;
+        w[i] = mu0 / sum / std::pow(2.0L, al + 1);  // dx = 2 dt, (1 - x)^a = 2^a (1 - t)^a
+    }
+}
+
+struct ProductRule {
+    std::vector<double> p, w;
+};
+
+static const ProductRule& product_rule(int order) {
+    static const std::vector<ProductRule> rules = [] {
+        std::vector<ProductRule> r(kTetMaxOrder + 1);
+        for (int o = ANIFEM_TET_TABLE_MAX_ORDER + 1; o <= kTetMaxOrder; ++o) {
+            const int m = o / 2 + 1;
+            std::vector<long double> tu, wu, tv, wv, ts, ws;
+            gauss_jacobi01(m, 2, tu, wu);
+            gauss_jacobi01(m, 1, tv, wv);
+            gauss_jacobi01(m, 0, ts, ws);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j)
+                    for (int k = 0; k < m; ++k) {
+                        const long double x = tu[i], y = (1 - tu[i]) * tv[j], z = (1 - tu[i]) * (1 - tv[j]) * ts[k];
+                        r[o].p.push_back((double)(1 - x - y - z));
+                        r[o].p.push_back((double)x);
+                        r[o].p.push_back((double)y);
+                        r[o].p.push_back((double)z);
+                        r[o].w.push_back((double)(6 * wu[i] * wv[j] * ws[k]));
+                    }
+        }
+        return r;
+    }();
+    return rules[order];
+}
+
 int quadrature(int order, const double** p, const double** w) {
-    if (order < 0 || order > ANIFEM_TET_MAX_ORDER) return -1;
+    if (order < 0 || order > kTetMaxOrder) return -1;
+    if (order > ANIFEM_TET_TABLE_MAX_ORDER) {
+        const ProductRule& r = product_rule(order);
+        *p = r.p.data();
+        *w = r.w.data();
+        return (int)r.w.size();
+    }
     int q = ANIFEM_TET_NPTS_BY_ORDER[order];
     switch (q) {
         case 1: *p = ANIFEM_TET_Q1_P, *w = ANIFEM_TET_Q1_W; break;
         case 4: *p = ANIFEM_TET_Q4_P, *w = ANIFEM_TET_Q4_W; break;
+        case 8: *p = ANIFEM_TET_Q8_P, *w = ANIFEM_TET_Q8_W; break;
         case 14: *p = ANIFEM_TET_Q14_P, *w = ANIFEM_TET_Q14_W; break;
         case 24: *p = ANIFEM_TET_Q24_P, *w = ANIFEM_TET_Q24_W; break;
         default: return -1;

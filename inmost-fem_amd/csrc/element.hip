@@ -2,6 +2,11 @@
 // helpers shared with the fused assembly.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include "anifem_common.h"
 #include "element_kernels.h"
 
@@ -152,7 +157,8 @@ int launch_element_mfma(const anifem_op& opA, const anifem_op& opB, const ElemAr
 // per tet, stored X[(c*q + n)*n + r] (coalesced over tets; the layout of
 // per-point coefficient arrays).
 struct RulePoints {
-    double lam[24][3];  // lambda_1..3 of each point
+    double lam[24][3];    // lambda_1..3 of each point (rules of up to 24 points)
+    const double* lamd;   // the same on the device, 3 per point (larger rules), or null
     int q;
 };
 
@@ -165,9 +171,32 @@ __global__ __launch_bounds__(kBlock) void physical_points_kernel(ElemArgs a, Rul
     for (int n = 0; n < rp.q; ++n)
         for (int c = 0; c < 3; ++c) {
             double x = P[0][c];
-            for (int l = 0; l < 3; ++l) x += rp.lam[n][l] * (P[l + 1][c] - P[0][c]);
+            for (int l = 0; l < 3; ++l) x += (rp.lamd ? rp.lamd[3 * n + l] : rp.lam[n][l]) * (P[l + 1][c] - P[0][c]);
             X[((int64_t)c * rp.q + n) * a.n + r] = x;
         }
+}
+
+// lambda_1..3 of the points of rule `order` on the current device, uploaded
+// once per device and kept until the process ends.
+static const double* rule_lambda_dev(int order, int q, const double* p) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, double*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, order});
+    if (it != cache.end()) return it->second;
+    std::vector<double> lam((size_t)3 * q);
+    for (int k = 0; k < q; ++k)
+        for (int l = 0; l < 3; ++l) lam[3 * k + l] = p[4 * k + l + 1];
+    double* d = nullptr;
+    if (hipMalloc(&d, lam.size() * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    cache[{dev, order}] = d;
+    return d;
 }
 
 // Physical points of the volume rule `order` for n tets, from SoA vertices
@@ -177,11 +206,16 @@ int launch_physical_points(int order, int64_t n, const double* xyz, const double
     const double *p, *w;
     const int q = quadrature(order, &p, &w);
     if (q < 0) return ANIFEM_ERR_ORDER;
-    if (q > 24 || n < 0 || !X || (!xyz && (!coords || !tetnodes))) return ANIFEM_ERR_ARG;
+    if (n < 0 || !X || (!xyz && (!coords || !tetnodes))) return ANIFEM_ERR_ARG;
     RulePoints rp{};
     rp.q = q;
-    for (int k = 0; k < q; ++k)
-        for (int l = 0; l < 3; ++l) rp.lam[k][l] = p[4 * k + l + 1];
+    if (q <= 24) {
+        for (int k = 0; k < q; ++k)
+            for (int l = 0; l < 3; ++l) rp.lam[k][l] = p[4 * k + l + 1];
+    } else {
+        rp.lamd = rule_lambda_dev(order, q, p);
+        if (!rp.lamd) return ANIFEM_ERR_HIP;
+    }
     ElemArgs a{};
     a.n = n;
     a.xyz = xyz;

@@ -1,31 +1,43 @@
 #!/usr/bin/env python3
 """Generate the symmetric tetrahedron quadrature tables used by the HIP kernels.
 
-The rules are the fully symmetric, positive-weight, interior-point rules that
-AniFem++ selects in ``tetrahedron_quadrature_formulas(order)``
-(anifem++/fem/quadrature_formulas.cpp:526-1503, order map :1493-1501):
+AniFem++ selects its rule in ``tetrahedron_quadrature_formulas(order)``
+(anifem++/fem/quadrature_formulas.cpp:526-1503, order map :1493-1501) for
+orders 0..20.  This script tabulates orders 0..6:
 
     order 0,1 -> 1 point (centroid)
     order 2   -> 4 points, one S4 orbit                     (degree 2)
-    order 3   -> served here by the 14-point degree-5 rule (see below)
+    order 3   -> 8 points, two S4 orbits, equal weights     (degree 3)
     order 4,5 -> 14 points, two S4 orbits + one S6 orbit    (degree 5)
     order 6   -> 24 points, three S4 orbits + one S12 orbit (degree 6)
 
-The point/weight values are not copied: for each orbit structure this script
-solves the moment equations  sum_i w_i lambda(p_i)^alpha = 3! alpha! / (|alpha|+3)!
+Orders 7..20 are built at run time by basis.cpp as conical (collapsed) products
+of Gauss-Jacobi rules; see quadrature() there.
+
+The point/weight values are not copied: for the 4/14/24-point structures this
+script solves the moment equations
+    sum_i w_i lambda(p_i)^alpha = 3! alpha! / (|alpha|+3)!
 for every barycentric monomial of degree <= d by a seeded multi-start
 least-squares search (numpy/scipy) and then refines the single admissible
 solution (all points interior, all weights positive) to 50 digits with
-Gauss-Newton in mpmath.  For the 4/14/24-point structures the admissible
-solution is unique, so the result equals the reference's rule up to rounding.
-The reference's 8-point order-3 rule is one member of a one-parameter family
-that the moment equations do not fix; order 3 therefore uses the 14-point
-degree-5 rule, which integrates every polynomial of degree <= 3 exactly, so
-results agree to rounding for polynomial integrands of that degree.
+Gauss-Newton in mpmath.  For these structures the admissible solution is
+unique, so the result equals the reference's rule up to rounding.
+
+Two S4 orbits of degree 3 leave a one-parameter family (three moment
+equations, four unknowns); the reference's 8-point rule is one member of it.
+This script takes the member with all eight weights equal to 1/8.  With the
+distinct coordinate of an orbit written 1/4 + d (the other three 1/4 - d/3),
+the moments of l0^2 and l0^3 read d1^2 + d2^2 = 9/40 and d1^3 + d2^3 = 9/160,
+so s = d1 + d2 solves 80 s^3 - 54 s + 9 = 0 and d1, d2 are the roots of
+t^2 - s t + (s^2 - 9/40)/2.  Exactly one root s gives real d1 != d2 with every
+coordinate in (0, 1).  Its points differ from the reference's member (the
+reference's weights are unequal); both integrate every polynomial of degree
+<= 3 exactly, so results agree to rounding for such integrands.
 
 Orbit point order: an S4 orbit lists the point whose distinct coordinate sits
 at position 0,1,2,3 (the same order as the reference's TWIST4 macro); S6 and
-S12 orbits use lexicographic position order.
+S12 orbits use lexicographic position order.  The 8-point rule lists the orbit
+with the smaller distinct coordinate first.
 
 Usage: python gen_tet_rules.py > tet_rules.inc
 """
@@ -44,7 +56,8 @@ NPAR = {"s1": 0, "s4": 1, "s6": 1, "s12": 2}
 RULES = [
     ((0, 1), ["s1"], 1),
     ((2,), ["s4"], 2),
-    ((3, 4, 5), ["s4", "s4", "s6"], 5),
+    ((3,), ["s4", "s4"], 3),
+    ((4, 5), ["s4", "s4", "s6"], 5),
     ((6,), ["s4", "s4", "s4", "s12"], 6),
 ]
 
@@ -185,6 +198,34 @@ def refine(struct, deg, z0):
     return z
 
 
+def equal_weight_rule8():
+    """The 8-point degree-3 rule with all weights 1/8 (closed form above):
+    the S4 parameters (repeated coordinates) of the two orbits and the weights."""
+    sols = []
+    for s in mp.polyroots([80, 0, -54, 9], maxsteps=200, extraprec=200):
+        s = mp.re(s)
+        p = (s * s - mp.mpf(9) / 40) / 2
+        disc = s * s - 4 * p
+        if disc <= 0:
+            continue
+        d = sorted([(s - mp.sqrt(disc)) / 2, (s + mp.sqrt(disc)) / 2])
+        coords = [mp.mpf(1) / 4 + x for x in d] + [mp.mpf(1) / 4 - x / 3 for x in d]
+        if all(0 < c < 1 for c in coords):
+            sols.append(d)
+    if len(sols) != 1:
+        raise RuntimeError(f"equal-weight 8-point rule: {len(sols)} admissible roots")
+    d1, d2 = sols[0]
+    return [mp.mpf(1) / 4 - d1 / 3, mp.mpf(1) / 4 - d2 / 3, mp.mpf(1) / 8, mp.mpf(1) / 8]
+
+
+def check_exact(P, W, deg):
+    worst = max(abs(sum(W[i] * P[i][0] ** m[0] * P[i][1] ** m[1] * P[i][2] ** m[2] * P[i][3] ** m[3]
+                        for i in range(len(P))) - mp.mpf(6) * mp.factorial(m[0]) * mp.factorial(m[1])
+                    * mp.factorial(m[2]) * mp.factorial(m[3]) / mp.factorial(sum(m) + 3)) for m in monomials(deg))
+    if worst > mp.mpf(10) ** -40:
+        raise RuntimeError(f"rule of degree {deg} is not exact ({worst})")
+
+
 def main():
     out = sys.stdout
     out.write("// Generated by gen_tet_rules.py -- symmetric tetrahedron quadrature rules.\n")
@@ -197,9 +238,12 @@ def main():
             z = []
             P, W = expand(struct, [mp.mpf(1)], mp.mpf(1))
             W = [mp.mpf(1)]
+        elif struct == ["s4", "s4"] and deg == 3:
+            P, W = expand(struct, equal_weight_rule8(), mp.mpf(1))
         else:
             z = refine(struct, deg, search(struct, deg))
             P, W = expand(struct, z, mp.mpf(1))
+        check_exact(P, W, deg)
         n = len(P)
         name = f"ANIFEM_TET_Q{n}"
         names.append((orders, name, n, deg))
@@ -212,8 +256,9 @@ def main():
         for w in W:
             out.write("    " + mp.nstr(w, 20, strip_zeros=False) + ",\n")
         out.write("};\n\n")
-    out.write("#define ANIFEM_TET_MAX_ORDER 6\n")
-    out.write("#define ANIFEM_TET_MAX_NPTS 24\n")
+    out.write("// orders above ANIFEM_TET_TABLE_MAX_ORDER are built at run time (basis.cpp)\n")
+    out.write("#define ANIFEM_TET_TABLE_MAX_ORDER 6\n")
+    out.write("#define ANIFEM_TET_TABLE_MAX_NPTS 24\n")
     npts = [0] * 7
     for orders, name, n, deg in names:
         for o in orders:

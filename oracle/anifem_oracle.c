@@ -214,16 +214,128 @@ static void orc_build_rule(int id) {
     orc_rule_cache_ready[id] = 1;
 }
 
+/* Order 3: the 8-point rule of two S4 orbits with all weights 1/8 (the
+ * reference's order-3 rule, quadrature_formulas.cpp:542-549, is another member
+ * of the same one-parameter family).  Newton on the moments of l0^2 and l0^3
+ * (1/10 and 1/20) in the two distinct coordinates a1 < a2, from coarse seeds. */
+static void orc_rule8(double* p, double* w) {
+    ld a[2] = {0.0134L, 0.6611L};
+    for (int it = 0; it < 60; ++it) {
+        ld f[2] = {-0.1L, -0.05L}, J[2][2];
+        for (int o = 0; o < 2; ++o) {
+            ld b = (1 - a[o]) / 3;
+            f[0] += (a[o] * a[o] + 3 * b * b) / 8;
+            f[1] += (a[o] * a[o] * a[o] + 3 * b * b * b) / 8;
+            J[0][o] = (2 * a[o] - 2 * b) / 8;
+            J[1][o] = (3 * a[o] * a[o] - 3 * b * b) / 8;
+        }
+        ld det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        ld d0 = (f[0] * J[1][1] - f[1] * J[0][1]) / det, d1 = (J[0][0] * f[1] - J[1][0] * f[0]) / det;
+        a[0] -= d0, a[1] -= d1;
+        if (fabsl(d0) + fabsl(d1) < 1e-20L) break;
+    }
+    for (int o = 0; o < 2; ++o)
+        for (int i = 0; i < 4; ++i) {
+            for (int k = 0; k < 4; ++k) p[4 * (4 * o + i) + k] = (double)((1 - a[o]) / 3);
+            p[4 * (4 * o + i) + i] = (double)a[o];
+            w[4 * o + i] = 0.125;
+        }
+}
+
+/* Orders 7..20: Stroud's conical product.  x = u, y = (1-u) v,
+ * z = (1-u)(1-v) s maps the unit cube onto the tet with volume element
+ * (1-u)^2 (1-v); Gauss-Jacobi rules with m = order/2 + 1 points for the
+ * weights (1-u)^2, (1-v), 1 on [0,1].  Nodes: zeros of the Jacobi polynomial
+ * P_m^(alpha,0)(x), x = 2t - 1 (standard three-term recurrence), by Newton
+ * with deflation; weights from the closed form for beta = 0,
+ * w = 2^(alpha+1) / ((1 - x^2) P_m'(x)^2) on [-1,1], i.e.
+ * 1 / ((1 - x^2) P_m'(x)^2) on [0,1].  Point order: u outer, s inner, each
+ * ascending; weights scaled to sum to 1. */
+static void orc_jacobi(int m, int alpha, ld x, ld* P, ld* dP) {
+    ld al = alpha, p0 = 1, p1 = ((al + 2) * x + al) / 2;
+    for (int n = 2; n <= m; ++n) {
+        ld c = 2 * n + al;
+        ld p2 = ((c - 1) * (c * (c - 2) * x + al * al) * p1 - 2 * (n + al - 1) * (n - 1) * c * p0) /
+                (2 * n * (n + al) * (c - 2));
+        p0 = p1, p1 = p2;
+    }
+    *P = m == 0 ? 1 : p1;
+    /* (2m+a)(1-x^2) P_m' = m (a - (2m+a) x) P_m + 2 (m+a) m P_{m-1} */
+    ld c = 2 * m + al;
+    *dP = (m * (al - c * x) * p1 + 2 * (m + al) * m * p0) / (c * (1 - x * x));
+}
+
+static void orc_gauss_jacobi01(int m, int alpha, ld* t, ld* w) {
+    ld x[16];
+    const ld pi = 3.141592653589793238462643383279502884L;
+    for (int i = 0; i < m; ++i) {
+        ld z = cosl(pi * (i + 0.75L) / (m + 0.5L)), P, dP;
+        for (int it = 0; it < 200; ++it) {
+            orc_jacobi(m, alpha, z, &P, &dP);
+            ld defl = 0;
+            for (int j = 0; j < i; ++j) defl += 1 / (z - x[j]);
+            ld dz = P / (dP - P * defl);
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        x[i] = z;
+    }
+    for (int i = 1; i < m; ++i) /* ascending */
+        for (int j = i; j > 0 && x[j - 1] > x[j]; --j) {
+            ld tmp = x[j];
+            x[j] = x[j - 1];
+            x[j - 1] = tmp;
+        }
+    for (int i = 0; i < m; ++i) {
+        ld P, dP;
+        orc_jacobi(m, alpha, x[i], &P, &dP);
+        t[i] = (1 + x[i]) / 2;
+        w[i] = 1 / ((1 - x[i] * x[i]) * dP * dP);
+    }
+}
+
+#define ORC_MAX_ORDER 20
+#define ORC_MAX_Q 1331
+
+/* Computed on every call (no shared state: callers may run in threads). */
+static int orc_product_rule(int order, double* p, double* w) {
+    int m = order / 2 + 1;
+    ld tu[16], wu[16], tv[16], wv[16], ts[16], ws[16];
+    orc_gauss_jacobi01(m, 2, tu, wu);
+    orc_gauss_jacobi01(m, 1, tv, wv);
+    orc_gauss_jacobi01(m, 0, ts, ws);
+    int n = 0;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            for (int k = 0; k < m; ++k, ++n) {
+                ld x = tu[i], y = (1 - tu[i]) * tv[j], z = (1 - tu[i]) * (1 - tv[j]) * ts[k];
+                if (p) {
+                    p[4 * n] = (double)(1 - x - y - z);
+                    p[4 * n + 1] = (double)x;
+                    p[4 * n + 2] = (double)y;
+                    p[4 * n + 3] = (double)z;
+                }
+                if (w) w[n] = (double)(6 * wu[i] * wv[j] * ws[k]);
+            }
+    return n;
+}
+
 int orc_quadrature(int order, double* p, double* w) {
-    if (order < 0 || order > 6) return -1;
+    if (order < 0 || order > ORC_MAX_ORDER) return -1;
     if (order <= 1) {
         if (p)
             for (int k = 0; k < 4; ++k) p[k] = 0.25;
         if (w) w[0] = 1.0;
         return 1;
     }
-    /* order 3 uses the degree-5 rule (the reference's 8-point rule is one
-     * member of a one-parameter family that the moment equations leave open) */
+    if (order == 3) {
+        double pp[32], ww[8];
+        orc_rule8(pp, ww);
+        if (p) memcpy(p, pp, sizeof pp);
+        if (w) memcpy(w, ww, sizeof ww);
+        return 8;
+    }
+    if (order >= 7) return orc_product_rule(order, p, w);
     int id = order == 2 ? 0 : (order <= 5 ? 1 : 2);
     if (!orc_rule_cache_ready[id]) orc_build_rule(id);
     int q = orc_rule_n[id];
@@ -535,14 +647,22 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
                  int order, long n, const double* xyz, const double* D, double* A) {
     int nfA = orc_nfa(femA, ncA), nfB = orc_nfa(femB, ncB);
     if (nfA <= 0 || nfB <= 0 || !orc_op_ok(opA, femA, ncA) || !orc_op_ok(opB, femB, ncB)) return -1;
-    double XYL[4 * 24], WG[24];
+    double* XYL = malloc(sizeof(double) * 4 * ORC_MAX_Q);
+    double* WG = malloc(sizeof(double) * ORC_MAX_Q);
     int q = orc_quadrature(order, XYL, WG);
-    if (q <= 0) return -2;
+    if (q <= 0) {
+        free(XYL);
+        free(WG);
+        return -2;
+    }
     int dimA = orc_opdim(opA, femA, ncA), dimB = orc_opdim(opB, femB, ncB);
     int ddim = tensor_type == ORC_TENSOR_NULL ? 0 : (tensor_type == ORC_TENSOR_SCALAR ? 1 : dimA * dimB);
     if ((tensor_type == ORC_TENSOR_NULL || tensor_type == ORC_TENSOR_SCALAR) && dimA != dimB &&
-        !(nfA == 1 && dimA == 1))
+        !(nfA == 1 && dimA == 1)) {
+        free(XYL);
+        free(WG);
         return -3;
+    }
     double* U = malloc(sizeof(double) * (size_t)dimA * q * nfA);
     double* V = malloc(sizeof(double) * (size_t)dimB * q * nfB);
     double* DU = malloc(sizeof(double) * (size_t)dimB * q * nfA);
@@ -601,6 +721,8 @@ int orc_fem3dtet(int opA, int femA, int ncA, int opB, int femB, int ncB, int ten
     free(V);
     free(DU);
     free(Dfull);
+    free(XYL);
+    free(WG);
     return 0;
 }
 

@@ -57,7 +57,21 @@ int run_case(int order, long n, const double* xyz, const double* D, long ddim, d
         Tetra<const double> XYZ(p, p + 3, p + 6, p + 9);
         DenseMatrix<double> Am(buf, nfb, nfa, nfa * nfb);
         TetData td{D ? D + ddim * r : nullptr, ddim};
-        fem3Dtet<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, StoredTensor<TT, XDEP>(), Am, order, &td);
+        if (order <= 6) {
+            fem3Dtet<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, StoredTensor<TT, XDEP>(), Am, order, &td);
+        } else {
+            // the stack-memory version stops at order 6 (int_tet.inl:99-100);
+            // higher orders run through the external-memory overload (int_tet.h:23-29)
+            PlainMemory<double, int> req = fem3Dtet_memory_requirements<OpA, OpB, double, int>(order, 1);
+            std::vector<double> dm(req.dSize);
+            std::vector<int> im(req.iSize);
+            PlainMemory<double, int> mem;
+            mem.ddata = dm.data();
+            mem.idata = im.data();
+            mem.dSize = dm.size();
+            mem.iSize = im.size();
+            fem3Dtet<OpA, OpB, DfuncTraits<SPARSITY, false>>(XYZ, StoredTensor<TT, XDEP>(), Am, mem, order, &td);
+        }
         std::memcpy(A + nfa * nfb * r, buf, sizeof(double) * nfa * nfb);
     }
     return 0;

@@ -55,8 +55,8 @@ def dim(op, nc=1, fem=None):
 
 
 def orc_quadrature(order):
-    p = np.zeros(4 * 24)
-    w = np.zeros(24)
+    p = np.zeros(4 * 1331)
+    w = np.zeros(1331)
     q = oracle().orc_quadrature(order, P(p), P(w))
     return p[:4 * q].reshape(q, 4), w[:q]
 

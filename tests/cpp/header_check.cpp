@@ -37,9 +37,10 @@ int main() {
     CHECK((check_view<DfuncTraits<TENSOR_SCALAR, false>>(sym_tet) == ANIFEM_ERR_ARG));
     CHECK((check_view<DfuncTraits<TENSOR_SYMMETRIC, true>>(sym_tet) == ANIFEM_ERR_ARG));
     CHECK((check_view<DfuncTraits<PerTetra, false>>(sym_tet) == ANIFEM_OK));
-    // the reference throws for order > 6 (int_tet.inl:99-100)
+    // the batch works on device memory, as the reference's dynamic-memory
+    // fem3Dtet does, whose rules stop at order 20 (quadrature_formulas.cpp:1497-1498)
     double buf[16] = {0};
-    CHECK((fem3Dtet<GP2, GP2, DfuncTraits<TENSOR_SYMMETRIC, false>>(buf, 1, sym_tet, buf, 7) == ANIFEM_ERR_ORDER));
+    CHECK((fem3Dtet<GP2, GP2, DfuncTraits<TENSOR_SYMMETRIC, false>>(buf, 1, sym_tet, buf, 21) == ANIFEM_ERR_ORDER));
     // identity tensor between GRAD (dim 3) and IDEN (dim 1) (diff_tensor.h:313-315)
     const TensorView null_const{TENSOR_NULL, ANIFEM_COEF_CONST, nullptr};
     CHECK((fem3Dtet<GP2, Operator<IDEN, FemFix<FEM_P2>>, DfuncTraits<TENSOR_NULL, true>>(buf, 1, null_const, buf, 2) ==

@@ -119,6 +119,27 @@ def cases(rng):
     return out
 
 
+def high_cases(rng):
+    """Orders 3 and 7..20 (the reference's 8-point order-3 rule and its rules
+    of orders 7..20, the latter through the external-memory fem3Dtet).  Every
+    integrand is a polynomial of degree <= order, so any exact rule of that
+    order gives these matrices to rounding."""
+    n = 24
+    xyz = kuhn_tets(rng, n)
+    kappa = 10.0 ** rng.uniform(-1, 1, (n, 1))
+    D = np.array([aniso_tensor(rng).ravel(order="F") for _ in range(n)])
+    return [
+        ("hi_o3_p2_grad_scalar", 1, 2, 0, 3, xyz, kappa),                      # degree 2
+        ("hi_o3_rt0_iden_null", 21, 1, 0, 3, xyz, np.zeros((n, 1))),           # degree 2
+        ("hi_o7_p3_grad_general", 2, 4, 0, 7, xyz, D),                         # degree 4
+        ("hi_o8_p2_grad_general_xdep", 1, 4, 2, 8, xyz, D),                    # degree 2 + 2
+        ("hi_o9_nd0_curl_scalar", 25, 2, 0, 9, xyz, kappa),                    # degree 0
+        ("hi_o12_p3_mass_scalar_xdep", 11, 2, 2, 12, xyz, kappa),              # degree 6 + 2
+        ("hi_o15_p2vec_grad_general", 4, 4, 0, 15, xyz[:12], rng.normal(size=(12, 81))),  # degree 2
+        ("hi_o20_p3_mass_scalar", 11, 2, 0, 20, xyz, kappa),                   # degree 6
+    ]
+
+
 def op_cases(rng):
     """DIV / CURL / DUDX..DUDZ operators (operators.h:286-400)."""
     n = 32
@@ -263,7 +284,8 @@ def main():
     L.ref_fem3dtet.restype = ctypes.c_int
     dp = ctypes.POINTER(ctypes.c_double)
     # groups to write (default all): elem (with the operator and space cases), faces, apply;
-    # "spaces" writes the cases of the RT0 / ND0 / CR1 / B2 / B4 spaces and fem3DapplyX only
+    # "spaces" writes the cases of the RT0 / ND0 / CR1 / B2 / B4 spaces and fem3DapplyX only;
+    # "high" the cases of quadrature orders 3 and 7..20
     groups = set(sys.argv[1:]) or {"elem", "faces", "apply"}
     rng = np.random.default_rng(20260101)
     todo = []
@@ -271,6 +293,8 @@ def main():
         todo += cases(rng) + op_cases(np.random.default_rng(20260404))
     if "elem" in groups or "spaces" in groups:
         todo += space_cases(np.random.default_rng(20260505))
+    if "elem" in groups or "high" in groups:
+        todo += high_cases(np.random.default_rng(20260808))
     for case in todo:
         name, pair, tt, pp, order, xyz, D = case[:7]
         extra = case[7] if len(case) > 7 else None

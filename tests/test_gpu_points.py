@@ -43,6 +43,19 @@ def test_device_points_reproduce_reference_xdep_fixtures(case):
     assert T.rel_frob(A, Aref) <= 1e-12, case
 
 
+@pytest.mark.parametrize("order", [3, 10, 20])
+def test_device_points_of_large_rules(order):
+    """Rules beyond 24 points (conical products, orders 7..20) read their
+    barycentric points from device memory; the 8-point order-3 rule from the
+    kernel arguments."""
+    rng = np.random.default_rng(order)
+    n = 300
+    xyz = rng.uniform(-1, 1, (n, 12))
+    X = af.quadrature_points_batch(order, dev(T.soa_xyz(xyz))).cpu().numpy()
+    assert X.shape[1] == len(af.quadrature(order)[1])
+    np.testing.assert_allclose(X.transpose(2, 1, 0), T.product_physical_points(xyz, order), rtol=0, atol=1e-14)
+
+
 def test_context_points_drive_an_x_dependent_assembly():
     """P2 diffusion with kappa(x) = m(x) evaluated on the device at the
     context's points (reordered cells of a shuffled mesh), against the

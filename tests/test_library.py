@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import anifem_amd as af
+import anifem_testlib as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -36,12 +37,18 @@ def test_version_and_op_sizes():
 
 
 def test_quadrature_rules():
-    for order, q in [(0, 1), (1, 1), (2, 4), (3, 14), (4, 14), (5, 14), (6, 24)]:
+    counts = [(0, 1), (1, 1), (2, 4), (3, 8), (4, 14), (5, 14), (6, 24)]
+    counts += [(o, (o // 2 + 1) ** 3) for o in range(7, 21)]  # conical products
+    for order, q in counts:
         p, w = af.quadrature(order)
-        assert len(w) == q and abs(w.sum() - 1) < 1e-15 and np.all(p > 0)
+        assert len(w) == q and abs(w.sum() - 1) < 1e-15 and np.all(p > 0) and np.all(w > 0)
         assert np.allclose(p.sum(axis=1), 1, atol=1e-15)
-    with pytest.raises(af.AnifemError):
-        af.quadrature(7)
+        if order == 3 or order >= 7:  # same construction, same point order as the oracle's
+            po, wo = T.orc_quadrature(order)
+            assert np.abs(po - p).max() < 1e-15 and np.abs(wo - w).max() < 1e-17
+    for bad in (-1, 21):  # quadrature_formulas.cpp:1497-1498 throws outside 0..20
+        with pytest.raises(af.AnifemError):
+            af.quadrature(bad)
 
 
 def test_argument_errors_without_device():
@@ -49,8 +56,8 @@ def test_argument_errors_without_device():
     t = af.anifem_tensor(af.TENSOR_NULL, af.COEF_CONST, None)
     g1, g2 = af.Op(af.GRAD, af.FEM_P1).c(), af.Op(af.IDEN, af.FEM_P1).c()
     buf = np.zeros(16)
-    # order > 6 (int_tet.inl:99-100 throws)
-    assert L.anifem_fem3dtet_batch(g1, g1, ctypes.byref(t), 7, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
+    # order > 20 (quadrature_formulas.cpp:1497-1498 throws)
+    assert L.anifem_fem3dtet_batch(g1, g1, ctypes.byref(t), 21, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
         == af.ERR_ORDER
     # identity tensor between GRAD (dim 3) and IDEN (dim 1) (diff_tensor.h:313-315)
     assert L.anifem_fem3dtet_batch(g1, g2, ctypes.byref(t), 2, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \

@@ -24,7 +24,8 @@ def mono_mean(al):
     return 6 * f(a) * f(b) * f(c) * f(d) / f(a + b + c + d + 3)
 
 
-@pytest.mark.parametrize("order,deg", [(1, 1), (2, 2), (3, 3), (4, 5), (5, 5), (6, 6)])
+@pytest.mark.parametrize("order,deg", [(1, 1), (2, 2), (3, 3), (4, 5), (5, 5), (6, 6)] +
+                         [(o, o + (o % 2 == 0)) for o in range(7, 21)])
 def test_quadrature_exactness(order, deg):
     # restates QuadratureFormulas.TetraFormulas (tests/fem/quadrature_formulas_test.cpp:7-45)
     # over every barycentric monomial of degree <= deg
@@ -43,6 +44,42 @@ def test_quadrature_exactness(order, deg):
     f = x + 10 * y ** d + 1000 * z ** deg
     exact = 1 / 24 + 10 / ((d + 1) * (d + 2) * (d + 3)) + 1000 / ((deg + 1) * (deg + 2) * (deg + 3))
     assert abs(np.sum(w * f) / 6 - exact) < 1e-13 * exact
+
+
+def test_order3_equal_weight_rule():
+    """Order 3: two S4 orbits with weights 1/8.  With the distinct coordinate
+    of an orbit 1/4 + d, the moments of l0^2, l0^3 give d1^2 + d2^2 = 9/40 and
+    d1^3 + d2^3 = 9/160; so s = d1 + d2 is a root of 80 s^3 - 54 s + 9.  The
+    reference's 8-point rule (another member of the family, unequal weights)
+    has the same size and the same integrals through degree 3, not beyond."""
+    p, w = T.orc_quadrature(3)
+    assert len(w) == 8 and np.all(w == 0.125)
+    d = []
+    for o in range(2):
+        orb = p[4 * o:4 * o + 4]
+        dist = np.diag(orb)  # point i of an orbit carries its distinct coordinate at position i
+        assert np.ptp(dist) < 1e-16
+        rest = orb[~np.eye(4, dtype=bool)]
+        assert np.ptp(rest) < 1e-16 and abs(dist[0] + 3 * rest[0] - 1) < 1e-15
+        d.append(dist[0] - 0.25)
+    s = d[0] + d[1]
+    assert abs(d[0] ** 2 + d[1] ** 2 - 9 / 40) < 1e-15 and abs(d[0] ** 3 + d[1] ** 3 - 9 / 160) < 1e-15
+    assert abs(80 * s ** 3 - 54 * s + 9) < 1e-13
+    R = T.ref_lib()
+    if R is None:
+        return
+    q = R.ref_quadrature(3, None, None)
+    pr, wr = np.zeros(4 * q), np.zeros(q)
+    R.ref_quadrature(3, T.P(pr), T.P(wr))
+    pr = pr.reshape(q, 4)
+    assert q == 8
+    for al in [(1, 1, 1, 0), (3, 0, 0, 0), (2, 1, 0, 0), (4, 0, 0, 0), (2, 2, 0, 0)]:
+        mine = np.sum(w * np.prod(p ** np.array(al), axis=1))
+        ref = np.sum(wr * np.prod(pr ** np.array(al), axis=1))
+        if sum(al) <= 3:
+            assert abs(mine - ref) < 1e-16
+        else:
+            assert abs(mine - ref) > 1e-6
 
 
 def test_quadrature_matches_reference_rules():
