@@ -240,6 +240,28 @@ int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, 
 int anifem_assembly_create_budget(anifem_assembly** ctx, const anifem_mesh_desc* mesh,
                                   const anifem_problem_desc* prob, int max_slots, int max_values, int max_pairs,
                                   void* stream);
+/* Problems over several variables (the reference's assembler over a list of
+ * FemSpace variables, e.g. mixed Poisson: a flux in RT0 and a pressure in
+ * P0).  NATURAL numbering VAR, DIM, ELEM_TYPE, ELEM_ID, DOF_ID
+ * (global_enumerator.h:396): every DOF of variable v precedes those of
+ * variable v+1, and each variable is numbered as it would be alone (vector
+ * variables by component).  A tet's local DOFs list the variables in order,
+ * each in its own local order.  Such a context assembles element matrices the
+ * caller forms over that local order (anifem_assembly_run_elements,
+ * anifem_assembly_run_rhs: the AssemblerT::Assemble path with a local
+ * function); anifem_assembly_run, anifem_assembly_run_plain and
+ * anifem_assembly_eval return ANIFEM_ERR_ARG.  Of prob only dirichlet and
+ * dirichlet_dof_mask are read (BOUNDARY flags the boundary-closure DOFs of
+ * every variable).  At most 4 variables and 64 local DOFs. */
+typedef struct anifem_vars_desc {
+    int32_t n_var;
+    anifem_op var[4]; /* fem and ncomp of each variable (op is not read) */
+} anifem_vars_desc;
+int anifem_assembly_create_vars(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_vars_desc* vars,
+                                const anifem_problem_desc* prob, void* stream);
+/* Host-only variant (inspection of the numbering and pattern). */
+int anifem_assembly_create_vars_host(anifem_assembly** ctx, const anifem_mesh_desc* mesh,
+                                     const anifem_vars_desc* vars, const anifem_problem_desc* prob);
 int anifem_assembly_destroy(anifem_assembly* ctx);
 /* Host-only variant: builds the same setup without touching the device
  * (inspection of the enumeration, pattern, slot map and row blocks through
@@ -374,7 +396,10 @@ int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int3
  * vectors of nf, or NULL (rhs is then zeroed); both in the caller's tet order
  * and in the local DOF order of the tet's nodes as anifem_assembly_tets
  * returns them (after the orientation reorder, ordering.inl:8-26, as the
- * reference hands the reordered nodes to the local function).  elem_precision:
+ * reference hands the reordered nodes to the local function), in the cell's
+ * local basis: on oriented DOFs (RT0 faces, ND0 edges) the scatter multiplies
+ * entry (ib, ia) by s_ib s_ia and load entry ib by s_ib, the signs of
+ * anifem_assembly_copy_signs (assembler.inl:154-164).  elem_precision:
  * ANIFEM_PREC_FP64 (double) or ANIFEM_PREC_FP32 (float) element values; the
  * CSR and rhs accumulate in FP64, with element-level homogeneous Dirichlet and
  * the same deterministic order as anifem_assembly_run. */

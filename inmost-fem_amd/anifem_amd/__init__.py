@@ -69,6 +69,10 @@ class anifem_problem_desc(ctypes.Structure):
                 ("dirichlet_dof_mask", ctypes.c_void_p)]
 
 
+class anifem_vars_desc(ctypes.Structure):
+    _fields_ = [("n_var", ctypes.c_int32), ("var", anifem_op * 4)]
+
+
 class anifem_status(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("n_nonfinite", ctypes.c_int64), ("n_degenerate", ctypes.c_int64)]
@@ -123,6 +127,12 @@ SIGNATURES = {
                                       ctypes.POINTER(anifem_problem_desc), _VP]),
     "anifem_assembly_create_host": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
                                            ctypes.POINTER(anifem_problem_desc)]),
+    "anifem_assembly_create_vars": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
+                                           ctypes.POINTER(anifem_vars_desc), ctypes.POINTER(anifem_problem_desc),
+                                           _VP]),
+    "anifem_assembly_create_vars_host": (_I32, [ctypes.POINTER(_VP), ctypes.POINTER(anifem_mesh_desc),
+                                                ctypes.POINTER(anifem_vars_desc),
+                                                ctypes.POINTER(anifem_problem_desc)]),
     "anifem_assembly_destroy": (_I32, [_VP]),
     "anifem_assembly_info": (_I32, [_VP, _VP, _VP, _VP, _VP]),
     "anifem_assembly_csr": (_I32, [_VP, _VP, _VP]),
@@ -398,12 +408,17 @@ class Assembler:
     AssemblerT::PrepareProblem + Assemble (inmost_interface/assembler.h:232-446).
     With host_only=True only the setup is built (no device use)."""
 
-    def __init__(self, coords, tets, trial: Op, test: Op = None, order=2, tensor_type=TENSOR_NULL,
+    def __init__(self, coords, tets, trial: Op = None, test: Op = None, order=2, tensor_type=TENSOR_NULL,
                  tensor_storage=COEF_CONST, rhs_order=None, rhs_tensor_type=TENSOR_SCALAR,
                  rhs_tensor_storage=COEF_CONST, dirichlet=DIRICHLET_NONE, dirichlet_dof_mask=None, host_only=False,
-                 budget=None, stream=None):
+                 budget=None, stream=None, variables=None):
+        """variables: a list of Op (fem and ncomp are read) for a
+        multi-variable problem (anifem_assembly_create_vars): caller element
+        matrices over the variables' concatenated local DOFs only."""
         self._coords = np.ascontiguousarray(coords, dtype=np.float64)
         self._tets = np.ascontiguousarray(tets, dtype=np.int32)
+        if variables is not None:
+            trial = trial or Op(IDEN, variables[0].fem, variables[0].ncomp)
         test = test or trial
         self.trial, self.test = trial, test
         self.tensor_type, self.tensor_storage = tensor_type, tensor_storage
@@ -416,7 +431,21 @@ class Assembler:
                                    dirichlet, _ptr(self._mask))
         h = ctypes.c_void_p()
         L = lib()
-        if host_only:
+        if variables is not None:
+            if len(variables) > 4:
+                raise ValueError("at most 4 variables")
+            vd = anifem_vars_desc()
+            vd.n_var = len(variables)
+            for k, v in enumerate(variables):
+                vd.var[k] = v.c()
+            self.variables = list(variables)
+            if host_only:
+                rc = L.anifem_assembly_create_vars_host(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(vd),
+                                                        ctypes.byref(prob))
+            else:
+                rc = L.anifem_assembly_create_vars(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(vd),
+                                                   ctypes.byref(prob), _stream_handle(stream))
+        elif host_only:
             rc = L.anifem_assembly_create_host(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob))
         elif budget is not None:  # (max_slots, max_values, max_pairs) per row block
             rc = L.anifem_assembly_create_budget(ctypes.byref(h), ctypes.byref(mesh), ctypes.byref(prob),

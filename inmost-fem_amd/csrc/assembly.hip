@@ -70,6 +70,8 @@ struct anifem_assembly {
     // matrices and load vectors of the last run
     bool elem_path = false;
     int8_t* d_sign = nullptr;
+    uint64_t* d_signmask = nullptr;   // per tet: bit j = local DOF j has sign -1 (oriented spaces)
+    bool vars_only = false;           // anifem_assembly_create_vars: caller element matrices only
     double* d_xyz = nullptr;
     void* d_ebuf = nullptr;
     void* d_fbuf = nullptr;
@@ -483,9 +485,25 @@ int greedy_colour(int64_t n_nodes, int64_t n_tets, const int32_t* tets, std::vec
 // The setup of a context: its enumeration phase on the device for a device
 // context, on the host for a host-only one (identical results).
 int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-               const anifem::BlockCaps* caps = nullptr, bool on_device = false, void* stream = nullptr) {
+               const anifem::BlockCaps* caps = nullptr, bool on_device = false, void* stream = nullptr,
+               const anifem_vars_desc* vars = nullptr) {
     using namespace anifem;
     if (!mesh || !prob || !mesh->coords || !mesh->tets) return ANIFEM_ERR_ARG;
+    if (vars) {
+        // a multi-variable problem: element matrices only, numbering on the host
+        if (vars->n_var < 1 || vars->n_var > kMaxVars) return ANIFEM_ERR_ARG;
+        int nf = 0;
+        for (int v = 0; v < vars->n_var; ++v) {
+            if (!space_layout(vars->var[v].fem).valid || vars->var[v].ncomp < 1) return ANIFEM_ERR_ARG;
+            nf += nf_scalar(vars->var[v].fem) * vars->var[v].ncomp;
+        }
+        ctx->prob = *prob;
+        ctx->prob.dirichlet_dof_mask = nullptr;
+        ctx->prob.trial = ctx->prob.test = anifem_op{ANIFEM_IDEN, vars->var[0].fem, vars->var[0].ncomp};
+        ctx->vars_only = true;
+        const BlockCaps c = caps ? *caps : default_caps(nf);
+        return build_setup_vars(*mesh, vars->n_var, vars->var, *prob, ctx->S, c);
+    }
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
     // the global assembly numbers the DOFs of the spaces of space_layout
     if (!space_layout(prob->trial.fem).valid || !space_layout(prob->test.fem).valid) return ANIFEM_ERR_ARG;
@@ -500,7 +518,20 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
 
 namespace anifem {
 int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-                         const BlockCaps& caps, void* stream);
+                         const BlockCaps& caps, void* stream, const anifem_vars_desc* vars = nullptr);
+}
+
+extern "C" int anifem_assembly_create_vars(anifem_assembly** out, const anifem_mesh_desc* mesh,
+                                           const anifem_vars_desc* vars, const anifem_problem_desc* prob,
+                                           void* stream) {
+    using namespace anifem;
+    if (!vars || vars->n_var < 1 || vars->n_var > kMaxVars) return ANIFEM_ERR_ARG;
+    int nf = 0;
+    for (int v = 0; v < vars->n_var; ++v) {
+        if (!space_layout(vars->var[v].fem).valid || vars->var[v].ncomp < 1) return ANIFEM_ERR_ARG;
+        nf += nf_scalar(vars->var[v].fem) * vars->var[v].ncomp;
+    }
+    return assembly_create_caps(out, mesh, prob, default_caps(nf), stream, vars);
 }
 
 extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
@@ -522,11 +553,12 @@ extern "C" int anifem_assembly_create_budget(anifem_assembly** out, const anifem
 
 // Assembly context with given row-block budgets (the batch autotuner's trials).
 int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh,
-                                 const anifem_problem_desc* prob, const BlockCaps& caps, void* stream) {
+                                 const anifem_problem_desc* prob, const BlockCaps& caps, void* stream,
+                                 const anifem_vars_desc* vars) {
     using namespace anifem;
     if (!out) return ANIFEM_ERR_ARG;
     std::unique_ptr<anifem_assembly> ctx(new anifem_assembly());
-    int rc = build_host(ctx.get(), mesh, prob, &caps, true, stream);
+    int rc = build_host(ctx.get(), mesh, prob, &caps, true, stream, vars);
     if (rc) return rc;
     AssemblySetup& S = ctx->S;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -541,10 +573,10 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
         RefTensorDev R;
         std::vector<double> pts, wts;
         const bool pp = prob->tensor_storage == ANIFEM_COEF_PER_POINT;
-        if (rule_points(prob->order, -1, pts, wts) > 0)
+        if (!vars && rule_points(prob->order, -1, pts, wts) > 0)
             (void)get_ref_tensor(prob->trial.op, prob->trial.fem, prob->test.op, prob->test.fem, prob->order, pp, &R);
         const bool ppf = prob->rhs_tensor_storage == ANIFEM_COEF_PER_POINT;
-        if (rule_points(prob->rhs_order, -1, pts, wts) > 0)
+        if (!vars && rule_points(prob->rhs_order, -1, pts, wts) > 0)
             (void)get_ref_tensor(ANIFEM_IDEN, ANIFEM_FEM_P0, ANIFEM_IDEN, prob->test.fem, prob->rhs_order, ppf, &R);
     }
     const int64_t NT = S.n_tets;
@@ -577,8 +609,16 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_status, 4 * sizeof(int)));
     // spaces whose elements need the Piola map and per-DOF factors (RT0,
     // ND0) or have one DOF (P0, B4) are assembled from element matrices
-    ctx->elem_path = fem_vector(prob->trial.fem) || (S.nf & 1);
-    if (!S.sign.empty() && (rc = upload(&ctx->d_sign, S.sign.data(), S.sign.size(), s))) return rc;
+    ctx->elem_path = vars || fem_vector(prob->trial.fem) || (S.nf & 1);
+    if (!S.sign.empty()) {
+        if ((rc = upload(&ctx->d_sign, S.sign.data(), S.sign.size(), s))) return rc;
+        std::vector<uint64_t> mask((size_t)S.n_tets, 0);
+        for (int64_t t = 0; t < S.n_tets; ++t)
+            for (int j = 0; j < S.nf; ++j)
+                if (S.sign[(size_t)t * S.nf + j] < 0) mask[(size_t)t] |= uint64_t(1) << j;
+        if ((rc = upload(&ctx->d_signmask, mask.data(), mask.size(), s))) return rc;
+        ANIFEM_HIP_CHECK(hipStreamSynchronize(s));  // mask is a host temporary
+    }
     // rows an inhomogeneous Dirichlet condition changes: Dirichlet rows and
     // rows with a dropped (Dirichlet-column) entry in one of their pairs
     {
@@ -622,6 +662,17 @@ int anifem_assembly_create_host(anifem_assembly** out, const anifem_mesh_desc* m
     return ANIFEM_OK;
 }
 
+int anifem_assembly_create_vars_host(anifem_assembly** out, const anifem_mesh_desc* mesh,
+                                     const anifem_vars_desc* vars, const anifem_problem_desc* prob) {
+    if (!out || !vars) return ANIFEM_ERR_ARG;
+    std::unique_ptr<anifem_assembly> ctx(new anifem_assembly());
+    ctx->host_only = true;
+    int rc = build_host(ctx.get(), mesh, prob, nullptr, false, nullptr, vars);
+    if (rc) return rc;
+    *out = ctx.release();
+    return ANIFEM_OK;
+}
+
 int anifem_assembly_destroy(anifem_assembly* ctx) {
     if (!ctx) return ANIFEM_OK;
     for (hipEvent_t e : ctx->ev)
@@ -632,7 +683,7 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
                         ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets,
                         ctx->d_lift,     ctx->d_pair_pos8, ctx->d_sign,   ctx->d_xyz,      ctx->d_ebuf,
-                        ctx->d_fbuf};
+                        ctx->d_fbuf,     ctx->d_signmask};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -712,7 +763,7 @@ int anifem_assembly_copy_blocks(const anifem_assembly* ctx, int32_t* blk_row0, i
 
 int anifem_assembly_set_scatter(anifem_assembly* ctx, int method) {
     using namespace anifem;
-    if (!ctx || ctx->host_only) return ANIFEM_ERR_ARG;
+    if (!ctx || ctx->host_only || (ctx->vars_only && method != ANIFEM_SCATTER_ROW_BLOCKS)) return ANIFEM_ERR_ARG;
     if (method == ANIFEM_SCATTER_ROW_BLOCKS) {
         ctx->coloured = false;
         return ANIFEM_OK;
@@ -799,21 +850,6 @@ __global__ void gather_xyz_kernel(const double* __restrict__ coords, const int32
     }
 }
 
-// Global basis = sign x local basis on oriented DOFs (assembler.inl:154-164):
-// A(ib, ia) *= s_ib s_ia, F(ib) *= s_ib, one lane per matrix entry.
-template <typename T>
-__global__ void apply_signs_kernel(int64_t n, int nf, const int8_t* __restrict__ sign, T* A, T* F) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t E = (int64_t)nf * nf;
-    if (i >= n * E) return;
-    const int64_t r = i / E;
-    const int e = (int)(i - r * E), ib = e % nf, ia = e / nf;
-    const int8_t* sg = sign + r * nf;
-    const int s = sg[ib] * sg[ia];
-    if (s < 0) A[i] = -A[i];
-    if (F && ia == 0 && sg[ib] < 0) F[r * nf + ib] = -F[r * nf + ib];
-}
-
 template <typename T>
 int grow(void** buf, size_t* have, size_t need) {
     if (*have >= need) return ANIFEM_OK;
@@ -829,9 +865,9 @@ int grow(void** buf, size_t* have, size_t need) {
 // fused pass does not form (RT0 / ND0: Piola map, per-DOF factors and
 // orientation signs; one-DOF spaces): the element batch (anifem_fem3dtet_batch,
 // the reference's runtime fem3Dtet over any operator pair, int_tet.inl:2-27)
-// on the reordered cells, the signs of oriented DOFs, then the row-block
-// scatter of element matrices (anifem_assembly_run_elements) with
-// element-level Dirichlet.
+// on the reordered cells, then the row-block scatter of element matrices
+// (anifem_assembly_run_elements, which applies the signs of oriented DOFs)
+// with element-level Dirichlet.
 int run_via_elements(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, int precision,
                      double* csr_val, double* rhs, anifem_status* status, hipStream_t s) {
     using namespace anifem;
@@ -860,16 +896,6 @@ int run_via_elements(anifem_assembly* ctx, const anifem_tensor* D, const anifem_
         if ((rc = anifem_fem3dtet_batch(p0, tb, F, ctx->prob.rhs_order, n, ctx->d_xyz, precision, ctx->d_fbuf, s)))
             return rc;
     }
-    if (ctx->d_sign) {
-        const unsigned sb = (unsigned)((n * nf * nf + 255) / 256);
-        if (precision == ANIFEM_PREC_FP64)
-            hipLaunchKernelGGL(apply_signs_kernel<double>, dim3(sb), dim3(256), 0, s, n, nf, ctx->d_sign,
-                               static_cast<double*>(ctx->d_ebuf), do_rhs ? static_cast<double*>(ctx->d_fbuf) : nullptr);
-        else
-            hipLaunchKernelGGL(apply_signs_kernel<float>, dim3(sb), dim3(256), 0, s, n, nf, ctx->d_sign,
-                               static_cast<float*>(ctx->d_ebuf), do_rhs ? static_cast<float*>(ctx->d_fbuf) : nullptr);
-        ANIFEM_HIP_CHECK(hipGetLastError());
-    }
     return anifem_assembly_run_elements(ctx, ctx->d_ebuf, do_rhs ? ctx->d_fbuf : nullptr, precision, csr_val, rhs,
                                         status, s);
 }
@@ -879,7 +905,7 @@ int run_via_elements(anifem_assembly* ctx, const anifem_tensor* D, const anifem_
 int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, const double* dir_values,
                  int precision, double* csr_val, double* rhs, anifem_status* status, void* stream, void* scratch) {
     using namespace anifem;
-    if (!ctx || ctx->host_only || !D || !csr_val) return ANIFEM_ERR_ARG;
+    if (!ctx || ctx->host_only || ctx->vars_only || !D || !csr_val) return ANIFEM_ERR_ARG;
     // Dirichlet values are applied by the row-block scatter (the coloured
     // scatter resolves homogeneous conditions only)
     if (dir_values && ctx->coloured) return ANIFEM_ERR_ARG;
@@ -938,6 +964,7 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     a.pair_pos8 = ctx->d_pair_pos8;
     a.sstart = ctx->d_sstart;
     a.pstart = ctx->d_pstart;
+    a.signmask = ctx->d_signmask;
     a.val = csr_val;
     a.rhs = rhs;
     a.status = ctx->d_status;
@@ -1023,6 +1050,7 @@ int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_p
     a.pair_tet = ctx->d_pair_tet;
     a.pair_ib = ctx->d_pair_ib;
     a.pstart = ctx->d_pstart;
+    a.signmask = ctx->d_signmask;
     a.rhs = rhs;
     a.status = ctx->d_status;
     if (S.max_block_pairs > kBlock) return ANIFEM_ERR_DIMS;
@@ -1054,7 +1082,7 @@ int anifem_assembly_run_rhs(anifem_assembly* ctx, const void* F_elem, int elem_p
 int anifem_assembly_eval(anifem_assembly* ctx, int op, const double* xyl, int q, const double* u, double* opU,
                          void* stream) {
     using namespace anifem;
-    if (!ctx || ctx->host_only || !u || !opU) return ANIFEM_ERR_ARG;
+    if (!ctx || ctx->host_only || ctx->vars_only || !u || !opU) return ANIFEM_ERR_ARG;
     // IDEN / GRAD, and DIV / CURL of the vector-valued spaces
     if (op != ANIFEM_IDEN && op != ANIFEM_GRAD && !(fem_vector(ctx->prob.trial.fem) && op != ANIFEM_GRAD))
         return ANIFEM_ERR_ARG;
@@ -1150,6 +1178,7 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
     a.pair_pos8 = ctx->d_pair_pos8;
     a.sstart = ctx->d_sstart;
     a.pstart = ctx->d_pstart;
+    a.signmask = ctx->d_signmask;
     a.val = csr_val;
     a.rhs = rhs;
     a.status = ctx->d_status;
@@ -1189,7 +1218,7 @@ int anifem_assembly_run_elements(anifem_assembly* ctx, const void* A_elem, const
 int anifem_assembly_run_plain(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tensor* F, double* csr_val,
                               double* rhs, anifem_status* status, void* stream) {
     using namespace anifem;
-    if (!ctx || ctx->host_only || !D || !csr_val) return ANIFEM_ERR_ARG;
+    if (!ctx || ctx->host_only || ctx->vars_only || !D || !csr_val) return ANIFEM_ERR_ARG;
     const anifem_problem_desc& P = ctx->prob;
     // scalar GRAD x GRAD spaces only (the C2 / C3 problems)
     if (P.trial.op != ANIFEM_GRAD || P.test.op != ANIFEM_GRAD || P.trial.ncomp != 1) return ANIFEM_ERR_ARG;

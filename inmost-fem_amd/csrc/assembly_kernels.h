@@ -60,6 +60,7 @@ struct BlockArgs {
     const uint32_t* pair_pos8;  // compact positions (compact_positions), or null
     const uint16_t* sstart;     // per slot: first value, relative to its block
     const uint16_t* pstart;     // per row: first pair, relative to its block
+    const uint64_t* signmask;   // element-matrix passes: per tet, bit j = local DOF j has orientation sign -1, or null
     double* val;
     double* rhs;
     int* status;
@@ -474,6 +475,10 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a,
         const int ibf = a.pair_ib[gp];
         const int ib = ibf & 63;
         const bool dir = (ibf & kPairDirichlet) != 0;
+        // orientation signs (assembler.inl:154-164): entry (ib, ia) of the
+        // global basis is s_ib s_ia times the local one
+        const uint64_t sm = a.signmask ? a.signmask[tet] : 0;
+        const uint64_t flip = ((sm >> ib) & 1) ? ~sm : sm;  // bit ia: s_ib s_ia = -1
         const TE* At = Ae + tet * nf * nf + (SYM ? (int64_t)nf * ib : ib);
         const int64_t st = SYM ? 1 : nf;
         // eight entries at a time: their positions and values are requested
@@ -492,13 +497,14 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_kernel(BlockArgs a,
 #pragma unroll
             for (int j = 0; j < U; ++j) {
                 if (ps[j] == kPosDropped) continue;
-                const double x = dir ? 1.0 : (double)v[j];
+                const double x = dir ? 1.0 : ((flip >> ((ia0 + j) & 63)) & 1 ? -(double)v[j] : (double)v[j]);
                 nonfinite += isfinite(x) ? 0 : 1;
                 vals[ps[j]] = x;
             }
         }
         if (a.has_rhs) {
-            const double fv = dir ? 0.0 : (double)Fe[tet * nf + ib];
+            const double f0 = (double)Fe[tet * nf + ib];
+            const double fv = dir ? 0.0 : ((sm >> ib) & 1 ? -f0 : f0);
             nonfinite += isfinite(fv) ? 0 : 1;
             fvals[threadIdx.x] = fv;
         }
@@ -525,7 +531,9 @@ __global__ __launch_bounds__(kBlock) void row_block_rhs_kernel(BlockArgs a, cons
         const int64_t gp = M.p0 + threadIdx.x;
         const int64_t tet = a.pair_tet[gp];
         const int ibf = a.pair_ib[gp];
-        const double v = (ibf & kPairDirichlet) ? 0.0 : (double)Fe[tet * nf + (ibf & 63)];
+        const uint64_t sm = a.signmask ? a.signmask[tet] : 0;
+        const double f0 = (double)Fe[tet * nf + (ibf & 63)];
+        const double v = (ibf & kPairDirichlet) ? 0.0 : ((sm >> (ibf & 63)) & 1 ? -f0 : f0);
         nonfinite += isfinite(v) ? 0 : 1;
         fv[threadIdx.x] = v;
     }
@@ -1020,6 +1028,7 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_persistent(BlockArg
         uint32_t posw[PW];
         TE v[NF];
         TE fv;
+        uint64_t flip;  // bit ia: s_ib s_ia = -1 (orientation signs)
         uint32_t sw[SW];
         int pst;
     };
@@ -1040,6 +1049,11 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_persistent(BlockArg
 #pragma unroll
         for (int e = 0; e < NF; ++e) P.v[e] = hp ? At[(SYM ? 1 : NF) * e] : TE(0);
         P.fv = (hp && a.has_rhs) ? Fe[(int64_t)P.tet * NF + ib] : TE(0);
+        // orientation signs (assembler.inl:154-164): s_ib on the load entry
+        const uint64_t sm = (hp && a.signmask) ? a.signmask[P.tet] : 0;
+        const bool nb = (sm >> ib) & 1;
+        P.flip = nb ? ~sm : sm;
+        if (nb) P.fv = -P.fv;
         if (sw_regs) {
             const uint32_t* sg = reinterpret_cast<const uint32_t*>(a.sstart) + (Mx.s0 >> 1);
             const int nw = slot_words(Mx.s0, (int)Mx.ns);
@@ -1089,7 +1103,7 @@ __global__ __launch_bounds__(kBlock) void row_block_elements_persistent(BlockArg
                 const uint32_t w = cur.posw[e >> 1];
                 const int ps = (int)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
                 if (ps != kPosDropped) {
-                    const double v = dir ? 1.0 : (double)cur.v[e];
+                    const double v = dir ? 1.0 : ((cur.flip >> e) & 1 ? -(double)cur.v[e] : (double)cur.v[e]);
                     nonfinite += isfinite(v) ? 0 : 1;
                     vals[ps] = v;
                 }

@@ -33,7 +33,7 @@
 namespace anifem {
 int prepare_elem_args(const anifem_op& opA, const anifem_op& opB, const anifem_tensor* D, int order, ElemArgs& a);
 int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
-                         const BlockCaps& caps, void* stream);
+                         const BlockCaps& caps, void* stream, const anifem_vars_desc* vars = nullptr);
 }  // namespace anifem
 
 namespace {

@@ -139,18 +139,45 @@ inline int8_t face_sign(const int32_t* v, int k) {
 }
 }  // namespace
 
-int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
-    const int fem = prob.trial.fem;
-    if (!space_layout(fem).valid) return ANIFEM_ERR_ARG;
-    if (prob.test.fem != fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
-    // vector spaces FemVec<3, P1..P3> only
-    if (prob.trial.ncomp != 1 && (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3)) return ANIFEM_ERR_ARG;
-    S.fem = fem;
-    S.ncomp = prob.trial.ncomp;
-    const int nf0 = nf_scalar(fem);
-    S.nf = nf0 * S.ncomp;
+namespace {
+// The variables' spaces and sizes; every space is one of space_layout's,
+// vector variables FemVec<ncomp, P1..P3> only, at most 64 local DOFs in all.
+int set_vars(int n_var, const anifem_op* vars, AssemblySetup& S) {
+    if (n_var < 1 || n_var > kMaxVars) return ANIFEM_ERR_ARG;
+    S.n_var = n_var;
+    S.nf = 0;
+    for (int v = 0; v < n_var; ++v) {
+        const int fem = vars[v].fem, nc = vars[v].ncomp;
+        if (!space_layout(fem).valid || nc < 1) return ANIFEM_ERR_ARG;
+        if (nc != 1 && (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3)) return ANIFEM_ERR_ARG;
+        S.var_fem[v] = fem;
+        S.var_ncomp[v] = nc;
+        S.nf += nf_scalar(fem) * nc;
+        if (S.nf > 64) return ANIFEM_ERR_ARG;  // Dirichlet masks are 64-bit
+    }
+    S.fem = vars[0].fem;
+    S.ncomp = vars[0].ncomp;
     S.E = S.nf * S.nf;
-    if (S.nf > 64) return ANIFEM_ERR_ARG;  // Dirichlet masks are 64-bit
+    return ANIFEM_OK;
+}
+
+int reorder_nodes(const anifem_mesh_desc& mesh, AssemblySetup& S);
+}  // namespace
+
+int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
+    if (prob.test.fem != prob.trial.fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
+    int rc = set_vars(1, &prob.trial, S);
+    return rc ? rc : reorder_nodes(mesh, S);
+}
+
+int setup_prepare_vars(const anifem_mesh_desc& mesh, int n_var, const anifem_op* vars, AssemblySetup& S) {
+    if (!vars) return ANIFEM_ERR_ARG;
+    int rc = set_vars(n_var, vars, S);
+    return rc ? rc : reorder_nodes(mesh, S);
+}
+
+namespace {
+int reorder_nodes(const anifem_mesh_desc& mesh, AssemblySetup& S) {
     S.n_nodes = mesh.n_nodes;
     S.n_tets = mesh.n_tets;
     const int64_t NT = S.n_tets, NN = S.n_nodes;
@@ -192,18 +219,23 @@ int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob,
     S.n_degenerate = nonpos.load();
     return ANIFEM_OK;
 }
+}  // namespace
 
 int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
-    const int fem = S.fem;
-    const int nf0 = nf_scalar(fem);
     const int64_t NT = S.n_tets, NN = S.n_nodes;
-    const SpaceLayout L = space_layout(fem);
+    SpaceLayout L[kMaxVars];
+    bool any_edge = false, any_face = false;
+    for (int v = 0; v < S.n_var; ++v) {
+        L[v] = space_layout(S.var_fem[v]);
+        any_edge = any_edge || L[v].edge > 0;
+        any_face = any_face || L[v].face > 0;
+    }
 
     // edges and faces by first appearance
     std::vector<int64_t> eid, fid;
     std::vector<int32_t> fmult;
-    const bool need_edges = L.edge > 0 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
-    const bool need_faces = L.face > 0 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    const bool need_edges = any_edge || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
+    const bool need_faces = any_face || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
     if (need_edges) {
         std::vector<uint64_t> keys(6 * NT);
         host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
@@ -230,45 +262,68 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
         S.n_faces = number_by_first_appearance(keys, fid, &fmult);
     }
 
-    // NATURAL enumeration: nodes, edges, faces, cells (global_enumerator.h:392-404)
-    const int64_t per_edge = L.edge, per_face = L.face;
-    const int64_t edge_base = NN * L.node, face_base = edge_base + per_edge * S.n_edges;
-    const int64_t cell_base = face_base + per_face * S.n_faces;
-    const int64_t nscal = cell_base + (int64_t)L.cell * NT;
-    S.ndof = nscal * S.ncomp;
+    // NATURAL enumeration (global_enumerator.h:392-404): per variable, per
+    // component, nodes, edges, faces, cells; variable v starts at var_base[v]
+    struct VarNum {
+        int64_t edge_base, face_base, cell_base, nscal, base;
+        int nf0, lf0;  // scalar local DOFs, first local DOF of the variable
+    };
+    VarNum V[kMaxVars];
+    int64_t total = 0;
+    int lf = 0;
+    for (int v = 0; v < S.n_var; ++v) {
+        VarNum& n = V[v];
+        n.edge_base = NN * L[v].node;
+        n.face_base = n.edge_base + (int64_t)L[v].edge * S.n_edges;
+        n.cell_base = n.face_base + (int64_t)L[v].face * S.n_faces;
+        n.nscal = n.cell_base + (int64_t)L[v].cell * NT;
+        n.base = total;
+        n.nf0 = nf_scalar(S.var_fem[v]);
+        n.lf0 = lf;
+        total += n.nscal * S.var_ncomp[v];
+        lf += n.nf0 * S.var_ncomp[v];
+    }
+    S.ndof = total;
     if (S.ndof >= INT32_MAX) return ANIFEM_ERR_ARG;
     S.dofs.resize((size_t)S.nf * NT);
-    const bool oriented = L.edge_oriented || L.face_oriented;
+    bool oriented = false;
+    for (int v = 0; v < S.n_var; ++v) oriented = oriented || L[v].edge_oriented || L[v].face_oriented;
     if (oriented) S.sign.assign((size_t)S.nf * NT, 1);
     else std::vector<int8_t>().swap(S.sign);
     host_parallel_for(NT, 1 << 14, [&](int64_t b, int64_t e) {
         for (int64_t t = b; t < e; ++t) {
-            const int32_t* v = &S.tets[4 * t];
-            int64_t loc[20];
-            int8_t sg[20];
-            int i = 0;
-            for (int k = 0; k < 4 && L.node; ++k, ++i) loc[i] = v[k], sg[i] = 1;
-            for (int k = 0; k < 6 && per_edge; ++k) {
-                const int64_t base = edge_base + per_edge * eid[(size_t)6 * t + k];
-                if (per_edge == 1) {
-                    loc[i] = base;
-                    sg[i++] = L.edge_oriented ? edge_sign(v, k) : 1;
-                } else if (per_edge == 2) {
-                    const bool keep = edge_pair_kept(v, k);
-                    loc[i] = base + (keep ? 0 : 1), sg[i++] = 1;
-                    loc[i] = base + (keep ? 1 : 0), sg[i++] = 1;
+            const int32_t* vt = &S.tets[4 * t];
+            for (int var = 0; var < S.n_var; ++var) {
+                const SpaceLayout& Lv = L[var];
+                const VarNum& n = V[var];
+                const int64_t per_edge = Lv.edge, per_face = Lv.face;
+                int64_t loc[20];
+                int8_t sg[20];
+                int i = 0;
+                for (int k = 0; k < 4 && Lv.node; ++k, ++i) loc[i] = vt[k], sg[i] = 1;
+                for (int k = 0; k < 6 && per_edge; ++k) {
+                    const int64_t base = n.edge_base + per_edge * eid[(size_t)6 * t + k];
+                    if (per_edge == 1) {
+                        loc[i] = base;
+                        sg[i++] = Lv.edge_oriented ? edge_sign(vt, k) : 1;
+                    } else if (per_edge == 2) {
+                        const bool keep = edge_pair_kept(vt, k);
+                        loc[i] = base + (keep ? 0 : 1), sg[i++] = 1;
+                        loc[i] = base + (keep ? 1 : 0), sg[i++] = 1;
+                    }
                 }
-            }
-            for (int k = 0; k < 4 && per_face; ++k, ++i) {
-                loc[i] = face_base + fid[(size_t)4 * t + k];
-                sg[i] = L.face_oriented ? face_sign(v, k) : 1;
-            }
-            if (L.cell) loc[i] = cell_base + t, sg[i++] = 1;
-            for (int c = 0; c < S.ncomp; ++c)
-                for (int j = 0; j < nf0; ++j) {
-                    S.dofs[(size_t)S.nf * t + nf0 * c + j] = (int32_t)(c * nscal + loc[j]);
-                    if (oriented) S.sign[(size_t)S.nf * t + nf0 * c + j] = sg[j];
+                for (int k = 0; k < 4 && per_face; ++k, ++i) {
+                    loc[i] = n.face_base + fid[(size_t)4 * t + k];
+                    sg[i] = Lv.face_oriented ? face_sign(vt, k) : 1;
                 }
+                if (Lv.cell) loc[i] = n.cell_base + t, sg[i++] = 1;
+                for (int c = 0; c < S.var_ncomp[var]; ++c)
+                    for (int j = 0; j < n.nf0; ++j) {
+                        const size_t at = (size_t)S.nf * t + n.lf0 + n.nf0 * c + j;
+                        S.dofs[at] = (int32_t)(n.base + c * n.nscal + loc[j]);
+                        if (oriented) S.sign[at] = sg[j];
+                    }
+            }
         }
     });
 
@@ -295,11 +350,16 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
                     if (in && in2) sedge[eid[6 * t + q]] = 1;
                 }
             }
-        for (int c = 0; c < S.ncomp; ++c) {
-            for (int64_t i = 0; i < NN && L.node; ++i) S.is_dir[c * nscal + i] = snode[i];
-            for (int64_t i = 0; i < S.n_edges; ++i)
-                for (int s = 0; s < per_edge; ++s) S.is_dir[c * nscal + edge_base + per_edge * i + s] = sedge[i];
-            for (int64_t i = 0; i < S.n_faces && per_face; ++i) S.is_dir[c * nscal + face_base + i] = sface[i];
+        for (int var = 0; var < S.n_var; ++var) {
+            const VarNum& n = V[var];
+            const int64_t per_edge = L[var].edge;
+            for (int c = 0; c < S.var_ncomp[var]; ++c) {
+                uint8_t* d = S.is_dir.data() + n.base + c * n.nscal;
+                for (int64_t i = 0; i < NN && L[var].node; ++i) d[i] = snode[i];
+                for (int64_t i = 0; i < S.n_edges; ++i)
+                    for (int s = 0; s < per_edge; ++s) d[n.edge_base + per_edge * i + s] = sedge[i];
+                for (int64_t i = 0; i < S.n_faces && L[var].face; ++i) d[n.face_base + i] = sface[i];
+            }
         }
     } else if (prob.dirichlet != ANIFEM_DIRICHLET_NONE) {
         return ANIFEM_ERR_ARG;
@@ -370,6 +430,17 @@ void dof_adjacency(const AssemblySetup& S, std::vector<int64_t>& aptr, std::vect
     std::vector<int64_t> fill(aptr.begin(), aptr.end() - 1);
     for (int64_t t = 0; t < S.n_tets; ++t)
         for (int i = 0; i < S.nf; ++i) adj[fill[S.dofs[(size_t)S.nf * t + i]]++] = (int32_t)t;
+}
+
+int build_setup_vars(const anifem_mesh_desc& mesh, int n_var, const anifem_op* vars, const anifem_problem_desc& prob,
+                     AssemblySetup& S, const BlockCaps& caps) {
+    int rc = setup_prepare_vars(mesh, n_var, vars, S);
+    if (!rc) rc = setup_enumerate_host(prob, S);
+    if (rc) return rc;
+    std::vector<int64_t> aptr;
+    std::vector<int32_t> adj;
+    dof_adjacency(S, aptr, adj);
+    return build_blocks(S, aptr, adj, caps);
 }
 
 int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S,

@@ -50,8 +50,16 @@ struct BlockMeta {
 };
 static_assert(sizeof(BlockMeta) == 48, "block metadata is one 48-byte record");
 
+// Variables of a multi-variable problem (at most kMaxVars).
+constexpr int kMaxVars = 4;
+
 struct AssemblySetup {
-    int fem = 0, ncomp = 1, nf = 0, E = 0;
+    int fem = 0, ncomp = 1, nf = 0, E = 0;  // fem / ncomp: those of variable 0
+    // the variables (space, components) in NATURAL order VAR, DIM, ELEM_TYPE,
+    // ELEM_ID, DOF_ID (global_enumerator.h:396): variable v's DOFs follow
+    // variable v-1's; a tet's local DOFs list variable 0's first
+    int n_var = 1;
+    int var_fem[kMaxVars] = {}, var_ncomp[kMaxVars] = {};
     int64_t n_nodes = 0, n_tets = 0, n_edges = 0, n_faces = 0;
     int64_t ndof = 0, nnz = 0;
     std::vector<int32_t> tets;        // n_tets x 4 after the node reorder (caller's order)
@@ -84,6 +92,11 @@ int build_setup(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, A
 // enumeration, Dirichlet flags, CSR pattern and slot map (host or device, with
 // identical results); the DOF -> tet adjacency the row blocks are cut from.
 int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S);
+// setup_prepare for the variables vars[0..n_var) (op codes ignored); the
+// problem's trial and test are not used.
+int setup_prepare_vars(const anifem_mesh_desc& mesh, int n_var, const anifem_op* vars, AssemblySetup& S);
+int build_setup_vars(const anifem_mesh_desc& mesh, int n_var, const anifem_op* vars, const anifem_problem_desc& prob,
+                     AssemblySetup& S, const BlockCaps& caps = BlockCaps());
 int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S);
 int setup_enumerate_device(const anifem_problem_desc& prob, AssemblySetup& S, void* stream);
 void dof_adjacency(const AssemblySetup& S, std::vector<int64_t>& aptr, std::vector<int32_t>& adj);

@@ -293,7 +293,7 @@ int setup_enumerate_device(const anifem_problem_desc& prob, AssemblySetup& S, vo
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int fem = S.fem, nf0 = nf_scalar(fem), nf = S.nf;
     const int64_t NT = S.n_tets, NN = S.n_nodes, E = S.E;
-    if (fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return 1;  // the other spaces: host phase
+    if (S.n_var != 1 || fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return 1;  // the other spaces: host phase
     const bool need_edges = fem >= ANIFEM_FEM_P2 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
     const bool need_faces = fem >= ANIFEM_FEM_P3 || prob.dirichlet == ANIFEM_DIRICHLET_BOUNDARY;
     if (need_faces && NN >= (int64_t(1) << 21)) return 1;

@@ -87,9 +87,8 @@ def test_apply_polynomial_spaces_vs_oracle(op, fem):
 
 
 def test_refused_cases():
-    """GRAD of RT0 / ND0 (elements and evaluation), vector copies of them, the
-    MFMA mode and the global assembly of any space beyond P0..P3 return
-    ANIFEM_ERR_ARG."""
+    """GRAD of RT0 / ND0 (elements and evaluation), vector copies of them
+    (elements and global assembly) and the MFMA mode return ANIFEM_ERR_ARG."""
     import torch
     L = af.lib()
     xyz = dev(T.soa_xyz(np.random.default_rng(1).normal(size=(4, 12))))
@@ -108,6 +107,5 @@ def test_refused_cases():
         af.fem3dapply_batch(af.Op(af.GRAD, af.FEM_RT0), np.full((1, 4), 0.25), xyz, dev(np.zeros((4, 4))))
     coords, tets = af.kuhn_cube(2, 0.1, 1)
     for fem in (af.FEM_CR1, af.FEM_B2, af.FEM_RT0):
-        op = af.Op(af.IDEN if fem == af.FEM_RT0 else af.GRAD, fem)
         with pytest.raises(af.AnifemError):
-            af.Assembler(coords, tets, op)
+            af.Assembler(coords, tets, af.Op(af.IDEN, fem, 3))

@@ -126,3 +126,57 @@ def test_cr1_and_b2_values_are_continuous():
         v = _eval_local(fem, af.IDEN, coords, rt, X)[:, :, 0]
         assert np.abs(v).min() > 1e-3
         assert _spread(v, dofs, A.ndof) < 1e-12
+
+
+def _alone(coords, tets, fem, nc=1, dirichlet=af.DIRICHLET_NONE):
+    A = af.Assembler(coords, tets, af.Op(af.IDEN, fem, nc), host_only=True, dirichlet=dirichlet)
+    return A, A.setup()
+
+
+@pytest.mark.parametrize("vars_", [[(RT0, 1), (P0, 1)], [(af.FEM_P2, 3), (af.FEM_P1, 1)],
+                                   [(ND0, 1), (af.FEM_P1, 1), (B4, 1)]])
+def test_multi_variable_numbering(vars_):
+    """Variables in NATURAL order VAR, DIM, ELEM_TYPE, ELEM_ID, DOF_ID
+    (global_enumerator.h:396): each variable numbered as alone, shifted past
+    the DOFs of the variables before it; a tet's local DOFs concatenate the
+    variables' local DOFs; signs follow each variable's space; the boundary
+    flags are each variable's own; the pattern couples every pair of DOFs
+    that share a tet."""
+    coords, tets = _mesh()
+    for dirichlet in (af.DIRICHLET_NONE, af.DIRICHLET_BOUNDARY):
+        M = af.Assembler(coords, tets, variables=[af.Op(af.IDEN, f, c) for f, c in vars_], host_only=True,
+                         dirichlet=dirichlet)
+        s = M.setup()
+        sg = M.signs()
+        base, lf = 0, 0
+        for f, c in vars_:
+            A, sa = _alone(coords, tets, f, c, dirichlet)
+            nf = A.nf
+            np.testing.assert_array_equal(s["dofs"][:, lf:lf + nf], sa["dofs"] + base)
+            np.testing.assert_array_equal(sg[:, lf:lf + nf], A.signs())
+            np.testing.assert_array_equal(s["dirichlet"][base:base + A.ndof], sa["dirichlet"])
+            base += A.ndof
+            lf += nf
+        assert M.ndof == base and M.nf == lf
+        # pattern: sorted union of the tets' DOFs per row, Dirichlet rows and
+        # columns reduced to the diagonal (assembler.inl:114-136)
+        dofs, isd = s["dofs"], s["dirichlet"].astype(bool)
+        rows = [set() for _ in range(M.ndof)]
+        for t in range(len(dofs)):
+            for r in dofs[t]:
+                rows[r].add(r)
+                if isd[r]:
+                    continue
+                rows[r].update(int(c) for c in dofs[t] if not isd[c])
+        rp, ci = s["row_ptr"], s["col_idx"]
+        for r in range(M.ndof):
+            assert list(ci[rp[r]:rp[r + 1]]) == sorted(rows[r])
+
+
+def test_multi_variable_refusals():
+    coords, tets = _mesh()
+    with pytest.raises(af.AnifemError):  # vector variables are FemVec<n, P1..P3> only
+        af.Assembler(coords, tets, variables=[af.Op(af.IDEN, RT0, 3)], host_only=True)
+    with pytest.raises(af.AnifemError):  # 64 local DOFs at most
+        af.Assembler(coords, tets, variables=[af.Op(af.IDEN, af.FEM_P3, 3), af.Op(af.IDEN, af.FEM_P2, 1)],
+                     host_only=True)
