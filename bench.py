@@ -179,6 +179,22 @@ def timed_events(fn, steps, warmup, torch):
     return e0.elapsed_time(e1) / steps
 
 
+def hbm_roofline(alg_bytes, ms, kernel, per_unit):
+    """Roofline entry of a section: algorithmic bytes of one pass over its
+    measured time against the HBM peak."""
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_pass": int(alg_bytes), "kernel": kernel,
+            "bytes_model": per_unit}
+
+
+def assembly_bytes(n_tets, nf, coef_bytes, nnz, ndof):
+    """Algorithmic bytes of one FP64 assembly pass: per tet 96 B coordinates,
+    the coefficient, 4 nf^2 B slot map and 4 nf B DOF ids; 16 B per stored
+    nonzero and per DOF (read + write of the FP64 value)."""
+    return n_tets * (96 + coef_bytes + 4 * nf * nf + 4 * nf) + 16 * nnz + 16 * ndof
+
+
 def bench_batched(args, torch):
     """256 jittered cubes, side n log-uniform in [4, 40], P1/P2/P3 uniform,
     per-tet kappa = 10^U[-1,1], f = 1, boundary Dirichlet: one graph launch."""
@@ -197,9 +213,16 @@ def bench_batched(args, torch):
     ms = timed_events(lambda: B.run(kappa, fv, val, rhs, status=False), 10, 2, torch)
     tuning = {f"P{f - 1}": {"budget_slots_entries_pairs": [int(x) for x in B.tuning(f)[0]],
                             "candidate_ms": [float(x) for x in B.tuning(f)[1]]} for f in (2, 3, 4)}
+    nf_of = {af.FEM_P1: 4, af.FEM_P2: 10, af.FEM_P3: 20}
+    per_mesh_tets = [6 * int(m) ** 3 for m in sides]
+    alg = sum(assembly_bytes(nt, nf_of[int(fm)], 8, 0, 0) for nt, fm in zip(per_mesh_tets, fems)) \
+        + 16 * B.nnz + 16 * B.ndof
     out = {"meshes": 256, "n_tets": B.n_tets, "ndof": B.ndof, "nnz": B.nnz, "row_blocks": B.n_blocks,
            "ms_per_pass": ms, "element_matrices_per_s": B.n_tets / ms * 1e3, "nnz_per_s": B.nnz / ms * 1e3,
            "setup_s": setup_s, "autotune": tuning,
+           "roofline": hbm_roofline(alg, ms, "geometry_kernel + row_block_persistent (all meshes, one graph)",
+                                    "per tet 96 B xyz + 8 B kappa + 4 nf^2 B slots + 4 nf B dofs; "
+                                    "16 B per nnz and per DOF"),
            "note": "one geometry + one row-block launch for all meshes, replayed from a HIP graph (FP64)"}
     del B, val, rhs
     return out
@@ -234,6 +257,11 @@ def bench_elasticity(args, torch):
         ms = timed_events(lambda: A.assemble(D=D, F=g, precision=prec, val=val, rhs=rhs, status=False), 10, 2,
                           torch)
         res[name] = {"ms_per_pass": ms, "element_matrices_per_s": n / ms * 1e3, "nnz_per_s": A.nnz / ms * 1e3}
+        if name == "fp64":
+            res["roofline"] = hbm_roofline(assembly_bytes(n, 30, 16, A.nnz, A.ndof), ms,
+                                           "geometry_kernel + row_block_persistent (P2^3, Lame)",
+                                           "per tet 96 B xyz + 16 B (lambda, mu) + 4 nf^2 B slots + 4 nf B dofs; "
+                                           "16 B per nnz and per DOF")
     A.assemble(D=D, F=g, precision=af.PREC_FP32, val=val, rhs=rhs, status=False)
     torch.cuda.synchronize()
     res["fp32_max_entry_error_rel_to_max"] = float((val - ref).abs().max() / ref.abs().max())

@@ -22,7 +22,7 @@ extern "C" {
 #define ANIFEM_ERR_NONFINITE (-1)   /* Assemble's -1: NaN/Inf in a local matrix or rhs (assembler.inl:419-424) */
 #define ANIFEM_ERR_CONNECTIVITY (-2) /* Assemble's -2: broken cell connectivity (assembler.inl:309-312) */
 #define ANIFEM_ERR_ARG (-3)          /* invalid argument / unsupported combination */
-#define ANIFEM_ERR_ORDER (-4)        /* quadrature order outside 0..6 (int_tet.inl:99-100 throws) */
+#define ANIFEM_ERR_ORDER (-4)        /* quadrature order outside 0..20 (quadrature_formulas.cpp:1497-1498 throws) */
 #define ANIFEM_ERR_DIMS (-5)         /* NULL/SCALAR tensor with incompatible operator dims (diff_tensor.h:313-315) */
 #define ANIFEM_ERR_HIP (-6)          /* HIP runtime failure */
 #define ANIFEM_ERR_NOMEM (-7)        /* allocation failure */
@@ -123,7 +123,7 @@ enum {
  *           other (the reference's fused layout A[ib + nfB*(ia + nfA*r)]);
  *           double for ANIFEM_PREC_FP64, float for ANIFEM_PREC_FP32 and
  *           ANIFEM_PREC_FP16X3.
- * Errors mirror the reference's throws: ANIFEM_ERR_ORDER (order > 6),
+ * Errors mirror the reference's throws: ANIFEM_ERR_ORDER (order outside 0..20),
  * ANIFEM_ERR_DIMS (NULL/SCALAR with dimA != dimB), ANIFEM_ERR_ARG. */
 int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int64_t n_tets,
                           const double* xyz, int precision, void* A_out, void* stream);
@@ -138,7 +138,8 @@ int anifem_fem3dtet_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, 
  *     unit normal pointing out of the tet; D has 3*dimB rows, row d + 3k
  *     pairing normal component d with output component k
  *     (int_face.h:32-37, int_face.inl:46-82, core.h:176-195).
- * The triangle rule of `order` (0..6) on the face; per-point coefficients are
+ * The triangle rule of `order` (0..20: symmetric rules up to 6, collapsed
+ * Gauss-Jacobi products above) on the face; per-point coefficients are
  * given at the points anifem_face_quadrature returns.  Layouts, errors and
  * precisions as anifem_fem3dtet_batch, FP64 and FP32 only. */
 int anifem_fem3dface_batch(anifem_op opA, anifem_op opB, const anifem_tensor* D, int order, int face, int dot_normal,
@@ -207,7 +208,7 @@ enum { ANIFEM_DIRICHLET_NONE = 0, ANIFEM_DIRICHLET_BOUNDARY = 1, ANIFEM_DIRICHLE
 typedef struct anifem_problem_desc {
     anifem_op trial;        /* OpA: the unknown's operator, e.g. GRAD FemFix<P2> */
     anifem_op test;         /* OpB: must live on the same space as trial */
-    int32_t order;          /* quadrature order (0..6) */
+    int32_t order;          /* quadrature order (0..20) */
     int32_t tensor_type;    /* ANIFEM_TENSOR_* of D */
     int32_t tensor_storage; /* ANIFEM_COEF_* of D */
     int32_t rhs_order;      /* quadrature order of the load vector */

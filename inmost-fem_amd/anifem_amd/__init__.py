@@ -304,11 +304,12 @@ def quadrature_points_batch(order, xyz, out=None, stream=None):
 def face_quadrature(order, face):
     """Triangle rule of `order` on face `face` of the tet: (p (q, 4) barycentric,
     w (q,) summing to 1) -- the points of per-point face coefficients."""
-    p = np.zeros(4 * 12)
-    w = np.zeros(12)
-    q = lib().anifem_face_quadrature(order, face, _ptr(p), _ptr(w))
+    q = lib().anifem_face_quadrature(order, face, None, None)
     if q < 0:
         raise AnifemError(q, "face_quadrature")
+    p = np.zeros(4 * q)
+    w = np.zeros(q)
+    q = lib().anifem_face_quadrature(order, face, _ptr(p), _ptr(w))
     return p[:4 * q].reshape(q, 4), w[:q]
 
 

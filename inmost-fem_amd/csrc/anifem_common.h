@@ -29,6 +29,7 @@ void space_map(int op, int fem, int& map, int& scale);
 // Quadrature rule for `order` (0..kTetMaxOrder; generated table tet_rules.inc
 // up to order 6, conical Gauss-Jacobi products above); returns q or -1.
 constexpr int kTetMaxOrder = 20;
+constexpr int kTriMaxOrder = 20;  // triangle rules (face integrals), same construction above order 6
 int quadrature(int order, const double** p, const double** w);
 
 // Reference-space operator table of a space at the rule's points:

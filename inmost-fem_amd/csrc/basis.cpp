@@ -214,8 +214,41 @@ int quadrature(int order, const double** p, const double** w) {
     return q;
 }
 
+// Triangle rules above the tabulated orders: the collapsed product x = u,
+// y = (1 - u) v with area element (1 - u) du dv, Gauss-Jacobi rules for (1 - u)
+// and 1 with m = order/2 + 1 points each (exact for total degree <= order);
+// barycentric (1 - x - y, x, y), weights summing to 1.  The reference's
+// triangle rules stop at order 20 (quadrature_formulas.cpp:520-521).
+static const ProductRule& triangle_product_rule(int order) {
+    static const std::vector<ProductRule> rules = [] {
+        std::vector<ProductRule> r(kTriMaxOrder + 1);
+        for (int o = ANIFEM_TRI_MAX_ORDER + 1; o <= kTriMaxOrder; ++o) {
+            const int m = o / 2 + 1;
+            std::vector<long double> tu, wu, tv, wv;
+            gauss_jacobi01(m, 1, tu, wu);
+            gauss_jacobi01(m, 0, tv, wv);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) {
+                    const long double x = tu[i], y = (1 - tu[i]) * tv[j];
+                    r[o].p.push_back((double)(1 - x - y));
+                    r[o].p.push_back((double)x);
+                    r[o].p.push_back((double)y);
+                    r[o].w.push_back((double)(2 * wu[i] * wv[j]));
+                }
+        }
+        return r;
+    }();
+    return rules[order];
+}
+
 int triangle_quadrature(int order, const double** p, const double** w) {
-    if (order < 0 || order > ANIFEM_TRI_MAX_ORDER) return -1;
+    if (order < 0 || order > kTriMaxOrder) return -1;
+    if (order > ANIFEM_TRI_MAX_ORDER) {
+        const ProductRule& r = triangle_product_rule(order);
+        *p = r.p.data();
+        *w = r.w.data();
+        return (int)r.w.size();
+    }
     int q = ANIFEM_TRI_NPTS_BY_ORDER[order];
     switch (q) {
         case 1: *p = ANIFEM_TRI_Q1_P, *w = ANIFEM_TRI_Q1_W; break;

@@ -45,9 +45,9 @@ int main() {
     const TensorView null_const{TENSOR_NULL, ANIFEM_COEF_CONST, nullptr};
     CHECK((fem3Dtet<GP2, Operator<IDEN, FemFix<FEM_P2>>, DfuncTraits<TENSOR_NULL, true>>(buf, 1, null_const, buf, 2) ==
            ANIFEM_ERR_DIMS));
-    // face integrals: the triangle rules stop at order 6; faces are 0..3
+    // face integrals: the triangle rules stop at order 20; faces are 0..3
     using IP2 = Operator<IDEN, FemFix<FEM_P2>>;
-    CHECK((fem3Dface<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 0, null_const, buf, 7) == ANIFEM_ERR_ORDER));
+    CHECK((fem3Dface<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 0, null_const, buf, 21) == ANIFEM_ERR_ORDER));
     CHECK((fem3Dface<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 4, null_const, buf, 2) == ANIFEM_ERR_ARG));
     // the normal form of an identity needs 3 dimB == dimA (GRAD against IDEN: 3 == 3)
     CHECK((fem3DfaceN<IP2, IP2, DfuncTraits<TENSOR_NULL, true>>(buf, 1, 0, null_const, buf, 2) == ANIFEM_ERR_DIMS));

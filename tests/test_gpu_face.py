@@ -76,6 +76,26 @@ def test_face_symmetric_tensor_and_const_storage_vs_oracle(face):
     assert T.rel_frob(A, Ao) <= FP64_TOL
 
 
+@pytest.mark.parametrize("order", [7, 13, 20])
+def test_face_high_order_rules_vs_oracle(order):
+    """Triangle rules above order 6 (collapsed Gauss-Jacobi products): P3 face
+    mass (degree 6) and P2 stiffness with a per-tet tensor against the oracle,
+    whose own face rule integrates these polynomials exactly."""
+    rng = np.random.default_rng(order)
+    xyz = rng.normal(size=(30, 12))
+    n = len(xyz)
+    m3 = af.Op(af.IDEN, af.FEM_P3)
+    for face in range(4):
+        A = run_face(m3, m3, af.TENSOR_NULL, af.COEF_CONST, None, order, face, xyz, False)
+        Ao = T.orc_face(1, 4, 1, 1, 4, 1, af.TENSOR_NULL, 0, face, 0, xyz, None)
+        assert T.rel_frob(A, Ao) <= FP64_TOL, (order, face)
+    kap = 10 ** rng.uniform(-1, 1, (n, 1))
+    g2 = af.Op(af.GRAD, af.FEM_P2)
+    A = run_face(g2, g2, af.TENSOR_SCALAR, af.COEF_PER_TET, dev(kap.T.copy()), order, 2, xyz, False)
+    Ao = T.orc_face(2, 3, 1, 2, 3, 1, af.TENSOR_SCALAR, 1, 2, 0, xyz, kap)
+    assert T.rel_frob(A, Ao) <= FP64_TOL
+
+
 def test_face_errors():
     import torch
     op = af.Op(af.IDEN, af.FEM_P1)
@@ -89,7 +109,7 @@ def test_face_errors():
     assert call(0, 0, 2, af.PREC_FP64) == 0
     assert call(4, 0, 2, af.PREC_FP64) == af.ERR_ARG
     assert call(-1, 0, 2, af.PREC_FP64) == af.ERR_ARG
-    assert call(0, 0, 7, af.PREC_FP64) == af.ERR_ORDER
+    assert call(0, 0, 21, af.PREC_FP64) == af.ERR_ORDER
     assert call(0, 0, 2, af.PREC_FP16X3) == af.ERR_ARG
     # the normal form of a NULL tensor needs 3 dimB == dimA
     assert call(0, 1, 2, af.PREC_FP64) == af.ERR_DIMS

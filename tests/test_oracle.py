@@ -312,7 +312,7 @@ def test_face_rule_exactness_and_reference_rules():
     monomials to the same values."""
     import anifem_amd as af
     R = T.ref_lib()
-    for order, deg in [(1, 1), (2, 2), (3, 3), (4, 4), (5, 5), (6, 6)]:
+    for order, deg in [(o, max(o, 1)) for o in range(0, 21)]:
         for face in range(4):
             p, w = af.face_quadrature(order, face)
             assert abs(w.sum() - 1) < 1e-15
