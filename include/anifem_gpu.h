@@ -369,8 +369,10 @@ int anifem_assembly_run_noalloc(anifem_assembly* ctx, const anifem_tensor* D, co
 
 /* Row-block kernel choice (default 1): a persistent pass (each workgroup
  * walks many blocks and prefetches the next one's data) where it applies
- * (scalar spaces, tensor constant over each tet); 0 forces one workgroup per
- * block.  Both sum in the same order: the results are bitwise identical. */
+ * (scalar spaces, tensor constant over each tet), in its fixed-size form for
+ * P1 / P2 / P3; 2 takes the persistent pass in its general form (any element
+ * size); 0 forces one workgroup per block.  All sum in the same order: the
+ * results are bitwise identical.  No reference counterpart. */
 int anifem_assembly_set_persistent(anifem_assembly* ctx, int on);
 
 /* Scatter method of anifem_assembly_run (no reference counterpart: the

@@ -637,7 +637,10 @@ class Assembler:
         return out
 
     def set_persistent(self, on=True):
-        _check(lib().anifem_assembly_set_persistent(self._h, 1 if on else 0), "set_persistent")
+        """True / 1: persistent pass (fixed-size form for P1..P3); 2: its general
+        form; False / 0: one workgroup per block."""
+        mode = on if isinstance(on, int) and not isinstance(on, bool) else (1 if on else 0)
+        _check(lib().anifem_assembly_set_persistent(self._h, mode), "set_persistent")
 
     def set_scatter(self, method):
         """SCATTER_ROW_BLOCKS (default) or SCATTER_COLOURED."""

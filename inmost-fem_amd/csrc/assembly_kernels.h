@@ -178,7 +178,7 @@ __host__ __device__ inline BlockLds block_lds_layout(int max_values, int max_pai
     BlockLds L;
     size_t o = 0;
     L.vals = o;
-    o = align16(o + 8 * (size_t)max_values);
+    o = align16(o + 8 * ((size_t)max_values + 1));  // + one junk slot for dropped entries
     L.fvals = o;
     o = align16(o + 8 * (size_t)max_pairs);
     L.R = o;
@@ -602,7 +602,14 @@ __device__ __forceinline__ int slot_words(int64_t s0, int ns) { return (int)(((s
 // bitwise identical.
 // P8: the value positions come from the compact list (a 16-bit base and one
 // byte per entry, P8 words per pair) instead of 16-bit positions.
-template <int DA0, int DB0, int PW, typename T, bool P8 = false>
+// NFX > 0: every block is of one element kind with NFX local DOFs (a scalar
+// space, ib < NFX): the reference-tensor rows of a pair sit at compile-time
+// offsets, and phase A runs without per-entry branches: a dropped entry is
+// stored to a junk slot after the value list, a Dirichlet pair contracts
+// K = 0 onto the value 1, and non-finite entries are counted entry by entry
+// only for a pair whose K is not finite or beyond 1e300 (below that no entry
+// can overflow).  Same values and sums as NFX = 0.
+template <int DA0, int DB0, int PW, typename T, bool P8 = false, int NFX = 0>
 // Five waves per SIMD (the LDS of five 30 KB workgroups fills a CU): at most
 // 96 VGPRs, measured faster than four unconstrained waves and than spilling
 // to six or eight.
@@ -687,12 +694,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     BlockMeta M = a.meta[b];
     int staged_kind = (int)M.kind;
     stage(staged_kind);
-    Regs cur, nxt;
+    Regs cur, nxt, nn;
     load_records(M, cur);
     load_k(M, cur);
     int64_t bn = W.take();
     BlockMeta Mn{};
     if (bn >= 0) Mn = a.meta[bn];
+    // fixed-size form: records two blocks ahead (a block's geometry rows are
+    // then requested when the block before it starts, not after its phase A)
+    int64_t bn2 = -1;
+    BlockMeta Mn2{};
+    if constexpr (NFX > 0) {
+        load_records(Mn, nxt);
+        bn2 = W.take();
+        if (bn2 >= 0) Mn2 = a.meta[bn2];
+    }
     __syncthreads();  // reference tensors in LDS
 
     int nonfinite = 0;
@@ -722,13 +738,70 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         }
         if (!sst_copied) block_copy(sst, a.sstart + M.s0, (int)M.ns);
         if (a.has_rhs && (int)threadIdx.x < M.nr) pstl[threadIdx.x] = cur.pst;
-        load_records(Mn, nxt);  // past the last block: Mn has no pairs, nothing is read
-        const int64_t bnn = W.take();
+        int64_t bnn = -1;
         BlockMeta Mnn{};
-        if (bnn >= 0) Mnn = a.meta[bnn];
+        if constexpr (NFX > 0) {
+            load_k(Mn, nxt);        // records of b+1 arrived during the previous block
+            load_records(Mn2, nn);  // past the last block: no pairs, nothing is read
+            bnn = W.take();         // b+3
+            if (bnn >= 0) Mnn = a.meta[bnn];
+        } else {
+            load_records(Mn, nxt);  // past the last block: Mn has no pairs, nothing is read
+            bnn = W.take();
+            if (bnn >= 0) Mnn = a.meta[bnn];
+        }
 
         // phase A of b
-        if ((int)threadIdx.x < M.np) {
+        if (NFX > 0 && (int)threadIdx.x < M.np) {  // uniform over NFX
+            static_assert(NFX <= 2 * PW, "NFX entries per pair");
+            const int ib = cur.ibf & 63;
+            const bool dir = (cur.ibf & kPairDirichlet) != 0;
+            const T* Rb = R + ib * lenp;
+            const int base8 = (int)(cur.posw[0] & 0xFFFFu);
+            const int junk = a.max_values;
+            // finite K below 1e300 (R is finite and O(1)): no entry overflows
+            const T big = sizeof(T) == 8 ? T(1e300) : T(1e36);
+            bool fast = true;
+#pragma unroll
+            for (int j = 0; j < KB; ++j) fast = fast && fabs(cur.k[j]) <= big;
+            fast = fast && (!a.has_rhs || fabs(cur.kf) <= big);
+            // applyDir: a Dirichlet pair's kept entry (its diagonal) is 1
+            const T bias = dir ? T(1) : T(0);
+#pragma unroll
+            for (int j = 0; j < KB; ++j) cur.k[j] = dir ? T(0) : cur.k[j];
+            // value-list position of entry ia0 (the junk slot for a dropped one)
+            auto pos = [&](int ia0) {
+                if constexpr (P8) {
+                    const uint32_t o = (cur.posw[(2 + ia0) >> 2] >> (((2 + ia0) & 3) * 8)) & 0xFFu;
+                    return o == 0xFFu ? junk : base8 + (int)o;
+                } else {
+                    const uint32_t w = cur.posw[ia0 >> 1];
+                    const int p16 = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    return p16 == (int)kPosDropped ? junk : p16;
+                }
+            };
+#pragma unroll
+            for (int ia0 = 0; ia0 < NFX; ++ia0) {
+                const T* Rr = Rb + NFX * ia0 * lenp;
+                T acc = bias;
+#pragma unroll
+                for (int j = 0; j < KB; ++j) acc += cur.k[j] * Rr[j];
+                vals[pos(ia0)] = (double)acc;
+            }
+            if (!fast && !dir) {  // count the kept entries that are not finite
+#pragma unroll
+                for (int ia0 = 0; ia0 < NFX; ++ia0) {
+                    const int ps = pos(ia0);
+                    if (ps != junk) nonfinite += isfinite(vals[ps]) ? 0 : 1;
+                }
+            }
+            if (a.has_rhs) {
+                const T acc = cur.kf * RF[ib * lenFp];
+                const double fv = dir ? 0.0 : (double)acc;
+                if (!fast) nonfinite += isfinite(fv) ? 0 : 1;
+                fvals[threadIdx.x] = fv;
+            }
+        } else if (NFX == 0 && (int)threadIdx.x < M.np) {
             const int ib = cur.ibf & 63;
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + (ib % nfB0) * lenp;
@@ -762,8 +835,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
             }
         }
         // geometry rows of the next block (its records have arrived by now)
-        __builtin_amdgcn_sched_barrier(0);
-        load_k(Mn, nxt);
+        if constexpr (NFX == 0) {
+            __builtin_amdgcn_sched_barrier(0);
+            load_k(Mn, nxt);
+        }
         __syncthreads();
 
         // phase B of b
@@ -796,9 +871,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         if (bn < 0) break;
         cur = nxt;
         M = Mn;
-        Mn = Mnn;
         b = bn;
-        bn = bnn;
+        if constexpr (NFX > 0) {
+            nxt.tet = nn.tet;
+            nxt.ibf = nn.ibf;
+#pragma unroll
+            for (int w = 0; w < NPW; ++w) nxt.posw[w] = nn.posw[w];
+            Mn = Mn2;
+            bn = bn2;
+            Mn2 = Mnn;
+            bn2 = bnn;
+        } else {
+            Mn = Mnn;
+            bn = bnn;
+        }
     }
     if (nonfinite) atomicAdd(&a.status[0], nonfinite);
 }

@@ -210,6 +210,31 @@ def test_nonfinite_status():
     assert st.code == af.ERR_NONFINITE and st.n_nonfinite > 0
 
 
+@pytest.mark.parametrize("fem", [af.FEM_P1, af.FEM_P2, af.FEM_P3])
+def test_nonfinite_counts_equal_across_row_block_kernels(fem):
+    """NaN, Inf and overflowing (1e306) coefficients on a few tets, with
+    boundary Dirichlet rows: the fixed-size persistent pass (which counts
+    entry by entry only for pairs whose K is not finite or beyond 1e300), the
+    general one and the per-block pass count the same entries."""
+    coords, tets = af.kuhn_cube(4, 0.2, 2)
+    n = len(tets)
+    kappa = np.ones((1, n))
+    kappa[0, [3, 50]] = np.nan
+    kappa[0, 100] = np.inf
+    kappa[0, [7, 200]] = 1e306
+    kappa[0, 201] = 1e299
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem), tensor_type=af.TENSOR_SCALAR,
+                       tensor_storage=af.COEF_PER_TET, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_PER_TET, dirichlet=af.DIRICHLET_BOUNDARY)
+    counts = []
+    for mode in (1, 2, 0):
+        Asm.set_persistent(mode)
+        _, _, st = Asm.assemble(D=dev(kappa), F=dev(kappa))
+        counts.append(st.n_nonfinite)
+        assert st.code == af.ERR_NONFINITE
+    assert counts[0] == counts[1] == counts[2] and counts[0] > 0, counts
+
+
 @pytest.mark.parametrize("fem", [2, 3])
 def test_large_assembly_invariants(fem):
     """At 32^3 (196,608 tets, tens of thousands of row blocks): without
@@ -275,13 +300,16 @@ def test_persistent_and_per_block_kernels_bitwise_equal(fem, dirichlet, n):
     Dp = dev(T.pack_sym(D, 3).T.copy())
     f = dev(rng.uniform(0.5, 2.0, (1, len(tets))))
     for prec in (af.PREC_FP64, af.PREC_FP32):
-        Asm.set_persistent(True)
+        Asm.set_persistent(True)  # fixed-size form
         v1, r1, st1 = Asm.assemble(D=Dp, F=f, precision=prec)
         Asm.set_persistent(False)
         v2, r2, st2 = Asm.assemble(D=Dp, F=f, precision=prec)
-        assert st1.code == 0 and st2.code == 0
-        assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
-        assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
+        Asm.set_persistent(2)  # general form
+        v3, r3, st3 = Asm.assemble(D=Dp, F=f, precision=prec)
+        assert st1.code == 0 and st2.code == 0 and st3.code == 0
+        for v, r in ((v2, r2), (v3, r3)):
+            assert np.array_equal(v1.cpu().numpy(), v.cpu().numpy())
+            assert np.array_equal(r1.cpu().numpy(), r.cpu().numpy())
 
 
 @pytest.mark.parametrize("fem,dirichlet,n,rhs_storage", [(af.FEM_P2, 1, 12, af.COEF_PER_TET),
