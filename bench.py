@@ -339,8 +339,8 @@ def read_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=64, help="sub-cubes per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extras", action="store_true", help="only the headline assembly (no --sections)")
@@ -454,7 +454,7 @@ def main():
         "setup_s": setup_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double,true> + row_block_persistent<6,1,5,double,true>",
+                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double,true> + row_block_persistent<6,1,5,double,true,10>",
                      "ms_per_launch": ms_pass_events,
                      "algorithmic_bytes_per_pass": alg_bytes,
                      "traffic_source": traffic_src},
