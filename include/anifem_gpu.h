@@ -228,7 +228,11 @@ typedef struct anifem_status {
 /* Builds on the host, then uploads: node reorder (ordering.inl:8-26), NATURAL
  * DOF enumeration with P3 edge orientation (global_enumerator.cpp:562-777,
  * ordering.inl:137-151, tetdofmap.inl:98-127), CSR pattern with the diagonal
- * and Dirichlet drop rules (assembler.inl:114-136,416), the element-to-CSR-slot
+ * and Dirichlet drop rules (assembler.inl:114-136,416) -- for a vector space
+ * with component-wise operators (IDEN, GRAD, DUDX..DUDZ) and a NULL or SCALAR
+ * tensor also without the entries between components, which are zero in every
+ * cell and which the reference's |a| > 1e-100 rule therefore never inserts --
+ * the element-to-CSR-slot
  * map and the row blocks of the owner-computes assembly.  `stream` orders the
  * uploads. */
 int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
@@ -402,7 +406,12 @@ int anifem_assembly_colours(const anifem_assembly* ctx, int32_t* n_colours, int3
  * reference hands the reordered nodes to the local function), in the cell's
  * local basis: on oriented DOFs (RT0 faces, ND0 edges) the scatter multiplies
  * entry (ib, ia) by s_ib s_ia and load entry ib by s_ib, the signs of
- * anifem_assembly_copy_signs (assembler.inl:154-164).  elem_precision:
+ * anifem_assembly_copy_signs (assembler.inl:154-164).  The values go into the
+ * context's pattern: for a vector space whose problem has component-wise
+ * operators and a NULL or SCALAR tensor, the pattern leaves out the entries
+ * between components (zero for such a problem; the reference's value rule,
+ * assembler.inl:416, never inserts them), so matrices that couple components
+ * need a context whose tensor couples them.  elem_precision:
  * ANIFEM_PREC_FP64 (double) or ANIFEM_PREC_FP32 (float) element values; the
  * CSR and rhs accumulate in FP64, with element-level homogeneous Dirichlet and
  * the same deterministic order as anifem_assembly_run. */
@@ -535,7 +544,8 @@ int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_t* col_idx,
 
 /* ---- learned preconditioner ----------------------------------------------
  * No reference counterpart.  A 3-layer message-passing network (32 hidden
- * units, BF16 weights, FP32 accumulation) reads an assembled SPD matrix A and
+ * units; FP32 arithmetic on VALU, with the weights rounded to BF16 precision
+ * and held as FP32) reads an assembled SPD matrix A and
  * writes a lower-triangular factor G on A's pattern; M^-1 = G G^T.  With the
  * output heads at zero the network is exactly Jacobi (G = diag(a_ii)^-1/2).
  * Training minimises (1/(P n)) sum_p ||G^T A G z_p - z_p||^2 over P = 4

@@ -369,7 +369,8 @@ int launch_blocks_lift(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s)
     a.rowlen = pad16<T>(NCA * NCB * len + (a.has_rhs ? NCB * lenF : 0));  // as launch_blocks sets it
     const int64_t blocks = (ctx->n_lift + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((dirichlet_lift_kernel<KDA, KDB, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
-                       static_cast<const T*>(ctx->run_kg), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->run_bc);
+                       static_cast<const T*>(ctx->run_kg), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->d_isdir,
+                       ctx->run_bc);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -640,14 +641,18 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
                 const int64_t pb = S.blk_p0[b] + S.pstart[r];
                 const int64_t pe = r + 1 < r1 ? S.blk_p0[b] + S.pstart[r + 1] : S.blk_p0[b + 1];
                 bool lifted = S.is_dir[r] != 0;
-                for (int64_t p = pb; p < pe && !lifted; ++p)
-                    for (int ia = 0; ia < nf && !lifted; ++ia) lifted = S.pair_pos[(size_t)p * nf + ia] == kPosDropped;
+                for (int64_t p = pb; p < pe && !lifted; ++p) {
+                    const int32_t* td = &S.dofs[(size_t)S.pair_tet[p] * nf];
+                    for (int ia = 0; ia < nf && !lifted; ++ia)
+                        lifted = S.pair_pos[(size_t)p * nf + ia] == kPosDropped && S.is_dir[td[ia]];
+                }
                 if (lifted) lift.push_back(LiftRow{pb, r, (int32_t)(pe - pb)});
             }
         }
         ctx->n_lift = (int64_t)lift.size();
         if ((rc = upload(&ctx->d_lift, lift.data(), lift.size(), s))) return rc;
         if (ctx->n_lift && (rc = upload(&ctx->d_dofs, S.dofs.data(), S.dofs.size(), s))) return rc;
+        if (ctx->n_lift && (rc = upload(&ctx->d_isdir, S.is_dir.data(), S.is_dir.size(), s))) return rc;
     }
     ANIFEM_HIP_CHECK(hipStreamSynchronize(s));
     // the device holds the pair lists; the host keeps the rest for inspection

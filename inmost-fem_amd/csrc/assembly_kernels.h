@@ -400,6 +400,7 @@ template <int DA0, int DB0, int NCA, int NCB, typename T>
 __global__ __launch_bounds__(kBlock) void dirichlet_lift_kernel(BlockArgs a, const T* __restrict__ Kg,
                                                                 const LiftRow* __restrict__ lift, int64_t n_lift,
                                                                 const int32_t* __restrict__ dofs,
+                                                                const uint8_t* __restrict__ is_dir,
                                                                 const double* __restrict__ bc) {
     constexpr int KB = DA0 * DB0;
     const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
@@ -428,7 +429,9 @@ __global__ __launch_bounds__(kBlock) void dirichlet_lift_kernel(BlockArgs a, con
             fv = (double)f0;
             const uint16_t* pos = a.pair_pos + gp * nf;
             for (int ia = 0; ia < nf; ++ia) {
-                if (pos[ia] != kPosDropped) continue;
+                // dropped columns of Dirichlet DOFs (a decoupled pattern also
+                // drops zero entries between components)
+                if (pos[ia] != kPosDropped || !is_dir[dofs[tet * nf + ia]]) continue;
                 const int ca = ia / nfA0, ia0 = ia % nfA0;
                 const T* Kb = Kt + (cb * NCA + ca) * len;
                 const T* Rr = R + (ib0 + nfB0 * ia0) * len;

@@ -167,6 +167,9 @@ int reorder_nodes(const anifem_mesh_desc& mesh, AssemblySetup& S);
 int setup_prepare(const anifem_mesh_desc& mesh, const anifem_problem_desc& prob, AssemblySetup& S) {
     if (prob.test.fem != prob.trial.fem || prob.test.ncomp != prob.trial.ncomp) return ANIFEM_ERR_ARG;
     int rc = set_vars(1, &prob.trial, S);
+    auto componentwise = [](int op) { return op == ANIFEM_IDEN || op == ANIFEM_GRAD || (op >= ANIFEM_DUDX && op <= ANIFEM_DUDZ); };
+    S.decoupled = prob.trial.ncomp > 1 && componentwise(prob.trial.op) && componentwise(prob.test.op) &&
+                  (prob.tensor_type == ANIFEM_TENSOR_NULL || prob.tensor_type == ANIFEM_TENSOR_SCALAR);
     return rc ? rc : reorder_nodes(mesh, S);
 }
 
@@ -369,9 +372,11 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
     std::vector<int32_t> adj;
     dof_adjacency(S, aptr, adj);
 
-    // CSR pattern: per row, sort + unique of the DOFs of its tets
+    // CSR pattern: per row, sort + unique of the DOFs of its tets (of its own
+    // component when the components are decoupled)
     S.row_ptr.assign(S.ndof + 1, 0);
     const int nf = S.nf;
+    const int64_t nscal_dec = S.decoupled ? S.ndof / S.ncomp : 0;
     auto row_cols = [&](int64_t d, std::vector<int32_t>& buf) {
         buf.clear();
         buf.push_back((int32_t)d);
@@ -379,7 +384,8 @@ int setup_enumerate_host(const anifem_problem_desc& prob, AssemblySetup& S) {
         for (int64_t p = aptr[d]; p < aptr[d + 1]; ++p) {
             const int32_t* td = &S.dofs[(size_t)nf * adj[p]];
             for (int j = 0; j < nf; ++j)
-                if (!(rd || S.is_dir[td[j]])) buf.push_back(td[j]);
+                if (!(rd || S.is_dir[td[j]]) && (nscal_dec == 0 || td[j] / nscal_dec == d / nscal_dec))
+                    buf.push_back(td[j]);
         }
         std::sort(buf.begin(), buf.end());
         buf.erase(std::unique(buf.begin(), buf.end()), buf.end());

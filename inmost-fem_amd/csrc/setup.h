@@ -60,6 +60,12 @@ struct AssemblySetup {
     // variable v-1's; a tet's local DOFs list variable 0's first
     int n_var = 1;
     int var_fem[kMaxVars] = {}, var_ncomp[kMaxVars] = {};
+    // a vector space whose operator pair acts component by component (IDEN,
+    // GRAD, DUDX..DUDZ on both sides) with a NULL or SCALAR tensor: the
+    // entries coupling different components are exactly zero in every cell,
+    // so the reference's value rule (|a| > 1e-100, assembler.inl:416) never
+    // inserts them and the pattern leaves them out
+    bool decoupled = false;
     int64_t n_nodes = 0, n_tets = 0, n_edges = 0, n_faces = 0;
     int64_t ndof = 0, nnz = 0;
     std::vector<int32_t> tets;        // n_tets x 4 after the node reorder (caller's order)

@@ -192,8 +192,10 @@ __global__ void mask_kernel(int64_t ndof, const uint8_t* __restrict__ mask, uint
 
 // element entries (ib + nf*ia of tet t at i = t*E + ib + nf*ia) and one
 // diagonal per row; dropped entries get the key ndof^2 (sorted last)
+// (nscal_dec > 0: decoupled components, an entry between components
+// r / nscal_dec != c / nscal_dec is dropped as well)
 __global__ void entry_keys_kernel(const int32_t* __restrict__ dofs, const uint8_t* __restrict__ is_dir, int64_t m_el,
-                                  int nf, int64_t ndof, uint64_t* keys, int32_t* vals) {
+                                  int nf, int64_t ndof, int64_t nscal_dec, uint64_t* keys, int32_t* vals) {
     const int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x;
     const int64_t E = (int64_t)nf * nf;
     if (i < m_el) {
@@ -201,7 +203,7 @@ __global__ void entry_keys_kernel(const int32_t* __restrict__ dofs, const uint8_
         const int e = (int)(i % E);
         const int ib = e % nf, ia = e / nf;
         const int64_t r = dofs[nf * t + ib], c = dofs[nf * t + ia];
-        const bool keep = r == c || !(is_dir[r] || is_dir[c]);
+        const bool keep = r == c || (!(is_dir[r] || is_dir[c]) && (nscal_dec == 0 || r / nscal_dec == c / nscal_dec));
         keys[i] = keep ? (uint64_t)r * (uint64_t)ndof + (uint64_t)c : (uint64_t)ndof * (uint64_t)ndof;
         vals[i] = (int32_t)i;
     } else if (i < m_el + ndof) {
@@ -378,7 +380,9 @@ int setup_enumerate_device(const anifem_problem_desc& prob, AssemblySetup& S, vo
     int64_t* row_ptr;
     ANIFEM_HIP_CHECK(P.get(&keys, m));
     ANIFEM_HIP_CHECK(P.get(&vals, m));
-    hipLaunchKernelGGL(entry_keys_kernel, dim3(grid_of(m)), dim3(kT), 0, s, dofs, is_dir, m_el, nf, ndof, keys, vals);
+    const int64_t nscal_dec = S.decoupled ? ndof / S.ncomp : 0;
+    hipLaunchKernelGGL(entry_keys_kernel, dim3(grid_of(m)), dim3(kT), 0, s, dofs, is_dir, m_el, nf, ndof, nscal_dec,
+                       keys, vals);
     ANIFEM_HIP_CHECK(P.get(&k2, m));
     ANIFEM_HIP_CHECK(P.get(&v2, m));
     const uint64_t dropped = (uint64_t)ndof * (uint64_t)ndof;

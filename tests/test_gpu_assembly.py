@@ -349,7 +349,9 @@ def test_assemble_elements_matches_oracle_scatter(fem, nc):
     applyDir + sequential cell-order scatter: equal bit for bit."""
     rng = np.random.default_rng(fem * 7 + nc)
     coords, tets = af.kuhn_cube(3, 0.2, 7)
-    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY)
+    # random element matrices couple the components: a Lame-tensor context keeps every structural entry
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), tensor_type=af.TENSOR_LAME if nc == 3 else af.TENSOR_NULL,
+                       dirichlet=af.DIRICHLET_BOUNDARY)
     n, nf = Asm.n_tets, Asm.nf
     setup = Asm.setup()
     A = rng.normal(size=(n, nf * nf))
@@ -382,7 +384,9 @@ def test_assemble_elements_symmetric_reads_columns(fem, nc):
     import torch
     rng = np.random.default_rng(nc)
     coords, tets = af.kuhn_cube(3, 0.2, 9)
-    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY)
+    # random element matrices couple the components: a Lame-tensor context keeps every structural entry
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), tensor_type=af.TENSOR_LAME if nc == 3 else af.TENSOR_NULL,
+                       dirichlet=af.DIRICHLET_BOUNDARY)
     n, nf = Asm.n_tets, Asm.nf
     B = rng.normal(size=(n, nf, nf))
     for dt in (torch.float64, torch.float32):
@@ -406,7 +410,9 @@ def test_assemble_elements_persistent_and_per_block_bitwise_equal(fem, nc, n):
     import torch
     rng = np.random.default_rng(fem * 10 + nc + n)
     coords, tets = af.kuhn_cube(n, 0.2, 11)
-    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY)
+    # random element matrices couple the components: a Lame-tensor context keeps every structural entry
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), tensor_type=af.TENSOR_LAME if nc == 3 else af.TENSOR_NULL,
+                       dirichlet=af.DIRICHLET_BOUNDARY)
     nt, nf = Asm.n_tets, Asm.nf
     A = rng.normal(size=(nt, nf, nf))
     F = rng.normal(size=(nt, nf))

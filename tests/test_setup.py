@@ -27,7 +27,10 @@ def test_setup_matches_oracle(fem, ncomp, dirichlet, mesh):
         coords, tets = af.kuhn_cube(3, 0.2, 5)
     else:
         coords, tets = _shuffled_mesh(3, 7)
-    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), dirichlet=dirichlet, host_only=True)
+    # a vector space with the Lame tensor couples its components (every
+    # structural entry is kept, as the oracle's pattern)
+    tt = af.TENSOR_LAME if ncomp == 3 else af.TENSOR_NULL
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), tensor_type=tt, dirichlet=dirichlet, host_only=True)
     s = A.setup()
     mask = s["dirichlet"] if dirichlet else None
     O = T.oracle_assembly(coords, tets, fem, ncomp, dirichlet_mask=mask)
@@ -36,6 +39,60 @@ def test_setup_matches_oracle(fem, ncomp, dirichlet, mesh):
     np.testing.assert_array_equal(s["row_ptr"], O["row_ptr"])
     np.testing.assert_array_equal(s["col_idx"], O["col_idx"])
     np.testing.assert_array_equal(s["slot"], O["slot"])
+
+
+@pytest.mark.parametrize("op,tt", [(af.GRAD, af.TENSOR_SCALAR), (af.IDEN, af.TENSOR_NULL), (af.GRAD, af.TENSOR_LAME)])
+@pytest.mark.parametrize("fem", [2, 3])
+def test_vector_pattern_follows_the_value_rule(op, tt, fem):
+    """The reference inserts an element entry only when |a| > 1e-100
+    (assembler.inl:416-417) and always the diagonal (:114-136).  For a vector
+    space with a NULL or SCALAR tensor and component-wise operators the
+    entries between components are exactly zero in every cell, so the pattern
+    must leave them out; the Lame tensor couples them.  The expected pattern is
+    built from the oracle's element matrices of every cell by that rule (with
+    the Dirichlet rows and columns removed as applyDir zeroes them).  Entries
+    that vanish by a coincidence of the geometry (DUDX of a P2 basis function
+    on a right-angled cell, say) stay in this build's pattern: it is fixed at
+    set-up, before any value exists."""
+    coords, tets = af.kuhn_cube(2, 0.2, 3)
+    n = len(tets)
+    A = af.Assembler(coords, tets, af.Op(op, fem, 3), tensor_type=tt, tensor_storage=af.COEF_PER_TET,
+                     dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
+    s = A.setup()
+    rt = A.tets()
+    xyz = coords[rt].reshape(n, 12)
+    rng = np.random.default_rng(fem)
+    D = (rng.uniform(0.5, 2.0, (n, 1)) if tt == af.TENSOR_SCALAR else
+         np.stack([rng.uniform(1, 2, n), rng.uniform(0.5, 1, n)], axis=1) if tt == af.TENSOR_LAME else None)
+    if tt == af.TENSOR_LAME:  # the oracle takes the full 9 x 9 tensor
+        Dfull = np.zeros((n, 81))
+        for r in range(n):
+            lam, mu = D[r]
+            C = np.zeros((3, 3, 3, 3))
+            for i in range(3):
+                for j in range(3):
+                    for k in range(3):
+                        for l in range(3):
+                            C[i, j, k, l] = lam * (i == k) * (j == l) + mu * ((i == j) * (k == l) + (i == l) * (j == k))
+            Dfull[r] = C.reshape(9, 9).ravel(order="F")
+        Ae = T.orc_elem(op, fem, 3, op, fem, 3, af.TENSOR_GENERAL, 1, 2, xyz, Dfull)
+    else:
+        Ae = T.orc_elem(op, fem, 3, op, fem, 3, tt, 1 if D is not None else 0, 2, xyz, D)
+    nf = A.nf
+    dofs, isd = s["dofs"], s["dirichlet"].astype(bool)
+    rows = [{r} for r in range(A.ndof)]
+    for t in range(n):
+        M = Ae[t].reshape(nf, nf, order="F")  # M[ib, ia]
+        for ib in range(nf):
+            for ia in range(nf):
+                r, c = dofs[t, ib], dofs[t, ia]
+                if abs(M[ib, ia]) > 1e-100 and not isd[r] and not isd[c]:
+                    rows[r].add(int(c))
+    rp, ci = s["row_ptr"], s["col_idx"]
+    for r in range(A.ndof):
+        assert list(ci[rp[r]:rp[r + 1]]) == sorted(rows[r]), r
+    coupled = sum(len(x) for x in rows)
+    assert A.nnz == coupled
 
 
 def test_boundary_dirichlet_dofs_are_on_boundary():
@@ -157,7 +214,9 @@ def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
     once, in ascending tet order: replaying them gives the oracle's scatter bit
     for bit (random element matrices with element-level Dirichlet)."""
     coords, tets = _shuffled_mesh(3, 11) if fem != 3 else af.kuhn_cube(4, 0.2, 2)
-    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), dirichlet=af.DIRICHLET_BOUNDARY, host_only=True)
+    tt = af.TENSOR_LAME if ncomp == 3 else af.TENSOR_NULL  # components coupled: every structural entry kept
+    A = af.Assembler(coords, tets, af.Op(af.GRAD, fem, ncomp), tensor_type=tt, dirichlet=af.DIRICHLET_BOUNDARY,
+                     host_only=True)
     s = A.setup()
     B = A.blocks()
     nf, E = A.nf, A.nf * A.nf
