@@ -614,7 +614,7 @@ class Assembler:
         xyl = np.ascontiguousarray(xyl, dtype=np.float64)
         q = len(xyl)
         if self.trial.fem in (FEM_RT0, FEM_ND0):
-            dim = 1 if op == DIV else 3  # vector-valued spaces: IDEN and CURL 3, DIV 1
+            dim = 1 if op == DIV else (9 if op == GRAD else 3)  # vector-valued spaces: IDEN, CURL 3, DIV 1, GRAD 9
         else:
             dim = (3 if op == GRAD else 1) * self.trial.ncomp
         if out is None:

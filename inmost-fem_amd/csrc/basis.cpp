@@ -50,6 +50,9 @@ int ref_dim(int op, int fem) {
             case ANIFEM_IDEN:
             case ANIFEM_CURL: return 3;
             case ANIFEM_DIV: return 1;
+            // GRAD: 9 physical components from a reference field of 1 (RT0)
+            // or 3 (ND0) components (space_map maps 3 and 4)
+            case ANIFEM_GRAD: return fem == ANIFEM_FEM_RT0 ? 1 : 3;
             default: return -1;
         }
     }
@@ -66,7 +69,7 @@ int ref_dim(int op, int fem) {
 }
 
 int op_dim(const anifem_op& op) {
-    if (fem_vector(op.fem)) return ref_dim(op.op, op.fem);
+    if (fem_vector(op.fem)) return op.op == ANIFEM_GRAD ? 9 : ref_dim(op.op, op.fem);
     switch (op.op) {
         case ANIFEM_IDEN: return op.ncomp;
         case ANIFEM_GRAD: return 3 * op.ncomp;
@@ -92,12 +95,19 @@ void space_map(int op, int fem, int& map, int& scale) {
         // div phi_i = 3 c_i (raviant_thomas_0.h:35-47, :136-160); curl = 0
         if (op == ANIFEM_IDEN) map = 1, scale = 1;
         else if (op == ANIFEM_DIV) scale = 1;
+        // grad phi_i = c_i I (raviant_thomas_0.h:101-104: the sum over the
+        // vertices reduces to the identity): map 3, T = vec(I) (9 x 1)
+        else if (op == ANIFEM_GRAD) map = 3, scale = 1;
     } else if (fem == ANIFEM_FEM_ND0) {
         // ND0 on edge (a, b): phi = |e| (l_a grad l_b - l_b grad l_a)
         // = |e| PSI^T (l_a gh_b - l_b gh_a); curl phi = 2 |e| grad l_a x grad l_b
         // = (|e| / det J) J (2 gh_a x gh_b) (nedelec_0.h:35-53, :100-135); div = 0
         if (op == ANIFEM_IDEN) scale = 2;
         else if (op == ANIFEM_CURL) map = 1, scale = 3;
+        // grad phi is the skew matrix of curl phi / 2 (nedelec_0.h:167-180):
+        // (|e| / det J) S J (gh_a x gh_b), map 4, T = S J (9 x 3) with S the
+        // vector -> skew-matrix map in the reference's entry order
+        else if (op == ANIFEM_GRAD) map = 4, scale = 3;
     }
 }
 

@@ -114,8 +114,9 @@ __host__ __device__ inline void basis_gradients(int fem, const double* L, double
 
 // Reference fields of the vector-valued spaces (the part of OP(phi_i) that
 // space_map's map and scale turn into physical values), d components each:
-//   RT0 IDEN: xi - xi_{(i+3)%4} (xi_a = l_{a+1}); DIV: 3; CURL: 0
-//   ND0 IDEN on edge (a, b): l_a gh_b - l_b gh_a; CURL: 2 gh_a x gh_b; DIV: 0
+//   RT0 IDEN: xi - xi_{(i+3)%4} (xi_a = l_{a+1}); DIV: 3; CURL: 0; GRAD: 1
+//   ND0 IDEN on edge (a, b): l_a gh_b - l_b gh_a; CURL: 2 gh_a x gh_b; DIV: 0;
+//   GRAD: gh_a x gh_b (the axial vector of the skew gradient)
 __host__ __device__ inline void vector_field(int op, int fem, const double* L, double* u) {
     if (fem == ANIFEM_FEM_RT0) {
         for (int i = 0; i < 4; ++i) {
@@ -123,6 +124,7 @@ __host__ __device__ inline void vector_field(int op, int fem, const double* L, d
             if (op == ANIFEM_IDEN)
                 for (int a = 0; a < 3; ++a) u[i * 3 + a] = L[a + 1] - (v == a + 1 ? 1.0 : 0.0);
             else if (op == ANIFEM_DIV) u[i] = 3.0;
+            else if (op == ANIFEM_GRAD) u[i] = 1.0;
             else
                 for (int a = 0; a < 3; ++a) u[i * 3 + a] = 0.0;
         }
@@ -131,10 +133,11 @@ __host__ __device__ inline void vector_field(int op, int fem, const double* L, d
             const int A = edge_a(e), B = edge_b(e);
             if (op == ANIFEM_IDEN) {
                 for (int c = 0; c < 3; ++c) u[e * 3 + c] = L[A] * dlam(B, c) - L[B] * dlam(A, c);
-            } else if (op == ANIFEM_CURL) {
+            } else if (op == ANIFEM_CURL || op == ANIFEM_GRAD) {
+                const double f = op == ANIFEM_CURL ? 2.0 : 1.0;
                 for (int c = 0; c < 3; ++c) {
                     const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-                    u[e * 3 + c] = 2.0 * (dlam(A, c1) * dlam(B, c2) - dlam(A, c2) * dlam(B, c1));
+                    u[e * 3 + c] = f * (dlam(A, c1) * dlam(B, c2) - dlam(A, c2) * dlam(B, c1));
                 }
             } else {
                 u[e] = 0.0;

@@ -38,6 +38,9 @@ static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem
     a.selB = fem_vector(opB.fem) ? ANIFEM_IDEN : opB.op;
     space_map(opA.op, opA.fem, a.mapA, a.scA);
     space_map(opB.op, opB.fem, a.mapB, a.scB);
+    // GRAD of RT0 / ND0 (9 components) pairs with GRAD of RT0 / ND0 only, and
+    // not in the normal form
+    if ((a.mapA >= 3) != (a.mapB >= 3) || (a.mapA >= 3 && dot_normal)) return ANIFEM_ERR_ARG;
     a.dimA = op_dim(opA);
     a.dimB = op_dim(opB) * (dot_normal ? 3 : 1);
     a.E = a.nfA0 * opA.ncomp * a.nfB0 * opB.ncomp;
@@ -288,7 +291,8 @@ static int prepare_apply(const anifem_op& op, int q, ApplyArgs& a) {
     int m = 0, sc = 0;
     space_map(op.op, op.fem, m, sc);
     a.d = ref_dim(op.op, op.fem);
-    a.map = m == 1 ? 2 : ((op.op == ANIFEM_GRAD || (op.fem == ANIFEM_FEM_ND0 && op.op == ANIFEM_IDEN)) ? 1 : 0);
+    a.map = m >= 3 ? m
+                   : (m == 1 ? 2 : ((op.op == ANIFEM_GRAD || (op.fem == ANIFEM_FEM_ND0 && op.op == ANIFEM_IDEN)) ? 1 : 0));
     a.scale = sc;
     a.op = op.op;
     a.fem = op.fem;

@@ -190,10 +190,49 @@ __device__ __forceinline__ double face_measure(const double P[4][3], int f, doub
 // T = PSI^T for GRAD, 1 for IDEN, J for the Piola-mapped fields (map 1, with
 // JT given).  With a face normal nrm, D(k, j) is the normal contraction
 // sum_d D(d + 3k, j) nrm[d] of the 3 dimB x dimA tensor (core.h:176-195).
+// 9 x DA0 map of a GRAD of RT0 (map 3: T = vec(I), DA0 = 1) or ND0 (map 4:
+// T = S J, DA0 = 3, S the vector -> skew-matrix map in the entry order of
+// nedelec_0.h:175-179: entries 1, 2, 3, 5, 6, 7 take -a2, a1, a2, -a0, -a1, a0).
+__device__ __forceinline__ double grad9_map(int map, const double* JT, int l, int c) {
+    if (map == 3) return (l == 0 || l == 4 || l == 8) ? 1.0 : 0.0;
+    // row l of S: (component m of the axial vector, sign)
+    const int m = l == 1 || l == 3 ? 2 : (l == 2 || l == 6 ? 1 : (l == 5 || l == 7 ? 0 : -1));
+    if (m < 0) return 0.0;
+    const double sg = (l == 2 || l == 3 || l == 7) ? 1.0 : -1.0;
+    return sg * JT[c + 3 * m];  // J(m, c)
+}
+
 template <int DA0, int DB0, typename KT>
 __device__ __forceinline__ void geometry_block(const ElemArgs& a, int64_t cr, const double PSI[9], double s, int n,
                                                int cb, int ca, KT* Kb, const double* nrm = nullptr,
                                                const double* JT = nullptr) {
+    if (JT && a.mapA >= 3) {
+        // GRAD of RT0 / ND0 on both sides: 9 physical components,
+        // Kb = s TB^T D TA with the 9 x 9 tensor and the 9 x D0 maps
+        double M[9][DA0];
+        for (int k = 0; k < 9; ++k)
+            for (int a0 = 0; a0 < DA0; ++a0) M[k][a0] = 0.0;
+        for (int j = 0; j < 9; ++j) {
+            double tA[DA0];
+            bool any = false;
+            for (int a0 = 0; a0 < DA0; ++a0) {
+                tA[a0] = grad9_map(a.mapA, JT, j, a0);
+                any = any || tA[a0] != 0.0;
+            }
+            if (!any) continue;
+            for (int k = 0; k < 9; ++k) {
+                const double d = tensor_sel(a, cr, n, k, j);
+                for (int a0 = 0; a0 < DA0; ++a0) M[k][a0] += d * tA[a0];
+            }
+        }
+        for (int b0 = 0; b0 < DB0; ++b0)
+            for (int a0 = 0; a0 < DA0; ++a0) {
+                double acc = 0;
+                for (int k = 0; k < 9; ++k) acc += grad9_map(a.mapB, JT, k, b0) * M[k][a0];
+                Kb[b0 * DA0 + a0] = (KT)(s * acc);
+            }
+        return;
+    }
     const double* TAm = (JT && a.mapA) ? JT : PSI;
     const double* TBm = (JT && a.mapB) ? JT : PSI;
     double Dblk[DB0][DA0];
@@ -348,7 +387,8 @@ __global__ __launch_bounds__(kBlock) void element_kernel(ElemArgs a) {
 // ---- solution evaluation: fem3DapplyL (eval.inl:84-100, core.inl:370-405) ----
 struct ApplyArgs {
     int d;                 // reference components per basis function and component (1 or 3)
-    int map;               // 0: as tabulated; 1: PSI^T (gradients, ND0 values); 2: J (RT0 values, ND0 curls)
+    int map;               // 0: as tabulated; 1: PSI^T (gradients, ND0 values); 2: J (RT0 values, ND0 curls);
+                           // 3 / 4: GRAD of RT0 / ND0 (9 components, grad9_map)
     int scale;             // per-DOF factor (dof_scale kind), 0 none
     int op, fem;           // operator and space: the basis is evaluated on the device in X mode
     int nf0, ncomp, q;     // basis size per component, components, points per tet
@@ -378,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
     if (idx >= a.n * a.q) return;
     const int64_t r = idx / a.q;
     const int pt = (int)(idx % a.q);
-    const int nf = a.nf0 * a.ncomp, d = a.d, dim = d * a.ncomp;
+    const int nf = a.nf0 * a.ncomp, d = a.d, dim = a.map >= 3 ? 9 : d * a.ncomp;
     double P[4][3], PSI[9], JT[9], det = 1.0;
     if (a.map != 0 || a.scale != 0 || a.X) {
         for (int v = 0; v < 4; ++v)
@@ -410,7 +450,13 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
             if (a.scale) dv *= dof_scale(a.scale, P, det, i);
             for (int j = 0; j < d; ++j) g[j] += T[i * d + j] * dv;
         }
-        if (a.map == 1) {
+        if (a.map >= 3) {  // GRAD of RT0 (vec(I) g) / ND0 (S J g): 9 components
+            for (int l = 0; l < 9; ++l) {
+                double v = 0.0;
+                for (int j = 0; j < d; ++j) v += grad9_map(a.map, JT, l, j) * g[j];
+                o[l] = v;
+            }
+        } else if (a.map == 1) {
             for (int k = 0; k < 3; ++k) o[k + 3 * c] = PSI[0 + 3 * k] * g[0] + PSI[1 + 3 * k] * g[1] + PSI[2 + 3 * k] * g[2];
         } else if (a.map == 2) {
             for (int k = 0; k < 3; ++k) o[k + 3 * c] = JT[0 + 3 * k] * g[0] + JT[1 + 3 * k] * g[1] + JT[2 + 3 * k] * g[2];

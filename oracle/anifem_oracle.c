@@ -389,7 +389,7 @@ static int orc_is_vector(int fem) { return fem == ORC_RT0 || fem == ORC_ND0; }
 /* operator dimension on (fem, ncomp): raviant_thomas_0.h and nedelec_0.h
  * declare Dim 3 for IDEN and CURL, 1 for DIV */
 static int orc_opdim(int op, int fem, int ncomp) {
-    if (orc_is_vector(fem)) return op == ORC_DIV ? 1 : 3;
+    if (orc_is_vector(fem)) return op == ORC_DIV ? 1 : (op == ORC_GRAD ? 9 : 3);
     return orc_dim(op, ncomp);
 }
 
@@ -410,7 +410,7 @@ int orc_dim(int op, int ncomp) {
 
 /* operator code valid on (fem, ncomp): operators.h:37-45, :321-400 */
 static int orc_op_ok(int op, int fem, int ncomp) {
-    if (orc_is_vector(fem)) return ncomp == 1 && (op == ORC_IDEN || op == ORC_DIV || op == ORC_CURL);
+    if (orc_is_vector(fem)) return ncomp == 1 && (op == ORC_IDEN || op == ORC_GRAD || op == ORC_DIV || op == ORC_CURL);
     if (op != ORC_IDEN && op != ORC_GRAD && op != ORC_DIV && op != ORC_CURL && (op < ORC_DUDX || op > ORC_DUDZ))
         return 0;
     if (op != ORC_IDEN && fem == ORC_P0) return 0;
@@ -551,6 +551,15 @@ static void orc_vector_op(int op, int fem, int q, const double* XYL, const doubl
                         U[j + 3 * (n + q * i)] = sqr * (XYL[ip[0] + 4 * n] * (x0[j] - x3[j]) +
                                                         XYL[ip[1] + 4 * n] * (x1[j] - x3[j]) +
                                                         XYL[ip[2] + 4 * n] * (x2[j] - x3[j]));
+            } else if (op == ORC_GRAD) {
+                /* grad phi_i, entry k + 3 j: sqr sum_m dlambda_{ip[m]}/dx_k (x_{ip[m]} - x_{ip[3]})_j
+                 * (raviant_thomas_0.h:101-104), the same at every point */
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) {
+                        double g = sqr * (V[ip[0]][k] * (x0[j] - x3[j]) + V[ip[1]][k] * (x1[j] - x3[j]) +
+                                          V[ip[2]][k] * (x2[j] - x3[j]));
+                        for (int n = 0; n < q; ++n) U[k + 3 * j + 9 * (n + q * i)] = g;
+                    }
             } else {
                 double s = 0;
                 for (int j = 0; j < 3; ++j)
@@ -574,11 +583,22 @@ static void orc_vector_op(int op, int fem, int q, const double* XYL, const doubl
                 for (int k = 0; k < 3; ++k)
                     U[k + 3 * (n + q * i)] = sqr * (xyn[k] * XYL[ip[0] + 4 * n] + xym[k] * XYL[ip[1] + 4 * n]);
         } else {
+            double cu[3];
             for (int k = 0; k < 3; ++k) {
                 int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-                double v = sqr * ((V[ip[0]][k1] * xyn[k2] + V[ip[1]][k1] * xym[k2]) -
-                                  (V[ip[0]][k2] * xyn[k1] + V[ip[1]][k2] * xym[k1]));
-                for (int n = 0; n < q; ++n) U[k + 3 * (n + q * i)] = v;
+                cu[k] = sqr * ((V[ip[0]][k1] * xyn[k2] + V[ip[1]][k1] * xym[k2]) -
+                               (V[ip[0]][k2] * xyn[k1] + V[ip[1]][k2] * xym[k1]));
+            }
+            if (op == ORC_GRAD) {
+                /* grad phi_i: the skew matrix of curl/2, entries 0..8 as
+                 * nedelec_0.h:175-179 lays them out */
+                double a0 = cu[0] / 2, a1 = cu[1] / 2, a2 = cu[2] / 2;
+                const double g[9] = {0, -a2, a1, a2, 0, -a0, -a1, a0, 0};
+                for (int n = 0; n < q; ++n)
+                    for (int l = 0; l < 9; ++l) U[l + 9 * (n + q * i)] = g[l];
+            } else {
+                for (int k = 0; k < 3; ++k)
+                    for (int n = 0; n < q; ++n) U[k + 3 * (n + q * i)] = cu[k];
             }
         }
     }

@@ -296,7 +296,7 @@ int ref_quadrature(int order, double* p, double* w) {
 // 21=(IDEN RT0, IDEN RT0); 22=(DIV RT0, DIV RT0); 23=(DIV RT0, IDEN P0); 24=(IDEN ND0, IDEN ND0);
 // 25=(CURL ND0, CURL ND0); 26=(CURL ND0, IDEN RT0); 27=(GRAD CR1, GRAD CR1); 28=(IDEN CR1, IDEN P1);
 // 29=(GRAD B2, GRAD B2); 30=(IDEN B4, IDEN B4); 31=(GRAD B4, GRAD P1); 32=(IDEN P1, IDEN ND0);
-// 33=(GRAD B4, GRAD B4).
+// 33=(GRAD B4, GRAD B4); 34=(GRAD RT0, GRAD RT0); 35=(GRAD ND0, GRAD ND0); 36=(GRAD ND0, GRAD RT0).
 //  xyz is n x 12 (P0,P1,P2,P3 per tet), D is n x ddim.
 // A receives n element matrices, each nfB x nfA column-major.
 int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, const double* xyz, const double* D,
@@ -348,6 +348,9 @@ int ref_fem3dtet(int pair, int tensor_type, int per_point, int order, long n, co
             case 31: return dispatch_tensor<Operator<GRAD, B4>, Operator<GRAD, P1>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 32: return dispatch_tensor<Operator<IDEN, P1>, Operator<IDEN, ND0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             case 33: return dispatch_tensor<Operator<GRAD, B4>, Operator<GRAD, B4>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 34: return dispatch_tensor<Operator<GRAD, RT0>, Operator<GRAD, RT0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 35: return dispatch_tensor<Operator<GRAD, ND0>, Operator<GRAD, ND0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
+            case 36: return dispatch_tensor<Operator<GRAD, ND0>, Operator<GRAD, RT0>>(tensor_type, per_point, order, n, xyz, D, ddim, A);
             default: break;
         }
     } catch (const std::exception&) {

@@ -49,8 +49,8 @@ def nfa(fem, nc=1):
 
 
 def dim(op, nc=1, fem=None):
-    if fem in (21, 25):  # RT0, ND0: IDEN and CURL 3, DIV 1
-        return 1 if op == 3 else 3
+    if fem in (21, 25):  # RT0, ND0: IDEN and CURL 3, DIV 1, GRAD 9
+        return 1 if op == 3 else (9 if op == 2 else 3)
     return (3 if op == 2 else 1) * nc
 
 

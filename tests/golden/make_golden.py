@@ -31,7 +31,7 @@ PAIRS = {
     21: (1, 21, 1, 1, 21, 1), 22: (3, 21, 1, 3, 21, 1), 23: (3, 21, 1, 1, 1, 1), 24: (1, 25, 1, 1, 25, 1),
     25: (4, 25, 1, 4, 25, 1), 26: (4, 25, 1, 1, 21, 1), 27: (2, 31, 1, 2, 31, 1), 28: (1, 31, 1, 1, 2, 1),
     29: (2, 1002, 1, 2, 1002, 1), 30: (1, 1004, 1, 1, 1004, 1), 31: (2, 1004, 1, 2, 2, 1), 32: (1, 2, 1, 1, 25, 1),
-    33: (2, 1004, 1, 2, 1004, 1),
+    33: (2, 1004, 1, 2, 1004, 1), 34: (2, 21, 1, 2, 21, 1), 35: (2, 25, 1, 2, 25, 1), 36: (2, 25, 1, 2, 21, 1),
 }
 # basis functions per component: P0..P3, RT0, ND0, CR1, B2, B4
 NF = {1: 1, 2: 4, 3: 10, 4: 20, 21: 4, 25: 6, 31: 4, 1002: 6, 1004: 1}
@@ -117,6 +117,21 @@ def cases(rng):
     out.append(("grad_p2_iden_p2_general", 14, 4, 0, 2, xyz, rng.normal(size=(n, 3))))
     out.append(("vec_p1_general", 3, 4, 0, 2, xyz[:32], rng.normal(size=(32, 81))))
     return out
+
+
+def grad9_cases(rng):
+    """GRAD of RT0 and ND0 (9 components, raviant_thomas_0.h:55-113,
+    nedelec_0.h:149-183) with 9 x 9 tensors."""
+    n = 32
+    xyz = kuhn_tets(rng, n)
+    pos = lambda: 10.0 ** rng.uniform(-1, 1, (n, 1))
+    return [
+        ("g9_rt0_rt0_null", 34, 1, 0, 1, xyz, np.zeros((n, 1))),
+        ("g9_rt0_rt0_general", 34, 4, 0, 2, xyz, rng.normal(size=(n, 81))),
+        ("g9_nd0_nd0_scalar", 35, 2, 0, 1, xyz, pos()),
+        ("g9_nd0_nd0_general_xdep", 35, 4, 2, 3, xyz, rng.normal(size=(n, 81))),
+        ("g9_nd0_rt0_general", 36, 4, 0, 2, xyz, rng.normal(size=(n, 81))),
+    ]
 
 
 def high_cases(rng):
@@ -285,7 +300,7 @@ def main():
     dp = ctypes.POINTER(ctypes.c_double)
     # groups to write (default all): elem (with the operator and space cases), faces, apply;
     # "spaces" writes the cases of the RT0 / ND0 / CR1 / B2 / B4 spaces and fem3DapplyX only;
-    # "high" the cases of quadrature orders 3 and 7..20
+    # "high" the cases of quadrature orders 3 and 7..20; "grad9" those of GRAD of RT0 / ND0
     groups = set(sys.argv[1:]) or {"elem", "faces", "apply"}
     rng = np.random.default_rng(20260101)
     todo = []
@@ -295,6 +310,8 @@ def main():
         todo += space_cases(np.random.default_rng(20260505))
     if "elem" in groups or "high" in groups:
         todo += high_cases(np.random.default_rng(20260808))
+    if "elem" in groups or "grad9" in groups:
+        todo += grad9_cases(np.random.default_rng(20260909))
     for case in todo:
         name, pair, tt, pp, order, xyz, D = case[:7]
         extra = case[7] if len(case) > 7 else None

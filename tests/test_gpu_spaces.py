@@ -86,9 +86,26 @@ def test_apply_polynomial_spaces_vs_oracle(op, fem):
     assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("fem", [af.FEM_RT0, af.FEM_ND0])
+def test_apply_grad_of_vector_spaces_vs_oracle(fem):
+    """fem3DapplyL of GRAD RT0 / GRAD ND0 (9 components) against the oracle,
+    whose gradients are pinned by finite differences of the fields."""
+    rng = np.random.default_rng(fem)
+    n, q = 50, 3
+    xyz = rng.normal(size=(n, 12))
+    xyl = rng.dirichlet(np.ones(4), q)
+    o = af.Op(af.GRAD, fem)
+    dofs = rng.normal(size=(n, o.nfa))
+    out = af.fem3dapply_batch(o, xyl, dev(T.soa_xyz(xyz)), dev(dofs)).cpu().numpy()
+    ref = T.orc_apply(af.GRAD, fem, 1, xyl, xyz, dofs)
+    assert out.shape == (n, q, 9)
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_refused_cases():
-    """GRAD of RT0 / ND0 (elements and evaluation), vector copies of them
-    (elements and global assembly) and the MFMA mode return ANIFEM_ERR_ARG."""
+    """GRAD of RT0 / ND0 paired with another operator or in the normal form,
+    vector copies of RT0 / ND0 (elements and global assembly) and the MFMA mode
+    return ANIFEM_ERR_ARG."""
     import torch
     L = af.lib()
     xyz = dev(T.soa_xyz(np.random.default_rng(1).normal(size=(4, 12))))
@@ -100,11 +117,16 @@ def test_refused_cases():
     assert call(rt, rt, af.PREC_FP64) == 0
     assert call(nd, nd, af.PREC_FP32) == 0
     assert call(rt, rt, af.PREC_FP16X3) == af.ERR_ARG
-    assert call(af.Op(af.GRAD, af.FEM_RT0), af.Op(af.GRAD, af.FEM_RT0), af.PREC_FP64) == af.ERR_ARG
+    assert call(af.Op(af.GRAD, af.FEM_RT0), af.Op(af.GRAD, af.FEM_RT0), af.PREC_FP64) == 0
+    assert call(af.Op(af.GRAD, af.FEM_RT0), af.Op(af.GRAD, af.FEM_RT0), af.PREC_FP16X3) == af.ERR_ARG
+    # 9 components against a 9-component gradient of a vector P1 space: a GENERAL tensor fits the dims, the
+    # pairing is not built
+    eye9 = dev(np.eye(9).ravel())
+    tg = af.anifem_tensor(af.TENSOR_GENERAL, af.COEF_CONST, af._ptr(eye9))
+    assert L.anifem_fem3dtet_batch(af.Op(af.GRAD, af.FEM_RT0).c(), af.Op(af.GRAD, af.FEM_P1, 3).c(), ctypes.byref(tg),
+                                   2, 4, af._ptr(xyz), af.PREC_FP64, af._ptr(out), None) == af.ERR_ARG
     assert call(af.Op(af.IDEN, af.FEM_ND0, 3), af.Op(af.IDEN, af.FEM_ND0, 3), af.PREC_FP64) == af.ERR_ARG
-    assert af.Op(af.GRAD, af.FEM_ND0).nfa == -1
-    with pytest.raises(af.AnifemError):
-        af.fem3dapply_batch(af.Op(af.GRAD, af.FEM_RT0), np.full((1, 4), 0.25), xyz, dev(np.zeros((4, 4))))
+    assert af.Op(af.GRAD, af.FEM_ND0).nfa == 6 and af.Op(af.GRAD, af.FEM_ND0).dim == 9
     coords, tets = af.kuhn_cube(2, 0.1, 1)
     for fem in (af.FEM_CR1, af.FEM_B2, af.FEM_RT0):
         with pytest.raises(af.AnifemError):

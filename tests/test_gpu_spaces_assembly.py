@@ -17,7 +17,7 @@ RT0, ND0, CR1, B2, B4, P0 = af.FEM_RT0, af.FEM_ND0, af.FEM_CR1, af.FEM_B2, af.FE
 
 
 @pytest.mark.parametrize("op,fem", [(af.DIV, RT0), (af.IDEN, RT0), (af.CURL, ND0), (af.IDEN, ND0), (af.GRAD, CR1),
-                                    (af.GRAD, B2), (af.IDEN, B4), (af.IDEN, P0)])
+                                    (af.GRAD, B2), (af.IDEN, B4), (af.IDEN, P0), (af.GRAD, RT0), (af.GRAD, ND0)])
 def test_space_assembly_matches_oracle(op, fem):
     coords, tets = af.kuhn_cube(3, 0.2, 5)
     coords, tets = T.shuffled_mesh(coords, tets, 6)

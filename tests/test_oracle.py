@@ -536,3 +536,36 @@ def test_coloured_scatter_equals_sequential():
     assert rc == 0 and rc2 == 0
     np.testing.assert_allclose(v2, v1, rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(r2, r1, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("fem", [21, 25])
+def test_grad_of_rt0_nd0_is_the_jacobian_of_the_field(fem):
+    """GRAD of RT0 / ND0 (raviant_thomas_0.h:55-113, nedelec_0.h:149-183):
+    the 9 entries (entry d + 3 c = d phi_c / d x_d, the layout of GRAD of
+    FemVec<3, .>, operators.h:127-155) equal central differences of the
+    oracle's IDEN values in physical space; RT0's is c_i I and ND0's is
+    skew-symmetric."""
+    rng = np.random.default_rng(fem)
+    n = 6
+    xyz = np.stack([rng.uniform(-1, 1, 12) for _ in range(n)])
+    nf = 4 if fem == 21 else 6
+    P = xyz.reshape(n, 4, 3)
+    X = (P.mean(axis=1))[:, None, :]  # centroid, (n, 1, 3)
+    dofs = np.tile(np.eye(nf), (n, 1))
+    xr = np.repeat(xyz, nf, axis=0)
+    G = T.orc_applyx(GRAD, fem, 1, np.repeat(X, nf, axis=0), xr, dofs)[:, 0, :]  # (n*nf, 9)
+    h = 1e-6
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = h
+        up = T.orc_applyx(IDEN, fem, 1, np.repeat(X + e, nf, axis=0), xr, dofs)[:, 0, :]
+        dn = T.orc_applyx(IDEN, fem, 1, np.repeat(X - e, nf, axis=0), xr, dofs)[:, 0, :]
+        fd = (up - dn) / (2 * h)  # d phi_k / d x_j, k = 0..2
+        for k in range(3):
+            np.testing.assert_allclose(G[:, j + 3 * k], fd[:, k], rtol=1e-6, atol=1e-7 * np.abs(G).max())
+    M = G.reshape(-1, 3, 3)  # M[.., j, k] = d phi_k / d x_j
+    if fem == 21:
+        off = M - np.einsum("i,jk->ijk", np.trace(M, axis1=1, axis2=2) / 3, np.eye(3))
+        assert np.abs(off).max() < 1e-12 * np.abs(M).max()
+    else:
+        assert np.abs(M + M.transpose(0, 2, 1)).max() < 1e-12 * np.abs(M).max()
