@@ -259,7 +259,7 @@ def bench_elasticity(args, torch):
         res[name] = {"ms_per_pass": ms, "element_matrices_per_s": n / ms * 1e3, "nnz_per_s": A.nnz / ms * 1e3}
         if name == "fp64":
             res["roofline"] = hbm_roofline(assembly_bytes(n, 30, 16, A.nnz, A.ndof), ms,
-                                           "geometry_kernel + row_block_persistent (P2^3, Lame)",
+                                           "geometry_kernel<3,3,3,3,double,false,true> + row_block_persistent_vec<3,3,3,10,double,true> (P2^3, compact Lame rows)",
                                            "per tet 96 B xyz + 16 B (lambda, mu) + 4 nf^2 B slots + 4 nf B dofs; "
                                            "16 B per nnz and per DOF")
     A.assemble(D=D, F=g, precision=af.PREC_FP32, val=val, rhs=rhs, status=False)

@@ -43,7 +43,7 @@ enum {
     ANIFEM_DUDZ = 8
 };
 enum { ANIFEM_FEM_P0 = 1, ANIFEM_FEM_P1 = 2, ANIFEM_FEM_P2 = 3, ANIFEM_FEM_P3 = 4 };
-/* The other FemFix spaces of the reference (operators.h:26-35; spaces/*.h),
+/* The other FemFix spaces of the reference (operators.h:26-35; the spaces/ headers),
  * same codes.  They are accepted by the element-level entry points
  * (anifem_fem3dtet_batch in FP64 / FP32, anifem_fem3dface_batch,
  * anifem_fem3dapply_batch, anifem_fem3dapplyx_batch), not by the global

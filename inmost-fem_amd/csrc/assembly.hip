@@ -160,11 +160,17 @@ __global__ __launch_bounds__(anifem::kBlock) void colour_scatter_kernel(anifem::
     const int ib = e % nf, ia = e / nf;
     const int ib0 = ib % nfB0, cb = ib / nfB0, ia0 = ia % nfA0, ca = ia / nfA0;
     const int len = a.m.q * DA0 * DB0;
+    const int kstride = a.lamec ? anifem::kLameCompact : NCA * NCB * len;
     const T* R = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.Rd : (const void*)KT.Rf);
     const T* Kb = Kg + t * a.rowlen + (cb * NCA + ca) * len;
     const T* Rr = R + (ib0 + nfB0 * ia0) * len;
     T acc = 0;
-    for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
+    if constexpr (NCA == 3 && NCB == 3 && DA0 * DB0 == 9) {
+        // compact Lame row: the (cb, ca) block in closed form
+        if (a.lamec) acc = anifem::lame_entry(Kg + t * a.rowlen, cb, ca, Rr);
+    }
+    if (!a.lamec)
+        for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
     const int32_t dof = dofs[t * nf + ib];
     const bool dir = isdir[dof] != 0;
     int bad = 0;
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(anifem::kBlock) void colour_scatter_kernel(anifem::
     if (a.has_rhs && ia == 0) {
         const int lenF = a.f.q;
         const T* RF = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.RFd : (const void*)KT.RFf);
-        const T* KF = Kg + t * a.rowlen + NCA * NCB * len + cb * lenF;
+        const T* KF = Kg + t * a.rowlen + kstride + cb * lenF;
         T fa = 0;
         for (int j = 0; j < lenF; ++j) fa += KF[j] * RF[ib0 * lenF + j];
         const double fv = dir ? 0.0 : (double)fa;
@@ -255,20 +261,31 @@ int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T*
 
 // Persistent row-block pass for vector spaces (NC components of NF0 scalar
 // DOFs on both sides): occupancy x CUs workgroups, whole XCD groups.
-template <int DA0, int DB0, int NC, int NF0, typename T>
+template <int DA0, int DB0, int NC, int NF0, typename T, bool LC = false>
 int launch_persistent_vec(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
     using namespace anifem;
     int occ = 0, n_cu = 0;
     ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void*>(&row_block_persistent_vec<DA0, DB0, NC, NF0, T>), kBlock, lds));
+        &occ, reinterpret_cast<const void*>(&row_block_persistent_vec<DA0, DB0, NC, NF0, T, LC>), kBlock, lds));
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
     G = std::min<int64_t>(G, ctx->S.n_blocks);
     G = std::max<int64_t>(8, G / 8 * 8);
-    hipLaunchKernelGGL((row_block_persistent_vec<DA0, DB0, NC, NF0, T>), dim3((unsigned)G), dim3(kBlock), lds, s, a,
-                       Kg);
+    hipLaunchKernelGGL((row_block_persistent_vec<DA0, DB0, NC, NF0, T, LC>), dim3((unsigned)G), dim3(kBlock), lds, s,
+                       a, Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
+}
+
+// Compact Lame rows (lame_kblock) for GRAD x GRAD of 3-component spaces with
+// the Lame tensor constant over each tet, volume integrals.
+template <int NCA, int NCB, int KDA, int KDB>
+bool lame_compact(const anifem::BlockArgs& a) {
+    if constexpr (NCA == 3 && NCB == 3 && KDA == 3 && KDB == 3)
+        return a.m.ttype == ANIFEM_TENSOR_LAME && a.m.q == 1 && a.m.face < 0 && a.m.selA == ANIFEM_GRAD &&
+               a.m.selB == ANIFEM_GRAD && a.m.mapA == 0 && a.m.mapB == 0 && a.m.scA == 0 && a.m.scB == 0 &&
+               (a.m.storage == ANIFEM_COEF_CONST || a.m.storage == ANIFEM_COEF_PER_TET);
+    return false;
 }
 
 // SYM: K symmetric (the same 3-dimensional operator on both sides of a scalar
@@ -281,7 +298,8 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     constexpr int KDA = SYM ? 6 : DA0, KDB = SYM ? 1 : DB0;  // contraction block of the row-block passes
     const AssemblySetup& S = ctx->S;
     const int len = a.m.q * KDA * KDB, lenF = a.has_rhs ? a.f.q : 0;
-    const int kstride = NCA * NCB * len;
+    a.lamec = lame_compact<NCA, NCB, KDA, KDB>(a) ? 1 : 0;
+    const int kstride = a.lamec ? kLameCompact : NCA * NCB * len;
     a.rowlen = pad16<T>(kstride + (a.has_rhs ? NCB * lenF : 0));
     int max_lds = 0;
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
@@ -317,8 +335,17 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     }
     ctx->run_kg = Kg;
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T, SYM>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
-                       S.n_tets);
+    if constexpr (NCA == 3 && NCB == 3 && DA0 == 3 && DB0 == 3) {
+        if (a.lamec)
+            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T, false, true>), dim3((unsigned)gblocks), dim3(kBlock), 0,
+                               s, a, Kg, S.n_tets);
+        else
+            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
+                               S.n_tets);
+    } else {
+        hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T, SYM>), dim3((unsigned)gblocks), dim3(kBlock), 0, s,
+                           a, Kg, S.n_tets);
+    }
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));
     if (ctx->coloured) return launch_coloured<KDA, KDB, NCA, NCB, T>(ctx, a, Kg, s);
@@ -341,6 +368,12 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
         const size_t lds_v = lds_of(true);
         if (ctx->persistent && lds_v <= (size_t)max_lds && a.max_R_rows * len < 65536 && len == KDA * KDB &&
             (!a.has_rhs || lenF == 1) && K0.nfA0 == K0.nfB0) {
+            if constexpr (DA0 == 3 && DB0 == 3) {
+                if (a.lamec && K0.nfA0 == 4)
+                    return launch_persistent_vec<3, 3, 3, 4, T, true>(ctx, a, (const T*)Kg, lds_v, s);
+                if (a.lamec && K0.nfA0 == 10)
+                    return launch_persistent_vec<3, 3, 3, 10, T, true>(ctx, a, (const T*)Kg, lds_v, s);
+            }
             if (K0.nfA0 == 4) return launch_persistent_vec<DA0, DB0, 3, 4, T>(ctx, a, (const T*)Kg, lds_v, s);
             if (K0.nfA0 == 10) return launch_persistent_vec<DA0, DB0, 3, 10, T>(ctx, a, (const T*)Kg, lds_v, s);
         }
@@ -366,7 +399,9 @@ int launch_blocks_lift(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s)
     int rc = launch_blocks<DA0, DB0, NCA, NCB, T, SYM>(ctx, a, s);
     if (rc || !ctx->run_bc || !a.rhs || ctx->n_lift == 0) return rc;
     const int len = a.m.q * KDA * KDB, lenF = a.has_rhs ? a.f.q : 0;
-    a.rowlen = pad16<T>(NCA * NCB * len + (a.has_rhs ? NCB * lenF : 0));  // as launch_blocks sets it
+    // as launch_blocks sets them
+    a.lamec = lame_compact<NCA, NCB, KDA, KDB>(a) ? 1 : 0;
+    a.rowlen = pad16<T>((a.lamec ? kLameCompact : NCA * NCB * len) + (a.has_rhs ? NCB * lenF : 0));
     const int64_t blocks = (ctx->n_lift + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((dirichlet_lift_kernel<KDA, KDB, NCA, NCB, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a,
                        static_cast<const T*>(ctx->run_kg), ctx->d_lift, ctx->n_lift, ctx->d_dofs, ctx->d_isdir,

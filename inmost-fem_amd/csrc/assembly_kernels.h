@@ -61,6 +61,7 @@ struct BlockArgs {
     const uint16_t* sstart;     // per slot: first value, relative to its block
     const uint16_t* pstart;     // per row: first pair, relative to its block
     const uint64_t* signmask;   // element-matrix passes: per tet, bit j = local DOF j has orientation sign -1, or null
+    int lamec;                  // geometry rows hold the compact Lame form (lame_kblock), not the K blocks
     double* val;
     double* rhs;
     int* status;
@@ -114,6 +115,55 @@ __device__ __forceinline__ void pad_copy(V* __restrict__ dst, const V* __restric
     }
 }
 
+// Compact rows of the Lame tensor (lin_elast.cpp:101-117) on FemVec<3, .>
+// GRAD x GRAD: with C((i,j),(k,l)) = lam d_ij d_kl + mu (d_ik d_jl + d_il d_jk)
+// and the gradient map T(j, a) = Tm[a + 3 j] (PSI), the sum over gradient
+// entries of T^T C T collapses to
+//   K_(cb,ca)(b, a) = lam T(cb,b) T(ca,a) + mu (d_{cb,ca} (T^T T)(b,a) + T(ca,b) T(cb,a)),
+// lam and mu scaled by the cell measure.  A compact row holds Tm, lam, mu
+// (kLameCompact values) instead of the 81 values of the nine blocks; every
+// pass expands the blocks it needs with this function, with its roundings
+// fixed (no contraction left to the compiler), so all passes agree bitwise.
+constexpr int kLameCompact = 11;
+
+// u = row cb of T, v = row ca, same = (cb == ca); callers read u and v from
+// the row (or from c with fixed indices): a runtime index into a register
+// array would move the array to scratch memory.
+template <typename T>
+__device__ __forceinline__ void lame_kblock(const T* c, const T* u, const T* v, bool same, T* Kb) {
+#pragma clang fp contract(off)
+    const T lam = c[9], mu = c[10];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T e = lam * (u[b] * v[a]);
+            e = fma(mu, v[b] * u[a], e);
+            const T g = fma(c[b + 6], c[a + 6], fma(c[b + 3], c[a + 3], c[b] * c[a]));
+            const T eg = fma(mu, g, e);
+            Kb[b * 3 + a] = same ? eg : e;
+        }
+}
+
+// The (cb, ca) block of a compact row in global memory, contracted with one
+// reference row r of 9 values (the contraction order of the row-block passes).
+template <typename T>
+__device__ __forceinline__ T lame_entry(const T* Kt, int cb, int ca, const T* r) {
+    T c[kLameCompact], u[3], v[3], kb[9];
+#pragma unroll
+    for (int j = 0; j < kLameCompact; ++j) c[j] = Kt[j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        u[k] = Kt[3 * cb + k];
+        v[k] = Kt[3 * ca + k];
+    }
+    lame_kblock(c, u, v, cb == ca, kb);
+    T acc = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) acc += kb[j] * r[j];
+    return acc;
+}
+
 // Geometry pass over all tets (one lane per tet): FP64 Jacobian, inverse and
 // |T| (core.inl:218-248), then the geometry tensors of the matrix (NCA*NCB
 // blocks of q*DA0*DB0 values) and of the load vector (NCB blocks of qF
@@ -121,9 +171,11 @@ __device__ __forceinline__ void pad_copy(V* __restrict__ dst, const V* __restric
 // SYM (3 x 3 tensors of a symmetric K, one component, one carried point): the
 // matrix part of the row is K packed to its 6 distinct values (sym_pack),
 // which the row-block passes contract with the column-summed R.
-template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM = false>
+// LC: compact Lame rows (lame_kblock; a tensor constant over each tet).
+template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM = false, bool LC = false>
 __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __restrict__ Kg, int64_t n_tets) {
     static_assert(!SYM || (DA0 == 3 && DB0 == 3 && NCA == 1 && NCB == 1), "packed K: 3 x 3 scalar blocks");
+    static_assert(!LC || (DA0 == 3 && DB0 == 3 && NCA == 3 && NCB == 3), "compact Lame rows: FemVec<3> gradients");
     // short rows (scalar spaces, one carried point) are assembled in LDS and
     // stored by the workgroup as one contiguous range: coalesced stores
     // instead of one strided store per value and lane
@@ -134,14 +186,21 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
     const int64_t r = r0 + threadIdx.x;
     int degenerate = 0;
     if (r < n_tets) {
-        const int kstride = SYM ? 6 : NCA * NCB * a.m.q * DA0 * DB0;
+        const int kstride = LC ? kLameCompact : (SYM ? 6 : NCA * NCB * a.m.q * DA0 * DB0);
         double P[4][3], PSI[9], det;
         load_tet<true>(a.m, r, P);
         tet_geometry(P, PSI, det);
         degenerate = !(fabs(det) > 0.0 && isfinite(det));
         const double mes = fabs(det) / 6.0;
         T* row = staged ? stage + threadIdx.x * a.rowlen : Kg + r * a.rowlen;
-        if constexpr (SYM) {
+        if constexpr (LC) {
+            const bool cst = a.m.storage == ANIFEM_COEF_CONST;
+            const double lam = cst ? a.m.D[0] : a.m.D[r], mu = cst ? a.m.D[1] : a.m.D[a.m.coef_n + r];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) row[i] = (T)PSI[i];
+            row[9] = (T)(mes * lam);
+            row[10] = (T)(mes * mu);
+        } else if constexpr (SYM) {
             double K9[9];  // K(b, a) at b*3 + a
             geometry_block<3, 3, double>(a.m, r, PSI, mes, 0, 0, 0, K9);
 #pragma unroll
@@ -236,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int len = a.m.q * KB, lenp = pad16<T>(len);
     const int lenF = a.has_rhs ? a.f.q : 0, lenFp = a.has_rhs ? pad16<T>(lenF) : 0;
-    const int kstride = NCA * NCB * len;
+    const int kstride = a.lamec ? kLameCompact : NCA * NCB * len;
     const BlockLds L = block_lds_layout(a.max_values, a.max_pairs, a.max_slots, (int)sizeof(T),
                                         RS ? a.max_R_rows * lenp : 0, RS && a.has_rhs ? a.max_RF_rows * lenFp : 0);
     double* vals = reinterpret_cast<double*>(smem_raw + L.vals);
@@ -271,7 +330,18 @@ __global__ __launch_bounds__(kBlock) void row_block_kernel(BlockArgs a, const T*
     const T* Kt = Kg + (int64_t)tet * a.rowlen;
     const bool qc = len == KB;  // one carried point: K blocks held in registers
     T kreg[NCA][KB];
-    if (qc) {
+    if constexpr (NCA == 3 && NCB == 3 && KB == 9) {
+        if (a.lamec) {  // compact Lame row: the pair's three blocks in closed form
+            T c[kLameCompact], u[3];
+#pragma unroll
+            for (int j = 0; j < kLameCompact; ++j) c[j] = has_pair ? Kt[j] : T(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) u[k] = has_pair ? Kt[3 * cb + k] : T(0);
+#pragma unroll
+            for (int ca = 0; ca < NCA; ++ca) lame_kblock(c, u, c + 3 * ca, cb == ca, kreg[ca]);
+        }
+    }
+    if (qc && !a.lamec) {
 #pragma unroll
         for (int ca = 0; ca < NCA; ++ca)
 #pragma unroll
@@ -408,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void dirichlet_lift_kernel(BlockArgs a, con
     const KindTable& KT = a.kinds[0];
     const int nf = KT.nf, nfA0 = KT.nfA0, nfB0 = KT.nfB0;
     const int len = a.m.q * KB, lenF = a.has_rhs ? a.f.q : 0;
-    const int kstride = NCA * NCB * len;
+    const int kstride = a.lamec ? kLameCompact : NCA * NCB * len;
     const T* R = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.Rd : (const void*)KT.Rf);
     const T* RF = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.RFd : (const void*)KT.RFf);
     const LiftRow L = lift[i];
@@ -436,7 +506,11 @@ __global__ __launch_bounds__(kBlock) void dirichlet_lift_kernel(BlockArgs a, con
                 const T* Kb = Kt + (cb * NCA + ca) * len;
                 const T* Rr = R + (ib0 + nfB0 * ia0) * len;
                 T acc = 0;
-                for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
+                if constexpr (NCA == 3 && NCB == 3 && KB == 9) {
+                    if (a.lamec) acc = lame_entry(Kt, cb, ca, Rr);
+                }
+                if (!a.lamec)
+                    for (int j = 0; j < len; ++j) acc += Kb[j] * Rr[j];
                 fv -= (double)acc * bc[dofs[tet * nf + ia]];
             }
         }
@@ -901,8 +975,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 // registers (no occupancy cap on the register count), which is what hides
 // the gather latency.  Same arithmetic and summation order as
 // row_block_kernel: the results are bitwise identical.
-template <int DA0, int DB0, int NC, int NF0, typename T>
+// LC: compact Lame rows; a pair carries 11 values and expands its three
+// blocks in phase A (lame_kblock).
+template <int DA0, int DB0, int NC, int NF0, typename T, bool LC = false>
 __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, const T* __restrict__ Kg) {
+    static_assert(!LC || (NC == 3 && DA0 == 3 && DB0 == 3), "compact Lame rows: FemVec<3> gradients");
     constexpr int KB = DA0 * DB0;
     constexpr int NF = NC * NF0;       // entries per pair
     constexpr int PW = NF / 2;         // position words per pair
@@ -920,7 +997,7 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
     T* R = reinterpret_cast<T*>(smem_raw + L.R);
     T* RF = reinterpret_cast<T*>(smem_raw + L.RF);
     const bool sw_regs = a.max_slots <= 2 * SW * kBlock - 2;
-    const int kstride = NC * NC * KB;  // NC x NC component blocks, one point
+    const int kstride = LC ? kLameCompact : NC * NC * KB;  // NC x NC component blocks, one point
 
     BlockWalk W;
     W.init(a.n_blocks);
@@ -942,7 +1019,9 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
         int32_t tet;
         int ibf;
         uint32_t posw[PW];
-        T k[NC][KB];
+        T k[LC ? 1 : NC][LC ? 1 : KB];     // the pair's component blocks
+        T c[LC ? kLameCompact : 1];        // or its compact Lame row
+        T u[LC ? 3 : 1];                   // and row cb of its T
         T kf;
         uint32_t sw[SW];
         int pst;
@@ -962,10 +1041,17 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
         const bool hp = (int)threadIdx.x < Mx.np;
         const int cb = (P.ibf & 63) / NF0;
         const T* Kt = Kg + (int64_t)P.tet * a.rowlen;
+        if constexpr (LC) {
 #pragma unroll
-        for (int ca = 0; ca < NC; ++ca)
+            for (int j = 0; j < kLameCompact; ++j) P.c[j] = hp ? Kt[j] : T(0);
 #pragma unroll
-            for (int j = 0; j < KB; ++j) P.k[ca][j] = hp ? Kt[(cb * NC + ca) * KB + j] : T(0);
+            for (int k = 0; k < 3; ++k) P.u[k] = hp ? Kt[3 * cb + k] : T(0);
+        } else {
+#pragma unroll
+            for (int ca = 0; ca < NC; ++ca)
+#pragma unroll
+                for (int j = 0; j < KB; ++j) P.k[ca][j] = hp ? Kt[(cb * NC + ca) * KB + j] : T(0);
+        }
         P.kf = (hp && a.has_rhs) ? Kt[kstride + cb] : T(0);
         if (sw_regs) {
             const uint32_t* sg = reinterpret_cast<const uint32_t*>(a.sstart) + (Mx.s0 >> 1);
@@ -1017,6 +1103,11 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
             const int ib0 = ib % NF0;
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + ib0 * lenp;
+            T kx[LC ? NC : 1][LC ? KB : 1];  // the expanded blocks of a compact row
+            if constexpr (LC) {
+#pragma unroll
+                for (int ca = 0; ca < NC; ++ca) lame_kblock(cur.c, cur.u, cur.c + 3 * ca, ib / NF0 == ca, kx[ca]);
+            }
             // one reference row per ia0, shared by the NC component entries
 #pragma unroll
             for (int ia0 = 0; ia0 < NF0; ++ia0) {
@@ -1031,7 +1122,7 @@ __global__ __launch_bounds__(kBlock) void row_block_persistent_vec(BlockArgs a, 
                     if (ps != kPosDropped) {
                         T acc = 0;
 #pragma unroll
-                        for (int j = 0; j < KB; ++j) acc += cur.k[ca][j] * r[j];
+                        for (int j = 0; j < KB; ++j) acc += (LC ? kx[LC ? ca : 0][LC ? j : 0] : cur.k[LC ? 0 : ca][LC ? 0 : j]) * r[j];
                         const double v = dir ? 1.0 : (double)acc;
                         nonfinite += isfinite(v) ? 0 : 1;
                         vals[ps] = v;

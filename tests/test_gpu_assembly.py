@@ -342,6 +342,41 @@ def test_persistent_vector_and_per_block_kernels_bitwise_equal(fem, dirichlet, n
         assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
 
 
+@pytest.mark.parametrize("fem", [af.FEM_P1, af.FEM_P2])
+def test_lame_compact_rows_match_full_blocks(fem):
+    """A Lame tensor constant over each tet takes the compact geometry rows (T,
+    lambda |T|, mu |T|, expanded to the 3 x 3 component blocks in the row-block
+    pass); per-point storage with the same values at every point takes the full
+    K blocks.  Both against the row maximum of the other: within 1e-13, FP64;
+    the persistent and one-workgroup-per-block passes of the compact form are
+    bitwise equal, with and without Dirichlet values."""
+    rng = np.random.default_rng(17 + fem)
+    coords, tets = af.kuhn_cube(4, 0.2, 5)
+    n = len(tets)
+    lam, mu = rng.uniform(1, 10, n), rng.uniform(1, 10, n)
+    order = 2
+    kw = dict(order=order, rhs_tensor_type=af.TENSOR_GENERAL, rhs_tensor_storage=af.COEF_CONST,
+              dirichlet=af.DIRICHLET_BOUNDARY)
+    op = af.Op(af.GRAD, fem, 3)
+    A1 = af.Assembler(coords, tets, op, tensor_type=af.TENSOR_LAME, tensor_storage=af.COEF_PER_TET, **kw)
+    A2 = af.Assembler(coords, tets, op, tensor_type=af.TENSOR_LAME, tensor_storage=af.COEF_PER_POINT, **kw)
+    q = len(af.quadrature(order)[1])
+    Dp = np.ascontiguousarray(np.repeat(np.stack([lam, mu])[:, None, :], q, axis=1))
+    g = dev(np.array([0.3, 0.0, -1.0]))
+    v1, r1, s1 = A1.assemble(D=dev(np.stack([lam, mu])), F=g)
+    v2, r2, s2 = A2.assemble(D=dev(Dp), F=g)
+    assert s1.code == 0 and s2.code == 0
+    S = {"row_ptr": A1.setup()["row_ptr"]}
+    compare(v1.cpu().numpy(), r1.cpu().numpy(), S, v2.cpu().numpy(), r2.cpu().numpy(), tol=1e-13)
+    bc = dev(np.where(A1.setup()["dirichlet"] != 0, rng.normal(size=A1.ndof), 0.0))
+    out = []
+    for persistent in (True, False):
+        A1.set_persistent(persistent)
+        out.append(A1.assemble(D=dev(np.stack([lam, mu])), F=g, dir_values=bc))
+    assert np.array_equal(out[0][0].cpu().numpy(), out[1][0].cpu().numpy())
+    assert np.array_equal(out[0][1].cpu().numpy(), out[1][1].cpu().numpy())
+
+
 @pytest.mark.parametrize("fem,nc", [(af.FEM_P2, 1), (af.FEM_P1, 3), (af.FEM_P3, 1)])
 def test_assemble_elements_matches_oracle_scatter(fem, nc):
     """Caller-supplied element matrices (random, FP64 and FP32 values) through
