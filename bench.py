@@ -220,10 +220,11 @@ def bench_batched(args, torch):
     out = {"meshes": 256, "n_tets": B.n_tets, "ndof": B.ndof, "nnz": B.nnz, "row_blocks": B.n_blocks,
            "ms_per_pass": ms, "element_matrices_per_s": B.n_tets / ms * 1e3, "nnz_per_s": B.nnz / ms * 1e3,
            "setup_s": setup_s, "autotune": tuning,
-           "roofline": hbm_roofline(alg, ms, "geometry_kernel + row_block_persistent (all meshes, one graph)",
+           "roofline": hbm_roofline(alg, ms, "geometry_kernel + row_block_persistent per element order (one graph)",
                                     "per tet 96 B xyz + 8 B kappa + 4 nf^2 B slots + 4 nf B dofs; "
                                     "16 B per nnz and per DOF"),
-           "note": "one geometry + one row-block launch for all meshes, replayed from a HIP graph (FP64)"}
+           "note": "one geometry launch for all meshes and one row-block launch per element order, replayed from "
+                   "a HIP graph (FP64)"}
     del B, val, rhs
     return out
 

@@ -58,6 +58,7 @@ struct BlockArgs {
     const uint8_t* pair_ib;     // local row index | kPairDirichlet
     const uint16_t* pair_pos;   // nf per pair: value-list position or kPosDropped
     const uint32_t* pair_pos8;  // compact positions (compact_positions), or null
+    int64_t pos8_base;          // pair whose compact positions start pair_pos8 (0; batches: first pair of a kind)
     const uint16_t* sstart;     // per slot: first value, relative to its block
     const uint16_t* pstart;     // per row: first pair, relative to its block
     const uint64_t* signmask;   // element-matrix passes: per tet, bit j = local DOF j has orientation sign -1, or null
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         P.tet = hp ? a.pair_tet[gp] : 0;
         P.ibf = hp ? a.pair_ib[gp] : 0;
         if constexpr (P8) {
-            const uint32_t* pw = a.pair_pos8 + gp * NPW;
+            const uint32_t* pw = a.pair_pos8 + (gp - a.pos8_base) * NPW;
 #pragma unroll
             for (int w = 0; w < NPW; ++w) P.posw[w] = hp ? pw[w] : 0u;
         } else {

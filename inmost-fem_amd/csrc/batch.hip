@@ -7,12 +7,14 @@
 // quadrature order 2 for P1/P2 and 4 for P3; int_tet.h:17-20) and homogeneous
 // Dirichlet on its boundary.  The batch is assembled as one block-diagonal
 // system: the host setup of each mesh (setup.cpp, blocks.cpp) is concatenated
-// with offsets, so a single geometry launch covers all tets and a single
-// row-block launch covers the row blocks of all meshes, each block selecting
-// the reference tensor of its element order (KindTable).  Each mesh's CSR
-// values and rhs land in their own slices of the output arrays and are
-// bitwise identical to assembling that mesh alone.  The launch sequence is
-// captured once in a HIP graph and replayed.
+// with offsets, so a single geometry launch covers all tets.  The row blocks
+// are grouped by element order, and each order's blocks run in one persistent
+// row-block launch of that order's instantiation (the fixed-size form for P1
+// and P2, compact value positions where every mesh of the order has them),
+// each block selecting the reference tensor of its order (KindTable).  Each
+// mesh's CSR values and rhs land in their own slices of the output arrays and
+// are bitwise identical to assembling that mesh alone.  The launch sequence
+// is captured once in a HIP graph and replayed.
 //
 // Autotuning: the row-block budgets (slots / entries / pairs per block) are
 // chosen per element order by timing candidate budgets on the GPU on a
@@ -74,6 +76,11 @@ struct anifem_batch {
     std::vector<int64_t> tet_off, node_off, dof_off, nnz_off, rp_off;
     int64_t n_tets = 0, n_nodes = 0, ndof = 0, nnz = 0, n_blocks = 0;
     int max_values = 0, max_pairs = 0, max_slots = 0;
+    // blocks (metadata order) and pairs of each element order: contiguous
+    int64_t kind_blk[kKinds + 1] = {0, 0, 0, 0};
+    int64_t kind_pair0[kKinds] = {0, 0, 0};
+    int kmax_values[kKinds] = {0, 0, 0}, kmax_pairs[kKinds] = {0, 0, 0}, kmax_slots[kKinds] = {0, 0, 0};
+    uint32_t* d_pos8[kKinds] = {nullptr, nullptr, nullptr};  // compact positions of the order's pairs, or null
     anifem::BlockCaps caps[kKinds];
     double tune_ms[kKinds][kCandidates] = {};
     bool kind_used[kKinds] = {false, false, false};
@@ -167,6 +174,21 @@ int autotune_kind(anifem_batch* b, int kind, const anifem_batch_mesh& m, hipStre
     return rc;
 }
 
+// The persistent row-block instantiation of an element order: the fixed-size
+// form for P1 (4 DOFs) and P2 (10), the general one for P3; compact or 16-bit
+// value positions.
+const void* kind_kernel(int kind, bool p8) {
+    using namespace anifem;
+    switch (kind * 2 + (p8 ? 1 : 0)) {
+        case 0: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false, 4>);
+        case 1: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, true, 4>);
+        case 2: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, false, 10>);
+        case 3: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, true, 10>);
+        case 4: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false>);
+        default: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, true>);
+    }
+}
+
 int capture(anifem_batch* b, const double* kappa, const double* f, double* val, double* rhs) {
     using namespace anifem;
     BlockArgs a{};
@@ -230,23 +252,43 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
                               : block_lds_layout(a.max_values, a.max_pairs, a.max_slots, 8, 0, 0).total;
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
 
-    // persistent row-block pass (element kinds restaged per workgroup range):
-    // occupancy x CUs workgroups, whole XCD groups
-    // position words per pair: the shortest instantiation that holds the
-    // largest element of the kinds in use
-    int max_nf = 0;
-    for (int k = 0; k < kKinds; ++k)
-        if (b->kind_used[k]) max_nf = std::max(max_nf, a.kinds[k].nf);
-    const void* pkernel = max_nf <= 4    ? reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double>)
-                          : max_nf <= 10 ? reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double>)
-                                         : reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double>);
-    int64_t pgrid = 0;
-    if (staged && b->persistent) {
+    // persistent row-block passes, one per element order over that order's
+    // blocks: occupancy x CUs workgroups, whole XCD groups; LDS sized by the
+    // order's own block budgets
+    struct KindLaunch {
+        const void* fn = nullptr;
+        BlockArgs a{};
+        size_t lds = 0;
+        int64_t grid = 0;
+    };
+    KindLaunch kl[kKinds];
+    bool all_persistent = b->persistent;
+    for (int k = 0; k < kKinds && all_persistent; ++k) {
+        if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
+        KindLaunch& L = kl[k];
+        L.a = a;
+        L.a.meta = b->d_meta + b->kind_blk[k];
+        L.a.n_blocks = b->kind_blk[k + 1] - b->kind_blk[k];
+        L.a.max_values = b->kmax_values[k];
+        L.a.max_pairs = b->kmax_pairs[k];
+        L.a.max_slots = b->kmax_slots[k];
+        L.a.max_R_rows = a.kinds[k].R_rows;
+        L.a.max_RF_rows = a.kinds[k].RF_rows;
+        L.a.pair_pos8 = b->d_pos8[k];
+        L.a.pos8_base = b->kind_pair0[k];
+        L.lds = block_lds_layout(L.a.max_values, L.a.max_pairs, L.a.max_slots, 8, L.a.max_R_rows * pad16<double>(6),
+                                 L.a.max_RF_rows * pad16<double>(1))
+                    .total;
+        if (L.lds > (size_t)max_lds / 2) {  // the pass stages R: two workgroups per CU at least
+            all_persistent = false;
+            break;
+        }
+        L.fn = kind_kernel(k, b->d_pos8[k] != nullptr);
         int occ = 0, n_cu = 0;
-        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pkernel, kBlock, lds));
+        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, L.fn, kBlock, L.lds));
         ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-        pgrid = std::min<int64_t>((int64_t)std::max(occ, 1) * std::max(n_cu, 1), b->n_blocks);
-        pgrid = std::max<int64_t>(8, pgrid / 8 * 8);
+        L.grid = std::min<int64_t>((int64_t)std::max(occ, 1) * std::max(n_cu, 1), L.a.n_blocks);
+        L.grid = std::max<int64_t>(8, L.grid / 8 * 8);
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -259,24 +301,25 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
                        b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
-    if (pgrid > 0 && max_nf <= 4)
-        hipLaunchKernelGGL((row_block_persistent<6, 1, 2, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
-                           (const double*)b->d_K);
-    else if (pgrid > 0 && max_nf <= 10)
-        hipLaunchKernelGGL((row_block_persistent<6, 1, 5, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
-                           (const double*)b->d_K);
-    else if (pgrid > 0)
-        hipLaunchKernelGGL((row_block_persistent<6, 1, 10, double>), dim3((unsigned)pgrid), dim3(kBlock), lds, s, a,
-                           (const double*)b->d_K);
-    else if (staged)
-        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a,
-                           (const double*)b->d_K);
-    else
-        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, false>), dim3(grid), dim3(kBlock), lds, s, a,
-                           (const double*)b->d_K);
+    const double* Kc = b->d_K;
+    hipError_t le = hipSuccess;
+    if (all_persistent) {
+        for (int k = 0; k < kKinds && le == hipSuccess; ++k) {
+            if (!kl[k].fn) continue;
+            void* args[] = {&kl[k].a, &Kc};
+            le = hipLaunchKernel(kl[k].fn, dim3((unsigned)kl[k].grid), dim3(kBlock), args, kl[k].lds, s);
+        }
+    } else if (staged) {
+        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, true>), dim3(grid), dim3(kBlock), lds, s, a, Kc);
+    } else {
+        hipLaunchKernelGGL((row_block_kernel<6, 1, 1, 1, double, false>), dim3(grid), dim3(kBlock), lds, s, a, Kc);
+    }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
-    if (e != hipSuccess) return ANIFEM_ERR_HIP;
+    if (e != hipSuccess || le != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return ANIFEM_ERR_HIP;
+    }
     b->graph = g;
     ANIFEM_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->key[0] = kappa;
@@ -339,6 +382,8 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     // per-mesh setups on host threads, each with a stream of its own for the
     // device phase
     std::vector<AssemblySetup> Ss(n_meshes);
+    std::vector<std::vector<uint32_t>> P8(n_meshes);  // compact positions of each mesh, when they fit
+    std::vector<int> p8_words(n_meshes, 0);
     {
         std::vector<int> rcs(n_meshes, ANIFEM_OK);
         std::atomic<int> next{0};
@@ -358,6 +403,7 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
                     const anifem_batch_mesh& mj = meshes[j];
                     anifem_mesh_desc mdj{mj.n_nodes, mj.n_tets, mj.coords, mj.tets};
                     rcs[j] = build_setup_device(mdj, problem_of(mj.fem, dirichlet), Ss[j], b->caps[kind_of(mj.fem)], ws);
+                    if (!rcs[j]) p8_words[j] = compact_positions(Ss[j], P8[j]);
                     // what the concatenation does not read
                     std::vector<int32_t>().swap(Ss[j].slot);
                     std::vector<int32_t>().swap(Ss[j].dofs);
@@ -374,7 +420,6 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     b->dof_off.assign(1, 0);
     b->nnz_off.assign(1, 0);
     b->rp_off.assign(1, 0);
-    std::vector<int64_t> pair_off(1, 0), pos_off(1, 0), meta_off(1, 0);
     for (int i = 0; i < n_meshes; ++i) {
         const AssemblySetup& S = Ss[i];
         b->tet_off.push_back(b->tet_off.back() + S.n_tets);
@@ -382,20 +427,55 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
         b->dof_off.push_back(b->dof_off.back() + S.ndof);
         b->nnz_off.push_back(b->nnz_off.back() + S.nnz);
         b->rp_off.push_back(b->rp_off.back() + S.ndof + 1);
-        pair_off.push_back(pair_off.back() + (int64_t)S.pair_tet.size());
-        pos_off.push_back(pos_off.back() + (int64_t)S.pair_pos.size());
-        meta_off.push_back(meta_off.back() + (int64_t)S.blk_meta.size());
         b->max_values = std::max<int>(b->max_values, (int)S.max_block_values);
         b->max_pairs = std::max<int>(b->max_pairs, (int)S.max_block_pairs);
         b->max_slots = std::max<int>(b->max_slots, (int)S.max_block_slots);
         if (b->tet_off.back() >= INT32_MAX || b->dof_off.back() >= INT32_MAX) return ANIFEM_ERR_ARG;
     }
+    // pairs, value positions and block metadata in element-order order (the
+    // meshes of one order in mesh order), so that each order's blocks and
+    // pairs are contiguous; slots, rows and tets stay in mesh order
+    std::vector<int64_t> pair_off(n_meshes), pos_off(n_meshes), meta_off(n_meshes), p8_off(n_meshes, 0);
+    std::vector<uint32_t> P8k[kKinds];
+    {
+        int64_t np = 0, nq = 0, nm = 0;
+        for (int k = 0; k < kKinds; ++k) {
+            b->kind_blk[k] = nm;
+            b->kind_pair0[k] = np;
+            const int words = (2 + (k == 0 ? 4 : k == 1 ? 10 : 20) + 3) / 4;
+            bool p8 = b->kind_used[k];
+            int64_t n8 = 0;
+            for (int i = 0; i < n_meshes; ++i) {
+                if (kind_of(meshes[i].fem) != k) continue;
+                const AssemblySetup& S = Ss[i];
+                pair_off[i] = np;
+                pos_off[i] = nq;
+                meta_off[i] = nm;
+                p8_off[i] = n8;
+                np += (int64_t)S.pair_tet.size();
+                nq += (int64_t)S.pair_pos.size();
+                nm += (int64_t)S.blk_meta.size();
+                n8 += (int64_t)S.pair_tet.size() * words;
+                p8 = p8 && p8_words[i] == words;
+                b->kmax_values[k] = std::max<int>(b->kmax_values[k], (int)S.max_block_values);
+                b->kmax_pairs[k] = std::max<int>(b->kmax_pairs[k], (int)S.max_block_pairs);
+                b->kmax_slots[k] = std::max<int>(b->kmax_slots[k], (int)S.max_block_slots);
+            }
+            if (p8) P8k[k].resize(n8);
+        }
+        b->kind_blk[kKinds] = nm;
+    }
+    int64_t pos_total = 0, pair_total = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+        pair_total += (int64_t)Ss[i].pair_tet.size();
+        pos_total += (int64_t)Ss[i].pair_pos.size();
+    }
     coords.resize(3 * (size_t)b->node_off.back());
     tets.resize(4 * (size_t)b->tet_off.back());
-    meta.resize(meta_off.back());
-    pair_tet.resize(pair_off.back());
-    pair_ib.resize(pair_off.back());
-    pair_pos.resize(pos_off.back());
+    meta.resize(b->kind_blk[kKinds]);
+    pair_tet.resize(pair_total);
+    pair_ib.resize(pair_total);
+    pair_pos.resize(pos_total);
     sstart.resize(b->nnz_off.back());
     pstart.resize(b->dof_off.back());
     row_ptr.resize(b->rp_off.back());
@@ -422,6 +502,8 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
             for (size_t u = 0; u < S.pair_tet.size(); ++u) pair_tet[pair_off[i] + u] = (int32_t)(S.pair_tet[u] + t0);
             std::copy(S.pair_ib.begin(), S.pair_ib.end(), pair_ib.begin() + pair_off[i]);
             std::copy(S.pair_pos.begin(), S.pair_pos.end(), pair_pos.begin() + pos_off[i]);
+            if (!P8k[k].empty()) std::copy(P8[i].begin(), P8[i].end(), P8k[k].begin() + p8_off[i]);
+            std::vector<uint32_t>().swap(P8[i]);
             std::copy(S.sstart.begin(), S.sstart.end(), sstart.begin() + z0);
             std::copy(S.pstart.begin(), S.pstart.end(), pstart.begin() + d0);
             std::copy(S.row_ptr.begin(), S.row_ptr.end(), row_ptr.begin() + b->rp_off[i]);
@@ -446,6 +528,8 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     if ((rc = upload_b(p, &p->d_pair_tet, pair_tet, s))) return rc;
     if ((rc = upload_b(p, &p->d_pair_ib, pair_ib, s))) return rc;
     if ((rc = upload_b(p, &p->d_pair_pos, pair_pos, s))) return rc;
+    for (int k = 0; k < kKinds; ++k)
+        if (!P8k[k].empty() && (rc = upload_b(p, &p->d_pos8[k], P8k[k], s))) return rc;
     sstart.push_back(0);  // padding: slot starts are read as aligned 32-bit words
     if ((rc = upload_b(p, &p->d_sstart, sstart, s))) return rc;
     if ((rc = upload_b(p, &p->d_pstart, pstart, s))) return rc;
