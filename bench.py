@@ -212,7 +212,8 @@ def bench_batched(args, torch):
     assert st.code == 0
     ms = timed_events(lambda: B.run(kappa, fv, val, rhs, status=False), 10, 2, torch)
     tuning = {f"P{f - 1}": {"budget_slots_entries_pairs": [int(x) for x in B.tuning(f)[0]],
-                            "candidate_ms": [float(x) for x in B.tuning(f)[1]]} for f in (2, 3, 4)}
+                            "candidate_ms": [float(x) for x in B.tuning(f)[1]],
+                            "launch_shape": B.launch_tuning(f)} for f in (2, 3, 4)}
     nf_of = {af.FEM_P1: 4, af.FEM_P2: 10, af.FEM_P3: 20}
     per_mesh_tets = [6 * int(m) ** 3 for m in sides]
     alg = sum(assembly_bytes(nt, nf_of[int(fm)], 8, 0, 0) for nt, fm in zip(per_mesh_tets, fems)) \

@@ -495,6 +495,13 @@ int anifem_batch_info(const anifem_batch* ctx, int mesh, int64_t* tet_offset, in
 int anifem_batch_csr(const anifem_batch* ctx, int mesh, const int64_t** row_ptr, const int32_t** col_idx);
 /* chosen budgets {slots, entries, pairs} per block and the timed candidates (ms; -1: not applicable) */
 int anifem_batch_tuning(const anifem_batch* ctx, int fem, int32_t* caps, double* ms);
+/* Launch shape of element order fem's row-block pass: shape = {fixed-size
+ * phase A (P1 / P2), compact value positions, workgroups per CU (0: the
+ * occupancy)}; with autotune, timed at the first anifem_batch_run on the
+ * batch's own blocks over both forms, both position lists where the order
+ * has compact ones, and the occupancy and the two counts below it; ms: its
+ * time per launch (-1 until timed).  No counterpart in the reference. */
+int anifem_batch_launch_tuning(const anifem_batch* ctx, int fem, int32_t* shape, double* ms);
 /* Row-block kernel choice of the batch's graph (default 1: the persistent
  * pass, element kinds restaged per workgroup range; 0: one workgroup per
  * block).  Both sum in the same order: the results are bitwise identical. */

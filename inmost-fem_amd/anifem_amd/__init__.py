@@ -162,6 +162,7 @@ SIGNATURES = {
     "anifem_batch_info": (_I32, [_VP, _I32, _VP, _VP, _VP, _VP, _VP]),
     "anifem_batch_csr": (_I32, [_VP, _I32, _VP, _VP]),
     "anifem_batch_tuning": (_I32, [_VP, _I32, _VP, _VP]),
+    "anifem_batch_launch_tuning": (_I32, [_VP, _I32, _VP, _VP]),
     "anifem_batch_set_persistent": (_I32, [_VP, _I32]),
     "anifem_batch_run": (_I32, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(anifem_status), _VP]),
     "anifem_pcg_factored": (_I32, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _I64,
@@ -861,6 +862,16 @@ class Batch:
         ms = np.zeros(4)
         _check(lib().anifem_batch_tuning(self._h, fem, _ptr(caps), _ptr(ms)), "anifem_batch_tuning")
         return caps, ms
+
+    def launch_tuning(self, fem):
+        """Launch shape of order fem's row-block pass: fixed-size form,
+        compact positions, workgroups per CU (0: occupancy), ms per launch
+        (-1 until timed at the first run)."""
+        shape = np.zeros(3, np.int32)
+        ms = ctypes.c_double()
+        _check(lib().anifem_batch_launch_tuning(self._h, fem, _ptr(shape), ctypes.byref(ms)),
+               "anifem_batch_launch_tuning")
+        return {"fixed": int(shape[0]), "compact": int(shape[1]), "workgroups_per_cu": int(shape[2]), "ms": ms.value}
 
     def set_persistent(self, on=True):
         _check(lib().anifem_batch_set_persistent(self._h, 1 if on else 0), "anifem_batch_set_persistent")

@@ -16,9 +16,13 @@
 // are bitwise identical to assembling that mesh alone.  The launch sequence
 // is captured once in a HIP graph and replayed.
 //
-// Autotuning: the row-block budgets (slots / entries / pairs per block) are
-// chosen per element order by timing candidate budgets on the GPU on a
-// representative mesh of that order.
+// Autotuning, per element order: the row-block budgets (slots / entries /
+// pairs per block) by timing candidate budgets on the GPU on a representative
+// mesh of that order; then, at the first run, the launch shape of the order's
+// row-block pass on the batch's own blocks: the phase-A form (fixed-size or
+// general), the value positions (compact or 16-bit) and the workgroups per CU
+// of the persistent grid (the resident waves per SIMD).  Every candidate sums
+// in the same order, so the choice changes timings only, never values.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +85,11 @@ struct anifem_batch {
     int64_t kind_pair0[kKinds] = {0, 0, 0};
     int kmax_values[kKinds] = {0, 0, 0}, kmax_pairs[kKinds] = {0, 0, 0}, kmax_slots[kKinds] = {0, 0, 0};
     uint32_t* d_pos8[kKinds] = {nullptr, nullptr, nullptr};  // compact positions of the order's pairs, or null
+    // launch shape per order: fixed-size form, compact positions, workgroups
+    // per CU (0: occupancy) and its time per launch (ms; -1 until timed)
+    bool tune_launch = false, launch_tuned = false;
+    int shape[kKinds][3] = {{1, 1, 0}, {1, 1, 0}, {0, 1, 0}};
+    double shape_ms[kKinds] = {-1, -1, -1};
     anifem::BlockCaps caps[kKinds];
     double tune_ms[kKinds][kCandidates] = {};
     bool kind_used[kKinds] = {false, false, false};
@@ -177,19 +186,38 @@ int autotune_kind(anifem_batch* b, int kind, const anifem_batch_mesh& m, hipStre
 // The persistent row-block instantiation of an element order: the fixed-size
 // form for P1 (4 DOFs) and P2 (10), the general one for P3; compact or 16-bit
 // value positions.
-const void* kind_kernel(int kind, bool p8) {
+const void* kind_kernel(int kind, bool p8, bool fixed) {
     using namespace anifem;
-    switch (kind * 2 + (p8 ? 1 : 0)) {
-        case 0: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false, 4>);
-        case 1: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, true, 4>);
-        case 2: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, false, 10>);
-        case 3: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, true, 10>);
-        case 4: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false>);
+    const int v = kind * 4 + (fixed && kind < 2 ? 2 : 0) + (p8 ? 1 : 0);
+    switch (v) {
+        case 0: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false>);
+        case 1: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, true>);
+        case 2: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false, 4>);
+        case 3: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, true, 4>);
+        case 4: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, false>);
+        case 5: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, true>);
+        case 6: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, false, 10>);
+        case 7: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 5, double, true, 10>);
+        case 8: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false>);
         default: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, true>);
     }
 }
 
-int capture(anifem_batch* b, const double* kappa, const double* f, double* val, double* rhs) {
+// Persistent grid of a row-block launch: wpc workgroups per CU (at most the
+// occupancy; 0: the occupancy), whole XCD groups, at most one per block.
+int persistent_grid(const void* fn, size_t lds, int wpc, int64_t n_blocks, int device, int64_t* grid, int* occ_out) {
+    int occ = 0, n_cu = 0;
+    ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, anifem::kBlock, lds));
+    ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    occ = std::max(occ, 1);
+    const int w = wpc > 0 ? std::min(wpc, occ) : occ;
+    int64_t G = std::min<int64_t>((int64_t)w * std::max(n_cu, 1), n_blocks);
+    *grid = std::max<int64_t>(8, G / 8 * 8);
+    if (occ_out) *occ_out = occ;
+    return ANIFEM_OK;
+}
+
+int capture(anifem_batch* b, const double* kappa, const double* f, double* val, double* rhs, hipStream_t rs) {
     using namespace anifem;
     BlockArgs a{};
     int rc;
@@ -253,8 +281,9 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     if (lds > (size_t)max_lds) return ANIFEM_ERR_DIMS;
 
     // persistent row-block passes, one per element order over that order's
-    // blocks: occupancy x CUs workgroups, whole XCD groups; LDS sized by the
-    // order's own block budgets
+    // blocks, in the order's launch shape (default: fixed-size form for P1 and
+    // P2, compact positions where present, occupancy x CUs workgroups); LDS
+    // sized by the order's own block budgets
     struct KindLaunch {
         const void* fn = nullptr;
         BlockArgs a{};
@@ -283,12 +312,68 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
             all_persistent = false;
             break;
         }
-        L.fn = kind_kernel(k, b->d_pos8[k] != nullptr);
-        int occ = 0, n_cu = 0;
-        ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, L.fn, kBlock, L.lds));
-        ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-        L.grid = std::min<int64_t>((int64_t)std::max(occ, 1) * std::max(n_cu, 1), L.a.n_blocks);
-        L.grid = std::max<int64_t>(8, L.grid / 8 * 8);
+        if (!b->d_pos8[k]) b->shape[k][1] = 0;
+        if (k == 2) b->shape[k][0] = 0;
+    }
+    const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
+    // launch shapes timed once on the batch's own blocks (the geometry rows of
+    // this call first; every candidate writes the same values)
+    if (all_persistent && b->tune_launch && !b->launch_tuned) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ANIFEM_HIP_CHECK(hipEventCreate(&e0));
+        ANIFEM_HIP_CHECK(hipEventCreate(&e1));
+        hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, rs, b->d_status);
+        hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, rs, a,
+                           b->d_K, b->n_tets);
+        const double* Kt = b->d_K;
+        int rc = ANIFEM_OK;
+        for (int k = 0; k < kKinds && !rc; ++k) {
+            if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
+            KindLaunch& L = kl[k];
+            double best = 1e300;
+            for (int fx = (k < 2 ? 1 : 0); fx >= 0 && !rc; --fx)
+                for (int p8 = (b->d_pos8[k] ? 1 : 0); p8 >= 0 && !rc; --p8) {
+                    const void* fn = kind_kernel(k, p8 != 0, fx != 0);
+                    int occ = 1;
+                    int64_t G = 0;
+                    if ((rc = persistent_grid(fn, L.lds, 0, L.a.n_blocks, b->device, &G, &occ))) break;
+                    for (int w = occ; w >= std::max(1, occ - 2) && !rc; --w) {
+                        if ((rc = persistent_grid(fn, L.lds, w, L.a.n_blocks, b->device, &G, nullptr))) break;
+                        void* args[] = {&L.a, &Kt};
+                        const int reps = 3;
+                        hipError_t le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, L.lds, rs);
+                        le0 = le0 == hipSuccess ? hipEventRecord(e0, rs) : le0;
+                        for (int r = 0; r < reps && le0 == hipSuccess; ++r)
+                            le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, L.lds, rs);
+                        le0 = le0 == hipSuccess ? hipEventRecord(e1, rs) : le0;
+                        le0 = le0 == hipSuccess ? hipEventSynchronize(e1) : le0;
+                        float t = 0.f;
+                        le0 = le0 == hipSuccess ? hipEventElapsedTime(&t, e0, e1) : le0;
+                        if (le0 != hipSuccess) {
+                            rc = ANIFEM_ERR_HIP;
+                            break;
+                        }
+                        if (t / reps < best) {
+                            best = t / reps;
+                            b->shape[k][0] = fx;
+                            b->shape[k][1] = p8;
+                            b->shape[k][2] = w;
+                            b->shape_ms[k] = best;
+                        }
+                    }
+                }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        b->launch_tuned = true;
+    }
+    for (int k = 0; k < kKinds && all_persistent; ++k) {
+        if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
+        KindLaunch& L = kl[k];
+        L.fn = kind_kernel(k, b->shape[k][1] != 0, b->shape[k][0] != 0);
+        int rc = persistent_grid(L.fn, L.lds, b->shape[k][2], L.a.n_blocks, b->device, &L.grid, nullptr);
+        if (rc) return rc;
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -297,7 +382,6 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     hipStream_t s = b->cap_stream;
     ANIFEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, s, b->d_status);
-    const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
                        b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
@@ -342,6 +426,7 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
     std::unique_ptr<anifem_batch> b(new anifem_batch());
     b->n_meshes = n_meshes;
     b->dirichlet = dirichlet;
+    b->tune_launch = autotune != 0;
     ANIFEM_HIP_CHECK(hipGetDevice(&b->device));
     for (int i = 0; i < n_meshes; ++i) {
         const anifem_batch_mesh& m = meshes[i];
@@ -595,6 +680,15 @@ int anifem_batch_tuning(const anifem_batch* b, int fem, int32_t* caps, double* m
     return ANIFEM_OK;
 }
 
+int anifem_batch_launch_tuning(const anifem_batch* b, int fem, int32_t* shape, double* ms) {
+    if (!b || fem < ANIFEM_FEM_P1 || fem > ANIFEM_FEM_P3) return ANIFEM_ERR_ARG;
+    const int k = kind_of(fem);
+    if (shape)
+        for (int i = 0; i < 3; ++i) shape[i] = b->shape[k][i];
+    if (ms) *ms = b->shape_ms[k];
+    return ANIFEM_OK;
+}
+
 int anifem_batch_set_persistent(anifem_batch* b, int on) {
     if (!b) return ANIFEM_ERR_ARG;
     if (b->persistent != (on != 0)) {
@@ -612,7 +706,7 @@ int anifem_batch_run(anifem_batch* b, const double* kappa, const double* f, doub
     if (!b || !kappa || !f || !val || !rhs) return ANIFEM_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (!b->exec || b->key[0] != kappa || b->key[1] != f || b->key[2] != val || b->key[3] != rhs) {
-        const int rc = capture(b, kappa, f, val, rhs);
+        const int rc = capture(b, kappa, f, val, rhs, s);
         if (rc) return rc;
     }
     ANIFEM_HIP_CHECK(hipGraphLaunch(b->exec, s));

@@ -80,6 +80,29 @@ def test_autotuner_reports_budgets():
         assert ms[k] == ms[ms > 0].min()
 
 
+def test_launch_shapes_timed_at_first_run_and_values_unchanged():
+    """The launch-shape tuner times each order's row-block pass at the first
+    run (fixed-size or general form, compact or 16-bit positions, workgroups
+    per CU); the tuned graph gives the untuned batch's values bit for bit."""
+    import torch
+    meshes, _, kappa = build(SPECS, seed=5)
+    f = torch.ones(1, dtype=torch.float64, device="cuda")
+    B1 = af.Batch(meshes, autotune=True)
+    for fem in (af.FEM_P1, af.FEM_P2, af.FEM_P3):
+        assert B1.launch_tuning(fem)["ms"] == -1
+    v1, r1, st = B1.run(kappa, f)
+    assert st.code == 0
+    for fem in (af.FEM_P1, af.FEM_P2, af.FEM_P3):
+        t = B1.launch_tuning(fem)
+        assert t["ms"] > 0 and t["workgroups_per_cu"] >= 1 and t["fixed"] in (0, 1) and t["compact"] in (0, 1)
+        if fem == af.FEM_P3:
+            assert t["fixed"] == 0
+    B2 = af.Batch(meshes, autotune=False)
+    v2, r2, _ = B2.run(kappa, f)
+    assert B2.launch_tuning(af.FEM_P2)["ms"] == -1
+    assert torch.equal(v1, v2) and torch.equal(r1, r2)
+
+
 def test_batch_persistent_and_per_block_bitwise_equal():
     """The persistent row-block pass (element kinds restaged as a workgroup's
     block range crosses meshes of another order) against one workgroup per
