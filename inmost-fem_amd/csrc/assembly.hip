@@ -335,16 +335,17 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     }
     ctx->run_kg = Kg;
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
+    const size_t glds = geometry_stage_bytes<NCA, NCB, T>(a.rowlen);
     if constexpr (NCA == 3 && NCB == 3 && DA0 == 3 && DB0 == 3) {
         if (a.lamec)
-            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T, false, true>), dim3((unsigned)gblocks), dim3(kBlock), 0,
-                               s, a, Kg, S.n_tets);
+            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T, false, true>), dim3((unsigned)gblocks), dim3(kBlock),
+                               glds, s, a, Kg, S.n_tets);
         else
-            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, Kg,
+            hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T>), dim3((unsigned)gblocks), dim3(kBlock), glds, s, a, Kg,
                                S.n_tets);
     } else {
-        hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T, SYM>), dim3((unsigned)gblocks), dim3(kBlock), 0, s,
-                           a, Kg, S.n_tets);
+        hipLaunchKernelGGL((geometry_kernel<DA0, DB0, NCA, NCB, T, SYM>), dim3((unsigned)gblocks), dim3(kBlock), glds,
+                           s, a, Kg, S.n_tets);
     }
     ANIFEM_HIP_CHECK(hipGetLastError());
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[1], s));

@@ -172,17 +172,24 @@ __device__ __forceinline__ T lame_entry(const T* Kt, int cb, int ca, const T* r)
 // SYM (3 x 3 tensors of a symmetric K, one component, one carried point): the
 // matrix part of the row is K packed to its 6 distinct values (sym_pack),
 // which the row-block passes contract with the column-summed R.
+// Dynamic LDS of a geometry launch: short rows (scalar spaces, at most 16
+// values) are staged, one row per lane; the launch passes exactly this size.
+template <int NCA, int NCB, typename T>
+__host__ __device__ inline size_t geometry_stage_bytes(int rowlen) {
+    return (NCA * NCB == 1 && rowlen <= 16) ? (size_t)kBlock * rowlen * sizeof(T) : 0;
+}
+
 // LC: compact Lame rows (lame_kblock; a tensor constant over each tet).
 template <int DA0, int DB0, int NCA, int NCB, typename T, bool SYM = false, bool LC = false>
 __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __restrict__ Kg, int64_t n_tets) {
     static_assert(!SYM || (DA0 == 3 && DB0 == 3 && NCA == 1 && NCB == 1), "packed K: 3 x 3 scalar blocks");
     static_assert(!LC || (DA0 == 3 && DB0 == 3 && NCA == 3 && NCB == 3), "compact Lame rows: FemVec<3> gradients");
-    // short rows (scalar spaces, one carried point) are assembled in LDS and
-    // stored by the workgroup as one contiguous range: coalesced stores
-    // instead of one strided store per value and lane
-    constexpr int kStageRow = 16;
-    __shared__ T stage[NCA * NCB == 1 ? kBlock * kStageRow : 1];
-    const bool staged = NCA * NCB == 1 && a.rowlen <= kStageRow;
+    // short rows are assembled in LDS (sized to the row, so that more
+    // workgroups fit a CU) and stored by the workgroup as one contiguous
+    // range: coalesced stores instead of one strided store per value and lane
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* stage = reinterpret_cast<T*>(smem_raw);
+    const bool staged = geometry_stage_bytes<NCA, NCB, T>(a.rowlen) > 0;
     const int64_t r0 = blockIdx.x * (int64_t)kBlock;
     const int64_t r = r0 + threadIdx.x;
     int degenerate = 0;

@@ -316,6 +316,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         if (k == 2) b->shape[k][0] = 0;
     }
     const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
+    const size_t glds = geometry_stage_bytes<1, 1, double>(a.rowlen);
     // launch shapes timed once on the batch's own blocks (the geometry rows of
     // this call first; every candidate writes the same values)
     if (all_persistent && b->tune_launch && !b->launch_tuned) {
@@ -323,8 +324,8 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         ANIFEM_HIP_CHECK(hipEventCreate(&e0));
         ANIFEM_HIP_CHECK(hipEventCreate(&e1));
         hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, rs, b->d_status);
-        hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, rs, a,
-                           b->d_K, b->n_tets);
+        hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), glds, rs,
+                           a, b->d_K, b->n_tets);
         const double* Kt = b->d_K;
         int rc = ANIFEM_OK;
         for (int k = 0; k < kKinds && !rc; ++k) {
@@ -382,8 +383,8 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     hipStream_t s = b->cap_stream;
     ANIFEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_status_clear, dim3(1), dim3(64), 0, s, b->d_status);
-    hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), 0, s, a, b->d_K,
-                       b->n_tets);
+    hipLaunchKernelGGL((geometry_kernel<3, 3, 1, 1, double, true>), dim3((unsigned)gblocks), dim3(kBlock), glds, s, a,
+                       b->d_K, b->n_tets);
     const unsigned grid = (unsigned)(8 * ((b->n_blocks + 7) / 8));
     const double* Kc = b->d_K;
     hipError_t le = hipSuccess;
