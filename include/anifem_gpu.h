@@ -493,7 +493,11 @@ int anifem_batch_info(const anifem_batch* ctx, int mesh, int64_t* tet_offset, in
                       int64_t* nnz_offset, int64_t* ndof, int64_t* nnz);
 /* device views of mesh i's own CSR pattern (row_ptr starts at 0) */
 int anifem_batch_csr(const anifem_batch* ctx, int mesh, const int64_t** row_ptr, const int32_t** col_idx);
-/* chosen budgets {slots, entries, pairs} per block and the timed candidates (ms; -1: not applicable) */
+/* chosen budgets {slots, entries, pairs} per block and the times of the
+ * ANIFEM_BATCH_BUDGET_CANDIDATES candidates (ms[] of that length; -1: not
+ * applicable), in the order {2048, 1024, 128}, {2048, 2048, 256},
+ * {4096, 4096, 256}, {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256} */
+#define ANIFEM_BATCH_BUDGET_CANDIDATES 6
 int anifem_batch_tuning(const anifem_batch* ctx, int fem, int32_t* caps, double* ms);
 /* Launch shape of element order fem's row-block pass: shape = {fixed-size
  * phase A (P1 / P2), compact value positions, workgroups per CU (0: the

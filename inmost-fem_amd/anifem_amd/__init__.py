@@ -808,6 +808,12 @@ class LearnedPreconditioner:
         return res, rc == OK
 
 
+# the batch autotuner's candidate row-block budgets {slots, entries, pairs}
+# (ANIFEM_BATCH_BUDGET_CANDIDATES, in the order of anifem_batch_tuning's times)
+BATCH_BUDGET_CANDIDATES = [(2048, 1024, 128), (2048, 2048, 256), (4096, 4096, 256), (1024, 1024, 256),
+                           (4096, 3072, 256), (4096, 2560, 256)]
+
+
 class Batch:
     """Batched scalar-diffusion assembly of many meshes (anifem_batch_*).
     meshes: list of (coords (n,3) float64, tets (m,4) int32, fem)."""
@@ -859,7 +865,7 @@ class Batch:
 
     def tuning(self, fem):
         caps = np.zeros(3, np.int32)
-        ms = np.zeros(4)
+        ms = np.zeros(len(BATCH_BUDGET_CANDIDATES))
         _check(lib().anifem_batch_tuning(self._h, fem, _ptr(caps), _ptr(ms)), "anifem_batch_tuning")
         return caps, ms
 

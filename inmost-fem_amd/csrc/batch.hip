@@ -45,9 +45,12 @@ int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, co
 namespace {
 
 constexpr int kKinds = 3;  // P1, P2, P3
-constexpr int kCandidates = 4;
-const anifem::BlockCaps kCandidateCaps[kCandidates] = {
-    {2048, 1024, 128}, {2048, 2048, 256}, {4096, 4096, 256}, {1024, 1024, 256}};
+// Candidate budgets {slots, entries, pairs}: the last two let three P3
+// workgroups (20 entries per pair) share a CU's LDS, with fuller blocks of
+// two workgroups per CU as the alternative.
+constexpr int kCandidates = ANIFEM_BATCH_BUDGET_CANDIDATES;
+const anifem::BlockCaps kCandidateCaps[kCandidates] = {{2048, 1024, 128}, {2048, 2048, 256}, {4096, 4096, 256},
+                                                       {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256}};
 
 int kind_of(int fem) { return fem - ANIFEM_FEM_P1; }
 int order_of(int fem) { return fem == ANIFEM_FEM_P3 ? 4 : 2; }

@@ -70,13 +70,13 @@ def test_batch_matches_single_mesh_assembly_bitwise():
 def test_autotuner_reports_budgets():
     meshes, _, _ = build([(6, af.FEM_P1), (5, af.FEM_P2), (3, af.FEM_P3)])
     B = af.Batch(meshes, autotune=True)
-    cands = {(2048, 1024, 128), (2048, 2048, 256), (4096, 4096, 256), (1024, 1024, 256)}
+    cands = af.BATCH_BUDGET_CANDIDATES
+    assert len(cands) == 6
     for fem in (af.FEM_P1, af.FEM_P2, af.FEM_P3):
         caps, ms = B.tuning(fem)
-        assert tuple(int(x) for x in caps) in cands
+        assert tuple(int(x) for x in caps) in cands and len(ms) == len(cands)
         assert np.all((ms > 0) | (ms == -1)) and np.any(ms > 0)
-        k = [i for i, c in enumerate([(2048, 1024, 128), (2048, 2048, 256), (4096, 4096, 256),
-                                      (1024, 1024, 256)]) if c == tuple(int(x) for x in caps)][0]
+        k = cands.index(tuple(int(x) for x in caps))
         assert ms[k] == ms[ms > 0].min()
 
 
