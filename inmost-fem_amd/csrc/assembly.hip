@@ -549,7 +549,10 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
         ctx->prob.trial = ctx->prob.test = anifem_op{ANIFEM_IDEN, vars->var[0].fem, vars->var[0].ncomp};
         ctx->vars_only = true;
         const BlockCaps c = caps ? *caps : default_caps(nf);
-        return build_setup_vars(*mesh, vars->n_var, vars->var, *prob, ctx->S, c);
+        const int rc = build_setup_vars(*mesh, vars->n_var, vars->var, *prob, ctx->S, c);
+        if (rc != ANIFEM_ERR_DIMS || caps) return rc;
+        ctx->S = AssemblySetup();  // the default budgets, then wider ones when one row exceeds them
+        return build_setup_vars(*mesh, vars->n_var, vars->var, *prob, ctx->S, wide_caps(nf));
     }
     if (!op_valid(prob->trial) || !op_valid(prob->test)) return ANIFEM_ERR_ARG;
     // the global assembly numbers the DOFs of the spaces of space_layout
@@ -558,7 +561,12 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
     ctx->prob.dirichlet_dof_mask = nullptr;
     const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
     const BlockCaps c = caps ? *caps : default_caps(nf);
-    return on_device ? build_setup_device(*mesh, *prob, ctx->S, c, stream) : build_setup(*mesh, *prob, ctx->S, c);
+    const int rc =
+        on_device ? build_setup_device(*mesh, *prob, ctx->S, c, stream) : build_setup(*mesh, *prob, ctx->S, c);
+    if (rc != ANIFEM_ERR_DIMS || caps) return rc;
+    ctx->S = AssemblySetup();  // the default budgets, then wider ones when one row exceeds them
+    const BlockCaps w = wide_caps(nf);
+    return on_device ? build_setup_device(*mesh, *prob, ctx->S, w, stream) : build_setup(*mesh, *prob, ctx->S, w);
 }
 
 }  // namespace
@@ -567,6 +575,17 @@ namespace anifem {
 int assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob,
                          const BlockCaps& caps, void* stream, const anifem_vars_desc* vars = nullptr);
 }
+
+namespace {
+// The default budgets, then wider ones when a single row exceeds them.
+int create_default(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob, int nf,
+                   void* stream, const anifem_vars_desc* vars) {
+    using namespace anifem;
+    const int rc = assembly_create_caps(out, mesh, prob, default_caps(nf), stream, vars);
+    if (rc != ANIFEM_ERR_DIMS) return rc;
+    return assembly_create_caps(out, mesh, prob, wide_caps(nf), stream, vars);
+}
+}  // namespace
 
 extern "C" int anifem_assembly_create_vars(anifem_assembly** out, const anifem_mesh_desc* mesh,
                                            const anifem_vars_desc* vars, const anifem_problem_desc* prob,
@@ -578,14 +597,14 @@ extern "C" int anifem_assembly_create_vars(anifem_assembly** out, const anifem_m
         if (!space_layout(vars->var[v].fem).valid || vars->var[v].ncomp < 1) return ANIFEM_ERR_ARG;
         nf += nf_scalar(vars->var[v].fem) * vars->var[v].ncomp;
     }
-    return assembly_create_caps(out, mesh, prob, default_caps(nf), stream, vars);
+    return create_default(out, mesh, prob, nf, stream, vars);
 }
 
 extern "C" int anifem_assembly_create(anifem_assembly** out, const anifem_mesh_desc* mesh,
                                       const anifem_problem_desc* prob, void* stream) {
     if (!prob || !anifem::op_valid(prob->test) || !anifem::space_layout(prob->test.fem).valid) return ANIFEM_ERR_ARG;
     const int nf = anifem::nf_scalar(prob->test.fem) * prob->test.ncomp;
-    return anifem::assembly_create_caps(out, mesh, prob, anifem::default_caps(nf), stream);
+    return create_default(out, mesh, prob, nf, stream, nullptr);
 }
 
 extern "C" int anifem_assembly_create_budget(anifem_assembly** out, const anifem_mesh_desc* mesh,

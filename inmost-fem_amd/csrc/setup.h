@@ -22,7 +22,8 @@ struct BlockCaps {
 };
 // Default budgets by entries per (row, tet) pair: a block of 256 pairs of nf
 // entries needs about 256 nf values, so larger element matrices get larger
-// value / slot budgets (measured: P3 at 4096 / 4096, P2^3 at 8192 values).
+// value / slot budgets (measured: P2^3 at 8192 values; P3 at 3072, where the
+// LDS of three workgroups fits a CU: 0.279 against 0.322 ms at 4096).
 inline BlockCaps default_caps(int nf) {
     BlockCaps c;
     if (nf > 20) {
@@ -30,8 +31,16 @@ inline BlockCaps default_caps(int nf) {
         c.max_values = 8192;
     } else if (nf > 10) {
         c.max_slots = 4096;
-        c.max_values = 4096;
+        c.max_values = 3072;
     }
+    return c;
+}
+// Budgets for a mesh one of whose rows exceeds default_caps (a node with
+// more incident cells than the default blocks hold: with P3, more than 153).
+inline BlockCaps wide_caps(int nf) {
+    BlockCaps c;
+    c.max_slots = 8192;
+    c.max_values = nf > 20 ? 16384 : 8192;
     return c;
 }
 constexpr uint16_t kPosDropped = 0xFFFF;  // pair_pos of an entry the pattern drops

@@ -310,3 +310,16 @@ def dof_positions(coords, rtets, dofs, nodes):
     """Physical position of every global DOF seen from every cell: (n_tets, nf, 3)."""
     P = coords[rtets]  # (n, 4, 3)
     return np.einsum("iv,tvc->tic", nodes, P)
+
+
+def fan_mesh(m):
+    """2m tets around the origin: a ring of m points in the plane z = 0 joined
+    to (0, 0, 1) and (0, 0, -1)."""
+    th = 2 * np.pi * np.arange(m) / m
+    coords = np.concatenate([np.zeros((1, 3)), np.stack([np.cos(th), np.sin(th), np.zeros(m)], 1),
+                             np.array([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])])
+    k = np.arange(m)
+    r0, r1 = 1 + k, 1 + (k + 1) % m
+    top = np.stack([np.zeros(m, int), r0, r1, np.full(m, m + 1)], 1)
+    bot = np.stack([np.zeros(m, int), r1, r0, np.full(m, m + 2)], 1)
+    return coords, np.concatenate([top, bot]).astype(np.int32)

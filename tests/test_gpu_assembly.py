@@ -83,6 +83,26 @@ def test_scalar_assembly_matches_oracle(fem, dirichlet):
     assert np.array_equal(rhs.cpu().numpy(), rhs2.cpu().numpy())
 
 
+def test_p3_node_beyond_the_default_budget_matches_oracle():
+    """A P3 node with 200 incident tets: its row holds 4000 entries, beyond the
+    default 3072 per block; the context is built with the wider budgets and
+    the assembly equals the oracle's (FP64, within 1e-12 of the row maximum)."""
+    coords, tets = T.fan_mesh(100)
+    n = len(tets)
+    kappa = 10 ** np.random.default_rng(2).uniform(-1, 1, (n, 1))
+    Asm = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P3), order=4, tensor_type=af.TENSOR_SCALAR,
+                       tensor_storage=af.COEF_PER_TET, rhs_order=4, rhs_tensor_type=af.TENSOR_SCALAR,
+                       rhs_tensor_storage=af.COEF_CONST, dirichlet=af.DIRICHLET_NONE)
+    val, rhs, st = Asm.assemble(D=dev(kappa.T.copy()), F=dev(np.ones(1)))
+    assert st.code == 0 and st.n_degenerate == 0
+    S, xyz, A, F = oracle_system(coords, tets, 4, 1, af.TENSOR_SCALAR, kappa, 4, np.ones((1, 1)),
+                                 af.TENSOR_SCALAR, 4, 0)
+    S = T.oracle_assembly(coords, tets, 4, 1)
+    rc, v_or, r_or = T.oracle_scatter(n, S["dofs"].shape[1], S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
+    assert rc == 0
+    compare(val.cpu().numpy(), rhs.cpu().numpy(), S, v_or, r_or)
+
+
 def test_anisotropic_p2_assembly_fp64_and_fp32():
     rng = np.random.default_rng(5)
     coords, tets = af.kuhn_cube(5, 0.2, 8)
