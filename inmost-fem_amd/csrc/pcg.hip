@@ -1110,7 +1110,7 @@ extern "C" int anifem_pcg_factored(int64_t n, const int64_t* row_ptr, const int3
 }
 
 // Learned-preconditioned solve: the network's factor G for this matrix, then
-// flexible PCG with M^-1 = G G^T; the FP64 true residual judges the result
+// flexible PCG with M^-1 = G G^T; the FP64 true residual decides convergence
 // (||b - A x|| <= rtol ||b||, up to 1% for the recursive-residual drift) and
 // Jacobi-PCG from the same initial guess takes over when it fails.
 extern "C" int anifem_pcg_learned(anifem_gnn* g, const double* val, const double* b, double* x, double rtol,
