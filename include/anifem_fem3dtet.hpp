@@ -19,8 +19,10 @@
 #define ANIFEM_FEM3DTET_HPP
 
 #include <cstdint>
+#include <initializer_list>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "anifem_gpu.h"
 
@@ -175,20 +177,199 @@ inline int dim() {
     return anifem_op_dim(Op::c());
 }
 
-// Whole-mesh assembler: PrepareProblem at construction (assembler.inl:194-275),
-// Assemble(matrix, rhs) per run (assembler.inl:314-488).  The test and trial
-// spaces are the Op tags' spaces; the load vector is fem3Dtet<IDEN P0, IDEN
-// test space> of the right-hand-side tensor (examples/tutorials/ex1.cpp:95).
-template <typename OpA, typename OpB>
-class Assembler {
+// ---- Operators and spaces chosen at run time ------------------------------
+// ApplyOp (operators.h:418-533: ApplyOpBase / ApplyOp / ApplyOpFromTemplate)
+// and FemSpace (fem_space.h:404-596) for callers that pick the operator and
+// the space at run time (the runtime fem3Dtet of int_tet.h:145-147).  Here an
+// ApplyOp is the anifem_op descriptor every entry point takes; its Nfa / Dim
+// come from the library, its Order (the polynomial degree of the operator's
+// values: Operator<..>::Order of spaces/*.h) from the space's degree.
+namespace detail {
+inline int space_degree(int fem) {
+    switch (fem) {
+        case FEM_P0: return 0;
+        case FEM_P1: return 1;
+        case FEM_P2: return 2;
+        case FEM_P3: return 3;
+        case FEM_RT0: return 1;
+        case FEM_ND0: return 1;
+        case FEM_CR1: return 1;
+        case FEM_B2: return 2;
+        case FEM_B4: return 4;
+        default: return -1;
+    }
+}
+// vector-valued reference fields (RT0, ND0): 3 components per basis function
+inline int space_dim(int fem) { return fem == FEM_RT0 || fem == FEM_ND0 ? 3 : 1; }
+}  // namespace detail
+
+class ApplyOp {
 public:
-    Assembler(int64_t n_nodes, const double* coords, int64_t n_tets, const int32_t* tets, int order,
-              TensorView D_kind, int rhs_order, TensorView F_kind, int dirichlet,
-              const uint8_t* dirichlet_dof_mask = nullptr, void* stream = nullptr) {
+    ApplyOp() = default;
+    explicit ApplyOp(anifem_op op) : op_(op), set_(true) {}
+    // a descriptor the library does not know has no Nfa / Dim (-1)
+    int Nfa() const { return set_ ? anifem_op_nfa(op_) : -1; }
+    int Dim() const { return set_ ? anifem_op_dim(op_) : -1; }
+    int Order() const {
+        const int d = set_ ? detail::space_degree(op_.fem) : -1;
+        if (d < 0 || Dim() < 0) return -1;
+        return op_.op == IDEN ? d : (d > 0 ? d - 1 : 0);
+    }
+    bool valid() const { return Nfa() > 0 && Dim() > 0; }
+    anifem_op c() const { return op_; }
+    bool operator==(const ApplyOp& o) const {
+        return set_ == o.set_ && (!set_ || (op_.op == o.op_.op && op_.fem == o.op_.fem && op_.ncomp == o.op_.ncomp));
+    }
+    bool operator!=(const ApplyOp& o) const { return !(*this == o); }
+
+private:
+    anifem_op op_{0, 0, 0};
+    bool set_ = false;
+};
+
+// ApplyOpFromTemplate<OPERATOR, FEMTYPE> (operators.h:475-533)
+template <int OPERATOR, typename FEMTYPE>
+inline ApplyOp ApplyOpFromTemplate() {
+    return ApplyOp(Operator<OPERATOR, FEMTYPE>::c());
+}
+
+// The spaces of spaces/*.h by their reference names (P1Space = GenSpace<
+// Polynomial, FEM_P1>, poly_1.h:170; ...), converted to a FemSpace.
+template <int FEM>
+struct GenSpaceTag {
+    static constexpr int fem = FEM;
+};
+using P0Space = GenSpaceTag<FEM_P0>;
+using P1Space = GenSpaceTag<FEM_P1>;
+using P2Space = GenSpaceTag<FEM_P2>;
+using P3Space = GenSpaceTag<FEM_P3>;
+using RT0Space = GenSpaceTag<FEM_RT0>;
+using ND0Space = GenSpaceTag<FEM_ND0>;
+using CR1Space = GenSpaceTag<FEM_CR1>;
+using Bubble2Space = GenSpaceTag<FEM_B2>;
+using Bubble4Space = GenSpaceTag<FEM_B4>;
+using BubbleSpace = Bubble4Space;
+
+// FemSpace: a space of one element type (UniteFemSpace), its vector power
+// (VectorFemSpace, a^k) or a product of such spaces (ComplexFemSpace, a*b):
+// the several variables of anifem_assembly_create_vars, local DOFs
+// concatenated in factor order.
+class FemSpace {
+public:
+    FemSpace() = default;
+    template <int FEM>
+    FemSpace(GenSpaceTag<FEM>) : parts_{anifem_op{IDEN, FEM, 1}} {}
+    bool empty() const { return parts_.empty(); }
+    bool isComplex() const { return parts_.size() > 1; }
+    // number of factors and factor i (a unite or vector space)
+    int nparts() const { return (int)parts_.size(); }
+    FemSpace part(int i) const {
+        FemSpace f;
+        if (i >= 0 && i < nparts()) f.parts_.push_back(parts_[i]);
+        return f;
+    }
+    // dimension of the space's values (fem_space.h: dim())
+    int dim() const {
+        int d = 0;
+        for (const anifem_op& p : parts_) d += p.ncomp * detail::space_dim(p.fem);
+        return d;
+    }
+    // local DOFs per cell
+    int nfa() const {
+        int n = 0;
+        for (const anifem_op& p : parts_) n += anifem_op_nfa(p);
+        return n;
+    }
+    // the operator op on a unite or vector space (getOP, dimOP, orderOP); a
+    // product space has no single operator here: its variables are assembled
+    // through anifem_assembly_create_vars
+    ApplyOp getOP(int op) const {
+        if (parts_.size() != 1) return ApplyOp();
+        return ApplyOp(anifem_op{op, parts_[0].fem, parts_[0].ncomp});
+    }
+    int dimOP(int op) const { return getOP(op).Dim(); }
+    int orderOP(int op) const { return getOP(op).Order(); }
+    // the descriptor of anifem_assembly_create_vars (at most ANIFEM_MAX_VARS factors)
+    int vars(anifem_vars_desc* out) const {
+        if (!out || parts_.empty() || parts_.size() > ANIFEM_MAX_VARS) return ANIFEM_ERR_ARG;
+        *out = anifem_vars_desc{};
+        out->n_var = (int32_t)parts_.size();
+        for (size_t i = 0; i < parts_.size(); ++i) out->var[i] = parts_[i];
+        return ANIFEM_OK;
+    }
+    // a^k: the vector space of k copies of a unite space (VectorFemSpace)
+    friend FemSpace operator^(const FemSpace& a, int k) {
+        FemSpace v;
+        if (a.parts_.size() == 1 && a.parts_[0].ncomp == 1 && k >= 1)
+            v.parts_.push_back(anifem_op{IDEN, a.parts_[0].fem, k});
+        return v;
+    }
+    // a*b: the product of two spaces (ComplexFemSpace), factors concatenated
+    friend FemSpace operator*(const FemSpace& a, const FemSpace& b) {
+        FemSpace c;
+        if (a.empty() || b.empty()) return c;
+        c.parts_ = a.parts_;
+        c.parts_.insert(c.parts_.end(), b.parts_.begin(), b.parts_.end());
+        return c;
+    }
+    bool operator==(const FemSpace& o) const {
+        if (parts_.size() != o.parts_.size()) return false;
+        for (size_t i = 0; i < parts_.size(); ++i)
+            if (parts_[i].fem != o.parts_[i].fem || parts_[i].ncomp != o.parts_[i].ncomp) return false;
+        return true;
+    }
+
+private:
+    std::vector<anifem_op> parts_;  // op unused: IDEN
+};
+
+// The batched calls with operators chosen at run time (int_tet.h:145-147)
+inline int fem3Dtet(const double* xyz, int64_t n_tets, const TensorView& D, const ApplyOp& opA, const ApplyOp& opB,
+                    void* A_out, int order = 5, Precision precision = FP64, void* stream = nullptr) {
+    if (!opA.valid() || !opB.valid()) return ANIFEM_ERR_ARG;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dtet_batch(opA.c(), opB.c(), &t, order, n_tets, xyz, precision, A_out, stream);
+}
+inline int fem3Dface(const double* xyz, int64_t n_tets, int face_num, const TensorView& D, const ApplyOp& opA,
+                     const ApplyOp& opB, void* A_out, int order = 5, Precision precision = FP64,
+                     void* stream = nullptr) {
+    if (!opA.valid() || !opB.valid()) return ANIFEM_ERR_ARG;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dface_batch(opA.c(), opB.c(), &t, order, face_num, 0, n_tets, xyz, precision, A_out, stream);
+}
+inline int fem3DfaceN(const double* xyz, int64_t n_tets, int face_num, const TensorView& D, const ApplyOp& opA,
+                      const ApplyOp& opB, void* A_out, int order = 5, Precision precision = FP64,
+                      void* stream = nullptr) {
+    if (!opA.valid() || !opB.valid()) return ANIFEM_ERR_ARG;
+    const anifem_tensor t{D.type, D.storage, D.data};
+    return anifem_fem3dface_batch(opA.c(), opB.c(), &t, order, face_num, 1, n_tets, xyz, precision, A_out, stream);
+}
+inline int fem3DapplyL(const double* xyz, int64_t n_tets, const double* xyl, int q, const ApplyOp& op,
+                       const double* dofs, double* opU, void* stream = nullptr) {
+    if (!op.valid()) return ANIFEM_ERR_ARG;
+    return anifem_fem3dapply_batch(op.c(), xyl, q, n_tets, xyz, dofs, opU, stream);
+}
+
+// Whole-mesh assembler with operators chosen at run time: PrepareProblem at
+// construction (assembler.inl:194-275), Assemble(matrix, rhs) per run
+// (assembler.inl:314-488).  The trial and test spaces are the operators'
+// spaces; the load vector is fem3Dtet<IDEN P0, IDEN test space> of the
+// right-hand-side tensor (examples/tutorials/ex1.cpp:95).  Constructed from a
+// FemSpace, it assembles the space's variables (a product space: several)
+// from element matrices the caller forms over the concatenated local DOFs.
+class RuntimeAssembler {
+public:
+    RuntimeAssembler(const ApplyOp& trial, const ApplyOp& test, int64_t n_nodes, const double* coords, int64_t n_tets,
+                     const int32_t* tets, int order, TensorView D_kind, int rhs_order, TensorView F_kind,
+                     int dirichlet, const uint8_t* dirichlet_dof_mask = nullptr, void* stream = nullptr) {
+        if (!trial.valid() || !test.valid()) {
+            status_ = ANIFEM_ERR_ARG;
+            return;
+        }
         anifem_mesh_desc mesh{n_nodes, n_tets, coords, tets};
         anifem_problem_desc prob{};
-        prob.trial = OpA::c();
-        prob.test = OpB::c();
+        prob.trial = trial.c();
+        prob.test = test.c();
         prob.order = order;
         prob.tensor_type = D_kind.type;
         prob.tensor_storage = D_kind.storage;
@@ -200,9 +381,24 @@ public:
         status_ = anifem_assembly_create(&ctx_, &mesh, &prob, stream);
         if (status_) ctx_ = nullptr;
     }
-    Assembler(const Assembler&) = delete;
-    Assembler& operator=(const Assembler&) = delete;
-    ~Assembler() {
+    // the variables of a space (AssembleElements only); dirichlet as above,
+    // on every variable
+    RuntimeAssembler(const FemSpace& space, int64_t n_nodes, const double* coords, int64_t n_tets,
+                     const int32_t* tets, int dirichlet, const uint8_t* dirichlet_dof_mask = nullptr,
+                     void* stream = nullptr) {
+        anifem_vars_desc vars{};
+        status_ = space.vars(&vars);
+        if (status_) return;
+        anifem_mesh_desc mesh{n_nodes, n_tets, coords, tets};
+        anifem_problem_desc prob{};
+        prob.dirichlet = dirichlet;
+        prob.dirichlet_dof_mask = dirichlet_dof_mask;
+        status_ = anifem_assembly_create_vars(&ctx_, &mesh, &vars, &prob, stream);
+        if (status_) ctx_ = nullptr;
+    }
+    RuntimeAssembler(const RuntimeAssembler&) = delete;
+    RuntimeAssembler& operator=(const RuntimeAssembler&) = delete;
+    ~RuntimeAssembler() {
         if (ctx_) anifem_assembly_destroy(ctx_);
     }
     int status() const { return status_; }
@@ -260,6 +456,18 @@ private:
     }
     anifem_assembly* ctx_ = nullptr;
     int status_ = ANIFEM_OK;
+};
+
+// Whole-mesh assembler of the reference's template form: the test and trial
+// spaces are the Op tags' spaces.
+template <typename OpA, typename OpB>
+class Assembler : public RuntimeAssembler {
+public:
+    Assembler(int64_t n_nodes, const double* coords, int64_t n_tets, const int32_t* tets, int order,
+              TensorView D_kind, int rhs_order, TensorView F_kind, int dirichlet,
+              const uint8_t* dirichlet_dof_mask = nullptr, void* stream = nullptr)
+        : RuntimeAssembler(ApplyOp(OpA::c()), ApplyOp(OpB::c()), n_nodes, coords, n_tets, tets, order, D_kind,
+                           rhs_order, F_kind, dirichlet, dirichlet_dof_mask, stream) {}
 };
 
 }  // namespace anifem_gpu

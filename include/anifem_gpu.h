@@ -260,9 +260,10 @@ int anifem_assembly_create_budget(anifem_assembly** ctx, const anifem_mesh_desc*
  * anifem_assembly_eval return ANIFEM_ERR_ARG.  Of prob only dirichlet and
  * dirichlet_dof_mask are read (BOUNDARY flags the boundary-closure DOFs of
  * every variable).  At most 4 variables and 64 local DOFs. */
+#define ANIFEM_MAX_VARS 4
 typedef struct anifem_vars_desc {
     int32_t n_var;
-    anifem_op var[4]; /* fem and ncomp of each variable (op is not read) */
+    anifem_op var[ANIFEM_MAX_VARS]; /* fem and ncomp of each variable (op is not read) */
 } anifem_vars_desc;
 int anifem_assembly_create_vars(anifem_assembly** ctx, const anifem_mesh_desc* mesh, const anifem_vars_desc* vars,
                                 const anifem_problem_desc* prob, void* stream);

@@ -273,8 +273,9 @@ def fem3dtet_batch(opA: Op, opB: Op, ttype, storage, D, order, xyz, precision=PR
     """Element matrices of n tets (anifem_fem3dtet_batch).
 
     xyz: device float64 (12, n) SoA; D: device float64 coefficient array in the
-    documented layout (or None for TENSOR_NULL); returns out (n, nfB, nfA) in
-    column-major per tet, i.e. out[r].T is the nfB x nfA matrix in row-major.
+    documented layout (or None for TENSOR_NULL); returns out of shape
+    (n, nfA, nfB): tet r's nfB x nfA matrix stored column-major, so out[r].T
+    is that matrix.
     """
     import torch
     n = xyz.shape[1]

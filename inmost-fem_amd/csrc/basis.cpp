@@ -82,7 +82,7 @@ int op_dim(const anifem_op& op) {
 bool op_valid(const anifem_op& op) {
     if (ref_dim(op.op, op.fem) < 0 || nf_scalar(op.fem) < 0 || op.ncomp < 1 || op.ncomp > 3) return false;
     if (fem_vector(op.fem)) return op.ncomp == 1;
-    if (op.op != ANIFEM_IDEN && op.fem == ANIFEM_FEM_P0) return false;
+    // derivatives of P0 are zero operators of the usual dims (poly_0.h:11-59)
     if ((op.op == ANIFEM_DIV || op.op == ANIFEM_CURL) && op.ncomp != 3) return false;
     return true;
 }

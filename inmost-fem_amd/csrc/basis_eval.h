@@ -53,6 +53,9 @@ __host__ __device__ inline void basis_values(int fem, const double* L, double* p
 // g[i*3 + a] = d phi_i / d x_a (reference coordinates)
 __host__ __device__ inline void basis_gradients(int fem, const double* L, double* g) {
     switch (fem) {
+        case ANIFEM_FEM_P0:  // the constant: a zero gradient (poly_0.h:11-31)
+            g[0] = g[1] = g[2] = 0.0;
+            break;
         case ANIFEM_FEM_P1:
             for (int i = 0; i < 4; ++i)
                 for (int a = 0; a < 3; ++a) g[i * 3 + a] = dlam(i, a);

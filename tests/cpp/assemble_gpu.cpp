@@ -3,7 +3,9 @@
 // fem3Dtet<GRAD P2, GRAD P2, DfuncTraits<TENSOR_SYMMETRIC, false>> on a batch
 // of tets, and Assembler<GRAD P2, GRAD P2>::Assemble of the whole mesh
 // (per-tet symmetric tensor, unit load, boundary Dirichlet; homogeneous, then
-// with Dirichlet values).  Inputs and outputs are raw binary files in the
+// with Dirichlet values), the same calls with operators chosen at run time
+// (ApplyOp, FemSpace, RuntimeAssembler), and the several-variable context of
+// a product space.  Inputs and outputs are raw binary files in the
 // directory given on the command line; the test that drives it compares the
 // outputs with the oracle.
 #include <hip/hip_runtime_api.h>
@@ -132,6 +134,42 @@ int main(int argc, char** argv) {
     rc = asmb.Assemble(D, &F, d_val, d_rhs, &st, FP64, nullptr, d_bc);
     if (rc || st.code) return 8;
     save("rhs_bc.bin", to_host(d_rhs, ndof).data(), ndof);
+
+    // operators chosen at run time (int_tet.h:145-147; operators.h:418-533,
+    // fem_space.h:404-596): the same calls through ApplyOp / FemSpace give the
+    // template calls' results bit for bit
+    const FemSpace P2{P2Space{}};
+    const ApplyOp g2 = P2.getOP(GRAD);
+    double* d_A2 = nullptr;
+    if (hipMalloc((void**)&d_A2, sizeof(double) * 100 * n_tets) != hipSuccess) return 3;
+    rc = fem3Dtet(d_xyz, n_tets, D, g2, g2, d_A2, 2);
+    if (rc) return 9;
+    if (to_host(d_A2, 100 * (size_t)n_tets) != to_host(d_A, 100 * (size_t)n_tets)) {
+        std::fprintf(stderr, "runtime fem3Dtet differs from the template call\n");
+        return 10;
+    }
+    RuntimeAssembler rasm(g2, P2.getOP(GRAD), n_nodes, coords.data(), n_tets, tets.data(), 2, D, 2, F,
+                          ANIFEM_DIRICHLET_BOUNDARY);
+    if (rasm.status() || rasm.ndof() != ndof || rasm.nnz() != nnz) return 11;
+    double *d_val2 = nullptr, *d_rhs2 = nullptr;
+    if (hipMalloc((void**)&d_val2, sizeof(double) * nnz) != hipSuccess ||
+        hipMalloc((void**)&d_rhs2, sizeof(double) * ndof) != hipSuccess)
+        return 3;
+    rc = rasm.Assemble(D, &F, d_val2, d_rhs2, &st);
+    if (rc || st.code) return 12;
+    rc = asmb.Assemble(D, &F, d_val, d_rhs, &st);
+    if (rc || st.code) return 12;
+    if (to_host(d_val2, nnz) != to_host(d_val, nnz) || to_host(d_rhs2, ndof) != to_host(d_rhs, ndof)) {
+        std::fprintf(stderr, "runtime assembler differs from the template one\n");
+        return 13;
+    }
+    // a product space (mixed Poisson: RT0 x P0) builds the several-variable
+    // context: one DOF per face, then one per cell
+    const FemSpace mixed = FemSpace(RT0Space{}) * FemSpace(P0Space{});
+    RuntimeAssembler masm(mixed, n_nodes, coords.data(), n_tets, tets.data(), ANIFEM_DIRICHLET_NONE);
+    if (masm.status()) return 14;
+    const int64_t mixed_sizes[2] = {masm.ndof(), masm.nnz()};
+    save("mixed_sizes.bin", mixed_sizes, 2);
     std::printf("cpp face ok: %lld dofs, %lld nnz\n", (long long)ndof, (long long)nnz);
     return 0;
 }

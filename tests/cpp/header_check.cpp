@@ -57,6 +57,35 @@ int main() {
     CHECK(bad.AssembleElements<double>(nullptr, nullptr, nullptr, nullptr) == bad.status());
     CHECK(bad.AssembleElements<float>(nullptr, nullptr, nullptr, nullptr) == bad.status());
     CHECK(bad.AssembleRHS<double>(nullptr, nullptr) == bad.status());
+    // operators and spaces chosen at run time (operators.h:418-533, fem_space.h:404-596)
+    const FemSpace p2 = P2Space{}, p0 = P0Space{}, rt0 = RT0Space{}, nd0 = ND0Space{};
+    const ApplyOp gp2 = p2.getOP(GRAD);
+    CHECK(gp2.Nfa() == 10 && gp2.Dim() == 3 && gp2.Order() == 1 && gp2.valid());
+    CHECK((gp2 == ApplyOpFromTemplate<GRAD, FemFix<FEM_P2>>()));
+    CHECK((p2.getOP(IDEN).Order() == 2 && p2.dimOP(IDEN) == 1 && p2.dim() == 1 && p2.nfa() == 10));
+    const FemSpace v2 = p2 ^ 3;
+    CHECK((v2.dim() == 3 && v2.nfa() == 30 && v2.dimOP(GRAD) == 9 && v2.dimOP(DIV) == 1 && v2.dimOP(CURL) == 3));
+    CHECK((v2.getOP(GRAD) == ApplyOpFromTemplate<GRAD, FemVec<3, FEM_P2>>()));
+    CHECK((rt0.dim() == 3 && rt0.nfa() == 4 && rt0.dimOP(DIV) == 1 && rt0.orderOP(DIV) == 0 && rt0.orderOP(IDEN) == 1));
+    CHECK((nd0.nfa() == 6 && nd0.dimOP(CURL) == 3 && nd0.orderOP(CURL) == 0));
+    CHECK((FemSpace(Bubble4Space{}).orderOP(GRAD) == 3 && FemSpace(CR1Space{}).orderOP(IDEN) == 1));
+    CHECK((FemSpace(P0Space{}).orderOP(GRAD) == 0 && FemSpace(P1Space{}).orderOP(GRAD) == 0));
+    CHECK((p2.getOP(DIV).valid() == false && p2.getOP(DIV).Dim() < 0));  // DIV of a scalar space
+    // a product space (mixed Poisson: RT0 x P0): its variables, no single operator
+    const FemSpace mixed = rt0 * p0;
+    CHECK((mixed.isComplex() && mixed.nparts() == 2 && mixed.dim() == 4 && mixed.nfa() == 5));
+    CHECK((!mixed.getOP(IDEN).valid() && mixed.part(1) == p0 && mixed.part(0) == rt0));
+    // vector copies of RT0 / ND0 and more than 3 components are not built: no operator
+    CHECK((!(rt0 ^ 2).getOP(IDEN).valid() && !(p2 ^ 4).getOP(GRAD).valid() && (v2 ^ 2).empty()));
+    anifem_vars_desc vd{};
+    CHECK((mixed.vars(&vd) == ANIFEM_OK && vd.n_var == 2 && vd.var[0].fem == ANIFEM_FEM_RT0 && vd.var[1].fem == ANIFEM_FEM_P0));
+    CHECK(((p2 * p2 * p2 * p2 * p2).vars(&vd) == ANIFEM_ERR_ARG));
+    // the runtime calls reach the same argument checks as the template ones
+    CHECK((fem3Dtet(buf, 1, sym_tet, gp2, gp2, buf, 21) == ANIFEM_ERR_ORDER));
+    CHECK((fem3Dtet(buf, 1, sym_tet, ApplyOp(), gp2, buf, 2) == ANIFEM_ERR_ARG));
+    CHECK((fem3Dface(buf, 1, 4, null_const, p2.getOP(IDEN), p2.getOP(IDEN), buf, 2) == ANIFEM_ERR_ARG));
+    RuntimeAssembler bad_rt(ApplyOp(), gp2, 0, nullptr, 0, nullptr, 2, sym_tet, 2, null_const, ANIFEM_DIRICHLET_NONE);
+    CHECK(bad_rt.status() == ANIFEM_ERR_ARG);
     std::printf("header ok\n");
     return 0;
 }

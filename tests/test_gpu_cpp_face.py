@@ -85,3 +85,11 @@ def test_cpp_face_element_batch_and_assembler(tmp_path):
         assert np.all(np.abs(val - v_or) <= 1e-12 * np.repeat(rowmax, np.diff(rp)))
         r = np.fromfile(os.path.join(d, name))
         assert np.abs(r - r_or).max() <= 1e-12 * np.abs(r_or).max(), name
+    # the product space RT0 x P0 through RuntimeAssembler(FemSpace): the
+    # several-variable context of the Python binding (host-only numbering)
+    mnd, mnz = np.fromfile(os.path.join(d, "mixed_sizes.bin"), np.int64)
+    H = af.Assembler(coords, tets, None, variables=[af.Op(af.IDEN, af.FEM_RT0), af.Op(af.IDEN, af.FEM_P0)],
+                     host_only=True)
+    assert (mnd, mnz) == (H.ndof, H.nnz)
+    faces = {tuple(sorted(t[[k, (k + 1) % 4, (k + 2) % 4]])) for t in tets for k in range(4)}
+    assert mnd == len(faces) + n

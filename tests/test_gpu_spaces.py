@@ -102,6 +102,34 @@ def test_apply_grad_of_vector_spaces_vs_oracle(fem):
     assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_derivatives_of_p0_are_zero():
+    """GRAD and DUDX..DUDZ of P0 (poly_0.h:11-59): the constant's derivatives
+    vanish, so every element matrix with them is exactly zero, as is its
+    evaluation; IDEN P0 is unchanged (its mass is |T|)."""
+    rng = np.random.default_rng(4)
+    n = 40
+    xyz = rng.normal(size=(n, 12))
+    X = dev(T.soa_xyz(xyz))
+    D = dev(rng.normal(size=(9, n)))
+    g0, g1 = af.Op(af.GRAD, af.FEM_P0), af.Op(af.GRAD, af.FEM_P1)
+    for prec in (af.PREC_FP64, af.PREC_FP32):
+        A = af.fem3dtet_batch(g0, g1, af.TENSOR_GENERAL, af.COEF_PER_TET, D, 2, X, prec).cpu().numpy()
+        assert A.shape == (n, 1, 4) and not A.any()
+        A = af.fem3dtet_batch(g0, g0, af.TENSOR_NULL, af.COEF_CONST, None, 2, X, prec).cpu().numpy()
+        assert A.shape == (n, 1, 1) and not A.any()
+    for op in (af.DUDX, af.DUDY, af.DUDZ):
+        A = af.fem3dtet_batch(af.Op(op, af.FEM_P0), af.Op(op, af.FEM_P2), af.TENSOR_NULL, af.COEF_CONST, None, 2,
+                              X).cpu().numpy()
+        assert A.shape == (n, 1, 10) and not A.any()
+    xyl = rng.dirichlet(np.ones(4), 3)
+    out = af.fem3dapply_batch(g0, xyl, X, dev(rng.normal(size=(n, 1)))).cpu().numpy()
+    assert out.shape == (n, 3, 3) and not out.any()
+    i0 = af.Op(af.IDEN, af.FEM_P0)
+    M = af.fem3dtet_batch(i0, i0, af.TENSOR_NULL, af.COEF_CONST, None, 2, X).cpu().numpy().reshape(n)
+    vol = np.abs(np.linalg.det((xyz.reshape(n, 4, 3)[:, 1:] - xyz.reshape(n, 4, 3)[:, :1]))) / 6
+    assert np.abs(M - vol).max() <= 1e-14 * vol.max()
+
+
 def test_refused_cases():
     """GRAD of RT0 / ND0 paired with another operator or in the normal form,
     vector copies of RT0 / ND0 (elements and global assembly) and the MFMA mode

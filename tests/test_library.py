@@ -33,7 +33,11 @@ def test_version_and_op_sizes():
     assert af.Op(af.GRAD, af.FEM_P2).nfa == 10 and af.Op(af.GRAD, af.FEM_P2).dim == 3
     assert af.Op(af.GRAD, af.FEM_P2, 3).nfa == 30 and af.Op(af.GRAD, af.FEM_P2, 3).dim == 9
     assert af.Op(af.IDEN, af.FEM_P3).nfa == 20 and af.Op(af.IDEN, af.FEM_P0).nfa == 1
-    assert af.lib().anifem_op_nfa(af.anifem_op(af.GRAD, af.FEM_P0, 1)) < 0
+    # derivatives of P0: zero operators with the dims of poly_0.h:11-59
+    assert af.Op(af.GRAD, af.FEM_P0).nfa == 1 and af.Op(af.GRAD, af.FEM_P0).dim == 3
+    assert af.Op(af.DUDY, af.FEM_P0).dim == 1 and af.Op(af.DIV, af.FEM_P0, 3).dim == 1
+    # DIV / CURL need 3 components (operators.h:286-400)
+    assert af.lib().anifem_op_nfa(af.anifem_op(af.DIV, af.FEM_P1, 1)) < 0
 
 
 def test_quadrature_rules():
@@ -62,7 +66,7 @@ def test_argument_errors_without_device():
     # identity tensor between GRAD (dim 3) and IDEN (dim 1) (diff_tensor.h:313-315)
     assert L.anifem_fem3dtet_batch(g1, g2, ctypes.byref(t), 2, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
         == af.ERR_DIMS
-    bad = af.anifem_op(af.GRAD, af.FEM_P0, 1)
+    bad = af.anifem_op(af.DIV, af.FEM_P1, 1)
     assert L.anifem_fem3dtet_batch(bad, g1, ctypes.byref(t), 2, 1, buf.ctypes.data, 0, buf.ctypes.data, None) \
         == af.ERR_ARG
     assert L.anifem_mesh_kuhn_cube(0, 0.2, 0, None, None) == af.ERR_ARG
