@@ -558,8 +558,9 @@ int anifem_pcg_matrix(int64_t n, const int64_t* row_ptr, const int32_t* col_idx,
 
 /* ---- learned preconditioner ----------------------------------------------
  * No reference counterpart.  A 3-layer message-passing network (32 hidden
- * units; FP32 arithmetic on VALU, with the weights rounded to BF16 precision
- * and held as FP32) reads an assembled SPD matrix A and
+ * units; BF16 weights, stored as 16-bit words and widened in the kernels, with
+ * FP32 arithmetic and accumulation on VALU; FP32 master weights and Adam
+ * moments) reads an assembled SPD matrix A and
  * writes a lower-triangular factor G on A's pattern; M^-1 = G G^T.  With the
  * output heads at zero the network is exactly Jacobi (G = diag(a_ii)^-1/2).
  * Training minimises (1/(P n)) sum_p ||G^T A G z_p - z_p||^2 over P = 4

@@ -12,8 +12,9 @@
 //   L_s = at_s (w_o.(h_r*h_c) + c_o) on lower slots; d_r = w_d.h_r + c_d
 //   G_s = exp(d_r) dinv_r (delta_rc + L_s)
 //   loss = (1/(P n)) sum_p ||G^T A G z_p - z_p||^2 over P Rademacher probes
-// FP32 arithmetic with weights rounded to BF16 (round to nearest even) for
-// the forward and backward passes; FP32 master weights and Adam moments.
+// The forward and backward passes read BF16 weights (16-bit words, rounded
+// to nearest even from the FP32 master weights after each Adam step) and
+// compute in FP32; FP32 master weights and Adam moments.
 // Every reduction is a fixed two-stage tree: results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
@@ -46,11 +47,14 @@ __host__ __device__ constexpr int off_ws(int l) { return OFF_L0 + l * LAYER; }
 __host__ __device__ constexpr int off_wm(int l) { return OFF_L0 + l * LAYER + H * H; }
 __host__ __device__ constexpr int off_b(int l) { return OFF_L0 + l * LAYER + 2 * H * H; }
 
-__device__ __forceinline__ float bf16_round(float x) {
+// BF16 by round to nearest even: the 16 high bits of the rounded FP32 value
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
     uint32_t u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xFFFF0000u);
+    return (uint16_t)(u >> 16);
 }
+// a stored BF16 weight, widened to FP32 exactly
+__device__ __forceinline__ float wt(const uint16_t* pw, int i) { return __uint_as_float((uint32_t)pw[i] << 16); }
 
 // ---- features -------------------------------------------------------------
 __global__ void diag_kernel(int64_t n, const int64_t* rp, const int32_t* ci, const double* a, float* dinv,
@@ -89,14 +93,14 @@ __global__ void feat_kernel(int64_t n, const int64_t* rp, const int32_t* ci, con
 }
 
 // ---- forward --------------------------------------------------------------
-__global__ void input_kernel(int64_t n, const float* x, const float* pw, float* z0, float* h0) {
+__global__ void input_kernel(int64_t n, const float* x, const uint16_t* pw, float* z0, float* h0) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n * H) return;
     const int64_t r = i / H;
     const int k = (int)(i % H);
-    float z = pw[OFF_BIN + k];
+    float z = wt(pw, OFF_BIN + k);
 #pragma unroll
-    for (int j = 0; j < FIN; ++j) z += pw[OFF_WIN + k * FIN + j] * x[r * FIN + j];
+    for (int j = 0; j < FIN; ++j) z += wt(pw, OFF_WIN + k * FIN + j) * x[r * FIN + j];
     z0[i] = z;
     h0[i] = fmaxf(z, 0.0f);
 }
@@ -138,13 +142,13 @@ __global__ void spmm_kernel(int64_t n, const int64_t* rp, const int32_t* ci, con
 // broadcasts.
 constexpr int LN = 32;
 __global__ __launch_bounds__(NT) void mp_layer_kernel(int64_t n, const int64_t* rp, const int32_t* ci,
-                                                      const float* at, const float* h, const float* pw, int l,
+                                                      const float* at, const float* h, const uint16_t* pw, int l,
                                                       float* m, float* z, float* hn) {
     __shared__ float Wst[H][H + 1], Wmt[H][H + 1], hs[LN][H], ms[LN][H];
     for (int i = threadIdx.x; i < H * H; i += NT) {
         const int k = i / H, j = i % H;
-        Wst[j][k] = pw[off_ws(l) + i];
-        Wmt[j][k] = pw[off_wm(l) + i];
+        Wst[j][k] = wt(pw, off_ws(l) + i);
+        Wmt[j][k] = wt(pw, off_wm(l) + i);
     }
     constexpr int G = NT / H, PER = LN / G;
     const int k = threadIdx.x % H, g = threadIdx.x / H;
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(NT) void mp_layer_kernel(int64_t n, const int64_t* 
         ms[ln][k] = mv;
     }
     __syncthreads();
-    const float b = pw[off_b(l) + k];
+    const float b = wt(pw, off_b(l) + k);
     float acc[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) acc[t] = b;
@@ -186,25 +190,25 @@ __global__ __launch_bounds__(NT) void mp_layer_kernel(int64_t n, const int64_t* 
 
 
 __global__ void head_edge_kernel(int64_t nnz, const int32_t* row, const int32_t* ci, const float* at, const float* h,
-                                 const float* pw, float* Lv) {
+                                 const uint16_t* pw, float* Lv) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= nnz) return;
     const int32_t r = row[s], c = ci[s];
     float v = 0;
     if (r > c) {
-        float t = pw[OFF_CO];
-        for (int k = 0; k < H; ++k) t += pw[OFF_WO + k] * h[(int64_t)r * H + k] * h[(int64_t)c * H + k];
+        float t = wt(pw, OFF_CO);
+        for (int k = 0; k < H; ++k) t += wt(pw, OFF_WO + k) * h[(int64_t)r * H + k] * h[(int64_t)c * H + k];
         v = at[s] * t;
     }
     Lv[s] = v;
 }
 
-__global__ void head_node_kernel(int64_t n, const int64_t* rp, const int32_t* ci, const float* h, const float* pw,
+__global__ void head_node_kernel(int64_t n, const int64_t* rp, const int32_t* ci, const float* h, const uint16_t* pw,
                                  const float* dinv, const float* Lv, float* Dh, float* G32, double* G64) {
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= n) return;
-    float d = pw[OFF_CD];
-    for (int k = 0; k < H; ++k) d += pw[OFF_WD + k] * h[r * H + k];
+    float d = wt(pw, OFF_CD);
+    for (int k = 0; k < H; ++k) d += wt(pw, OFF_WD + k) * h[r * H + k];
     const float dh = expf(d) * dinv[r];
     Dh[r] = dh;
     for (int64_t s = rp[r]; s < rp[r + 1]; ++s) {
@@ -309,7 +313,7 @@ __global__ void dG_kernel(int64_t nnz, const int32_t* row, const int32_t* ci, co
 // and the per-node terms of the head's weight gradients.
 __global__ void head_back_kernel(int64_t n, const int64_t* rp, const int32_t* ci, const int32_t* tslot,
                                  const float* dG, const float* Lv, const float* Dh, const float* at, const float* h,
-                                 const float* pw, float* dh, float* gwo, float* gwd, float* gsc) {
+                                 const uint16_t* pw, float* dh, float* gwo, float* gwd, float* gsc) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n * H) return;
     const int64_t r = i / H;
@@ -318,7 +322,7 @@ __global__ void head_back_kernel(int64_t n, const int64_t* rp, const int32_t* ci
     for (int64_t s = rp[r]; s < rp[r + 1]; ++s) dDh += dG[s] * ((ci[s] == r ? 1.0f : 0.0f) + Lv[s]);
     const float dd = dDh * Dh[r];
     const float hrk = h[r * H + k];
-    float acc = dd * pw[OFF_WD + k];
+    float acc = dd * wt(pw, OFF_WD + k);
     float wo_part = 0, co_part = 0;
     for (int64_t s = rp[r]; s < rp[r + 1]; ++s) {
         const int32_t c = ci[s];
@@ -333,7 +337,7 @@ __global__ void head_back_kernel(int64_t n, const int64_t* rp, const int32_t* ci
             const int32_t q = tslot[s];  // lower slot (c, r) in row c
             dt = dG[q] * Dh[c] * at[q];
         }
-        acc += dt * pw[OFF_WO + k] * h[(int64_t)c * H + k];
+        acc += dt * wt(pw, OFF_WO + k) * h[(int64_t)c * H + k];
     }
     dh[i] = acc;
     gwo[i] = wo_part;
@@ -345,12 +349,12 @@ __global__ void head_back_kernel(int64_t n, const int64_t* rp, const int32_t* ci
 }
 
 // Layer backward: dz = dh * [z > 0]; dm = W_m^T dz; tmp = dh + W_s^T dz
-__global__ __launch_bounds__(NT) void layer_back_kernel(int64_t n, const float* dh, const float* z, const float* pw,
+__global__ __launch_bounds__(NT) void layer_back_kernel(int64_t n, const float* dh, const float* z, const uint16_t* pw,
                                                         int l, float* dz, float* dm, float* tmp) {
     __shared__ float Ws[H * H], Wm[H * H], dzs[NODES][H];
     for (int i = threadIdx.x; i < H * H; i += NT) {
-        Ws[i] = pw[off_ws(l) + i];
-        Wm[i] = pw[off_wm(l) + i];
+        Ws[i] = wt(pw, off_ws(l) + i);
+        Wm[i] = wt(pw, off_wm(l) + i);
     }
     const int ln = threadIdx.x / H, k = threadIdx.x % H;
     const int64_t r = (int64_t)blockIdx.x * NODES + ln;
@@ -449,7 +453,7 @@ __global__ void loss_total_kernel(int nblk, const float* partial, float scale, f
 }
 
 __global__ void adam_kernel(int np, float* theta, const float* g, float* m, float* v, float lr, float b1, float b2,
-                            float eps, float bc1, float bc2, float* pw) {
+                            float eps, float bc1, float bc2, uint16_t* pw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const float gi = g[i];
@@ -459,12 +463,12 @@ __global__ void adam_kernel(int np, float* theta, const float* g, float* m, floa
     v[i] = vi;
     const float t = theta[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
     theta[i] = t;
-    pw[i] = bf16_round(t);
+    pw[i] = bf16_bits(t);
 }
 
-__global__ void round_params_kernel(int np, const float* theta, float* pw) {
+__global__ void round_params_kernel(int np, const float* theta, uint16_t* pw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) pw[i] = bf16_round(theta[i]);
+    if (i < np) pw[i] = bf16_bits(theta[i]);
 }
 
 // Gaussian random field by random Fourier features (squared-exponential
@@ -506,7 +510,8 @@ struct anifem_gnn {
     // probes
     float *Z = nullptr, *Y = nullptr, *W = nullptr, *U = nullptr, *GU = nullptr, *GW = nullptr, *GY = nullptr;
     // parameters
-    float *theta = nullptr, *pw = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
+    float *theta = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
+    uint16_t* pw = nullptr;  // BF16 weights of the forward and backward passes
     float *partial = nullptr, *loss = nullptr;
     // M = G G^T as an explicit matrix: its CSR pattern, and per entry the slot
     // pairs (G_ik, G_jk) of its terms in ascending k
@@ -748,7 +753,7 @@ int anifem_gnn_create(anifem_gnn** out, int64_t n, const int64_t* row_ptr, const
         ANIFEM_HIP_CHECK(hipMemcpyAsync(g->m_tb, m_tb.data(), m_tb.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     }
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->theta, 0, NPARAM * sizeof(float), s));
-    ANIFEM_HIP_CHECK(hipMemsetAsync(g->pw, 0, NPARAM * sizeof(float), s));
+    ANIFEM_HIP_CHECK(hipMemsetAsync(g->pw, 0, NPARAM * sizeof(uint16_t), s));
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->grad, 0, NPARAM * sizeof(float), s));
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->am, 0, NPARAM * sizeof(float), s));
     ANIFEM_HIP_CHECK(hipMemsetAsync(g->av, 0, NPARAM * sizeof(float), s));
@@ -778,7 +783,14 @@ int anifem_gnn_set_params(anifem_gnn* g, const float* theta) {
 int anifem_gnn_get_params(const anifem_gnn* g, float* theta, float* theta_bf16) {
     if (!g) return ANIFEM_ERR_ARG;
     if (theta) ANIFEM_HIP_CHECK(hipMemcpy(theta, g->theta, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
-    if (theta_bf16) ANIFEM_HIP_CHECK(hipMemcpy(theta_bf16, g->pw, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    if (theta_bf16) {  // the BF16 weights, widened to FP32
+        std::vector<uint16_t> b(NPARAM);
+        ANIFEM_HIP_CHECK(hipMemcpy(b.data(), g->pw, NPARAM * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < NPARAM; ++i) {
+            const uint32_t u = (uint32_t)b[i] << 16;
+            std::memcpy(&theta_bf16[i], &u, sizeof(float));
+        }
+    }
     return ANIFEM_OK;
 }
 
