@@ -164,6 +164,8 @@ def cpu_baseline(coords, tets_reordered, Dfull, dofs, slot, is_dir, nnz, ndof):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "assembly_p2_aniso_64cube_fp64_traffic.txt")
+TRAFFIC_FILE_C3 = os.path.join(ROOT, "profiles", "batched_uneven_256_traffic.txt")
+TRAFFIC_FILE_C4 = os.path.join(ROOT, "profiles", "elasticity_p2vec_48cube_traffic.txt")
 
 
 def timed_events(fn, steps, warmup, torch):
@@ -179,13 +181,19 @@ def timed_events(fn, steps, warmup, torch):
     return e0.elapsed_time(e1) / steps
 
 
-def hbm_roofline(alg_bytes, ms, kernel, per_unit):
+def hbm_roofline(alg_bytes, ms, kernel, per_unit, traffic_file=None):
     """Roofline entry of a section: algorithmic bytes of one pass over its
-    measured time against the HBM peak."""
+    measured time against the HBM peak; with the measured HBM bytes per pass
+    of the committed counter summary `traffic_file` when it is present."""
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_pass": int(alg_bytes), "kernel": kernel,
-            "bytes_model": per_unit}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_pass": int(alg_bytes), "kernel": kernel,
+           "bytes_model": per_unit}
+    if traffic_file:
+        traffic, src = read_traffic(traffic_file)
+        out["traffic"] = traffic
+        out["traffic_source"] = src
+    return out
 
 
 def assembly_bytes(n_tets, nf, coef_bytes, nnz, ndof):
@@ -223,7 +231,7 @@ def bench_batched(args, torch):
            "setup_s": setup_s, "autotune": tuning,
            "roofline": hbm_roofline(alg, ms, "geometry_kernel + row_block_persistent per element order (one graph)",
                                     "per tet 96 B xyz + 8 B kappa + 4 nf^2 B slots + 4 nf B dofs; "
-                                    "16 B per nnz and per DOF"),
+                                    "16 B per nnz and per DOF", TRAFFIC_FILE_C3),
            "note": "one geometry launch for all meshes and one row-block launch per element order, replayed from "
                    "a HIP graph (FP64)"}
     del B, val, rhs
@@ -263,7 +271,7 @@ def bench_elasticity(args, torch):
             res["roofline"] = hbm_roofline(assembly_bytes(n, 30, 16, A.nnz, A.ndof), ms,
                                            "geometry_kernel<3,3,3,3,double,false,true> + row_block_persistent_vec<3,3,3,10,double,true> (P2^3, compact Lame rows)",
                                            "per tet 96 B xyz + 16 B (lambda, mu) + 4 nf^2 B slots + 4 nf B dofs; "
-                                           "16 B per nnz and per DOF")
+                                           "16 B per nnz and per DOF", TRAFFIC_FILE_C4)
     A.assemble(D=D, F=g, precision=af.PREC_FP32, val=val, rhs=rhs, status=False)
     torch.cuda.synchronize()
     res["fp32_max_entry_error_rel_to_max"] = float((val - ref).abs().max() / ref.abs().max())
@@ -328,12 +336,12 @@ def bench_elasticity(args, torch):
     return res
 
 
-def read_traffic():
-    """HBM bytes per assembly pass from the committed FETCH_SIZE / WRITE_SIZE
+def read_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per assembly pass from a committed FETCH_SIZE / WRITE_SIZE
     summary (copy-kernel-calibrated), or None when it is absent."""
     try:
-        kv = dict(line.split(": ", 1) for line in open(TRAFFIC_FILE).read().splitlines() if ": " in line)
-        return int(kv["traffic_bytes_per_pass"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+        kv = dict(line.split(": ", 1) for line in open(path).read().splitlines() if ": " in line)
+        return int(kv["traffic_bytes_per_pass"]), os.path.relpath(path, ROOT)
     except (OSError, KeyError, ValueError):
         return None, None
 

@@ -337,9 +337,10 @@ int launch_blocks(anifem_assembly* ctx, anifem::BlockArgs a, hipStream_t s) {
     const int64_t gblocks = (S.n_tets + kBlock - 1) / kBlock;
     const size_t glds = geometry_stage_bytes<NCA, NCB, T>(a.rowlen);
     if constexpr (NCA == 3 && NCB == 3 && DA0 == 3 && DB0 == 3) {
+        const size_t glds_lc = geometry_stage_bytes<3, 3, T, true>(a.rowlen);
         if (a.lamec)
             hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T, false, true>), dim3((unsigned)gblocks), dim3(kBlock),
-                               glds, s, a, Kg, S.n_tets);
+                               glds_lc, s, a, Kg, S.n_tets);
         else
             hipLaunchKernelGGL((geometry_kernel<3, 3, 3, 3, T>), dim3((unsigned)gblocks), dim3(kBlock), glds, s, a, Kg,
                                S.n_tets);

@@ -172,11 +172,12 @@ __device__ __forceinline__ T lame_entry(const T* Kt, int cb, int ca, const T* r)
 // SYM (3 x 3 tensors of a symmetric K, one component, one carried point): the
 // matrix part of the row is K packed to its 6 distinct values (sym_pack),
 // which the row-block passes contract with the column-summed R.
-// Dynamic LDS of a geometry launch: short rows (scalar spaces, at most 16
-// values) are staged, one row per lane; the launch passes exactly this size.
-template <int NCA, int NCB, typename T>
+// Dynamic LDS of a geometry launch: short rows (scalar spaces or compact Lame
+// rows, at most 16 values) are staged, one row per lane; the launch passes
+// exactly this size.
+template <int NCA, int NCB, typename T, bool LC = false>
 __host__ __device__ inline size_t geometry_stage_bytes(int rowlen) {
-    return (NCA * NCB == 1 && rowlen <= 16) ? (size_t)kBlock * rowlen * sizeof(T) : 0;
+    return ((NCA * NCB == 1 || LC) && rowlen <= 16) ? (size_t)kBlock * rowlen * sizeof(T) : 0;
 }
 
 // LC: compact Lame rows (lame_kblock; a tensor constant over each tet).
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
     // range: coalesced stores instead of one strided store per value and lane
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* stage = reinterpret_cast<T*>(smem_raw);
-    const bool staged = geometry_stage_bytes<NCA, NCB, T>(a.rowlen) > 0;
+    const bool staged = geometry_stage_bytes<NCA, NCB, T, LC>(a.rowlen) > 0;
     const int64_t r0 = blockIdx.x * (int64_t)kBlock;
     const int64_t r = r0 + threadIdx.x;
     int degenerate = 0;
