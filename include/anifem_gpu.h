@@ -240,7 +240,7 @@ int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, 
 /* anifem_assembly_create with explicit row-block budgets (CSR slots, element
  * entries and (row, tet) pairs per block; at most 65535, 65535 and 256).  The
  * default (anifem_assembly_create) chooses them by the entries per pair:
- * 2048 / 2048 / 256 up to 10, 4096 / 3072 / 256 up to 20, 4096 / 8192 / 256
+ * 2048 / 2048 / 256 up to 10, 4096 / 2048 / 128 up to 20, 4096 / 8192 / 256
  * above, and 8192 / 8192 (16384 above 20) / 256 when one row alone exceeds
  * those.  Returns ANIFEM_ERR_DIMS when one row alone exceeds the budgets.  No
  * reference counterpart. */
@@ -499,15 +499,18 @@ int anifem_batch_csr(const anifem_batch* ctx, int mesh, const int64_t** row_ptr,
 /* chosen budgets {slots, entries, pairs} per block and the times of the
  * ANIFEM_BATCH_BUDGET_CANDIDATES candidates (ms[] of that length; -1: not
  * applicable), in the order {2048, 1024, 128}, {2048, 2048, 256},
- * {4096, 4096, 256}, {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256} */
-#define ANIFEM_BATCH_BUDGET_CANDIDATES 6
+ * {4096, 4096, 256}, {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256},
+ * {4096, 2560, 128}, {4096, 2048, 128} */
+#define ANIFEM_BATCH_BUDGET_CANDIDATES 8
 int anifem_batch_tuning(const anifem_batch* ctx, int fem, int32_t* caps, double* ms);
-/* Launch shape of element order fem's row-block pass: shape = {fixed-size
- * phase A (P1 / P2), compact value positions, workgroups per CU (0: the
+/* Launch shape of element order fem's row-block pass: shape = {phase-A form
+ * (0 general, 1 fixed-size for P1 / P2, 2 two lanes per pair for P3 blocks of
+ * at most 128 pairs), compact value positions, workgroups per CU (0: the
  * occupancy)}; with autotune, timed at the first anifem_batch_run on the
- * batch's own blocks over both forms, both position lists where the order
- * has compact ones, and the occupancy and the two counts below it; ms: its
- * time per launch (-1 until timed).  No counterpart in the reference. */
+ * batch's own blocks over the order's forms, both position lists where the
+ * order has compact ones (form 2 takes 16-bit ones), and the occupancy and the
+ * two counts below it; ms: its time per launch (-1 until timed).  No
+ * counterpart in the reference. */
 int anifem_batch_launch_tuning(const anifem_batch* ctx, int fem, int32_t* shape, double* ms);
 /* Row-block kernel choice of the batch's graph (default 1: the persistent
  * pass, element kinds restaged per workgroup range; 0: one workgroup per

@@ -812,7 +812,7 @@ class LearnedPreconditioner:
 # the batch autotuner's candidate row-block budgets {slots, entries, pairs}
 # (ANIFEM_BATCH_BUDGET_CANDIDATES, in the order of anifem_batch_tuning's times)
 BATCH_BUDGET_CANDIDATES = [(2048, 1024, 128), (2048, 2048, 256), (4096, 4096, 256), (1024, 1024, 256),
-                           (4096, 3072, 256), (4096, 2560, 256)]
+                           (4096, 3072, 256), (4096, 2560, 256), (4096, 2560, 128), (4096, 2048, 128)]
 
 
 class Batch:
@@ -871,14 +871,15 @@ class Batch:
         return caps, ms
 
     def launch_tuning(self, fem):
-        """Launch shape of order fem's row-block pass: fixed-size form,
-        compact positions, workgroups per CU (0: occupancy), ms per launch
-        (-1 until timed at the first run)."""
+        """Launch shape of order fem's row-block pass: phase-A form (0
+        general, 1 fixed-size, 2 two lanes per pair), compact positions,
+        workgroups per CU (0: occupancy), ms per launch (-1 until timed at the
+        first run)."""
         shape = np.zeros(3, np.int32)
         ms = ctypes.c_double()
         _check(lib().anifem_batch_launch_tuning(self._h, fem, _ptr(shape), ctypes.byref(ms)),
                "anifem_batch_launch_tuning")
-        return {"fixed": int(shape[0]), "compact": int(shape[1]), "workgroups_per_cu": int(shape[2]), "ms": ms.value}
+        return {"form": int(shape[0]), "compact": int(shape[1]), "workgroups_per_cu": int(shape[2]), "ms": ms.value}
 
     def set_persistent(self, on=True):
         _check(lib().anifem_batch_set_persistent(self._h, 1 if on else 0), "anifem_batch_set_persistent")

@@ -226,18 +226,18 @@ int upload_maps(anifem_assembly* ctx, hipStream_t s) {
 
 // Persistent row-block pass: occupancy x CUs workgroups, whole XCD groups
 // (surplus workgroups exit at once).
-template <int DA0, int DB0, int PW, typename T, bool P8, int NFX = 0>
+template <int DA0, int DB0, int PW, typename T, bool P8, int NFX = 0, bool SPLIT = false>
 int launch_persistent_p(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
     using namespace anifem;
     int occ = 0, n_cu = 0;
     ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T, P8, NFX>), kBlock, lds));
+        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT>), kBlock, lds));
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
     G = std::min<int64_t>(G, ctx->S.n_blocks);
     G = std::max<int64_t>(8, G / 8 * 8);
-    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T, P8, NFX>), dim3((unsigned)G), dim3(kBlock), lds, s, a,
-                       Kg);
+    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT>), dim3((unsigned)G), dim3(kBlock), lds,
+                       s, a, Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -246,12 +246,17 @@ int launch_persistent_p(anifem_assembly* ctx, const anifem::BlockArgs& a, const 
 // word count, else the 16-bit positions (identical values either way).
 // A single element kind of exactly 2*PW local DOFs (P1: 4, P2: 10) takes the
 // fixed-size instantiation (for P3 the general one measured faster: 0.62 against
-// 0.65 ms for the row blocks of 384,000 tets).
+// 0.65 ms for the row blocks of 384,000 tets).  Blocks of at most kBlock/2
+// pairs with longer pairs (P3) take two lanes per pair.
 template <int DA0, int DB0, int PW, typename T>
 int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
     const bool fixed = PW <= 5 && ctx->fixed_nf && a.n_kinds == 1 && a.kinds[0].nf == 2 * PW &&
                        a.kinds[0].nfA0 == 2 * PW && a.kinds[0].nfB0 == 2 * PW;
     constexpr int NFX = PW <= 5 ? 2 * PW : 0;
+    if constexpr (PW > 5 && PW % 2 == 0) {
+        if (ctx->S.max_block_pairs <= anifem::kBlock / 2)
+            return launch_persistent_p<DA0, DB0, PW, T, false, 0, true>(ctx, a, Kg, lds, s);
+    }
     if (ctx->d_pair_pos8 && ctx->pos8_words == (2 + 2 * PW + 3) / 4)
         return fixed ? launch_persistent_p<DA0, DB0, PW, T, true, NFX>(ctx, a, Kg, lds, s)
                      : launch_persistent_p<DA0, DB0, PW, T, true>(ctx, a, Kg, lds, s);
@@ -549,7 +554,7 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
         ctx->prob.dirichlet_dof_mask = nullptr;
         ctx->prob.trial = ctx->prob.test = anifem_op{ANIFEM_IDEN, vars->var[0].fem, vars->var[0].ncomp};
         ctx->vars_only = true;
-        const BlockCaps c = caps ? *caps : default_caps(nf);
+        const BlockCaps c = caps ? *caps : default_caps(nf, false);
         const int rc = build_setup_vars(*mesh, vars->n_var, vars->var, *prob, ctx->S, c);
         if (rc != ANIFEM_ERR_DIMS || caps) return rc;
         ctx->S = AssemblySetup();  // the default budgets, then wider ones when one row exceeds them
@@ -561,7 +566,7 @@ int build_host(anifem_assembly* ctx, const anifem_mesh_desc* mesh, const anifem_
     ctx->prob = *prob;
     ctx->prob.dirichlet_dof_mask = nullptr;
     const int nf = nf_scalar(prob->test.fem) * prob->test.ncomp;
-    const BlockCaps c = caps ? *caps : default_caps(nf);
+    const BlockCaps c = caps ? *caps : default_caps(nf, prob->test.ncomp == 1);
     const int rc =
         on_device ? build_setup_device(*mesh, *prob, ctx->S, c, stream) : build_setup(*mesh, *prob, ctx->S, c);
     if (rc != ANIFEM_ERR_DIMS || caps) return rc;
@@ -582,7 +587,8 @@ namespace {
 int create_default(anifem_assembly** out, const anifem_mesh_desc* mesh, const anifem_problem_desc* prob, int nf,
                    void* stream, const anifem_vars_desc* vars) {
     using namespace anifem;
-    const int rc = assembly_create_caps(out, mesh, prob, default_caps(nf), stream, vars);
+    const bool scalar_space = !vars && prob->test.ncomp == 1;
+    const int rc = assembly_create_caps(out, mesh, prob, default_caps(nf, scalar_space), stream, vars);
     if (rc != ANIFEM_ERR_DIMS) return rc;
     return assembly_create_caps(out, mesh, prob, wide_caps(nf), stream, vars);
 }

@@ -695,13 +695,25 @@ __device__ __forceinline__ int slot_words(int64_t s0, int ns) { return (int)(((s
 // K = 0 onto the value 1, and non-finite entries are counted entry by entry
 // only for a pair whose K is not finite or beyond 1e300 (below that no entry
 // can overflow).  Same values and sums as NFX = 0.
-template <int DA0, int DB0, int PW, typename T, bool P8 = false, int NFX = 0>
+// SPLIT (general form, 16-bit positions, blocks of at most kBlock/2 pairs):
+// two lanes per pair, lane t and lane t + kBlock/2 forming entries 0..PW-1 and
+// PW..2*PW-1 of pair t, so that every lane works in phase A when a block's
+// pairs are long (P3: 20 entries).  Same values and sums.
+template <int DA0, int DB0, int PW, typename T, bool P8 = false, int NFX = 0, bool SPLIT = false>
 // Five waves per SIMD (the LDS of five 30 KB workgroups fills a CU): at most
 // 96 VGPRs, measured faster than four unconstrained waves and than spilling
 // to six or eight.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void row_block_persistent(BlockArgs a, const T* __restrict__ Kg) {
+    static_assert(!SPLIT || (!P8 && NFX == 0 && PW % 2 == 0), "split pairs: general form, 16-bit positions");
     constexpr int KB = DA0 * DB0;
-    constexpr int NPW = P8 ? (2 + 2 * PW + 3) / 4 : PW;  // position words per pair
+    // position words per lane: a pair's, or half of them when two lanes share it
+    constexpr int NPW = P8 ? (2 + 2 * PW + 3) / 4 : (SPLIT ? PW / 2 : PW);
+    constexpr int NE = SPLIT ? PW : 2 * PW;  // entries per lane
+    constexpr int kHalf = kBlock / 2;
+    // the lane's pair and its first entry
+    const int pl = SPLIT ? ((int)threadIdx.x & (kHalf - 1)) : (int)threadIdx.x;
+    const int ia_lane = SPLIT ? ((int)threadIdx.x / kHalf) * NE : 0;
+    const bool rhs_lane = !SPLIT || (int)threadIdx.x < kHalf;
     // slot starts carried in registers for blocks of at most 2*SW*kBlock - 2
     // slots (the registers freed by the shorter position lists of P1 and P2);
     // else copied to LDS when the block's turn comes
@@ -741,8 +753,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     using Regs = PairRegs<KB, NPW, SW, T>;
     auto load_records = [&](const BlockMeta& Mx, Regs& P) {
         const int nf = a.kinds[Mx.kind].nf;
-        const bool hp = (int)threadIdx.x < Mx.np;
-        const int64_t gp = Mx.p0 + (hp ? threadIdx.x : 0);
+        const bool hp = pl < Mx.np;
+        const int64_t gp = Mx.p0 + (hp ? pl : 0);
         P.tet = hp ? a.pair_tet[gp] : 0;
         P.ibf = hp ? a.pair_ib[gp] : 0;
         if constexpr (P8) {
@@ -750,14 +762,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 #pragma unroll
             for (int w = 0; w < NPW; ++w) P.posw[w] = hp ? pw[w] : 0u;
         } else {
+            const int w0 = ia_lane / 2;
             const uint32_t* pw =
-                reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? threadIdx.x : 0) * nf);
+                reinterpret_cast<const uint32_t*>(a.pair_pos + Mx.q0 + (int64_t)(hp ? pl : 0) * nf) + w0;
 #pragma unroll
-            for (int w = 0; w < PW; ++w) P.posw[w] = (hp && 2 * w < nf) ? pw[w] : 0u;
+            for (int w = 0; w < NPW; ++w) P.posw[w] = (hp && 2 * (w0 + w) < nf) ? pw[w] : 0u;
         }
     };
     auto load_k = [&](const BlockMeta& Mx, Regs& P) {
-        const bool hp = (int)threadIdx.x < Mx.np;
+        const bool hp = pl < Mx.np;
         const T* Kt = Kg + (int64_t)P.tet * a.rowlen;
 #pragma unroll
         for (int j = 0; j < KB; ++j) P.k[j] = hp ? Kt[j] : T(0);
@@ -887,21 +900,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                 if (!fast) nonfinite += isfinite(fv) ? 0 : 1;
                 fvals[threadIdx.x] = fv;
             }
-        } else if (NFX == 0 && (int)threadIdx.x < M.np) {
+        } else if (NFX == 0 && pl < M.np) {
             const int ib = cur.ibf & 63;
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + (ib % nfB0) * lenp;
             const int base8 = (int)(cur.posw[0] & 0xFFFFu);
 #pragma unroll
-            for (int ia0 = 0; ia0 < 2 * PW; ++ia0) {
+            for (int i = 0; i < NE; ++i) {
+                const int ia0 = ia_lane + i;  // ia_lane is even
                 int ps;
                 if constexpr (P8) {
                     // byte 2 + ia0 of the pair's words: offset from the base, 0xFF dropped
                     const uint32_t o = (cur.posw[(2 + ia0) >> 2] >> (((2 + ia0) & 3) * 8)) & 0xFFu;
                     ps = o == 0xFFu ? (int)kPosDropped : base8 + (int)o;
                 } else {
-                    const uint32_t w = cur.posw[ia0 >> 1];
-                    ps = (int)((ia0 & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    const uint32_t w = cur.posw[i >> 1];
+                    ps = (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
                 }
                 if (ia0 < nfA0 && ps != kPosDropped) {
                     const T* Rr = Rb + nfB0 * ia0 * lenp;
@@ -913,11 +927,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                     vals[ps] = v;
                 }
             }
-            if (a.has_rhs) {
+            if (a.has_rhs && rhs_lane) {
                 const T acc = cur.kf * RF[(ib % nfB0) * lenFp];
                 const double fv = dir ? 0.0 : (double)acc;
                 nonfinite += isfinite(fv) ? 0 : 1;
-                fvals[threadIdx.x] = fv;
+                fvals[pl] = fv;
             }
         }
         // geometry rows of the next block (its records have arrived by now)

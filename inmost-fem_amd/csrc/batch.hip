@@ -50,7 +50,8 @@ constexpr int kKinds = 3;  // P1, P2, P3
 // two workgroups per CU as the alternative.
 constexpr int kCandidates = ANIFEM_BATCH_BUDGET_CANDIDATES;
 const anifem::BlockCaps kCandidateCaps[kCandidates] = {{2048, 1024, 128}, {2048, 2048, 256}, {4096, 4096, 256},
-                                                       {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256}};
+                                                       {1024, 1024, 256}, {4096, 3072, 256}, {4096, 2560, 256},
+                                                       {4096, 2560, 128}, {4096, 2048, 128}};
 
 int kind_of(int fem) { return fem - ANIFEM_FEM_P1; }
 int order_of(int fem) { return fem == ANIFEM_FEM_P3 ? 4 : 2; }
@@ -88,8 +89,9 @@ struct anifem_batch {
     int64_t kind_pair0[kKinds] = {0, 0, 0};
     int kmax_values[kKinds] = {0, 0, 0}, kmax_pairs[kKinds] = {0, 0, 0}, kmax_slots[kKinds] = {0, 0, 0};
     uint32_t* d_pos8[kKinds] = {nullptr, nullptr, nullptr};  // compact positions of the order's pairs, or null
-    // launch shape per order: fixed-size form, compact positions, workgroups
-    // per CU (0: occupancy) and its time per launch (ms; -1 until timed)
+    // launch shape per order: phase-A form (0 general, 1 fixed-size, 2 two
+    // lanes per pair), compact positions, workgroups per CU (0: occupancy) and
+    // its time per launch (ms; -1 until timed)
     bool tune_launch = false, launch_tuned = false;
     int shape[kKinds][3] = {{1, 1, 0}, {1, 1, 0}, {0, 1, 0}};
     double shape_ms[kKinds] = {-1, -1, -1};
@@ -186,12 +188,15 @@ int autotune_kind(anifem_batch* b, int kind, const anifem_batch_mesh& m, hipStre
     return rc;
 }
 
-// The persistent row-block instantiation of an element order: the fixed-size
-// form for P1 (4 DOFs) and P2 (10), the general one for P3; compact or 16-bit
-// value positions.
-const void* kind_kernel(int kind, bool p8, bool fixed) {
+// The persistent row-block instantiation of an element order and phase-A form
+// (0 general, 1 fixed-size for P1 (4 DOFs) and P2 (10), 2 general with two
+// lanes per pair for P3 blocks of at most kBlock/2 pairs); compact or 16-bit
+// value positions (16-bit only in form 2).
+const void* kind_kernel(int kind, bool p8, int form) {
     using namespace anifem;
-    const int v = kind * 4 + (fixed && kind < 2 ? 2 : 0) + (p8 ? 1 : 0);
+    if (kind == 2 && form == 2)
+        return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false, 0, true>);
+    const int v = kind * 4 + (form == 1 && kind < 2 ? 2 : 0) + (p8 ? 1 : 0);
     switch (v) {
         case 0: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false>);
         case 1: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, true>);
@@ -316,7 +321,10 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
             break;
         }
         if (!b->d_pos8[k]) b->shape[k][1] = 0;
-        if (k == 2) b->shape[k][0] = 0;
+        if (k == 2 && !b->launch_tuned) {  // P3: two lanes per pair when the blocks allow it
+            b->shape[k][0] = L.a.max_pairs <= kBlock / 2 ? 2 : 0;
+            if (b->shape[k][0] == 2) b->shape[k][1] = 0;
+        }
     }
     const int64_t gblocks = (b->n_tets + kBlock - 1) / kBlock;
     const size_t glds = geometry_stage_bytes<1, 1, double>(a.rowlen);
@@ -335,9 +343,12 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
             if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
             KindLaunch& L = kl[k];
             double best = 1e300;
-            for (int fx = (k < 2 ? 1 : 0); fx >= 0 && !rc; --fx)
-                for (int p8 = (b->d_pos8[k] ? 1 : 0); p8 >= 0 && !rc; --p8) {
-                    const void* fn = kind_kernel(k, p8 != 0, fx != 0);
+            // phase-A forms of this order: fixed-size and general (P1, P2);
+            // two lanes per pair where the blocks allow it, and general (P3)
+            const int fx_first = k < 2 ? 1 : (L.a.max_pairs <= kBlock / 2 ? 2 : 0);
+            for (int fx = fx_first; fx >= 0 && !rc; fx = fx == 2 ? 0 : fx - 1)
+                for (int p8 = (b->d_pos8[k] && fx != 2 ? 1 : 0); p8 >= 0 && !rc; --p8) {
+                    const void* fn = kind_kernel(k, p8 != 0, fx);
                     int occ = 1;
                     int64_t G = 0;
                     if ((rc = persistent_grid(fn, L.lds, 0, L.a.n_blocks, b->device, &G, &occ))) break;
@@ -375,7 +386,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     for (int k = 0; k < kKinds && all_persistent; ++k) {
         if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
         KindLaunch& L = kl[k];
-        L.fn = kind_kernel(k, b->shape[k][1] != 0, b->shape[k][0] != 0);
+        L.fn = kind_kernel(k, b->shape[k][1] != 0, b->shape[k][0]);
         int rc = persistent_grid(L.fn, L.lds, b->shape[k][2], L.a.n_blocks, b->device, &L.grid, nullptr);
         if (rc) return rc;
     }
@@ -440,9 +451,10 @@ int anifem_batch_create(anifem_batch** out, int n_meshes, const anifem_batch_mes
         b->kind_used[kind_of(m.fem)] = true;
     }
     // budgets per element order: timed candidates on the largest mesh of that
-    // order within 200k tets (else the smallest)
+    // order within 200k tets (else the smallest); untuned, those of a single
+    // mesh of that order
     for (int k = 0; k < kKinds; ++k) {
-        b->caps[k] = BlockCaps();
+        b->caps[k] = default_caps(k == 0 ? 4 : k == 1 ? 10 : 20, true);
         if (!b->kind_used[k] || !autotune) continue;
         int pick = -1;
         for (int i = 0; i < n_meshes; ++i) {

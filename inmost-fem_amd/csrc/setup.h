@@ -22,21 +22,25 @@ struct BlockCaps {
 };
 // Default budgets by entries per (row, tet) pair: a block of 256 pairs of nf
 // entries needs about 256 nf values, so larger element matrices get larger
-// value / slot budgets (measured: P2^3 at 8192 values; P3 at 3072, where the
-// LDS of three workgroups fits a CU: 0.279 against 0.322 ms at 4096).
-inline BlockCaps default_caps(int nf) {
+// value / slot budgets (measured: P2^3 at 8192 values).  A scalar space of
+// 11-20 DOFs per cell (P3) takes blocks of at most 128 pairs and 2048 values:
+// its pass then runs two lanes per pair and the LDS of four workgroups fits a
+// CU (P3 on the 40^3 cube: 0.503 ms against 0.565 at 3072 values and 256
+// pairs, three workgroups).
+inline BlockCaps default_caps(int nf, bool scalar_space) {
     BlockCaps c;
     if (nf > 20) {
         c.max_slots = 4096;
         c.max_values = 8192;
     } else if (nf > 10) {
         c.max_slots = 4096;
-        c.max_values = 3072;
+        c.max_values = scalar_space ? 2048 : 3072;
+        c.max_pairs = scalar_space ? 128 : 256;
     }
     return c;
 }
 // Budgets for a mesh one of whose rows exceeds default_caps (a node with
-// more incident cells than the default blocks hold: with P3, more than 153).
+// more incident cells than the default blocks hold: with P3, more than 102).
 inline BlockCaps wide_caps(int nf) {
     BlockCaps c;
     c.max_slots = 8192;

@@ -85,7 +85,7 @@ def test_scalar_assembly_matches_oracle(fem, dirichlet):
 
 def test_p3_node_beyond_the_default_budget_matches_oracle():
     """A P3 node with 200 incident tets: its row holds 4000 entries, beyond the
-    default 3072 per block; the context is built with the wider budgets and
+    default 2048 per block; the context is built with the wider budgets and
     the assembly equals the oracle's (FP64, within 1e-12 of the row maximum)."""
     coords, tets = T.fan_mesh(100)
     n = len(tets)
@@ -101,6 +101,43 @@ def test_p3_node_beyond_the_default_budget_matches_oracle():
     rc, v_or, r_or = T.oracle_scatter(n, S["dofs"].shape[1], S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
     assert rc == 0
     compare(val.cpu().numpy(), rhs.cpu().numpy(), S, v_or, r_or)
+
+
+@pytest.mark.parametrize("dirichlet", [0, 1])
+def test_p3_two_lanes_per_pair_match_one_lane_and_oracle(dirichlet):
+    """P3 row blocks of at most 128 pairs (the default) run with two lanes per
+    pair (entries 0-9 and 10-19 of a pair on lanes t and t+128): bitwise equal
+    to the one-lane pass over blocks of 256 pairs, equal to the oracle within
+    1e-12 of the row maximum, and the same count of non-finite entries."""
+    rng = np.random.default_rng(31 + dirichlet)
+    coords, tets = af.kuhn_cube(4, 0.2, 13)
+    n = len(tets)
+    kappa = 10 ** rng.uniform(-1, 1, (n, 1))
+    kw = dict(order=4, tensor_type=af.TENSOR_SCALAR, tensor_storage=af.COEF_PER_TET, rhs_order=4,
+              rhs_tensor_type=af.TENSOR_SCALAR, rhs_tensor_storage=af.COEF_CONST, dirichlet=dirichlet)
+    A1 = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P3), budget=(4096, 3072, 256), **kw)
+    A2 = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P3), **kw)
+    assert A1.block_layout().max_block_pairs > 128 >= A2.block_layout().max_block_pairs
+    v1, r1, s1 = A1.assemble(D=dev(kappa.T.copy()), F=dev(np.ones(1)))
+    v2, r2, s2 = A2.assemble(D=dev(kappa.T.copy()), F=dev(np.ones(1)))
+    assert s1.code == 0 and s2.code == 0
+    assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
+    assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
+    setup = A2.setup()
+    S, xyz, A, F = oracle_system(coords, tets, 4, 1, af.TENSOR_SCALAR, kappa, 4, np.ones((1, 1)),
+                                 af.TENSOR_SCALAR, 4, dirichlet)
+    S = T.oracle_assembly(coords, tets, 4, 1, dirichlet_mask=setup["dirichlet"] if dirichlet else None)
+    if dirichlet:
+        A, F = T.apply_dir_elementwise(A, F, S["dofs"], setup["dirichlet"])
+    rc, v_or, r_or = T.oracle_scatter(n, S["dofs"].shape[1], S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
+    assert rc == 0
+    compare(v2.cpu().numpy(), r2.cpu().numpy(), S, v_or, r_or)
+    # a non-finite coefficient on one tet: both passes count the same entries
+    kbad = kappa.copy()
+    kbad[n // 2, 0] = np.inf
+    _, _, b1 = A1.assemble(D=dev(kbad.T.copy()), F=dev(np.ones(1)))
+    _, _, b2 = A2.assemble(D=dev(kbad.T.copy()), F=dev(np.ones(1)))
+    assert b1.n_nonfinite > 0 and b2.n_nonfinite == b1.n_nonfinite
 
 
 def test_anisotropic_p2_assembly_fp64_and_fp32():

@@ -232,19 +232,25 @@ def test_row_blocks_reproduce_oracle_scatter(fem, ncomp):
     # budgets: rows contiguous and covering, slot / value / pair limits
     r0 = B["blk_row0"]
     assert r0[0] == 0 and r0[-1] == A.ndof and np.all(np.diff(r0) > 0)
-    # default budgets by entries per pair: (slots, values) = (2048, 2048) up to
-    # nf = 10, (4096, 3072) up to 20, (4096, 8192) above; 256 pairs
-    max_slots, max_values = (2048, 2048) if nf <= 10 else (4096, 3072) if nf <= 20 else (4096, 8192)
+    # default budgets by entries per pair: (slots, values, pairs) = (2048,
+    # 2048, 256) up to nf = 10, (4096, 2048, 128) for scalar spaces up to 20
+    # (4096, 3072, 256) for vector ones, (4096, 8192, 256) above
+    if nf <= 10:
+        max_slots, max_values, max_pairs = 2048, 2048, 256
+    elif nf <= 20:
+        max_slots, max_values, max_pairs = (4096, 2048, 128) if ncomp == 1 else (4096, 3072, 256)
+    else:
+        max_slots, max_values, max_pairs = 4096, 8192, 256
     assert np.all(np.diff(s["row_ptr"][r0]) <= max_slots)
     assert np.all(np.diff(B["blk_v0"]) <= max_values)
-    assert np.all(np.diff(B["blk_p0"]) <= 256)
+    assert np.all(np.diff(B["blk_p0"]) <= max_pairs)
     assert B["blk_v0"][-1] == int((s["slot"] >= 0).sum())
     assert B["blk_p0"][-1] == len(tets) * nf
 
 
 def test_row_beyond_default_budget_takes_wider_blocks():
     """A P3 node with 200 incident tets (4000 entries in its row, beyond the
-    default 3072 per block): the host setup falls back to the wider budgets
+    default 2048 per block): the host setup falls back to the wider budgets
     (8192 slots and entries) instead of refusing the mesh; the block lists
     still replay the oracle's scatter bit for bit."""
     coords, tets = T.fan_mesh(100)
@@ -253,7 +259,7 @@ def test_row_beyond_default_budget_takes_wider_blocks():
     B = A.blocks()
     nf = A.nf
     vmax = np.diff(B["blk_v0"]).max()
-    assert 3072 < vmax <= 8192
+    assert 2048 < vmax <= 8192
     assert np.diff(s["row_ptr"][B["blk_row0"]]).max() <= 8192
     rng = np.random.default_rng(7)
     Ae = rng.standard_normal((len(tets), nf * nf))
