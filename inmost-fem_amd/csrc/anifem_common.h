@@ -68,6 +68,9 @@ struct RefTensorDev {
     // terms); E0 x 6, FP64 and FP32
     const double* rs64 = nullptr;
     const float* rs32 = nullptr;
+    // rs64 row (ib + nfB ia) equals row (ia + nfB ib) bit for bit (one space
+    // and one operator on both sides; checked when the table is built)
+    bool rs_sym = false;
 };
 // Packed index of K(a, b) = K(b, a) in the symmetric layout above.
 __host__ __device__ inline int sym_pack(int a, int b) { return a <= b ? a + b * (b + 1) / 2 : b + a * (a + 1) / 2; }

@@ -226,18 +226,19 @@ int upload_maps(anifem_assembly* ctx, hipStream_t s) {
 
 // Persistent row-block pass: occupancy x CUs workgroups, whole XCD groups
 // (surplus workgroups exit at once).
-template <int DA0, int DB0, int PW, typename T, bool P8, int NFX = 0, bool SPLIT = false>
+template <int DA0, int DB0, int PW, typename T, bool P8, int NFX = 0, bool SPLIT = false, bool TRI = false>
 int launch_persistent_p(anifem_assembly* ctx, const anifem::BlockArgs& a, const T* Kg, size_t lds, hipStream_t s) {
     using namespace anifem;
     int occ = 0, n_cu = 0;
     ANIFEM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT>), kBlock, lds));
+        &occ, reinterpret_cast<const void*>(&row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT, TRI>), kBlock,
+        lds));
     ANIFEM_HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int64_t G = (int64_t)std::max(occ, 1) * std::max(n_cu, 1);
     G = std::min<int64_t>(G, ctx->S.n_blocks);
     G = std::max<int64_t>(8, G / 8 * 8);
-    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT>), dim3((unsigned)G), dim3(kBlock), lds,
-                       s, a, Kg);
+    hipLaunchKernelGGL((row_block_persistent<DA0, DB0, PW, T, P8, NFX, SPLIT, TRI>), dim3((unsigned)G), dim3(kBlock),
+                       lds, s, a, Kg);
     ANIFEM_HIP_CHECK(hipGetLastError());
     return ANIFEM_OK;
 }
@@ -254,8 +255,26 @@ int launch_persistent(anifem_assembly* ctx, const anifem::BlockArgs& a, const T*
                        a.kinds[0].nfA0 == 2 * PW && a.kinds[0].nfB0 == 2 * PW;
     constexpr int NFX = PW <= 5 ? 2 * PW : 0;
     if constexpr (PW > 5 && PW % 2 == 0) {
-        if (ctx->S.max_block_pairs <= anifem::kBlock / 2)
+        if (ctx->S.max_block_pairs <= anifem::kBlock / 2) {
+            // a packed symmetric K against a reference tensor symmetric in
+            // (ia, ib): its triangle only in LDS
+            if constexpr (DA0 == 6 && DB0 == 1) {
+                const anifem::KindTable& K0 = a.kinds[0];
+                constexpr int NF = 2 * PW;
+                if (a.n_kinds == 1 && a.m.rds_sym && K0.Rd == a.m.Rds && K0.nfA0 == NF && K0.nfB0 == NF &&
+                    K0.R_rows == NF * NF) {
+                    anifem::BlockArgs at = a;
+                    at.max_R_rows = NF * (NF + 1) / 2;
+                    const size_t lds_t =
+                        anifem::block_lds_layout(at.max_values, at.max_pairs, at.max_slots, (int)sizeof(T),
+                                                 at.max_R_rows * anifem::pad16<T>(DA0 * DB0),
+                                                 at.has_rhs ? at.max_RF_rows * anifem::pad16<T>(1) : 0)
+                            .total;
+                    return launch_persistent_p<DA0, DB0, PW, T, false, 0, true, true>(ctx, at, Kg, lds_t, s);
+                }
+            }
             return launch_persistent_p<DA0, DB0, PW, T, false, 0, true>(ctx, a, Kg, lds, s);
+        }
     }
     if (ctx->d_pair_pos8 && ctx->pos8_words == (2 + 2 * PW + 3) / 4)
         return fixed ? launch_persistent_p<DA0, DB0, PW, T, true, NFX>(ctx, a, Kg, lds, s)

@@ -24,6 +24,8 @@
 // load-vector entries are 0).
 #pragma once
 
+#include <type_traits>
+
 #include "element_kernels.h"
 #include "setup.h"
 
@@ -113,6 +115,25 @@ __device__ __forceinline__ void pad_copy(V* __restrict__ dst, const V* __restric
     for (int i = threadIdx.x; i < n; i += kBlock) {
         const int row = div_small(i, len, rl);
         dst[row * lenp + (i - row * len)] = src[i];
+    }
+}
+
+// Row (lo, hi) of a reference tensor symmetric in its two DOF indices, lo <= hi
+// < NF, in the triangle layout: the rows of column lo (hi = lo .. NF-1) start
+// at lo NF - lo (lo - 1) / 2.
+template <int NF>
+__host__ __device__ constexpr int tri_row(int lo, int hi) {
+    return lo * NF - ((lo * (lo - 1)) >> 1) + (hi - lo);
+}
+
+// The rows ib + NF ia with ib >= ia of an NF^2 x len table (the triangle of a
+// table symmetric in (ia, ib)) copied to LDS rows of lenp values.
+template <int NF, int LEN, typename V>
+__device__ __forceinline__ void tri_copy(V* __restrict__ dst, const V* __restrict__ src, int lenp) {
+    for (int i = threadIdx.x; i < NF * NF * LEN; i += kBlock) {
+        const int r = i / LEN, j = i - (i / LEN) * LEN;
+        const int ia = r / NF, ib = r - (r / NF) * NF;
+        if (ib >= ia) dst[tri_row<NF>(ia, ib) * lenp + j] = src[i];
     }
 }
 
@@ -699,13 +720,20 @@ __device__ __forceinline__ int slot_words(int64_t s0, int ns) { return (int)(((s
 // two lanes per pair, lane t and lane t + kBlock/2 forming entries 0..PW-1 and
 // PW..2*PW-1 of pair t, so that every lane works in phase A when a block's
 // pairs are long (P3: 20 entries).  Same values and sums.
-template <int DA0, int DB0, int PW, typename T, bool P8 = false, int NFX = 0, bool SPLIT = false>
+// TRI (with SPLIT, K packed to 6 values, one kind of 2*PW DOFs on both sides
+// whose reference tensor is symmetric in (ia, ib) bit for bit): only the
+// triangle ib >= ia of the reference tensor is staged (P3: 210 of 400 rows,
+// 10 KB less LDS per workgroup, a.max_R_rows set to the triangle's rows);
+// entry (ib, ia) reads row (min, max).  Same values and sums.
+template <int DA0, int DB0, int PW, typename T, bool P8 = false, int NFX = 0, bool SPLIT = false, bool TRI = false>
 // Five waves per SIMD (the LDS of five 30 KB workgroups fills a CU): at most
 // 96 VGPRs, measured faster than four unconstrained waves and than spilling
 // to six or eight.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void row_block_persistent(BlockArgs a, const T* __restrict__ Kg) {
     static_assert(!SPLIT || (!P8 && NFX == 0 && PW % 2 == 0), "split pairs: general form, 16-bit positions");
+    static_assert(!TRI || (SPLIT && DA0 == 6 && DB0 == 1), "triangle of a symmetric packed reference tensor");
     constexpr int KB = DA0 * DB0;
+    constexpr int NF = 2 * PW;  // DOFs per cell of the TRI kind
     // position words per lane: a pair's, or half of them when two lanes share it
     constexpr int NPW = P8 ? (2 + 2 * PW + 3) / 4 : (SPLIT ? PW / 2 : PW);
     constexpr int NE = SPLIT ? PW : 2 * PW;  // entries per lane
@@ -735,13 +763,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     // only when it does)
     auto stage = [&](int kind) {
         const KindTable& KT = a.kinds[kind];
-        if constexpr (sizeof(T) == 8) {
-            pad_copy(R, reinterpret_cast<const T*>(KT.Rd), KT.R_rows, KB, lenp);
-            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFd), KT.RF_rows, 1, lenFp);
-        } else {
-            pad_copy(R, reinterpret_cast<const T*>(KT.Rf), KT.R_rows, KB, lenp);
-            if (a.has_rhs) pad_copy(RF, reinterpret_cast<const T*>(KT.RFf), KT.RF_rows, 1, lenFp);
-        }
+        const T* Rsrc = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.Rd : (const void*)KT.Rf);
+        const T* RFsrc = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)KT.RFd : (const void*)KT.RFf);
+        if constexpr (TRI)
+            tri_copy<NF, KB>(R, Rsrc, lenp);
+        else
+            pad_copy(R, Rsrc, KT.R_rows, KB, lenp);
+        if (a.has_rhs) pad_copy(RF, RFsrc, KT.RF_rows, 1, lenFp);
     };
     const int kstride = KB;  // one component block, one point
 
@@ -905,27 +933,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
             const bool dir = (cur.ibf & kPairDirichlet) != 0;
             const T* Rb = R + (ib % nfB0) * lenp;
             const int base8 = (int)(cur.posw[0] & 0xFFFFu);
+            const int tri_ib = TRI ? tri_row<NF>(ib, 0) : 0;  // row (ib, ia) = tri_ib + ia for ia >= ib
+            // entries IA .. IA+NE-1 of the pair (IA = 0, or NE on the second lane of a split pair)
+            auto entries = [&](auto ia_c) {
+                constexpr int IA = decltype(ia_c)::value;
 #pragma unroll
-            for (int i = 0; i < NE; ++i) {
-                const int ia0 = ia_lane + i;  // ia_lane is even
-                int ps;
-                if constexpr (P8) {
-                    // byte 2 + ia0 of the pair's words: offset from the base, 0xFF dropped
-                    const uint32_t o = (cur.posw[(2 + ia0) >> 2] >> (((2 + ia0) & 3) * 8)) & 0xFFu;
-                    ps = o == 0xFFu ? (int)kPosDropped : base8 + (int)o;
-                } else {
-                    const uint32_t w = cur.posw[i >> 1];
-                    ps = (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
-                }
-                if (ia0 < nfA0 && ps != kPosDropped) {
-                    const T* Rr = Rb + nfB0 * ia0 * lenp;
-                    T acc = 0;
+                for (int i = 0; i < NE; ++i) {
+                    const int ia0 = IA + i;
+                    int ps;
+                    if constexpr (P8) {
+                        // byte 2 + ia0 of the pair's words: offset from the base, 0xFF dropped
+                        const uint32_t o = (cur.posw[(2 + ia0) >> 2] >> (((2 + ia0) & 3) * 8)) & 0xFFu;
+                        ps = o == 0xFFu ? (int)kPosDropped : base8 + (int)o;
+                    } else {
+                        const uint32_t w = cur.posw[i >> 1];
+                        ps = (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    }
+                    if (ia0 < nfA0 && ps != kPosDropped) {
+                        const T* Rr;
+                        if constexpr (TRI)
+                            Rr = R + (ia0 <= ib ? tri_row<NF>(ia0, 0) + ib : tri_ib + ia0) * lenp;
+                        else
+                            Rr = Rb + nfB0 * ia0 * lenp;
+                        T acc = 0;
 #pragma unroll
-                    for (int j = 0; j < KB; ++j) acc += cur.k[j] * Rr[j];
-                    const double v = dir ? 1.0 : (double)acc;
-                    nonfinite += isfinite(v) ? 0 : 1;
-                    vals[ps] = v;
+                        for (int j = 0; j < KB; ++j) acc += cur.k[j] * Rr[j];
+                        const double v = dir ? 1.0 : (double)acc;
+                        nonfinite += isfinite(v) ? 0 : 1;
+                        vals[ps] = v;
+                    }
                 }
+            };
+            if constexpr (SPLIT) {
+                if ((int)threadIdx.x < kHalf)  // uniform over each wave
+                    entries(std::integral_constant<int, 0>{});
+                else
+                    entries(std::integral_constant<int, NE>{});
+            } else {
+                entries(std::integral_constant<int, 0>{});
             }
             if (a.has_rhs && rhs_lane) {
                 const T acc = cur.kf * RF[(ib % nfB0) * lenFp];

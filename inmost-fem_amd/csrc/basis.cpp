@@ -410,6 +410,13 @@ int get_ref_tensor(int opA, int femA, int opB, int femB, int order, bool per_poi
         ANIFEM_HIP_CHECK(hipMemcpy(ds32, Rsf.data(), Rsf.size() * sizeof(float), hipMemcpyHostToDevice));
         t.rs64 = static_cast<const double*>(ds64);
         t.rs32 = static_cast<const float*>(ds32);
+        const int nA = nf_scalar(femA), nB = nf_scalar(femB);
+        bool sym = nA == nB;
+        for (int ia = 0; ia < nA && sym; ++ia)
+            for (int ib = 0; ib < nB && sym; ++ib)
+                for (int p = 0; p < 6; ++p)
+                    sym = sym && Rs[(size_t)(ib + nB * ia) * 6 + p] == Rs[(size_t)(ia + nB * ib) * 6 + p];
+        t.rs_sym = sym;
     }
     if (!per_point && t.KB <= 16) {
         // R / max|R| in [-1, 1], rows padded to 16 columns and to a multiple of 16 rows

@@ -190,12 +190,14 @@ int autotune_kind(anifem_batch* b, int kind, const anifem_batch_mesh& m, hipStre
 
 // The persistent row-block instantiation of an element order and phase-A form
 // (0 general, 1 fixed-size for P1 (4 DOFs) and P2 (10), 2 general with two
-// lanes per pair for P3 blocks of at most kBlock/2 pairs); compact or 16-bit
-// value positions (16-bit only in form 2).
-const void* kind_kernel(int kind, bool p8, int form) {
+// lanes per pair for P3 blocks of at most kBlock/2 pairs, with the triangle of
+// the reference tensor in LDS when it is symmetric); compact or 16-bit value
+// positions (16-bit only in form 2).
+const void* kind_kernel(int kind, bool p8, int form, bool tri = false) {
     using namespace anifem;
     if (kind == 2 && form == 2)
-        return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false, 0, true>);
+        return tri ? reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false, 0, true, true>)
+                   : reinterpret_cast<const void*>(&row_block_persistent<6, 1, 10, double, false, 0, true>);
     const int v = kind * 4 + (form == 1 && kind < 2 ? 2 : 0) + (p8 ? 1 : 0);
     switch (v) {
         case 0: return reinterpret_cast<const void*>(&row_block_persistent<6, 1, 2, double, false>);
@@ -232,6 +234,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     const anifem_tensor D{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_PER_TET, kappa};
     const anifem_tensor F{ANIFEM_TENSOR_SCALAR, ANIFEM_COEF_CONST, f};
     int first = -1;
+    bool rsym[kKinds] = {false, false, false};  // reference tensor of the order symmetric in (ia, ib)
     for (int k = 0; k < kKinds; ++k) {
         if (!b->kind_used[k]) continue;
         const int fem = k + ANIFEM_FEM_P1;
@@ -245,6 +248,7 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         // kappa PSI^T PSI is symmetric: K packed to 6 values, the column-summed R
         if (!em.Rds) return ANIFEM_ERR_ARG;
         a.kinds[k] = KindTable{em.Rds, em.Rfs, ef.Rd, ef.Rf, em.nfA0, em.nfB0, em.nfB0, em.nfA0 * em.nfB0, ef.nfB0, 0};
+        rsym[k] = em.rds_sym != 0;
         a.max_R_rows = std::max(a.max_R_rows, a.kinds[k].R_rows);
         a.max_RF_rows = std::max(a.max_RF_rows, a.kinds[k].RF_rows);
         if (first < 0) {
@@ -297,6 +301,10 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         BlockArgs a{};
         size_t lds = 0;
         int64_t grid = 0;
+        // form 2 of P3 with the triangle of a symmetric reference tensor
+        bool tri = false;
+        BlockArgs at{};
+        size_t lds_t = 0;
     };
     KindLaunch kl[kKinds];
     bool all_persistent = b->persistent;
@@ -319,6 +327,15 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
         if (L.lds > (size_t)max_lds / 2) {  // the pass stages R: two workgroups per CU at least
             all_persistent = false;
             break;
+        }
+        const KindTable& KT = a.kinds[k];
+        L.tri = k == 2 && rsym[k] && KT.nfA0 == 20 && KT.nfB0 == 20 && KT.R_rows == 400;
+        if (L.tri) {
+            L.at = L.a;
+            L.at.max_R_rows = 20 * 21 / 2;
+            L.lds_t = block_lds_layout(L.at.max_values, L.at.max_pairs, L.at.max_slots, 8,
+                                       L.at.max_R_rows * pad16<double>(6), L.at.max_RF_rows * pad16<double>(1))
+                          .total;
         }
         if (!b->d_pos8[k]) b->shape[k][1] = 0;
         if (k == 2 && !b->launch_tuned) {  // P3: two lanes per pair when the blocks allow it
@@ -348,18 +365,21 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
             const int fx_first = k < 2 ? 1 : (L.a.max_pairs <= kBlock / 2 ? 2 : 0);
             for (int fx = fx_first; fx >= 0 && !rc; fx = fx == 2 ? 0 : fx - 1)
                 for (int p8 = (b->d_pos8[k] && fx != 2 ? 1 : 0); p8 >= 0 && !rc; --p8) {
-                    const void* fn = kind_kernel(k, p8 != 0, fx);
+                    const bool tri = fx == 2 && L.tri;
+                    const void* fn = kind_kernel(k, p8 != 0, fx, tri);
+                    BlockArgs& A = tri ? L.at : L.a;
+                    const size_t lx = tri ? L.lds_t : L.lds;
                     int occ = 1;
                     int64_t G = 0;
-                    if ((rc = persistent_grid(fn, L.lds, 0, L.a.n_blocks, b->device, &G, &occ))) break;
+                    if ((rc = persistent_grid(fn, lx, 0, A.n_blocks, b->device, &G, &occ))) break;
                     for (int w = occ; w >= std::max(1, occ - 2) && !rc; --w) {
-                        if ((rc = persistent_grid(fn, L.lds, w, L.a.n_blocks, b->device, &G, nullptr))) break;
-                        void* args[] = {&L.a, &Kt};
+                        if ((rc = persistent_grid(fn, lx, w, A.n_blocks, b->device, &G, nullptr))) break;
+                        void* args[] = {&A, &Kt};
                         const int reps = 3;
-                        hipError_t le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, L.lds, rs);
+                        hipError_t le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, lx, rs);
                         le0 = le0 == hipSuccess ? hipEventRecord(e0, rs) : le0;
                         for (int r = 0; r < reps && le0 == hipSuccess; ++r)
-                            le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, L.lds, rs);
+                            le0 = hipLaunchKernel(fn, dim3((unsigned)G), dim3(kBlock), args, lx, rs);
                         le0 = le0 == hipSuccess ? hipEventRecord(e1, rs) : le0;
                         le0 = le0 == hipSuccess ? hipEventSynchronize(e1) : le0;
                         float t = 0.f;
@@ -386,7 +406,12 @@ int capture(anifem_batch* b, const double* kappa, const double* f, double* val, 
     for (int k = 0; k < kKinds && all_persistent; ++k) {
         if (b->kind_blk[k + 1] == b->kind_blk[k]) continue;
         KindLaunch& L = kl[k];
-        L.fn = kind_kernel(k, b->shape[k][1] != 0, b->shape[k][0]);
+        const bool tri = b->shape[k][0] == 2 && L.tri;
+        L.fn = kind_kernel(k, b->shape[k][1] != 0, b->shape[k][0], tri);
+        if (tri) {  // the launch takes the triangle's arguments and LDS
+            L.a = L.at;
+            L.lds = L.lds_t;
+        }
         int rc = persistent_grid(L.fn, L.lds, b->shape[k][2], L.a.n_blocks, b->device, &L.grid, nullptr);
         if (rc) return rc;
     }

@@ -62,6 +62,7 @@ static int prepare_args(const anifem_op& opA, const anifem_op& opB, const anifem
     a.Rf = R.r32;
     a.Rds = R.rs64;
     a.Rfs = R.rs32;
+    a.rds_sym = R.rs_sym ? 1 : 0;
     a.wq = R.w;
     a.r16 = R.r16;
     a.r16_scale = R.r16_scale;

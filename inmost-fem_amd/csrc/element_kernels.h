@@ -59,6 +59,7 @@ struct ElemArgs {
     const void* r16;   // FP16 hi + lo split of R / r16_scale ([2][e0pad][16])
     float r16_scale;
     int e0pad;
+    int rds_sym;       // Rds is symmetric in (ia, ib) bit for bit (RefTensorDev::rs_sym)
 };
 
 __device__ __forceinline__ void tet_geometry(const double P[4][3], double PSI[9], double& det) {

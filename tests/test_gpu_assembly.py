@@ -103,17 +103,22 @@ def test_p3_node_beyond_the_default_budget_matches_oracle():
     compare(val.cpu().numpy(), rhs.cpu().numpy(), S, v_or, r_or)
 
 
-@pytest.mark.parametrize("dirichlet", [0, 1])
-def test_p3_two_lanes_per_pair_match_one_lane_and_oracle(dirichlet):
+@pytest.mark.parametrize("tt,dirichlet", [(af.TENSOR_SCALAR, 0), (af.TENSOR_SCALAR, 1), (af.TENSOR_GENERAL, 1)])
+def test_p3_two_lanes_per_pair_match_one_lane_and_oracle(tt, dirichlet):
     """P3 row blocks of at most 128 pairs (the default) run with two lanes per
-    pair (entries 0-9 and 10-19 of a pair on lanes t and t+128): bitwise equal
-    to the one-lane pass over blocks of 256 pairs, equal to the oracle within
-    1e-12 of the row maximum, and the same count of non-finite entries."""
-    rng = np.random.default_rng(31 + dirichlet)
+    pair (entries 0-9 and 10-19 of a pair on lanes t and t+128; with a scalar
+    coefficient only the triangle of the symmetric reference tensor is staged,
+    with a general 3 x 3 one the full tensor): bitwise equal to the one-lane
+    pass over blocks of 256 pairs, equal to the oracle within 1e-12 of the row
+    maximum, and the same count of non-finite entries."""
+    rng = np.random.default_rng(31 + dirichlet + tt)
     coords, tets = af.kuhn_cube(4, 0.2, 13)
     n = len(tets)
-    kappa = 10 ** rng.uniform(-1, 1, (n, 1))
-    kw = dict(order=4, tensor_type=af.TENSOR_SCALAR, tensor_storage=af.COEF_PER_TET, rhs_order=4,
+    if tt == af.TENSOR_SCALAR:
+        kappa = 10 ** rng.uniform(-1, 1, (n, 1))
+    else:  # a non-symmetric coefficient: I + 0.3 N, N standard normal
+        kappa = (np.eye(3).ravel(order="F")[None, :] + 0.3 * rng.standard_normal((n, 9)))
+    kw = dict(order=4, tensor_type=tt, tensor_storage=af.COEF_PER_TET, rhs_order=4,
               rhs_tensor_type=af.TENSOR_SCALAR, rhs_tensor_storage=af.COEF_CONST, dirichlet=dirichlet)
     A1 = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P3), budget=(4096, 3072, 256), **kw)
     A2 = af.Assembler(coords, tets, af.Op(af.GRAD, af.FEM_P3), **kw)
@@ -124,8 +129,7 @@ def test_p3_two_lanes_per_pair_match_one_lane_and_oracle(dirichlet):
     assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
     assert np.array_equal(r1.cpu().numpy(), r2.cpu().numpy())
     setup = A2.setup()
-    S, xyz, A, F = oracle_system(coords, tets, 4, 1, af.TENSOR_SCALAR, kappa, 4, np.ones((1, 1)),
-                                 af.TENSOR_SCALAR, 4, dirichlet)
+    S, xyz, A, F = oracle_system(coords, tets, 4, 1, tt, kappa, 4, np.ones((1, 1)), af.TENSOR_SCALAR, 4, dirichlet)
     S = T.oracle_assembly(coords, tets, 4, 1, dirichlet_mask=setup["dirichlet"] if dirichlet else None)
     if dirichlet:
         A, F = T.apply_dir_elementwise(A, F, S["dofs"], setup["dirichlet"])
