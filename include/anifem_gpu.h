@@ -240,9 +240,10 @@ int anifem_assembly_create(anifem_assembly** ctx, const anifem_mesh_desc* mesh, 
 /* anifem_assembly_create with explicit row-block budgets (CSR slots, element
  * entries and (row, tet) pairs per block; at most 65535, 65535 and 256).  The
  * default (anifem_assembly_create) chooses them by the entries per pair:
- * 2048 / 2048 / 256 up to 10, 4096 / 2048 / 128 up to 20, 4096 / 8192 / 256
- * above, and 8192 / 8192 (16384 above 20) / 256 when one row alone exceeds
- * those.  Returns ANIFEM_ERR_DIMS when one row alone exceeds the budgets.  No
+ * 2048 / 2048 / 256 up to 10; up to 20, 4096 / 2048 / 128 for a scalar space
+ * (its pass then runs two lanes per pair) and 4096 / 3072 / 256 for a vector
+ * one; 4096 / 8192 / 256 above; and 8192 / 8192 (16384 above 20) / 256 when
+ * one row alone exceeds those.  Returns ANIFEM_ERR_DIMS when one row alone exceeds the budgets.  No
  * reference counterpart. */
 int anifem_assembly_create_budget(anifem_assembly** ctx, const anifem_mesh_desc* mesh,
                                   const anifem_problem_desc* prob, int max_slots, int max_values, int max_pairs,

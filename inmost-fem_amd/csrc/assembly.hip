@@ -47,6 +47,7 @@ struct anifem_assembly {
     int64_t* d_row_ptr = nullptr;
     int32_t* d_col_idx = nullptr;
     int* d_status = nullptr;          // [0] non-finite, [1] degenerate
+    int* d_deg = nullptr;             // degenerate cells per geometry workgroup (fused pass)
     void* d_kbuf = nullptr;           // per-tet geometry tensors of the last run
     // plain baseline (anifem_assembly_run_plain): slot map, DOFs, flags, tables
     int32_t* d_slot = nullptr;
@@ -120,6 +121,20 @@ __global__ void fill_kernel(double* p, int64_t n, double v) {
 __global__ void fill_int_kernel(int* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+// *out = the sum of the per-workgroup degenerate counts (one workgroup).
+__global__ __launch_bounds__(anifem::kBlock) void sum_counts_kernel(const int* __restrict__ c, int64_t n, int* out) {
+    constexpr int B = anifem::kBlock;
+    __shared__ int part[B];
+    int s = 0;
+    for (int64_t i = threadIdx.x; i < n; i += B) s += c[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = B / 2; h > 0; h /= 2) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
 }
 
 int fill(double* p, int64_t n, hipStream_t s) {
@@ -699,6 +714,8 @@ int anifem::assembly_create_caps(anifem_assembly** out, const anifem_mesh_desc* 
     if ((rc = upload(&ctx->d_row_ptr, S.row_ptr.data(), S.row_ptr.size(), s))) return rc;
     if ((rc = upload(&ctx->d_col_idx, S.col_idx.data(), S.col_idx.size(), s))) return rc;
     ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_status, 4 * sizeof(int)));
+    if (S.n_tets > 0)
+        ANIFEM_HIP_CHECK(hipMalloc((void**)&ctx->d_deg, (size_t)((S.n_tets + kBlock - 1) / kBlock) * sizeof(int)));
     // spaces whose elements need the Piola map and per-DOF factors (RT0,
     // ND0) or have one DOF (P0, B4) are assembled from element matrices
     ctx->elem_path = vars || fem_vector(prob->trial.fem) || (S.nf & 1);
@@ -779,7 +796,7 @@ int anifem_assembly_destroy(anifem_assembly* ctx) {
                         ctx->d_pstart,   ctx->d_row_ptr,  ctx->d_col_idx,  ctx->d_status,   ctx->d_kbuf,
                         ctx->d_slot,     ctx->d_dofs,     ctx->d_isdir,    ctx->d_plain_tab, ctx->d_col_tets,
                         ctx->d_lift,     ctx->d_pair_pos8, ctx->d_sign,   ctx->d_xyz,      ctx->d_ebuf,
-                        ctx->d_fbuf,     ctx->d_signmask};
+                        ctx->d_fbuf,     ctx->d_signmask, ctx->d_deg};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -1072,8 +1089,15 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
     if (ctx->timers_on) ANIFEM_HIP_CHECK(hipEventRecord(ctx->ev[0], s));
     // a rhs without a load vector is zero
     if (rhs && !do_rhs && (rc = fill(rhs, S.ndof, s))) return rc;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
-    ANIFEM_HIP_CHECK(hipGetLastError());
+    // the geometry launch clears the non-finite count and stores a degenerate
+    // count per workgroup (summed below when the status is asked for)
+    const bool per_wg = ctx->d_deg != nullptr;
+    if (per_wg) {
+        a.deg = ctx->d_deg;
+    } else {
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ctx->d_status, 4);
+        ANIFEM_HIP_CHECK(hipGetLastError());
+    }
     ctx->run_bc = dir_values;
     ctx->run_scratch = scratch;
     rc = precision == ANIFEM_PREC_FP64 ? dispatch_blocks<double>(ctx, a, sym, s) : dispatch_blocks<float>(ctx, a, sym, s);
@@ -1089,10 +1113,15 @@ int assembly_run(anifem_assembly* ctx, const anifem_tensor* D, const anifem_tens
         ANIFEM_HIP_CHECK(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
         ctx->timers.ms_geometry = t01;
         ctx->timers.ms_gather = t12;
-        ctx->timers.n_launches = 2 + (rhs && !do_rhs ? 1 : 0) + (S.n_blocks > 0 ? 1 : 0);
+        ctx->timers.n_launches = (per_wg ? 1 : 2) + (rhs && !do_rhs ? 1 : 0) + (S.n_blocks > 0 ? 1 : 0);
         ctx->timers.reserved = 0;
     }
     if (status) {
+        if (per_wg) {
+            hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(kBlock), 0, s, ctx->d_deg,
+                               (S.n_tets + kBlock - 1) / kBlock, ctx->d_status + 1);
+            ANIFEM_HIP_CHECK(hipGetLastError());
+        }
         int h[4] = {0, 0, 0, 0};
         ANIFEM_HIP_CHECK(hipMemcpyAsync(h, ctx->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
         ANIFEM_HIP_CHECK(hipStreamSynchronize(s));

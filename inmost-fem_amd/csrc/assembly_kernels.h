@@ -68,6 +68,9 @@ struct BlockArgs {
     double* val;
     double* rhs;
     int* status;
+    // geometry_kernel: one degenerate-cell count per workgroup (and status[0]
+    // cleared by the first workgroup), or null: counts added to status[1]
+    int* deg;
     int max_values, max_pairs, max_slots;  // over all blocks (at most the workgroup size of pairs)
     int max_R_rows, max_RF_rows;           // over all kinds
 };
@@ -249,10 +252,23 @@ __global__ __launch_bounds__(kBlock) void geometry_kernel(BlockArgs a, T* __rest
         T* dst = Kg + r0 * a.rowlen;
         for (int i = threadIdx.x; i < nv; i += kBlock) dst[i] = stage[i];
     }
-    // one counter update per wavefront
     const unsigned long long ballot = __ballot(degenerate);
-    if (ballot && (threadIdx.x & 63) == 0) atomicAdd(&a.status[1], (int)__popcll(ballot));
+    if (a.deg) {  // uniform: every workgroup stores its count, so a pass clears nothing beforehand
+        __shared__ int wave_count[kBlock / 64];
+        if ((threadIdx.x & 63) == 0) wave_count[threadIdx.x / 64] = (int)__popcll(ballot);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int c = 0;
+            for (int w = 0; w < kBlock / 64; ++w) c += wave_count[w];
+            a.deg[blockIdx.x] = c;
+            // the non-finite count of this pass is added by the passes after this one
+            if (blockIdx.x == 0) a.status[0] = 0;
+        }
+    } else if (ballot && (threadIdx.x & 63) == 0) {  // one counter update per wavefront
+        atomicAdd(&a.status[1], (int)__popcll(ballot));
+    }
 }
+
 
 // LDS layout of one row-block workgroup, in bytes from the start (all parts
 // 16-byte aligned): entry values | load-vector values | R | R_F | slot starts.

@@ -271,6 +271,36 @@ def test_nonfinite_status():
     assert st.code == af.ERR_NONFINITE and st.n_nonfinite > 0
 
 
+@pytest.mark.parametrize("fem,nc", [(af.FEM_P1, 1), (af.FEM_P2, 1), (af.FEM_P2, 3)])
+def test_status_counts_per_pass(fem, nc):
+    """A node with NaN coordinates: every tet around it is degenerate (its
+    Jacobian determinant is not finite), and the status of each pass counts
+    exactly those tets and the non-finite entries, pass after pass (the counts
+    are rewritten, not accumulated); a clean pass in between reports none."""
+    coords, tets = af.kuhn_cube(3, 0.2, 4)
+    n = len(tets)
+    bad = coords.copy()
+    node = 21  # an interior node of the 4 x 4 x 4 lattice
+    bad[node] = np.nan
+    expect = int((tets == node).any(axis=1).sum())
+    assert expect > 0
+    if nc == 3:  # elasticity with a Lame tensor: the compact-row geometry launch
+        kw = dict(order=2, tensor_type=af.TENSOR_LAME, tensor_storage=af.COEF_CONST)
+        D = dev(np.array([1.0, 0.5]))
+    else:
+        kw = dict(tensor_type=af.TENSOR_SCALAR, tensor_storage=af.COEF_PER_TET)
+        D = dev(np.ones((1, n)))
+    A_bad = af.Assembler(bad, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY, **kw)
+    A_ok = af.Assembler(coords, tets, af.Op(af.GRAD, fem, nc), dirichlet=af.DIRICHLET_BOUNDARY, **kw)
+    counts = []
+    for A in (A_bad, A_ok, A_bad, A_bad):
+        _, _, st = A.assemble(D=D)
+        counts.append((st.n_degenerate, st.n_nonfinite, st.code))
+    assert counts[0][0] == expect and counts[0][1] > 0 and counts[0][2] == af.ERR_NONFINITE
+    assert counts[1] == (0, 0, 0)
+    assert counts[2] == counts[0] and counts[3] == counts[0]
+
+
 @pytest.mark.parametrize("fem", [af.FEM_P1, af.FEM_P2, af.FEM_P3])
 def test_nonfinite_counts_equal_across_row_block_kernels(fem):
     """NaN, Inf and overflowing (1e306) coefficients on a few tets, with
