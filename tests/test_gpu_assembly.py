@@ -136,6 +136,12 @@ def test_p3_two_lanes_per_pair_match_one_lane_and_oracle(tt, dirichlet):
     rc, v_or, r_or = T.oracle_scatter(n, S["dofs"].shape[1], S["dofs"], S["slot"], A, F, len(S["col_idx"]), S["ndof"])
     assert rc == 0
     compare(v2.cpu().numpy(), r2.cpu().numpy(), S, v_or, r_or)
+    # FP32 contraction (the FP32 copy of the reference tensor, triangle or full)
+    w1, q1, _ = A1.assemble(D=dev(kappa.T.copy()), F=dev(np.ones(1)), precision=af.PREC_FP32)
+    w2, q2, _ = A2.assemble(D=dev(kappa.T.copy()), F=dev(np.ones(1)), precision=af.PREC_FP32)
+    assert np.array_equal(w1.cpu().numpy(), w2.cpu().numpy())
+    assert np.array_equal(q1.cpu().numpy(), q2.cpu().numpy())
+    compare(w2.cpu().numpy(), q2.cpu().numpy(), S, v_or, r_or, tol=2e-5)
     # a non-finite coefficient on one tet: both passes count the same entries
     kbad = kappa.copy()
     kbad[n // 2, 0] = np.inf
