@@ -464,7 +464,7 @@ def main():
         "setup_s": setup_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double,true> + row_block_persistent<6,1,5,double,true,10>",
+                     "kernel": "assembly pass: geometry_kernel<3,3,1,1,double,true,false> + row_block_persistent<6,1,5,double,true,10,false,false>",
                      "ms_per_launch": ms_pass_events,
                      "algorithmic_bytes_per_pass": alg_bytes,
                      "traffic_source": traffic_src},
